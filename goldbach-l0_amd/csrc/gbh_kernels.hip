@@ -1,0 +1,454 @@
+// Device kernels of the per-segment verification pipeline, written for
+// gfx950 (CDNA4: wave64, 160 KB LDS per CU). All arithmetic is integer.
+//
+// Map layout (same contract as the reference's class map, goldbach.cl:234-252
+// and host_math.hpp:88-107): eight class arrays, one per residue
+// r in {1,7,11,13,17,19,23,29} mod 30; bit t of class array c is 1 iff
+// q = Q[c] + 30 t is prime (for q in [q_first, q_last]). Words are 64-bit,
+// little-endian, so the same memory read as 32-bit words gives bit t at
+// word t >> 5, bit t & 31 (the LDS atomics use that view).
+//
+// Stages (one launch each, in stream order):
+//   k_setup        first class bit of every window prime and class
+//                  (reference mid_setup, goldbach.cl:3528-3563)
+//   k_window       one work-group per (window, class): masks for 7..61, LDS
+//                  marks of the window primes, one coalesced store
+//                  (reference make_prime_map + mid_window, 1092-1608, 3600-3701)
+//   k_large        large primes: first multiple per class, global 64-bit
+//                  atomic AND-clears (reference sieve_large_packed / hit lists)
+//   k_coff         per-segment candidate table of the checker
+//   k_check_a      exact ascending candidates, 64 evens per lane, first ka
+//                  candidates (reference check30_a, 2028-2189)
+//   k_check_b      the rest for every pending even (reference check30_b,
+//                  2210-2306)
+//   k_count_a/b    primes, twins and the gap walk of the counting range
+//                  (reference count30 + gap_tail_exact, 2499-2825)
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gbh.h"
+#include "gbh_tables.h"
+
+namespace gbh_dev {
+
+__constant__ uint16_t c_cand_p[GBH_EVEN_CLASSES * GBH_NCAND];   // useful candidates ascending, per even class
+__constant__ uint8_t c_cand_c[GBH_EVEN_CLASSES * GBH_NCAND];    // class array of q = n - p
+__constant__ uint16_t c_cand_pi[GBH_EVEN_CLASSES * GBH_NCAND];  // prime index of the candidate
+__constant__ uint32_t c_cand_n[GBH_EVEN_CLASSES];               // useful candidates per even class
+
+__device__ __forceinline__ uint32_t wave_lane() { return __lane_id(); }
+
+// Exclusive prefix over the wave of v (64 lanes); returns the total in *tot.
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t* tot)
+{
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if ((int)wave_lane() >= d) x += y;
+    }
+    *tot = __shfl(x, 63, 64);
+    return x - v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// First class bit of prime p in class c for q in [max(p^2, q_first), q_last]:
+// the smallest multiplier m >= max(ceil(q_first / p), p) with m p = res(c)
+// (mod 30); returns false when that multiple exceeds q_last.
+__device__ __forceinline__ uint64_t first_mult(uint64_t q_first, uint32_t p)
+{
+    uint64_t m0 = (q_first + p - 1) / p;
+    if (m0 < p) m0 = p;
+    return m0;
+}
+
+// ---------------------------------------------------------------- stage 1
+__global__ void __launch_bounds__(256) k_setup(gbh_geom g, const uint32_t* __restrict__ primes, uint32_t np,
+                                               uint32_t* __restrict__ i0)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= np) return;
+    const uint32_t p = primes[j];
+    const uint64_t m0 = first_mult(g.q_first, p);
+    const uint32_t m0r = (uint32_t)(m0 % 30);
+    const uint32_t pinv = kInv30[p % 30];
+#pragma unroll
+    for (int c = 0; c < GBH_CLASSES; ++c) {
+        const uint32_t mt = (kRes[c] * pinv) % 30;
+        const uint64_t m = m0 + (mt + 30 - m0r) % 30;
+        const uint64_t q = m * p;
+        uint32_t t = 0xFFFFFFFFu;
+        if (q <= g.q_last && q >= g.Q[c]) {
+            const uint64_t tt = (q - g.Q[c]) / 30;
+            if (tt < g.nc[c]) t = (uint32_t)tt;
+        }
+        i0[(size_t)c * np + j] = t;
+    }
+}
+
+// ---------------------------------------------------------------- stage 2
+// First mark >= wb0 of a prime whose first class bit is t (t may be below wb0).
+__device__ __forceinline__ uint64_t advance_to(uint32_t t, uint32_t wb0, uint32_t p)
+{
+    if (t >= wb0) return t;
+    const uint32_t d = wb0 - t;
+    const uint32_t k = d / p + (d % p != 0);
+    return (uint64_t)t + (uint64_t)k * p;
+}
+
+__global__ void __launch_bounds__(256) k_window(gbh_geom g, const uint32_t* __restrict__ primes, uint32_t np,
+                                                uint32_t ncoop, const uint32_t* __restrict__ i0,
+                                                uint64_t* __restrict__ map)
+{
+    extern __shared__ uint64_t win[];
+    uint32_t* win32 = reinterpret_cast<uint32_t*>(win);
+    const uint32_t c = blockIdx.y;
+    const uint32_t nwords = g.wbits >> 6;
+    const uint64_t gw0 = (uint64_t)blockIdx.x * nwords;     // first word of the window in class c
+    const uint32_t wb0 = (uint32_t)(gw0 * 64);              // first bit (class arrays < 2^32 bits)
+    const uint32_t wb1 = wb0 + g.wbits;
+    const uint32_t tid = threadIdx.x;
+
+    // 1) mask stage: every multiple of 7..61 cleared by whole-word masks
+    {
+        uint32_t ph[GBH_NSMALL];
+#pragma unroll
+        for (int k = 0; k < GBH_NSMALL; ++k) {
+            const uint32_t p = kSmall[k];
+            const uint32_t a = (uint32_t)(((gw0 + tid) * 64) % p);
+            ph[k] = (g.s0[c][k] + p - a) % p;
+        }
+        for (uint32_t x = tid; x < nwords; x += 256) {
+            uint64_t m = 0;
+#pragma unroll
+            for (int k = 0; k < GBH_NSMALL; ++k) {
+                m |= kPat[k] << ph[k];
+                const uint32_t p = kSmall[k];
+                const uint32_t st = kStep256[k];                 // (64 * 256) mod p
+                ph[k] = ph[k] >= st ? ph[k] - st : ph[k] + p - st;
+            }
+            uint64_t w = ~m;
+            if (gw0 + x == 0) w |= g.restore[c];
+            win[x] = w;
+        }
+    }
+    __syncthreads();
+
+    // 2) window primes shared by the whole work-group (many marks each)
+    for (uint32_t j = 0; j < ncoop; ++j) {
+        const uint32_t t0 = i0[(size_t)c * np + j];
+        if (t0 == 0xFFFFFFFFu) continue;
+        const uint32_t p = primes[j];
+        const uint64_t t = advance_to(t0, wb0, p);
+        for (uint64_t m = t + (uint64_t)tid * p; m < wb1; m += 256ull * p) {
+            const uint32_t b = (uint32_t)m - wb0;
+            atomicAnd(&win32[b >> 5], ~(1u << (b & 31)));
+        }
+    }
+    // 3) the other window primes: one prime per lane
+    for (uint32_t j = ncoop + tid; j < np; j += 256) {
+        const uint32_t t0 = i0[(size_t)c * np + j];
+        if (t0 == 0xFFFFFFFFu || t0 >= wb1) continue;
+        const uint32_t p = primes[j];
+        for (uint64_t m = advance_to(t0, wb0, p); m < wb1; m += p) {
+            const uint32_t b = (uint32_t)m - wb0;
+            atomicAnd(&win32[b >> 5], ~(1u << (b & 31)));
+        }
+    }
+    __syncthreads();
+
+    // 4) coalesced store of the window
+    uint64_t* dst = map + (size_t)c * g.cw + gw0;
+    for (uint32_t x = tid; x < nwords; x += 256) dst[x] = win[x];
+}
+
+// ---------------------------------------------------------------- stage 3
+__global__ void __launch_bounds__(256) k_large(gbh_geom g, const uint32_t* __restrict__ primes, uint32_t np,
+                                               unsigned long long* __restrict__ map)
+{
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < np; j += gridDim.x * blockDim.x) {
+        const uint32_t p = primes[j];
+        const uint64_t m0 = first_mult(g.q_first, p);
+        const uint32_t m0r = (uint32_t)(m0 % 30);
+        const uint32_t pinv = kInv30[p % 30];
+        const uint64_t q0 = m0 * p;
+        if (q0 > g.q_last) continue;
+#pragma unroll
+        for (int c = 0; c < GBH_CLASSES; ++c) {
+            const uint32_t mt = (kRes[c] * pinv) % 30;
+            uint64_t q = q0 + (uint64_t)((mt + 30 - m0r) % 30) * p;
+            for (; q <= g.q_last; q += 30ull * p) {
+                const uint64_t t = (q - g.Q[c]) / 30;
+                atomicAnd(&map[(size_t)c * g.cw + (t >> 6)], ~(1ull << (t & 63)));
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- checker
+struct ClassEvens {
+    uint64_t n0;      // first checked even of the class
+    uint32_t cnt;     // checked evens of the class
+    uint32_t i0;      // even index of n0 in the segment
+};
+
+__device__ __forceinline__ ClassEvens class_evens(const gbh_geom& g, uint64_t n_gpu, uint32_t cr)
+{
+    ClassEvens e;
+    const uint32_t r = (uint32_t)(n_gpu % 30);
+    e.n0 = n_gpu + (2 * cr + 30 - r) % 30;
+    e.cnt = e.n0 <= g.seg_end ? (uint32_t)((g.seg_end - e.n0) / 30 + 1) : 0u;
+    e.i0 = (uint32_t)((e.n0 - g.seg_start) >> 1);
+    return e;
+}
+
+// coff[cr * GBH_NCAND + j] = class << 16 | bit offset of candidate j of class cr
+__global__ void __launch_bounds__(256) k_coff(gbh_geom g, uint64_t n_gpu, uint32_t* __restrict__ coff)
+{
+    const uint32_t cr = blockIdx.y;
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= c_cand_n[cr]) return;
+    const ClassEvens e = class_evens(g, n_gpu, cr);
+    const uint32_t p = c_cand_p[cr * GBH_NCAND + j];
+    const uint32_t c = c_cand_c[cr * GBH_NCAND + j];
+    const uint64_t d = e.n0 - p - g.Q[c];          // >= 0 and a multiple of 30 (host contract)
+    coff[cr * GBH_NCAND + j] = (c << 16) | (uint32_t)(d / 30);
+}
+
+constexpr uint32_t kStageW = GBH_CHECK_WG + 8;     // staged words per class array
+constexpr uint32_t kKaMax = 128;
+
+__global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_geom g, const uint64_t* __restrict__ map,
+                                                          const uint32_t* __restrict__ coff, uint64_t n_gpu,
+                                                          uint32_t ka, uint32_t hist_on,
+                                                          uint32_t* __restrict__ pending, uint32_t pending_cap,
+                                                          gbh_segout* __restrict__ out)
+{
+    __shared__ uint64_t st[GBH_CLASSES * kStageW];
+    __shared__ uint32_t hl[GBH_EVEN_CLASSES * kKaMax];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t w0 = blockIdx.x * GBH_CHECK_WG;
+    for (uint32_t i = tid; i < GBH_CLASSES * kStageW; i += GBH_CHECK_WG) {
+        const uint32_t c = i / kStageW, x = i % kStageW;
+        const uint32_t w = w0 + x;
+        st[i] = w < g.cw ? map[(size_t)c * g.cw + w] : 0ull;
+    }
+    if (hist_on)
+        for (uint32_t i = tid; i < GBH_EVEN_CLASSES * kKaMax; i += GBH_CHECK_WG) hl[i] = 0;
+    __syncthreads();
+
+    uint32_t resolved = 0;
+    for (uint32_t cr = 0; cr < GBH_EVEN_CLASSES; ++cr) {
+        const ClassEvens e = class_evens(g, n_gpu, cr);
+        const uint32_t w = w0 + tid;
+        const uint32_t k0 = w * 64;
+        uint64_t valid = 0;
+        if (k0 < e.cnt) valid = (e.cnt - k0 >= 64) ? ~0ull : ((1ull << (e.cnt - k0)) - 1);
+        uint64_t unres = valid;
+        const uint32_t nj = min(ka, c_cand_n[cr]);
+        const uint32_t* ct = coff + cr * GBH_NCAND;
+        for (uint32_t j = 0; j < nj && unres; ++j) {
+            const uint32_t ent = ct[j];
+            const uint32_t c = ent >> 16, off = ent & 0xFFFF;
+            const uint32_t xw = c * kStageW + tid + (off >> 6);
+            const uint32_t s = off & 63;
+            const uint64_t lo = st[xw], hi = st[xw + 1];
+            const uint64_t wv = s ? ((lo >> s) | (hi << (64 - s))) : lo;
+            const uint64_t hit = wv & unres;
+            if (hit) {
+                unres &= ~hit;
+                if (hist_on) atomicAdd(&hl[cr * kKaMax + j], (uint32_t)__popcll(hit));
+            }
+        }
+        resolved += (uint32_t)__popcll(valid & ~unres);
+        // pending evens of this word: one atomic per wave
+        const uint32_t u = (uint32_t)__popcll(unres);
+        uint32_t tot = 0;
+        const uint32_t pre = wave_excl_scan(u, &tot);
+        uint32_t base = 0;
+        if (tot) {
+            if (wave_lane() == 63) base = atomicAdd(&out->pending_n, tot);
+            base = __shfl(base, 63, 64);
+        }
+        if (u) {
+            uint32_t pos = base + pre;
+            uint64_t m = unres;
+            while (m) {
+                const uint32_t b = (uint32_t)__ffsll((long long)m) - 1;
+                m &= m - 1;
+                if (pos < pending_cap) pending[pos] = e.i0 + 15u * (k0 + b);
+                ++pos;
+            }
+        }
+    }
+    const uint32_t rs = wave_sum(resolved);
+    if (wave_lane() == 0 && rs) atomicAdd(&out->acct[0], rs);
+    if (hist_on) {
+        __syncthreads();
+        for (uint32_t i = tid; i < GBH_EVEN_CLASSES * kKaMax; i += GBH_CHECK_WG) {
+            const uint32_t v = hl[i];
+            if (v) atomicAdd(&out->hist[(i / kKaMax) * GBH_NCAND + (i % kKaMax)], v);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_check_b(gbh_geom g, const uint64_t* __restrict__ map,
+                                                 const uint32_t* __restrict__ coff, uint64_t n_gpu, uint32_t ka,
+                                                 uint32_t tail_t, const uint32_t* __restrict__ pending,
+                                                 uint32_t pending_max, gbh_segout* __restrict__ out)
+{
+    const uint32_t np = min(out->pending_n, pending_max);
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < np; x += gridDim.x * blockDim.x) {
+        const uint32_t i = pending[x];
+        const uint64_t n = g.seg_start + 2ull * i;
+        const uint32_t cr = (uint32_t)(n % 30) >> 1;
+        const ClassEvens e = class_evens(g, n_gpu, cr);
+        const uint32_t k = (uint32_t)((n - e.n0) / 30);
+        const uint32_t nj = c_cand_n[cr];
+        const uint32_t* ct = coff + cr * GBH_NCAND;
+        uint32_t jf = 0xFFFFFFFFu;
+        for (uint32_t j = ka; j < nj; ++j) {
+            const uint32_t ent = ct[j];
+            const uint32_t c = ent >> 16, t = k + (ent & 0xFFFF);
+            if ((map[(size_t)c * g.cw + (t >> 6)] >> (t & 63)) & 1ull) { jf = j; break; }
+        }
+        if (jf != 0xFFFFFFFFu) {
+            atomicAdd(&out->hist[cr * GBH_NCAND + jf], 1u);
+            if (c_cand_p[cr * GBH_NCAND + jf] >= tail_t) atomicMin(&out->first[c_cand_pi[cr * GBH_NCAND + jf]], i);
+            atomicAdd(&out->acct[1], 1u);
+        } else {
+            const uint32_t s = atomicAdd(&out->left_n, 1u);
+            if (s < GBH_LEFT_CAP) out->left[s] = i;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- counter
+struct CountArgs {
+    uint32_t cls[8], pos0[8], lo[8], hi[8];
+    uint32_t tk[3], tk2[3], td[3], tlo[3], thi[3];
+    uint32_t nwords;
+};
+
+__device__ __forceinline__ uint64_t range_mask(uint32_t x, uint32_t lo, uint32_t hi)
+{
+    const uint32_t b0 = x * 64, b1 = b0 + 64;
+    if (hi <= b0 || lo >= b1 || lo >= hi) return 0;
+    const uint32_t a = lo > b0 ? lo - b0 : 0, b = hi < b1 ? hi - b0 : 64;
+    const uint64_t upto = b == 64 ? ~0ull : ((1ull << b) - 1);
+    return upto & ~((1ull << a) - 1);
+}
+
+__device__ __forceinline__ void note_gap(uint32_t g, uint32_t up, uint32_t* hl, gbh_segout* out, uint64_t& best)
+{
+    const uint32_t bin = g >> 1;
+    atomicAdd(&hl[bin < GBH_GAP_BINS ? bin : GBH_GAP_BINS], 1u);
+    if (g >= GBH_GAP_TAIL) {
+        const uint32_t s = atomicAdd(&out->gtail_n, 1u);
+        if (s < GBH_GAP_TCAP) { out->gtail[2 * s] = up; out->gtail[2 * s + 1] = g; }
+    }
+    const uint64_t key = ((uint64_t)g << 32) | (uint32_t)~up;
+    if (key > best) best = key;
+}
+
+__global__ void __launch_bounds__(256) k_count_a(gbh_geom g, const uint64_t* __restrict__ map, CountArgs a,
+                                                 uint32_t* __restrict__ fl, gbh_segout* __restrict__ out)
+{
+    __shared__ uint32_t hl[GBH_GAP_BINS + 1];
+    __shared__ uint32_t cs[11];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i <= GBH_GAP_BINS; i += 256) hl[i] = 0;
+    if (tid < 11) cs[tid] = 0;
+    __syncthreads();
+    const uint32_t x = blockIdx.x * 256 + tid;
+    uint64_t best = 0;
+    if (x < a.nwords) {
+        uint64_t w[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            w[k] = map[(size_t)a.cls[k] * g.cw + x] & range_mask(x, a.lo[k], a.hi[k]);
+            const uint32_t pc = (uint32_t)__popcll(w[k]);
+            if (pc) atomicAdd(&cs[k], pc);
+        }
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const uint64_t lw = map[(size_t)a.tk[j] * g.cw + x] & range_mask(x, a.tlo[j], a.thi[j]);
+            const uint64_t* p2 = map + (size_t)a.tk2[j] * g.cw + x;
+            const uint64_t pw = a.td[j] ? ((p2[0] >> 1) | (p2[1] << 63)) : p2[0];
+            const uint32_t pc = (uint32_t)__popcll(lw & pw);
+            if (pc) atomicAdd(&cs[8 + j], pc);
+        }
+        // the walk: primes in position order (block t, then ascending phase)
+        uint64_t any = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) any |= w[k];
+        uint32_t first = 0xFFFFFFFFu, last = 0, prev = 0;
+        bool have = false;
+        while (any) {
+            const uint32_t b = (uint32_t)__ffsll((long long)any) - 1;
+            any &= any - 1;
+            const uint32_t base = 30u * (x * 64 + b);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                if ((w[k] >> b) & 1ull) {
+                    const uint32_t pos = base + a.pos0[k];
+                    if (have) note_gap(pos - prev, pos, hl, out, best);
+                    else { first = pos; have = true; }
+                    prev = pos;
+                }
+            }
+        }
+        last = prev;
+        fl[2 * x] = first;
+        fl[2 * x + 1] = have ? last : 0xFFFFFFFFu;
+        if (have) {
+            atomicMin(&out->pfirst, first);
+            atomicMax(&out->plast, last);
+        }
+    }
+    if (best) atomicMax((unsigned long long*)&out->gmax, (unsigned long long)best);
+    __syncthreads();
+    for (uint32_t i = tid; i <= GBH_GAP_BINS; i += 256)
+        if (hl[i]) atomicAdd(&out->ghist[i], hl[i]);
+    if (tid < 8 && cs[tid]) atomicAdd(&out->cnt[a.cls[tid]], cs[tid]);
+    if (tid >= 8 && tid < 11 && cs[tid]) atomicAdd(&out->twins[tid - 8], cs[tid]);
+}
+
+// Gap from the last prime of word x to the first prime of the next word that
+// has one (the walk across word boundaries).
+__global__ void __launch_bounds__(256) k_count_b(CountArgs a, const uint32_t* __restrict__ fl,
+                                                 gbh_segout* __restrict__ out)
+{
+    __shared__ uint32_t hl[GBH_GAP_BINS + 1];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i <= GBH_GAP_BINS; i += 256) hl[i] = 0;
+    __syncthreads();
+    const uint32_t x = blockIdx.x * 256 + tid;
+    uint64_t best = 0;
+    if (x < a.nwords) {
+        const uint32_t last = fl[2 * x + 1];
+        if (last != 0xFFFFFFFFu) {
+            for (uint32_t y = x + 1; y < a.nwords; ++y) {
+                const uint32_t f = fl[2 * y];
+                if (f != 0xFFFFFFFFu) { note_gap(f - last, f, hl, out, best); break; }
+            }
+        }
+    }
+    if (best) atomicMax((unsigned long long*)&out->gmax, (unsigned long long)best);
+    __syncthreads();
+    for (uint32_t i = tid; i <= GBH_GAP_BINS; i += 256)
+        if (hl[i]) atomicAdd(&out->ghist[i], hl[i]);
+}
+
+__global__ void __launch_bounds__(256) k_fill(uint32_t* __restrict__ p, size_t n, uint32_t v)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+}  // namespace gbh_dev

@@ -1,0 +1,53 @@
+#pragma once
+// Host prime utilities of the verifier: base primes for the sieve, the
+// deterministic 64-bit Miller-Rabin and the CPU witness search of the
+// leftover fallback (reference include/primes.hpp:180-255, src/primes.cpp).
+#include <cmath>
+#include <functional>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace gb {
+
+/// Largest limit accepted: 2^64 - 2^40 (primes.hpp:212-222): first-hit
+/// arithmetic adds less than 2^40 to values <= limit.
+inline constexpr uint64_t MAX_SUPPORTED_LIMIT = 0ull - (1ull << 40);
+
+inline void require_supported_limit(uint64_t limit)
+{
+    if (limit > MAX_SUPPORTED_LIMIT)
+        throw std::runtime_error("limit " + std::to_string(limit) + " is above the supported maximum 2^64 - 2^40 = " +
+                                 std::to_string(MAX_SUPPORTED_LIMIT));
+}
+
+/// floor(sqrt(limit)) + 16, saturated at 2^32 - 1 (primes.hpp:224-235).
+inline uint32_t base_prime_bound(uint64_t limit)
+{
+    const double r = std::sqrt(double(limit)) + 16.0;
+    return r >= 4294967295.0 ? 0xFFFFFFFFu : uint32_t(r);
+}
+
+/// The odd primes 3 <= p <= n, ascending.
+std::vector<uint32_t> odd_primes_up_to(uint32_t n);
+
+/// The odd primes 3 <= p <= n by a segmented, multi-threaded sieve (the
+/// base primes: up to ~2^32, ~200M entries).
+std::vector<uint32_t> base_primes(uint32_t n, unsigned threads);
+
+/// Deterministic Miller-Rabin (bases 2..37): exact for every uint64_t.
+bool is_prime_u64(uint64_t n);
+
+/// Smallest odd prime p with n - p prime: the candidates @p cands first
+/// (ascending odd primes up to @p cand_bound), then every odd p above it up
+/// to n / 2 (primes.cpp:482-493). 2 for n = 4; 0 when none exists.
+uint64_t cpu_min_goldbach_prime(uint64_t n, const uint32_t* cands, size_t ncand, uint64_t cand_bound);
+
+/// Largest prime < x (x > 3).
+uint64_t prev_prime(uint64_t x);
+
+/// Runs fn(i) for i in [0, n) on up to @p threads threads (fn must not throw).
+void parallel_for(uint64_t n, unsigned threads, const std::function<void(uint64_t)>& fn);
+
+}  // namespace gb

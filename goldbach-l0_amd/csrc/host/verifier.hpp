@@ -1,0 +1,67 @@
+#pragma once
+// GPU verification driver of the dense path: the reference's host-level API
+// (include/verifier.hpp:12-123: SegmentStats, RunStats, run_gpu,
+// print_device_info, verify_cpu), same names, arguments and error behaviour,
+// running the MI355X pipeline through the gbh C ABI (include/gbh.h).
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace gb {
+
+struct SegmentStats {
+    uint64_t seg_start = 0, seg_end = 0, evens = 0;
+    uint64_t tiles = 0;          // window work-groups of the window sieve
+    uint64_t large_items = 0;    // large-prime work-items
+    uint32_t sieve_split = 0;
+    uint32_t mask_count = 0;
+    uint32_t leftovers = 0;
+    double sieve_seconds = 0, large_seconds = 0, check_seconds = 0, offset_seconds = 0, wall_seconds = 0;
+};
+
+struct RidgeEntry {
+    uint64_t n = 0, p = 0, q = 0, seg = 0, found = 0;
+};
+
+struct RunStats {
+    uint64_t limit = 0, start = 0, segment_used = 0, segments = 0, leftovers = 0;
+    double init_seconds = 0, init_cache_s = 0, init_module_s = 0, init_upload_s = 0;
+    double gpu_seconds = 0, total_seconds = 0, evens_per_second = 0;
+    double sieve_seconds = 0, large_seconds = 0, verify_seconds = 0, sync_seconds = 0;
+    double offset_seconds = 0, table_seconds = 0;
+    double count_seconds = 0;    // counter + gap walk (part of verify_seconds)
+    uint64_t total_tiles = 0, total_large_items = 0;
+    uint32_t split_used = 0;
+    uint32_t ridge_min = 0;
+    uint64_t ridge_found = 0, ridge_dropped = 0;
+    std::vector<RidgeEntry> ridge_top;
+    bool stats_on = false;
+    uint64_t stats_pmax = 0, stats_primes = 0, stats_twins = 0, stats_sampled = 0, stats_tail = 0;
+    std::string stats_line;      // the run's GBSTATS line (empty: stats off)
+    std::vector<SegmentStats> per_segment;
+};
+
+/// Verifies Goldbach for every even n in [max(start_even, 4) rounded up to
+/// even, limit] on the GPU (verifier.hpp:71-110 of the reference).
+/// @param spv_path  accepted for interface compatibility; the device code is
+///                  linked into this library (the argument is ignored).
+/// @throws std::runtime_error on any HIP error, device error flag, overflow,
+///         coverage mismatch, statistics file problem or a Goldbach
+///         counterexample; nothing is reported as PASS then.
+RunStats run_gpu(uint64_t limit, uint64_t segment_even, const std::string& spv_path, bool verbose,
+                 uint64_t start_even = 4, uint32_t sieve_split_arg = 0, const char* prime_cache_path = nullptr,
+                 bool collect_segments = false);
+
+void print_device_info();
+
+/// Independent CPU check by trial division for every even n in [4, limit].
+bool verify_cpu(uint64_t limit, bool verbose);
+
+/// One segment's class map built on the GPU, copied back (8 class arrays of
+/// @p words_out words each, stride @p cw words); a validator entry point
+/// (the reference's goldbach-tools debug_map role).
+/// @return the class stride cw; fills Q[8] and nc[8].
+uint32_t gpu_segment_map(uint64_t seg_start, uint64_t limit, uint64_t span, std::vector<uint64_t>& words,
+                         uint64_t Q[8], uint32_t nc[8]);
+
+}  // namespace gb

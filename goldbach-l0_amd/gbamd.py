@@ -1,0 +1,132 @@
+"""Python binding of the MI355X verifier (ctypes over the run-level C ABI,
+include/goldbach_amd.h).
+
+This is the host-side mirror of the reference's entry point
+``gb::run_gpu`` (include/verifier.hpp:103-110): same arguments, same range
+contract, the GBSTATS statistics files written as the reference writes them,
+and failures raised (never reported as PASS). The compute path is the HIP
+library ``libgoldbach_amd.so`` built next to this file; there is no fallback:
+without it, or without a gfx950 GPU, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import pathlib
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "libgoldbach_amd.so"
+CLI_PATH = HERE / "goldbach_amd"
+
+
+class GbaRunResult(ctypes.Structure):
+    _fields_ = [
+        ("limit", ctypes.c_uint64), ("start", ctypes.c_uint64), ("segments", ctypes.c_uint64),
+        ("leftovers", ctypes.c_uint64),
+        ("init_seconds", ctypes.c_double), ("gpu_seconds", ctypes.c_double), ("total_seconds", ctypes.c_double),
+        ("evens_per_second", ctypes.c_double),
+        ("sieve_seconds", ctypes.c_double), ("large_seconds", ctypes.c_double), ("verify_seconds", ctypes.c_double),
+        ("count_seconds", ctypes.c_double), ("sync_seconds", ctypes.c_double),
+        ("total_tiles", ctypes.c_uint64), ("total_large_items", ctypes.c_uint64),
+        ("split_used", ctypes.c_uint32), ("stats_on", ctypes.c_int32),
+        ("stats_pmax", ctypes.c_uint64), ("stats_primes", ctypes.c_uint64), ("stats_twins", ctypes.c_uint64),
+        ("stats_sampled", ctypes.c_uint64), ("stats_tail", ctypes.c_uint64),
+    ]
+
+
+class VerificationError(RuntimeError):
+    """A run that did not pass (device error, coverage, counterexample, ...)."""
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Loads the HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise FileNotFoundError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        lib = ctypes.CDLL(str(LIB_PATH))
+        lib.gba_run_gpu.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32,
+                                    ctypes.POINTER(GbaRunResult), ctypes.c_char_p, ctypes.c_size_t, ctypes.c_char_p,
+                                    ctypes.c_size_t]
+        lib.gba_run_gpu.restype = ctypes.c_int
+        lib.gba_segment_map.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p,
+                                        ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint32),
+                                        ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint32),
+                                        ctypes.c_char_p, ctypes.c_size_t]
+        lib.gba_segment_map.restype = ctypes.c_int
+        lib.gba_version.restype = ctypes.c_char_p
+        _lib = lib
+    return _lib
+
+
+def version() -> str:
+    return load().gba_version().decode()
+
+
+def run_gpu(limit: int, segment_even: int = 467_500_000, verbose: bool = False, start_even: int = 4,
+            sieve_split: int = 0) -> dict:
+    """Verifies every even n in [start_even, limit] on the GPU.
+
+    Returns the run's figures and its GBSTATS line ("" when statistics are
+    off). Raises VerificationError when the run does not pass.
+    """
+    lib = load()
+    res = GbaRunResult()
+    line = ctypes.create_string_buffer(1 << 20)
+    err = ctypes.create_string_buffer(4096)
+    rc = lib.gba_run_gpu(int(limit), int(segment_even), int(verbose), int(start_even), int(sieve_split),
+                         ctypes.byref(res), line, len(line), err, len(err))
+    if rc:
+        raise VerificationError(err.value.decode())
+    out = {name: getattr(res, name) for name, _ in GbaRunResult._fields_}
+    out["stats_line"] = line.value.decode()
+    return out
+
+
+def segment_map(seg_start: int, limit: int, span: int):
+    """One segment's class map from the GPU: (words [8, cw] uint64, Q[8], nc[8])."""
+    import numpy as np
+
+    lib = load()
+    # upper bound of the stride: the merged last segment spans below 2 x span
+    cap = 8 * ((span * 2 + 20000) // 30 // 64 + (1 << 20) // 64 + 64)
+    words = np.zeros(cap, dtype=np.uint64)
+    cw = ctypes.c_uint32(0)
+    Q = (ctypes.c_uint64 * 8)()
+    nc = (ctypes.c_uint32 * 8)()
+    err = ctypes.create_string_buffer(4096)
+    rc = lib.gba_segment_map(int(seg_start), int(limit), int(span), words.ctypes.data, cap, ctypes.byref(cw), Q, nc,
+                             err, len(err))
+    if rc:
+        raise VerificationError(err.value.decode())
+    n = 8 * cw.value
+    if n > cap:
+        raise VerificationError("segment map larger than the buffer")
+    return words[:n].reshape(8, cw.value), list(Q), list(nc)
+
+
+def env_scope(**kv):
+    """Context manager that sets GB_* knobs for the calls inside it."""
+    import contextlib
+
+    @contextlib.contextmanager
+    def scope():
+        old = {k: os.environ.get(k) for k in kv}
+        try:
+            for k, v in kv.items():
+                if v is None:
+                    os.environ.pop(k, None)
+                else:
+                    os.environ[k] = str(v)
+            yield
+        finally:
+            for k, v in old.items():
+                if v is None:
+                    os.environ.pop(k, None)
+                else:
+                    os.environ[k] = v
+
+    return scope()

@@ -1,0 +1,215 @@
+/* gbh.h — device-level C ABI of the MI355X (gfx950) segment pipeline.
+ *
+ * This layer replaces the reference's GPU runtime boundary and kernel ABI:
+ * the Level Zero wrapper include/l0_context.hpp:14-215 (Context, Module,
+ * Kernel::set_arg/launch, alloc_device/alloc_shared, host_sync, timestamp
+ * events) together with the positional kernel argument lists that
+ * src/verifier.cpp sets in enqueue_sieve / enqueue_large_primes /
+ * enqueue_check (verifier.cpp:1808-2330). Every stage launch takes one POD
+ * argument struct instead of set_arg calls; all pointers are device
+ * pointers obtained from gbh_alloc, sizes are in bytes unless stated.
+ *
+ * Error convention: every function returns 0 on success and a non-zero
+ * GBH_E_* code on failure; gbh_last_error() then holds the message (the
+ * ZE_CHECK text of l0_context.hpp:14-20 becomes this string). Callers that
+ * want the reference's fail-closed behaviour turn a non-zero status into an
+ * exception (the C++ driver does).
+ *
+ * Threading: one driver thread per gbh_ctx (as the reference's single
+ * immediate command list). Streams are in-order HIP streams at the default
+ * priority.
+ */
+#ifndef GBH_H
+#define GBH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    GBH_OK = 0,
+    GBH_E_HIP = 1,        /* a HIP runtime call failed */
+    GBH_E_ARG = 2,        /* an argument violates the stage's contract */
+    GBH_E_NODEV = 3,      /* no usable gfx950 device */
+    GBH_E_NOMEM = 4
+};
+
+/* Layout constants shared by host and kernels. */
+#define GBH_CLASSES 8            /* residues 1 7 11 13 17 19 23 29 mod 30 */
+#define GBH_EVEN_CLASSES 15      /* n mod 30 = 0, 2, ..., 28 */
+#define GBH_NCAND 1206           /* odd primes 3..9781 (MAX_GB_PRIME, config.hpp:86) */
+#define GBH_NSMALL 15            /* mask-stage primes 7..61 (MASK_MAX_P = 64, config.hpp:141) */
+#define GBH_GAP_BINS 1024        /* gap histogram bins g / 2 (GAPS30_HBINS, gstats.hpp:75) */
+#define GBH_GAP_TAIL 700         /* tail gaps (GAPS30_TAIL, gstats.hpp:74) */
+#define GBH_GAP_TCAP 4096        /* tail entries per segment (GAPS30_TCAP, gstats.hpp:114) */
+#define GBH_LEFT_CAP 65536       /* leftovers per segment copied back */
+#define GBH_CHECK_WG 256         /* check_a work-group: 256 lanes x 64 evens of one class */
+
+typedef struct gbh_ctx gbh_ctx;
+typedef struct gbh_event gbh_event;
+
+/* Per-segment geometry of the mod-30 class map (host_math.hpp:53-107
+ * SegmentGeometry / ClassGeo): bit t of class array c <-> q = Q[c] + 30 t. */
+typedef struct {
+    uint64_t seg_start, seg_end;   /* first / last even of the segment */
+    uint64_t q_first, q_last;      /* odd q range sieved */
+    uint64_t Q[GBH_CLASSES];       /* first q of each class array */
+    uint32_t nc[GBH_CLASSES];      /* bits of each class array */
+    uint32_t cw;                   /* class array stride, 64-bit words */
+    uint32_t wbits;                /* LDS window bits (power of two, 2^15..2^20) */
+    uint32_t nwin;                 /* windows per class array */
+    uint32_t pad0;
+    uint64_t restore[GBH_CLASSES]; /* bits of word 0 set again after the mask stage (q = p <= 61) */
+    uint32_t s0[GBH_CLASSES][16];  /* mask primes: class bit of every multiple, residue mod p */
+} gbh_geom;
+
+/* Stage 1: first class bit of every window prime (reference mid_setup,
+ * goldbach.cl:3528-3563, and the host's first_bits_class_range,
+ * host_math.cpp:80-113). */
+typedef struct {
+    const gbh_geom* geom;          /* host pointer: copied into the launch */
+    const uint32_t* primes;        /* window primes (device), ascending, all >= 67 */
+    uint32_t nprimes;
+    uint32_t* i0;                  /* [8][nprimes] first bit, ~0u: none in the segment */
+} gbh_setup_args;
+
+/* Stage 2: LDS window sieve of every class array (reference make_prime_map +
+ * mid_window, goldbach.cl:1092-1608, 3600-3701): masks for 7..61, marks of
+ * the window primes, coalesced store of the window. */
+typedef struct {
+    const gbh_geom* geom;
+    const uint32_t* primes;        /* window primes (device) */
+    uint32_t nprimes;
+    uint32_t ncoop;                /* primes [0, ncoop) are marked by the whole work-group */
+    const uint32_t* i0;            /* stage-1 table */
+    uint64_t* map;                 /* [8][cw] words */
+} gbh_window_args;
+
+/* Stage 3: direct marks of the large primes (p >= the window bound) with
+ * global atomics (reference sieve_large_packed + hit lists,
+ * goldbach.cl:1738-1849, 3434-3501, here without the batch lists). */
+typedef struct {
+    const gbh_geom* geom;
+    const uint32_t* primes;        /* large primes (device), ascending */
+    uint32_t nprimes;
+    uint32_t pad0;
+    uint64_t* map;
+} gbh_large_args;
+
+/* Per-segment outputs of the checker and the counter, copied back as one
+ * block (reference: the slot's shared USM counters, verifier.cpp:42-87). */
+typedef struct {
+    uint32_t acct[2];              /* evens resolved by phase A / phase B */
+    uint32_t pending_n;            /* evens phase A left */
+    uint32_t left_n;               /* evens no candidate resolved (CPU fallback) */
+    uint32_t err;                  /* error flags (1: pending overflow, 2: tail-list overflow) */
+    uint32_t cnt[GBH_CLASSES];     /* primes per class array in the counting range */
+    uint32_t twins[3];             /* twin pairs (11,13) (17,19) (29,31) */
+    uint32_t pfirst, plast;        /* first / last counted prime: position q - q_first (~0u / 0: none) */
+    uint32_t gtail_n;              /* tail gaps (g >= 700) inside the segment */
+    uint32_t pad0;
+    uint64_t gmax;                 /* largest inner gap: g << 32 | ~(upper position) (0: none) */
+    uint32_t ghist[GBH_GAP_BINS + 1];   /* inner gaps by g / 2; [GBH_GAP_BINS]: overflow */
+    uint32_t pad1;
+    uint32_t gtail[2 * GBH_GAP_TCAP];   /* (upper position, g) */
+    uint32_t hist[GBH_EVEN_CLASSES * GBH_NCAND];   /* p(n) by even class and candidate slot */
+    uint32_t first[GBH_NCAND];     /* smallest even index with p(n) = candidate (by prime index; ~0u none) */
+    uint32_t left[GBH_LEFT_CAP];   /* even indices of the leftovers */
+} gbh_segout;
+
+/* Stage 4: checker phase A (reference check30_a, goldbach.cl:2028-2189):
+ * exact ascending candidates, 64 evens per lane, for the first ka candidates
+ * of every even class; histogram of the resolving candidate when hist_on. */
+typedef struct {
+    const gbh_geom* geom;
+    const uint64_t* map;
+    uint64_t n_gpu;                /* first even the GPU checks (smaller ones: CPU) */
+    uint32_t ka;                   /* phase-A candidates per even class (<= 128) */
+    uint32_t hist_on;              /* record p(n) of every even (sampled segment) */
+    uint32_t* coff;                /* [15][GBH_NCAND] per-segment candidate table (written here) */
+    uint32_t* pending;             /* even indices left for phase B */
+    uint32_t pending_cap;
+    uint32_t pad0;
+    gbh_segout* out;
+} gbh_check_args;
+
+/* Stage 5: phase B (reference check30_b, goldbach.cl:2210-2306): the
+ * remaining candidates of every pending even, ascending; the first hit is
+ * p(n). */
+typedef struct {
+    const gbh_geom* geom;
+    const uint64_t* map;
+    uint64_t n_gpu;
+    uint32_t ka;                   /* first candidate slot to probe */
+    uint32_t tail_t;               /* first occurrences are tracked for p >= tail_t */
+    const uint32_t* coff;          /* the table check_a wrote */
+    const uint32_t* pending;
+    uint32_t pending_max;          /* grid bound: entries beyond out->pending_n are skipped */
+    uint32_t pad0;
+    gbh_segout* out;
+} gbh_checkb_args;
+
+/* Stage 6: prime / twin counts and the gap walk of the counting range
+ * (reference count30 + gap_tail_exact, goldbach.cl:2499-2825). */
+typedef struct {
+    const gbh_geom* geom;
+    const uint64_t* map;
+    /* the eight class arrays in ascending phase (position order inside a
+     * block of 30): class index, position of bit 0 (q - q_first), counted
+     * bits [lo, hi) (count30_limits, gstats.hpp:262-296) */
+    uint32_t cls[GBH_CLASSES], pos0[GBH_CLASSES], lo[GBH_CLASSES], hi[GBH_CLASSES];
+    /* twin pairs (11,13) (17,19) (29,31): lower class, partner class,
+     * partner word offset (0 or 1 bit), counted bits [lo, hi) of the lower */
+    uint32_t tk[3], tk2[3], td[3], tlo[3], thi[3];
+    uint32_t nwords;               /* words walked per class array */
+    uint32_t* scratch;             /* >= 2 * nwords u32: per-word first / last prime positions */
+    uint32_t scratch_words;
+    uint32_t pad0;
+    gbh_segout* out;
+} gbh_count_args;
+
+typedef struct {
+    char name[128];
+    char arch[64];
+    int cus;
+    int xcds_hint;
+    size_t lds_per_block;
+    size_t hbm_bytes;
+    int clock_khz;
+    int mem_clock_khz;
+} gbh_device_info_t;
+
+/* ---- context, memory, streams, events ---- */
+int gbh_open(int device, gbh_ctx** out);
+void gbh_close(gbh_ctx* ctx);
+const char* gbh_last_error(void);
+int gbh_device_info(gbh_ctx* ctx, gbh_device_info_t* info);
+/* kind: 0 device, 1 pinned host (device-visible) */
+int gbh_alloc(gbh_ctx* ctx, size_t bytes, int kind, void** out);
+int gbh_free(gbh_ctx* ctx, void* ptr, int kind);
+int gbh_copy_h2d(gbh_ctx* ctx, int stream, void* dst, const void* src, size_t bytes);
+int gbh_copy_d2h(gbh_ctx* ctx, int stream, void* dst, const void* src, size_t bytes);
+int gbh_zero(gbh_ctx* ctx, int stream, void* dst, size_t bytes);  /* fill kernel (graph-safe) */
+int gbh_fill32(gbh_ctx* ctx, int stream, void* dst, size_t count, uint32_t value);
+int gbh_stream_sync(gbh_ctx* ctx, int stream);
+int gbh_event_create(gbh_ctx* ctx, gbh_event** ev);
+int gbh_event_destroy(gbh_ctx* ctx, gbh_event* ev);
+int gbh_event_record(gbh_ctx* ctx, int stream, gbh_event* ev);
+int gbh_event_sync(gbh_ctx* ctx, gbh_event* ev);
+int gbh_event_elapsed_ms(gbh_ctx* ctx, gbh_event* a, gbh_event* b, float* ms);
+
+/* ---- stage launches (all asynchronous on `stream`) ---- */
+int gbh_launch_setup(gbh_ctx* ctx, int stream, const gbh_setup_args* a);
+int gbh_launch_window(gbh_ctx* ctx, int stream, const gbh_window_args* a);
+int gbh_launch_large(gbh_ctx* ctx, int stream, const gbh_large_args* a);
+int gbh_launch_check_a(gbh_ctx* ctx, int stream, const gbh_check_args* a);
+int gbh_launch_check_b(gbh_ctx* ctx, int stream, const gbh_checkb_args* a);
+int gbh_launch_count(gbh_ctx* ctx, int stream, const gbh_count_args* a);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

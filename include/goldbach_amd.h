@@ -1,0 +1,56 @@
+/* goldbach_amd.h — run-level C ABI of the MI355X verifier.
+ *
+ * gba_run_gpu replaces the reference's host entry point
+ *     RunStats gb::run_gpu(uint64_t limit, uint64_t segment_even,
+ *                          const std::string& spv_path, bool verbose,
+ *                          uint64_t start_even, uint32_t sieve_split_arg,
+ *                          const char* prime_cache_path, bool collect_segments)
+ * (include/verifier.hpp:103-110, src/verifier.cpp:2646) for callers that bind
+ * C: plain integers in, a POD result out. Same range contract, same
+ * fail-closed behaviour: any failure returns a non-zero status and nothing is
+ * reported as PASS; the message is copied to `err`. The statistics files
+ * (GB_STATS_STATE / GB_STATS_LOG / ...) are written exactly as by the C++
+ * entry point.
+ *
+ * gba_segment_map exposes one segment's class map built on the GPU (the
+ * reference's map validator role, tools/debug_map, which its tree does not
+ * ship).
+ */
+#ifndef GOLDBACH_AMD_H
+#define GOLDBACH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    uint64_t limit, start, segments, leftovers;
+    double init_seconds, gpu_seconds, total_seconds, evens_per_second;
+    double sieve_seconds, large_seconds, verify_seconds, count_seconds, sync_seconds;
+    uint64_t total_tiles, total_large_items;
+    uint32_t split_used;
+    int32_t stats_on;
+    uint64_t stats_pmax, stats_primes, stats_twins, stats_sampled, stats_tail;
+} gba_run_result;
+
+/* Returns 0 on PASS. stats_line (if non-null, capacity stats_cap) receives
+ * the run's GBSTATS line when statistics were on. */
+int gba_run_gpu(uint64_t limit, uint64_t segment_even, int verbose, uint64_t start_even, uint32_t sieve_split,
+                gba_run_result* out, char* stats_line, size_t stats_cap, char* err, size_t err_cap);
+
+/* One segment's class map: words [8 * cw] (cw returned in *cw_out, at most
+ * cap words are written), Q[8] first q per class, nc[8] bits per class.
+ * Returns 0 on success. */
+int gba_segment_map(uint64_t seg_start, uint64_t limit, uint64_t span, uint64_t* words, size_t cap,
+                    uint32_t* cw_out, uint64_t* Q, uint32_t* nc, char* err, size_t err_cap);
+
+/* The library's version string. */
+const char* gba_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,128 @@
+"""GPU parity: the HIP pipeline (through the C ABI) against the CPU oracle
+and the reference's golden stats_ref output.
+
+* the class map of a segment equals Miller-Rabin primality of every q;
+* a run's GBSTATS line and state file pass the reference's own comparison
+  with stats_ref on the golden windows the reference validates on
+  (tests/gpu_validate.sh:45-80 windows and segment sizes);
+* on small windows the GBSTATS line (after its version field) and the state
+  histograms are byte-identical to the oracle's restatement, at several
+  sampling and gap settings.
+"""
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import compare
+import gbamd
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _isprime_many(nums):
+    out = subprocess.run([str(compare.ORACLE), "isprime", "-"], input="\n".join(map(str, nums)) + "\n",
+                         capture_output=True, text=True, check=True).stdout.split()
+    return np.array([int(x) for x in out], dtype=np.uint8)
+
+
+@pytest.mark.parametrize("seg_start,limit,span", [
+    (1_000_000_000_000, 1_000_002_000_000, 1_000_000),
+    (4_000_000_000_000_000_000, 4_000_000_000_004_000_000, 2_000_000),
+    (18446742974194923840, 18446742974197923840, 1_500_000),
+    (10_000, 2_000_000, 1_000_000),
+])
+def test_segment_map_is_exact(seg_start, limit, span):
+    _say(f"map case [{seg_start}, {limit}] span {span}")
+    words, Q, nc = gbamd.segment_map(seg_start, limit, span)
+    seg_end = limit if limit - seg_start <= span else seg_start + span - 2
+    q_last = seg_end - 3
+    for c in range(8):
+        n = nc[c]
+        bits = np.unpackbits(words[c].view(np.uint8), bitorder="little")[:n]
+        q = [Q[c] + 30 * t for t in range(n)]
+        assert q[-1] <= q_last
+        want = _isprime_many(q)
+        bad = np.nonzero(bits != want)[0]
+        assert bad.size == 0, (c, [(int(t), Q[c] + 30 * int(t), int(bits[t]), int(want[t])) for t in bad[:5]])
+
+
+# (fixture, --segment, environment) as the reference validates them
+WINDOWS = [
+    ("w1e6", 500_000, {}),
+    ("w1e12s", 500_000, {"GB_STATS_SAMPLE": "1"}),
+    ("w4e18s", 600_000, {}),
+    ("w2p63s", 400_000, {"GB_STATS_SAMPLE": "2"}),
+    ("wgap1476", 1_000_000, {}),
+    ("wgap1476", 1_000_000, {"GB_GAPS": "sampled"}),
+    ("wgap1442", 1_000_000, {}),
+    ("wtop", 1_000_000, {}),
+    ("w4e18", 2_000_000, {}),
+    ("w44e18", 1_000_000, {"GB_STATS_SAMPLE": "1"}),
+    ("w5e18end", 3_000_000, {}),
+    ("w1e12", 1_000_000, {"GB_STATS_SAMPLE": "5"}),
+    ("w2p63", 1_000_000, {}),
+    ("wgapcross", 1_000_000, {}),
+    ("w4e18l", 467_500_000, {}),
+    ("w4e18l", 20_000_000, {"GB_STATS_SAMPLE": "3"}),
+    ("cfg1", 467_500_000, {}),
+]
+
+
+@pytest.mark.parametrize("name,segment,env", WINDOWS, ids=[f"{w[0]}-{w[1]}-{'-'.join(w[2].values()) or 'dflt'}" for w in WINDOWS])
+def test_run_matches_reference_stats_ref(stats_env, monkeypatch, name, segment, env):
+    a, b, ref = compare.load_fixture(GOLDEN / f"ref_{name}.txt")
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    _say(f"run case {name} [{a}, {b}] segment {segment} env {env}")
+    r = gbamd.run_gpu(b, segment, start_even=a)
+    assert r["leftovers"] == 0
+    log = (stats_env / "st.log").read_text().strip().splitlines()[-1]
+    assert log == r["stats_line"].strip()
+    bad = compare.compare_ref(log, (stats_env / "st.state").read_text(), ref)
+    assert not bad, bad
+
+
+SMALL = [
+    (1_000_000, 3_000_000, 500_000, "256", "exact"),
+    (1_000_000, 3_000_000, 300_000, "2", "sampled"),
+    (4_000_000_000_000_000_000, 4_000_000_000_002_000_000, 700_000, "1", "exact"),
+    (1425172824437600000, 1425172824437800000, 60_000, "3", "sampled"),
+    (1425172824437600000, 1425172824437800000, 60_000, "256", "0"),
+    (9223372036853775808, 9223372036855775808, 250_000, "4", "exact"),
+]
+
+
+@pytest.mark.parametrize("a,b,seg,sample,gaps", SMALL)
+def test_gbstats_line_byte_exact(stats_env, monkeypatch, a, b, seg, sample, gaps):
+    monkeypatch.setenv("GB_STATS_SAMPLE", sample)
+    monkeypatch.setenv("GB_GAPS", gaps)
+    _say(f"gbstats case [{a}, {b}] segment {seg} sample {sample} gaps {gaps}")
+    r = gbamd.run_gpu(b, seg, start_even=a)
+    want_line, want_recs = compare.oracle_gbstats(a, b, seg, int(sample), gaps)
+    got = compare.strip_version(r["stats_line"])
+    if gaps == "0":      # no gap fields at all
+        want_line = " ".join(t for t in want_line.split() if not t.startswith(("gaps=", "gd1=", "gmax=", "gtail=", "gwalk=", "gsmom=", "gsamp=")))
+    assert got == want_line
+    assert compare.state_hist_records((stats_env / "st.state").read_text()) == want_recs
+
+
+def test_state_merge_and_overlap_refusal(stats_env):
+    """A second run over the same range is not counted twice; an adjacent
+    run merges into the campaign totals."""
+    _say("state merge case")
+    gbamd.run_gpu(2_000_000, 500_000, start_even=1_000_000)
+    st1 = (stats_env / "st.state").read_text()
+    gbamd.run_gpu(2_000_000, 500_000, start_even=1_000_000)
+    assert (stats_env / "st.state").read_text() == st1
+    gbamd.run_gpu(3_000_000, 500_000, start_even=2_000_002)
+    st = compare.kv(" ".join(l.replace(" ", "=", 1) for l in (stats_env / "st.state").read_text().splitlines()
+                             if l.startswith(("evens ", "range "))))
+    assert st["range"] == "1000000 3000000"
