@@ -19,11 +19,9 @@ extern "C" {
 
 const char* gba_version(void) { return gb::VERSION; }
 
-int gba_run_gpu(uint64_t limit, uint64_t segment_even, int verbose, uint64_t start_even, uint32_t sieve_split,
-                gba_run_result* out, char* stats_line, size_t stats_cap, char* err, size_t err_cap)
+namespace {
+void fill(const gb::RunStats& s, gba_run_result* out)
 {
-    try {
-        const gb::RunStats s = gb::run_gpu(limit, segment_even, "", verbose != 0, start_even, sieve_split);
         if (out) {
             std::memset(out, 0, sizeof *out);
             out->limit = s.limit;
@@ -49,6 +47,52 @@ int gba_run_gpu(uint64_t limit, uint64_t segment_even, int verbose, uint64_t sta
             out->stats_sampled = s.stats_sampled;
             out->stats_tail = s.stats_tail;
         }
+}
+}  // namespace
+
+struct gba_engine {
+    gb::Engine e;
+    gba_engine(uint64_t l, uint64_t s) : e(l, s) {}
+};
+
+int gba_engine_open(uint64_t max_limit, uint64_t max_span, gba_engine** out, double* init_seconds, char* err,
+                    size_t err_cap)
+{
+    try {
+        *out = new gba_engine(max_limit, max_span);
+        if (init_seconds) *init_seconds = (*out)->e.init_seconds();
+        put(err, err_cap, "");
+        return 0;
+    } catch (const std::exception& ex) {
+        *out = nullptr;
+        put(err, err_cap, ex.what());
+        return 1;
+    }
+}
+
+int gba_engine_run(gba_engine* e, uint64_t limit, uint64_t segment_even, uint64_t start_even, gba_run_result* out,
+                   char* stats_line, size_t stats_cap, char* err, size_t err_cap)
+{
+    try {
+        const gb::RunStats s = e->e.run(limit, segment_even, false, start_even);
+        fill(s, out);
+        put(stats_line, stats_cap, s.stats_line.c_str());
+        put(err, err_cap, "");
+        return 0;
+    } catch (const std::exception& ex) {
+        put(err, err_cap, ex.what());
+        return 1;
+    }
+}
+
+void gba_engine_close(gba_engine* e) { delete e; }
+
+int gba_run_gpu(uint64_t limit, uint64_t segment_even, int verbose, uint64_t start_even, uint32_t sieve_split,
+                gba_run_result* out, char* stats_line, size_t stats_cap, char* err, size_t err_cap)
+{
+    try {
+        const gb::RunStats s = gb::run_gpu(limit, segment_even, "", verbose != 0, start_even, sieve_split);
+        fill(s, out);
         put(stats_line, stats_cap, s.stats_line.c_str());
         put(err, err_cap, "");
         return 0;

@@ -105,6 +105,98 @@ struct Candidates {
 
 enum { EV_BEGIN, EV_WINDOW, EV_LARGE, EV_CHECK, EV_COUNT, EV_END, EV_N };
 
+// Knobs of removed reference pipelines are refused, not ignored: a run that
+// asks for one would otherwise silently run something else
+// (src/verifier.cpp:577-589 of the reference).
+void refuse_removed_knobs()
+{
+    for (const char* gone : {"GB_SPLIT", "GB_SPLIT_CAP", "GB_SPLIT_STAGES", "GB_TWIN", "GB_TWIN_CAP", "GB_C30",
+                             "GB_CLASSMAP", "GB_HITLIST", "GB_MASK_GROUPS", "GB_C30_LATE", "GB_C30_PREB",
+                             "GB_ML_OVERLAP", "GB_HL_LAM2", "GB_HL_CAPG", "GB_HLH_WG", "GB_HL_UNCOMP", "GB_MT_UNCOMP",
+                             "GB_MASK_XP", "GB_MED_ITEMS", "GB_MINP_FLOOR", "GB_HL_FILLDBG"})
+        if (env::present(gone))
+            throw std::runtime_error(std::string(gone) + " is a removed knob (the mod-30 map and checker are the only "
+                                     "pipeline)");
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- Engine state
+
+struct Engine::State {
+    clock_::time_point t0;
+    uint64_t max_limit = 0, max_span = 0;
+    std::vector<uint32_t> base;          // odd primes <= sqrt(max_limit) + 16
+    uint32_t win_lo = 0, win_hi = 0;     // window primes base[win_lo, win_hi), large base[win_hi, ...)
+    uint32_t wbits = 0, ka = 0, ncoop = 0;
+    double init_s = 0, cache_s = 0, module_s = 0, upload_s = 0;
+    std::optional<Ctx> ctx;
+    gbh_ctx* c = nullptr;
+    Buf d_win, d_large, d_i0, d_map, d_coff, d_pending, d_scratch;
+    Buf d_out[2], h_out[2];
+    uint64_t cw_cap = 0;
+    uint32_t pending_cap = 0, scratch_words = 0;
+    Event ev[2][EV_N];
+    Candidates cand;
+
+    State(uint64_t max_limit_, uint64_t max_span_)
+    {
+        t0 = clock_::now();
+        require_supported_limit(max_limit_);
+        max_limit = max_limit_;
+        max_span = std::max<uint64_t>(max_span_ + (max_span_ & 1), 4);
+        if (max_span > (uint64_t(1) << 35)) throw std::runtime_error("segment span above 2^35 integers");
+        const uint32_t kb = env::u32("GB_WIN_KB", WIN_KB_DEFAULT, 4, 128);
+        if (kb & (kb - 1)) throw std::runtime_error("GB_WIN_KB must be a power of two");
+        wbits = kb * 8192;
+        ka = env::u32("GB_KA", KA_DEFAULT, 1, 128);
+        // base primes (the reference loads them from its prime cache;
+        // here a segmented sieve on the host threads)
+        const auto tc = clock_::now();
+        base = base_primes(std::max<uint32_t>(base_prime_bound(max_limit), 64), host_threads());
+        cache_s = since(tc);
+        win_lo = uint32_t(std::lower_bound(base.begin(), base.end(), 67u) - base.begin());
+        const uint64_t win_p = env::u64("GB_WIN_P", WIN_P_DEFAULT, 67, UINT32_MAX);
+        win_hi = std::max(win_lo, uint32_t(std::lower_bound(base.begin(), base.end(), win_p) - base.begin()));
+        // window primes with at least two marks per lane per window are
+        // marked by the whole work-group
+        while (win_lo + ncoop < win_hi && base[win_lo + ncoop] < wbits / 128) ++ncoop;
+        const auto tm = clock_::now();
+        ctx.emplace();
+        c = ctx->c;
+        module_s = since(tm);
+        const auto tu = clock_::now();
+        const uint32_t nwin = win_hi - win_lo, nlarge = uint32_t(base.size()) - win_hi;
+        d_win = Buf(c, size_t(nwin) * 4, 0);
+        d_large = Buf(c, size_t(nlarge) * 4, 0);
+        if (nwin) check(gbh_copy_h2d(c, 0, d_win.p, base.data() + win_lo, size_t(nwin) * 4), "upload window primes");
+        if (nlarge) check(gbh_copy_h2d(c, 0, d_large.p, base.data() + win_hi, size_t(nlarge) * 4), "upload large primes");
+        d_i0 = Buf(c, size_t(nwin) * 8 * 4, 0);
+        // the largest segment: the merged last one spans below 2 x span
+        const uint64_t qodds_max = (2 * max_span + MAX_GB_PRIME) / 2 + 2;
+        const uint64_t nc_max = (qodds_max + 14) / 15 + 1;
+        const uint64_t nwin_max = (nc_max + wbits - 1) / wbits;
+        cw_cap = nwin_max * (wbits / 64) + 16;
+        d_map = Buf(c, size_t(8) * cw_cap * 8, 0);
+        check(gbh_zero(c, 0, d_map.p, size_t(8) * cw_cap * 8), "zero map");
+        d_coff = Buf(c, size_t(15) * GBH_NCAND * 4, 0);
+        pending_cap = uint32_t(std::min<uint64_t>((2 * max_span + 4) / 4 + 4096, UINT32_MAX / 2));
+        d_pending = Buf(c, size_t(pending_cap) * 4, 0);
+        scratch_words = uint32_t(2 * (nc_max / 64 + 4));
+        d_scratch = Buf(c, size_t(scratch_words) * 4, 0);
+        for (int s = 0; s < 2; ++s) {
+            d_out[s] = Buf(c, sizeof(gbh_segout), 0);
+            h_out[s] = Buf(c, sizeof(gbh_segout), 1);
+            for (auto& e : ev[s]) e.create(c);
+        }
+        check(gbh_stream_sync(c, 0), "upload sync");
+        upload_s = since(tu);
+        init_s = since(t0);
+    }
+};
+
+namespace {
+
 struct SegJob {
     SegmentGeometry sg;
     ClassGeo cg;
@@ -117,11 +209,14 @@ struct SegJob {
     clock_::time_point t_wall;
 };
 
+// One run over [start, limit] on an engine's resident state (the
+// reference's DenseRun phases: resolve_range, statistics set-up, the
+// two-slot segment loop, finish and commit).
 class DenseRun {
 public:
-    DenseRun(uint64_t limit, uint64_t segment_even, bool verbose, uint64_t start_even, uint32_t split_arg,
-             bool collect)
-        : limit_(limit), span_(segment_even), verbose_(verbose), start_(start_even), split_arg_(split_arg),
+    DenseRun(Engine::State& e, uint64_t limit, uint64_t segment_even, bool verbose, uint64_t start_even,
+             uint32_t split_arg, bool collect)
+        : E(e), limit_(limit), span_(segment_even), verbose_(verbose), start_(start_even), split_arg_(split_arg),
           collect_(collect)
     {
     }
@@ -129,12 +224,11 @@ public:
 
 private:
     void resolve_range();
-    void load_base_primes();
     void configure_stats();
-    void setup_device();
     void enqueue(SegJob& j);
     void collect(SegJob& j);
 
+    Engine::State& E;
     uint64_t limit_, span_;
     bool verbose_;
     uint64_t start_;
@@ -143,27 +237,11 @@ private:
     RunStats rs_;
     uint64_t total_evens_ = 0;
     clock_::time_point t_start_;
-
-    // base primes and the stage split
-    std::vector<uint32_t> base_;
-    uint32_t win_lo_ = 0, win_hi_ = 0;   // window primes base_[win_lo_, win_hi_), large primes base_[win_hi_, end)
-    uint32_t wbits_ = 0, ka_ = 0, ncoop_ = 0;
-    Candidates cand_;
-
-    // statistics
+    uint32_t nwin_ = 0, nlarge_ = 0, split_ = 0;   // this run's window / large prime counts
     bool stats_on_ = false, gaps_on_ = false, gaps_exact_ = false;
     uint32_t sample_ = 256;
     std::optional<GoldbachStats> gs_;
     uint64_t sampled_total_ = 0, evens_done_ = 0;
-
-    // device
-    std::optional<Ctx> ctx_;
-    gbh_ctx* c_ = nullptr;
-    Buf d_win_, d_large_, d_i0_, d_map_, d_coff_, d_pending_, d_scratch_;
-    Buf d_out_[2], h_out_[2];
-    uint64_t cw_cap_ = 0;
-    uint32_t pending_cap_ = 0, scratch_words_ = 0;
-    Event ev_[2][EV_N];
     bool phase_trace_ = false;
     double t_sieve_ = 0, t_large_ = 0, t_check_ = 0, t_count_ = 0;
 };
@@ -171,17 +249,11 @@ private:
 void DenseRun::resolve_range()
 {
     if (limit_ < 4) throw std::runtime_error("limit < 4: no even n >= 4 to verify");
-    // knobs of removed reference pipelines are refused, not ignored
-    for (const char* gone : {"GB_SPLIT", "GB_SPLIT_CAP", "GB_SPLIT_STAGES", "GB_TWIN", "GB_TWIN_CAP", "GB_C30",
-                             "GB_CLASSMAP", "GB_HITLIST", "GB_MASK_GROUPS", "GB_C30_LATE", "GB_C30_PREB",
-                             "GB_ML_OVERLAP", "GB_HL_LAM2", "GB_HL_CAPG", "GB_HLH_WG", "GB_HL_UNCOMP", "GB_MT_UNCOMP",
-                             "GB_MASK_XP", "GB_MED_ITEMS", "GB_MINP_FLOOR", "GB_HL_FILLDBG"})
-        if (env::present(gone))
-            throw std::runtime_error(std::string(gone) + " is a removed knob (the mod-30 map and checker are the only "
-                                     "pipeline)");
+    refuse_removed_knobs();
     require_supported_limit(limit_);
     limit_ &= ~uint64_t(1);
     rs_.limit = limit_;
+    if (limit_ > E.max_limit) throw std::runtime_error("limit above the engine's prime bound");
     if (start_ < 4) start_ = 4;
     if (start_ & 1) start_ += 1;
     if (start_ > limit_) throw std::runtime_error("no even n in [start, limit]");
@@ -196,31 +268,27 @@ void DenseRun::resolve_range()
                                      "--cpu-check 4 verifies it)");
         span_ = std::max<uint64_t>(span_, 4);
     }
-    // the device offsets are 32-bit: class arrays of one segment below 2^31 bits
-    if (span_ > (uint64_t(1) << 35)) throw std::runtime_error("segment span above 2^35 integers");
-}
-
-void DenseRun::load_base_primes()
-{
-    const auto t0 = clock_::now();
+    if (span_ > E.max_span) throw std::runtime_error("segment span above the engine's buffers");
+    // this run's primes: those <= sqrt(limit) + 16
     const uint32_t bound = std::max<uint32_t>(base_prime_bound(limit_), 64);
-    base_ = base_primes(bound, host_threads());
-    rs_.init_cache_s = since(t0);
-    // base_ = 3, 5, 7, ..., 61 (mask stage), 67 ... (window), large
-    win_lo_ = uint32_t(std::lower_bound(base_.begin(), base_.end(), 67u) - base_.begin());
-    const uint64_t win_p = env::u64("GB_WIN_P", WIN_P_DEFAULT, 67, UINT32_MAX);
-    uint32_t split = uint32_t(std::lower_bound(base_.begin(), base_.end(), uint64_t(win_p)) - base_.begin());
-    if (split_arg_ > 0) split = std::min<uint32_t>(split_arg_, uint32_t(base_.size()));   // --split: prime index
-    win_hi_ = std::max(split, win_lo_);
-    const uint32_t kb = env::u32("GB_WIN_KB", WIN_KB_DEFAULT, 4, 128);
-    if (kb & (kb - 1)) throw std::runtime_error("GB_WIN_KB must be a power of two");
-    wbits_ = kb * 8192;
-    ka_ = env::u32("GB_KA", KA_DEFAULT, 1, 128);
-    // window primes with at least two marks per lane per window are shared
-    // by the whole work-group
-    ncoop_ = 0;
-    while (win_lo_ + ncoop_ < win_hi_ && base_[win_lo_ + ncoop_] < wbits_ / 128) ++ncoop_;
-    rs_.split_used = win_hi_;
+    const uint32_t nb = uint32_t(std::upper_bound(E.base.begin(), E.base.end(), bound) - E.base.begin());
+    uint32_t hi = std::min(E.win_hi, nb);
+    if (split_arg_ > 0) {          // --split: the first large prime's index among the base primes
+        if (split_arg_ < E.win_lo || split_arg_ > E.win_hi)
+            throw std::runtime_error("--split must lie in [" + std::to_string(E.win_lo) + ", " +
+                                     std::to_string(E.win_hi) + "] on this engine");
+        hi = std::min(split_arg_, nb);
+    }
+    nwin_ = hi > E.win_lo ? hi - E.win_lo : 0;
+    split_ = E.win_lo + nwin_;
+    nlarge_ = nb > E.win_hi ? nb - E.win_hi : 0;
+    if (split_ < E.win_hi && nb > split_) {
+        // a lowered split: primes [split, win_hi) would go to the large stage,
+        // which takes the device list from win_hi on; keep them in the windows
+        nwin_ = std::min(E.win_hi, nb) - E.win_lo;
+        split_ = E.win_lo + nwin_;
+    }
+    rs_.split_used = split_;
 }
 
 void DenseRun::configure_stats()
@@ -232,7 +300,7 @@ void DenseRun::configure_stats()
     gaps_exact_ = gaps_on_ && gaps_env != "sampled";
     sample_ = env::u32("GB_STATS_SAMPLE", 256, 1);
     if (stats_on_) {
-        gs_.emplace(cand_.primes, CHECK_TAIL);
+        gs_.emplace(E.cand.primes, CHECK_TAIL);
         gs_->set_run_start(start_);
         if (gaps_on_) gs_->begin_gaps(start_);
     } else if (start_ < STATS_MIN_START && !env::present("GB_STATS")) {
@@ -241,50 +309,16 @@ void DenseRun::configure_stats()
     // every phase-A candidate must lie below the tail bound: evens with
     // p(n) >= tail_t then all reach phase B, which records them exactly
     for (uint32_t cr = 0; cr < 15; ++cr)
-        if (ka_ > 0 && cand_.primes[cand_.slot_pi[cr][std::min<size_t>(ka_, cand_.slot_pi[cr].size()) - 1]] >= CHECK_TAIL)
+        if (E.cand.primes[E.cand.slot_pi[cr][std::min<size_t>(E.ka, E.cand.slot_pi[cr].size()) - 1]] >= CHECK_TAIL)
             throw std::runtime_error("GB_KA too large: a phase-A candidate reaches the tail bound");
-}
-
-void DenseRun::setup_device()
-{
-    const auto t0 = clock_::now();
-    ctx_.emplace();
-    c_ = ctx_->c;
-    rs_.init_module_s = since(t0);
-    const auto t1 = clock_::now();
-    const uint32_t nwin = win_hi_ - win_lo_, nlarge = uint32_t(base_.size()) - win_hi_;
-    d_win_ = Buf(c_, size_t(nwin) * 4, 0);
-    d_large_ = Buf(c_, size_t(nlarge) * 4, 0);
-    if (nwin) check(gbh_copy_h2d(c_, 0, d_win_.p, base_.data() + win_lo_, size_t(nwin) * 4), "upload window primes");
-    if (nlarge) check(gbh_copy_h2d(c_, 0, d_large_.p, base_.data() + win_hi_, size_t(nlarge) * 4), "upload large primes");
-    d_i0_ = Buf(c_, size_t(nwin) * 8 * 4, 0);
-    // the largest segment: the merged last one spans below 2 x span
-    const uint64_t qodds_max = (2 * span_ + MAX_GB_PRIME) / 2 + 2;
-    const uint64_t nc_max = (qodds_max + 14) / 15 + 1;
-    const uint64_t nwin_max = (nc_max + wbits_ - 1) / wbits_;
-    cw_cap_ = nwin_max * (wbits_ / 64) + 16;
-    d_map_ = Buf(c_, size_t(8) * cw_cap_ * 8, 0);
-    check(gbh_zero(c_, 0, d_map_.p, size_t(8) * cw_cap_ * 8), "zero map");
-    d_coff_ = Buf(c_, size_t(15) * GBH_NCAND * 4, 0);
-    const uint64_t evens_max = span_ + 2;
-    pending_cap_ = uint32_t(std::min<uint64_t>(evens_max / 4 + 4096, UINT32_MAX / 2));
-    d_pending_ = Buf(c_, size_t(pending_cap_) * 4, 0);
-    scratch_words_ = uint32_t(2 * (nc_max / 64 + 4));
-    d_scratch_ = Buf(c_, size_t(scratch_words_) * 4, 0);
-    for (int s = 0; s < 2; ++s) {
-        d_out_[s] = Buf(c_, sizeof(gbh_segout), 0);
-        h_out_[s] = Buf(c_, sizeof(gbh_segout), 1);
-        for (auto& e : ev_[s]) e.create(c_);
-    }
-    check(gbh_stream_sync(c_, 0), "upload sync");
-    rs_.init_upload_s = since(t1);
 }
 
 void DenseRun::enqueue(SegJob& j)
 {
+    gbh_ctx* c = E.c;
     j.cg = class_geometry(j.sg.q_first, j.sg.q_odds);
-    j.geom = make_geom(j.sg, j.cg, wbits_);
-    if (j.geom.cw > cw_cap_) throw std::runtime_error("segment class stride above the allocation");
+    j.geom = make_geom(j.sg, j.cg, E.wbits);
+    if (j.geom.cw > E.cw_cap) throw std::runtime_error("segment class stride above the allocation");
     // evens below q_first + 9780 (only at the very bottom of the range) have
     // candidate q outside the map: the CPU resolves them
     uint64_t n_gpu = std::max<uint64_t>(j.sg.seg_start, j.sg.q_first + 9780);
@@ -292,60 +326,62 @@ void DenseRun::enqueue(SegJob& j)
     j.n_gpu = n_gpu;
     j.gpu_evens = n_gpu <= j.sg.seg_end ? (j.sg.seg_end - n_gpu) / 2 + 1 : 0;
     const int s = j.slot;
-    gbh_segout* out = d_out_[s].as<gbh_segout>();
-    check(gbh_event_record(c_, 0, ev_[s][EV_BEGIN].e), "event");
-    check(gbh_zero(c_, 0, out, sizeof(gbh_segout)), "zero outputs");
-    check(gbh_fill32(c_, 0, out->first, GBH_NCAND, ~0u), "fill first");
-    check(gbh_fill32(c_, 0, &out->pfirst, 1, ~0u), "fill pfirst");
-    const uint32_t nwin = win_hi_ - win_lo_, nlarge = uint32_t(base_.size()) - win_hi_;
-    if (nwin) {
-        gbh_setup_args a{&j.geom, d_win_.as<uint32_t>(), nwin, d_i0_.as<uint32_t>()};
-        check(gbh_launch_setup(c_, 0, &a), "setup");
+    gbh_segout* out = E.d_out[s].as<gbh_segout>();
+    check(gbh_event_record(c, 0, E.ev[s][EV_BEGIN].e), "event");
+    check(gbh_zero(c, 0, out, sizeof(gbh_segout)), "zero outputs");
+    check(gbh_fill32(c, 0, out->first, GBH_NCAND, ~0u), "fill first");
+    check(gbh_fill32(c, 0, &out->pfirst, 1, ~0u), "fill pfirst");
+    if (nwin_) {
+        gbh_setup_args a{&j.geom, E.d_win.as<uint32_t>(), nwin_, E.d_i0.as<uint32_t>()};
+        check(gbh_launch_setup(c, 0, &a), "setup");
     }
-    gbh_window_args w{&j.geom, d_win_.as<uint32_t>(), nwin, ncoop_, d_i0_.as<uint32_t>(), d_map_.as<uint64_t>()};
-    check(gbh_launch_window(c_, 0, &w), "window sieve");
-    check(gbh_event_record(c_, 0, ev_[s][EV_WINDOW].e), "event");
-    if (nlarge) {
-        gbh_large_args l{&j.geom, d_large_.as<uint32_t>(), nlarge, 0, d_map_.as<uint64_t>()};
-        check(gbh_launch_large(c_, 0, &l), "large primes");
+    gbh_window_args w{&j.geom, E.d_win.as<uint32_t>(), nwin_, std::min(E.ncoop, nwin_), E.d_i0.as<uint32_t>(),
+                      E.d_map.as<uint64_t>()};
+    check(gbh_launch_window(c, 0, &w), "window sieve");
+    check(gbh_event_record(c, 0, E.ev[s][EV_WINDOW].e), "event");
+    if (nlarge_) {
+        gbh_large_args l{&j.geom, E.d_large.as<uint32_t>(), nlarge_, 0, E.d_map.as<uint64_t>()};
+        check(gbh_launch_large(c, 0, &l), "large primes");
     }
-    check(gbh_event_record(c_, 0, ev_[s][EV_LARGE].e), "event");
+    check(gbh_event_record(c, 0, E.ev[s][EV_LARGE].e), "event");
     if (j.gpu_evens) {
-        gbh_check_args ca{&j.geom, d_map_.as<uint64_t>(), n_gpu, ka_, j.sampled ? 1u : 0u, d_coff_.as<uint32_t>(),
-                          d_pending_.as<uint32_t>(), pending_cap_, 0, out};
-        check(gbh_launch_check_a(c_, 0, &ca), "check phase A");
-        gbh_checkb_args cb{&j.geom, d_map_.as<uint64_t>(), n_gpu, ka_, CHECK_TAIL, d_coff_.as<uint32_t>(),
-                           d_pending_.as<uint32_t>(), pending_cap_, 0, out};
-        check(gbh_launch_check_b(c_, 0, &cb), "check phase B");
+        gbh_check_args ca{&j.geom, E.d_map.as<uint64_t>(), n_gpu, E.ka, j.sampled ? 1u : 0u, E.d_coff.as<uint32_t>(),
+                          E.d_pending.as<uint32_t>(), E.pending_cap, 0, out};
+        check(gbh_launch_check_a(c, 0, &ca), "check phase A");
+        gbh_checkb_args cb{&j.geom, E.d_map.as<uint64_t>(), n_gpu, E.ka, CHECK_TAIL, E.d_coff.as<uint32_t>(),
+                           E.d_pending.as<uint32_t>(), E.pending_cap, 0, out};
+        check(gbh_launch_check_b(c, 0, &cb), "check phase B");
     }
-    check(gbh_event_record(c_, 0, ev_[s][EV_CHECK].e), "event");
+    check(gbh_event_record(c, 0, E.ev[s][EV_CHECK].e), "event");
     if (j.counted) {
         gbh_count_args ct = make_count_args(j.sg, j.cg);
         ct.geom = &j.geom;
-        ct.map = d_map_.as<uint64_t>();
-        ct.scratch = d_scratch_.as<uint32_t>();
-        ct.scratch_words = scratch_words_;
+        ct.map = E.d_map.as<uint64_t>();
+        ct.scratch = E.d_scratch.as<uint32_t>();
+        ct.scratch_words = E.scratch_words;
         ct.out = out;
-        check(gbh_launch_count(c_, 0, &ct), "count");
+        check(gbh_launch_count(c, 0, &ct), "count");
     }
-    check(gbh_event_record(c_, 0, ev_[s][EV_COUNT].e), "event");
-    check(gbh_copy_d2h(c_, 0, h_out_[s].p, out, sizeof(gbh_segout)), "copy outputs");
-    check(gbh_event_record(c_, 0, ev_[s][EV_END].e), "event");
+    check(gbh_event_record(c, 0, E.ev[s][EV_COUNT].e), "event");
+    check(gbh_copy_d2h(c, 0, E.h_out[s].p, out, sizeof(gbh_segout)), "copy outputs");
+    check(gbh_event_record(c, 0, E.ev[s][EV_END].e), "event");
 }
 
 void DenseRun::collect(SegJob& j)
 {
+    gbh_ctx* c = E.c;
     const int s = j.slot;
-    check(gbh_event_sync(c_, ev_[s][EV_END].e), "segment sync");
-    const gbh_segout& o = *h_out_[s].as<gbh_segout>();
-    const double ts = elapsed_s(c_, ev_[s][EV_BEGIN], ev_[s][EV_WINDOW]);
-    const double tl = elapsed_s(c_, ev_[s][EV_WINDOW], ev_[s][EV_LARGE]);
-    const double tc = elapsed_s(c_, ev_[s][EV_LARGE], ev_[s][EV_CHECK]);
-    const double tn = elapsed_s(c_, ev_[s][EV_CHECK], ev_[s][EV_COUNT]);
+    check(gbh_event_sync(c, E.ev[s][EV_END].e), "segment sync");
+    const gbh_segout& o = *E.h_out[s].as<gbh_segout>();
+    const double ts = elapsed_s(c, E.ev[s][EV_BEGIN], E.ev[s][EV_WINDOW]);
+    const double tl = elapsed_s(c, E.ev[s][EV_WINDOW], E.ev[s][EV_LARGE]);
+    const double tc = elapsed_s(c, E.ev[s][EV_LARGE], E.ev[s][EV_CHECK]);
+    const double tn = elapsed_s(c, E.ev[s][EV_CHECK], E.ev[s][EV_COUNT]);
     t_sieve_ += ts; t_large_ += tl; t_check_ += tc; t_count_ += tn;
     const std::string seg = " in segment [" + std::to_string(j.sg.seg_start) + ", " + std::to_string(j.sg.seg_end) + "]";
     if (o.err) throw std::runtime_error("device error flags " + std::to_string(o.err) + seg);
-    if (o.pending_n > pending_cap_) throw std::runtime_error("pending overflow (" + std::to_string(o.pending_n) + ")" + seg);
+    if (o.pending_n > E.pending_cap)
+        throw std::runtime_error("pending overflow (" + std::to_string(o.pending_n) + ")" + seg);
     // coverage: every even accounted for, by phase A, phase B or a leftover
     if (uint64_t(o.acct[0]) + o.pending_n != j.gpu_evens)
         throw std::runtime_error("coverage: phase A resolved " + std::to_string(o.acct[0]) + " + pending " +
@@ -357,15 +393,14 @@ void DenseRun::collect(SegJob& j)
     // the CPU fallback: leftovers, and the evens below the GPU's first checkable even
     std::vector<uint64_t> cpu_n;
     for (uint64_t n = j.sg.seg_start; n < j.n_gpu && n <= j.sg.seg_end; n += 2) cpu_n.push_back(n);
-    std::vector<uint32_t> left(o.left, o.left + o.left_n);
-    std::sort(left.begin(), left.end());
-    for (uint32_t i : left) {
+    for (uint32_t k = 0; k < o.left_n; ++k) {
+        const uint32_t i = o.left[k];
         if (i >= j.sg.seg_evens) throw std::runtime_error("leftover index " + std::to_string(i) + " outside the segment");
         cpu_n.push_back(j.sg.seg_start + 2ull * i);
     }
     std::sort(cpu_n.begin(), cpu_n.end());
     for (uint64_t n : cpu_n) {
-        const uint64_t p = cpu_min_goldbach_prime(n, cand_.primes.data(), cand_.primes.size(), MAX_GB_PRIME);
+        const uint64_t p = cpu_min_goldbach_prime(n, E.cand.primes.data(), E.cand.primes.size(), MAX_GB_PRIME);
         if (!p) throw std::runtime_error("Goldbach verification FAILED for n=" + std::to_string(n));
         if (p > MAX_GB_PRIME)
             std::fprintf(stderr, "note: n=%llu needs p=%llu > MAX_GB_PRIME (new p_min record candidate)\n",
@@ -374,11 +409,11 @@ void DenseRun::collect(SegJob& j)
     }
     rs_.leftovers += o.left_n;
     if (gs_) {
-        const size_t np = cand_.primes.size();
+        const size_t np = E.cand.primes.size();
         std::vector<uint64_t> h(15 * np, 0);
         for (uint32_t cr = 0; cr < 15; ++cr)
-            for (size_t k = 0; k < cand_.slot_pi[cr].size(); ++k)
-                h[cr * np + cand_.slot_pi[cr][k]] += o.hist[cr * GBH_NCAND + k];
+            for (size_t k = 0; k < E.cand.slot_pi[cr].size(); ++k)
+                h[cr * np + E.cand.slot_pi[cr][k]] += o.hist[cr * GBH_NCAND + k];
         gs_->add_hist(j.sampled, h);
         gs_->add_first(j.sg.seg_start, o.first);
         if (gaps_on_) {
@@ -386,7 +421,7 @@ void DenseRun::collect(SegJob& j)
             SegmentGaps g;
             g.q_first = j.sg.q_first;
             g.evens = j.sg.seg_evens;
-            for (int c = 0; c < 8; ++c) g.primes += o.cnt[c];
+            for (int k = 0; k < 8; ++k) g.primes += o.cnt[k];
             g.pfirst = o.pfirst;
             g.plast = o.plast;
             g.tail = o.gtail;
@@ -415,9 +450,9 @@ void DenseRun::collect(SegJob& j)
         st.seg_end = j.sg.seg_end;
         st.evens = j.sg.seg_evens;
         st.tiles = uint64_t(j.geom.nwin) * 8;
-        st.large_items = base_.size() - win_hi_;
-        st.sieve_split = win_hi_;
-        st.mask_count = win_lo_;
+        st.large_items = nlarge_;
+        st.sieve_split = split_;
+        st.mask_count = E.win_lo;
         st.leftovers = o.left_n;
         st.sieve_seconds = ts;
         st.large_seconds = tl;
@@ -431,11 +466,8 @@ RunStats DenseRun::run()
 {
     t_start_ = clock_::now();
     resolve_range();
-    load_base_primes();
     configure_stats();
     phase_trace_ = env::present("GB_PHASE");
-    setup_device();
-    rs_.init_seconds = since(t_start_);
     const auto t_gpu = clock_::now();
     SegJob jobs[2];
     SegJob* prev = nullptr;
@@ -452,7 +484,7 @@ RunStats DenseRun::run()
         j.t_wall = clock_::now();
         enqueue(j);
         rs_.total_tiles += uint64_t(j.geom.nwin) * 8;
-        rs_.total_large_items += base_.size() - win_hi_;
+        rs_.total_large_items += nlarge_;
         if (prev) collect(*prev);
         prev = &j;
         if (j.sg.seg_end == limit_) { ++k; break; }
@@ -460,6 +492,7 @@ RunStats DenseRun::run()
     if (prev) collect(*prev);
     rs_.segments = k;
     rs_.gpu_seconds = since(t_gpu);
+    rs_.init_seconds = since(t_start_) - rs_.gpu_seconds;
     rs_.total_seconds = rs_.init_seconds + rs_.gpu_seconds;
     rs_.evens_per_second = double(total_evens_) / rs_.gpu_seconds;
     rs_.sieve_seconds = t_sieve_;
@@ -468,7 +501,6 @@ RunStats DenseRun::run()
     rs_.count_seconds = t_count_;
     rs_.sync_seconds = std::max(0.0, rs_.gpu_seconds - t_sieve_ - t_large_ - t_check_ - t_count_);
     rs_.start = start_;
-    rs_.segment_used = 0;
     if (gs_) {
         gs_->finish_run(start_, limit_, total_evens_);
         rs_.stats_on = true;
@@ -486,13 +518,43 @@ RunStats DenseRun::run()
 
 }  // namespace
 
+Engine::Engine(uint64_t max_limit, uint64_t max_span) : s_(std::make_unique<State>(max_limit, max_span)) {}
+Engine::~Engine() = default;
+double Engine::init_seconds() const { return s_->init_s; }
+uint64_t Engine::max_limit() const { return s_->max_limit; }
+void Engine::init_times(double& cache, double& module, double& upload) const
+{
+    cache = s_->cache_s;
+    module = s_->module_s;
+    upload = s_->upload_s;
+}
+
+RunStats Engine::run(uint64_t limit, uint64_t segment_even, bool verbose, uint64_t start_even, uint32_t sieve_split_arg,
+                     bool collect_segments)
+{
+    DenseRun r(*s_, limit, segment_even, verbose, start_even, sieve_split_arg, collect_segments);
+    return r.run();
+}
+
 RunStats run_gpu(uint64_t limit, uint64_t segment_even, const std::string& spv_path, bool verbose, uint64_t start_even,
                  uint32_t sieve_split_arg, const char* prime_cache_path, bool collect_segments)
 {
     (void)spv_path;
     (void)prime_cache_path;
-    DenseRun r(limit, segment_even, verbose, start_even, sieve_split_arg, collect_segments);
-    return r.run();
+    refuse_removed_knobs();
+    if (limit < 4) throw std::runtime_error("limit < 4: no even n >= 4 to verify");
+    require_supported_limit(limit);
+    // the engine is sized for this run: the span the range resolution keeps
+    const uint64_t lim = limit & ~uint64_t(1);
+    const uint64_t st = std::max<uint64_t>(start_even + (start_even & 1), 4);
+    uint64_t span = std::max<uint64_t>(segment_even + (segment_even & 1), 2);
+    if (st <= lim) span = std::min<uint64_t>(span, std::max<uint64_t>((((lim - st) / 2 + 1) & ~uint64_t(1)), 4));
+    Engine e(lim, std::max<uint64_t>(span, 4));
+    RunStats s = e.run(limit, segment_even, verbose, start_even, sieve_split_arg, collect_segments);
+    s.init_seconds += e.init_seconds();
+    s.total_seconds = s.init_seconds + s.gpu_seconds;
+    e.init_times(s.init_cache_s, s.init_module_s, s.init_upload_s);
+    return s;
 }
 
 void print_device_info()

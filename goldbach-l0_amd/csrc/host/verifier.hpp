@@ -4,6 +4,7 @@
 // print_device_info, verify_cpu), same names, arguments and error behaviour,
 // running the MI355X pipeline through the gbh C ABI (include/gbh.h).
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -51,6 +52,32 @@ struct RunStats {
 RunStats run_gpu(uint64_t limit, uint64_t segment_even, const std::string& spv_path, bool verbose,
                  uint64_t start_even = 4, uint32_t sieve_split_arg = 0, const char* prime_cache_path = nullptr,
                  bool collect_segments = false);
+
+/// A verifier with its device state resident between runs: the base primes
+/// up to sqrt(max_limit), the prime tables on the GPU and the segment
+/// buffers for spans up to max_span. run() then verifies any range below
+/// max_limit without repeating that set-up (serving and batched callers;
+/// run_gpu is one Engine, one run).
+class Engine {
+public:
+    /// @throws std::runtime_error when no gfx950 device is usable.
+    Engine(uint64_t max_limit, uint64_t max_span = 467'500'000ULL);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+    /// Same contract as run_gpu; limit <= max_limit and the resolved
+    /// segment span <= max_span.
+    RunStats run(uint64_t limit, uint64_t segment_even, bool verbose = false, uint64_t start_even = 4,
+                 uint32_t sieve_split_arg = 0, bool collect_segments = false);
+    double init_seconds() const;
+    /// init breakdown: base primes, device context, uploads and buffers
+    void init_times(double& cache, double& module, double& upload) const;
+    uint64_t max_limit() const;
+    struct State;
+
+private:
+    std::unique_ptr<State> s_;
+};
 
 void print_device_info();
 

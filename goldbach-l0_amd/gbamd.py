@@ -58,6 +58,15 @@ def load() -> ctypes.CDLL:
                                         ctypes.c_char_p, ctypes.c_size_t]
         lib.gba_segment_map.restype = ctypes.c_int
         lib.gba_version.restype = ctypes.c_char_p
+        lib.gba_engine_open.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p),
+                                        ctypes.POINTER(ctypes.c_double), ctypes.c_char_p, ctypes.c_size_t]
+        lib.gba_engine_open.restype = ctypes.c_int
+        lib.gba_engine_run.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                                       ctypes.POINTER(GbaRunResult), ctypes.c_char_p, ctypes.c_size_t,
+                                       ctypes.c_char_p, ctypes.c_size_t]
+        lib.gba_engine_run.restype = ctypes.c_int
+        lib.gba_engine_close.argtypes = [ctypes.c_void_p]
+        lib.gba_engine_close.restype = None
         _lib = lib
     return _lib
 
@@ -84,6 +93,50 @@ def run_gpu(limit: int, segment_even: int = 467_500_000, verbose: bool = False, 
     out = {name: getattr(res, name) for name, _ in GbaRunResult._fields_}
     out["stats_line"] = line.value.decode()
     return out
+
+
+class Engine:
+    """Device state resident between runs (base primes up to sqrt(max_limit),
+    prime tables in HBM, buffers for spans up to max_span); run() has the
+    contract of run_gpu for limit <= max_limit."""
+
+    def __init__(self, max_limit: int, max_span: int = 467_500_000):
+        self._lib = load()
+        self._h = ctypes.c_void_p()
+        init = ctypes.c_double(0)
+        err = ctypes.create_string_buffer(4096)
+        if self._lib.gba_engine_open(int(max_limit), int(max_span), ctypes.byref(self._h), ctypes.byref(init), err,
+                                     len(err)):
+            raise VerificationError(err.value.decode())
+        self.init_seconds = init.value
+
+    def run(self, limit: int, segment_even: int = 467_500_000, start_even: int = 4) -> dict:
+        res = GbaRunResult()
+        line = ctypes.create_string_buffer(1 << 20)
+        err = ctypes.create_string_buffer(4096)
+        if self._lib.gba_engine_run(self._h, int(limit), int(segment_even), int(start_even), ctypes.byref(res), line,
+                                    len(line), err, len(err)):
+            raise VerificationError(err.value.decode())
+        out = {name: getattr(res, name) for name, _ in GbaRunResult._fields_}
+        out["stats_line"] = line.value.decode()
+        return out
+
+    def close(self):
+        if self._h:
+            self._lib.gba_engine_close(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def segment_map(seg_start: int, limit: int, span: int):

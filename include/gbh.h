@@ -40,7 +40,7 @@ enum {
 /* Layout constants shared by host and kernels. */
 #define GBH_CLASSES 8            /* residues 1 7 11 13 17 19 23 29 mod 30 */
 #define GBH_EVEN_CLASSES 15      /* n mod 30 = 0, 2, ..., 28 */
-#define GBH_NCAND 1206           /* odd primes 3..9781 (MAX_GB_PRIME, config.hpp:86) */
+#define GBH_NCAND 1205           /* odd primes 3..9781 (MAX_GB_PRIME, config.hpp:86) */
 #define GBH_NSMALL 15            /* mask-stage primes 7..61 (MASK_MAX_P = 64, config.hpp:141) */
 #define GBH_GAP_BINS 1024        /* gap histogram bins g / 2 (GAPS30_HBINS, gstats.hpp:75) */
 #define GBH_GAP_TAIL 700         /* tail gaps (GAPS30_TAIL, gstats.hpp:74) */

@@ -47,6 +47,20 @@ int gba_run_gpu(uint64_t limit, uint64_t segment_even, int verbose, uint64_t sta
 int gba_segment_map(uint64_t seg_start, uint64_t limit, uint64_t span, uint64_t* words, size_t cap,
                     uint32_t* cw_out, uint64_t* Q, uint32_t* nc, char* err, size_t err_cap);
 
+/* Engine: the device state kept resident between runs (base primes up to
+ * sqrt(max_limit), the prime tables in HBM, segment buffers for spans up to
+ * max_span). No counterpart in the reference, which rebuilds it per process
+ * (its init phases, verifier.cpp:2615-2642); for callers that verify many
+ * ranges (serving, batches) without repeating the set-up. gba_engine_run has
+ * the contract of gba_run_gpu with limit <= max_limit; one thread per
+ * engine. */
+typedef struct gba_engine gba_engine;
+int gba_engine_open(uint64_t max_limit, uint64_t max_span, gba_engine** out, double* init_seconds, char* err,
+                    size_t err_cap);
+int gba_engine_run(gba_engine* e, uint64_t limit, uint64_t segment_even, uint64_t start_even, gba_run_result* out,
+                   char* stats_line, size_t stats_cap, char* err, size_t err_cap);
+void gba_engine_close(gba_engine* e);
+
 /* The library's version string. */
 const char* gba_version(void);
 
