@@ -124,6 +124,9 @@ int gbh_open(int device, gbh_ctx** out)
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&gbh_dev::k_window),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) (void)hipGetLastError();   // optional: windows up to the default limit still work
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&gbh_dev::k_harvest),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024) != hipSuccess)
+        (void)hipGetLastError();
     *out = c;
     return GBH_OK;
 }
@@ -230,97 +233,98 @@ int gbh_event_elapsed_ms(gbh_ctx*, gbh_event* a, gbh_event* b, float* ms)
     return GBH_OK;
 }
 
+namespace {
+
+int check_batch(const gbh_batch& b)
+{
+    if (!b.nseg || b.nseg > GBH_MAX_BATCH) return fail(GBH_E_ARG, "batch size outside [1, GBH_MAX_BATCH]");
+    if (b.wbits < (1u << 12) || b.wbits > (1u << 20) || (b.wbits & (b.wbits - 1)))
+        return fail(GBH_E_ARG, "window bits must be a power of two in [2^12, 2^20]");
+    if (uint64_t(b.nwin) * (b.wbits / 64) + 16 > b.cw) return fail(GBH_E_ARG, "windows exceed the class stride");
+    if (!b.geoms || !b.map) return fail(GBH_E_ARG, "batch without geometry or map");
+    return GBH_OK;
+}
+
+}  // namespace
+
+int gbh_reset_outputs(gbh_ctx* c, int s, gbh_segout* out, uint32_t nseg)
+{
+    if (!nseg) return GBH_OK;
+    hipLaunchKernelGGL(gbh_dev::k_reset_out, dim3(64, nseg), dim3(256), 0, S(c, s), out);
+    GBH_HIP(hipGetLastError());
+    return GBH_OK;
+}
+
 int gbh_launch_setup(gbh_ctx* c, int s, const gbh_setup_args* a)
 {
+    if (const int rc = check_batch(a->b)) return rc;
     if (!a->nprimes) return GBH_OK;
-    hipLaunchKernelGGL(gbh_dev::k_setup, dim3(grid_of(a->nprimes, 256, 1u << 30)), dim3(256), 0, S(c, s),
-                       *a->geom, a->primes, a->nprimes, a->i0);
+    hipLaunchKernelGGL(gbh_dev::k_setup, dim3(grid_of(a->nprimes, 256, 1u << 30), a->b.nseg), dim3(256), 0, S(c, s),
+                       a->b, a->primes, a->nprimes, a->i0);
+    GBH_HIP(hipGetLastError());
+    return GBH_OK;
+}
+
+int gbh_launch_harvest(gbh_ctx* c, int s, const gbh_harvest_args* a)
+{
+    if (const int rc = check_batch(a->b)) return rc;
+    if (!a->nchunks) return GBH_OK;
+    const uint64_t nb = uint64_t(a->b.nseg) * 8 * a->b.nwin;
+    if (nb > gbh_dev::kMaxBuckets) return fail(GBH_E_ARG, "harvest: more buckets than the LDS counters hold");
+    hipLaunchKernelGGL(gbh_dev::k_harvest, dim3(a->nchunks), dim3(256), size_t(2 * nb * 4), S(c, s), a->b, a->primes,
+                       a->chunks, a->cap, a->hits, a->hit_n, a->err);
     GBH_HIP(hipGetLastError());
     return GBH_OK;
 }
 
 int gbh_launch_window(gbh_ctx* c, int s, const gbh_window_args* a)
 {
-    const gbh_geom& g = *a->geom;
-    if (g.wbits < (1u << 12) || g.wbits > (1u << 20) || (g.wbits & (g.wbits - 1)))
-        return fail(GBH_E_ARG, "window bits must be a power of two in [2^12, 2^20]");
-    if (uint64_t(g.nwin) * (g.wbits / 64) > g.cw) return fail(GBH_E_ARG, "windows exceed the class stride");
-    for (int k = 0; k < GBH_CLASSES; ++k)
-        if (g.nc[k] > uint64_t(g.nwin) * g.wbits) return fail(GBH_E_ARG, "class array longer than its windows");
+    if (const int rc = check_batch(a->b)) return rc;
     if (a->ncoop > a->nprimes) return fail(GBH_E_ARG, "ncoop > nprimes");
-    const size_t lds = g.wbits / 8;
-    hipLaunchKernelGGL(gbh_dev::k_window, dim3(g.nwin, GBH_CLASSES), dim3(256), lds, S(c, s), g, a->primes,
-                       a->nprimes, a->ncoop, a->i0, a->map);
+    const size_t lds = a->b.wbits / 8;
+    hipLaunchKernelGGL(gbh_dev::k_window, dim3(a->b.nwin, GBH_CLASSES, a->b.nseg), dim3(256), lds, S(c, s), a->b,
+                       a->primes, a->nprimes, a->ncoop, a->i0, a->hits, a->hit_n, a->cap);
     GBH_HIP(hipGetLastError());
     return GBH_OK;
 }
-
-int gbh_launch_large(gbh_ctx* c, int s, const gbh_large_args* a)
-{
-    if (!a->nprimes) return GBH_OK;
-    hipLaunchKernelGGL(gbh_dev::k_large, dim3(grid_of(a->nprimes, 256, 65536)), dim3(256), 0, S(c, s), *a->geom,
-                       a->primes, a->nprimes, reinterpret_cast<unsigned long long*>(a->map));
-    GBH_HIP(hipGetLastError());
-    return GBH_OK;
-}
-
-namespace {
-uint32_t max_class_evens(const gbh_geom& g, uint64_t n_gpu)
-{
-    uint32_t m = 0;
-    for (uint32_t cr = 0; cr < GBH_EVEN_CLASSES; ++cr) {
-        const uint64_t n0 = n_gpu + (2 * cr + 30 - n_gpu % 30) % 30;
-        const uint64_t cnt = n0 <= g.seg_end ? (g.seg_end - n0) / 30 + 1 : 0;
-        if (cnt > m) m = uint32_t(cnt);
-    }
-    return m;
-}
-}  // namespace
 
 int gbh_launch_check_a(gbh_ctx* c, int s, const gbh_check_args* a)
 {
-    const gbh_geom& g = *a->geom;
-    if (a->ka > gbh_dev::kKaMax) return fail(GBH_E_ARG, "ka above 128");
-    if (a->n_gpu < g.q_first + 9780 || a->n_gpu < g.seg_start || (a->n_gpu & 1))
-        return fail(GBH_E_ARG, "n_gpu below the first checkable even");
-    hipLaunchKernelGGL(gbh_dev::k_coff, dim3((GBH_NCAND + 255) / 256, GBH_EVEN_CLASSES), dim3(256), 0, S(c, s), g,
-                       a->n_gpu, a->coff);
+    if (const int rc = check_batch(a->b)) return rc;
+    if (a->ka > GBH_KA_MAX) return fail(GBH_E_ARG, "ka above GBH_KA_MAX");
+    // every class array holds nc <= nwin * wbits bits; the checker's words
+    // stay below that (host contract: n_gpu >= q_first + 9780)
+    const uint32_t words = uint32_t((uint64_t(a->b.nwin) * a->b.wbits + 63) / 64);
+    hipLaunchKernelGGL(gbh_dev::k_coff, dim3((GBH_NCAND + 255) / 256, GBH_EVEN_CLASSES, a->b.nseg), dim3(256), 0,
+                       S(c, s), a->b, a->coff);
     GBH_HIP(hipGetLastError());
-    if (a->n_gpu > g.seg_end) return GBH_OK;
-    const uint32_t words = (max_class_evens(g, a->n_gpu) + 63) / 64;
-    // every tested bit (k + offset < class bits) lies inside the class stride;
-    // the staging loads past it read zeros
-    if (uint64_t(words) > g.cw) return fail(GBH_E_ARG, "class stride too small for the checker");
-    hipLaunchKernelGGL(gbh_dev::k_check_a, dim3(grid_of(words, GBH_CHECK_WG, 1u << 30)), dim3(GBH_CHECK_WG), 0,
-                       S(c, s), g, a->map, a->coff, a->n_gpu, a->ka, a->hist_on, a->pending, a->pending_cap, a->out);
+    hipLaunchKernelGGL(gbh_dev::k_check_a, dim3(grid_of(words, GBH_CHECK_WG, 1u << 30), 3, a->b.nseg),
+                       dim3(GBH_CHECK_WG), 0, S(c, s), a->b, a->coff, a->ka, a->pending, a->pcap, a->out);
     GBH_HIP(hipGetLastError());
     return GBH_OK;
 }
 
 int gbh_launch_check_b(gbh_ctx* c, int s, const gbh_checkb_args* a)
 {
-    const gbh_geom& g = *a->geom;
-    if (a->n_gpu > g.seg_end) return GBH_OK;
-    hipLaunchKernelGGL(gbh_dev::k_check_b, dim3(grid_of(a->pending_max, 256, 2048)), dim3(256), 0, S(c, s), g,
-                       a->map, a->coff, a->n_gpu, a->ka, a->tail_t, a->pending, a->pending_max, a->out);
+    if (const int rc = check_batch(a->b)) return rc;
+    hipLaunchKernelGGL(gbh_dev::k_check_b, dim3(grid_of(a->pcap, 256, 512), a->b.nseg), dim3(256), 0, S(c, s), a->b,
+                       a->coff, a->ka, a->tail_t, a->pending, a->pcap, a->out);
     GBH_HIP(hipGetLastError());
     return GBH_OK;
 }
 
 int gbh_launch_count(gbh_ctx* c, int s, const gbh_count_args* a)
 {
-    const gbh_geom& g = *a->geom;
-    if (a->scratch_words < 2ull * a->nwords) return fail(GBH_E_ARG, "count scratch too small");
-    if (uint64_t(a->nwords) + 1 > g.cw) return fail(GBH_E_ARG, "count words exceed the class stride");
+    if (const int rc = check_batch(a->b)) return rc;
+    if (a->swords < 2ull * a->nwords) return fail(GBH_E_ARG, "count scratch too small");
+    if (uint64_t(a->nwords) + 1 > a->b.cw) return fail(GBH_E_ARG, "count words exceed the class stride");
     if (!a->nwords) return GBH_OK;
-    gbh_dev::CountArgs k{};
-    for (int i = 0; i < 8; ++i) { k.cls[i] = a->cls[i]; k.pos0[i] = a->pos0[i]; k.lo[i] = a->lo[i]; k.hi[i] = a->hi[i]; }
-    for (int j = 0; j < 3; ++j) { k.tk[j] = a->tk[j]; k.tk2[j] = a->tk2[j]; k.td[j] = a->td[j]; k.tlo[j] = a->tlo[j]; k.thi[j] = a->thi[j]; }
-    k.nwords = a->nwords;
     const unsigned grid = grid_of(a->nwords, 256, 1u << 30);
-    hipLaunchKernelGGL(gbh_dev::k_count_a, dim3(grid), dim3(256), 0, S(c, s), g, a->map, k, a->scratch, a->out);
+    hipLaunchKernelGGL(gbh_dev::k_count_a, dim3(grid, a->b.nseg), dim3(256), 0, S(c, s), a->b, a->cs, a->scratch,
+                       a->swords, a->out);
     GBH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(gbh_dev::k_count_b, dim3(grid), dim3(256), 0, S(c, s), k, a->scratch, a->out);
+    hipLaunchKernelGGL(gbh_dev::k_count_b, dim3(grid, a->b.nseg), dim3(256), 0, S(c, s), a->cs, a->scratch, a->swords,
+                       a->out);
     GBH_HIP(hipGetLastError());
     return GBH_OK;
 }

@@ -2,27 +2,29 @@
 // gfx950 (CDNA4: wave64, 160 KB LDS per CU). All arithmetic is integer.
 //
 // Map layout (same contract as the reference's class map, goldbach.cl:234-252
-// and host_math.hpp:88-107): eight class arrays, one per residue
+// and host_math.hpp:88-107): eight class arrays per segment, one per residue
 // r in {1,7,11,13,17,19,23,29} mod 30; bit t of class array c is 1 iff
 // q = Q[c] + 30 t is prime (for q in [q_first, q_last]). Words are 64-bit,
 // little-endian, so the same memory read as 32-bit words gives bit t at
-// word t >> 5, bit t & 31 (the LDS atomics use that view).
+// word t >> 5, bit t & 31 (the LDS atomics use that view). A batch of
+// consecutive segments keeps its maps side by side: map + (s * 8 + c) * cw.
 //
-// Stages (one launch each, in stream order):
-//   k_setup        first class bit of every window prime and class
-//                  (reference mid_setup, goldbach.cl:3528-3563)
-//   k_window       one work-group per (window, class): masks for 7..61, LDS
-//                  marks of the window primes, one coalesced store
-//                  (reference make_prime_map + mid_window, 1092-1608, 3600-3701)
-//   k_large        large primes: first multiple per class, global 64-bit
-//                  atomic AND-clears (reference sieve_large_packed / hit lists)
-//   k_coff         per-segment candidate table of the checker
-//   k_check_a      exact ascending candidates, 64 evens per lane, first ka
-//                  candidates (reference check30_a, 2028-2189)
-//   k_check_b      the rest for every pending even (reference check30_b,
-//                  2210-2306)
-//   k_count_a/b    primes, twins and the gap walk of the counting range
-//                  (reference count30 + gap_tail_exact, 2499-2825)
+// Stages (one launch each per batch, in stream order):
+//   k_setup     first class bit of every window prime, per segment and class
+//               (reference mid_setup, goldbach.cl:3528-3563)
+//   k_harvest   hit lists of the large primes over the whole batch, filed by
+//               (segment, class, window) (reference hl_harvest, 3065-3144)
+//   k_window    one work-group per (window, class, segment): masks for
+//               7..61, LDS marks of the window primes, the window's hit list,
+//               one coalesced store (reference make_prime_map + mid_window +
+//               hl_apply, 1092-1608, 3600-3701, 3434-3457)
+//   k_coff      per-segment candidate table of the checker
+//   k_check_a   exact ascending candidates, 64 evens per lane, first ka
+//               candidates (reference check30_a, 2028-2189)
+//   k_check_b   the rest for every pending even (reference check30_b,
+//               2210-2306)
+//   k_count_a/b primes, twins and the gap walk of the counting range
+//               (reference count30 + gap_tail_exact, 2499-2825)
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -58,9 +60,13 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
     return v;
 }
 
-// First class bit of prime p in class c for q in [max(p^2, q_first), q_last]:
-// the smallest multiplier m >= max(ceil(q_first / p), p) with m p = res(c)
-// (mod 30); returns false when that multiple exceeds q_last.
+__device__ __forceinline__ uint64_t* seg_map(const gbh_batch& b, uint32_t s)
+{
+    return b.map + (size_t)s * 8 * b.cw;
+}
+
+// The smallest multiplier m >= max(ceil(q_first / p), p): multiples are
+// marked from max(p^2, q_first), so a prime p inside the range stays prime.
 __device__ __forceinline__ uint64_t first_mult(uint64_t q_first, uint32_t p)
 {
     uint64_t m0 = (q_first + p - 1) / p;
@@ -69,15 +75,18 @@ __device__ __forceinline__ uint64_t first_mult(uint64_t q_first, uint32_t p)
 }
 
 // ---------------------------------------------------------------- stage 1
-__global__ void __launch_bounds__(256) k_setup(gbh_geom g, const uint32_t* __restrict__ primes, uint32_t np,
+__global__ void __launch_bounds__(256) k_setup(gbh_batch b, const uint32_t* __restrict__ primes, uint32_t np,
                                                uint32_t* __restrict__ i0)
 {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t s = blockIdx.y;
     if (j >= np) return;
+    const gbh_geom& g = b.geoms[s];
     const uint32_t p = primes[j];
     const uint64_t m0 = first_mult(g.q_first, p);
     const uint32_t m0r = (uint32_t)(m0 % 30);
     const uint32_t pinv = kInv30[p % 30];
+    uint32_t* out = i0 + (size_t)s * 8 * np;
 #pragma unroll
     for (int c = 0; c < GBH_CLASSES; ++c) {
         const uint32_t mt = (kRes[c] * pinv) % 30;
@@ -88,11 +97,133 @@ __global__ void __launch_bounds__(256) k_setup(gbh_geom g, const uint32_t* __res
             const uint64_t tt = (q - g.Q[c]) / 30;
             if (tt < g.nc[c]) t = (uint32_t)tt;
         }
-        i0[(size_t)c * np + j] = t;
+        out[(size_t)c * np + j] = t;
     }
 }
 
 // ---------------------------------------------------------------- stage 2
+constexpr uint32_t kMaxBuckets = 8192;
+
+struct HarvestCtx {
+    uint64_t qb0, qb1;       // the batch's q range
+    double inv_span;         // 1 / (q_first[s + 1] - q_first[s])
+    uint32_t nseg, nwin, lw, wmask;
+};
+
+// Files hit q of class c into the bucket of every segment whose q range
+// holds it (consecutive segments overlap by 9776 in q).
+template <bool kWrite>
+__device__ __forceinline__ void file_hit(const HarvestCtx& h, const uint64_t* __restrict__ qf,
+                                         const uint64_t* __restrict__ ql, const uint64_t* __restrict__ Q, uint64_t q,
+                                         uint32_t c, uint32_t* cnt, const uint32_t* base, uint32_t* __restrict__ hits,
+                                         uint32_t cap, uint32_t* __restrict__ err)
+{
+    int s = (int)((double)(q - h.qb0) * h.inv_span);
+    if (s >= (int)h.nseg) s = h.nseg - 1;
+    while (s > 0 && q < qf[s]) --s;
+    while (s + 1 < (int)h.nseg && q >= qf[s + 1]) ++s;
+    for (int k = s; k >= 0 && k >= s - 1; --k) {
+        if (q < qf[k] || q > ql[k]) continue;
+        const uint64_t t = (q - Q[k * 8 + c]) / 30;
+        const uint32_t bucket = ((uint32_t)k * 8 + c) * h.nwin + (uint32_t)(t >> h.lw);
+        if (!kWrite) {
+            atomicAdd(&cnt[bucket], 1u);
+        } else {
+            const uint32_t pos = base[bucket] + atomicAdd(&cnt[bucket], 1u);
+            if (pos < cap) hits[(size_t)bucket * cap + pos] = (uint32_t)t & h.wmask;
+            else atomicOr(err, 1u);
+        }
+    }
+}
+
+// ceil(x / p) for p >= 2^12: a double-precision quotient (off by at most
+// one here: x rounds to within 2^11 and the product to 2^-52 relative) made
+// exact by the integer remainder.
+__device__ __forceinline__ uint64_t ceil_div_big(uint64_t x, double xd, uint32_t p)
+{
+    const double qd = xd * (1.0 / (double)p);
+    const uint64_t hi = (uint64_t)(uint32_t)(qd * (1.0 / 4294967296.0));
+    uint64_t q = (hi << 32) + (uint64_t)(uint32_t)(qd - (double)hi * 4294967296.0);
+    int64_t r = (int64_t)(x - q * p);
+    while (r < 0) { --q; r += p; }
+    while (r >= (int64_t)p) { ++q; r -= p; }
+    return r ? q + 1 : q;
+}
+
+// Every multiple q = p m in [qb0, qb1] with m >= p and m coprime to 30030
+// (a multiple of 2 3 5 7 11 13 is not in the class map or is cleared by the
+// mask stage): m walks the 48 residues coprime to 210, 11 and 13 are
+// tested by running residues, the class of q follows from m mod 30.
+template <bool kWrite>
+__device__ __forceinline__ void harvest_prime(const HarvestCtx& h, double qb0d, uint32_t p, const uint64_t* qf,
+                                              const uint64_t* ql, const uint64_t* Q, uint32_t* cnt,
+                                              const uint32_t* base, uint32_t* __restrict__ hits, uint32_t cap,
+                                              uint32_t* __restrict__ err)
+{
+    uint64_t m = ceil_div_big(h.qb0, qb0d, p);
+    if (m < p) m = p;
+    uint64_t q = m * p;
+    if (q > h.qb1) return;
+    const uint32_t r = (uint32_t)(m % 30030);
+    uint32_t r210 = r % 210;
+    const uint32_t d0 = c_w210_next[r210];
+    uint32_t r11 = (r + d0) % 11, r13 = (r + d0) % 13, r30 = (r + d0) % 30;
+    uint32_t idx = c_w210_idx[(r210 + d0) % 210];
+    q += (uint64_t)d0 * p;
+    const uint32_t pr = p % 30;
+    while (q <= h.qb1) {
+        if (r11 && r13) {
+            const uint32_t c = c_cls_of[(r30 * pr) % 30];
+            file_hit<kWrite>(h, qf, ql, Q, q, c, cnt, base, hits, cap, err);
+        }
+        const uint32_t inc = c_w210_inc[idx];
+        idx = idx == 47 ? 0 : idx + 1;
+        q += (uint64_t)inc * p;
+        r11 += inc; if (r11 >= 11) r11 -= 11;
+        r13 += inc; if (r13 >= 13) r13 -= 13;
+        r30 += inc; if (r30 >= 30) r30 -= 30;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __restrict__ primes,
+                                                 const uint32_t* __restrict__ chunks, uint32_t cap,
+                                                 uint32_t* __restrict__ hits, uint32_t* __restrict__ hit_n,
+                                                 uint32_t* __restrict__ err)
+{
+    extern __shared__ uint32_t lds[];
+    __shared__ uint64_t qf[GBH_MAX_BATCH], ql[GBH_MAX_BATCH], Q[GBH_MAX_BATCH * 8];
+    const uint32_t nb = b.nseg * 8 * b.nwin;
+    uint32_t* cnt = lds;
+    uint32_t* base = lds + nb;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t s = tid; s < b.nseg; s += 256) { qf[s] = b.geoms[s].q_first; ql[s] = b.geoms[s].q_last; }
+    for (uint32_t i = tid; i < b.nseg * 8; i += 256) Q[i] = b.geoms[i / 8].Q[i % 8];
+    for (uint32_t i = tid; i < nb; i += 256) cnt[i] = 0;
+    __syncthreads();
+    HarvestCtx h;
+    h.qb0 = qf[0];
+    h.qb1 = ql[b.nseg - 1];
+    h.inv_span = b.nseg > 1 ? 1.0 / (double)(qf[1] - qf[0]) : 0.0;
+    h.nseg = b.nseg;
+    h.nwin = b.nwin;
+    h.lw = 31 - __clz(b.wbits);
+    h.wmask = b.wbits - 1;
+    const double qb0d = (double)h.qb0;
+    const uint32_t j0 = chunks[blockIdx.x], j1 = chunks[blockIdx.x + 1];
+    for (uint32_t j = j0 + tid; j < j1; j += 256)
+        harvest_prime<false>(h, qb0d, primes[j], qf, ql, Q, cnt, base, hits, cap, err);
+    __syncthreads();
+    for (uint32_t i = tid; i < nb; i += 256) {
+        const uint32_t c = cnt[i];
+        base[i] = c ? atomicAdd(&hit_n[i], c) : 0u;
+        cnt[i] = 0;
+    }
+    __syncthreads();
+    for (uint32_t j = j0 + tid; j < j1; j += 256)
+        harvest_prime<true>(h, qb0d, primes[j], qf, ql, Q, cnt, base, hits, cap, err);
+}
+
+// ---------------------------------------------------------------- stage 3
 // First mark >= wb0 of a prime whose first class bit is t (t may be below wb0).
 __device__ __forceinline__ uint64_t advance_to(uint32_t t, uint32_t wb0, uint32_t p)
 {
@@ -102,18 +233,22 @@ __device__ __forceinline__ uint64_t advance_to(uint32_t t, uint32_t wb0, uint32_
     return (uint64_t)t + (uint64_t)k * p;
 }
 
-__global__ void __launch_bounds__(256) k_window(gbh_geom g, const uint32_t* __restrict__ primes, uint32_t np,
-                                                uint32_t ncoop, const uint32_t* __restrict__ i0,
-                                                uint64_t* __restrict__ map)
+__global__ void __launch_bounds__(256) k_window(gbh_batch b, const uint32_t* __restrict__ primes, uint32_t np,
+                                                uint32_t ncoop, const uint32_t* __restrict__ i0tab,
+                                                const uint32_t* __restrict__ hits, const uint32_t* __restrict__ hit_n,
+                                                uint32_t cap)
 {
     extern __shared__ uint64_t win[];
     uint32_t* win32 = reinterpret_cast<uint32_t*>(win);
-    const uint32_t c = blockIdx.y;
-    const uint32_t nwords = g.wbits >> 6;
+    const uint32_t c = blockIdx.y, s = blockIdx.z;
+    const gbh_geom& g = b.geoms[s];
+    if (blockIdx.x >= g.nwin) return;
+    const uint32_t nwords = b.wbits >> 6;
     const uint64_t gw0 = (uint64_t)blockIdx.x * nwords;     // first word of the window in class c
-    const uint32_t wb0 = (uint32_t)(gw0 * 64);              // first bit (class arrays < 2^32 bits)
-    const uint32_t wb1 = wb0 + g.wbits;
+    const uint32_t wb0 = (uint32_t)(gw0 * 64);              // first bit (class arrays < 2^31 bits)
+    const uint32_t wb1 = wb0 + b.wbits;
     const uint32_t tid = threadIdx.x;
+    const uint32_t* i0 = i0tab + ((size_t)s * 8 + c) * np;
 
     // 1) mask stage: every multiple of 7..61 cleared by whole-word masks
     {
@@ -142,53 +277,40 @@ __global__ void __launch_bounds__(256) k_window(gbh_geom g, const uint32_t* __re
 
     // 2) window primes shared by the whole work-group (many marks each)
     for (uint32_t j = 0; j < ncoop; ++j) {
-        const uint32_t t0 = i0[(size_t)c * np + j];
+        const uint32_t t0 = i0[j];
         if (t0 == 0xFFFFFFFFu) continue;
         const uint32_t p = primes[j];
         const uint64_t t = advance_to(t0, wb0, p);
         for (uint64_t m = t + (uint64_t)tid * p; m < wb1; m += 256ull * p) {
-            const uint32_t b = (uint32_t)m - wb0;
-            atomicAnd(&win32[b >> 5], ~(1u << (b & 31)));
+            const uint32_t bt = (uint32_t)m - wb0;
+            atomicAnd(&win32[bt >> 5], ~(1u << (bt & 31)));
         }
     }
     // 3) the other window primes: one prime per lane
     for (uint32_t j = ncoop + tid; j < np; j += 256) {
-        const uint32_t t0 = i0[(size_t)c * np + j];
+        const uint32_t t0 = i0[j];
         if (t0 == 0xFFFFFFFFu || t0 >= wb1) continue;
         const uint32_t p = primes[j];
         for (uint64_t m = advance_to(t0, wb0, p); m < wb1; m += p) {
-            const uint32_t b = (uint32_t)m - wb0;
-            atomicAnd(&win32[b >> 5], ~(1u << (b & 31)));
+            const uint32_t bt = (uint32_t)m - wb0;
+            atomicAnd(&win32[bt >> 5], ~(1u << (bt & 31)));
+        }
+    }
+    // 4) the large primes' hits of this window
+    if (hits) {
+        const uint32_t bucket = (s * 8 + c) * b.nwin + blockIdx.x;
+        const uint32_t n = min(hit_n[bucket], cap);
+        const uint32_t* hl = hits + (size_t)bucket * cap;
+        for (uint32_t i = tid; i < n; i += 256) {
+            const uint32_t bt = hl[i];
+            atomicAnd(&win32[bt >> 5], ~(1u << (bt & 31)));
         }
     }
     __syncthreads();
 
-    // 4) coalesced store of the window
-    uint64_t* dst = map + (size_t)c * g.cw + gw0;
+    // 5) coalesced store of the window
+    uint64_t* dst = seg_map(b, s) + (size_t)c * b.cw + gw0;
     for (uint32_t x = tid; x < nwords; x += 256) dst[x] = win[x];
-}
-
-// ---------------------------------------------------------------- stage 3
-__global__ void __launch_bounds__(256) k_large(gbh_geom g, const uint32_t* __restrict__ primes, uint32_t np,
-                                               unsigned long long* __restrict__ map)
-{
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < np; j += gridDim.x * blockDim.x) {
-        const uint32_t p = primes[j];
-        const uint64_t m0 = first_mult(g.q_first, p);
-        const uint32_t m0r = (uint32_t)(m0 % 30);
-        const uint32_t pinv = kInv30[p % 30];
-        const uint64_t q0 = m0 * p;
-        if (q0 > g.q_last) continue;
-#pragma unroll
-        for (int c = 0; c < GBH_CLASSES; ++c) {
-            const uint32_t mt = (kRes[c] * pinv) % 30;
-            uint64_t q = q0 + (uint64_t)((mt + 30 - m0r) % 30) * p;
-            for (; q <= g.q_last; q += 30ull * p) {
-                const uint64_t t = (q - g.Q[c]) / 30;
-                atomicAnd(&map[(size_t)c * g.cw + (t >> 6)], ~(1ull << (t & 63)));
-            }
-        }
-    }
 }
 
 // ---------------------------------------------------------------- checker
@@ -198,9 +320,10 @@ struct ClassEvens {
     uint32_t i0;      // even index of n0 in the segment
 };
 
-__device__ __forceinline__ ClassEvens class_evens(const gbh_geom& g, uint64_t n_gpu, uint32_t cr)
+__device__ __forceinline__ ClassEvens class_evens(const gbh_geom& g, uint32_t cr)
 {
     ClassEvens e;
+    const uint64_t n_gpu = g.n_gpu;
     const uint32_t r = (uint32_t)(n_gpu % 30);
     e.n0 = n_gpu + (2 * cr + 30 - r) % 30;
     e.cnt = e.n0 <= g.seg_end ? (uint32_t)((g.seg_end - e.n0) / 30 + 1) : 0u;
@@ -208,62 +331,69 @@ __device__ __forceinline__ ClassEvens class_evens(const gbh_geom& g, uint64_t n_
     return e;
 }
 
-// coff[cr * GBH_NCAND + j] = class << 16 | bit offset of candidate j of class cr
-__global__ void __launch_bounds__(256) k_coff(gbh_geom g, uint64_t n_gpu, uint32_t* __restrict__ coff)
+// coff[(s * 15 + cr) * GBH_NCAND + j] = class << 16 | bit offset of candidate j of class cr
+__global__ void __launch_bounds__(256) k_coff(gbh_batch b, uint32_t* __restrict__ coff)
 {
-    const uint32_t cr = blockIdx.y;
+    const uint32_t cr = blockIdx.y, s = blockIdx.z;
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= c_cand_n[cr]) return;
-    const ClassEvens e = class_evens(g, n_gpu, cr);
+    const gbh_geom& g = b.geoms[s];
+    if (g.n_gpu > g.seg_end) return;
+    const ClassEvens e = class_evens(g, cr);
     const uint32_t p = c_cand_p[cr * GBH_NCAND + j];
     const uint32_t c = c_cand_c[cr * GBH_NCAND + j];
-    const uint64_t d = e.n0 - p - g.Q[c];          // >= 0 and a multiple of 30 (host contract)
-    coff[cr * GBH_NCAND + j] = (c << 16) | (uint32_t)(d / 30);
+    const uint64_t d = e.n0 - p - g.Q[c];          // >= 0 and a multiple of 30 (host contract on n_gpu)
+    coff[((size_t)s * 15 + cr) * GBH_NCAND + j] = (c << 16) | (uint32_t)(d / 30);
 }
 
 constexpr uint32_t kStageW = GBH_CHECK_WG + 8;     // staged words per class array
-constexpr uint32_t kKaMax = 128;
+constexpr uint32_t kClassesPerWg = 5;              // even classes per work-group (grid.y = 3)
 
-__global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_geom g, const uint64_t* __restrict__ map,
-                                                          const uint32_t* __restrict__ coff, uint64_t n_gpu,
-                                                          uint32_t ka, uint32_t hist_on,
-                                                          uint32_t* __restrict__ pending, uint32_t pending_cap,
-                                                          gbh_segout* __restrict__ out)
+__global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_batch b, const uint32_t* __restrict__ coff, uint32_t ka,
+                                                          uint32_t* __restrict__ pending, uint32_t pcap,
+                                                          gbh_segout* __restrict__ outs)
 {
     __shared__ uint64_t st[GBH_CLASSES * kStageW];
-    __shared__ uint32_t hl[GBH_EVEN_CLASSES * kKaMax];
-    const uint32_t tid = threadIdx.x;
+    __shared__ uint32_t hl[kClassesPerWg * GBH_KA_MAX];
+    const uint32_t tid = threadIdx.x, s = blockIdx.z;
+    const gbh_geom& g = b.geoms[s];
+    if (g.n_gpu > g.seg_end) return;
+    gbh_segout* out = outs + s;
+    const uint64_t* map = seg_map(b, s);
+    const uint32_t hist_on = g.hist_on;
     const uint32_t w0 = blockIdx.x * GBH_CHECK_WG;
     for (uint32_t i = tid; i < GBH_CLASSES * kStageW; i += GBH_CHECK_WG) {
         const uint32_t c = i / kStageW, x = i % kStageW;
         const uint32_t w = w0 + x;
-        st[i] = w < g.cw ? map[(size_t)c * g.cw + w] : 0ull;
+        st[i] = w < b.cw ? map[(size_t)c * b.cw + w] : 0ull;
     }
     if (hist_on)
-        for (uint32_t i = tid; i < GBH_EVEN_CLASSES * kKaMax; i += GBH_CHECK_WG) hl[i] = 0;
+        for (uint32_t i = tid; i < kClassesPerWg * GBH_KA_MAX; i += GBH_CHECK_WG) hl[i] = 0;
     __syncthreads();
 
     uint32_t resolved = 0;
-    for (uint32_t cr = 0; cr < GBH_EVEN_CLASSES; ++cr) {
-        const ClassEvens e = class_evens(g, n_gpu, cr);
+    uint32_t* pend = pending + (size_t)s * pcap;
+    for (uint32_t k = 0; k < kClassesPerWg; ++k) {
+        const uint32_t cr = blockIdx.y * kClassesPerWg + k;
+        const ClassEvens e = class_evens(g, cr);
         const uint32_t w = w0 + tid;
         const uint32_t k0 = w * 64;
         uint64_t valid = 0;
         if (k0 < e.cnt) valid = (e.cnt - k0 >= 64) ? ~0ull : ((1ull << (e.cnt - k0)) - 1);
         uint64_t unres = valid;
         const uint32_t nj = min(ka, c_cand_n[cr]);
-        const uint32_t* ct = coff + cr * GBH_NCAND;
+        const uint32_t* ct = coff + ((size_t)s * 15 + cr) * GBH_NCAND;
         for (uint32_t j = 0; j < nj && unres; ++j) {
             const uint32_t ent = ct[j];
             const uint32_t c = ent >> 16, off = ent & 0xFFFF;
             const uint32_t xw = c * kStageW + tid + (off >> 6);
-            const uint32_t s = off & 63;
+            const uint32_t sh = off & 63;
             const uint64_t lo = st[xw], hi = st[xw + 1];
-            const uint64_t wv = s ? ((lo >> s) | (hi << (64 - s))) : lo;
+            const uint64_t wv = sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
             const uint64_t hit = wv & unres;
             if (hit) {
                 unres &= ~hit;
-                if (hist_on) atomicAdd(&hl[cr * kKaMax + j], (uint32_t)__popcll(hit));
+                if (hist_on) atomicAdd(&hl[k * GBH_KA_MAX + j], (uint32_t)__popcll(hit));
             }
         }
         resolved += (uint32_t)__popcll(valid & ~unres);
@@ -271,18 +401,18 @@ __global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_geom g, const uint
         const uint32_t u = (uint32_t)__popcll(unres);
         uint32_t tot = 0;
         const uint32_t pre = wave_excl_scan(u, &tot);
-        uint32_t base = 0;
+        uint32_t pbase = 0;
         if (tot) {
-            if (wave_lane() == 63) base = atomicAdd(&out->pending_n, tot);
-            base = __shfl(base, 63, 64);
+            if (wave_lane() == 63) pbase = atomicAdd(&out->pending_n, tot);
+            pbase = __shfl(pbase, 63, 64);
         }
         if (u) {
-            uint32_t pos = base + pre;
+            uint32_t pos = pbase + pre;
             uint64_t m = unres;
             while (m) {
-                const uint32_t b = (uint32_t)__ffsll((long long)m) - 1;
+                const uint32_t bt = (uint32_t)__ffsll((long long)m) - 1;
                 m &= m - 1;
-                if (pos < pending_cap) pending[pos] = e.i0 + 15u * (k0 + b);
+                if (pos < pcap) pend[pos] = e.i0 + 15u * (k0 + bt);
                 ++pos;
             }
         }
@@ -291,57 +421,56 @@ __global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_geom g, const uint
     if (wave_lane() == 0 && rs) atomicAdd(&out->acct[0], rs);
     if (hist_on) {
         __syncthreads();
-        for (uint32_t i = tid; i < GBH_EVEN_CLASSES * kKaMax; i += GBH_CHECK_WG) {
+        for (uint32_t i = tid; i < kClassesPerWg * GBH_KA_MAX; i += GBH_CHECK_WG) {
             const uint32_t v = hl[i];
-            if (v) atomicAdd(&out->hist[(i / kKaMax) * GBH_NCAND + (i % kKaMax)], v);
+            if (v) atomicAdd(&out->hist[(blockIdx.y * kClassesPerWg + i / GBH_KA_MAX) * GBH_NCAND + (i % GBH_KA_MAX)], v);
         }
     }
 }
 
-__global__ void __launch_bounds__(256) k_check_b(gbh_geom g, const uint64_t* __restrict__ map,
-                                                 const uint32_t* __restrict__ coff, uint64_t n_gpu, uint32_t ka,
-                                                 uint32_t tail_t, const uint32_t* __restrict__ pending,
-                                                 uint32_t pending_max, gbh_segout* __restrict__ out)
+__global__ void __launch_bounds__(256) k_check_b(gbh_batch b, const uint32_t* __restrict__ coff, uint32_t ka,
+                                                 uint32_t tail_t, const uint32_t* __restrict__ pending, uint32_t pcap,
+                                                 gbh_segout* __restrict__ outs)
 {
-    const uint32_t np = min(out->pending_n, pending_max);
+    const uint32_t s = blockIdx.y;
+    const gbh_geom& g = b.geoms[s];
+    if (g.n_gpu > g.seg_end) return;
+    gbh_segout* out = outs + s;
+    const uint64_t* map = seg_map(b, s);
+    const uint32_t* pend = pending + (size_t)s * pcap;
+    const uint32_t np = min(out->pending_n, pcap);
     for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < np; x += gridDim.x * blockDim.x) {
-        const uint32_t i = pending[x];
+        const uint32_t i = pend[x];
         const uint64_t n = g.seg_start + 2ull * i;
         const uint32_t cr = (uint32_t)(n % 30) >> 1;
-        const ClassEvens e = class_evens(g, n_gpu, cr);
+        const ClassEvens e = class_evens(g, cr);
         const uint32_t k = (uint32_t)((n - e.n0) / 30);
         const uint32_t nj = c_cand_n[cr];
-        const uint32_t* ct = coff + cr * GBH_NCAND;
+        const uint32_t* ct = coff + ((size_t)s * 15 + cr) * GBH_NCAND;
         uint32_t jf = 0xFFFFFFFFu;
         for (uint32_t j = ka; j < nj; ++j) {
             const uint32_t ent = ct[j];
             const uint32_t c = ent >> 16, t = k + (ent & 0xFFFF);
-            if ((map[(size_t)c * g.cw + (t >> 6)] >> (t & 63)) & 1ull) { jf = j; break; }
+            if ((map[(size_t)c * b.cw + (t >> 6)] >> (t & 63)) & 1ull) { jf = j; break; }
         }
         if (jf != 0xFFFFFFFFu) {
             atomicAdd(&out->hist[cr * GBH_NCAND + jf], 1u);
             if (c_cand_p[cr * GBH_NCAND + jf] >= tail_t) atomicMin(&out->first[c_cand_pi[cr * GBH_NCAND + jf]], i);
             atomicAdd(&out->acct[1], 1u);
         } else {
-            const uint32_t s = atomicAdd(&out->left_n, 1u);
-            if (s < GBH_LEFT_CAP) out->left[s] = i;
+            const uint32_t sl = atomicAdd(&out->left_n, 1u);
+            if (sl < GBH_LEFT_CAP) out->left[sl] = i;
         }
     }
 }
 
 // ---------------------------------------------------------------- counter
-struct CountArgs {
-    uint32_t cls[8], pos0[8], lo[8], hi[8];
-    uint32_t tk[3], tk2[3], td[3], tlo[3], thi[3];
-    uint32_t nwords;
-};
-
 __device__ __forceinline__ uint64_t range_mask(uint32_t x, uint32_t lo, uint32_t hi)
 {
     const uint32_t b0 = x * 64, b1 = b0 + 64;
     if (hi <= b0 || lo >= b1 || lo >= hi) return 0;
-    const uint32_t a = lo > b0 ? lo - b0 : 0, b = hi < b1 ? hi - b0 : 64;
-    const uint64_t upto = b == 64 ? ~0ull : ((1ull << b) - 1);
+    const uint32_t a = lo > b0 ? lo - b0 : 0, e = hi < b1 ? hi - b0 : 64;
+    const uint64_t upto = e == 64 ? ~0ull : ((1ull << e) - 1);
     return upto & ~((1ull << a) - 1);
 }
 
@@ -350,19 +479,25 @@ __device__ __forceinline__ void note_gap(uint32_t g, uint32_t up, uint32_t* hl, 
     const uint32_t bin = g >> 1;
     atomicAdd(&hl[bin < GBH_GAP_BINS ? bin : GBH_GAP_BINS], 1u);
     if (g >= GBH_GAP_TAIL) {
-        const uint32_t s = atomicAdd(&out->gtail_n, 1u);
-        if (s < GBH_GAP_TCAP) { out->gtail[2 * s] = up; out->gtail[2 * s + 1] = g; }
+        const uint32_t sl = atomicAdd(&out->gtail_n, 1u);
+        if (sl < GBH_GAP_TCAP) { out->gtail[2 * sl] = up; out->gtail[2 * sl + 1] = g; }
     }
     const uint64_t key = ((uint64_t)g << 32) | (uint32_t)~up;
     if (key > best) best = key;
 }
 
-__global__ void __launch_bounds__(256) k_count_a(gbh_geom g, const uint64_t* __restrict__ map, CountArgs a,
-                                                 uint32_t* __restrict__ fl, gbh_segout* __restrict__ out)
+__global__ void __launch_bounds__(256) k_count_a(gbh_batch b, const gbh_countseg* __restrict__ css,
+                                                 uint32_t* __restrict__ scratch, uint32_t swords,
+                                                 gbh_segout* __restrict__ outs)
 {
     __shared__ uint32_t hl[GBH_GAP_BINS + 1];
     __shared__ uint32_t cs[11];
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x, s = blockIdx.y;
+    const gbh_countseg& a = css[s];
+    if (!a.on || blockIdx.x * 256 >= a.nwords) return;
+    gbh_segout* out = outs + s;
+    const uint64_t* map = seg_map(b, s);
+    uint32_t* fl = scratch + (size_t)s * swords;
     for (uint32_t i = tid; i <= GBH_GAP_BINS; i += 256) hl[i] = 0;
     if (tid < 11) cs[tid] = 0;
     __syncthreads();
@@ -372,14 +507,14 @@ __global__ void __launch_bounds__(256) k_count_a(gbh_geom g, const uint64_t* __r
         uint64_t w[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            w[k] = map[(size_t)a.cls[k] * g.cw + x] & range_mask(x, a.lo[k], a.hi[k]);
+            w[k] = map[(size_t)a.cls[k] * b.cw + x] & range_mask(x, a.lo[k], a.hi[k]);
             const uint32_t pc = (uint32_t)__popcll(w[k]);
             if (pc) atomicAdd(&cs[k], pc);
         }
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            const uint64_t lw = map[(size_t)a.tk[j] * g.cw + x] & range_mask(x, a.tlo[j], a.thi[j]);
-            const uint64_t* p2 = map + (size_t)a.tk2[j] * g.cw + x;
+            const uint64_t lw = map[(size_t)a.tk[j] * b.cw + x] & range_mask(x, a.tlo[j], a.thi[j]);
+            const uint64_t* p2 = map + (size_t)a.tk2[j] * b.cw + x;
             const uint64_t pw = a.td[j] ? ((p2[0] >> 1) | (p2[1] << 63)) : p2[0];
             const uint32_t pc = (uint32_t)__popcll(lw & pw);
             if (pc) atomicAdd(&cs[8 + j], pc);
@@ -388,15 +523,15 @@ __global__ void __launch_bounds__(256) k_count_a(gbh_geom g, const uint64_t* __r
         uint64_t any = 0;
 #pragma unroll
         for (int k = 0; k < 8; ++k) any |= w[k];
-        uint32_t first = 0xFFFFFFFFu, last = 0, prev = 0;
+        uint32_t first = 0xFFFFFFFFu, prev = 0;
         bool have = false;
         while (any) {
-            const uint32_t b = (uint32_t)__ffsll((long long)any) - 1;
+            const uint32_t bt = (uint32_t)__ffsll((long long)any) - 1;
             any &= any - 1;
-            const uint32_t base = 30u * (x * 64 + b);
+            const uint32_t base = 30u * (x * 64 + bt);
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                if ((w[k] >> b) & 1ull) {
+                if ((w[k] >> bt) & 1ull) {
                     const uint32_t pos = base + a.pos0[k];
                     if (have) note_gap(pos - prev, pos, hl, out, best);
                     else { first = pos; have = true; }
@@ -404,12 +539,11 @@ __global__ void __launch_bounds__(256) k_count_a(gbh_geom g, const uint64_t* __r
                 }
             }
         }
-        last = prev;
         fl[2 * x] = first;
-        fl[2 * x + 1] = have ? last : 0xFFFFFFFFu;
+        fl[2 * x + 1] = have ? prev : 0xFFFFFFFFu;
         if (have) {
             atomicMin(&out->pfirst, first);
-            atomicMax(&out->plast, last);
+            atomicMax(&out->plast, prev);
         }
     }
     if (best) atomicMax((unsigned long long*)&out->gmax, (unsigned long long)best);
@@ -422,11 +556,16 @@ __global__ void __launch_bounds__(256) k_count_a(gbh_geom g, const uint64_t* __r
 
 // Gap from the last prime of word x to the first prime of the next word that
 // has one (the walk across word boundaries).
-__global__ void __launch_bounds__(256) k_count_b(CountArgs a, const uint32_t* __restrict__ fl,
-                                                 gbh_segout* __restrict__ out)
+__global__ void __launch_bounds__(256) k_count_b(const gbh_countseg* __restrict__ css,
+                                                 const uint32_t* __restrict__ scratch, uint32_t swords,
+                                                 gbh_segout* __restrict__ outs)
 {
     __shared__ uint32_t hl[GBH_GAP_BINS + 1];
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x, s = blockIdx.y;
+    const gbh_countseg& a = css[s];
+    if (!a.on || blockIdx.x * 256 >= a.nwords) return;
+    gbh_segout* out = outs + s;
+    const uint32_t* fl = scratch + (size_t)s * swords;
     for (uint32_t i = tid; i <= GBH_GAP_BINS; i += 256) hl[i] = 0;
     __syncthreads();
     const uint32_t x = blockIdx.x * 256 + tid;
@@ -449,6 +588,20 @@ __global__ void __launch_bounds__(256) k_count_b(CountArgs a, const uint32_t* __
 __global__ void __launch_bounds__(256) k_fill(uint32_t* __restrict__ p, size_t n, uint32_t v)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// Output blocks of a batch reset for a new batch: zeros, first[] and pfirst ~0u.
+__global__ void __launch_bounds__(256) k_reset_out(gbh_segout* __restrict__ outs)
+{
+    gbh_segout* o = outs + blockIdx.y;
+    uint32_t* w = reinterpret_cast<uint32_t*>(o);
+    const size_t n = offsetof(gbh_segout, left) / 4;                // the leftover list needs no reset
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t byte = i * 4;
+        const bool ones = (byte >= offsetof(gbh_segout, first) && byte < offsetof(gbh_segout, left)) ||
+                          byte == offsetof(gbh_segout, pfirst);
+        w[i] = ones ? 0xFFFFFFFFu : 0u;
+    }
 }
 
 }  // namespace gbh_dev

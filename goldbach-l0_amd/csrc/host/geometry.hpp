@@ -89,9 +89,7 @@ inline gbh_geom make_geom(const SegmentGeometry& sg, const ClassGeo& cg, uint32_
         g.nc[c] = uint32_t(cg.n[c]);
         nmax = std::max<uint64_t>(nmax, cg.n[c]);
     }
-    g.wbits = wbits;
     g.nwin = uint32_t((nmax + wbits - 1) / wbits);
-    g.cw = g.nwin * (wbits / 64) + 16;
     for (int c = 0; c < 8; ++c) {
         uint64_t restore = 0;
         for (int k = 0; k < 15; ++k) {
@@ -122,9 +120,10 @@ inline void class_range(const ClassGeo& cg, int c, uint64_t lo, uint64_t hi, uin
 /// twin pairs (q, q + 2) with q in [seg_start - 5, seg_end - 5] (the ranges
 /// of consecutive segments tile the q axis).
 /// @pre seg_start >= 6.
-inline gbh_count_args make_count_args(const SegmentGeometry& sg, const ClassGeo& cg)
+inline gbh_countseg make_count_args(const SegmentGeometry& sg, const ClassGeo& cg)
 {
-    gbh_count_args a{};
+    gbh_countseg a{};
+    a.on = 1;
     int order[8] = {0, 1, 2, 3, 4, 5, 6, 7};
     std::sort(order, order + 8, [&](int x, int y) { return cg.phi[x] < cg.phi[y]; });
     uint32_t nbits = 0;

@@ -1,14 +1,15 @@
 // GPU verification driver (contracts in verifier.hpp). The run is the
 // reference's DenseRun (src/verifier.cpp:2615-2642) restated for the MI355X
 // pipeline: range resolution and segmentation exactly as the reference
-// (statistics depend on them), base primes, then a loop that enqueues
-// segment k on one in-order HIP stream while the host collects segment k-1
-// (two output slots), fail-closed coverage accounting, the leftover CPU
-// fallback and the exact statistics.
+// (statistics depend on them), then batches of consecutive segments, each
+// batch one launch per stage on one in-order HIP stream while the host
+// collects the previous batch (two output slots), fail-closed coverage
+// accounting, the leftover CPU fallback and the exact statistics.
 #include "verifier.hpp"
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -48,17 +49,25 @@ struct Buf {
     int kind = 0;
     size_t bytes = 0;
     Buf() = default;
-    Buf(gbh_ctx* ctx, size_t n, int k) : c(ctx), kind(k), bytes(n)
-    {
-        check(gbh_alloc(c, std::max<size_t>(n, 256), k, &p), "gbh_alloc");
-    }
-    ~Buf() { if (p) (void)gbh_free(c, p, kind); }
     Buf(const Buf&) = delete;
     Buf& operator=(const Buf&) = delete;
-    Buf& operator=(Buf&& o) noexcept
+    ~Buf() { release(); }
+    void release()
     {
-        std::swap(c, o.c); std::swap(p, o.p); std::swap(kind, o.kind); std::swap(bytes, o.bytes);
-        return *this;
+        if (p) (void)gbh_free(c, p, kind);
+        p = nullptr;
+        bytes = 0;
+    }
+    // at least n bytes (contents are not kept on growth)
+    void ensure(gbh_ctx* ctx, size_t n, int k)
+    {
+        n = std::max<size_t>(n, 256);
+        if (p && bytes >= n) return;
+        release();
+        c = ctx;
+        kind = k;
+        check(gbh_alloc(c, n, k, &p), "gbh_alloc");
+        bytes = n;
     }
     template <class T> T* as() const { return static_cast<T*>(p); }
 };
@@ -103,7 +112,7 @@ struct Candidates {
     }
 };
 
-enum { EV_BEGIN, EV_WINDOW, EV_LARGE, EV_CHECK, EV_COUNT, EV_END, EV_N };
+enum { EV_BEGIN, EV_SETUP, EV_WINDOW, EV_CHECK, EV_COUNT, EV_END, EV_N };
 
 // Knobs of removed reference pipelines are refused, not ignored: a run that
 // asks for one would otherwise silently run something else
@@ -119,6 +128,19 @@ void refuse_removed_knobs()
                                      "pipeline)");
 }
 
+// wheel fraction of the harvest: multipliers coprime to 7, 11 and 13
+constexpr double kWheelFrac = (6.0 / 7) * (10.0 / 11) * (12.0 / 13);
+
+// One segment of a batch, host side.
+struct SegJob {
+    SegmentGeometry sg;
+    ClassGeo cg;
+    uint64_t index = 0;
+    uint64_t n_gpu = 0;
+    uint64_t gpu_evens = 0;
+    bool sampled = false, walked = false, counted = false;
+};
+
 }  // namespace
 
 // ---------------------------------------------------------------- Engine state
@@ -128,14 +150,15 @@ struct Engine::State {
     uint64_t max_limit = 0, max_span = 0;
     std::vector<uint32_t> base;          // odd primes <= sqrt(max_limit) + 16
     uint32_t win_lo = 0, win_hi = 0;     // window primes base[win_lo, win_hi), large base[win_hi, ...)
-    uint32_t wbits = 0, ka = 0, ncoop = 0;
+    uint32_t ka = 0, kbatch = 0, win_kb = 0;
     double init_s = 0, cache_s = 0, module_s = 0, upload_s = 0;
+    std::vector<double> inv_prefix;      // prefix sums of 1/p over the large primes
     std::optional<Ctx> ctx;
     gbh_ctx* c = nullptr;
-    Buf d_win, d_large, d_i0, d_map, d_coff, d_pending, d_scratch;
-    Buf d_out[2], h_out[2];
-    uint64_t cw_cap = 0;
-    uint32_t pending_cap = 0, scratch_words = 0;
+    int cus = 256;
+    Buf d_win, d_large, d_chunks, d_i0, d_map, d_coff, d_pending, d_scratch, d_hits, d_hit_n;
+    Buf d_geoms, d_cs;
+    Buf d_out[2], h_out[2], h_geoms[2], h_cs[2], h_chunks;
     Event ev[2][EV_N];
     Candidates cand;
 
@@ -146,10 +169,10 @@ struct Engine::State {
         max_limit = max_limit_;
         max_span = std::max<uint64_t>(max_span_ + (max_span_ & 1), 4);
         if (max_span > (uint64_t(1) << 35)) throw std::runtime_error("segment span above 2^35 integers");
-        const uint32_t kb = env::u32("GB_WIN_KB", WIN_KB_DEFAULT, 4, 128);
-        if (kb & (kb - 1)) throw std::runtime_error("GB_WIN_KB must be a power of two");
-        wbits = kb * 8192;
-        ka = env::u32("GB_KA", KA_DEFAULT, 1, 128);
+        win_kb = env::u32("GB_WIN_KB", 0, 0, 128);
+        if (win_kb & (win_kb - 1)) throw std::runtime_error("GB_WIN_KB must be a power of two");
+        ka = env::u32("GB_KA", KA_DEFAULT, 1, GBH_KA_MAX);
+        kbatch = env::u32("GB_BATCH", 16, 1, GBH_MAX_BATCH);
         // base primes (the reference loads them from its prime cache;
         // here a segmented sieve on the host threads)
         const auto tc = clock_::now();
@@ -158,37 +181,31 @@ struct Engine::State {
         win_lo = uint32_t(std::lower_bound(base.begin(), base.end(), 67u) - base.begin());
         const uint64_t win_p = env::u64("GB_WIN_P", WIN_P_DEFAULT, 67, UINT32_MAX);
         win_hi = std::max(win_lo, uint32_t(std::lower_bound(base.begin(), base.end(), win_p) - base.begin()));
-        // window primes with at least two marks per lane per window are
-        // marked by the whole work-group
-        while (win_lo + ncoop < win_hi && base[win_lo + ncoop] < wbits / 128) ++ncoop;
+        inv_prefix.assign(base.size() - win_hi + 1, 0.0);
+        for (size_t i = win_hi; i < base.size(); ++i) inv_prefix[i - win_hi + 1] = inv_prefix[i - win_hi] + 1.0 / base[i];
         const auto tm = clock_::now();
         ctx.emplace();
         c = ctx->c;
+        gbh_device_info_t info{};
+        check(gbh_device_info(c, &info), "gbh_device_info");
+        cus = info.cus > 0 ? info.cus : 256;
         module_s = since(tm);
         const auto tu = clock_::now();
         const uint32_t nwin = win_hi - win_lo, nlarge = uint32_t(base.size()) - win_hi;
-        d_win = Buf(c, size_t(nwin) * 4, 0);
-        d_large = Buf(c, size_t(nlarge) * 4, 0);
+        d_win.ensure(c, size_t(nwin) * 4, 0);
+        d_large.ensure(c, size_t(nlarge) * 4, 0);
         if (nwin) check(gbh_copy_h2d(c, 0, d_win.p, base.data() + win_lo, size_t(nwin) * 4), "upload window primes");
         if (nlarge) check(gbh_copy_h2d(c, 0, d_large.p, base.data() + win_hi, size_t(nlarge) * 4), "upload large primes");
-        d_i0 = Buf(c, size_t(nwin) * 8 * 4, 0);
-        // the largest segment: the merged last one spans below 2 x span
-        const uint64_t qodds_max = (2 * max_span + MAX_GB_PRIME) / 2 + 2;
-        const uint64_t nc_max = (qodds_max + 14) / 15 + 1;
-        const uint64_t nwin_max = (nc_max + wbits - 1) / wbits;
-        cw_cap = nwin_max * (wbits / 64) + 16;
-        d_map = Buf(c, size_t(8) * cw_cap * 8, 0);
-        check(gbh_zero(c, 0, d_map.p, size_t(8) * cw_cap * 8), "zero map");
-        d_coff = Buf(c, size_t(15) * GBH_NCAND * 4, 0);
-        pending_cap = uint32_t(std::min<uint64_t>((2 * max_span + 4) / 4 + 4096, UINT32_MAX / 2));
-        d_pending = Buf(c, size_t(pending_cap) * 4, 0);
-        scratch_words = uint32_t(2 * (nc_max / 64 + 4));
-        d_scratch = Buf(c, size_t(scratch_words) * 4, 0);
         for (int s = 0; s < 2; ++s) {
-            d_out[s] = Buf(c, sizeof(gbh_segout), 0);
-            h_out[s] = Buf(c, sizeof(gbh_segout), 1);
+            d_out[s].ensure(c, sizeof(gbh_segout) * kbatch, 0);
+            h_out[s].ensure(c, sizeof(gbh_segout) * kbatch, 1);
+            h_geoms[s].ensure(c, sizeof(gbh_geom) * kbatch, 1);
+            h_cs[s].ensure(c, sizeof(gbh_countseg) * kbatch, 1);
             for (auto& e : ev[s]) e.create(c);
         }
+        d_geoms.ensure(c, sizeof(gbh_geom) * kbatch, 0);
+        d_cs.ensure(c, sizeof(gbh_countseg) * kbatch, 0);
+        d_coff.ensure(c, size_t(kbatch) * 15 * GBH_NCAND * 4, 0);
         check(gbh_stream_sync(c, 0), "upload sync");
         upload_s = since(tu);
         init_s = since(t0);
@@ -197,21 +214,9 @@ struct Engine::State {
 
 namespace {
 
-struct SegJob {
-    SegmentGeometry sg;
-    ClassGeo cg;
-    gbh_geom geom{};
-    uint64_t index = 0;
-    uint64_t n_gpu = 0;
-    uint64_t gpu_evens = 0;
-    int slot = 0;
-    bool sampled = false, walked = false, counted = false;
-    clock_::time_point t_wall;
-};
-
 // One run over [start, limit] on an engine's resident state (the
 // reference's DenseRun phases: resolve_range, statistics set-up, the
-// two-slot segment loop, finish and commit).
+// segment loop, finish and commit), issued in batches of segments.
 class DenseRun {
 public:
     DenseRun(Engine::State& e, uint64_t limit, uint64_t segment_even, bool verbose, uint64_t start_even,
@@ -221,12 +226,16 @@ public:
     {
     }
     RunStats run();
+    /// The class map of segment [seg_start, ...] alone (validators).
+    uint32_t map_of(uint64_t seg_start, std::vector<uint64_t>& words, uint64_t Q[8], uint32_t nc[8]);
 
 private:
     void resolve_range();
     void configure_stats();
-    void enqueue(SegJob& j);
-    void collect(SegJob& j);
+    void plan_buffers();
+    void enqueue(std::vector<SegJob>& batch, int slot, bool check_and_count);
+    void collect(std::vector<SegJob>& batch, int slot);
+    void collect_segment(const SegJob& j, const gbh_segout& o, double share[4]);
 
     Engine::State& E;
     uint64_t limit_, span_;
@@ -237,7 +246,8 @@ private:
     RunStats rs_;
     uint64_t total_evens_ = 0;
     clock_::time_point t_start_;
-    uint32_t nwin_ = 0, nlarge_ = 0, split_ = 0;   // this run's window / large prime counts
+    uint32_t nwin_ = 0, nlarge_ = 0, split_ = 0, ncoop_ = 0;   // this run's window / large prime counts
+    uint32_t wbits_ = 0, nwin_max_ = 0, cw_ = 0, cap_ = 0, pcap_ = 0, swords_ = 0, nchunks_ = 0, kb_ = 1;
     bool stats_on_ = false, gaps_on_ = false, gaps_exact_ = false;
     uint32_t sample_ = 256;
     std::optional<GoldbachStats> gs_;
@@ -248,11 +258,11 @@ private:
 
 void DenseRun::resolve_range()
 {
-    if (limit_ < 4) throw std::runtime_error("limit < 4: no even n >= 4 to verify");
     refuse_removed_knobs();
     require_supported_limit(limit_);
     limit_ &= ~uint64_t(1);
     rs_.limit = limit_;
+    if (limit_ < 4) throw std::runtime_error("limit < 4: no even n >= 4 to verify");
     if (limit_ > E.max_limit) throw std::runtime_error("limit above the engine's prime bound");
     if (start_ < 4) start_ = 4;
     if (start_ & 1) start_ += 1;
@@ -277,17 +287,12 @@ void DenseRun::resolve_range()
         if (split_arg_ < E.win_lo || split_arg_ > E.win_hi)
             throw std::runtime_error("--split must lie in [" + std::to_string(E.win_lo) + ", " +
                                      std::to_string(E.win_hi) + "] on this engine");
-        hi = std::min(split_arg_, nb);
+        if (split_arg_ < hi)
+            std::fprintf(stderr, "note: --split below the engine's window bound: the windows keep their primes\n");
     }
     nwin_ = hi > E.win_lo ? hi - E.win_lo : 0;
     split_ = E.win_lo + nwin_;
     nlarge_ = nb > E.win_hi ? nb - E.win_hi : 0;
-    if (split_ < E.win_hi && nb > split_) {
-        // a lowered split: primes [split, win_hi) would go to the large stage,
-        // which takes the device list from win_hi on; keep them in the windows
-        nwin_ = std::min(E.win_hi, nb) - E.win_lo;
-        split_ = E.win_lo + nwin_;
-    }
     rs_.split_used = split_;
 }
 
@@ -313,75 +318,162 @@ void DenseRun::configure_stats()
             throw std::runtime_error("GB_KA too large: a phase-A candidate reaches the tail bound");
 }
 
-void DenseRun::enqueue(SegJob& j)
+// Window size, strides, hit-list capacity and buffer sizes of this run.
+void DenseRun::plan_buffers()
 {
+    const uint64_t nseg = std::min<uint64_t>(E.kbatch, (total_evens_ + span_ / 2 - 1) / std::max<uint64_t>(span_ / 2, 1) + 1);
+    // the largest segment: the merged last one spans below 2 x span
+    const uint64_t qodds_max = (2 * span_ + MAX_GB_PRIME) / 2 + 2;
+    const uint64_t nc_max = (qodds_max + 14) / 15 + 1;
+    const uint64_t nc_typ = (span_ / 2 + MAX_GB_PRIME / 2 + 14) / 15 + 1;
+    // window: the largest power of two that still gives two work-groups per
+    // CU over a batch of typical segments (GB_WIN_KB fixes it)
+    if (E.win_kb) {
+        wbits_ = E.win_kb * 8192;
+    } else {
+        wbits_ = 1u << 19;
+        while (wbits_ > (1u << 15) && nseg * 8 * ((nc_typ + wbits_ - 1) / wbits_) < 2ull * E.cus) wbits_ >>= 1;
+    }
+    nwin_max_ = uint32_t((nc_max + wbits_ - 1) / wbits_);
+    while (nseg * 8 * nwin_max_ > 8192 && wbits_ < (1u << 20)) {   // harvest LDS counters
+        wbits_ <<= 1;
+        nwin_max_ = uint32_t((nc_max + wbits_ - 1) / wbits_);
+    }
+    if (nseg * 8 * nwin_max_ > 8192) throw std::runtime_error("GB_BATCH too large for this span");
+    cw_ = nwin_max_ * (wbits_ / 64) + 16;
+    ncoop_ = 0;
+    while (ncoop_ < nwin_ && E.base[E.win_lo + ncoop_] < wbits_ / 128) ++ncoop_;
+    // hits per window bucket: wbits * (wheel fraction) * sum of 1/p
+    const double lam = nlarge_ ? wbits_ * kWheelFrac * E.inv_prefix[nlarge_] : 0.0;
+    cap_ = uint32_t(std::min(lam * 1.1 + 8 * std::sqrt(lam) + 1024, 4e9 / (double(E.kbatch) * 8 * nwin_max_)));
+    const uint64_t evens_max = span_ + 2;
+    pcap_ = uint32_t(std::min<uint64_t>(evens_max / 32 + 65536, UINT32_MAX / 2 / E.kbatch));
+    swords_ = uint32_t(2 * (nc_max / 64 + 4));
     gbh_ctx* c = E.c;
-    j.cg = class_geometry(j.sg.q_first, j.sg.q_odds);
-    j.geom = make_geom(j.sg, j.cg, E.wbits);
-    if (j.geom.cw > E.cw_cap) throw std::runtime_error("segment class stride above the allocation");
-    // evens below q_first + 9780 (only at the very bottom of the range) have
-    // candidate q outside the map: the CPU resolves them
-    uint64_t n_gpu = std::max<uint64_t>(j.sg.seg_start, j.sg.q_first + 9780);
-    n_gpu += n_gpu & 1;
-    j.n_gpu = n_gpu;
-    j.gpu_evens = n_gpu <= j.sg.seg_end ? (j.sg.seg_end - n_gpu) / 2 + 1 : 0;
-    const int s = j.slot;
-    gbh_segout* out = E.d_out[s].as<gbh_segout>();
-    check(gbh_event_record(c, 0, E.ev[s][EV_BEGIN].e), "event");
-    check(gbh_zero(c, 0, out, sizeof(gbh_segout)), "zero outputs");
-    check(gbh_fill32(c, 0, out->first, GBH_NCAND, ~0u), "fill first");
-    check(gbh_fill32(c, 0, &out->pfirst, 1, ~0u), "fill pfirst");
-    if (nwin_) {
-        gbh_setup_args a{&j.geom, E.d_win.as<uint32_t>(), nwin_, E.d_i0.as<uint32_t>()};
-        check(gbh_launch_setup(c, 0, &a), "setup");
-    }
-    gbh_window_args w{&j.geom, E.d_win.as<uint32_t>(), nwin_, std::min(E.ncoop, nwin_), E.d_i0.as<uint32_t>(),
-                      E.d_map.as<uint64_t>()};
-    check(gbh_launch_window(c, 0, &w), "window sieve");
-    check(gbh_event_record(c, 0, E.ev[s][EV_WINDOW].e), "event");
+    kb_ = uint32_t(nseg);
+    const size_t K = kb_;
+    E.d_map.ensure(c, K * 8 * cw_ * 8, 0);
+    E.d_i0.ensure(c, K * 8 * size_t(std::max(nwin_, 1u)) * 4, 0);
+    E.d_pending.ensure(c, K * size_t(pcap_) * 4, 0);
+    E.d_scratch.ensure(c, K * size_t(swords_) * 4, 0);
+    E.d_hit_n.ensure(c, K * 8 * size_t(nwin_max_) * 4, 0);
+    if (nlarge_) E.d_hits.ensure(c, K * 8 * size_t(nwin_max_) * cap_ * 4, 0);
+    // harvest chunks: contiguous prime ranges with about the same expected
+    // hits (sum of 1/p) and at most 64k primes each
+    std::vector<uint32_t> ch{0};
     if (nlarge_) {
-        gbh_large_args l{&j.geom, E.d_large.as<uint32_t>(), nlarge_, 0, E.d_map.as<uint64_t>()};
-        check(gbh_launch_large(c, 0, &l), "large primes");
+        const double per_chunk = 2e5 / (double(K) * span_ * (8.0 / 30) * kWheelFrac);
+        uint32_t j = 0;
+        while (j < nlarge_) {
+            const double target = E.inv_prefix[j] + per_chunk;
+            uint32_t k = uint32_t(std::upper_bound(E.inv_prefix.begin() + j + 1, E.inv_prefix.begin() + nlarge_ + 1,
+                                                   target) - E.inv_prefix.begin()) - 1;
+            k = std::max(k, j + 256);
+            k = std::min({k, j + 65536, nlarge_});
+            ch.push_back(k);
+            j = k;
+        }
     }
-    check(gbh_event_record(c, 0, E.ev[s][EV_LARGE].e), "event");
-    if (j.gpu_evens) {
-        gbh_check_args ca{&j.geom, E.d_map.as<uint64_t>(), n_gpu, E.ka, j.sampled ? 1u : 0u, E.d_coff.as<uint32_t>(),
-                          E.d_pending.as<uint32_t>(), E.pending_cap, 0, out};
-        check(gbh_launch_check_a(c, 0, &ca), "check phase A");
-        gbh_checkb_args cb{&j.geom, E.d_map.as<uint64_t>(), n_gpu, E.ka, CHECK_TAIL, E.d_coff.as<uint32_t>(),
-                           E.d_pending.as<uint32_t>(), E.pending_cap, 0, out};
-        check(gbh_launch_check_b(c, 0, &cb), "check phase B");
-    }
-    check(gbh_event_record(c, 0, E.ev[s][EV_CHECK].e), "event");
-    if (j.counted) {
-        gbh_count_args ct = make_count_args(j.sg, j.cg);
-        ct.geom = &j.geom;
-        ct.map = E.d_map.as<uint64_t>();
-        ct.scratch = E.d_scratch.as<uint32_t>();
-        ct.scratch_words = E.scratch_words;
-        ct.out = out;
-        check(gbh_launch_count(c, 0, &ct), "count");
-    }
-    check(gbh_event_record(c, 0, E.ev[s][EV_COUNT].e), "event");
-    check(gbh_copy_d2h(c, 0, E.h_out[s].p, out, sizeof(gbh_segout)), "copy outputs");
-    check(gbh_event_record(c, 0, E.ev[s][EV_END].e), "event");
+    nchunks_ = uint32_t(ch.size() - 1);
+    E.h_chunks.ensure(c, ch.size() * 4, 1);
+    E.d_chunks.ensure(c, ch.size() * 4, 0);
+    std::memcpy(E.h_chunks.p, ch.data(), ch.size() * 4);
+    check(gbh_copy_h2d(c, 0, E.d_chunks.p, E.h_chunks.p, ch.size() * 4), "upload chunks");
+    check(gbh_stream_sync(c, 0), "chunk sync");
 }
 
-void DenseRun::collect(SegJob& j)
+void DenseRun::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_count)
 {
     gbh_ctx* c = E.c;
-    const int s = j.slot;
-    check(gbh_event_sync(c, E.ev[s][EV_END].e), "segment sync");
-    const gbh_segout& o = *E.h_out[s].as<gbh_segout>();
-    const double ts = elapsed_s(c, E.ev[s][EV_BEGIN], E.ev[s][EV_WINDOW]);
-    const double tl = elapsed_s(c, E.ev[s][EV_WINDOW], E.ev[s][EV_LARGE]);
-    const double tc = elapsed_s(c, E.ev[s][EV_LARGE], E.ev[s][EV_CHECK]);
-    const double tn = elapsed_s(c, E.ev[s][EV_CHECK], E.ev[s][EV_COUNT]);
+    const uint32_t nseg = uint32_t(batch.size());
+    gbh_geom* hg = E.h_geoms[slot].as<gbh_geom>();
+    gbh_countseg* hc = E.h_cs[slot].as<gbh_countseg>();
+    uint32_t nwin = 1, nwords = 0;
+    for (uint32_t s = 0; s < nseg; ++s) {
+        SegJob& j = batch[s];
+        j.cg = class_geometry(j.sg.q_first, j.sg.q_odds);
+        gbh_geom g = make_geom(j.sg, j.cg, wbits_);
+        // evens below q_first + 9780 (only at the very bottom of the range)
+        // have candidate q outside the map: the CPU resolves them
+        uint64_t n_gpu = std::max<uint64_t>(j.sg.seg_start, j.sg.q_first + 9780);
+        n_gpu += n_gpu & 1;
+        g.n_gpu = n_gpu;
+        j.n_gpu = n_gpu;
+        g.hist_on = j.sampled ? 1u : 0u;
+        j.gpu_evens = n_gpu <= j.sg.seg_end ? (j.sg.seg_end - n_gpu) / 2 + 1 : 0;
+        if (g.nwin > nwin_max_) throw std::runtime_error("segment longer than the run's buffers");
+        nwin = std::max(nwin, g.nwin);
+        hg[s] = g;
+        gbh_countseg cs{};
+        if (j.counted) {
+            cs = make_count_args(j.sg, j.cg);
+            nwords = std::max(nwords, cs.nwords);
+        }
+        hc[s] = cs;
+    }
+    gbh_batch b{E.d_geoms.as<gbh_geom>(), nseg, cw_, wbits_, nwin, E.d_map.as<uint64_t>()};
+    gbh_segout* out = E.d_out[slot].as<gbh_segout>();
+    auto& ev = E.ev[slot];
+    check(gbh_event_record(c, 0, ev[EV_BEGIN].e), "event");
+    check(gbh_copy_h2d(c, 0, E.d_geoms.p, hg, sizeof(gbh_geom) * nseg), "upload geometry");
+    check(gbh_copy_h2d(c, 0, E.d_cs.p, hc, sizeof(gbh_countseg) * nseg), "upload count parameters");
+    check(gbh_reset_outputs(c, 0, out, nseg), "reset outputs");
+    if (nwin_) {
+        gbh_setup_args a{b, E.d_win.as<uint32_t>(), nwin_, 0, E.d_i0.as<uint32_t>()};
+        check(gbh_launch_setup(c, 0, &a), "setup");
+    }
+    if (nlarge_) {
+        check(gbh_zero(c, 0, E.d_hit_n.p, size_t(nseg) * 8 * nwin * 4), "zero hit counts");
+        gbh_harvest_args h{b, E.d_large.as<uint32_t>(), E.d_chunks.as<uint32_t>(), nchunks_, cap_,
+                           E.d_hits.as<uint32_t>(), E.d_hit_n.as<uint32_t>(), &out[0].err};
+        check(gbh_launch_harvest(c, 0, &h), "harvest");
+    }
+    check(gbh_event_record(c, 0, ev[EV_SETUP].e), "event");
+    gbh_window_args w{b, E.d_win.as<uint32_t>(), nwin_, std::min(ncoop_, nwin_), E.d_i0.as<uint32_t>(),
+                      nlarge_ ? E.d_hits.as<uint32_t>() : nullptr, E.d_hit_n.as<uint32_t>(), cap_, 0};
+    check(gbh_launch_window(c, 0, &w), "window sieve");
+    check(gbh_event_record(c, 0, ev[EV_WINDOW].e), "event");
+    if (check_and_count) {
+        gbh_check_args ca{b, E.ka, pcap_, E.d_coff.as<uint32_t>(), E.d_pending.as<uint32_t>(), out};
+        check(gbh_launch_check_a(c, 0, &ca), "check phase A");
+        gbh_checkb_args cb{b, E.ka, CHECK_TAIL, E.d_coff.as<uint32_t>(), E.d_pending.as<uint32_t>(), pcap_, 0, out};
+        check(gbh_launch_check_b(c, 0, &cb), "check phase B");
+    }
+    check(gbh_event_record(c, 0, ev[EV_CHECK].e), "event");
+    if (check_and_count && nwords) {
+        gbh_count_args ct{b, E.d_cs.as<gbh_countseg>(), nwords, swords_, E.d_scratch.as<uint32_t>(), out};
+        check(gbh_launch_count(c, 0, &ct), "count");
+    }
+    check(gbh_event_record(c, 0, ev[EV_COUNT].e), "event");
+    check(gbh_copy_d2h(c, 0, E.h_out[slot].p, out, sizeof(gbh_segout) * nseg), "copy outputs");
+    check(gbh_event_record(c, 0, ev[EV_END].e), "event");
+}
+
+void DenseRun::collect(std::vector<SegJob>& batch, int slot)
+{
+    gbh_ctx* c = E.c;
+    auto& ev = E.ev[slot];
+    check(gbh_event_sync(c, ev[EV_END].e), "batch sync");
+    const double tl = elapsed_s(c, ev[EV_BEGIN], ev[EV_SETUP]);
+    const double ts = elapsed_s(c, ev[EV_SETUP], ev[EV_WINDOW]);
+    const double tc = elapsed_s(c, ev[EV_WINDOW], ev[EV_CHECK]);
+    const double tn = elapsed_s(c, ev[EV_CHECK], ev[EV_COUNT]);
     t_sieve_ += ts; t_large_ += tl; t_check_ += tc; t_count_ += tn;
+    const gbh_segout* outs = E.h_out[slot].as<gbh_segout>();
+    if (outs[0].err & 1u) throw std::runtime_error("hit-list bucket overflow (raise the capacity)");
+    double share[4] = {ts / batch.size(), tl / batch.size(), tc / batch.size(), tn / batch.size()};
+    for (size_t s = 0; s < batch.size(); ++s) collect_segment(batch[s], outs[s], share);
+    if (phase_trace_)
+        std::fprintf(stderr, "batch of %zu segments from %llu: harvest+setup %.3f ms window %.3f ms check %.3f ms "
+                     "count %.3f ms\n", batch.size(), (unsigned long long)batch[0].sg.seg_start, tl * 1e3, ts * 1e3,
+                     tc * 1e3, tn * 1e3);
+}
+
+void DenseRun::collect_segment(const SegJob& j, const gbh_segout& o, double share[4])
+{
     const std::string seg = " in segment [" + std::to_string(j.sg.seg_start) + ", " + std::to_string(j.sg.seg_end) + "]";
-    if (o.err) throw std::runtime_error("device error flags " + std::to_string(o.err) + seg);
-    if (o.pending_n > E.pending_cap)
-        throw std::runtime_error("pending overflow (" + std::to_string(o.pending_n) + ")" + seg);
+    if (o.err & ~1u) throw std::runtime_error("device error flags " + std::to_string(o.err) + seg);
+    if (o.pending_n > pcap_) throw std::runtime_error("pending overflow (" + std::to_string(o.pending_n) + ")" + seg);
     // coverage: every even accounted for, by phase A, phase B or a leftover
     if (uint64_t(o.acct[0]) + o.pending_n != j.gpu_evens)
         throw std::runtime_error("coverage: phase A resolved " + std::to_string(o.acct[0]) + " + pending " +
@@ -392,7 +484,7 @@ void DenseRun::collect(SegJob& j)
     if (o.left_n > GBH_LEFT_CAP || o.left_n > MAX_LEFTOVERS) throw std::runtime_error("leftover overflow" + seg);
     // the CPU fallback: leftovers, and the evens below the GPU's first checkable even
     std::vector<uint64_t> cpu_n;
-    for (uint64_t n = j.sg.seg_start; n < j.n_gpu && n <= j.sg.seg_end; n += 2) cpu_n.push_back(n);
+    for (uint64_t n = j.sg.seg_start; n <= j.sg.seg_end && n < j.n_gpu; n += 2) cpu_n.push_back(n);
     for (uint32_t k = 0; k < o.left_n; ++k) {
         const uint32_t i = o.left[k];
         if (i >= j.sg.seg_evens) throw std::runtime_error("leftover index " + std::to_string(i) + " outside the segment");
@@ -436,10 +528,6 @@ void DenseRun::collect(SegJob& j)
         evens_done_ += j.sg.seg_evens;
         gs_->write_live(start_, limit_, evens_done_, total_evens_, since(t_start_), false);
     }
-    if (phase_trace_)
-        std::fprintf(stderr, "seg %llu [%llu, %llu]: window %.3f ms large %.3f ms check %.3f ms count %.3f ms pending %u\n",
-                     (unsigned long long)j.index, (unsigned long long)j.sg.seg_start, (unsigned long long)j.sg.seg_end,
-                     ts * 1e3, tl * 1e3, tc * 1e3, tn * 1e3, o.pending_n);
     if (verbose_)
         std::printf("  seg %llu: [%llu, %llu] evens=%llu n_left=%u\n", (unsigned long long)j.index,
                     (unsigned long long)j.sg.seg_start, (unsigned long long)j.sg.seg_end,
@@ -449,15 +537,14 @@ void DenseRun::collect(SegJob& j)
         st.seg_start = j.sg.seg_start;
         st.seg_end = j.sg.seg_end;
         st.evens = j.sg.seg_evens;
-        st.tiles = uint64_t(j.geom.nwin) * 8;
+        st.tiles = uint64_t((j.cg.n[0] + wbits_ - 1) / wbits_) * 8;
         st.large_items = nlarge_;
         st.sieve_split = split_;
         st.mask_count = E.win_lo;
         st.leftovers = o.left_n;
-        st.sieve_seconds = ts;
-        st.large_seconds = tl;
-        st.check_seconds = tc + tn;
-        st.wall_seconds = since(j.t_wall);
+        st.sieve_seconds = share[0];
+        st.large_seconds = share[1];
+        st.check_seconds = share[2] + share[3];
         rs_.per_segment.push_back(st);
     }
 }
@@ -467,29 +554,38 @@ RunStats DenseRun::run()
     t_start_ = clock_::now();
     resolve_range();
     configure_stats();
+    plan_buffers();
     phase_trace_ = env::present("GB_PHASE");
     const auto t_gpu = clock_::now();
-    SegJob jobs[2];
-    SegJob* prev = nullptr;
+    std::vector<SegJob> batches[2];
+    int cur = 0;
+    bool pending_batch = false;
     uint64_t k = 0;
-    for (uint64_t seg_start = start_; seg_start <= limit_; seg_start += span_, ++k) {
-        SegJob& j = jobs[k & 1];
-        j = SegJob{};
-        j.sg = segment_geometry(seg_start, limit_, span_);
-        j.index = k;
-        j.slot = int(k & 1);
-        j.sampled = stats_on_ && k % sample_ == 0;
-        j.walked = gaps_on_ && (j.sampled || gaps_exact_);
-        j.counted = stats_on_;
-        j.t_wall = clock_::now();
-        enqueue(j);
-        rs_.total_tiles += uint64_t(j.geom.nwin) * 8;
-        rs_.total_large_items += nlarge_;
-        if (prev) collect(*prev);
-        prev = &j;
-        if (j.sg.seg_end == limit_) { ++k; break; }
+    bool done = false;
+    uint64_t seg_start = start_;
+    while (!done) {
+        std::vector<SegJob>& bt = batches[cur];
+        bt.clear();
+        while (bt.size() < kb_ && !done) {
+            SegJob j;
+            j.sg = segment_geometry(seg_start, limit_, span_);
+            j.index = k;
+            j.sampled = stats_on_ && k % sample_ == 0;
+            j.walked = gaps_on_ && (j.sampled || gaps_exact_);
+            j.counted = stats_on_;
+            bt.push_back(j);
+            ++k;
+            rs_.total_large_items += nlarge_;
+            if (j.sg.seg_end == limit_) done = true;
+            else seg_start += span_;
+        }
+        enqueue(bt, cur, true);
+        for (const SegJob& j : bt) rs_.total_tiles += uint64_t((j.cg.n[0] + wbits_ - 1) / wbits_) * 8;
+        if (pending_batch) collect(batches[cur ^ 1], cur ^ 1);
+        pending_batch = true;
+        cur ^= 1;
     }
-    if (prev) collect(*prev);
+    if (pending_batch) collect(batches[cur ^ 1], cur ^ 1);
     rs_.segments = k;
     rs_.gpu_seconds = since(t_gpu);
     rs_.init_seconds = since(t_start_) - rs_.gpu_seconds;
@@ -516,6 +612,22 @@ RunStats DenseRun::run()
     return rs_;
 }
 
+uint32_t DenseRun::map_of(uint64_t seg_start, std::vector<uint64_t>& words, uint64_t Q[8], uint32_t nc[8])
+{
+    resolve_range();
+    plan_buffers();
+    std::vector<SegJob> bt(1);
+    bt[0].sg = segment_geometry(seg_start, limit_, span_);
+    enqueue(bt, 0, false);
+    check(gbh_event_sync(E.c, E.ev[0][EV_END].e), "sync");
+    if (E.h_out[0].as<gbh_segout>()[0].err & 1u) throw std::runtime_error("hit-list bucket overflow");
+    words.assign(size_t(8) * cw_, 0);
+    check(gbh_copy_d2h(E.c, 0, words.data(), E.d_map.p, words.size() * 8), "copy map");
+    check(gbh_stream_sync(E.c, 0), "sync");
+    for (int c = 0; c < 8; ++c) { Q[c] = bt[0].cg.Q[c]; nc[c] = uint32_t(bt[0].cg.n[c]); }
+    return cw_;
+}
+
 }  // namespace
 
 Engine::Engine(uint64_t max_limit, uint64_t max_span) : s_(std::make_unique<State>(max_limit, max_span)) {}
@@ -536,6 +648,18 @@ RunStats Engine::run(uint64_t limit, uint64_t segment_even, bool verbose, uint64
     return r.run();
 }
 
+namespace {
+// The span a run over [start, limit] keeps after the range resolution.
+uint64_t resolved_span(uint64_t limit, uint64_t start_even, uint64_t segment_even)
+{
+    const uint64_t lim = limit & ~uint64_t(1);
+    const uint64_t st = std::max<uint64_t>(start_even + (start_even & 1), 4);
+    uint64_t span = std::max<uint64_t>(segment_even + (segment_even & 1), 2);
+    if (st <= lim) span = std::min<uint64_t>(span, std::max<uint64_t>((((lim - st) / 2 + 1) & ~uint64_t(1)), 4));
+    return std::max<uint64_t>(span, 4);
+}
+}  // namespace
+
 RunStats run_gpu(uint64_t limit, uint64_t segment_even, const std::string& spv_path, bool verbose, uint64_t start_even,
                  uint32_t sieve_split_arg, const char* prime_cache_path, bool collect_segments)
 {
@@ -544,12 +668,7 @@ RunStats run_gpu(uint64_t limit, uint64_t segment_even, const std::string& spv_p
     refuse_removed_knobs();
     if (limit < 4) throw std::runtime_error("limit < 4: no even n >= 4 to verify");
     require_supported_limit(limit);
-    // the engine is sized for this run: the span the range resolution keeps
-    const uint64_t lim = limit & ~uint64_t(1);
-    const uint64_t st = std::max<uint64_t>(start_even + (start_even & 1), 4);
-    uint64_t span = std::max<uint64_t>(segment_even + (segment_even & 1), 2);
-    if (st <= lim) span = std::min<uint64_t>(span, std::max<uint64_t>((((lim - st) / 2 + 1) & ~uint64_t(1)), 4));
-    Engine e(lim, std::max<uint64_t>(span, 4));
+    Engine e(limit & ~uint64_t(1), resolved_span(limit, start_even, segment_even));
     RunStats s = e.run(limit, segment_even, verbose, start_even, sieve_split_arg, collect_segments);
     s.init_seconds += e.init_seconds();
     s.total_seconds = s.init_seconds + s.gpu_seconds;
@@ -563,8 +682,8 @@ void print_device_info()
     gbh_device_info_t i{};
     check(gbh_device_info(c.c, &i), "gbh_device_info");
     std::printf("device 0: %s (%s)\n  compute units : %d\n  LDS per block : %zu bytes\n  HBM           : %.1f GB\n"
-                "  clock         : %d MHz\n  window default: %u KB LDS, window primes < %llu, phase A %u candidates\n",
-                i.name, i.arch, i.cus, i.lds_per_block, double(i.hbm_bytes) / 1e9, i.clock_khz / 1000, WIN_KB_DEFAULT,
+                "  clock         : %d MHz\n  window primes < %llu, phase A %u candidates\n",
+                i.name, i.arch, i.cus, i.lds_per_block, double(i.hbm_bytes) / 1e9, i.clock_khz / 1000,
                 (unsigned long long)WIN_P_DEFAULT, KA_DEFAULT);
 }
 
@@ -594,38 +713,9 @@ uint32_t gpu_segment_map(uint64_t seg_start, uint64_t limit, uint64_t span, std:
                          uint64_t Q[8], uint32_t nc[8])
 {
     if (seg_start & 1 || limit & 1 || seg_start < 4 || seg_start > limit) throw std::runtime_error("bad segment");
-    const SegmentGeometry sg = segment_geometry(seg_start, limit, span);
-    const ClassGeo cg = class_geometry(sg.q_first, sg.q_odds);
-    const uint32_t kb = env::u32("GB_WIN_KB", WIN_KB_DEFAULT, 4, 128);
-    const gbh_geom g = make_geom(sg, cg, kb * 8192);
-    const uint32_t bound = std::max<uint32_t>(base_prime_bound(limit), 64);
-    const std::vector<uint32_t> base = base_primes(bound, host_threads());
-    const uint32_t lo = uint32_t(std::lower_bound(base.begin(), base.end(), 67u) - base.begin());
-    const uint64_t win_p = env::u64("GB_WIN_P", WIN_P_DEFAULT, 67, UINT32_MAX);
-    const uint32_t hi = std::max(lo, uint32_t(std::lower_bound(base.begin(), base.end(), win_p) - base.begin()));
-    Ctx c;
-    Buf dw(c.c, size_t(hi - lo) * 4, 0), dl(c.c, (base.size() - hi) * 4, 0), di0(c.c, size_t(hi - lo) * 32, 0);
-    Buf dm(c.c, size_t(8) * g.cw * 8, 0);
-    check(gbh_zero(c.c, 0, dm.p, size_t(8) * g.cw * 8), "zero");
-    if (hi > lo) check(gbh_copy_h2d(c.c, 0, dw.p, base.data() + lo, size_t(hi - lo) * 4), "h2d");
-    if (base.size() > hi) check(gbh_copy_h2d(c.c, 0, dl.p, base.data() + hi, (base.size() - hi) * 4), "h2d");
-    uint32_t ncoop = 0;
-    while (lo + ncoop < hi && base[lo + ncoop] < g.wbits / 128) ++ncoop;
-    if (hi > lo) {
-        gbh_setup_args a{&g, dw.as<uint32_t>(), hi - lo, di0.as<uint32_t>()};
-        check(gbh_launch_setup(c.c, 0, &a), "setup");
-    }
-    gbh_window_args w{&g, dw.as<uint32_t>(), hi - lo, ncoop, di0.as<uint32_t>(), dm.as<uint64_t>()};
-    check(gbh_launch_window(c.c, 0, &w), "window");
-    if (base.size() > hi) {
-        gbh_large_args l{&g, dl.as<uint32_t>(), uint32_t(base.size() - hi), 0, dm.as<uint64_t>()};
-        check(gbh_launch_large(c.c, 0, &l), "large");
-    }
-    words.assign(size_t(8) * g.cw, 0);
-    check(gbh_copy_d2h(c.c, 0, words.data(), dm.p, words.size() * 8), "d2h");
-    check(gbh_stream_sync(c.c, 0), "sync");
-    for (int k = 0; k < 8; ++k) { Q[k] = g.Q[k]; nc[k] = g.nc[k]; }
-    return g.cw;
+    Engine e(limit, std::max<uint64_t>(span, 4));
+    DenseRun r(*e.state(), limit, span, false, seg_start, 0, false);
+    return r.map_of(seg_start, words, Q, nc);
 }
 
 }  // namespace gb

@@ -74,6 +74,7 @@ public:
     void init_times(double& cache, double& module, double& upload) const;
     uint64_t max_limit() const;
     struct State;
+    State* state() { return s_.get(); }
 
 private:
     std::unique_ptr<State> s_;

@@ -9,6 +9,11 @@
  * argument struct instead of set_arg calls; all pointers are device
  * pointers obtained from gbh_alloc, sizes are in bytes unless stated.
  *
+ * Work is issued per BATCH of consecutive segments (up to a few dozen):
+ * every stage covers all segments of the batch in one launch, the batch's
+ * class maps being resident in HBM side by side. (The reference launches
+ * every stage once per segment; its hit-list harvest alone spans a batch.)
+ *
  * Error convention: every function returns 0 on success and a non-zero
  * GBH_E_* code on failure; gbh_last_error() then holds the message (the
  * ZE_CHECK text of l0_context.hpp:14-20 becomes this string). Callers that
@@ -46,7 +51,9 @@ enum {
 #define GBH_GAP_TAIL 700         /* tail gaps (GAPS30_TAIL, gstats.hpp:74) */
 #define GBH_GAP_TCAP 4096        /* tail entries per segment (GAPS30_TCAP, gstats.hpp:114) */
 #define GBH_LEFT_CAP 65536       /* leftovers per segment copied back */
-#define GBH_CHECK_WG 256         /* check_a work-group: 256 lanes x 64 evens of one class */
+#define GBH_CHECK_WG 256         /* check_a work-group: 256 lanes x 64 evens */
+#define GBH_KA_MAX 128           /* phase-A candidates per even class, at most */
+#define GBH_MAX_BATCH 64         /* segments per batch, at most */
 
 typedef struct gbh_ctx gbh_ctx;
 typedef struct gbh_event gbh_event;
@@ -57,47 +64,37 @@ typedef struct {
     uint64_t seg_start, seg_end;   /* first / last even of the segment */
     uint64_t q_first, q_last;      /* odd q range sieved */
     uint64_t Q[GBH_CLASSES];       /* first q of each class array */
+    uint64_t n_gpu;                /* first even the checker takes (smaller ones: CPU) */
     uint32_t nc[GBH_CLASSES];      /* bits of each class array */
-    uint32_t cw;                   /* class array stride, 64-bit words */
-    uint32_t wbits;                /* LDS window bits (power of two, 2^15..2^20) */
-    uint32_t nwin;                 /* windows per class array */
-    uint32_t pad0;
+    uint32_t nwin;                 /* windows of this segment's class arrays */
+    uint32_t hist_on;              /* record p(n) of every even (a sampled segment) */
     uint64_t restore[GBH_CLASSES]; /* bits of word 0 set again after the mask stage (q = p <= 61) */
     uint32_t s0[GBH_CLASSES][16];  /* mask primes: class bit of every multiple, residue mod p */
 } gbh_geom;
 
-/* Stage 1: first class bit of every window prime (reference mid_setup,
- * goldbach.cl:3528-3563, and the host's first_bits_class_range,
- * host_math.cpp:80-113). */
+/* Counter parameters of one segment (count30_limits, gstats.hpp:262-296):
+ * the eight class arrays in ascending phase (position order inside a block
+ * of 30), each with its class index, the position of bit 0 (q - q_first)
+ * and the counted bits [lo, hi); the three twin pairs (11,13) (17,19)
+ * (29,31) with the lower / partner class, the partner's bit offset (0 or
+ * 1) and the counted bits of the lower class. */
 typedef struct {
-    const gbh_geom* geom;          /* host pointer: copied into the launch */
-    const uint32_t* primes;        /* window primes (device), ascending, all >= 67 */
-    uint32_t nprimes;
-    uint32_t* i0;                  /* [8][nprimes] first bit, ~0u: none in the segment */
-} gbh_setup_args;
+    uint32_t cls[GBH_CLASSES], pos0[GBH_CLASSES], lo[GBH_CLASSES], hi[GBH_CLASSES];
+    uint32_t tk[3], tk2[3], td[3], tlo[3], thi[3];
+    uint32_t nwords;               /* words walked per class array */
+    uint32_t on;                   /* 0: no counting for this segment */
+} gbh_countseg;
 
-/* Stage 2: LDS window sieve of every class array (reference make_prime_map +
- * mid_window, goldbach.cl:1092-1608, 3600-3701): masks for 7..61, marks of
- * the window primes, coalesced store of the window. */
+/* A batch: nseg consecutive segments, their class maps side by side
+ * (map + s * 8 * cw + c * cw), one LDS window size. */
 typedef struct {
-    const gbh_geom* geom;
-    const uint32_t* primes;        /* window primes (device) */
-    uint32_t nprimes;
-    uint32_t ncoop;                /* primes [0, ncoop) are marked by the whole work-group */
-    const uint32_t* i0;            /* stage-1 table */
-    uint64_t* map;                 /* [8][cw] words */
-} gbh_window_args;
-
-/* Stage 3: direct marks of the large primes (p >= the window bound) with
- * global atomics (reference sieve_large_packed + hit lists,
- * goldbach.cl:1738-1849, 3434-3501, here without the batch lists). */
-typedef struct {
-    const gbh_geom* geom;
-    const uint32_t* primes;        /* large primes (device), ascending */
-    uint32_t nprimes;
-    uint32_t pad0;
+    const gbh_geom* geoms;         /* device array [nseg] */
+    uint32_t nseg;
+    uint32_t cw;                   /* class array stride, 64-bit words (>= nwin * wbits / 64 + 16) */
+    uint32_t wbits;                /* LDS window bits (power of two, 2^12..2^20) */
+    uint32_t nwin;                 /* windows per class array, the batch's maximum */
     uint64_t* map;
-} gbh_large_args;
+} gbh_batch;
 
 /* Per-segment outputs of the checker and the counter, copied back as one
  * block (reference: the slot's shared USM counters, verifier.cpp:42-87). */
@@ -105,7 +102,7 @@ typedef struct {
     uint32_t acct[2];              /* evens resolved by phase A / phase B */
     uint32_t pending_n;            /* evens phase A left */
     uint32_t left_n;               /* evens no candidate resolved (CPU fallback) */
-    uint32_t err;                  /* error flags (1: pending overflow, 2: tail-list overflow) */
+    uint32_t err;                  /* error flags (1: hit-list overflow) */
     uint32_t cnt[GBH_CLASSES];     /* primes per class array in the counting range */
     uint32_t twins[3];             /* twin pairs (11,13) (17,19) (29,31) */
     uint32_t pfirst, plast;        /* first / last counted prime: position q - q_first (~0u / 0: none) */
@@ -120,34 +117,72 @@ typedef struct {
     uint32_t left[GBH_LEFT_CAP];   /* even indices of the leftovers */
 } gbh_segout;
 
-/* Stage 4: checker phase A (reference check30_a, goldbach.cl:2028-2189):
- * exact ascending candidates, 64 evens per lane, for the first ka candidates
- * of every even class; histogram of the resolving candidate when hist_on. */
+/* Stage 1: first class bit of every window prime in every segment
+ * (reference mid_setup, goldbach.cl:3528-3563, and the host's
+ * first_bits_class_range, host_math.cpp:80-113). */
 typedef struct {
-    const gbh_geom* geom;
-    const uint64_t* map;
-    uint64_t n_gpu;                /* first even the GPU checks (smaller ones: CPU) */
-    uint32_t ka;                   /* phase-A candidates per even class (<= 128) */
-    uint32_t hist_on;              /* record p(n) of every even (sampled segment) */
-    uint32_t* coff;                /* [15][GBH_NCAND] per-segment candidate table (written here) */
-    uint32_t* pending;             /* even indices left for phase B */
-    uint32_t pending_cap;
+    gbh_batch b;
+    const uint32_t* primes;        /* window primes (device), ascending, all >= 67 */
+    uint32_t nprimes;
     uint32_t pad0;
-    gbh_segout* out;
+    uint32_t* i0;                  /* [nseg][8][nprimes] first bit, ~0u: none in the segment */
+} gbh_setup_args;
+
+/* Stage 2: hit lists of the large primes for the whole batch (reference
+ * hl_harvest / hl_harvest_mt, goldbach.cl:3065-3426): every multiple p m in
+ * the batch's q range with m coprime to 2 3 5 7 11 13 (the other multiples
+ * are cleared by the mask stage), filed as a window offset into the bucket
+ * of its (segment, class, window). Work-group g takes the primes
+ * [chunks[g], chunks[g + 1]). */
+typedef struct {
+    gbh_batch b;
+    const uint32_t* primes;        /* large primes (device), ascending */
+    const uint32_t* chunks;        /* [nchunks + 1] prime index boundaries */
+    uint32_t nchunks;
+    uint32_t cap;                  /* entries per bucket */
+    uint32_t* hits;                /* [nseg][8][nwin][cap] window offsets */
+    uint32_t* hit_n;               /* [nseg][8][nwin] fill counts (zeroed before the launch) */
+    uint32_t* err;                 /* bit 0 set on a bucket overflow */
+} gbh_harvest_args;
+
+/* Stage 3: LDS window sieve of every class array of every segment
+ * (reference make_prime_map + mid_window + hl_apply, goldbach.cl:1092-1608,
+ * 3434-3457, 3600-3701): masks for 7..61, marks of the window primes, the
+ * window's hit list, one coalesced store. */
+typedef struct {
+    gbh_batch b;
+    const uint32_t* primes;        /* window primes (device) */
+    uint32_t nprimes;
+    uint32_t ncoop;                /* primes [0, ncoop) are marked by the whole work-group */
+    const uint32_t* i0;            /* stage-1 table */
+    const uint32_t* hits;          /* stage-2 lists (nullptr: no large primes) */
+    const uint32_t* hit_n;
+    uint32_t cap;
+    uint32_t pad0;
+} gbh_window_args;
+
+/* Stage 4: checker phase A (reference check30_a, goldbach.cl:2028-2189):
+ * exact ascending candidates, 64 evens per lane, the first ka candidates of
+ * every even class; the p(n) histogram on segments with hist_on. */
+typedef struct {
+    gbh_batch b;
+    uint32_t ka;                   /* phase-A candidates per even class (<= GBH_KA_MAX) */
+    uint32_t pcap;                 /* pending entries per segment */
+    uint32_t* coff;                /* [nseg][15][GBH_NCAND] per-segment candidate table (written here) */
+    uint32_t* pending;             /* [nseg][pcap] even indices left for phase B */
+    gbh_segout* out;               /* [nseg] */
 } gbh_check_args;
 
 /* Stage 5: phase B (reference check30_b, goldbach.cl:2210-2306): the
  * remaining candidates of every pending even, ascending; the first hit is
  * p(n). */
 typedef struct {
-    const gbh_geom* geom;
-    const uint64_t* map;
-    uint64_t n_gpu;
+    gbh_batch b;
     uint32_t ka;                   /* first candidate slot to probe */
     uint32_t tail_t;               /* first occurrences are tracked for p >= tail_t */
-    const uint32_t* coff;          /* the table check_a wrote */
+    const uint32_t* coff;
     const uint32_t* pending;
-    uint32_t pending_max;          /* grid bound: entries beyond out->pending_n are skipped */
+    uint32_t pcap;
     uint32_t pad0;
     gbh_segout* out;
 } gbh_checkb_args;
@@ -155,19 +190,11 @@ typedef struct {
 /* Stage 6: prime / twin counts and the gap walk of the counting range
  * (reference count30 + gap_tail_exact, goldbach.cl:2499-2825). */
 typedef struct {
-    const gbh_geom* geom;
-    const uint64_t* map;
-    /* the eight class arrays in ascending phase (position order inside a
-     * block of 30): class index, position of bit 0 (q - q_first), counted
-     * bits [lo, hi) (count30_limits, gstats.hpp:262-296) */
-    uint32_t cls[GBH_CLASSES], pos0[GBH_CLASSES], lo[GBH_CLASSES], hi[GBH_CLASSES];
-    /* twin pairs (11,13) (17,19) (29,31): lower class, partner class,
-     * partner word offset (0 or 1 bit), counted bits [lo, hi) of the lower */
-    uint32_t tk[3], tk2[3], td[3], tlo[3], thi[3];
-    uint32_t nwords;               /* words walked per class array */
-    uint32_t* scratch;             /* >= 2 * nwords u32: per-word first / last prime positions */
-    uint32_t scratch_words;
-    uint32_t pad0;
+    gbh_batch b;
+    const gbh_countseg* cs;        /* device array [nseg] */
+    uint32_t nwords;               /* the batch's largest nwords */
+    uint32_t swords;               /* scratch u32 per segment (>= 2 * nwords) */
+    uint32_t* scratch;             /* [nseg][swords] per-word first / last prime positions */
     gbh_segout* out;
 } gbh_count_args;
 
@@ -194,6 +221,9 @@ int gbh_copy_h2d(gbh_ctx* ctx, int stream, void* dst, const void* src, size_t by
 int gbh_copy_d2h(gbh_ctx* ctx, int stream, void* dst, const void* src, size_t bytes);
 int gbh_zero(gbh_ctx* ctx, int stream, void* dst, size_t bytes);  /* fill kernel (graph-safe) */
 int gbh_fill32(gbh_ctx* ctx, int stream, void* dst, size_t count, uint32_t value);
+/* out[s] of every segment of a batch reset for a new run: zeros, with
+ * first[] and pfirst set to ~0u */
+int gbh_reset_outputs(gbh_ctx* ctx, int stream, gbh_segout* out, uint32_t nseg);
 int gbh_stream_sync(gbh_ctx* ctx, int stream);
 int gbh_event_create(gbh_ctx* ctx, gbh_event** ev);
 int gbh_event_destroy(gbh_ctx* ctx, gbh_event* ev);
@@ -203,8 +233,8 @@ int gbh_event_elapsed_ms(gbh_ctx* ctx, gbh_event* a, gbh_event* b, float* ms);
 
 /* ---- stage launches (all asynchronous on `stream`) ---- */
 int gbh_launch_setup(gbh_ctx* ctx, int stream, const gbh_setup_args* a);
+int gbh_launch_harvest(gbh_ctx* ctx, int stream, const gbh_harvest_args* a);
 int gbh_launch_window(gbh_ctx* ctx, int stream, const gbh_window_args* a);
-int gbh_launch_large(gbh_ctx* ctx, int stream, const gbh_large_args* a);
 int gbh_launch_check_a(gbh_ctx* ctx, int stream, const gbh_check_args* a);
 int gbh_launch_check_b(gbh_ctx* ctx, int stream, const gbh_checkb_args* a);
 int gbh_launch_count(gbh_ctx* ctx, int stream, const gbh_count_args* a);

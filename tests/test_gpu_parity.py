@@ -125,5 +125,5 @@ def test_state_merge_and_overlap_refusal(stats_env):
     gbamd.run_gpu(3_000_000, 500_000, start_even=2_000_002)
     lines = (stats_env / "st.state").read_text().splitlines()
     assert [l for l in lines if l.startswith("range ")] == ["range 1000000 3000000"]
-    # 500001 + 499500 evens; segment 0 of each run sampled: 250000 + 249500
-    assert [l for l in lines if l.startswith("evens ")] == ["evens 999501 499500"]
+    # 500001 + 500000 evens; segment 0 of each run sampled: 250000 + 250000
+    assert [l for l in lines if l.startswith("evens ")] == ["evens 1000001 500000"]
