@@ -5,6 +5,7 @@
 // compiled into this translation unit so the library is one code object.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -25,6 +26,7 @@ struct gbh_event {
 namespace {
 
 thread_local std::string g_err;
+uint32_t g_min_useful = 0;   // fewest useful candidates of any even class
 
 int fail(int code, const std::string& what)
 {
@@ -77,6 +79,7 @@ int upload_candidates()
     GBH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(gbh_dev::c_cand_pi), cpi.data(), cpi.size() * 2));
     GBH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(gbh_dev::c_cand_c), cc.data(), cc.size()));
     GBH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(gbh_dev::c_cand_n), cn, sizeof cn));
+    g_min_useful = *std::min_element(cn, cn + GBH_EVEN_CLASSES);
     return GBH_OK;
 }
 
@@ -269,10 +272,14 @@ int gbh_launch_harvest(gbh_ctx* c, int s, const gbh_harvest_args* a)
 {
     if (const int rc = check_batch(a->b)) return rc;
     if (!a->nchunks) return GBH_OK;
-    const uint64_t nb = uint64_t(a->b.nseg) * 8 * a->b.nwin;
-    if (nb > gbh_dev::kMaxBuckets) return fail(GBH_E_ARG, "harvest: more buckets than the LDS counters hold");
-    hipLaunchKernelGGL(gbh_dev::k_harvest, dim3(a->nchunks), dim3(256), size_t(2 * nb * 4), S(c, s), a->b, a->primes,
-                       a->chunks, a->cap, a->hits, a->hit_n, a->err);
+    if (a->b.nwin > gbh_dev::kMaxWin) return fail(GBH_E_ARG, "harvest: more windows per class than the split holds");
+    if (uint64_t(a->b.nwin) * a->cap > a->capsc) return fail(GBH_E_ARG, "harvest: class lists smaller than the windows'");
+    hipLaunchKernelGGL(gbh_dev::k_harvest, dim3(a->nchunks), dim3(256), 0, S(c, s), a->b, a->primes, a->chunks,
+                       a->capsc, a->hits_sc, a->hit_n_sc, a->err);
+    GBH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(gbh_dev::k_split, dim3((a->capsc + gbh_dev::kSplitChunk - 1) / gbh_dev::kSplitChunk,
+                       a->b.nseg * 8), dim3(256), 0, S(c, s), a->b, a->hits_sc, a->hit_n_sc, a->capsc, a->hits,
+                       a->hit_n, a->cap, a->err);
     GBH_HIP(hipGetLastError());
     return GBH_OK;
 }
@@ -291,16 +298,27 @@ int gbh_launch_window(gbh_ctx* c, int s, const gbh_window_args* a)
 int gbh_launch_check_a(gbh_ctx* c, int s, const gbh_check_args* a)
 {
     if (const int rc = check_batch(a->b)) return rc;
-    if (a->ka > GBH_KA_MAX) return fail(GBH_E_ARG, "ka above GBH_KA_MAX");
+    if (a->ka > GBH_KA_MAX || a->ka % 4 || a->ka > g_min_useful)
+        return fail(GBH_E_ARG, "ka must be a multiple of 4, at most GBH_KA_MAX and every class's useful candidates");
     // every class array holds nc <= nwin * wbits bits; the checker's words
     // stay below that (host contract: n_gpu >= q_first + 9780)
     const uint32_t words = uint32_t((uint64_t(a->b.nwin) * a->b.wbits + 63) / 64);
-    hipLaunchKernelGGL(gbh_dev::k_coff, dim3((GBH_NCAND + 255) / 256, GBH_EVEN_CLASSES, a->b.nseg), dim3(256), 0,
-                       S(c, s), a->b, a->coff);
+    hipLaunchKernelGGL(gbh_dev::k_coff, dim3((GBH_NCAND_PAD + 255) / 256, GBH_EVEN_CLASSES, a->b.nseg), dim3(256),
+                       0, S(c, s), a->b, a->coff, a->coffa);
     GBH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(gbh_dev::k_check_a, dim3(grid_of(words, GBH_CHECK_WG, 1u << 30), 3, a->b.nseg),
-                       dim3(GBH_CHECK_WG), 0, S(c, s), a->b, a->coff, a->ka, a->pending, a->pcap, a->out);
-    GBH_HIP(hipGetLastError());
+    // sampled segments (histogram on) and the others: one launch each, every
+    // work-group leaves at once on the other kind of segment
+    const dim3 grid(grid_of(words, GBH_CHECK_WG, 1u << 30), 3, a->b.nseg);
+    if (a->kinds & 1u) {
+        hipLaunchKernelGGL(gbh_dev::k_check_a<false>, grid, dim3(GBH_CHECK_WG), 0, S(c, s), a->b, a->coffa, a->ka,
+                           a->pending, a->pcap, a->out);
+        GBH_HIP(hipGetLastError());
+    }
+    if (a->kinds & 2u) {
+        hipLaunchKernelGGL(gbh_dev::k_check_a<true>, grid, dim3(GBH_CHECK_WG), 0, S(c, s), a->b, a->coffa, a->ka,
+                           a->pending, a->pcap, a->out);
+        GBH_HIP(hipGetLastError());
+    }
     return GBH_OK;
 }
 

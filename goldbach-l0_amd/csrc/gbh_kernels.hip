@@ -102,38 +102,52 @@ __global__ void __launch_bounds__(256) k_setup(gbh_batch b, const uint32_t* __re
 }
 
 // ---------------------------------------------------------------- stage 2
-constexpr uint32_t kMaxBuckets = 8192;
+constexpr uint32_t kRep = 8;     // bucket counters replicated by lane & 7 (fewer same-address LDS atomics)
 
+// Positions inside a batch are half-offsets d = (q - qb0) / 2 < 2^32 (the
+// host keeps a batch's q range below 2^33), so the per-hit work is 32-bit.
 struct HarvestCtx {
-    uint64_t qb0, qb1;       // the batch's q range
-    double inv_span;         // 1 / (q_first[s + 1] - q_first[s])
-    uint32_t nseg, nwin, lw, wmask;
+    uint64_t qb0;            // the batch's first q
+    uint32_t dmax;           // half-offset of the batch's last q
+    uint32_t hs_magic;       // floor(2^32 / half span): estimates d / half span from below
+    uint32_t nseg;
 };
 
-// Files hit q of class c into the bucket of every segment whose q range
-// holds it (consecutive segments overlap by 9776 in q).
+// Files the multiple at half-offset d of class c into the (segment, class)
+// list of every segment whose q range holds it (consecutive segments overlap
+// by 9776 in q); the entry is the multiple's class bit.
 template <bool kWrite>
-__device__ __forceinline__ void file_hit(const HarvestCtx& h, const uint64_t* __restrict__ qf,
-                                         const uint64_t* __restrict__ ql, const uint64_t* __restrict__ Q, uint64_t q,
-                                         uint32_t c, uint32_t* cnt, const uint32_t* base, uint32_t* __restrict__ hits,
-                                         uint32_t cap, uint32_t* __restrict__ err)
+__device__ __forceinline__ void file_one(uint32_t k, uint32_t c, uint32_t d, const uint32_t* dq, uint32_t lane8,
+                                         uint32_t* cnt, const uint32_t* base, uint32_t* __restrict__ hits,
+                                         uint32_t capsc, uint32_t* __restrict__ err)
 {
-    int s = (int)((double)(q - h.qb0) * h.inv_span);
-    if (s >= (int)h.nseg) s = h.nseg - 1;
-    while (s > 0 && q < qf[s]) --s;
-    while (s + 1 < (int)h.nseg && q >= qf[s + 1]) ++s;
-    for (int k = s; k >= 0 && k >= s - 1; --k) {
-        if (q < qf[k] || q > ql[k]) continue;
-        const uint64_t t = (q - Q[k * 8 + c]) / 30;
-        const uint32_t bucket = ((uint32_t)k * 8 + c) * h.nwin + (uint32_t)(t >> h.lw);
-        if (!kWrite) {
-            atomicAdd(&cnt[bucket], 1u);
-        } else {
-            const uint32_t pos = base[bucket] + atomicAdd(&cnt[bucket], 1u);
-            if (pos < cap) hits[(size_t)bucket * cap + pos] = (uint32_t)t & h.wmask;
-            else atomicOr(err, 1u);
-        }
+    const uint32_t t = __umulhi(d - dq[k * 8 + c], 0x88888889u) >> 3;   // (q - Q) / 30 = (d - dq) / 15
+    const uint32_t b = k * 8 + c;
+    const uint32_t slot = b * kRep + lane8;
+    if (!kWrite) {
+        atomicAdd(&cnt[slot], 1u);
+    } else {
+        const uint32_t pos = base[slot] + atomicAdd(&cnt[slot], 1u);
+        if (pos < capsc) hits[(size_t)b * capsc + pos] = t;
+        else atomicOr(err, 1u);
     }
+}
+
+template <bool kWrite>
+__device__ __forceinline__ void file_hit(const HarvestCtx& h, const uint32_t* df, const uint32_t* dl,
+                                         const uint32_t* dq, uint32_t d, uint32_t c, uint32_t lane8, uint32_t* cnt,
+                                         const uint32_t* base, uint32_t* __restrict__ hits, uint32_t capsc,
+                                         uint32_t* __restrict__ err)
+{
+    // the segment starting at or below d: an estimate at most two below,
+    // corrected without branches
+    uint32_t s = min(__umulhi(d, h.hs_magic), h.nseg - 1);
+    s += (s + 1 < h.nseg && d >= df[min(s + 1, h.nseg - 1)]);
+    s += (s + 1 < h.nseg && d >= df[min(s + 1, h.nseg - 1)]);
+    while (s + 1 < h.nseg && d >= df[s + 1]) ++s;     // irregular spacing only (a run's first segment below 9783)
+    while (s > 0 && d < df[s]) --s;
+    if (d <= dl[s]) file_one<kWrite>(s, c, d, dq, lane8, cnt, base, hits, capsc, err);
+    if (s > 0 && d <= dl[s - 1]) file_one<kWrite>(s - 1, c, d, dq, lane8, cnt, base, hits, capsc, err);   // overlap
 }
 
 // ceil(x / p) for p >= 2^12: a double-precision quotient (off by at most
@@ -154,31 +168,37 @@ __device__ __forceinline__ uint64_t ceil_div_big(uint64_t x, double xd, uint32_t
 // (a multiple of 2 3 5 7 11 13 is not in the class map or is cleared by the
 // mask stage): m walks the 48 residues coprime to 210, 11 and 13 are
 // tested by running residues, the class of q follows from m mod 30.
+// LDS copies of the wheel tables (per-lane lookups stay out of memory)
+struct WheelLds {
+    uint8_t next[210], idx[210], inc[48], cls[30];
+};
+
 template <bool kWrite>
-__device__ __forceinline__ void harvest_prime(const HarvestCtx& h, double qb0d, uint32_t p, const uint64_t* qf,
-                                              const uint64_t* ql, const uint64_t* Q, uint32_t* cnt,
-                                              const uint32_t* base, uint32_t* __restrict__ hits, uint32_t cap,
-                                              uint32_t* __restrict__ err)
+__device__ __forceinline__ void harvest_prime(const HarvestCtx& h, double qb0d, uint32_t p, const uint32_t* df,
+                                              const uint32_t* dl, const uint32_t* dq, const WheelLds& wl,
+                                              uint32_t lane8, uint32_t* cnt, const uint32_t* base,
+                                              uint32_t* __restrict__ hits, uint32_t capsc, uint32_t* __restrict__ err)
 {
     uint64_t m = ceil_div_big(h.qb0, qb0d, p);
     if (m < p) m = p;
-    uint64_t q = m * p;
-    if (q > h.qb1) return;
     const uint32_t r = (uint32_t)(m % 30030);
-    uint32_t r210 = r % 210;
-    const uint32_t d0 = c_w210_next[r210];
+    const uint32_t r210 = r % 210;
+    const uint32_t d0 = wl.next[r210];
+    m += d0;                                   // m odd from here on: q - qb0 is even
+    const uint64_t dd = (m * p - h.qb0) >> 1;
+    if (dd > h.dmax) return;
+    uint32_t d = (uint32_t)dd;
     uint32_t r11 = (r + d0) % 11, r13 = (r + d0) % 13, r30 = (r + d0) % 30;
-    uint32_t idx = c_w210_idx[(r210 + d0) % 210];
-    q += (uint64_t)d0 * p;
+    uint32_t idx = wl.idx[(r210 + d0) % 210];
     const uint32_t pr = p % 30;
-    while (q <= h.qb1) {
-        if (r11 && r13) {
-            const uint32_t c = c_cls_of[(r30 * pr) % 30];
-            file_hit<kWrite>(h, qf, ql, Q, q, c, cnt, base, hits, cap, err);
-        }
-        const uint32_t inc = c_w210_inc[idx];
+    for (;;) {
+        if (r11 && r13)
+            file_hit<kWrite>(h, df, dl, dq, d, wl.cls[(r30 * pr) % 30], lane8, cnt, base, hits, capsc, err);
+        const uint32_t inc = wl.inc[idx];       // even: the walk stays on odd m
         idx = idx == 47 ? 0 : idx + 1;
-        q += (uint64_t)inc * p;
+        const uint64_t step = (uint64_t)(inc >> 1) * p;
+        if (step > h.dmax - d) break;
+        d += (uint32_t)step;
         r11 += inc; if (r11 >= 11) r11 -= 11;
         r13 += inc; if (r13 >= 13) r13 -= 13;
         r30 += inc; if (r30 >= 30) r30 -= 30;
@@ -186,41 +206,97 @@ __device__ __forceinline__ void harvest_prime(const HarvestCtx& h, double qb0d, 
 }
 
 __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __restrict__ primes,
-                                                 const uint32_t* __restrict__ chunks, uint32_t cap,
+                                                 const uint32_t* __restrict__ chunks, uint32_t capsc,
                                                  uint32_t* __restrict__ hits, uint32_t* __restrict__ hit_n,
                                                  uint32_t* __restrict__ err)
 {
-    extern __shared__ uint32_t lds[];
-    __shared__ uint64_t qf[GBH_MAX_BATCH], ql[GBH_MAX_BATCH], Q[GBH_MAX_BATCH * 8];
-    const uint32_t nb = b.nseg * 8 * b.nwin;
-    uint32_t* cnt = lds;
-    uint32_t* base = lds + nb;
-    const uint32_t tid = threadIdx.x;
-    for (uint32_t s = tid; s < b.nseg; s += 256) { qf[s] = b.geoms[s].q_first; ql[s] = b.geoms[s].q_last; }
-    for (uint32_t i = tid; i < b.nseg * 8; i += 256) Q[i] = b.geoms[i / 8].Q[i % 8];
+    __shared__ uint32_t df[GBH_MAX_BATCH], dl[GBH_MAX_BATCH], dq[GBH_MAX_BATCH * 8];
+    __shared__ uint32_t cnt[GBH_MAX_BATCH * 8 * kRep], base[GBH_MAX_BATCH * 8 * kRep];
+    __shared__ WheelLds wl;
+    const uint32_t nb = b.nseg * 8 * kRep;
+    const uint32_t tid = threadIdx.x, lane8 = tid & 7;
+    if (tid < 210) { wl.next[tid] = c_w210_next[tid]; wl.idx[tid] = c_w210_idx[tid]; }
+    if (tid < 48) wl.inc[tid] = c_w210_inc[tid];
+    if (tid < 30) wl.cls[tid] = c_cls_of[tid];
+    const uint64_t qb0 = b.geoms[0].q_first;
+    for (uint32_t s = tid; s < b.nseg; s += 256) {
+        df[s] = (uint32_t)((b.geoms[s].q_first - qb0) >> 1);
+        dl[s] = (uint32_t)((b.geoms[s].q_last - qb0) >> 1);
+    }
+    for (uint32_t i = tid; i < b.nseg * 8; i += 256) dq[i] = (uint32_t)((b.geoms[i / 8].Q[i % 8] - qb0) >> 1);
     for (uint32_t i = tid; i < nb; i += 256) cnt[i] = 0;
     __syncthreads();
     HarvestCtx h;
-    h.qb0 = qf[0];
-    h.qb1 = ql[b.nseg - 1];
-    h.inv_span = b.nseg > 1 ? 1.0 / (double)(qf[1] - qf[0]) : 0.0;
+    h.qb0 = qb0;
+    h.dmax = dl[b.nseg - 1];
+    h.hs_magic = (b.nseg > 1 && df[1] > df[0]) ? (uint32_t)((1ull << 32) / (df[1] - df[0] + 1)) : 0u;
     h.nseg = b.nseg;
-    h.nwin = b.nwin;
-    h.lw = 31 - __clz(b.wbits);
-    h.wmask = b.wbits - 1;
-    const double qb0d = (double)h.qb0;
+    const double qb0d = (double)qb0;
     const uint32_t j0 = chunks[blockIdx.x], j1 = chunks[blockIdx.x + 1];
     for (uint32_t j = j0 + tid; j < j1; j += 256)
-        harvest_prime<false>(h, qb0d, primes[j], qf, ql, Q, cnt, base, hits, cap, err);
+        harvest_prime<false>(h, qb0d, primes[j], df, dl, dq, wl, lane8, cnt, base, hits, capsc, err);
     __syncthreads();
     for (uint32_t i = tid; i < nb; i += 256) {
         const uint32_t c = cnt[i];
-        base[i] = c ? atomicAdd(&hit_n[i], c) : 0u;
+        base[i] = c ? atomicAdd(&hit_n[i / kRep], c) : 0u;
         cnt[i] = 0;
     }
     __syncthreads();
     for (uint32_t j = j0 + tid; j < j1; j += 256)
-        harvest_prime<true>(h, qb0d, primes[j], qf, ql, Q, cnt, base, hits, cap, err);
+        harvest_prime<true>(h, qb0d, primes[j], df, dl, dq, wl, lane8, cnt, base, hits, capsc, err);
+}
+
+// Splits every (segment, class) list by window into the window lists the
+// window sieve reads, 4096 entries per work-group, sorted in LDS so each
+// window's part is written as one contiguous run.
+constexpr uint32_t kSplitChunk = 4096;
+constexpr uint32_t kMaxWin = 256;
+
+__global__ void __launch_bounds__(256) k_split(gbh_batch b, const uint32_t* __restrict__ hits_sc,
+                                               const uint32_t* __restrict__ hit_n_sc, uint32_t capsc,
+                                               uint32_t* __restrict__ hits_w, uint32_t* __restrict__ hit_n_w,
+                                               uint32_t cap, uint32_t* __restrict__ err)
+{
+    __shared__ uint32_t stage[kSplitChunk], sorted[kSplitChunk];
+    __shared__ uint32_t wc[kMaxWin], ws[kMaxWin], wb[kMaxWin];
+    const uint32_t bsc = blockIdx.y, tid = threadIdx.x;
+    const uint32_t n = min(hit_n_sc[bsc], capsc);
+    const uint32_t i0 = blockIdx.x * kSplitChunk;
+    if (i0 >= n) return;
+    const uint32_t cnt = min(n - i0, kSplitChunk);
+    const uint32_t lw = 31 - __clz(b.wbits), wmask = b.wbits - 1, nwin = b.nwin;
+    for (uint32_t w = tid; w < nwin; w += 256) wc[w] = 0;
+    __syncthreads();
+    const uint32_t* src = hits_sc + (size_t)bsc * capsc + i0;
+    for (uint32_t i = tid; i < cnt; i += 256) {
+        const uint32_t t = src[i];
+        stage[i] = t;
+        atomicAdd(&wc[min(t >> lw, nwin - 1)], 1u);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t run = 0;
+        for (uint32_t w = 0; w < nwin; ++w) { ws[w] = run; run += wc[w]; }
+    }
+    __syncthreads();
+    for (uint32_t w = tid; w < nwin; w += 256) {
+        wb[w] = wc[w] ? atomicAdd(&hit_n_w[bsc * nwin + w], wc[w]) : 0u;
+        wc[w] = 0;
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < cnt; i += 256) {
+        const uint32_t t = stage[i];
+        const uint32_t w = min(t >> lw, nwin - 1);
+        sorted[ws[w] + atomicAdd(&wc[w], 1u)] = t;
+    }
+    __syncthreads();
+    for (uint32_t k = tid; k < cnt; k += 256) {
+        const uint32_t t = sorted[k];
+        const uint32_t w = min(t >> lw, nwin - 1);
+        const uint32_t pos = wb[w] + (k - ws[w]);
+        if (pos < cap) hits_w[((size_t)bsc * nwin + w) * cap + pos] = t & wmask;
+        else atomicOr(err, 1u);
+    }
 }
 
 // ---------------------------------------------------------------- stage 3
@@ -331,43 +407,63 @@ __device__ __forceinline__ ClassEvens class_evens(const gbh_geom& g, uint32_t cr
     return e;
 }
 
-// coff[(s * 15 + cr) * GBH_NCAND + j] = class << 16 | bit offset of candidate j of class cr
-__global__ void __launch_bounds__(256) k_coff(gbh_batch b, uint32_t* __restrict__ coff)
+constexpr uint32_t kStageW = GBH_CHECK_WG + 8;     // staged words per class array
+constexpr uint32_t kClassesPerWg = 5;              // even classes per work-group (grid.y = 3)
+constexpr uint32_t kZeroWord = GBH_CLASSES * kStageW;   // staged zero word (padding entries)
+
+// Per segment and even class cr, candidate j:
+//   coff[(s * 15 + cr) * GBH_NCAND + j] = class << 16 | bit offset (phase B)
+//   coffa[(s * 15 + cr) * GBH_NCAND_PAD + j] = staged word << 6 | shift (phase A)
+// where the bit offset is (n0 - p - Q[class]) / 30 for the class's first
+// checked even n0.
+__global__ void __launch_bounds__(256) k_coff(gbh_batch b, uint32_t* __restrict__ coff, uint32_t* __restrict__ coffa)
 {
     const uint32_t cr = blockIdx.y, s = blockIdx.z;
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= c_cand_n[cr]) return;
+    if (j >= GBH_NCAND_PAD) return;
     const gbh_geom& g = b.geoms[s];
     if (g.n_gpu > g.seg_end) return;
+    uint32_t* pa = coffa + ((size_t)s * 15 + cr) * GBH_NCAND_PAD;
+    if (j >= c_cand_n[cr]) {
+        pa[j] = kZeroWord << 6;
+        return;
+    }
     const ClassEvens e = class_evens(g, cr);
     const uint32_t p = c_cand_p[cr * GBH_NCAND + j];
     const uint32_t c = c_cand_c[cr * GBH_NCAND + j];
-    const uint64_t d = e.n0 - p - g.Q[c];          // >= 0 and a multiple of 30 (host contract on n_gpu)
-    coff[((size_t)s * 15 + cr) * GBH_NCAND + j] = (c << 16) | (uint32_t)(d / 30);
+    const uint32_t off = (uint32_t)((e.n0 - p - g.Q[c]) / 30);   // exact and >= 0 (host contract on n_gpu)
+    coff[((size_t)s * 15 + cr) * GBH_NCAND + j] = (c << 16) | off;
+    pa[j] = ((c * kStageW + (off >> 6)) << 6) | (off & 63);
 }
 
-constexpr uint32_t kStageW = GBH_CHECK_WG + 8;     // staged words per class array
-constexpr uint32_t kClassesPerWg = 5;              // even classes per work-group (grid.y = 3)
+// 64 class-array bits starting at staged word xw, bit sh (branch free:
+// (hi << 1) << (63 - sh) is hi << (64 - sh) for sh >= 1 and 0 for sh = 0)
+__device__ __forceinline__ uint64_t window64(const uint64_t* st, uint32_t xw, uint32_t sh)
+{
+    const uint64_t lo = st[xw], hi = st[xw + 1];
+    return (lo >> sh) | ((hi << 1) << (63 - sh));
+}
 
-__global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_batch b, const uint32_t* __restrict__ coff, uint32_t ka,
-                                                          uint32_t* __restrict__ pending, uint32_t pcap,
+template <bool kHist>
+__global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_batch b, const uint32_t* __restrict__ coffa,
+                                                          uint32_t ka, uint32_t* __restrict__ pending, uint32_t pcap,
                                                           gbh_segout* __restrict__ outs)
 {
-    __shared__ uint64_t st[GBH_CLASSES * kStageW];
-    __shared__ uint32_t hl[kClassesPerWg * GBH_KA_MAX];
+    __shared__ uint64_t st[GBH_CLASSES * kStageW + GBH_CHECK_WG + 2];   // tail: zero words of padding entries
+    __shared__ uint32_t hl[kHist ? kClassesPerWg * GBH_KA_MAX : 1];
     const uint32_t tid = threadIdx.x, s = blockIdx.z;
     const gbh_geom& g = b.geoms[s];
-    if (g.n_gpu > g.seg_end) return;
+    if (g.n_gpu > g.seg_end || (g.hist_on != 0) != kHist) return;
     gbh_segout* out = outs + s;
     const uint64_t* map = seg_map(b, s);
-    const uint32_t hist_on = g.hist_on;
     const uint32_t w0 = blockIdx.x * GBH_CHECK_WG;
     for (uint32_t i = tid; i < GBH_CLASSES * kStageW; i += GBH_CHECK_WG) {
         const uint32_t c = i / kStageW, x = i % kStageW;
         const uint32_t w = w0 + x;
         st[i] = w < b.cw ? map[(size_t)c * b.cw + w] : 0ull;
     }
-    if (hist_on)
+    for (uint32_t i = tid; i < GBH_CHECK_WG + 2; i += GBH_CHECK_WG) st[kZeroWord + i] = 0;
+    if (kHist)
         for (uint32_t i = tid; i < kClassesPerWg * GBH_KA_MAX; i += GBH_CHECK_WG) hl[i] = 0;
     __syncthreads();
 
@@ -381,19 +477,19 @@ __global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_batch b, const uin
         uint64_t valid = 0;
         if (k0 < e.cnt) valid = (e.cnt - k0 >= 64) ? ~0ull : ((1ull << (e.cnt - k0)) - 1);
         uint64_t unres = valid;
-        const uint32_t nj = min(ka, c_cand_n[cr]);
-        const uint32_t* ct = coff + ((size_t)s * 15 + cr) * GBH_NCAND;
-        for (uint32_t j = 0; j < nj && unres; ++j) {
-            const uint32_t ent = ct[j];
-            const uint32_t c = ent >> 16, off = ent & 0xFFFF;
-            const uint32_t xw = c * kStageW + tid + (off >> 6);
-            const uint32_t sh = off & 63;
-            const uint64_t lo = st[xw], hi = st[xw + 1];
-            const uint64_t wv = sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
-            const uint64_t hit = wv & unres;
-            if (hit) {
-                unres &= ~hit;
-                if (hist_on) atomicAdd(&hl[k * GBH_KA_MAX + j], (uint32_t)__popcll(hit));
+        // ka is a multiple of 4 and at most every class's useful candidates
+        const uint4* ct = reinterpret_cast<const uint4*>(coffa + ((size_t)s * 15 + cr) * GBH_NCAND_PAD);
+        for (uint32_t j4 = 0; j4 < ka / 4 && unres; ++j4) {
+            const uint4 q4 = ct[j4];
+            const uint32_t ent[4] = {q4.x, q4.y, q4.z, q4.w};
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint64_t wv = window64(st, (ent[u] >> 6) + tid, ent[u] & 63);
+                if (kHist) {
+                    const uint64_t hit = wv & unres;
+                    if (hit) atomicAdd(&hl[k * GBH_KA_MAX + j4 * 4 + u], (uint32_t)__popcll(hit));
+                }
+                unres &= ~wv;
             }
         }
         resolved += (uint32_t)__popcll(valid & ~unres);
@@ -419,7 +515,7 @@ __global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_batch b, const uin
     }
     const uint32_t rs = wave_sum(resolved);
     if (wave_lane() == 0 && rs) atomicAdd(&out->acct[0], rs);
-    if (hist_on) {
+    if (kHist) {
         __syncthreads();
         for (uint32_t i = tid; i < kClassesPerWg * GBH_KA_MAX; i += GBH_CHECK_WG) {
             const uint32_t v = hl[i];

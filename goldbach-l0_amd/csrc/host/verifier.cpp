@@ -156,7 +156,8 @@ struct Engine::State {
     std::optional<Ctx> ctx;
     gbh_ctx* c = nullptr;
     int cus = 256;
-    Buf d_win, d_large, d_chunks, d_i0, d_map, d_coff, d_pending, d_scratch, d_hits, d_hit_n;
+    Buf d_win, d_large, d_chunks, d_i0, d_map, d_coff, d_coffa, d_pending, d_scratch, d_hits, d_hit_n, d_hits_sc,
+        d_hit_n_sc;
     Buf d_geoms, d_cs;
     Buf d_out[2], h_out[2], h_geoms[2], h_cs[2], h_chunks;
     Event ev[2][EV_N];
@@ -171,7 +172,8 @@ struct Engine::State {
         if (max_span > (uint64_t(1) << 35)) throw std::runtime_error("segment span above 2^35 integers");
         win_kb = env::u32("GB_WIN_KB", 0, 0, 128);
         if (win_kb & (win_kb - 1)) throw std::runtime_error("GB_WIN_KB must be a power of two");
-        ka = env::u32("GB_KA", KA_DEFAULT, 1, GBH_KA_MAX);
+        ka = env::u32("GB_KA", KA_DEFAULT, 4, GBH_KA_MAX);
+        if (ka % 4) throw std::runtime_error("GB_KA must be a multiple of 4");
         kbatch = env::u32("GB_BATCH", 16, 1, GBH_MAX_BATCH);
         // base primes (the reference loads them from its prime cache;
         // here a segmented sieve on the host threads)
@@ -206,6 +208,7 @@ struct Engine::State {
         d_geoms.ensure(c, sizeof(gbh_geom) * kbatch, 0);
         d_cs.ensure(c, sizeof(gbh_countseg) * kbatch, 0);
         d_coff.ensure(c, size_t(kbatch) * 15 * GBH_NCAND * 4, 0);
+        d_coffa.ensure(c, size_t(kbatch) * 15 * GBH_NCAND_PAD * 4, 0);
         check(gbh_stream_sync(c, 0), "upload sync");
         upload_s = since(tu);
         init_s = since(t0);
@@ -247,7 +250,7 @@ private:
     uint64_t total_evens_ = 0;
     clock_::time_point t_start_;
     uint32_t nwin_ = 0, nlarge_ = 0, split_ = 0, ncoop_ = 0;   // this run's window / large prime counts
-    uint32_t wbits_ = 0, nwin_max_ = 0, cw_ = 0, cap_ = 0, pcap_ = 0, swords_ = 0, nchunks_ = 0, kb_ = 1;
+    uint32_t wbits_ = 0, nwin_max_ = 0, cw_ = 0, cap_ = 0, capsc_ = 0, pcap_ = 0, swords_ = 0, nchunks_ = 0, kb_ = 1;
     bool stats_on_ = false, gaps_on_ = false, gaps_exact_ = false;
     uint32_t sample_ = 256;
     std::optional<GoldbachStats> gs_;
@@ -279,6 +282,9 @@ void DenseRun::resolve_range()
         span_ = std::max<uint64_t>(span_, 4);
     }
     if (span_ > E.max_span) throw std::runtime_error("segment span above the engine's buffers");
+    // a batch's positions are 32-bit half-offsets: one merged segment (below
+    // two spans) must stay below 2^33 integers
+    if (span_ >= (uint64_t(1) << 32)) throw std::runtime_error("segment span of 2^32 integers or more");
     // this run's primes: those <= sqrt(limit) + 16
     const uint32_t bound = std::max<uint32_t>(base_prime_bound(limit_), 64);
     const uint32_t nb = uint32_t(std::upper_bound(E.base.begin(), E.base.end(), bound) - E.base.begin());
@@ -335,11 +341,11 @@ void DenseRun::plan_buffers()
         while (wbits_ > (1u << 15) && nseg * 8 * ((nc_typ + wbits_ - 1) / wbits_) < 2ull * E.cus) wbits_ >>= 1;
     }
     nwin_max_ = uint32_t((nc_max + wbits_ - 1) / wbits_);
-    while (nseg * 8 * nwin_max_ > 8192 && wbits_ < (1u << 20)) {   // harvest LDS counters
+    while (nwin_max_ > 256 && wbits_ < (1u << 20)) {   // the window split's LDS tables
         wbits_ <<= 1;
         nwin_max_ = uint32_t((nc_max + wbits_ - 1) / wbits_);
     }
-    if (nseg * 8 * nwin_max_ > 8192) throw std::runtime_error("GB_BATCH too large for this span");
+    if (nwin_max_ > 256) throw std::runtime_error("segment span too large for the window split");
     cw_ = nwin_max_ * (wbits_ / 64) + 16;
     ncoop_ = 0;
     while (ncoop_ < nwin_ && E.base[E.win_lo + ncoop_] < wbits_ / 128) ++ncoop_;
@@ -350,14 +356,22 @@ void DenseRun::plan_buffers()
     pcap_ = uint32_t(std::min<uint64_t>(evens_max / 32 + 65536, UINT32_MAX / 2 / E.kbatch));
     swords_ = uint32_t(2 * (nc_max / 64 + 4));
     gbh_ctx* c = E.c;
-    kb_ = uint32_t(nseg);
+    // a batch's q range stays below 2^33 (the harvest's 32-bit half-offsets):
+    // at most kb_ - 1 spans plus the merged last segment (below 2 spans)
+    const uint64_t kmax = std::max<uint64_t>(1, ((uint64_t(1) << 33) - 4 * MAX_GB_PRIME) / span_ - 1);
+    kb_ = uint32_t(std::min<uint64_t>(nseg, kmax));
     const size_t K = kb_;
     E.d_map.ensure(c, K * 8 * cw_ * 8, 0);
     E.d_i0.ensure(c, K * 8 * size_t(std::max(nwin_, 1u)) * 4, 0);
     E.d_pending.ensure(c, K * size_t(pcap_) * 4, 0);
     E.d_scratch.ensure(c, K * size_t(swords_) * 4, 0);
     E.d_hit_n.ensure(c, K * 8 * size_t(nwin_max_) * 4, 0);
-    if (nlarge_) E.d_hits.ensure(c, K * 8 * size_t(nwin_max_) * cap_ * 4, 0);
+    E.d_hit_n_sc.ensure(c, K * 8 * 4, 0);
+    capsc_ = nwin_max_ * cap_;
+    if (nlarge_) {
+        E.d_hits.ensure(c, K * 8 * size_t(nwin_max_) * cap_ * 4, 0);
+        E.d_hits_sc.ensure(c, K * 8 * size_t(capsc_) * 4, 0);
+    }
     // harvest chunks: contiguous prime ranges with about the same expected
     // hits (sum of 1/p) and at most 64k primes each
     std::vector<uint32_t> ch{0};
@@ -424,8 +438,10 @@ void DenseRun::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     }
     if (nlarge_) {
         check(gbh_zero(c, 0, E.d_hit_n.p, size_t(nseg) * 8 * nwin * 4), "zero hit counts");
-        gbh_harvest_args h{b, E.d_large.as<uint32_t>(), E.d_chunks.as<uint32_t>(), nchunks_, cap_,
-                           E.d_hits.as<uint32_t>(), E.d_hit_n.as<uint32_t>(), &out[0].err};
+        check(gbh_zero(c, 0, E.d_hit_n_sc.p, size_t(nseg) * 8 * 4), "zero class hit counts");
+        gbh_harvest_args h{b, E.d_large.as<uint32_t>(), E.d_chunks.as<uint32_t>(), nchunks_, cap_, capsc_, 0,
+                           E.d_hits_sc.as<uint32_t>(), E.d_hit_n_sc.as<uint32_t>(), E.d_hits.as<uint32_t>(),
+                           E.d_hit_n.as<uint32_t>(), &out[0].err};
         check(gbh_launch_harvest(c, 0, &h), "harvest");
     }
     check(gbh_event_record(c, 0, ev[EV_SETUP].e), "event");
@@ -434,7 +450,10 @@ void DenseRun::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     check(gbh_launch_window(c, 0, &w), "window sieve");
     check(gbh_event_record(c, 0, ev[EV_WINDOW].e), "event");
     if (check_and_count) {
-        gbh_check_args ca{b, E.ka, pcap_, E.d_coff.as<uint32_t>(), E.d_pending.as<uint32_t>(), out};
+        uint32_t kinds = 0;
+        for (const SegJob& j : batch) kinds |= j.sampled ? 2u : 1u;
+        gbh_check_args ca{b, E.ka, pcap_, E.d_coff.as<uint32_t>(), E.d_coffa.as<uint32_t>(),
+                          E.d_pending.as<uint32_t>(), out, kinds, 0};
         check(gbh_launch_check_a(c, 0, &ca), "check phase A");
         gbh_checkb_args cb{b, E.ka, CHECK_TAIL, E.d_coff.as<uint32_t>(), E.d_pending.as<uint32_t>(), pcap_, 0, out};
         check(gbh_launch_check_b(c, 0, &cb), "check phase B");

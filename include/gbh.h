@@ -46,6 +46,7 @@ enum {
 #define GBH_CLASSES 8            /* residues 1 7 11 13 17 19 23 29 mod 30 */
 #define GBH_EVEN_CLASSES 15      /* n mod 30 = 0, 2, ..., 28 */
 #define GBH_NCAND 1205           /* odd primes 3..9781 (MAX_GB_PRIME, config.hpp:86) */
+#define GBH_NCAND_PAD 1208       /* candidate rows padded to a multiple of 4 */
 #define GBH_NSMALL 15            /* mask-stage primes 7..61 (MASK_MAX_P = 64, config.hpp:141) */
 #define GBH_GAP_BINS 1024        /* gap histogram bins g / 2 (GAPS30_HBINS, gstats.hpp:75) */
 #define GBH_GAP_TAIL 700         /* tail gaps (GAPS30_TAIL, gstats.hpp:74) */
@@ -130,19 +131,24 @@ typedef struct {
 
 /* Stage 2: hit lists of the large primes for the whole batch (reference
  * hl_harvest / hl_harvest_mt, goldbach.cl:3065-3426): every multiple p m in
- * the batch's q range with m coprime to 2 3 5 7 11 13 (the other multiples
- * are cleared by the mask stage), filed as a window offset into the bucket
- * of its (segment, class, window). Work-group g takes the primes
- * [chunks[g], chunks[g + 1]). */
+ * the batch's q range with m >= p coprime to 2 3 5 7 11 13 (the other
+ * multiples are cleared by the mask stage), first filed as a class bit into
+ * the list of its (segment, class), then split by window into the lists the
+ * window sieve applies. Work-group g of the harvest takes the primes
+ * [chunks[g], chunks[g + 1]). The batch's q range must stay below 2^33. */
 typedef struct {
     gbh_batch b;
     const uint32_t* primes;        /* large primes (device), ascending */
     const uint32_t* chunks;        /* [nchunks + 1] prime index boundaries */
     uint32_t nchunks;
-    uint32_t cap;                  /* entries per bucket */
+    uint32_t cap;                  /* entries per window list */
+    uint32_t capsc;                /* entries per (segment, class) list */
+    uint32_t pad0;
+    uint32_t* hits_sc;             /* [nseg][8][capsc] class bits */
+    uint32_t* hit_n_sc;            /* [nseg][8] fill counts (zeroed before the launch) */
     uint32_t* hits;                /* [nseg][8][nwin][cap] window offsets */
     uint32_t* hit_n;               /* [nseg][8][nwin] fill counts (zeroed before the launch) */
-    uint32_t* err;                 /* bit 0 set on a bucket overflow */
+    uint32_t* err;                 /* bit 0 set on a list overflow */
 } gbh_harvest_args;
 
 /* Stage 3: LDS window sieve of every class array of every segment
@@ -166,11 +172,14 @@ typedef struct {
  * every even class; the p(n) histogram on segments with hist_on. */
 typedef struct {
     gbh_batch b;
-    uint32_t ka;                   /* phase-A candidates per even class (<= GBH_KA_MAX) */
+    uint32_t ka;                   /* phase-A candidates per even class (a multiple of 4, <= GBH_KA_MAX) */
     uint32_t pcap;                 /* pending entries per segment */
     uint32_t* coff;                /* [nseg][15][GBH_NCAND] per-segment candidate table (written here) */
+    uint32_t* coffa;               /* [nseg][15][GBH_NCAND_PAD] the same as staged-word offsets (written here) */
     uint32_t* pending;             /* [nseg][pcap] even indices left for phase B */
     gbh_segout* out;               /* [nseg] */
+    uint32_t kinds;                /* bit 0: the batch has segments without histogram, bit 1: with */
+    uint32_t pad0;
 } gbh_check_args;
 
 /* Stage 5: phase B (reference check30_b, goldbach.cl:2210-2306): the
