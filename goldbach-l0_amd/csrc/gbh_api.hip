@@ -128,7 +128,7 @@ int gbh_open(int device, gbh_ctx** out)
                                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) (void)hipGetLastError();   // optional: windows up to the default limit still work
     if (hipFuncSetAttribute(reinterpret_cast<const void*>(&gbh_dev::k_harvest),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024) != hipSuccess)
+                            hipFuncAttributeMaxDynamicSharedMemorySize, GBH_HL_BUCKETS * 4) != hipSuccess)
         (void)hipGetLastError();
     *out = c;
     return GBH_OK;
@@ -272,14 +272,22 @@ int gbh_launch_harvest(gbh_ctx* c, int s, const gbh_harvest_args* a)
 {
     if (const int rc = check_batch(a->b)) return rc;
     if (!a->nchunks) return GBH_OK;
-    if (a->b.nwin > gbh_dev::kMaxWin) return fail(GBH_E_ARG, "harvest: more windows per class than the split holds");
-    if (uint64_t(a->b.nwin) * a->cap > a->capsc) return fail(GBH_E_ARG, "harvest: class lists smaller than the windows'");
-    hipLaunchKernelGGL(gbh_dev::k_harvest, dim3(a->nchunks), dim3(256), 0, S(c, s), a->b, a->primes, a->chunks,
-                       a->capsc, a->hits_sc, a->hit_n_sc, a->err);
+    const uint64_t nb = uint64_t(a->b.nseg) * 8 * a->b.nwin;
+    if (nb > GBH_HL_BUCKETS) return fail(GBH_E_ARG, "harvest: more window lists than the work-group counts");
+    if (a->m_base % 240) return fail(GBH_E_ARG, "harvest: bitmap base not a multiple of 240");
+    const size_t lds = size_t(nb) * 4;
+    hipLaunchKernelGGL(gbh_dev::k_harvest, dim3(a->nchunks), dim3(256), lds, S(c, s), a->b, a->primes, a->chunks,
+                       a->t_hl, a->mbits, a->m_base, a->m_span, a->cap, a->hits, a->hit_n, a->err);
     GBH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(gbh_dev::k_split, dim3((a->capsc + gbh_dev::kSplitChunk - 1) / gbh_dev::kSplitChunk,
-                       a->b.nseg * 8), dim3(256), 0, S(c, s), a->b, a->hits_sc, a->hit_n_sc, a->capsc, a->hits,
-                       a->hit_n, a->cap, a->err);
+    return GBH_OK;
+}
+
+int gbh_launch_mtrans(gbh_ctx* c, int s, const gbh_mtrans_args* a)
+{
+    if (const int rc = check_batch(a->b)) return rc;
+    if (a->nc == 0 || a->nc % 64 || uint64_t(a->nc) / 64 > a->b.cw) return fail(GBH_E_ARG, "mtrans: class array size");
+    hipLaunchKernelGGL(gbh_dev::k_mtrans, dim3((a->nc / 64 + 255) / 256, a->b.nseg), dim3(256), 0, S(c, s), a->b,
+                       a->nc, a->mbits, a->word0, a->mwords);
     GBH_HIP(hipGetLastError());
     return GBH_OK;
 }

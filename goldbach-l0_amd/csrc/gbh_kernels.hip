@@ -13,7 +13,10 @@
 //   k_setup     first class bit of every window prime, per segment and class
 //               (reference mid_setup, goldbach.cl:3528-3563)
 //   k_harvest   hit lists of the large primes over the whole batch, filed by
-//               (segment, class, window) (reference hl_harvest, 3065-3144)
+//               (segment, class, window): multiples p m with m prime from
+//               the multiplier bitmap (reference hl_harvest_mt, 3251-3426)
+//   k_mtrans    builds the multiplier bitmap from class maps sieved over m
+//               (reference mt_build, 3181-3242)
 //   k_window    one work-group per (window, class, segment): masks for
 //               7..61, LDS marks of the window primes, the window's hit list,
 //               one coalesced store (reference make_prime_map + mid_window +
@@ -102,42 +105,65 @@ __global__ void __launch_bounds__(256) k_setup(gbh_batch b, const uint32_t* __re
 }
 
 // ---------------------------------------------------------------- stage 2
-constexpr uint32_t kRep = 8;     // bucket counters replicated by lane & 7 (fewer same-address LDS atomics)
-
+// Large primes p >= P (the window bound, with P^3 above every q of the
+// batch) are listed per batch, filed by (segment, class, window) so the
+// window sieve applies each window's list in LDS.
+//
+// A composite q with no prime factor below P has exactly two prime factors
+// (q < P^3), so q = p m with p <= m and m prime. Primes p >= t_hl therefore
+// list only the multiples p m whose multiplier m is prime, read from the
+// resident multiplier bitmap (reference mt_build / hl_harvest_mt,
+// goldbach.cl:3181-3426; here the bitmap covers every m the run can need
+// instead of a capped table). Primes in [P, t_hl) (present only when the
+// bitmap budget caps m) walk every multiplier coprime to 30030 (reference
+// sieve_large_packed's wheel, goldbach.cl:480-748).
+//
+// Multiplier bitmap: word w, bit 8 j + i <-> m = m_base + 240 w + 30 j + R[i]
+// with R the unit residues mod 30 ascending, so the bits ascend with m.
+//
 // Positions inside a batch are half-offsets d = (q - qb0) / 2 < 2^32 (the
 // host keeps a batch's q range below 2^33), so the per-hit work is 32-bit.
-struct HarvestCtx {
+
+constexpr uint64_t kResPacked = 0x1D1713110D0B0701ull;   // R[i] in byte i
+constexpr uint32_t kUnitMask = 0x208A2882u;               // bit r set for the units r mod 30
+
+__device__ __forceinline__ uint32_t res_of(uint32_t i) { return (uint32_t)(kResPacked >> (8 * i)) & 0xFFu; }
+// unit residues mod 30 below r, r in [0, 30]
+__device__ __forceinline__ uint32_t units_below(uint32_t r) { return (uint32_t)__popc(kUnitMask & ((1u << r) - 1)); }
+
+struct HlCtx {
     uint64_t qb0;            // the batch's first q
     uint32_t dmax;           // half-offset of the batch's last q
     uint32_t hs_magic;       // floor(2^32 / half span): estimates d / half span from below
-    uint32_t nseg;
+    uint32_t nseg, nwin, lw, wmask;
+    uint32_t t_hl;           // first prime served by the bitmap
+    const uint64_t* mbits;
+    uint64_t m_base, m_span; // the bitmap covers m in [m_base, m_base + m_span)
 };
 
-// Files the multiple at half-offset d of class c into the (segment, class)
-// list of every segment whose q range holds it (consecutive segments overlap
-// by 9776 in q); the entry is the multiple's class bit.
+// Files the multiple at half-offset d of class c into the window list of
+// every segment whose q range holds it (consecutive segments overlap by 9776
+// in q); the entry is the multiple's bit inside its window.
 template <bool kWrite>
-__device__ __forceinline__ void file_one(uint32_t k, uint32_t c, uint32_t d, const uint32_t* dq, uint32_t lane8,
-                                         uint32_t* cnt, const uint32_t* base, uint32_t* __restrict__ hits,
-                                         uint32_t capsc, uint32_t* __restrict__ err)
+__device__ __forceinline__ void file_one(const HlCtx& h, uint32_t s, uint32_t c, uint32_t d, const uint32_t* dq,
+                                         uint32_t* cnt, uint32_t* __restrict__ hits, uint32_t cap,
+                                         uint32_t* __restrict__ err)
 {
-    const uint32_t t = __umulhi(d - dq[k * 8 + c], 0x88888889u) >> 3;   // (q - Q) / 30 = (d - dq) / 15
-    const uint32_t b = k * 8 + c;
-    const uint32_t slot = b * kRep + lane8;
+    const uint32_t t = __umulhi(d - dq[s * 8 + c], 0x88888889u) >> 3;   // (q - Q) / 30 = (d - dq) / 15
+    const uint32_t bk = (s * 8 + c) * h.nwin + (t >> h.lw);
     if (!kWrite) {
-        atomicAdd(&cnt[slot], 1u);
+        atomicAdd(&cnt[bk], 1u);
     } else {
-        const uint32_t pos = base[slot] + atomicAdd(&cnt[slot], 1u);
-        if (pos < capsc) hits[(size_t)b * capsc + pos] = t;
+        const uint32_t pos = atomicAdd(&cnt[bk], 1u);
+        if (pos < cap) hits[(size_t)bk * cap + pos] = t & h.wmask;
         else atomicOr(err, 1u);
     }
 }
 
 template <bool kWrite>
-__device__ __forceinline__ void file_hit(const HarvestCtx& h, const uint32_t* df, const uint32_t* dl,
-                                         const uint32_t* dq, uint32_t d, uint32_t c, uint32_t lane8, uint32_t* cnt,
-                                         const uint32_t* base, uint32_t* __restrict__ hits, uint32_t capsc,
-                                         uint32_t* __restrict__ err)
+__device__ __forceinline__ void file_hit(const HlCtx& h, const uint32_t* df, const uint32_t* dl, const uint32_t* dq,
+                                         uint32_t d, uint32_t c, uint32_t* cnt, uint32_t* __restrict__ hits,
+                                         uint32_t cap, uint32_t* __restrict__ err)
 {
     // the segment starting at or below d: an estimate at most two below,
     // corrected without branches
@@ -146,14 +172,14 @@ __device__ __forceinline__ void file_hit(const HarvestCtx& h, const uint32_t* df
     s += (s + 1 < h.nseg && d >= df[min(s + 1, h.nseg - 1)]);
     while (s + 1 < h.nseg && d >= df[s + 1]) ++s;     // irregular spacing only (a run's first segment below 9783)
     while (s > 0 && d < df[s]) --s;
-    if (d <= dl[s]) file_one<kWrite>(s, c, d, dq, lane8, cnt, base, hits, capsc, err);
-    if (s > 0 && d <= dl[s - 1]) file_one<kWrite>(s - 1, c, d, dq, lane8, cnt, base, hits, capsc, err);   // overlap
+    if (d <= dl[s]) file_one<kWrite>(h, s, c, d, dq, cnt, hits, cap, err);
+    if (s > 0 && d <= dl[s - 1]) file_one<kWrite>(h, s - 1, c, d, dq, cnt, hits, cap, err);   // overlap
 }
 
-// ceil(x / p) for p >= 2^12: a double-precision quotient (off by at most
-// one here: x rounds to within 2^11 and the product to 2^-52 relative) made
-// exact by the integer remainder.
-__device__ __forceinline__ uint64_t ceil_div_big(uint64_t x, double xd, uint32_t p)
+// floor(x / p) for p >= 2^12 and its remainder: a double-precision quotient
+// (off by at most one here: x rounds to within 2^11 and the product to
+// 2^-52 relative) made exact by the integer remainder.
+__device__ __forceinline__ uint64_t div_big(uint64_t x, double xd, uint32_t p, uint32_t* rem)
 {
     const double qd = xd * (1.0 / (double)p);
     const uint64_t hi = (uint64_t)(uint32_t)(qd * (1.0 / 4294967296.0));
@@ -161,25 +187,26 @@ __device__ __forceinline__ uint64_t ceil_div_big(uint64_t x, double xd, uint32_t
     int64_t r = (int64_t)(x - q * p);
     while (r < 0) { --q; r += p; }
     while (r >= (int64_t)p) { ++q; r -= p; }
-    return r ? q + 1 : q;
+    *rem = (uint32_t)r;
+    return q;
 }
 
-// Every multiple q = p m in [qb0, qb1] with m >= p and m coprime to 30030
-// (a multiple of 2 3 5 7 11 13 is not in the class map or is cleared by the
-// mask stage): m walks the 48 residues coprime to 210, 11 and 13 are
-// tested by running residues, the class of q follows from m mod 30.
-// LDS copies of the wheel tables (per-lane lookups stay out of memory)
+// The wheel walk: every multiple q = p m in [qb0, qb1] with m >= p and m
+// coprime to 30030 (a multiple of 2 3 5 7 11 13 is not in the class map or
+// is cleared by the mask stage): m walks the 48 residues coprime to 210, 11
+// and 13 are tested by running residues, the class of q follows from m mod 30.
 struct WheelLds {
     uint8_t next[210], idx[210], inc[48], cls[30];
 };
 
 template <bool kWrite>
-__device__ __forceinline__ void harvest_prime(const HarvestCtx& h, double qb0d, uint32_t p, const uint32_t* df,
-                                              const uint32_t* dl, const uint32_t* dq, const WheelLds& wl,
-                                              uint32_t lane8, uint32_t* cnt, const uint32_t* base,
-                                              uint32_t* __restrict__ hits, uint32_t capsc, uint32_t* __restrict__ err)
+__device__ __forceinline__ void walk_wheel(const HlCtx& h, double qb0d, uint32_t p, const uint32_t* df,
+                                           const uint32_t* dl, const uint32_t* dq, const WheelLds& wl, uint32_t* cnt,
+                                           uint32_t* __restrict__ hits, uint32_t cap, uint32_t* __restrict__ err)
 {
-    uint64_t m = ceil_div_big(h.qb0, qb0d, p);
+    uint32_t rem;
+    uint64_t m = div_big(h.qb0, qb0d, p, &rem);
+    m += rem != 0;
     if (m < p) m = p;
     const uint32_t r = (uint32_t)(m % 30030);
     const uint32_t r210 = r % 210;
@@ -192,8 +219,7 @@ __device__ __forceinline__ void harvest_prime(const HarvestCtx& h, double qb0d, 
     uint32_t idx = wl.idx[(r210 + d0) % 210];
     const uint32_t pr = p % 30;
     for (;;) {
-        if (r11 && r13)
-            file_hit<kWrite>(h, df, dl, dq, d, wl.cls[(r30 * pr) % 30], lane8, cnt, base, hits, capsc, err);
+        if (r11 && r13) file_hit<kWrite>(h, df, dl, dq, d, wl.cls[(r30 * pr) % 30], cnt, hits, cap, err);
         const uint32_t inc = wl.inc[idx];       // even: the walk stays on odd m
         idx = idx == 47 ? 0 : idx + 1;
         const uint64_t step = (uint64_t)(inc >> 1) * p;
@@ -205,19 +231,72 @@ __device__ __forceinline__ void harvest_prime(const HarvestCtx& h, double qb0d, 
     }
 }
 
-__global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __restrict__ primes,
-                                                 const uint32_t* __restrict__ chunks, uint32_t capsc,
-                                                 uint32_t* __restrict__ hits, uint32_t* __restrict__ hit_n,
-                                                 uint32_t* __restrict__ err)
+// The bitmap walk: every multiple q = p m in [qb0, qb1] with m >= p prime.
+// cmt[k] holds, 3 bits per residue index i, the class of R[k] R[i] mod 30.
+template <bool kWrite>
+__device__ __forceinline__ void walk_bitmap(const HlCtx& h, double qb0d, double qb1d, uint32_t p,
+                                            const uint32_t* df, const uint32_t* dl, const uint32_t* dq,
+                                            const uint32_t* cmt, uint32_t* cnt, uint32_t* __restrict__ hits,
+                                            uint32_t cap, uint32_t* __restrict__ err)
 {
-    __shared__ uint32_t df[GBH_MAX_BATCH], dl[GBH_MAX_BATCH], dq[GBH_MAX_BATCH * 8];
-    __shared__ uint32_t cnt[GBH_MAX_BATCH * 8 * kRep], base[GBH_MAX_BATCH * 8 * kRep];
+    const uint64_t qb1 = h.qb0 + 2ull * h.dmax;
+    uint32_t rem;
+    uint64_t m0 = div_big(h.qb0, qb0d, p, &rem);
+    m0 += rem != 0;
+    if (m0 < p) m0 = p;
+    const uint64_t m1 = div_big(qb1, qb1d, p, &rem);
+    if (m0 > m1) return;
+    if (m0 < h.m_base || m1 - h.m_base >= h.m_span) {   // outside the bitmap: fail closed
+        atomicOr(err, 8u);
+        return;
+    }
+    const uint64_t x0 = m0 - h.m_base, x1 = m1 - h.m_base;
+    const uint64_t w0 = x0 / 240, w1 = x1 / 240;
+    const uint32_t e0 = (uint32_t)(x0 - w0 * 240), e1 = (uint32_t)(x1 - w1 * 240);
+    const uint32_t n0 = 8 * (e0 / 30) + units_below(e0 % 30);        // bits of m < m0 in word w0 (<= 63)
+    const uint32_t n1 = 8 * (e1 / 30) + units_below(e1 % 30 + 1);    // bits of m <= m1 in word w1 (0..64)
+    const uint64_t lo_mask = ~0ull << n0;
+    const uint64_t hi_mask = n1 >= 64 ? ~0ull : ((1ull << n1) - 1);
+    const uint32_t cmap = cmt[units_below(p % 30)];
+    for (uint64_t w = w0; w <= w1; ++w) {
+        uint64_t v = h.mbits[w];
+        if (w == w0) v &= lo_mask;
+        if (w == w1) v &= hi_mask;
+        if (!v) continue;
+        const uint64_t qw = (uint64_t)p * (h.m_base + 240 * w) - h.qb0;   // mod 2^64; q - qb0 >= 0 for listed bits
+        do {
+            const uint32_t bit = (uint32_t)__ffsll((long long)v) - 1;
+            v &= v - 1;
+            const uint32_t i = bit & 7;
+            const uint32_t delta = 30 * (bit >> 3) + res_of(i);
+            const uint32_t d = (uint32_t)((qw + (uint64_t)p * delta) >> 1);
+            file_hit<kWrite>(h, df, dl, dq, d, (cmap >> (3 * i)) & 7, cnt, hits, cap, err);
+        } while (v);
+    }
+}
+
+// One work-group per chunk of primes; two passes over the chunk: count per
+// window list, reserve each list's share with one atomic, then write.
+// A bitmap prime with p^3 <= qb1 breaks the two-factor rule: fail closed.
+__global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __restrict__ primes,
+                                                 const uint32_t* __restrict__ chunks, uint32_t t_hl,
+                                                 const uint64_t* __restrict__ mbits, uint64_t m_base, uint64_t m_span,
+                                                 uint32_t cap, uint32_t* __restrict__ hits,
+                                                 uint32_t* __restrict__ hit_n, uint32_t* __restrict__ err)
+{
+    __shared__ uint32_t df[GBH_MAX_BATCH], dl[GBH_MAX_BATCH], dq[GBH_MAX_BATCH * 8], cmt[8];
     __shared__ WheelLds wl;
-    const uint32_t nb = b.nseg * 8 * kRep;
-    const uint32_t tid = threadIdx.x, lane8 = tid & 7;
+    extern __shared__ uint32_t cnt[];     // [nseg][8][nwin]
+    const uint32_t nb = b.nseg * 8 * b.nwin;
+    const uint32_t tid = threadIdx.x;
     if (tid < 210) { wl.next[tid] = c_w210_next[tid]; wl.idx[tid] = c_w210_idx[tid]; }
     if (tid < 48) wl.inc[tid] = c_w210_inc[tid];
     if (tid < 30) wl.cls[tid] = c_cls_of[tid];
+    if (tid < 8) {
+        uint32_t v = 0;
+        for (uint32_t i = 0; i < 8; ++i) v |= (uint32_t)c_cls_of[(res_of(tid) * res_of(i)) % 30] << (3 * i);
+        cmt[tid] = v;
+    }
     const uint64_t qb0 = b.geoms[0].q_first;
     for (uint32_t s = tid; s < b.nseg; s += 256) {
         df[s] = (uint32_t)((b.geoms[s].q_first - qb0) >> 1);
@@ -226,76 +305,80 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
     for (uint32_t i = tid; i < b.nseg * 8; i += 256) dq[i] = (uint32_t)((b.geoms[i / 8].Q[i % 8] - qb0) >> 1);
     for (uint32_t i = tid; i < nb; i += 256) cnt[i] = 0;
     __syncthreads();
-    HarvestCtx h;
+    HlCtx h;
     h.qb0 = qb0;
     h.dmax = dl[b.nseg - 1];
     h.hs_magic = (b.nseg > 1 && df[1] > df[0]) ? (uint32_t)((1ull << 32) / (df[1] - df[0] + 1)) : 0u;
     h.nseg = b.nseg;
-    const double qb0d = (double)qb0;
+    h.nwin = b.nwin;
+    h.lw = 31 - __clz(b.wbits);
+    h.wmask = b.wbits - 1;
+    h.t_hl = t_hl;
+    h.mbits = mbits;
+    h.m_base = m_base;
+    h.m_span = m_span;
+    const uint64_t qb1 = qb0 + 2ull * h.dmax;
+    const double qb0d = (double)qb0, qb1d = (double)qb1;
     const uint32_t j0 = chunks[blockIdx.x], j1 = chunks[blockIdx.x + 1];
-    for (uint32_t j = j0 + tid; j < j1; j += 256)
-        harvest_prime<false>(h, qb0d, primes[j], df, dl, dq, wl, lane8, cnt, base, hits, capsc, err);
+    for (uint32_t j = j0 + tid; j < j1; j += 256) {
+        const uint32_t p = primes[j];
+        if (p < t_hl) {
+            walk_wheel<false>(h, qb0d, p, df, dl, dq, wl, cnt, hits, cap, err);
+        } else {
+            const uint64_t p2 = (uint64_t)p * p;
+            if (__umul64hi(p2, p) == 0 && p2 * p <= qb1) atomicOr(err, 4u);   // two-factor rule broken
+            else walk_bitmap<false>(h, qb0d, qb1d, p, df, dl, dq, cmt, cnt, hits, cap, err);
+        }
+    }
     __syncthreads();
     for (uint32_t i = tid; i < nb; i += 256) {
         const uint32_t c = cnt[i];
-        base[i] = c ? atomicAdd(&hit_n[i / kRep], c) : 0u;
-        cnt[i] = 0;
+        cnt[i] = c ? atomicAdd(&hit_n[i], c) : 0u;
     }
     __syncthreads();
-    for (uint32_t j = j0 + tid; j < j1; j += 256)
-        harvest_prime<true>(h, qb0d, primes[j], df, dl, dq, wl, lane8, cnt, base, hits, capsc, err);
+    for (uint32_t j = j0 + tid; j < j1; j += 256) {
+        const uint32_t p = primes[j];
+        if (p < t_hl) {
+            walk_wheel<true>(h, qb0d, p, df, dl, dq, wl, cnt, hits, cap, err);
+        } else {
+            const uint64_t p2 = (uint64_t)p * p;
+            if (__umul64hi(p2, p) != 0 || p2 * p > qb1) walk_bitmap<true>(h, qb0d, qb1d, p, df, dl, dq, cmt, cnt, hits, cap, err);
+        }
+    }
 }
 
-// Splits every (segment, class) list by window into the window lists the
-// window sieve reads, 4096 entries per work-group, sorted in LDS so each
-// window's part is written as one contiguous run.
-constexpr uint32_t kSplitChunk = 4096;
-constexpr uint32_t kMaxWin = 256;
-
-__global__ void __launch_bounds__(256) k_split(gbh_batch b, const uint32_t* __restrict__ hits_sc,
-                                               const uint32_t* __restrict__ hit_n_sc, uint32_t capsc,
-                                               uint32_t* __restrict__ hits_w, uint32_t* __restrict__ hit_n_w,
-                                               uint32_t cap, uint32_t* __restrict__ err)
+// 8 x 8 bit transpose: bit 8 r + c <-> bit 8 c + r (three delta swaps)
+__device__ __forceinline__ uint64_t transpose8(uint64_t x)
 {
-    __shared__ uint32_t stage[kSplitChunk], sorted[kSplitChunk];
-    __shared__ uint32_t wc[kMaxWin], ws[kMaxWin], wb[kMaxWin];
-    const uint32_t bsc = blockIdx.y, tid = threadIdx.x;
-    const uint32_t n = min(hit_n_sc[bsc], capsc);
-    const uint32_t i0 = blockIdx.x * kSplitChunk;
-    if (i0 >= n) return;
-    const uint32_t cnt = min(n - i0, kSplitChunk);
-    const uint32_t lw = 31 - __clz(b.wbits), wmask = b.wbits - 1, nwin = b.nwin;
-    for (uint32_t w = tid; w < nwin; w += 256) wc[w] = 0;
-    __syncthreads();
-    const uint32_t* src = hits_sc + (size_t)bsc * capsc + i0;
-    for (uint32_t i = tid; i < cnt; i += 256) {
-        const uint32_t t = src[i];
-        stage[i] = t;
-        atomicAdd(&wc[min(t >> lw, nwin - 1)], 1u);
-    }
-    __syncthreads();
-    if (tid == 0) {
-        uint32_t run = 0;
-        for (uint32_t w = 0; w < nwin; ++w) { ws[w] = run; run += wc[w]; }
-    }
-    __syncthreads();
-    for (uint32_t w = tid; w < nwin; w += 256) {
-        wb[w] = wc[w] ? atomicAdd(&hit_n_w[bsc * nwin + w], wc[w]) : 0u;
-        wc[w] = 0;
-    }
-    __syncthreads();
-    for (uint32_t i = tid; i < cnt; i += 256) {
-        const uint32_t t = stage[i];
-        const uint32_t w = min(t >> lw, nwin - 1);
-        sorted[ws[w] + atomicAdd(&wc[w], 1u)] = t;
-    }
-    __syncthreads();
-    for (uint32_t k = tid; k < cnt; k += 256) {
-        const uint32_t t = sorted[k];
-        const uint32_t w = min(t >> lw, nwin - 1);
-        const uint32_t pos = wb[w] + (k - ws[w]);
-        if (pos < cap) hits_w[((size_t)bsc * nwin + w) * cap + pos] = t & wmask;
-        else atomicOr(err, 1u);
+    uint64_t t = (x ^ (x >> 7)) & 0x00AA00AA00AA00AAull;
+    x ^= t ^ (t << 7);
+    t = (x ^ (x >> 14)) & 0x0000CCCC0000CCCCull;
+    x ^= t ^ (t << 14);
+    t = (x ^ (x >> 28)) & 0x00000000F0F0F0F0ull;
+    x ^= t ^ (t << 28);
+    return x;
+}
+
+// Multiplier bitmap from a batch of class maps over m: segment s holds the
+// bytes y in [s nc, (s + 1) nc) of the bitmap (all eight class arrays start
+// at the same byte), byte y holds the eight class bits of y.
+__global__ void __launch_bounds__(256) k_mtrans(gbh_batch b, uint32_t nc, uint64_t* __restrict__ mbits,
+                                                uint64_t word0, uint64_t mwords)
+{
+    const uint32_t x = blockIdx.x * 256 + threadIdx.x;   // class word of the segment
+    const uint32_t s = blockIdx.y;
+    if (x >= nc / 64) return;
+    const uint64_t* map = seg_map(b, s);
+    uint64_t cwd[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) cwd[c] = map[(size_t)c * b.cw + x];
+    const uint64_t ow = word0 + (uint64_t)s * (nc / 8) + 8ull * x;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        uint64_t v = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v |= ((cwd[i] >> (8 * j)) & 0xFFull) << (8 * i);
+        if (ow + j < mwords) mbits[ow + j] = transpose8(v);
     }
 }
 

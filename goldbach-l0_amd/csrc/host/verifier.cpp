@@ -22,6 +22,7 @@
 #include "gbh.h"
 #include "geometry.hpp"
 #include "gstats.hpp"
+#include "mbits.hpp"
 #include "primes.hpp"
 
 namespace gb {
@@ -156,8 +157,7 @@ struct Engine::State {
     std::optional<Ctx> ctx;
     gbh_ctx* c = nullptr;
     int cus = 256;
-    Buf d_win, d_large, d_chunks, d_i0, d_map, d_coff, d_coffa, d_pending, d_scratch, d_hits, d_hit_n, d_hits_sc,
-        d_hit_n_sc;
+    Buf d_win, d_large, d_chunks, d_i0, d_map, d_coff, d_coffa, d_pending, d_scratch, d_hits, d_hit_n;
     Buf d_geoms, d_cs;
     Buf d_out[2], h_out[2], h_geoms[2], h_cs[2], h_chunks;
     Event ev[2][EV_N];
@@ -234,8 +234,11 @@ public:
 
 private:
     void resolve_range();
+    void plan_bitmap();
     void configure_stats();
     void plan_buffers();
+    double hit_factor(uint32_t j) const;
+    double hit_density() const;
     void enqueue(std::vector<SegJob>& batch, int slot, bool check_and_count);
     void collect(std::vector<SegJob>& batch, int slot);
     void collect_segment(const SegJob& j, const gbh_segout& o, double share[4]);
@@ -250,7 +253,10 @@ private:
     uint64_t total_evens_ = 0;
     clock_::time_point t_start_;
     uint32_t nwin_ = 0, nlarge_ = 0, split_ = 0, ncoop_ = 0;   // this run's window / large prime counts
-    uint32_t wbits_ = 0, nwin_max_ = 0, cw_ = 0, cap_ = 0, capsc_ = 0, pcap_ = 0, swords_ = 0, nchunks_ = 0, kb_ = 1;
+    uint32_t wbits_ = 0, nwin_max_ = 0, cw_ = 0, cap_ = 0, pcap_ = 0, swords_ = 0, nchunks_ = 0, kb_ = 1;
+    // large primes: [0, nwheel_) walk the wheel, [nwheel_, nlarge_) read the multiplier bitmap from t_hl_ up
+    uint32_t nwheel_ = 0, t_hl_ = UINT32_MAX;
+    std::shared_ptr<const MultBitmap> mb_;
     bool stats_on_ = false, gaps_on_ = false, gaps_exact_ = false;
     uint32_t sample_ = 256;
     std::optional<GoldbachStats> gs_;
@@ -300,6 +306,37 @@ void DenseRun::resolve_range()
     split_ = E.win_lo + nwin_;
     nlarge_ = nb > E.win_hi ? nb - E.win_hi : 0;
     rs_.split_used = split_;
+    plan_bitmap();
+}
+
+// The large primes' split between the wheel walk and the multiplier bitmap:
+// a prime p >= t_hl lists p m with m prime, exact while every q < t_hl^3;
+// the bitmap covers m in [max(t_hl, sqrt(q_lo)), q_hi / t_hl] within its
+// budget (t_hl rises until it fits).
+void DenseRun::plan_bitmap()
+{
+    nwheel_ = nlarge_;
+    t_hl_ = UINT32_MAX;
+    mb_.reset();
+    if (!nlarge_) return;
+    const uint64_t budget = mult_bitmap_budget();
+    if (!budget) return;
+    const uint64_t q_lo = start_ >= MAX_GB_PRIME + 2 ? start_ - MAX_GB_PRIME : 3, q_hi = limit_ - 3;
+    const uint64_t m_cap = (budget - std::min<uint64_t>(budget, uint64_t(1) << 24)) * 30;   // bitmap bytes -> m span
+    const uint64_t sq_lo = uint64_t(std::sqrt(double(q_lo)));
+    const uint32_t* lp = E.base.data() + E.win_hi;
+    // smallest bitmap prime: its cube above q_hi, and the m span within the budget
+    const auto cube_le = [](uint64_t x, uint64_t q) { return (unsigned __int128)x * x * x <= q; };
+    uint64_t cb = uint64_t(std::cbrt(double(q_hi)));
+    while (cube_le(cb + 1, q_hi)) ++cb;
+    while (cb > 1 && !cube_le(cb, q_hi)) --cb;
+    const uint64_t t = std::max<uint64_t>({lp[0], cb + 1, q_hi / (m_cap + sq_lo) + 1});
+    nwheel_ = uint32_t(std::lower_bound(lp, lp + nlarge_, uint32_t(std::min<uint64_t>(t, UINT32_MAX))) - lp);
+    if (nwheel_ == nlarge_) return;
+    t_hl_ = lp[nwheel_];
+    const uint64_t m_lo = std::max<uint64_t>(t_hl_, sq_lo > 1 ? sq_lo - 1 : 1), m_hi = q_hi / t_hl_;
+    if (m_hi < m_lo) { nwheel_ = nlarge_; t_hl_ = UINT32_MAX; return; }
+    mb_ = mult_bitmap(m_lo, m_hi, E.base, verbose_ || phase_trace_);
 }
 
 void DenseRun::configure_stats()
@@ -322,6 +359,34 @@ void DenseRun::configure_stats()
     for (uint32_t cr = 0; cr < 15; ++cr)
         if (E.cand.primes[E.cand.slot_pi[cr][std::min<size_t>(E.ka, E.cand.slot_pi[cr].size()) - 1]] >= CHECK_TAIL)
             throw std::runtime_error("GB_KA too large: a phase-A candidate reaches the tail bound");
+}
+
+// Expected hits per class bit and unit 1/p of large prime j (the j-th above
+// the window bound): the wheel lists every multiplier coprime to 7 11 13,
+// the bitmap lists prime multipliers, (30/8) / ln m of those coprime to 30
+// (m = q / p >= max(q_lo / p, p), the bound that makes the rate largest).
+double DenseRun::hit_factor(uint32_t j) const
+{
+    if (j < nwheel_) return kWheelFrac;
+    const double p = E.base[E.win_hi + j];
+    const double q_lo = double(start_ >= MAX_GB_PRIME + 2 ? start_ - MAX_GB_PRIME : 3);
+    return 3.75 / std::log(std::max({q_lo / p, p, 3.0}));
+}
+
+// Expected hits per class bit over all large primes: the wheel primes
+// exactly, the bitmap primes in 64 bands of the prime index, each at its
+// largest rate.
+double DenseRun::hit_density() const
+{
+    if (!nlarge_) return 0.0;
+    double d = kWheelFrac * E.inv_prefix[nwheel_];
+    const uint32_t n = nlarge_ - nwheel_;
+    for (uint32_t k = 0; k < 64 && n; ++k) {
+        const uint32_t a = nwheel_ + uint32_t(uint64_t(n) * k / 64), b = nwheel_ + uint32_t(uint64_t(n) * (k + 1) / 64);
+        if (b <= a) continue;
+        d += (E.inv_prefix[b] - E.inv_prefix[a]) * std::max(hit_factor(a), hit_factor(b - 1));
+    }
+    return d;
 }
 
 // Window size, strides, hit-list capacity and buffer sizes of this run.
@@ -349,9 +414,8 @@ void DenseRun::plan_buffers()
     cw_ = nwin_max_ * (wbits_ / 64) + 16;
     ncoop_ = 0;
     while (ncoop_ < nwin_ && E.base[E.win_lo + ncoop_] < wbits_ / 128) ++ncoop_;
-    // hits per window bucket: wbits * (wheel fraction) * sum of 1/p
-    const double lam = nlarge_ ? wbits_ * kWheelFrac * E.inv_prefix[nlarge_] : 0.0;
-    cap_ = uint32_t(std::min(lam * 1.1 + 8 * std::sqrt(lam) + 1024, 4e9 / (double(E.kbatch) * 8 * nwin_max_)));
+    const double lam = wbits_ * hit_density();
+    cap_ = uint32_t(std::min(lam * 1.15 + 8 * std::sqrt(lam) + 1024, 4e9 / (double(E.kbatch) * 8 * nwin_max_)));
     const uint64_t evens_max = span_ + 2;
     pcap_ = uint32_t(std::min<uint64_t>(evens_max / 32 + 65536, UINT32_MAX / 2 / E.kbatch));
     swords_ = uint32_t(2 * (nc_max / 64 + 4));
@@ -359,31 +423,29 @@ void DenseRun::plan_buffers()
     // a batch's q range stays below 2^33 (the harvest's 32-bit half-offsets):
     // at most kb_ - 1 spans plus the merged last segment (below 2 spans)
     const uint64_t kmax = std::max<uint64_t>(1, ((uint64_t(1) << 33) - 4 * MAX_GB_PRIME) / span_ - 1);
-    kb_ = uint32_t(std::min<uint64_t>(nseg, kmax));
+    kb_ = uint32_t(std::min<uint64_t>({nseg, kmax, GBH_HL_BUCKETS / (8 * nwin_max_)}));
+    if (!kb_) throw std::runtime_error("segment span too large for the hit-list work-group");
     const size_t K = kb_;
     E.d_map.ensure(c, K * 8 * cw_ * 8, 0);
     E.d_i0.ensure(c, K * 8 * size_t(std::max(nwin_, 1u)) * 4, 0);
     E.d_pending.ensure(c, K * size_t(pcap_) * 4, 0);
     E.d_scratch.ensure(c, K * size_t(swords_) * 4, 0);
     E.d_hit_n.ensure(c, K * 8 * size_t(nwin_max_) * 4, 0);
-    E.d_hit_n_sc.ensure(c, K * 8 * 4, 0);
-    capsc_ = nwin_max_ * cap_;
-    if (nlarge_) {
-        E.d_hits.ensure(c, K * 8 * size_t(nwin_max_) * cap_ * 4, 0);
-        E.d_hits_sc.ensure(c, K * 8 * size_t(capsc_) * 4, 0);
-    }
-    // harvest chunks: contiguous prime ranges with about the same expected
-    // hits (sum of 1/p) and at most 64k primes each
+    if (nlarge_) E.d_hits.ensure(c, K * 8 * size_t(nwin_max_) * cap_ * 4, 0);
+    // harvest chunks: contiguous prime ranges with about 2e5 expected hits
+    // (the per-prime rate is nearly constant inside a chunk), 256 to 64k
+    // primes each, split where the bitmap primes start
     std::vector<uint32_t> ch{0};
     if (nlarge_) {
-        const double per_chunk = 2e5 / (double(K) * span_ * (8.0 / 30) * kWheelFrac);
         uint32_t j = 0;
         while (j < nlarge_) {
+            const double per_chunk = 2e5 / (double(K) * span_ * (8.0 / 30) * hit_factor(j));
             const double target = E.inv_prefix[j] + per_chunk;
             uint32_t k = uint32_t(std::upper_bound(E.inv_prefix.begin() + j + 1, E.inv_prefix.begin() + nlarge_ + 1,
                                                    target) - E.inv_prefix.begin()) - 1;
             k = std::max(k, j + 256);
             k = std::min({k, j + 65536, nlarge_});
+            if (j < nwheel_ && k > nwheel_) k = nwheel_;
             ch.push_back(k);
             j = k;
         }
@@ -438,10 +500,9 @@ void DenseRun::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     }
     if (nlarge_) {
         check(gbh_zero(c, 0, E.d_hit_n.p, size_t(nseg) * 8 * nwin * 4), "zero hit counts");
-        check(gbh_zero(c, 0, E.d_hit_n_sc.p, size_t(nseg) * 8 * 4), "zero class hit counts");
-        gbh_harvest_args h{b, E.d_large.as<uint32_t>(), E.d_chunks.as<uint32_t>(), nchunks_, cap_, capsc_, 0,
-                           E.d_hits_sc.as<uint32_t>(), E.d_hit_n_sc.as<uint32_t>(), E.d_hits.as<uint32_t>(),
-                           E.d_hit_n.as<uint32_t>(), &out[0].err};
+        gbh_harvest_args h{b, E.d_large.as<uint32_t>(), E.d_chunks.as<uint32_t>(), nchunks_, cap_, t_hl_, 0,
+                           mb_ ? mb_->bits : nullptr, mb_ ? mb_->m_base : 0, mb_ ? mb_->m_span : 0,
+                           E.d_hits.as<uint32_t>(), E.d_hit_n.as<uint32_t>(), &out[0].err};
         check(gbh_launch_harvest(c, 0, &h), "harvest");
     }
     check(gbh_event_record(c, 0, ev[EV_SETUP].e), "event");
@@ -480,6 +541,8 @@ void DenseRun::collect(std::vector<SegJob>& batch, int slot)
     t_sieve_ += ts; t_large_ += tl; t_check_ += tc; t_count_ += tn;
     const gbh_segout* outs = E.h_out[slot].as<gbh_segout>();
     if (outs[0].err & 1u) throw std::runtime_error("hit-list bucket overflow (raise the capacity)");
+    if (outs[0].err & 4u) throw std::runtime_error("hit lists: a bitmap prime's cube is not above the batch's q range");
+    if (outs[0].err & 8u) throw std::runtime_error("hit lists: a multiplier outside the multiplier bitmap");
     double share[4] = {ts / batch.size(), tl / batch.size(), tc / batch.size(), tn / batch.size()};
     for (size_t s = 0; s < batch.size(); ++s) collect_segment(batch[s], outs[s], share);
     if (phase_trace_)
@@ -491,7 +554,7 @@ void DenseRun::collect(std::vector<SegJob>& batch, int slot)
 void DenseRun::collect_segment(const SegJob& j, const gbh_segout& o, double share[4])
 {
     const std::string seg = " in segment [" + std::to_string(j.sg.seg_start) + ", " + std::to_string(j.sg.seg_end) + "]";
-    if (o.err & ~1u) throw std::runtime_error("device error flags " + std::to_string(o.err) + seg);
+    if (o.err & ~13u) throw std::runtime_error("device error flags " + std::to_string(o.err) + seg);
     if (o.pending_n > pcap_) throw std::runtime_error("pending overflow (" + std::to_string(o.pending_n) + ")" + seg);
     // coverage: every even accounted for, by phase A, phase B or a leftover
     if (uint64_t(o.acct[0]) + o.pending_n != j.gpu_evens)
@@ -571,10 +634,10 @@ void DenseRun::collect_segment(const SegJob& j, const gbh_segout& o, double shar
 RunStats DenseRun::run()
 {
     t_start_ = clock_::now();
+    phase_trace_ = env::present("GB_PHASE");
     resolve_range();
     configure_stats();
     plan_buffers();
-    phase_trace_ = env::present("GB_PHASE");
     const auto t_gpu = clock_::now();
     std::vector<SegJob> batches[2];
     int cur = 0;
@@ -639,7 +702,7 @@ uint32_t DenseRun::map_of(uint64_t seg_start, std::vector<uint64_t>& words, uint
     bt[0].sg = segment_geometry(seg_start, limit_, span_);
     enqueue(bt, 0, false);
     check(gbh_event_sync(E.c, E.ev[0][EV_END].e), "sync");
-    if (E.h_out[0].as<gbh_segout>()[0].err & 1u) throw std::runtime_error("hit-list bucket overflow");
+    if (E.h_out[0].as<gbh_segout>()[0].err & 13u) throw std::runtime_error("hit-list device error");
     words.assign(size_t(8) * cw_, 0);
     check(gbh_copy_d2h(E.c, 0, words.data(), E.d_map.p, words.size() * 8), "copy map");
     check(gbh_stream_sync(E.c, 0), "sync");

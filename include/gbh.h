@@ -130,26 +130,47 @@ typedef struct {
 } gbh_setup_args;
 
 /* Stage 2: hit lists of the large primes for the whole batch (reference
- * hl_harvest / hl_harvest_mt, goldbach.cl:3065-3426): every multiple p m in
- * the batch's q range with m >= p coprime to 2 3 5 7 11 13 (the other
- * multiples are cleared by the mask stage), first filed as a class bit into
- * the list of its (segment, class), then split by window into the lists the
- * window sieve applies. Work-group g of the harvest takes the primes
- * [chunks[g], chunks[g + 1]). The batch's q range must stay below 2^33. */
+ * hl_harvest_mt, goldbach.cl:3251-3426, with sieve_large_packed's wheel,
+ * 480-748, for the primes below t_hl), filed by (segment, class, window)
+ * into the lists the window sieve applies. A prime p >= t_hl lists the
+ * multiples p m with m >= p prime, read from the multiplier bitmap; a prime
+ * below t_hl lists every multiple p m with m >= p coprime to 2 3 5 7 11 13.
+ * Exact when every q of the batch is below t_hl^3 (device guard: err bit 2)
+ * and the bitmap covers every multiplier asked for (err bit 3). Work-group g
+ * takes the primes [chunks[g], chunks[g + 1]). The batch's q range must stay
+ * below 2^33, and nseg * 8 * nwin <= GBH_HL_BUCKETS. */
+#define GBH_HL_BUCKETS 16384
 typedef struct {
     gbh_batch b;
     const uint32_t* primes;        /* large primes (device), ascending */
     const uint32_t* chunks;        /* [nchunks + 1] prime index boundaries */
     uint32_t nchunks;
     uint32_t cap;                  /* entries per window list */
-    uint32_t capsc;                /* entries per (segment, class) list */
+    uint32_t t_hl;                 /* first prime served by the bitmap */
     uint32_t pad0;
-    uint32_t* hits_sc;             /* [nseg][8][capsc] class bits */
-    uint32_t* hit_n_sc;            /* [nseg][8] fill counts (zeroed before the launch) */
+    const uint64_t* mbits;         /* multiplier bitmap (gbh_launch_mtrans layout) */
+    uint64_t m_base;               /* a multiple of 240 */
+    uint64_t m_span;               /* the bitmap covers m in [m_base, m_base + m_span) */
     uint32_t* hits;                /* [nseg][8][nwin][cap] window offsets */
     uint32_t* hit_n;               /* [nseg][8][nwin] fill counts (zeroed before the launch) */
-    uint32_t* err;                 /* bit 0 set on a list overflow */
+    uint32_t* err;                 /* bit 0: a list overflow; bit 2: t_hl^3 <= q; bit 3: m outside the bitmap */
 } gbh_harvest_args;
+
+/* Multiplier bitmap build (reference mt_build, goldbach.cl:3181-3242): the
+ * class maps of a batch sieved over m (segment s: class arrays of nc bits
+ * starting at the common byte y0 + s nc, Q[c] = 30 (y0 + s nc) + R[c]) are
+ * interleaved into bitmap words: word w, bit 8 j + i <-> m = m_base + 240 w +
+ * 30 j + R[i], R the unit residues mod 30 ascending; bit set iff m has no
+ * prime factor among the sieving primes. Writes words [word0, word0 +
+ * nseg nc / 8) below mwords. */
+typedef struct {
+    gbh_batch b;
+    uint32_t nc;                   /* bits per class array, a multiple of 64 */
+    uint32_t pad0;
+    uint64_t* mbits;
+    uint64_t word0;
+    uint64_t mwords;
+} gbh_mtrans_args;
 
 /* Stage 3: LDS window sieve of every class array of every segment
  * (reference make_prime_map + mid_window + hl_apply, goldbach.cl:1092-1608,
@@ -243,6 +264,7 @@ int gbh_event_elapsed_ms(gbh_ctx* ctx, gbh_event* a, gbh_event* b, float* ms);
 /* ---- stage launches (all asynchronous on `stream`) ---- */
 int gbh_launch_setup(gbh_ctx* ctx, int stream, const gbh_setup_args* a);
 int gbh_launch_harvest(gbh_ctx* ctx, int stream, const gbh_harvest_args* a);
+int gbh_launch_mtrans(gbh_ctx* ctx, int stream, const gbh_mtrans_args* a);
 int gbh_launch_window(gbh_ctx* ctx, int stream, const gbh_window_args* a);
 int gbh_launch_check_a(gbh_ctx* ctx, int stream, const gbh_check_args* a);
 int gbh_launch_check_b(gbh_ctx* ctx, int stream, const gbh_checkb_args* a);

@@ -73,6 +73,10 @@ WINDOWS = [
     ("w4e18l", 467_500_000, {}),
     ("w4e18l", 20_000_000, {"GB_STATS_SAMPLE": "3"}),
     ("cfg1", 467_500_000, {}),
+    # the large-prime split: every large prime on the wheel walk (no
+    # multiplier bitmap), and a small bitmap budget (wheel below ~1.7e7)
+    ("w4e18", 2_000_000, {"GB_MT_GB": "0"}),
+    ("w44e18", 1_000_000, {"GB_MT_GB": "8"}),
 ]
 
 
