@@ -306,8 +306,8 @@ int gbh_launch_window(gbh_ctx* c, int s, const gbh_window_args* a)
 int gbh_launch_check_a(gbh_ctx* c, int s, const gbh_check_args* a)
 {
     if (const int rc = check_batch(a->b)) return rc;
-    if (a->ka > GBH_KA_MAX || a->ka % 4 || a->ka > g_min_useful)
-        return fail(GBH_E_ARG, "ka must be a multiple of 4, at most GBH_KA_MAX and every class's useful candidates");
+    if (a->ka > GBH_KA_MAX || a->ka % 16 || a->ka > g_min_useful)
+        return fail(GBH_E_ARG, "ka must be a multiple of 16, at most GBH_KA_MAX and every class's useful candidates");
     // every class array holds nc <= nwin * wbits bits; the checker's words
     // stay below that (host contract: n_gpu >= q_first + 9780)
     const uint32_t words = uint32_t((uint64_t(a->b.nwin) * a->b.wbits + 63) / 64);
@@ -316,7 +316,7 @@ int gbh_launch_check_a(gbh_ctx* c, int s, const gbh_check_args* a)
     GBH_HIP(hipGetLastError());
     // sampled segments (histogram on) and the others: one launch each, every
     // work-group leaves at once on the other kind of segment
-    const dim3 grid(grid_of(words, GBH_CHECK_WG, 1u << 30), 3, a->b.nseg);
+    const dim3 grid(grid_of(words, gbh_dev::kBlockWords, 1u << 30), a->b.nseg);
     if (a->kinds & 1u) {
         hipLaunchKernelGGL(gbh_dev::k_check_a<false>, grid, dim3(GBH_CHECK_WG), 0, S(c, s), a->b, a->coffa, a->ka,
                            a->pending, a->pcap, a->out);

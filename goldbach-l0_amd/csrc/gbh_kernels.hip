@@ -490,13 +490,24 @@ __device__ __forceinline__ ClassEvens class_evens(const gbh_geom& g, uint32_t cr
     return e;
 }
 
-constexpr uint32_t kStageW = GBH_CHECK_WG + 8;     // staged words per class array
-constexpr uint32_t kClassesPerWg = 5;              // even classes per work-group (grid.y = 3)
-constexpr uint32_t kZeroWord = GBH_CLASSES * kStageW;   // staged zero word (padding entries)
+// Phase A layout: a work-group stages words [w0, w0 + kRowW) of the eight
+// class arrays as 32-bit halves in LDS. Lane i of wave v takes class-array
+// word w0 + 63 v + i of every even class (64 evens of the class, one per
+// bit); lane 63 only lends its first dword to lane 62. For candidate j the
+// 64 bits the lane needs start kA = (entry >> 5) dwords into the staged rows
+// at bit sh = entry & 31: dwords a, b read at kA + 2 (63 v + i), the third
+// is lane i + 1's a, and two funnel shifts give the 64 bits.
+constexpr uint32_t kWaveWords = 63;
+constexpr uint32_t kBlockWords = kWaveWords * (GBH_CHECK_WG / 64);   // 252 words of every class per work-group
+constexpr uint32_t kRowW = kBlockWords + 12;                         // staged words per class row (offsets < 12 words)
+constexpr uint32_t kRowDw = 2 * kRowW;
+constexpr uint32_t kPbuf = 2048;                                     // pending evens staged per work-group
+// a candidate's offset is below (9781 + 58) / 30 bits (n0 <= n_gpu + 28 <= q_first + 9810)
+static_assert((9781 + 58) / 30 / 32 + 2 * kBlockWords + 2 <= kRowDw, "staged rows too short for the offsets");
 
 // Per segment and even class cr, candidate j:
 //   coff[(s * 15 + cr) * GBH_NCAND + j] = class << 16 | bit offset (phase B)
-//   coffa[(s * 15 + cr) * GBH_NCAND_PAD + j] = staged word << 6 | shift (phase A)
+//   coffa[(s * 15 + cr) * GBH_NCAND_PAD + j] = staged dword << 5 | shift (phase A)
 // where the bit offset is (n0 - p - Q[class]) / 30 for the class's first
 // checked even n0.
 __global__ void __launch_bounds__(256) k_coff(gbh_batch b, uint32_t* __restrict__ coff, uint32_t* __restrict__ coffa)
@@ -508,7 +519,7 @@ __global__ void __launch_bounds__(256) k_coff(gbh_batch b, uint32_t* __restrict_
     if (g.n_gpu > g.seg_end) return;
     uint32_t* pa = coffa + ((size_t)s * 15 + cr) * GBH_NCAND_PAD;
     if (j >= c_cand_n[cr]) {
-        pa[j] = kZeroWord << 6;
+        pa[j] = 0;
         return;
     }
     const ClassEvens e = class_evens(g, cr);
@@ -516,15 +527,13 @@ __global__ void __launch_bounds__(256) k_coff(gbh_batch b, uint32_t* __restrict_
     const uint32_t c = c_cand_c[cr * GBH_NCAND + j];
     const uint32_t off = (uint32_t)((e.n0 - p - g.Q[c]) / 30);   // exact and >= 0 (host contract on n_gpu)
     coff[((size_t)s * 15 + cr) * GBH_NCAND + j] = (c << 16) | off;
-    pa[j] = ((c * kStageW + (off >> 6)) << 6) | (off & 63);
+    pa[j] = ((c * kRowDw + (off >> 5)) << 5) | (off & 31);
 }
 
-// 64 class-array bits starting at staged word xw, bit sh (branch free:
-// (hi << 1) << (63 - sh) is hi << (64 - sh) for sh >= 1 and 0 for sh = 0)
-__device__ __forceinline__ uint64_t window64(const uint64_t* st, uint32_t xw, uint32_t sh)
+// lane i receives lane i + 1's value (DPP wave_shl:1; lane 63 receives 0)
+__device__ __forceinline__ uint32_t from_next_lane(uint32_t v)
 {
-    const uint64_t lo = st[xw], hi = st[xw + 1];
-    return (lo >> sh) | ((hi << 1) << (63 - sh));
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x130, 0xF, 0xF, true);
 }
 
 template <bool kHist>
@@ -532,77 +541,123 @@ __global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_batch b, const uin
                                                           uint32_t ka, uint32_t* __restrict__ pending, uint32_t pcap,
                                                           gbh_segout* __restrict__ outs)
 {
-    __shared__ uint64_t st[GBH_CLASSES * kStageW + GBH_CHECK_WG + 2];   // tail: zero words of padding entries
-    __shared__ uint32_t hl[kHist ? kClassesPerWg * GBH_KA_MAX : 1];
-    const uint32_t tid = threadIdx.x, s = blockIdx.z;
+    __shared__ uint32_t st[GBH_CLASSES * kRowDw];
+    __shared__ uint32_t pbuf[kPbuf];
+    __shared__ uint32_t hl[kHist ? GBH_EVEN_CLASSES * GBH_KA_MAX : 1];
+    __shared__ uint32_t pn, pres, pbase;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, s = blockIdx.y;
     const gbh_geom& g = b.geoms[s];
     if (g.n_gpu > g.seg_end || (g.hist_on != 0) != kHist) return;
     gbh_segout* out = outs + s;
     const uint64_t* map = seg_map(b, s);
-    const uint32_t w0 = blockIdx.x * GBH_CHECK_WG;
-    for (uint32_t i = tid; i < GBH_CLASSES * kStageW; i += GBH_CHECK_WG) {
-        const uint32_t c = i / kStageW, x = i % kStageW;
-        const uint32_t w = w0 + x;
-        st[i] = w < b.cw ? map[(size_t)c * b.cw + w] : 0ull;
-    }
-    for (uint32_t i = tid; i < GBH_CHECK_WG + 2; i += GBH_CHECK_WG) st[kZeroWord + i] = 0;
-    if (kHist)
-        for (uint32_t i = tid; i < kClassesPerWg * GBH_KA_MAX; i += GBH_CHECK_WG) hl[i] = 0;
-    __syncthreads();
-
-    uint32_t resolved = 0;
-    uint32_t* pend = pending + (size_t)s * pcap;
-    for (uint32_t k = 0; k < kClassesPerWg; ++k) {
-        const uint32_t cr = blockIdx.y * kClassesPerWg + k;
-        const ClassEvens e = class_evens(g, cr);
-        const uint32_t w = w0 + tid;
-        const uint32_t k0 = w * 64;
-        uint64_t valid = 0;
-        if (k0 < e.cnt) valid = (e.cnt - k0 >= 64) ? ~0ull : ((1ull << (e.cnt - k0)) - 1);
-        uint64_t unres = valid;
-        // ka is a multiple of 4 and at most every class's useful candidates
-        const uint4* ct = reinterpret_cast<const uint4*>(coffa + ((size_t)s * 15 + cr) * GBH_NCAND_PAD);
-        for (uint32_t j4 = 0; j4 < ka / 4 && unres; ++j4) {
-            const uint4 q4 = ct[j4];
-            const uint32_t ent[4] = {q4.x, q4.y, q4.z, q4.w};
+    const uint32_t w0 = blockIdx.x * kBlockWords;
+    // stage the rows: every load issued before the first store
+    {
+        constexpr uint32_t kN = GBH_CLASSES * kRowW, kPer = (kN + GBH_CHECK_WG - 1) / GBH_CHECK_WG;
+        uint64_t v[kPer];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const uint64_t wv = window64(st, (ent[u] >> 6) + tid, ent[u] & 63);
-                if (kHist) {
-                    const uint64_t hit = wv & unres;
-                    if (hit) atomicAdd(&hl[k * GBH_KA_MAX + j4 * 4 + u], (uint32_t)__popcll(hit));
-                }
-                unres &= ~wv;
+        for (uint32_t k = 0; k < kPer; ++k) {
+            const uint32_t i = tid + k * GBH_CHECK_WG;
+            const uint32_t c = i / kRowW, x = i % kRowW;
+            v[k] = (i < kN && w0 + x < b.cw) ? map[(size_t)c * b.cw + w0 + x] : 0ull;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            const uint32_t i = tid + k * GBH_CHECK_WG;
+            if (i < kN) {
+                const uint32_t c = i / kRowW, x = i % kRowW;
+                *reinterpret_cast<uint2*>(&st[c * kRowDw + 2 * x]) = make_uint2((uint32_t)v[k], (uint32_t)(v[k] >> 32));
             }
         }
-        resolved += (uint32_t)__popcll(valid & ~unres);
-        // pending evens of this word: one atomic per wave
-        const uint32_t u = (uint32_t)__popcll(unres);
-        uint32_t tot = 0;
-        const uint32_t pre = wave_excl_scan(u, &tot);
-        uint32_t pbase = 0;
-        if (tot) {
-            if (wave_lane() == 63) pbase = atomicAdd(&out->pending_n, tot);
-            pbase = __shfl(pbase, 63, 64);
+    }
+    if (kHist)
+        for (uint32_t i = tid; i < GBH_EVEN_CLASSES * GBH_KA_MAX; i += GBH_CHECK_WG) hl[i] = 0;
+    if (tid == 0) { pn = 0; pres = 0; }
+    __syncthreads();
+
+    const uint32_t word = w0 + wv * kWaveWords + lane;     // this lane's word of every class array
+    const uint32_t* sb = st + 2 * (wv * kWaveWords + lane);
+    uint32_t resolved = 0;
+    for (uint32_t cr = 0; cr < GBH_EVEN_CLASSES; ++cr) {
+        const ClassEvens e = class_evens(g, cr);
+        const uint32_t k0 = word * 64;
+        uint64_t valid = 0;
+        if (lane < kWaveWords && k0 < e.cnt) valid = (e.cnt - k0 >= 64) ? ~0ull : ((1ull << (e.cnt - k0)) - 1);
+        const uint32_t v0 = (uint32_t)valid, v1 = (uint32_t)(valid >> 32);
+        uint32_t u0 = v0, u1 = v1;                        // unresolved evens of the word
+        // ka is a multiple of 16 and at most every class's useful candidates
+        const uint32_t* ct = coffa + ((size_t)s * 15 + cr) * GBH_NCAND_PAD;
+        for (uint32_t j0 = 0; j0 < ka; j0 += 16) {
+            if (kHist) {
+#pragma unroll
+                for (uint32_t u = 0; u < 16; ++u) {
+                    const uint32_t ent = ct[j0 + u];
+                    const uint32_t* p = sb + (ent >> 5);
+                    const uint32_t a = p[0], bw = p[1], cw = from_next_lane(a);
+                    const uint32_t x0 = __builtin_amdgcn_alignbit(bw, a, ent & 31);
+                    const uint32_t x1 = __builtin_amdgcn_alignbit(cw, bw, ent & 31);
+                    const uint32_t h = (uint32_t)__popc(x0 & u0) + (uint32_t)__popc(x1 & u1);
+                    if (h) atomicAdd(&hl[cr * GBH_KA_MAX + j0 + u], h);
+                    u0 &= ~x0;
+                    u1 &= ~x1;
+                }
+            } else {
+                uint32_t r0 = 0, r1 = 0;
+#pragma unroll
+                for (uint32_t u = 0; u < 16; ++u) {
+                    const uint32_t ent = ct[j0 + u];
+                    const uint32_t* p = sb + (ent >> 5);
+                    const uint32_t a = p[0], bw = p[1], cw = from_next_lane(a);
+                    r0 |= __builtin_amdgcn_alignbit(bw, a, ent & 31);
+                    r1 |= __builtin_amdgcn_alignbit(cw, bw, ent & 31);
+                }
+                u0 &= ~r0;
+                u1 &= ~r1;
+            }
+            if (!__any(u0 | u1)) break;
         }
-        if (u) {
-            uint32_t pos = pbase + pre;
+        const uint64_t unres = ((uint64_t)u1 << 32) | u0;
+        resolved += (uint32_t)__popcll(valid & ~unres);
+        // pending evens: staged in LDS, one LDS atomic per wave
+        const uint32_t cntp = (uint32_t)__popcll(unres);
+        uint32_t tot = 0;
+        const uint32_t pre = wave_excl_scan(cntp, &tot);
+        if (tot) {
+            uint32_t base = 0;
+            if (lane == 63) base = atomicAdd(&pn, tot);
+            base = __shfl(base, 63, 64);
+            uint32_t pos = base + pre;
             uint64_t m = unres;
             while (m) {
                 const uint32_t bt = (uint32_t)__ffsll((long long)m) - 1;
                 m &= m - 1;
-                if (pos < pcap) pend[pos] = e.i0 + 15u * (k0 + bt);
+                const uint32_t idx = e.i0 + 15u * (k0 + bt);
+                if (pos < kPbuf) {
+                    pbuf[pos] = idx;
+                } else {                                   // overflow: straight to the segment's list
+                    const uint32_t q = atomicAdd(&out->pending_n, 1u);
+                    if (q < pcap) pending[(size_t)s * pcap + q] = idx;
+                }
                 ++pos;
             }
         }
     }
     const uint32_t rs = wave_sum(resolved);
-    if (wave_lane() == 0 && rs) atomicAdd(&out->acct[0], rs);
+    if (lane == 0 && rs) atomicAdd(&pres, rs);
+    __syncthreads();
+    const uint32_t np = min(pn, kPbuf);
+    if (tid == 0) {
+        pbase = np ? atomicAdd(&out->pending_n, np) : 0u;
+        if (pres) atomicAdd(&out->acct[0], pres);
+    }
+    __syncthreads();
+    uint32_t* pend = pending + (size_t)s * pcap;
+    for (uint32_t i = tid; i < np; i += GBH_CHECK_WG)
+        if (pbase + i < pcap) pend[pbase + i] = pbuf[i];
     if (kHist) {
-        __syncthreads();
-        for (uint32_t i = tid; i < kClassesPerWg * GBH_KA_MAX; i += GBH_CHECK_WG) {
+        for (uint32_t i = tid; i < GBH_EVEN_CLASSES * GBH_KA_MAX; i += GBH_CHECK_WG) {
             const uint32_t v = hl[i];
-            if (v) atomicAdd(&out->hist[(blockIdx.y * kClassesPerWg + i / GBH_KA_MAX) * GBH_NCAND + (i % GBH_KA_MAX)], v);
+            if (v) atomicAdd(&out->hist[(i / GBH_KA_MAX) * GBH_NCAND + (i % GBH_KA_MAX)], v);
         }
     }
 }

@@ -24,6 +24,6 @@ inline constexpr uint32_t WIN_KB_DEFAULT = 64;
 // Window primes are [67, WIN_P_DEFAULT); larger primes take the large stage.
 inline constexpr uint64_t WIN_P_DEFAULT = 1u << 21;
 // Phase-A candidates per even class (the rest go to phase B).
-inline constexpr uint32_t KA_DEFAULT = 64;
+inline constexpr uint32_t KA_DEFAULT = 96;
 
 }  // namespace gb

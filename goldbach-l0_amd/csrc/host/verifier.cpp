@@ -173,7 +173,7 @@ struct Engine::State {
         win_kb = env::u32("GB_WIN_KB", 0, 0, 128);
         if (win_kb & (win_kb - 1)) throw std::runtime_error("GB_WIN_KB must be a power of two");
         ka = env::u32("GB_KA", KA_DEFAULT, 4, GBH_KA_MAX);
-        if (ka % 4) throw std::runtime_error("GB_KA must be a multiple of 4");
+        if (ka % 16) throw std::runtime_error("GB_KA must be a multiple of 16");
         kbatch = env::u32("GB_BATCH", 16, 1, GBH_MAX_BATCH);
         // base primes (the reference loads them from its prime cache;
         // here a segmented sieve on the host threads)

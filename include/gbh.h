@@ -193,7 +193,7 @@ typedef struct {
  * every even class; the p(n) histogram on segments with hist_on. */
 typedef struct {
     gbh_batch b;
-    uint32_t ka;                   /* phase-A candidates per even class (a multiple of 4, <= GBH_KA_MAX) */
+    uint32_t ka;                   /* phase-A candidates per even class (a multiple of 16, <= GBH_KA_MAX) */
     uint32_t pcap;                 /* pending entries per segment */
     uint32_t* coff;                /* [nseg][15][GBH_NCAND] per-segment candidate table (written here) */
     uint32_t* coffa;               /* [nseg][15][GBH_NCAND_PAD] the same as staged-word offsets (written here) */
