@@ -176,12 +176,21 @@ __device__ __forceinline__ void file_hit(const HlCtx& h, const uint32_t* df, con
     if (s > 0 && d <= dl[s - 1]) file_one<kWrite>(h, s - 1, c, d, dq, cnt, hits, cap, err);   // overlap
 }
 
-// floor(x / p) for p >= 2^12 and its remainder: a double-precision quotient
-// (off by at most one here: x rounds to within 2^11 and the product to
-// 2^-52 relative) made exact by the integer remainder.
-__device__ __forceinline__ uint64_t div_big(uint64_t x, double xd, uint32_t p, uint32_t* rem)
+// 1 / p in double: the hardware reciprocal and one Newton step
+__device__ __forceinline__ double inv_of(uint32_t p)
 {
-    const double qd = xd * (1.0 / (double)p);
+    const double pd = (double)p;
+    double r = __builtin_amdgcn_rcp(pd);
+    const double e = __builtin_fma(-pd, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
+// floor(x / p) for p >= 2^12 and its remainder: a double-precision quotient
+// (within a few units here: x rounds to within 2^11, inv and the product to
+// about 2^-50 relative) made exact by the integer remainder.
+__device__ __forceinline__ uint64_t div_big(uint64_t x, double xd, uint32_t p, double inv, uint32_t* rem)
+{
+    const double qd = xd * inv;
     const uint64_t hi = (uint64_t)(uint32_t)(qd * (1.0 / 4294967296.0));
     uint64_t q = (hi << 32) + (uint64_t)(uint32_t)(qd - (double)hi * 4294967296.0);
     int64_t r = (int64_t)(x - q * p);
@@ -205,7 +214,7 @@ __device__ __forceinline__ void walk_wheel(const HlCtx& h, double qb0d, uint32_t
                                            uint32_t* __restrict__ hits, uint32_t cap, uint32_t* __restrict__ err)
 {
     uint32_t rem;
-    uint64_t m = div_big(h.qb0, qb0d, p, &rem);
+    uint64_t m = div_big(h.qb0, qb0d, p, inv_of(p), &rem);
     m += rem != 0;
     if (m < p) m = p;
     const uint32_t r = (uint32_t)(m % 30030);
@@ -231,47 +240,143 @@ __device__ __forceinline__ void walk_wheel(const HlCtx& h, double qb0d, uint32_t
     }
 }
 
-// The bitmap walk: every multiple q = p m in [qb0, qb1] with m >= p prime.
-// cmt[k] holds, 3 bits per residue index i, the class of R[k] R[i] mod 30.
-template <bool kWrite>
-__device__ __forceinline__ void walk_bitmap(const HlCtx& h, double qb0d, double qb1d, uint32_t p,
-                                            const uint32_t* df, const uint32_t* dl, const uint32_t* dq,
-                                            const uint32_t* cmt, uint32_t* cnt, uint32_t* __restrict__ hits,
-                                            uint32_t cap, uint32_t* __restrict__ err)
+// floor(x / 240) for x < 2^42: a double quotient, corrected once
+__device__ __forceinline__ uint64_t words_of(uint64_t x)
 {
+    const double v = (double)x * (1.0 / 240.0);
+    const uint32_t hi = (uint32_t)(v * (1.0 / 4294967296.0));
+    uint64_t w = ((uint64_t)hi << 32) + (uint32_t)(v - (double)hi * 4294967296.0);
+    if (w * 240 > x) --w;
+    else if ((w + 1) * 240 <= x) ++w;
+    return w;
+}
+
+// The bitmap walk: every multiple q = p m in [qb0, qb1] with m >= p prime,
+// in two steps so a lane can have the first bitmap words of several primes
+// in flight: bm_setup finds the word range, bm_run files the hits.
+// cmt[k] holds, 3 bits per residue index i, the class of R[k] R[i] mod 30.
+struct BmWalk {
+    uint64_t w0, w1;              // first and last bitmap word
+    uint64_t lo_mask, hi_mask;    // bits of m >= m0 in w0, of m <= m1 in w1
+    uint32_t p, cmap;
+    bool live;
+};
+
+__device__ __forceinline__ BmWalk bm_setup(const HlCtx& h, double qb0d, double qb1d, uint32_t p, const uint32_t* cmt,
+                                           uint32_t* __restrict__ err)
+{
+    BmWalk k;
+    k.live = false;
+    k.p = p;
+    k.w0 = k.w1 = 0;
+    k.lo_mask = k.hi_mask = 0;
+    k.cmap = 0;
+    if (!p) return k;
     const uint64_t qb1 = h.qb0 + 2ull * h.dmax;
+    const uint64_t p2 = (uint64_t)p * p;
+    if (__umul64hi(p2, p) == 0 && p2 * p <= qb1) {   // the two-factor rule does not hold: fail closed
+        atomicOr(err, 4u);
+        return k;
+    }
     uint32_t rem;
-    uint64_t m0 = div_big(h.qb0, qb0d, p, &rem);
+    uint64_t m0 = div_big(h.qb0, qb0d, p, inv_of(p), &rem);
+    // m0 p = qb0 + r0 with r0 = (p - rem) % p; the last multiplier follows
+    // from the batch length (below 2^33): m1 = m0 + floor((qb1 - qb0 - r0) / p)
+    const uint32_t r0 = rem ? p - rem : 0u;
     m0 += rem != 0;
+    const uint64_t span = 2ull * h.dmax;
+    if (span < r0) return k;
+    const uint64_t y = span - r0;
+    uint32_t t = (uint32_t)((float)y * __builtin_amdgcn_rcpf((float)p));
+    while ((uint64_t)t * p > y) --t;
+    while ((uint64_t)(t + 1) * p <= y) ++t;
+    const uint64_t m1 = m0 + t;
+    (void)qb1d;
     if (m0 < p) m0 = p;
-    const uint64_t m1 = div_big(qb1, qb1d, p, &rem);
-    if (m0 > m1) return;
+    if (m0 > m1) return k;
     if (m0 < h.m_base || m1 - h.m_base >= h.m_span) {   // outside the bitmap: fail closed
         atomicOr(err, 8u);
-        return;
+        return k;
     }
-    const uint64_t x0 = m0 - h.m_base, x1 = m1 - h.m_base;
-    const uint64_t w0 = x0 / 240, w1 = x1 / 240;
-    const uint32_t e0 = (uint32_t)(x0 - w0 * 240), e1 = (uint32_t)(x1 - w1 * 240);
+    const uint64_t x0 = m0 - h.m_base;
+    k.w0 = words_of(x0);
+    const uint32_t e0 = (uint32_t)(x0 - k.w0 * 240);
+    const uint32_t x1r = e0 + (uint32_t)(m1 - m0);            // m1 - 240 w0 - m_base, below 2^32
+    const uint32_t dw = x1r / 240;
+    k.w1 = k.w0 + dw;
+    const uint32_t e1 = x1r - dw * 240;
     const uint32_t n0 = 8 * (e0 / 30) + units_below(e0 % 30);        // bits of m < m0 in word w0 (<= 63)
     const uint32_t n1 = 8 * (e1 / 30) + units_below(e1 % 30 + 1);    // bits of m <= m1 in word w1 (0..64)
-    const uint64_t lo_mask = ~0ull << n0;
-    const uint64_t hi_mask = n1 >= 64 ? ~0ull : ((1ull << n1) - 1);
-    const uint32_t cmap = cmt[units_below(p % 30)];
-    for (uint64_t w = w0; w <= w1; ++w) {
-        uint64_t v = h.mbits[w];
-        if (w == w0) v &= lo_mask;
-        if (w == w1) v &= hi_mask;
-        if (!v) continue;
-        const uint64_t qw = (uint64_t)p * (h.m_base + 240 * w) - h.qb0;   // mod 2^64; q - qb0 >= 0 for listed bits
-        do {
-            const uint32_t bit = (uint32_t)__ffsll((long long)v) - 1;
-            v &= v - 1;
-            const uint32_t i = bit & 7;
-            const uint32_t delta = 30 * (bit >> 3) + res_of(i);
-            const uint32_t d = (uint32_t)((qw + (uint64_t)p * delta) >> 1);
-            file_hit<kWrite>(h, df, dl, dq, d, (cmap >> (3 * i)) & 7, cnt, hits, cap, err);
-        } while (v);
+    k.lo_mask = ~0ull << n0;
+    k.hi_mask = n1 >= 64 ? ~0ull : ((1ull << n1) - 1);
+    k.cmap = cmt[units_below(p % 30)];
+    k.live = true;
+    return k;
+}
+
+// v: word w0 of the bitmap (already loaded). Returns whether any bit of the
+// range was set.
+template <bool kWrite>
+__device__ __forceinline__ bool bm_run(const HlCtx& h, const BmWalk& k, uint64_t v, const uint32_t* df,
+                                       const uint32_t* dl, const uint32_t* dq, uint32_t* cnt,
+                                       uint32_t* __restrict__ hits, uint32_t cap, uint32_t* __restrict__ err)
+{
+    if (!k.live) return false;
+    bool any = false;
+    for (uint64_t w = k.w0;; ) {
+        if (w == k.w0) v &= k.lo_mask;
+        if (w == k.w1) v &= k.hi_mask;
+        if (v) {
+            any = true;
+            const uint64_t qw = (uint64_t)k.p * (h.m_base + 240 * w) - h.qb0;   // mod 2^64; q - qb0 >= 0 for listed bits
+            do {
+                const uint32_t bit = (uint32_t)__ffsll((long long)v) - 1;
+                v &= v - 1;
+                const uint32_t i = bit & 7;
+                const uint32_t delta = 30 * (bit >> 3) + res_of(i);
+                const uint32_t d = (uint32_t)((qw + (uint64_t)k.p * delta) >> 1);
+                file_hit<kWrite>(h, df, dl, dq, d, (k.cmap >> (3 * i)) & 7, cnt, hits, cap, err);
+            } while (v);
+        }
+        if (w >= k.w1) break;
+        ++w;
+        v = h.mbits[w];
+    }
+    return any;
+}
+
+// kG bitmap primes per lane at a time: primes j0 + (k + u) 256 + tid for
+// u < kG; on the writing pass only those flagged by the counting pass.
+constexpr uint32_t kG = 4;
+
+template <bool kWrite>
+__device__ __forceinline__ void bm_group(const HlCtx& h, double qb0d, double qb1d, const uint32_t* __restrict__ primes,
+                                         uint32_t j0, uint32_t j1, uint32_t k, uint32_t tid, const uint32_t* df,
+                                         const uint32_t* dl, const uint32_t* dq, const uint32_t* cmt, uint32_t* cnt,
+                                         uint64_t* any_hit, uint32_t* __restrict__ hits, uint32_t cap,
+                                         uint32_t* __restrict__ err)
+{
+    uint32_t p[kG];
+#pragma unroll
+    for (uint32_t u = 0; u < kG; ++u) {
+        const uint32_t j = j0 + (k + u) * 256 + tid;
+        bool on = j < j1;
+        if (kWrite && on) on = (any_hit[(k + u) * 4 + (tid >> 6)] >> (tid & 63)) & 1;
+        p[u] = on ? primes[j] : 0u;
+    }
+    BmWalk wk[kG];
+    uint64_t v[kG];
+#pragma unroll
+    for (uint32_t u = 0; u < kG; ++u) wk[u] = bm_setup(h, qb0d, qb1d, p[u], cmt, err);
+#pragma unroll
+    for (uint32_t u = 0; u < kG; ++u) v[u] = wk[u].live ? h.mbits[wk[u].w0] : 0ull;
+#pragma unroll
+    for (uint32_t u = 0; u < kG; ++u) {
+        const bool any = bm_run<kWrite>(h, wk[u], v[u], df, dl, dq, cnt, hits, cap, err);
+        if (!kWrite) {
+            const uint64_t bal = __ballot(any);
+            if ((tid & 63) == 0) any_hit[(k + u) * 4 + (tid >> 6)] = bal;
+        }
     }
 }
 
@@ -285,6 +390,7 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
                                                  uint32_t* __restrict__ hit_n, uint32_t* __restrict__ err)
 {
     __shared__ uint32_t df[GBH_MAX_BATCH], dl[GBH_MAX_BATCH], dq[GBH_MAX_BATCH * 8], cmt[8];
+    __shared__ uint64_t any_hit[GBH_HL_CHUNK / 64 + 4 * kG];   // bitmap primes with a set bit in range (pass 2 walks those)
     __shared__ WheelLds wl;
     extern __shared__ uint32_t cnt[];     // [nseg][8][nwin]
     const uint32_t nb = b.nseg * 8 * b.nwin;
@@ -320,15 +426,17 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
     const uint64_t qb1 = qb0 + 2ull * h.dmax;
     const double qb0d = (double)qb0, qb1d = (double)qb1;
     const uint32_t j0 = chunks[blockIdx.x], j1 = chunks[blockIdx.x + 1];
-    for (uint32_t j = j0 + tid; j < j1; j += 256) {
-        const uint32_t p = primes[j];
-        if (p < t_hl) {
-            walk_wheel<false>(h, qb0d, p, df, dl, dq, wl, cnt, hits, cap, err);
-        } else {
-            const uint64_t p2 = (uint64_t)p * p;
-            if (__umul64hi(p2, p) == 0 && p2 * p <= qb1) atomicOr(err, 4u);   // two-factor rule broken
-            else walk_bitmap<false>(h, qb0d, qb1d, p, df, dl, dq, cmt, cnt, hits, cap, err);
-        }
+    if (j1 - j0 > GBH_HL_CHUNK) {            // beyond the pass-2 flags: fail closed
+        if (tid == 0) atomicOr(err, 1u);
+        return;
+    }
+    // a chunk is all wheel primes or all bitmap primes (the host splits at t_hl)
+    const bool wheel = j0 < j1 && primes[j0] < t_hl;
+    if (wheel) {
+        for (uint32_t j = j0 + tid; j < j1; j += 256) walk_wheel<false>(h, qb0d, primes[j], df, dl, dq, wl, cnt, hits, cap, err);
+    } else {
+        for (uint32_t k = 0; j0 + k * 256 < j1; k += kG)
+            bm_group<false>(h, qb0d, qb1d, primes, j0, j1, k, tid, df, dl, dq, cmt, cnt, any_hit, hits, cap, err);
     }
     __syncthreads();
     for (uint32_t i = tid; i < nb; i += 256) {
@@ -336,14 +444,11 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
         cnt[i] = c ? atomicAdd(&hit_n[i], c) : 0u;
     }
     __syncthreads();
-    for (uint32_t j = j0 + tid; j < j1; j += 256) {
-        const uint32_t p = primes[j];
-        if (p < t_hl) {
-            walk_wheel<true>(h, qb0d, p, df, dl, dq, wl, cnt, hits, cap, err);
-        } else {
-            const uint64_t p2 = (uint64_t)p * p;
-            if (__umul64hi(p2, p) != 0 || p2 * p > qb1) walk_bitmap<true>(h, qb0d, qb1d, p, df, dl, dq, cmt, cnt, hits, cap, err);
-        }
+    if (wheel) {
+        for (uint32_t j = j0 + tid; j < j1; j += 256) walk_wheel<true>(h, qb0d, primes[j], df, dl, dq, wl, cnt, hits, cap, err);
+    } else {
+        for (uint32_t k = 0; j0 + k * 256 < j1; k += kG)
+            bm_group<true>(h, qb0d, qb1d, primes, j0, j1, k, tid, df, dl, dq, cmt, cnt, any_hit, hits, cap, err);
     }
 }
 
@@ -530,6 +635,19 @@ __global__ void __launch_bounds__(256) k_coff(gbh_batch b, uint32_t* __restrict_
     pa[j] = ((c * kRowDw + (off >> 5)) << 5) | (off & 31);
 }
 
+// Sum over the wave, in lane 63: inclusive row sums by DPP row_shr 1 2 4 8,
+// then the row totals folded by row_bcast:15 and row_bcast:31.
+__device__ __forceinline__ uint32_t wave_total63(uint32_t v)
+{
+    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x111, 0xF, 0xF, true);
+    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x112, 0xF, 0xF, true);
+    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x114, 0xF, 0xF, true);
+    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x118, 0xF, 0xF, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);
+    return v;
+}
+
 // lane i receives lane i + 1's value (DPP wave_shl:1; lane 63 receives 0)
 __device__ __forceinline__ uint32_t from_next_lane(uint32_t v)
 {
@@ -589,6 +707,9 @@ __global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_batch b, const uin
         const uint32_t* ct = coffa + ((size_t)s * 15 + cr) * GBH_NCAND_PAD;
         for (uint32_t j0 = 0; j0 < ka; j0 += 16) {
             if (kHist) {
+                // evens first resolved by candidate j0 + u, two counts per
+                // register (16 bits each: at most 64 x 64 per wave)
+                uint32_t hc[8];
 #pragma unroll
                 for (uint32_t u = 0; u < 16; ++u) {
                     const uint32_t ent = ct[j0 + u];
@@ -597,9 +718,17 @@ __global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_batch b, const uin
                     const uint32_t x0 = __builtin_amdgcn_alignbit(bw, a, ent & 31);
                     const uint32_t x1 = __builtin_amdgcn_alignbit(cw, bw, ent & 31);
                     const uint32_t h = (uint32_t)__popc(x0 & u0) + (uint32_t)__popc(x1 & u1);
-                    if (h) atomicAdd(&hl[cr * GBH_KA_MAX + j0 + u], h);
+                    hc[u / 2] = (u & 1) ? (hc[u / 2] | (h << 16)) : h;
                     u0 &= ~x0;
                     u1 &= ~x1;
+                }
+#pragma unroll
+                for (uint32_t k = 0; k < 8; ++k) {
+                    const uint32_t t = wave_total63(hc[k]);
+                    if (lane == 63 && t) {
+                        atomicAdd(&hl[cr * GBH_KA_MAX + j0 + 2 * k], t & 0xFFFFu);
+                        atomicAdd(&hl[cr * GBH_KA_MAX + j0 + 2 * k + 1], t >> 16);
+                    }
                 }
             } else {
                 uint32_t r0 = 0, r1 = 0;

@@ -432,25 +432,44 @@ void DenseRun::plan_buffers()
     E.d_scratch.ensure(c, K * size_t(swords_) * 4, 0);
     E.d_hit_n.ensure(c, K * 8 * size_t(nwin_max_) * 4, 0);
     if (nlarge_) E.d_hits.ensure(c, K * 8 * size_t(nwin_max_) * cap_ * 4, 0);
-    // harvest chunks: contiguous prime ranges with about 2e5 expected hits
-    // (the per-prime rate is nearly constant inside a chunk), 256 to 64k
-    // primes each, split where the bitmap primes start
+    // harvest chunks: contiguous prime ranges of about equal cost (a fixed
+    // cost per prime plus its expected hits; a wheel step costs about as
+    // much as a hit), enough chunks to fill every CU several times, 256 to
+    // GBH_HL_CHUNK primes each, split where the bitmap primes start
     std::vector<uint32_t> ch{0};
     if (nlarge_) {
+        const double hits_unit = double(K) * double(span_) * (8.0 / 30);   // expected hits of prime p: unit * factor / p
+        constexpr double kPrimeCost = 4.0;
+        const auto cost = [&](uint32_t a, uint32_t b) {
+            const double f = a < nwheel_ ? 1.3 * kWheelFrac : hit_factor(b - 1);
+            return kPrimeCost * (b - a) + hits_unit * f * (E.inv_prefix[b] - E.inv_prefix[a]);
+        };
+        double total = 0;
+        for (uint32_t k = 0; k < 256; ++k) {
+            uint32_t a = uint32_t(uint64_t(nlarge_) * k / 256), b = uint32_t(uint64_t(nlarge_) * (k + 1) / 256);
+            if (a < nwheel_ && b > nwheel_) { total += cost(a, nwheel_); a = nwheel_; }
+            if (b > a) total += cost(a, b);
+        }
+        const double target = std::max(total / (8.0 * E.cus), 2e4);
         uint32_t j = 0;
         while (j < nlarge_) {
-            const double per_chunk = 2e5 / (double(K) * span_ * (8.0 / 30) * hit_factor(j));
-            const double target = E.inv_prefix[j] + per_chunk;
-            uint32_t k = uint32_t(std::upper_bound(E.inv_prefix.begin() + j + 1, E.inv_prefix.begin() + nlarge_ + 1,
-                                                   target) - E.inv_prefix.begin()) - 1;
-            k = std::max(k, j + 256);
-            k = std::min({k, j + 65536, nlarge_});
-            if (j < nwheel_ && k > nwheel_) k = nwheel_;
-            ch.push_back(k);
-            j = k;
+            uint32_t hi = std::min<uint32_t>(nlarge_, j + GBH_HL_CHUNK);
+            if (j < nwheel_) hi = std::min(hi, nwheel_);
+            uint32_t lo = std::min(j + 256, hi);
+            while (lo < hi) {    // the longest chunk from j within the target
+                const uint32_t mid = lo + (hi - lo + 1) / 2;
+                if (cost(j, mid) <= target) lo = mid;
+                else hi = mid - 1;
+            }
+            ch.push_back(lo);
+            j = lo;
         }
     }
     nchunks_ = uint32_t(ch.size() - 1);
+    if (phase_trace_)
+        std::fprintf(stderr, "plan: %u segments per batch, window %u bits, %u window primes, %u large (%u on the wheel, "
+                     "bitmap from %u), %u harvest chunks, list cap %u\n", kb_, wbits_, nwin_, nlarge_, nwheel_,
+                     t_hl_ == UINT32_MAX ? 0u : t_hl_, nchunks_, cap_);
     E.h_chunks.ensure(c, ch.size() * 4, 1);
     E.d_chunks.ensure(c, ch.size() * 4, 0);
     std::memcpy(E.h_chunks.p, ch.data(), ch.size() * 4);

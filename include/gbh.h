@@ -140,6 +140,7 @@ typedef struct {
  * takes the primes [chunks[g], chunks[g + 1]). The batch's q range must stay
  * below 2^33, and nseg * 8 * nwin <= GBH_HL_BUCKETS. */
 #define GBH_HL_BUCKETS 16384
+#define GBH_HL_CHUNK 65536     /* primes per harvest work-group, at most */
 typedef struct {
     gbh_batch b;
     const uint32_t* primes;        /* large primes (device), ascending */
