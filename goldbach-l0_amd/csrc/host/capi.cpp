@@ -46,6 +46,13 @@ void fill(const gb::RunStats& s, gba_run_result* out)
             out->stats_twins = s.stats_twins;
             out->stats_sampled = s.stats_sampled;
             out->stats_tail = s.stats_tail;
+            out->harvest_kernel_seconds = s.harvest_kernel_seconds;
+            out->window_kernel_seconds = s.window_kernel_seconds;
+            out->harvest_launches = s.harvest_launches;
+            out->window_launches = s.window_launches;
+            out->harvest_bytes = s.harvest_bytes;
+            out->window_bytes = s.window_bytes;
+            out->host_plan_seconds = s.host_plan_seconds;
         }
 }
 }  // namespace
@@ -86,6 +93,19 @@ int gba_engine_run(gba_engine* e, uint64_t limit, uint64_t segment_even, uint64_
 }
 
 void gba_engine_close(gba_engine* e) { delete e; }
+
+int gba_engine_prepare(gba_engine* e, uint64_t limit, uint64_t start_even, double* seconds, char* err, size_t err_cap)
+{
+    try {
+        const double s = e->e.prepare(limit, start_even);
+        if (seconds) *seconds = s;
+        put(err, err_cap, "");
+        return 0;
+    } catch (const std::exception& ex) {
+        put(err, err_cap, ex.what());
+        return 1;
+    }
+}
 
 int gba_run_gpu(uint64_t limit, uint64_t segment_even, int verbose, uint64_t start_even, uint32_t sieve_split,
                 gba_run_result* out, char* stats_line, size_t stats_cap, char* err, size_t err_cap)

@@ -150,6 +150,12 @@ std::shared_ptr<const MultBitmap> mult_bitmap(uint64_t m_lo, uint64_t m_hi, cons
     return cur;
 }
 
+std::shared_ptr<const MultBitmap> mult_bitmap_peek()
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    return *g_cur;
+}
+
 void mult_bitmap_release()
 {
     std::lock_guard<std::mutex> lk(g_mu);

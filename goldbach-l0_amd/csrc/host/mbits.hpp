@@ -36,6 +36,9 @@ uint64_t mult_bitmap_budget();
 std::shared_ptr<const MultBitmap> mult_bitmap(uint64_t m_lo, uint64_t m_hi, const std::vector<uint32_t>& base,
                                               bool verbose);
 
+/// The cached bitmap, if any (no build).
+std::shared_ptr<const MultBitmap> mult_bitmap_peek();
+
 /// Drops the process's cached bitmap (freed once no run holds it).
 void mult_bitmap_release();
 

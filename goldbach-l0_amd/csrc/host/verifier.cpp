@@ -113,7 +113,7 @@ struct Candidates {
     }
 };
 
-enum { EV_BEGIN, EV_SETUP, EV_WINDOW, EV_CHECK, EV_COUNT, EV_END, EV_N };
+enum { EV_BEGIN, EV_HL0, EV_HL1, EV_SETUP, EV_WINDOW, EV_CHECK, EV_COUNT, EV_END, EV_N };
 
 // Knobs of removed reference pipelines are refused, not ignored: a run that
 // asks for one would otherwise silently run something else
@@ -229,6 +229,13 @@ public:
     {
     }
     RunStats run();
+    /// Range resolution and the multiplier bitmap only; the build's seconds.
+    double prepare()
+    {
+        const auto t = clock_::now();
+        resolve_range();
+        return since(t);
+    }
     /// The class map of segment [seg_start, ...] alone (validators).
     uint32_t map_of(uint64_t seg_start, std::vector<uint64_t>& words, uint64_t Q[8], uint32_t nc[8]);
 
@@ -262,6 +269,7 @@ private:
     std::optional<GoldbachStats> gs_;
     uint64_t sampled_total_ = 0, evens_done_ = 0;
     bool phase_trace_ = false;
+    bool harvested_[2] = {false, false};
     double t_sieve_ = 0, t_large_ = 0, t_check_ = 0, t_count_ = 0;
 };
 
@@ -510,6 +518,17 @@ void DenseRun::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     gbh_segout* out = E.d_out[slot].as<gbh_segout>();
     auto& ev = E.ev[slot];
     check(gbh_event_record(c, 0, ev[EV_BEGIN].e), "event");
+    harvested_[slot] = nlarge_ != 0;
+    {   // algorithmic bytes of this batch's harvest and window launches
+        const double lq = double(hg[nseg - 1].q_last - hg[0].q_first);
+        double bits = 0;
+        for (uint32_t s = 0; s < nseg; ++s)
+            for (int k = 0; k < 8; ++k) bits += hg[s].nc[k];
+        const double hits = hit_density() * bits;
+        const double s_bm = nlarge_ ? E.inv_prefix[nlarge_] - E.inv_prefix[nwheel_] : 0.0;
+        rs_.harvest_bytes += nlarge_ ? 4.0 * nlarge_ + 8.0 * ((nlarge_ - nwheel_) + lq * s_bm / 240) + 4.0 * hits : 0.0;
+        rs_.window_bytes += bits / 8 + 4.0 * nseg * 8 * nwin_ + 4.0 * hits;
+    }
     check(gbh_copy_h2d(c, 0, E.d_geoms.p, hg, sizeof(gbh_geom) * nseg), "upload geometry");
     check(gbh_copy_h2d(c, 0, E.d_cs.p, hc, sizeof(gbh_countseg) * nseg), "upload count parameters");
     check(gbh_reset_outputs(c, 0, out, nseg), "reset outputs");
@@ -522,7 +541,9 @@ void DenseRun::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         gbh_harvest_args h{b, E.d_large.as<uint32_t>(), E.d_chunks.as<uint32_t>(), nchunks_, cap_, t_hl_, 0,
                            mb_ ? mb_->bits : nullptr, mb_ ? mb_->m_base : 0, mb_ ? mb_->m_span : 0,
                            E.d_hits.as<uint32_t>(), E.d_hit_n.as<uint32_t>(), &out[0].err};
+        check(gbh_event_record(c, 0, ev[EV_HL0].e), "event");
         check(gbh_launch_harvest(c, 0, &h), "harvest");
+        check(gbh_event_record(c, 0, ev[EV_HL1].e), "event");
     }
     check(gbh_event_record(c, 0, ev[EV_SETUP].e), "event");
     gbh_window_args w{b, E.d_win.as<uint32_t>(), nwin_, std::min(ncoop_, nwin_), E.d_i0.as<uint32_t>(),
@@ -555,6 +576,12 @@ void DenseRun::collect(std::vector<SegJob>& batch, int slot)
     check(gbh_event_sync(c, ev[EV_END].e), "batch sync");
     const double tl = elapsed_s(c, ev[EV_BEGIN], ev[EV_SETUP]);
     const double ts = elapsed_s(c, ev[EV_SETUP], ev[EV_WINDOW]);
+    if (harvested_[slot]) {
+        rs_.harvest_kernel_seconds += elapsed_s(c, ev[EV_HL0], ev[EV_HL1]);
+        ++rs_.harvest_launches;
+    }
+    rs_.window_kernel_seconds += ts;
+    ++rs_.window_launches;
     const double tc = elapsed_s(c, ev[EV_WINDOW], ev[EV_CHECK]);
     const double tn = elapsed_s(c, ev[EV_CHECK], ev[EV_COUNT]);
     t_sieve_ += ts; t_large_ += tl; t_check_ += tc; t_count_ += tn;
@@ -655,9 +682,15 @@ RunStats DenseRun::run()
     t_start_ = clock_::now();
     phase_trace_ = env::present("GB_PHASE");
     resolve_range();
+    const double t_res = since(t_start_);
     configure_stats();
+    const double t_cfg = since(t_start_);
     plan_buffers();
     const auto t_gpu = clock_::now();
+    rs_.host_plan_seconds = since(t_start_);
+    if (phase_trace_)
+        std::fprintf(stderr, "host: resolve %.3f ms, statistics set-up %.3f ms, plan %.3f ms\n", t_res * 1e3,
+                     (t_cfg - t_res) * 1e3, (rs_.host_plan_seconds - t_cfg) * 1e3);
     std::vector<SegJob> batches[2];
     int cur = 0;
     bool pending_batch = false;
@@ -708,7 +741,9 @@ RunStats DenseRun::run()
         rs_.stats_tail = gs_->run_tail();
         gs_->write_live(start_, limit_, total_evens_, total_evens_, since(t_start_), true);
         rs_.stats_line = gs_->log_line(VERSION);
+        const auto tc = clock_::now();
         gs_->commit(VERSION, sampled_total_);
+        if (phase_trace_) std::fprintf(stderr, "host: statistics commit %.3f ms\n", since(tc) * 1e3);
     }
     return rs_;
 }
@@ -734,6 +769,11 @@ uint32_t DenseRun::map_of(uint64_t seg_start, std::vector<uint64_t>& words, uint
 Engine::Engine(uint64_t max_limit, uint64_t max_span) : s_(std::make_unique<State>(max_limit, max_span)) {}
 Engine::~Engine() = default;
 double Engine::init_seconds() const { return s_->init_s; }
+double Engine::prepare(uint64_t limit, uint64_t start_even)
+{
+    DenseRun r(*s_, limit, s_->max_span, false, start_even, 0, false);
+    return r.prepare();
+}
 uint64_t Engine::max_limit() const { return s_->max_limit; }
 void Engine::init_times(double& cache, double& module, double& upload) const
 {

@@ -39,6 +39,15 @@ struct RunStats {
     bool stats_on = false;
     uint64_t stats_pmax = 0, stats_primes = 0, stats_twins = 0, stats_sampled = 0, stats_tail = 0;
     std::string stats_line;      // the run's GBSTATS line (empty: stats off)
+    // device time of the hit-list harvest and window sieve kernels alone
+    // (HIP events on the run's stream around each launch), their launches
+    // and their algorithmic HBM bytes over the run (the data each must move
+    // at least once: primes, bitmap words and lists written for the
+    // harvest; tables and lists read, map written for the window sieve)
+    double harvest_kernel_seconds = 0, window_kernel_seconds = 0;
+    uint64_t harvest_launches = 0, window_launches = 0;
+    double harvest_bytes = 0, window_bytes = 0;
+    double host_plan_seconds = 0;   // range resolution, statistics set-up and buffer planning of this run
     std::vector<SegmentStats> per_segment;
 };
 
@@ -69,6 +78,10 @@ public:
     /// segment span <= max_span.
     RunStats run(uint64_t limit, uint64_t segment_even, bool verbose = false, uint64_t start_even = 4,
                  uint32_t sieve_split_arg = 0, bool collect_segments = false);
+    /// Builds the range-dependent resident tables (the multiplier bitmap)
+    /// for runs over [start_even, limit]; returns the seconds the build took
+    /// (0 when they were resident).
+    double prepare(uint64_t limit, uint64_t start_even = 4);
     double init_seconds() const;
     /// init breakdown: base primes, device context, uploads and buffers
     void init_times(double& cache, double& module, double& upload) const;

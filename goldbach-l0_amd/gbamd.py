@@ -31,6 +31,10 @@ class GbaRunResult(ctypes.Structure):
         ("split_used", ctypes.c_uint32), ("stats_on", ctypes.c_int32),
         ("stats_pmax", ctypes.c_uint64), ("stats_primes", ctypes.c_uint64), ("stats_twins", ctypes.c_uint64),
         ("stats_sampled", ctypes.c_uint64), ("stats_tail", ctypes.c_uint64),
+        ("harvest_kernel_seconds", ctypes.c_double), ("window_kernel_seconds", ctypes.c_double),
+        ("harvest_launches", ctypes.c_uint64), ("window_launches", ctypes.c_uint64),
+        ("harvest_bytes", ctypes.c_double), ("window_bytes", ctypes.c_double),
+        ("host_plan_seconds", ctypes.c_double),
     ]
 
 
@@ -67,6 +71,9 @@ def load() -> ctypes.CDLL:
         lib.gba_engine_run.restype = ctypes.c_int
         lib.gba_engine_close.argtypes = [ctypes.c_void_p]
         lib.gba_engine_close.restype = None
+        lib.gba_engine_prepare.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
+                                           ctypes.POINTER(ctypes.c_double), ctypes.c_char_p, ctypes.c_size_t]
+        lib.gba_engine_prepare.restype = ctypes.c_int
         _lib = lib
     return _lib
 
@@ -120,6 +127,15 @@ class Engine:
         out = {name: getattr(res, name) for name, _ in GbaRunResult._fields_}
         out["stats_line"] = line.value.decode()
         return out
+
+    def prepare(self, limit: int, start_even: int = 4) -> float:
+        """Builds the range-dependent resident tables for runs over
+        [start_even, limit] ahead of the first run; returns the build seconds."""
+        sec = ctypes.c_double(0)
+        err = ctypes.create_string_buffer(4096)
+        if self._lib.gba_engine_prepare(self._h, int(limit), int(start_even), ctypes.byref(sec), err, len(err)):
+            raise VerificationError(err.value.decode())
+        return sec.value
 
     def close(self):
         if self._h:

@@ -34,6 +34,13 @@ typedef struct {
     uint32_t split_used;
     int32_t stats_on;
     uint64_t stats_pmax, stats_primes, stats_twins, stats_sampled, stats_tail;
+    /* device time of the hit-list harvest and window sieve kernels alone (HIP
+     * events around each launch on the run's stream), their launches, and
+     * their algorithmic HBM bytes over the run */
+    double harvest_kernel_seconds, window_kernel_seconds;
+    uint64_t harvest_launches, window_launches;
+    double harvest_bytes, window_bytes;
+    double host_plan_seconds;
 } gba_run_result;
 
 /* Returns 0 on PASS. stats_line (if non-null, capacity stats_cap) receives
@@ -60,6 +67,15 @@ int gba_engine_open(uint64_t max_limit, uint64_t max_span, gba_engine** out, dou
 int gba_engine_run(gba_engine* e, uint64_t limit, uint64_t segment_even, uint64_t start_even, gba_run_result* out,
                    char* stats_line, size_t stats_cap, char* err, size_t err_cap);
 void gba_engine_close(gba_engine* e);
+
+/* Builds, ahead of the first run, the engine's resident tables that depend
+ * on the range (the multiplier bitmap of the large primes) for runs over
+ * [start_even, limit]; a run builds them itself when they are missing.
+ * *seconds receives the build time (0 when already resident). Counterpart of
+ * the reference's start-up table build (build_mtab, src/verifier.cpp:
+ * 1444-1508). Returns 0 on success. */
+int gba_engine_prepare(gba_engine* e, uint64_t limit, uint64_t start_even, double* seconds, char* err,
+                       size_t err_cap);
 
 /* The library's version string. */
 const char* gba_version(void);
