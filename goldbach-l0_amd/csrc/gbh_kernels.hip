@@ -262,7 +262,7 @@ struct BmWalk {
     bool live;
 };
 
-__device__ __forceinline__ BmWalk bm_setup(const HlCtx& h, double qb0d, double qb1d, uint32_t p, const uint32_t* cmt,
+__device__ __forceinline__ BmWalk bm_setup(const HlCtx& h, double qb0d, uint32_t p, const uint32_t* cmt,
                                            uint32_t* __restrict__ err)
 {
     BmWalk k;
@@ -272,43 +272,49 @@ __device__ __forceinline__ BmWalk bm_setup(const HlCtx& h, double qb0d, double q
     k.lo_mask = k.hi_mask = 0;
     k.cmap = 0;
     if (!p) return k;
-    const uint64_t qb1 = h.qb0 + 2ull * h.dmax;
-    const uint64_t p2 = (uint64_t)p * p;
-    if (__umul64hi(p2, p) == 0 && p2 * p <= qb1) {   // the two-factor rule does not hold: fail closed
-        atomicOr(err, 4u);
-        return k;
-    }
-    uint32_t rem;
-    uint64_t m0 = div_big(h.qb0, qb0d, p, inv_of(p), &rem);
-    // m0 p = qb0 + r0 with r0 = (p - rem) % p; the last multiplier follows
-    // from the batch length (below 2^33): m1 = m0 + floor((qb1 - qb0 - r0) / p)
-    const uint32_t r0 = rem ? p - rem : 0u;
-    m0 += rem != 0;
+    // m = floor(qb0 / p) from a double quotient (1 / p: the float reciprocal
+    // and one Newton step, ~2^-44 relative), made exact by the remainder
+    const double pd = (double)p;
+    double inv = (double)__builtin_amdgcn_rcpf((float)p);
+    inv = __builtin_fma(inv, __builtin_fma(-pd, inv, 1.0), inv);
+    const double fd = __builtin_floor(qb0d * inv);
+    const uint32_t fhi = (uint32_t)(fd * (1.0 / 4294967296.0));
+    uint64_t m = ((uint64_t)fhi << 32) | (uint32_t)(fd - (double)fhi * 4294967296.0);
+    int64_t r = (int64_t)(h.qb0 - m * p);
+    while (r < 0) { --m; r += p; }
+    while (r >= (int64_t)p) { ++m; r -= p; }
+    uint32_t r0 = 0;                       // m0 p = qb0 + r0
+    if (r) { ++m; r0 = p - (uint32_t)r; }
+    // further multipliers up to qb1: t = floor((span - r0) / p)
     const uint64_t span = 2ull * h.dmax;
-    if (span < r0) return k;
+    if (r0 > span) return k;
     const uint64_t y = span - r0;
-    uint32_t t = (uint32_t)((float)y * __builtin_amdgcn_rcpf((float)p));
+    uint32_t t = (uint32_t)((float)(uint32_t)(y >> 1) * 2.0f * __builtin_amdgcn_rcpf((float)p));
     while ((uint64_t)t * p > y) --t;
     while ((uint64_t)(t + 1) * p <= y) ++t;
-    const uint64_t m1 = m0 + t;
-    (void)qb1d;
-    if (m0 < p) m0 = p;
-    if (m0 > m1) return k;
-    if (m0 < h.m_base || m1 - h.m_base >= h.m_span) {   // outside the bitmap: fail closed
+    uint64_t m0 = m;
+    if (m0 < p) {                          // multipliers below p belong to smaller primes
+        if ((uint64_t)p > m0 + t) return k;
+        t -= (uint32_t)(p - m0);
+        m0 = p;
+    }
+    if (m0 < h.m_base || m0 + t - h.m_base >= h.m_span) {   // outside the bitmap: fail closed
         atomicOr(err, 8u);
         return k;
     }
-    const uint64_t x0 = m0 - h.m_base;
-    k.w0 = words_of(x0);
-    const uint32_t e0 = (uint32_t)(x0 - k.w0 * 240);
-    const uint32_t x1r = e0 + (uint32_t)(m1 - m0);            // m1 - 240 w0 - m_base, below 2^32
-    const uint32_t dw = x1r / 240;
-    k.w1 = k.w0 + dw;
-    const uint32_t e1 = x1r - dw * 240;
-    const uint32_t n0 = 8 * (e0 / 30) + units_below(e0 % 30);        // bits of m < m0 in word w0 (<= 63)
-    const uint32_t n1 = 8 * (e1 / 30) + units_below(e1 % 30 + 1);    // bits of m <= m1 in word w1 (0..64)
+    // byte and residue of m0 (2^32 = 16 mod 30), then of m1 = m0 + t
+    const uint32_t r30 = ((uint32_t)(m0 >> 32) * 16u + (uint32_t)m0 % 30u) % 30u;
+    const uint64_t y0 = ((m0 - r30 - h.m_base) >> 1) * 0xEEEEEEEEEEEEEEEFull;   // exact / 30: 15^-1 mod 2^64
+    const uint32_t e = r30 + t;
+    const uint64_t y1 = y0 + e / 30;
+    const uint32_t r1 = e % 30;
+    k.w0 = y0 >> 3;
+    k.w1 = y1 >> 3;
+    const uint32_t n0 = 8 * (uint32_t)(y0 & 7) + units_below(r30);     // bits of m < m0 in word w0 (<= 63)
+    const uint32_t n1 = 8 * (uint32_t)(y1 & 7) + units_below(r1 + 1);  // bits of m <= m1 in word w1 (0..64)
     k.lo_mask = ~0ull << n0;
     k.hi_mask = n1 >= 64 ? ~0ull : ((1ull << n1) - 1);
+    if (k.w0 == k.w1 && !(k.lo_mask & k.hi_mask)) return k;           // no multiplier coprime to 30
     k.cmap = cmt[units_below(p % 30)];
     k.live = true;
     return k;
@@ -350,7 +356,7 @@ __device__ __forceinline__ bool bm_run(const HlCtx& h, const BmWalk& k, uint64_t
 constexpr uint32_t kG = 4;
 
 template <bool kWrite>
-__device__ __forceinline__ void bm_group(const HlCtx& h, double qb0d, double qb1d, const uint32_t* __restrict__ primes,
+__device__ __forceinline__ void bm_group(const HlCtx& h, double qb0d, const uint32_t* __restrict__ primes,
                                          uint32_t j0, uint32_t j1, uint32_t k, uint32_t tid, const uint32_t* df,
                                          const uint32_t* dl, const uint32_t* dq, const uint32_t* cmt, uint32_t* cnt,
                                          uint64_t* any_hit, uint32_t* __restrict__ hits, uint32_t cap,
@@ -367,7 +373,7 @@ __device__ __forceinline__ void bm_group(const HlCtx& h, double qb0d, double qb1
     BmWalk wk[kG];
     uint64_t v[kG];
 #pragma unroll
-    for (uint32_t u = 0; u < kG; ++u) wk[u] = bm_setup(h, qb0d, qb1d, p[u], cmt, err);
+    for (uint32_t u = 0; u < kG; ++u) wk[u] = bm_setup(h, qb0d, p[u], cmt, err);
 #pragma unroll
     for (uint32_t u = 0; u < kG; ++u) v[u] = wk[u].live ? h.mbits[wk[u].w0] : 0ull;
 #pragma unroll
@@ -424,7 +430,7 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
     h.m_base = m_base;
     h.m_span = m_span;
     const uint64_t qb1 = qb0 + 2ull * h.dmax;
-    const double qb0d = (double)qb0, qb1d = (double)qb1;
+    const double qb0d = (double)qb0;
     const uint32_t j0 = chunks[blockIdx.x], j1 = chunks[blockIdx.x + 1];
     if (j1 - j0 > GBH_HL_CHUNK) {            // beyond the pass-2 flags: fail closed
         if (tid == 0) atomicOr(err, 1u);
@@ -432,11 +438,18 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
     }
     // a chunk is all wheel primes or all bitmap primes (the host splits at t_hl)
     const bool wheel = j0 < j1 && primes[j0] < t_hl;
+    if (!wheel && j0 < j1) {              // the chunk's smallest prime carries the two-factor rule
+        const uint64_t p = primes[j0], p2 = p * p;
+        if (__umul64hi(p2, p) == 0 && p2 * p <= qb1) {
+            if (tid == 0) atomicOr(err, 4u);
+            return;
+        }
+    }
     if (wheel) {
         for (uint32_t j = j0 + tid; j < j1; j += 256) walk_wheel<false>(h, qb0d, primes[j], df, dl, dq, wl, cnt, hits, cap, err);
     } else {
         for (uint32_t k = 0; j0 + k * 256 < j1; k += kG)
-            bm_group<false>(h, qb0d, qb1d, primes, j0, j1, k, tid, df, dl, dq, cmt, cnt, any_hit, hits, cap, err);
+            bm_group<false>(h, qb0d, primes, j0, j1, k, tid, df, dl, dq, cmt, cnt, any_hit, hits, cap, err);
     }
     __syncthreads();
     for (uint32_t i = tid; i < nb; i += 256) {
@@ -448,7 +461,7 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
         for (uint32_t j = j0 + tid; j < j1; j += 256) walk_wheel<true>(h, qb0d, primes[j], df, dl, dq, wl, cnt, hits, cap, err);
     } else {
         for (uint32_t k = 0; j0 + k * 256 < j1; k += kG)
-            bm_group<true>(h, qb0d, qb1d, primes, j0, j1, k, tid, df, dl, dq, cmt, cnt, any_hit, hits, cap, err);
+            bm_group<true>(h, qb0d, primes, j0, j1, k, tid, df, dl, dq, cmt, cnt, any_hit, hits, cap, err);
     }
 }
 
@@ -488,13 +501,27 @@ __global__ void __launch_bounds__(256) k_mtrans(gbh_batch b, uint32_t nc, uint64
 }
 
 // ---------------------------------------------------------------- stage 3
-// First mark >= wb0 of a prime whose first class bit is t (t may be below wb0).
-__device__ __forceinline__ uint64_t advance_to(uint32_t t, uint32_t wb0, uint32_t p)
+// Offset from wb0 of the first mark >= wb0 of a prime whose first class bit
+// is t (t may be below wb0): (t - wb0) mod p. Exact integer division, or
+// (first_off_big) a float quotient (within a few units of d / p for d <
+// 2^31, p >= 67) corrected by the remainder.
+__device__ __forceinline__ uint32_t first_off(uint32_t t, uint32_t wb0, uint32_t p)
 {
-    if (t >= wb0) return t;
+    if (t >= wb0) return t - wb0;
     const uint32_t d = wb0 - t;
-    const uint32_t k = d / p + (d % p != 0);
-    return (uint64_t)t + (uint64_t)k * p;
+    const uint32_t r = d % p;
+    return r ? p - r : 0u;
+}
+
+__device__ __forceinline__ uint32_t first_off_big(uint32_t t, uint32_t wb0, uint32_t p, float rp)
+{
+    if (t >= wb0) return t - wb0;
+    const uint32_t d = wb0 - t;
+    const uint32_t q = (uint32_t)((float)d * rp);
+    int32_t r = (int32_t)(d - q * p);
+    while (r < 0) r += (int32_t)p;
+    while (r >= (int32_t)p) r -= (int32_t)p;
+    return r ? p - (uint32_t)r : 0u;
 }
 
 __global__ void __launch_bounds__(256) k_window(gbh_batch b, const uint32_t* __restrict__ primes, uint32_t np,
@@ -539,26 +566,23 @@ __global__ void __launch_bounds__(256) k_window(gbh_batch b, const uint32_t* __r
     }
     __syncthreads();
 
-    // 2) window primes shared by the whole work-group (many marks each)
-    for (uint32_t j = 0; j < ncoop; ++j) {
+    // 2) window primes below wbits / 128 (at least 128 marks each): one
+    //    wave per prime, lane l takes marks l, l + 64, ...
+    const uint32_t lane = tid & 63, wv = tid >> 6, wbits = b.wbits;
+    for (uint32_t j = wv; j < ncoop; j += 4) {
         const uint32_t t0 = i0[j];
         if (t0 == 0xFFFFFFFFu) continue;
         const uint32_t p = primes[j];
-        const uint64_t t = advance_to(t0, wb0, p);
-        for (uint64_t m = t + (uint64_t)tid * p; m < wb1; m += 256ull * p) {
-            const uint32_t bt = (uint32_t)m - wb0;
+        for (uint32_t bt = first_off(t0, wb0, p) + lane * p; bt < wbits; bt += 64 * p)
             atomicAnd(&win32[bt >> 5], ~(1u << (bt & 31)));
-        }
     }
     // 3) the other window primes: one prime per lane
     for (uint32_t j = ncoop + tid; j < np; j += 256) {
         const uint32_t t0 = i0[j];
         if (t0 == 0xFFFFFFFFu || t0 >= wb1) continue;
         const uint32_t p = primes[j];
-        for (uint64_t m = advance_to(t0, wb0, p); m < wb1; m += p) {
-            const uint32_t bt = (uint32_t)m - wb0;
+        for (uint32_t bt = first_off_big(t0, wb0, p, __builtin_amdgcn_rcpf((float)p)); bt < wbits; bt += p)
             atomicAnd(&win32[bt >> 5], ~(1u << (bt & 31)));
-        }
     }
     // 4) the large primes' hits of this window
     if (hits) {

@@ -448,10 +448,11 @@ void DenseRun::plan_buffers()
     if (nlarge_) {
         const double hits_unit = double(K) * double(span_) * (8.0 / 30);   // expected hits of prime p: unit * factor / p
         constexpr double kPrimeCost = 4.0;
-        const auto cost = [&](uint32_t a, uint32_t b) {
-            const double f = a < nwheel_ ? 1.3 * kWheelFrac : hit_factor(b - 1);
+        const auto cost_f = [&](uint32_t a, uint32_t b, double f) {
             return kPrimeCost * (b - a) + hits_unit * f * (E.inv_prefix[b] - E.inv_prefix[a]);
         };
+        const auto factor = [&](uint32_t a) { return a < nwheel_ ? 1.3 * kWheelFrac : hit_factor(a); };
+        const auto cost = [&](uint32_t a, uint32_t b) { return cost_f(a, b, std::max(factor(a), factor(b - 1))); };
         double total = 0;
         for (uint32_t k = 0; k < 256; ++k) {
             uint32_t a = uint32_t(uint64_t(nlarge_) * k / 256), b = uint32_t(uint64_t(nlarge_) * (k + 1) / 256);
@@ -464,9 +465,10 @@ void DenseRun::plan_buffers()
             uint32_t hi = std::min<uint32_t>(nlarge_, j + GBH_HL_CHUNK);
             if (j < nwheel_) hi = std::min(hi, nwheel_);
             uint32_t lo = std::min(j + 256, hi);
+            const double f = factor(j);     // the rate barely changes inside a chunk
             while (lo < hi) {    // the longest chunk from j within the target
                 const uint32_t mid = lo + (hi - lo + 1) / 2;
-                if (cost(j, mid) <= target) lo = mid;
+                if (cost_f(j, mid, f) <= target) lo = mid;
                 else hi = mid - 1;
             }
             ch.push_back(lo);
