@@ -22,7 +22,7 @@ inline constexpr uint32_t CHECK_TAIL = 3000;                    // check30.hpp:4
 // LDS window of the window sieve: 2^19 bits = 64 KB, two work-groups per CU.
 inline constexpr uint32_t WIN_KB_DEFAULT = 64;
 // Window primes are [67, WIN_P_DEFAULT); larger primes take the large stage.
-inline constexpr uint64_t WIN_P_DEFAULT = 1u << 21;
+inline constexpr uint64_t WIN_P_DEFAULT = uint64_t(1) << 19;   // window primes below this bound
 // Phase-A candidates per even class (the rest go to phase B).
 inline constexpr uint32_t KA_DEFAULT = 96;
 

@@ -136,15 +136,22 @@ std::shared_ptr<const MultBitmap> mult_bitmap(uint64_t m_lo, uint64_t m_hi, cons
     std::shared_ptr<MultBitmap>& cur = *g_cur;
     if (cur && cur->covers(m_lo, m_hi)) return cur;
     const uint64_t budget = mult_bitmap_budget();
+    const uint64_t root = base.empty() ? 0 : base.back();
+    const auto fits = [&](uint64_t lo, uint64_t hi) {
+        return (hi - lo) / 30 + (uint64_t(1) << 23) <= budget && root * root > hi + (uint64_t(30) << 22);
+    };
+    if (!fits(m_lo, m_hi))
+        throw std::runtime_error("multiplier bitmap: m range [" + std::to_string(m_lo) + ", " + std::to_string(m_hi) +
+                                 "] exceeds the GB_MT_GB budget or the base primes");
+    if (!g_ctx) chk(gbh_open(0, &g_ctx), "gbh_open");
+    // a small range next to a large cached one: a bitmap of its own, not cached
+    if (cur && (m_hi - m_lo) / 30 < (uint64_t(1) << 30) && cur->m_span / 30 > (uint64_t(4) << 30))
+        return build(m_lo, m_hi, base, verbose);
     uint64_t lo = m_lo, hi = m_hi;
     if (cur) {
         const uint64_t lo2 = std::min(lo, cur->m_base), hi2 = std::max(hi, cur->m_base + cur->m_span - 1);
-        if ((hi2 - lo2) / 30 + (uint64_t(1) << 23) <= budget) { lo = lo2; hi = hi2; }
+        if (fits(lo2, hi2)) { lo = lo2; hi = hi2; }
     }
-    if ((hi - lo) / 30 + (uint64_t(1) << 23) > budget)
-        throw std::runtime_error("multiplier bitmap: m range [" + std::to_string(lo) + ", " + std::to_string(hi) +
-                                 "] exceeds the GB_MT_GB budget");
-    if (!g_ctx) chk(gbh_open(0, &g_ctx), "gbh_open");
     cur.reset();   // its memory goes before the new one is allocated (unless a run still holds it)
     cur = build(lo, hi, base, verbose);
     return cur;
