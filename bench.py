@@ -120,7 +120,10 @@ def main():
     import gbamd
 
     lib = hip()
-    with tempfile.TemporaryDirectory() as td, gbamd.env_scope(
+    # the run's statistics files are thrown away after each step: on tmpfs
+    # their fsyncs measure memory, not whatever disk the host has
+    shm = "/dev/shm" if os.path.isdir("/dev/shm") and os.access("/dev/shm", os.W_OK) else None
+    with tempfile.TemporaryDirectory(dir=shm) as td, gbamd.env_scope(
             GB_STATS_STATE=f"{td}/st.state", GB_STATS_LOG=f"{td}/st.log", GB_RECORDS_FILE=f"{td}/rec.txt",
             GB_STATS_LIVE=f"{td}/live.json", GB_RECORD_HOOK=""):
         eng = gbamd.Engine(LIMIT, SEGMENT)

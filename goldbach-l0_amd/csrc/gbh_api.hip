@@ -128,7 +128,7 @@ int gbh_open(int device, gbh_ctx** out)
                                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) (void)hipGetLastError();   // optional: windows up to the default limit still work
     if (hipFuncSetAttribute(reinterpret_cast<const void*>(&gbh_dev::k_harvest),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, GBH_HL_BUCKETS * 4) != hipSuccess)
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024) != hipSuccess)
         (void)hipGetLastError();
     *out = c;
     return GBH_OK;
@@ -275,9 +275,11 @@ int gbh_launch_harvest(gbh_ctx* c, int s, const gbh_harvest_args* a)
     const uint64_t nb = uint64_t(a->b.nseg) * 8 * a->b.nwin;
     if (nb > GBH_HL_BUCKETS) return fail(GBH_E_ARG, "harvest: more window lists than the work-group counts");
     if (a->m_base % 240) return fail(GBH_E_ARG, "harvest: bitmap base not a multiple of 240");
-    const size_t lds = size_t(nb) * 4;
+    if (a->stage > GBH_HL_STAGE) return fail(GBH_E_ARG, "harvest: stage above GBH_HL_STAGE");
+    const size_t lds = a->stage ? size_t(nb) * 8 + size_t(a->stage) * 6 + 16 : size_t(nb) * 4;
+    if (lds > 144 * 1024) return fail(GBH_E_ARG, "harvest: lists and stage exceed the work-group's LDS");
     hipLaunchKernelGGL(gbh_dev::k_harvest, dim3(a->nchunks), dim3(256), lds, S(c, s), a->b, a->primes, a->chunks,
-                       a->t_hl, a->mbits, a->m_base, a->m_span, a->cap, a->hits, a->hit_n, a->err);
+                       a->t_hl, a->mbits, a->m_base, a->m_span, a->cap, a->stage, a->hits, a->hit_n, a->err);
     GBH_HIP(hipGetLastError());
     return GBH_OK;
 }

@@ -139,6 +139,10 @@ struct HlCtx {
     uint32_t t_hl;           // first prime served by the bitmap
     const uint64_t* mbits;
     uint64_t m_base, m_span; // the bitmap covers m in [m_base, m_base + m_span)
+    uint32_t* stv;           // staged entries of the counting pass (nullptr: no staging)
+    uint16_t* stb;           // their lists
+    uint32_t* sn;            // entries offered to the stage
+    uint32_t scap;
 };
 
 // Files the multiple at half-offset d of class c into the window list of
@@ -153,6 +157,13 @@ __device__ __forceinline__ void file_one(const HlCtx& h, uint32_t s, uint32_t c,
     const uint32_t bk = (s * 8 + c) * h.nwin + (t >> h.lw);
     if (!kWrite) {
         atomicAdd(&cnt[bk], 1u);
+        if (h.stv) {
+            const uint32_t pos = atomicAdd(h.sn, 1u);
+            if (pos < h.scap) {
+                h.stv[pos] = t & h.wmask;
+                h.stb[pos] = (uint16_t)bk;
+            }
+        }
     } else {
         const uint32_t pos = atomicAdd(&cnt[bk], 1u);
         if (pos < cap) hits[(size_t)bk * cap + pos] = t & h.wmask;
@@ -289,9 +300,17 @@ __device__ __forceinline__ BmWalk bm_setup(const HlCtx& h, double qb0d, uint32_t
     const uint64_t span = 2ull * h.dmax;
     if (r0 > span) return k;
     const uint64_t y = span - r0;
-    uint32_t t = (uint32_t)((float)(uint32_t)(y >> 1) * 2.0f * __builtin_amdgcn_rcpf((float)p));
-    while ((uint64_t)t * p > y) --t;
-    while ((uint64_t)(t + 1) * p <= y) ++t;
+    uint32_t t = 0;
+    if (y >= p) {
+        t = (uint32_t)((float)(uint32_t)(y >> 1) * 2.0f * __builtin_amdgcn_rcpf((float)p));
+        while ((uint64_t)t * p > y) --t;
+        while ((uint64_t)(t + 1) * p <= y) ++t;
+    } else if (m >= p) {
+        // one multiplier only (the common case of a short batch): it must be
+        // a unit mod 30 (2^32 = 16 mod 30) to be in the bitmap at all
+        const uint32_t r1 = ((uint32_t)(m >> 32) * 16u + (uint32_t)m % 30u) % 30u;
+        if (!((kUnitMask >> r1) & 1u)) return k;
+    }
     uint64_t m0 = m;
     if (m0 < p) {                          // multipliers below p belong to smaller primes
         if ((uint64_t)p > m0 + t) return k;
@@ -387,19 +406,25 @@ __device__ __forceinline__ void bm_group(const HlCtx& h, double qb0d, const uint
 }
 
 // One work-group per chunk of primes; two passes over the chunk: count per
-// window list, reserve each list's share with one atomic, then write.
+// window list, reserve each list's share with one atomic, then write. With
+// `stage` slots, the counting pass also keeps the entries in LDS; when they
+// all fit, the work-group writes them from there in one burst (its runs of
+// each list complete their lines in L2) and skips the second walk.
 // A bitmap prime with p^3 <= qb1 breaks the two-factor rule: fail closed.
 __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __restrict__ primes,
                                                  const uint32_t* __restrict__ chunks, uint32_t t_hl,
                                                  const uint64_t* __restrict__ mbits, uint64_t m_base, uint64_t m_span,
-                                                 uint32_t cap, uint32_t* __restrict__ hits,
+                                                 uint32_t cap, uint32_t stage, uint32_t* __restrict__ hits,
                                                  uint32_t* __restrict__ hit_n, uint32_t* __restrict__ err)
 {
-    __shared__ uint32_t df[GBH_MAX_BATCH], dl[GBH_MAX_BATCH], dq[GBH_MAX_BATCH * 8], cmt[8];
+    __shared__ uint32_t df[GBH_MAX_BATCH], dl[GBH_MAX_BATCH], dq[GBH_MAX_BATCH * 8], cmt[8], sn;
     __shared__ uint64_t any_hit[GBH_HL_CHUNK / 64 + 4 * kG];   // bitmap primes with a set bit in range (pass 2 walks those)
     __shared__ WheelLds wl;
-    extern __shared__ uint32_t cnt[];     // [nseg][8][nwin]
+    extern __shared__ uint32_t cnt[];     // [nseg][8][nwin] counts; with a stage: run counters, the stage
     const uint32_t nb = b.nseg * 8 * b.nwin;
+    uint32_t* runc = cnt + nb;
+    uint32_t* stv = runc + nb;
+    uint16_t* stb = reinterpret_cast<uint16_t*>(stv + stage);
     const uint32_t tid = threadIdx.x;
     if (tid < 210) { wl.next[tid] = c_w210_next[tid]; wl.idx[tid] = c_w210_idx[tid]; }
     if (tid < 48) wl.inc[tid] = c_w210_inc[tid];
@@ -415,9 +440,17 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
         dl[s] = (uint32_t)((b.geoms[s].q_last - qb0) >> 1);
     }
     for (uint32_t i = tid; i < b.nseg * 8; i += 256) dq[i] = (uint32_t)((b.geoms[i / 8].Q[i % 8] - qb0) >> 1);
-    for (uint32_t i = tid; i < nb; i += 256) cnt[i] = 0;
+    for (uint32_t i = tid; i < nb; i += 256) {
+        cnt[i] = 0;
+        if (stage) runc[i] = 0;
+    }
+    if (tid == 0) sn = 0;
     __syncthreads();
     HlCtx h;
+    h.stv = stage ? stv : nullptr;
+    h.stb = stb;
+    h.sn = &sn;
+    h.scap = stage;
     h.qb0 = qb0;
     h.dmax = dl[b.nseg - 1];
     h.hs_magic = (b.nseg > 1 && df[1] > df[0]) ? (uint32_t)((1ull << 32) / (df[1] - df[0] + 1)) : 0u;
@@ -457,6 +490,16 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
         cnt[i] = c ? atomicAdd(&hit_n[i], c) : 0u;
     }
     __syncthreads();
+    if (stage && sn <= stage) {          // every entry is in the stage: write it from there
+        const uint32_t n = sn;
+        for (uint32_t i = tid; i < n; i += 256) {
+            const uint32_t bk = stb[i];
+            const uint32_t pos = cnt[bk] + atomicAdd(&runc[bk], 1u);
+            if (pos < cap) hits[(size_t)bk * cap + pos] = stv[i];
+            else atomicOr(err, 1u);
+        }
+        return;
+    }
     if (wheel) {
         for (uint32_t j = j0 + tid; j < j1; j += 256) walk_wheel<true>(h, qb0d, primes[j], df, dl, dq, wl, cnt, hits, cap, err);
     } else {

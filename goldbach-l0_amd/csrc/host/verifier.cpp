@@ -153,7 +153,15 @@ struct Engine::State {
     uint32_t win_lo = 0, win_hi = 0;     // window primes base[win_lo, win_hi), large base[win_hi, ...)
     uint32_t ka = 0, kbatch = 0, win_kb = 0;
     double init_s = 0, cache_s = 0, module_s = 0, upload_s = 0;
-    std::vector<double> inv_prefix;      // prefix sums of 1/p over the large primes
+    std::vector<double> inv_coarse;      // sum of 1/p over the first 256 k large primes
+    /// sum of 1/p over the first j large primes
+    double inv_at(uint32_t j) const
+    {
+        const uint32_t k = j >> 8;
+        double s = inv_coarse[k];
+        for (uint32_t i = k << 8; i < j; ++i) s += 1.0 / base[win_hi + i];
+        return s;
+    }
     std::optional<Ctx> ctx;
     gbh_ctx* c = nullptr;
     int cus = 256;
@@ -183,8 +191,16 @@ struct Engine::State {
         win_lo = uint32_t(std::lower_bound(base.begin(), base.end(), 67u) - base.begin());
         const uint64_t win_p = env::u64("GB_WIN_P", WIN_P_DEFAULT, 67, UINT32_MAX);
         win_hi = std::max(win_lo, uint32_t(std::lower_bound(base.begin(), base.end(), win_p) - base.begin()));
-        inv_prefix.assign(base.size() - win_hi + 1, 0.0);
-        for (size_t i = win_hi; i < base.size(); ++i) inv_prefix[i - win_hi + 1] = inv_prefix[i - win_hi] + 1.0 / base[i];
+        {
+            const size_t nl = base.size() - win_hi;
+            inv_coarse.assign(nl / 256 + 2, 0.0);
+            double acc = 0;
+            for (size_t i = 0; i < nl; ++i) {
+                if (i % 256 == 0) inv_coarse[i / 256] = acc;
+                acc += 1.0 / base[win_hi + i];
+            }
+            for (size_t k = (nl + 255) / 256; k < inv_coarse.size(); ++k) inv_coarse[k] = acc;
+        }
         const auto tm = clock_::now();
         ctx.emplace();
         c = ctx->c;
@@ -246,6 +262,8 @@ private:
     void plan_buffers();
     double hit_factor(uint32_t j) const;
     double hit_density() const;
+    double compute_hit_density() const;
+    double hit_density_ = 0;
     void enqueue(std::vector<SegJob>& batch, int slot, bool check_and_count);
     void collect(std::vector<SegJob>& batch, int slot);
     void collect_segment(const SegJob& j, const gbh_segout& o, double share[4]);
@@ -263,6 +281,8 @@ private:
     uint32_t wbits_ = 0, nwin_max_ = 0, cw_ = 0, cap_ = 0, pcap_ = 0, swords_ = 0, nchunks_ = 0, kb_ = 1;
     // large primes: [0, nwheel_) walk the wheel, [nwheel_, nlarge_) read the multiplier bitmap from t_hl_ up
     uint32_t nwheel_ = 0, t_hl_ = UINT32_MAX;
+    uint32_t stage_ = 0;       // harvest LDS stage per work-group (short batches: one walk per prime)
+    double inv_wheel_ = 0, inv_large_ = 0;   // sum of 1/p over the wheel primes, over all large primes
     std::shared_ptr<const MultBitmap> mb_;
     bool stats_on_ = false, gaps_on_ = false, gaps_exact_ = false;
     uint32_t sample_ = 256;
@@ -384,15 +404,17 @@ double DenseRun::hit_factor(uint32_t j) const
 // Expected hits per class bit over all large primes: the wheel primes
 // exactly, the bitmap primes in 64 bands of the prime index, each at its
 // largest rate.
-double DenseRun::hit_density() const
+double DenseRun::hit_density() const { return hit_density_; }
+
+double DenseRun::compute_hit_density() const
 {
     if (!nlarge_) return 0.0;
-    double d = kWheelFrac * E.inv_prefix[nwheel_];
+    double d = kWheelFrac * E.inv_at(nwheel_);
     const uint32_t n = nlarge_ - nwheel_;
     for (uint32_t k = 0; k < 64 && n; ++k) {
         const uint32_t a = nwheel_ + uint32_t(uint64_t(n) * k / 64), b = nwheel_ + uint32_t(uint64_t(n) * (k + 1) / 64);
         if (b <= a) continue;
-        d += (E.inv_prefix[b] - E.inv_prefix[a]) * std::max(hit_factor(a), hit_factor(b - 1));
+        d += (E.inv_at(b) - E.inv_at(a)) * std::max(hit_factor(a), hit_factor(b - 1));
     }
     return d;
 }
@@ -422,6 +444,7 @@ void DenseRun::plan_buffers()
     cw_ = nwin_max_ * (wbits_ / 64) + 16;
     ncoop_ = 0;
     while (ncoop_ < nwin_ && E.base[E.win_lo + ncoop_] < wbits_ / 128) ++ncoop_;
+    hit_density_ = compute_hit_density();
     const double lam = wbits_ * hit_density();
     cap_ = uint32_t(std::min(lam * 1.15 + 8 * std::sqrt(lam) + 1024, 4e9 / (double(E.kbatch) * 8 * nwin_max_)));
     const uint64_t evens_max = span_ + 2;
@@ -440,46 +463,79 @@ void DenseRun::plan_buffers()
     E.d_scratch.ensure(c, K * size_t(swords_) * 4, 0);
     E.d_hit_n.ensure(c, K * 8 * size_t(nwin_max_) * 4, 0);
     if (nlarge_) E.d_hits.ensure(c, K * 8 * size_t(nwin_max_) * cap_ * 4, 0);
+    const auto tp0 = clock_::now();
     // harvest chunks: contiguous prime ranges of about equal cost (a fixed
     // cost per prime plus its expected hits; a wheel step costs about as
     // much as a hit), enough chunks to fill every CU several times, 256 to
     // GBH_HL_CHUNK primes each, split where the bitmap primes start
     std::vector<uint32_t> ch{0};
+    inv_wheel_ = E.inv_at(nwheel_);
+    inv_large_ = E.inv_at(nlarge_);
     if (nlarge_) {
         const double hits_unit = double(K) * double(span_) * (8.0 / 30);   // expected hits of prime p: unit * factor / p
         constexpr double kPrimeCost = 4.0;
-        const auto cost_f = [&](uint32_t a, uint32_t b, double f) {
-            return kPrimeCost * (b - a) + hits_unit * f * (E.inv_prefix[b] - E.inv_prefix[a]);
+        // sum of 1/p up to j: exact at the split and the end, coarse (every
+        // 256 primes) elsewhere, where chunk ends fall
+        const auto inv = [&](uint32_t j) {
+            return (j & 255) == 0 ? E.inv_coarse[j >> 8]
+                 : j == nwheel_ ? inv_wheel_ : j == nlarge_ ? inv_large_ : E.inv_at(j);
         };
         const auto factor = [&](uint32_t a) { return a < nwheel_ ? 1.3 * kWheelFrac : hit_factor(a); };
-        const auto cost = [&](uint32_t a, uint32_t b) { return cost_f(a, b, std::max(factor(a), factor(b - 1))); };
+        const auto hits_f = [&](uint32_t a) { return a < nwheel_ ? kWheelFrac : hit_factor(a); };
         double total = 0;
         for (uint32_t k = 0; k < 256; ++k) {
-            uint32_t a = uint32_t(uint64_t(nlarge_) * k / 256), b = uint32_t(uint64_t(nlarge_) * (k + 1) / 256);
-            if (a < nwheel_ && b > nwheel_) { total += cost(a, nwheel_); a = nwheel_; }
-            if (b > a) total += cost(a, b);
+            uint32_t a = uint32_t(uint64_t(nlarge_) * k / 256) & ~255u, b = uint32_t(uint64_t(nlarge_) * (k + 1) / 256) & ~255u;
+            if (k == 255) b = nlarge_;
+            if (a < nwheel_ && b > nwheel_) {
+                total += kPrimeCost * (nwheel_ - a) + hits_unit * factor(a) * (inv(nwheel_) - inv(a));
+                a = nwheel_;
+            }
+            if (b > a) total += kPrimeCost * (b - a) + hits_unit * std::max(factor(a), factor(b - 1)) * (inv(b) - inv(a));
         }
         const double target = std::max(total / (8.0 * E.cus), 2e4);
+        // a short batch's hits fit the work-groups' LDS stages: chunks of at
+        // most kStageHits expected hits, one walk per prime
+        constexpr double kStageHits = 0.7 * GBH_HL_STAGE;
+        const double all_hits = hits_unit * hit_density();
+        stage_ = all_hits <= kStageHits * 8192 ? GBH_HL_STAGE : 0u;
         uint32_t j = 0;
         while (j < nlarge_) {
             uint32_t hi = std::min<uint32_t>(nlarge_, j + GBH_HL_CHUNK);
             if (j < nwheel_) hi = std::min(hi, nwheel_);
-            uint32_t lo = std::min(j + 256, hi);
-            const double f = factor(j);     // the rate barely changes inside a chunk
-            while (lo < hi) {    // the longest chunk from j within the target
-                const uint32_t mid = lo + (hi - lo + 1) / 2;
-                if (cost_f(j, mid, f) <= target) lo = mid;
-                else hi = mid - 1;
+            const double f = factor(j), fh = hits_f(j), ij = j == nwheel_ ? inv_wheel_ : E.inv_at(j);
+            const auto ok = [&](uint32_t e) {   // chunk [j, e) within the targets (the rates barely change inside)
+                const double si = inv(e) - ij;
+                return kPrimeCost * (e - j) + hits_unit * f * si <= target && (!stage_ || hits_unit * fh * si <= kStageHits);
+            };
+            uint32_t e;
+            if (ok(hi)) {
+                e = hi;
+            } else {   // the last multiple of 256 that keeps the chunk within the targets (at least 256 primes)
+                uint32_t lo = ((j >> 8) + 1) << 8, up = ((hi - 1) >> 8) << 8;
+                if (lo > up) {
+                    e = hi;
+                } else {
+                    if (lo < j + 256) lo += 256;
+                    if (lo > up) lo = up;
+                    while (lo < up) {
+                        const uint32_t mid = ((lo >> 8) + ((up >> 8) - (lo >> 8) + 1) / 2) << 8;
+                        if (ok(mid)) lo = mid;
+                        else up = mid - 256;
+                    }
+                    e = lo;
+                }
             }
-            ch.push_back(lo);
-            j = lo;
+            ch.push_back(e);
+            j = e;
         }
     }
+    const double t_chunks = since(tp0);
     nchunks_ = uint32_t(ch.size() - 1);
     if (phase_trace_)
         std::fprintf(stderr, "plan: %u segments per batch, window %u bits, %u window primes, %u large (%u on the wheel, "
-                     "bitmap from %u), %u harvest chunks, list cap %u\n", kb_, wbits_, nwin_, nlarge_, nwheel_,
-                     t_hl_ == UINT32_MAX ? 0u : t_hl_, nchunks_, cap_);
+                     "bitmap from %u), %u harvest chunks, list cap %u, stage %u\n", kb_, wbits_, nwin_, nlarge_,
+                     nwheel_, t_hl_ == UINT32_MAX ? 0u : t_hl_, nchunks_, cap_, stage_);
+    if (phase_trace_) std::fprintf(stderr, "plan: chunking %.3f ms\n", t_chunks * 1e3);
     E.h_chunks.ensure(c, ch.size() * 4, 1);
     E.d_chunks.ensure(c, ch.size() * 4, 0);
     std::memcpy(E.h_chunks.p, ch.data(), ch.size() * 4);
@@ -527,7 +583,7 @@ void DenseRun::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         for (uint32_t s = 0; s < nseg; ++s)
             for (int k = 0; k < 8; ++k) bits += hg[s].nc[k];
         const double hits = hit_density() * bits;
-        const double s_bm = nlarge_ ? E.inv_prefix[nlarge_] - E.inv_prefix[nwheel_] : 0.0;
+        const double s_bm = nlarge_ ? inv_large_ - inv_wheel_ : 0.0;
         rs_.harvest_bytes += nlarge_ ? 4.0 * nlarge_ + 8.0 * ((nlarge_ - nwheel_) + lq * s_bm / 240) + 4.0 * hits : 0.0;
         rs_.window_bytes += bits / 8 + 4.0 * nseg * 8 * nwin_ + 4.0 * hits;
     }
@@ -540,7 +596,7 @@ void DenseRun::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     }
     if (nlarge_) {
         check(gbh_zero(c, 0, E.d_hit_n.p, size_t(nseg) * 8 * nwin * 4), "zero hit counts");
-        gbh_harvest_args h{b, E.d_large.as<uint32_t>(), E.d_chunks.as<uint32_t>(), nchunks_, cap_, t_hl_, 0,
+        gbh_harvest_args h{b, E.d_large.as<uint32_t>(), E.d_chunks.as<uint32_t>(), nchunks_, cap_, t_hl_, stage_,
                            mb_ ? mb_->bits : nullptr, mb_ ? mb_->m_base : 0, mb_ ? mb_->m_span : 0,
                            E.d_hits.as<uint32_t>(), E.d_hit_n.as<uint32_t>(), &out[0].err};
         check(gbh_event_record(c, 0, ev[EV_HL0].e), "event");

@@ -139,8 +139,9 @@ typedef struct {
  * and the bitmap covers every multiplier asked for (err bit 3). Work-group g
  * takes the primes [chunks[g], chunks[g + 1]). The batch's q range must stay
  * below 2^33, and nseg * 8 * nwin <= GBH_HL_BUCKETS. */
-#define GBH_HL_BUCKETS 16384
+#define GBH_HL_BUCKETS 12288
 #define GBH_HL_CHUNK 65536     /* primes per harvest work-group, at most */
+#define GBH_HL_STAGE 4096      /* staged entries per harvest work-group, at most */
 typedef struct {
     gbh_batch b;
     const uint32_t* primes;        /* large primes (device), ascending */
@@ -148,7 +149,7 @@ typedef struct {
     uint32_t nchunks;
     uint32_t cap;                  /* entries per window list */
     uint32_t t_hl;                 /* first prime served by the bitmap */
-    uint32_t pad0;
+    uint32_t stage;                /* LDS entries a work-group may stage (0: always two walks), <= GBH_HL_STAGE */
     const uint64_t* mbits;         /* multiplier bitmap (gbh_launch_mtrans layout) */
     uint64_t m_base;               /* a multiple of 240 */
     uint64_t m_span;               /* the bitmap covers m in [m_base, m_base + m_span) */
