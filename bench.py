@@ -125,7 +125,7 @@ def main():
     shm = "/dev/shm" if os.path.isdir("/dev/shm") and os.access("/dev/shm", os.W_OK) else None
     with tempfile.TemporaryDirectory(dir=shm) as td, gbamd.env_scope(
             GB_STATS_STATE=f"{td}/st.state", GB_STATS_LOG=f"{td}/st.log", GB_RECORDS_FILE=f"{td}/rec.txt",
-            GB_STATS_LIVE=f"{td}/live.json", GB_RECORD_HOOK=""):
+            GB_STATS_LIVE=f"{td}/live.json", GB_RIDGE_FILE=f"{td}/ridges.csv", GB_RECORD_HOOK=""):
         eng = gbamd.Engine(LIMIT, SEGMENT)
         prep_s = eng.prepare(LIMIT, START)
 

@@ -860,7 +860,7 @@ __global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_batch b, const uin
 
 __global__ void __launch_bounds__(256) k_check_b(gbh_batch b, const uint32_t* __restrict__ coff, uint32_t ka,
                                                  uint32_t tail_t, const uint32_t* __restrict__ pending, uint32_t pcap,
-                                                 gbh_segout* __restrict__ outs)
+                                                 uint32_t ridge_thresh, gbh_segout* __restrict__ outs)
 {
     const uint32_t s = blockIdx.y;
     const gbh_geom& g = b.geoms[s];
@@ -885,7 +885,15 @@ __global__ void __launch_bounds__(256) k_check_b(gbh_batch b, const uint32_t* __
         }
         if (jf != 0xFFFFFFFFu) {
             atomicAdd(&out->hist[cr * GBH_NCAND + jf], 1u);
-            if (c_cand_p[cr * GBH_NCAND + jf] >= tail_t) atomicMin(&out->first[c_cand_pi[cr * GBH_NCAND + jf]], i);
+            const uint32_t pj = c_cand_p[cr * GBH_NCAND + jf];
+            if (pj >= tail_t) atomicMin(&out->first[c_cand_pi[cr * GBH_NCAND + jf]], i);
+            if (pj >= ridge_thresh) {
+                const uint32_t sl = atomicAdd(&out->ridge_n, 1u);
+                if (sl < GBH_RIDGE_CAP) {
+                    out->ridge[2 * sl] = n;
+                    out->ridge[2 * sl + 1] = pj;
+                }
+            }
             atomicAdd(&out->acct[1], 1u);
         } else {
             const uint32_t sl = atomicAdd(&out->left_n, 1u);
@@ -1028,7 +1036,7 @@ __global__ void __launch_bounds__(256) k_reset_out(gbh_segout* __restrict__ outs
     const size_t n = offsetof(gbh_segout, left) / 4;                // the leftover list needs no reset
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const size_t byte = i * 4;
-        const bool ones = (byte >= offsetof(gbh_segout, first) && byte < offsetof(gbh_segout, left)) ||
+        const bool ones = (byte >= offsetof(gbh_segout, first) && byte < offsetof(gbh_segout, ridge_n)) ||
                           byte == offsetof(gbh_segout, pfirst);
         w[i] = ones ? 0xFFFFFFFFu : 0u;
     }

@@ -108,11 +108,18 @@ void print_json(const gb::RunStats& s, uint64_t segment, uint32_t split)
                 "  \"sieve_seconds\": %.4f,\n  \"large_seconds\": %.4f,\n  \"verify_seconds\": %.4f,\n"
                 "  \"offset_seconds\": %.4f,\n  \"table_seconds\": %.4f,\n  \"sync_seconds\": %.4f,\n"
                 "  \"evens_per_second\": %.6g,\n  \"total_tiles\": %llu,\n  \"total_large_items\": %llu,\n"
-                "  \"ridge_min\": %u,\n  \"ridge_found\": %llu,\n  \"ridge_dropped\": %llu,\n  \"ridge_top\": []",
+                "  \"ridge_min\": %u,\n  \"ridge_found\": %llu,\n  \"ridge_dropped\": %llu,\n  \"ridge_top\": [",
                 s.init_seconds, s.gpu_seconds, s.total_seconds, s.sieve_seconds, s.large_seconds, s.verify_seconds,
                 s.offset_seconds, s.table_seconds, s.sync_seconds, s.evens_per_second,
                 (unsigned long long)s.total_tiles, (unsigned long long)s.total_large_items, s.ridge_min,
                 (unsigned long long)s.ridge_found, (unsigned long long)s.ridge_dropped);
+    for (size_t i = 0; i < s.ridge_top.size(); ++i) {
+        const auto& e = s.ridge_top[i];
+        std::printf("%s\n    {\"rank\": %zu, \"ridge\": %llu, \"even\": %llu, \"p\": %llu, \"q\": %llu, "
+                    "\"segment\": %llu}", i ? "," : "", i + 1, (unsigned long long)e.p, (unsigned long long)e.n,
+                    (unsigned long long)e.p, (unsigned long long)e.q, (unsigned long long)e.seg);
+    }
+    std::printf("%s]", s.ridge_top.empty() ? "" : "\n  ");
     std::printf(",\n  \"sysman\": {\"status\": \"off\", \"window_s\": 0.0000}");
     if (s.stats_on)
         std::printf(",\n  \"stats\": {\"tail_evens\": %llu, \"sampled_evens\": %llu, \"pmax\": %llu, \"primes\": %llu, "
@@ -135,6 +142,23 @@ void print_json(const gb::RunStats& s, uint64_t segment, uint32_t split)
         std::printf("  ]");
     }
     std::printf("\n}\n");
+}
+
+// The persistent ridge leaderboard (the reference's print_ridges,
+// src/main.cpp:244-256): the largest recorded ridges, descending.
+void print_ridges(const gb::RunStats& s)
+{
+    if (!s.ridge_min) return;
+    std::printf("\nMy Top 30 Largest Goldbach Ridges\n");
+    if (s.ridge_top.empty()) {
+        std::printf("  (none recorded at or above the floor %u yet)\n", s.ridge_min);
+        return;
+    }
+    for (size_t i = 0; i < s.ridge_top.size(); ++i) {
+        const auto& e = s.ridge_top[i];
+        std::printf("  %2zu. ridge %-5llu  %llu = %llu + %llu\n", i + 1, (unsigned long long)e.p,
+                    (unsigned long long)e.n, (unsigned long long)e.q, (unsigned long long)e.p);
+    }
 }
 
 void print_per_segment(const gb::RunStats& s)
@@ -246,6 +270,7 @@ int main(int argc, char** argv)
                         (unsigned long long)s.stats_pmax, (unsigned long long)s.stats_tail,
                         (unsigned long long)s.stats_sampled, (unsigned long long)s.stats_primes,
                         (unsigned long long)s.stats_twins);
+        if (!csv && !json) print_ridges(s);   // --json / --csv carry ridge_top / ridge_max
         if (per_seg && !json) print_per_segment(s);
         return 0;
     } catch (const std::exception& e) {

@@ -24,6 +24,7 @@
 #include "gstats.hpp"
 #include "mbits.hpp"
 #include "primes.hpp"
+#include "ridge.hpp"
 
 namespace gb {
 namespace {
@@ -259,6 +260,7 @@ private:
     void resolve_range();
     void plan_bitmap();
     void configure_stats();
+    void configure_ridges();
     void plan_buffers();
     double hit_factor(uint32_t j) const;
     double hit_density() const;
@@ -287,6 +289,8 @@ private:
     bool stats_on_ = false, gaps_on_ = false, gaps_exact_ = false;
     uint32_t sample_ = 256;
     std::optional<GoldbachStats> gs_;
+    std::optional<RidgeTracker> ridges_;
+    std::string ridge_path_;
     uint64_t sampled_total_ = 0, evens_done_ = 0;
     bool phase_trace_ = false;
     bool harvested_[2] = {false, false};
@@ -365,6 +369,23 @@ void DenseRun::plan_bitmap()
     const uint64_t m_lo = std::max<uint64_t>(t_hl_, sq_lo > 1 ? sq_lo - 1 : 1), m_hi = q_hi / t_hl_;
     if (m_hi < m_lo) { nwheel_ = nlarge_; t_hl_ = UINT32_MAX; return; }
     mb_ = mult_bitmap(m_lo, m_hi, E.base, verbose_ || phase_trace_);
+}
+
+// Goldbach ridges (the reference's configure_ridges_and_stats,
+// src/verifier.cpp:718-744): the floor is clamped above every phase-A
+// candidate, so above it phase B sees every even; the store is loaded after
+// the clamp.
+void DenseRun::configure_ridges()
+{
+    ridges_.emplace(&E.cand.primes);
+    ridge_path_ = env::path("GB_RIDGE_FILE", "goldbach_ridges.csv");
+    uint32_t pa_max = 0;
+    for (uint32_t cr = 0; cr < 15; ++cr)
+        pa_max = std::max(pa_max, E.cand.primes[E.cand.slot_pi[cr][std::min<size_t>(E.ka, E.cand.slot_pi[cr].size()) - 1]]);
+    if (ridges_->active()) {
+        ridges_->clamp_floor_above(pa_max);
+        ridges_->load(ridge_path_);
+    }
 }
 
 void DenseRun::configure_stats()
@@ -614,7 +635,8 @@ void DenseRun::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         gbh_check_args ca{b, E.ka, pcap_, E.d_coff.as<uint32_t>(), E.d_coffa.as<uint32_t>(),
                           E.d_pending.as<uint32_t>(), out, kinds, 0};
         check(gbh_launch_check_a(c, 0, &ca), "check phase A");
-        gbh_checkb_args cb{b, E.ka, CHECK_TAIL, E.d_coff.as<uint32_t>(), E.d_pending.as<uint32_t>(), pcap_, 0, out};
+        gbh_checkb_args cb{b, E.ka, CHECK_TAIL, E.d_coff.as<uint32_t>(), E.d_pending.as<uint32_t>(), pcap_,
+                           ridges_ ? ridges_->thresh() : 0xFFFFFFFFu, out};
         check(gbh_launch_check_b(c, 0, &cb), "check phase B");
     }
     check(gbh_event_record(c, 0, ev[EV_CHECK].e), "event");
@@ -684,8 +706,10 @@ void DenseRun::collect_segment(const SegJob& j, const gbh_segout& o, double shar
             std::fprintf(stderr, "note: n=%llu needs p=%llu > MAX_GB_PRIME (new p_min record candidate)\n",
                          (unsigned long long)n, (unsigned long long)p);
         if (p > 2 && gs_) gs_->add_cpu(n, p, j.sampled);
+        if (p > 2 && ridges_) ridges_->consider(n, p, j.sg.seg_start);
     }
     rs_.leftovers += o.left_n;
+    if (ridges_) ridges_->merge(o.ridge, o.ridge_n, GBH_RIDGE_CAP, j.sg.seg_start);
     if (gs_) {
         const size_t np = E.cand.primes.size();
         std::vector<uint64_t> h(15 * np, 0);
@@ -742,6 +766,7 @@ RunStats DenseRun::run()
     resolve_range();
     const double t_res = since(t_start_);
     configure_stats();
+    configure_ridges();
     const double t_cfg = since(t_start_);
     plan_buffers();
     const auto t_gpu = clock_::now();
@@ -789,6 +814,16 @@ RunStats DenseRun::run()
     rs_.count_seconds = t_count_;
     rs_.sync_seconds = std::max(0.0, rs_.gpu_seconds - t_sieve_ - t_large_ - t_check_ - t_count_);
     rs_.start = start_;
+    if (ridges_ && ridges_->active()) {
+        ridges_->save(ridge_path_);
+        rs_.ridge_min = ridges_->floor();
+        rs_.ridge_found = ridges_->appends();
+        rs_.ridge_dropped = ridges_->dropped();
+        rs_.ridge_top = ridges_->top();
+        if (ridges_->overflowed())
+            std::fprintf(stderr, "warning: a segment recorded more than %u ridges at or above the threshold; the "
+                         "excess was not considered\n", GBH_RIDGE_CAP);
+    }
     if (gs_) {
         gs_->finish_run(start_, limit_, total_evens_);
         rs_.stats_on = true;

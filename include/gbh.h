@@ -54,6 +54,7 @@ enum {
 #define GBH_LEFT_CAP 65536       /* leftovers per segment copied back */
 #define GBH_CHECK_WG 256         /* check_a work-group: 256 lanes x 64 evens */
 #define GBH_KA_MAX 128           /* phase-A candidates per even class, at most */
+#define GBH_RIDGE_CAP 1024       /* ridge records per segment (reference RIDGE_CAP role) */
 #define GBH_MAX_BATCH 64         /* segments per batch, at most */
 
 typedef struct gbh_ctx gbh_ctx;
@@ -115,7 +116,10 @@ typedef struct {
     uint32_t gtail[2 * GBH_GAP_TCAP];   /* (upper position, g) */
     uint32_t hist[GBH_EVEN_CLASSES * GBH_NCAND];   /* p(n) by even class and candidate slot */
     uint32_t first[GBH_NCAND];     /* smallest even index with p(n) = candidate (by prime index; ~0u none) */
+    uint32_t ridge_n;              /* phase-B evens with p(n) >= the ridge threshold */
+    uint32_t pad2;
     uint32_t left[GBH_LEFT_CAP];   /* even indices of the leftovers */
+    uint64_t ridge[2 * GBH_RIDGE_CAP];   /* (n, p(n)) of the first GBH_RIDGE_CAP of them */
 } gbh_segout;
 
 /* Stage 1: first class bit of every window prime in every segment
@@ -215,7 +219,7 @@ typedef struct {
     const uint32_t* coff;
     const uint32_t* pending;
     uint32_t pcap;
-    uint32_t pad0;
+    uint32_t ridge_thresh;         /* record (n, p(n)) for p(n) >= this (Goldbach ridges; ~0u: none) */
     gbh_segout* out;
 } gbh_checkb_args;
 

@@ -18,7 +18,11 @@
  *       everything after "version=..." so the product's line can be compared
  *       byte for byte;
  *   gb_oracle pn N
- *       p(n) of one even (gstats.hpp:139-144).
+ *       p(n) of one even (gstats.hpp:139-144);
+ *   gb_oracle ridges START LIMIT FLOOR K [THREADS]
+ *       the K largest Goldbach ridges (n, p(n)) with p(n) >= FLOOR, ordered
+ *       by p then n descending (include/ridge.hpp:52-60, ridge_before), one
+ *       "n p" line each.
  *
  * Parity pins (tests/test_oracle.py): the reference's own stats_ref output on
  * the golden windows (tests/golden/ref_*.txt), its --known table
@@ -552,6 +556,66 @@ static int cmd_gbstats(uint64_t start, uint64_t limit, uint64_t seg, uint32_t sa
     return 0;
 }
 
+/* ---------------------------------------------------------------- ridges */
+
+typedef struct {
+    uint64_t start, evens, floor;
+    unsigned nt, t;
+    uint64_t* n;
+    uint64_t* p;
+    size_t cnt, cap;
+} RidgeTask;
+
+static void* ridge_worker(void* arg)
+{
+    RidgeTask* a = (RidgeTask*)arg;
+    for (uint64_t i = a->t; i < a->evens; i += a->nt) {
+        const uint64_t n = a->start + 2 * i;
+        const uint64_t p = pn_scan(n);
+        if (p < a->floor) continue;
+        if (a->cnt == a->cap) {
+            a->cap = a->cap ? 2 * a->cap : 256;
+            a->n = realloc(a->n, a->cap * sizeof *a->n);
+            a->p = realloc(a->p, a->cap * sizeof *a->p);
+        }
+        a->n[a->cnt] = n;
+        a->p[a->cnt++] = p;
+    }
+    return NULL;
+}
+
+typedef struct { uint64_t n, p; } Ridge;
+static int ridge_cmp(const void* x, const void* y)
+{
+    const Ridge* a = x;
+    const Ridge* b = y;
+    if (a->p != b->p) return a->p > b->p ? -1 : 1;
+    return a->n > b->n ? -1 : a->n < b->n;
+}
+
+static int cmd_ridges(uint64_t start, uint64_t limit, uint64_t floor, size_t k, unsigned nt)
+{
+    RidgeTask* t = calloc(nt, sizeof *t);
+    pthread_t* th = calloc(nt, sizeof *th);
+    for (unsigned i = 0; i < nt; ++i) {
+        t[i].start = start;
+        t[i].evens = (limit - start) / 2 + 1;
+        t[i].floor = floor;
+        t[i].nt = nt;
+        t[i].t = i;
+        pthread_create(&th[i], NULL, ridge_worker, &t[i]);
+    }
+    size_t total = 0;
+    for (unsigned i = 0; i < nt; ++i) { pthread_join(th[i], NULL); total += t[i].cnt; }
+    Ridge* r = malloc((total + 1) * sizeof *r);
+    size_t m = 0;
+    for (unsigned i = 0; i < nt; ++i)
+        for (size_t j = 0; j < t[i].cnt; ++j) { r[m].n = t[i].n[j]; r[m++].p = t[i].p[j]; }
+    qsort(r, m, sizeof *r, ridge_cmp);
+    for (size_t j = 0; j < m && j < k; ++j) printf("%llu %llu\n", (unsigned long long)r[j].n, (unsigned long long)r[j].p);
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
     init_primes();
@@ -574,12 +638,19 @@ int main(int argc, char** argv)
         if (s < 1000000 || l < s || !sample) { fprintf(stderr, "bad range\n"); return 2; }
         return cmd_gbstats(s, l, seg, sample, argv[6], nt ? nt : 1);
     }
+    if (argc >= 6 && !strcmp(argv[1], "ridges")) {
+        const uint64_t s = strtoull(argv[2], NULL, 10), l = strtoull(argv[3], NULL, 10);
+        const unsigned nt = argc > 6 ? (unsigned)atoi(argv[6]) : 8;
+        if (s % 2 || l % 2 || s < 1000000 || l < s) { fprintf(stderr, "bad range\n"); return 2; }
+        return cmd_ridges(s, l, strtoull(argv[4], NULL, 10), (size_t)strtoull(argv[5], NULL, 10), nt ? nt : 1);
+    }
     if (argc >= 3 && !strcmp(argv[1], "isprime")) {
         /* one number per line on stdin -> 0/1 per line (the MR pin) */
         char line[64];
         while (fgets(line, sizeof line, stdin)) printf("%d\n", gb_is_prime_u64(strtoull(line, NULL, 10)));
         return 0;
     }
-    fprintf(stderr, "usage: gb_oracle ref START LIMIT [THREADS] | gbstats START LIMIT SEGMENT SAMPLE GAPS [THREADS] | pn N | isprime -\n");
+    fprintf(stderr, "usage: gb_oracle ref START LIMIT [THREADS] | gbstats START LIMIT SEGMENT SAMPLE GAPS [THREADS] | "
+                    "ridges START LIMIT FLOOR K [THREADS] | pn N | isprime -\n");
     return 2;
 }

@@ -131,3 +131,31 @@ def test_state_merge_and_overlap_refusal(stats_env):
     assert [l for l in lines if l.startswith("range ")] == ["range 1000000 3000000"]
     # 500001 + 500000 evens; segment 0 of each run sampled: 250000 + 250000
     assert [l for l in lines if l.startswith("evens ")] == ["evens 1000001 500000"]
+
+
+def test_ridges_match_oracle(stats_env, monkeypatch):
+    """Goldbach ridges (include/ridge.hpp): the store's top 30 after a run equals
+    the oracle's 30 largest p(n) of the window (floor 1800, above every phase-A
+    candidate), the store format is the reference's v2, and a second run over
+    the same window keeps the store unchanged."""
+    a, b = 4_000_000_000_169_000_000, 4_000_000_000_170_000_000
+    monkeypatch.setenv("GB_RIDGE_MIN", "1800")
+    _say(f"ridges [{a}, {b}]")
+    r = gbamd.run_gpu(b, 300_000, start_even=a)
+    rows = (stats_env / "ridges.csv").read_text().splitlines()
+    assert rows[0].startswith("# goldbach-l0 ridge store v2")
+    got = [tuple(int(x) for x in row.split(",")[1:3]) for row in rows if not row.startswith("#")]
+    out = subprocess.run([str(compare.ORACLE), "ridges", str(a), str(b), "1800", "30", "8"], capture_output=True,
+                         text=True, check=True).stdout
+    want = [tuple(int(x) for x in line.split()) for line in out.splitlines()]
+    assert got == want
+    assert got[0] == (4_000_000_000_169_717_704, 4133)
+    for row in rows:
+        if not row.startswith("#"):
+            rank, n, p, q, seg, found = (int(x) for x in row.split(","))
+            assert q == n - p and a <= seg <= n
+    before = (stats_env / "ridges.csv").read_text()
+    (stats_env / "st.state").unlink()
+    (stats_env / "st.log").unlink()
+    gbamd.run_gpu(b, 300_000, start_even=a)
+    assert (stats_env / "ridges.csv").read_text() == before

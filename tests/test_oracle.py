@@ -111,3 +111,37 @@ def test_segmentation_rule():
     # span 600: segments of 300, 300, 300 evens, then the 101-even rest (the
     # remaining 800 > 600 is not merged); segments 0 and 2 are sampled
     assert compare.kv(line)["evens_sampled"] == str(300 + 300)
+
+
+def _pn_small(n, isp):
+    """p(n) for small n from a sieve: the smallest odd prime p with n - p prime."""
+    p = 3
+    while not (isp[p] and isp[n - p]):
+        p += 2
+    return p
+
+
+def test_oracle_ridges_against_brute_force():
+    """The ridges mode (top-K largest p(n), ties by larger n) on a window a
+    plain sieve covers, against the definition (include/ridge.hpp:52-60)."""
+    a, b, floor, k = 1_000_000, 1_100_000, 40, 30
+    lim = b + 2
+    isp = bytearray([1]) * (lim + 1)
+    isp[0] = isp[1] = 0
+    for i in range(2, int(lim ** 0.5) + 1):
+        if isp[i]:
+            isp[i * i::i] = bytearray(len(isp[i * i::i]))
+    want = sorted(((_pn_small(n, isp), n) for n in range(a, b + 1, 2)), reverse=True)
+    want = [(n, p) for p, n in want if p >= floor][:k]
+    got = [tuple(map(int, line.split())) for line in _oracle("ridges", a, b, floor, k, 4).splitlines()]
+    assert got == want
+
+
+def test_oracle_ridge_record_matches_reference_fixture():
+    """The 4133 ridge near 4e18: the oracle's largest ridge of the window is
+    the reference's pmax and first occurrence (tests/golden/ref_w4e18l.txt)."""
+    fx = compare.kv((GOLDEN / "ref_w4e18l.txt").read_text())
+    first = dict(item.split(":") for item in fx["first"].split(";"))
+    top = _oracle("ridges", 4_000_000_000_169_000_000, 4_000_000_000_170_000_000, 3000, 5, 8).split()
+    assert int(top[1]) == int(fx["pmax"]) == 4133
+    assert int(top[0]) == int(first["4133"])
