@@ -341,6 +341,15 @@ int gbh_launch_check_b(gbh_ctx* c, int s, const gbh_checkb_args* a)
     return GBH_OK;
 }
 
+int gbh_launch_mapcmp(gbh_ctx* c, int s, const gbh_mapcmp_args* a)
+{
+    if (!a->map || !a->ref || !a->geom || !a->diff) return fail(GBH_E_ARG, "mapcmp: null argument");
+    hipLaunchKernelGGL(gbh_dev::k_mapcmp, dim3(256, GBH_CLASSES), dim3(256), 0, S(c, s), a->map, a->ref, a->geom,
+                       a->cw, a->diff);
+    GBH_HIP(hipGetLastError());
+    return GBH_OK;
+}
+
 int gbh_launch_count(gbh_ctx* c, int s, const gbh_count_args* a)
 {
     if (const int rc = check_batch(a->b)) return rc;

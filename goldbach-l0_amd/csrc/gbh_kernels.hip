@@ -1023,6 +1023,24 @@ __global__ void __launch_bounds__(256) k_count_b(const gbh_countseg* __restrict_
         if (hl[i]) atomicAdd(&out->ghist[i], hl[i]);
 }
 
+// Words of class arrays that differ between two maps of one segment (bits
+// below nc[c] only).
+__global__ void __launch_bounds__(256) k_mapcmp(const uint64_t* __restrict__ a, const uint64_t* __restrict__ r,
+                                                const gbh_geom* __restrict__ g, uint32_t cw, uint32_t* __restrict__ diff)
+{
+    const uint32_t c = blockIdx.y;
+    const uint32_t nc = g->nc[c];
+    const uint32_t nw = (nc + 63) / 64;
+    uint32_t bad = 0;
+    for (uint32_t x = blockIdx.x * 256 + threadIdx.x; x < nw; x += gridDim.x * 256) {
+        uint64_t d = a[(size_t)c * cw + x] ^ r[(size_t)c * cw + x];
+        if (x == nw - 1 && nc % 64) d &= (1ull << (nc % 64)) - 1;
+        bad += d != 0;
+    }
+    bad = wave_sum(bad);
+    if ((threadIdx.x & 63) == 0 && bad) atomicAdd(diff, bad);
+}
+
 __global__ void __launch_bounds__(256) k_fill(uint32_t* __restrict__ p, size_t n, uint32_t v)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;

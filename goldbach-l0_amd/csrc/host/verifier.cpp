@@ -167,6 +167,7 @@ struct Engine::State {
     gbh_ctx* c = nullptr;
     int cus = 256;
     Buf d_win, d_large, d_chunks, d_i0, d_map, d_coff, d_coffa, d_pending, d_scratch, d_hits, d_hit_n;
+    Buf d_vmap, d_vhits, d_vhit_n, d_vchunks, h_vchunks;   // the GB_BUCKET_VERIFY rebuild
     Buf d_geoms, d_cs;
     Buf d_out[2], h_out[2], h_geoms[2], h_cs[2], h_chunks;
     Event ev[2][EV_N];
@@ -284,6 +285,11 @@ private:
     // large primes: [0, nwheel_) walk the wheel, [nwheel_, nlarge_) read the multiplier bitmap from t_hl_ up
     uint32_t nwheel_ = 0, t_hl_ = UINT32_MAX;
     uint32_t stage_ = 0;       // harvest LDS stage per work-group (short batches: one walk per prime)
+    // GB_BUCKET_VERIFY=N: every Nth segment's map is rebuilt with every large
+    // prime on the wheel walk (no multiplier bitmap) and compared word for word
+    uint32_t verify_every_ = 0, cap_v_ = 0, nvchunks_ = 0;
+    uint64_t verified_ = 0;
+    void enqueue_verify(uint32_t s, const gbh_batch& b, gbh_segout* out);
     double inv_wheel_ = 0, inv_large_ = 0;   // sum of 1/p over the wheel primes, over all large primes
     std::shared_ptr<const MultBitmap> mb_;
     bool stats_on_ = false, gaps_on_ = false, gaps_exact_ = false;
@@ -561,6 +567,36 @@ void DenseRun::plan_buffers()
     E.d_chunks.ensure(c, ch.size() * 4, 0);
     std::memcpy(E.h_chunks.p, ch.data(), ch.size() * 4);
     check(gbh_copy_h2d(c, 0, E.d_chunks.p, E.h_chunks.p, ch.size() * 4), "upload chunks");
+    verify_every_ = env::present("GB_BUCKET_VERIFY") ? env::u32("GB_BUCKET_VERIFY", 0, 0) : 0u;
+    if (verify_every_ && nlarge_) {
+        // one segment, every large prime on the wheel: its own lists and chunks
+        const double lam_v = wbits_ * kWheelFrac * inv_large_;
+        cap_v_ = uint32_t(lam_v * 1.15 + 8 * std::sqrt(lam_v) + 1024);
+        E.d_vmap.ensure(c, size_t(8) * cw_ * 8, 0);
+        E.d_vhits.ensure(c, size_t(8) * nwin_max_ * cap_v_ * 4, 0);
+        E.d_vhit_n.ensure(c, size_t(8) * nwin_max_ * 4, 0);
+        const double unit = double(span_) * (8.0 / 30) * 1.3 * kWheelFrac;
+        const double target = std::max((4.0 * nlarge_ + unit * inv_large_) / (8.0 * E.cus), 2e4);
+        std::vector<uint32_t> vc{0};
+        for (uint32_t j = 0; j < nlarge_;) {
+            const uint32_t hi = std::min<uint32_t>(nlarge_, j + GBH_HL_CHUNK);
+            const double ij = E.inv_at(j);
+            uint32_t e = std::min(j + 256, hi);
+            while (e < hi) {   // grow by 256 primes while within the target
+                const uint32_t nx = std::min(e + 256, hi);
+                const double inx = nx == hi ? E.inv_at(nx) : E.inv_coarse[nx >> 8];
+                if (4.0 * (nx - j) + unit * (inx - ij) > target && (nx & 255) == 0) break;
+                e = nx;
+            }
+            vc.push_back(e);
+            j = e;
+        }
+        nvchunks_ = uint32_t(vc.size() - 1);
+        E.h_vchunks.ensure(c, vc.size() * 4, 1);
+        E.d_vchunks.ensure(c, vc.size() * 4, 0);
+        std::memcpy(E.h_vchunks.p, vc.data(), vc.size() * 4);
+        check(gbh_copy_h2d(c, 0, E.d_vchunks.p, E.h_vchunks.p, vc.size() * 4), "upload verify chunks");
+    }
     check(gbh_stream_sync(c, 0), "chunk sync");
 }
 
@@ -629,6 +665,9 @@ void DenseRun::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
                       nlarge_ ? E.d_hits.as<uint32_t>() : nullptr, E.d_hit_n.as<uint32_t>(), cap_, 0};
     check(gbh_launch_window(c, 0, &w), "window sieve");
     check(gbh_event_record(c, 0, ev[EV_WINDOW].e), "event");
+    if (verify_every_ && nlarge_ && check_and_count)
+        for (uint32_t s = 0; s < nseg; ++s)
+            if (batch[s].index % verify_every_ == 0) enqueue_verify(s, b, out);
     if (check_and_count) {
         uint32_t kinds = 0;
         for (const SegJob& j : batch) kinds |= j.sampled ? 2u : 1u;
@@ -647,6 +686,31 @@ void DenseRun::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     check(gbh_event_record(c, 0, ev[EV_COUNT].e), "event");
     check(gbh_copy_d2h(c, 0, E.h_out[slot].p, out, sizeof(gbh_segout) * nseg), "copy outputs");
     check(gbh_event_record(c, 0, ev[EV_END].e), "event");
+}
+
+// The GB_BUCKET_VERIFY rebuild of segment s (the reference's
+// enqueue_bucket_verify, src/verifier.cpp:2059-2085): window primes as in
+// production, every large prime by the wheel walk (no bitmap), then a word
+// compare with the production map into out[s].vdiff.
+void DenseRun::enqueue_verify(uint32_t s, const gbh_batch& b, gbh_segout* out)
+{
+    gbh_ctx* c = E.c;
+    const gbh_batch vb{E.d_geoms.as<gbh_geom>() + s, 1, cw_, wbits_, b.nwin, E.d_vmap.as<uint64_t>()};
+    if (nwin_) {
+        const gbh_setup_args a{vb, E.d_win.as<uint32_t>(), nwin_, 0, E.d_i0.as<uint32_t>()};
+        check(gbh_launch_setup(c, 0, &a), "verify setup");
+    }
+    check(gbh_zero(c, 0, E.d_vhit_n.p, size_t(8) * b.nwin * 4), "verify zero");
+    const gbh_harvest_args h{vb, E.d_large.as<uint32_t>(), E.d_vchunks.as<uint32_t>(), nvchunks_, cap_v_, UINT32_MAX, 0,
+                             nullptr, 0, 0, E.d_vhits.as<uint32_t>(), E.d_vhit_n.as<uint32_t>(), &out[s].err};
+    check(gbh_launch_harvest(c, 0, &h), "verify harvest");
+    const gbh_window_args w{vb, E.d_win.as<uint32_t>(), nwin_, std::min(ncoop_, nwin_), E.d_i0.as<uint32_t>(),
+                            E.d_vhits.as<uint32_t>(), E.d_vhit_n.as<uint32_t>(), cap_v_, 0};
+    check(gbh_launch_window(c, 0, &w), "verify window");
+    const gbh_mapcmp_args m{E.d_map.as<uint64_t>() + size_t(s) * 8 * cw_, E.d_vmap.as<uint64_t>(),
+                            E.d_geoms.as<gbh_geom>() + s, cw_, 0, &out[s].vdiff};
+    check(gbh_launch_mapcmp(c, 0, &m), "verify compare");
+    ++verified_;
 }
 
 void DenseRun::collect(std::vector<SegJob>& batch, int slot)
@@ -680,7 +744,11 @@ void DenseRun::collect(std::vector<SegJob>& batch, int slot)
 void DenseRun::collect_segment(const SegJob& j, const gbh_segout& o, double share[4])
 {
     const std::string seg = " in segment [" + std::to_string(j.sg.seg_start) + ", " + std::to_string(j.sg.seg_end) + "]";
+    if (o.err & 1u) throw std::runtime_error("hit-list bucket overflow (raise the capacity)" + seg);
     if (o.err & ~13u) throw std::runtime_error("device error flags " + std::to_string(o.err) + seg);
+    if (o.vdiff)
+        throw std::runtime_error("GB_BUCKET_VERIFY: " + std::to_string(o.vdiff) + " map words differ from the "
+                                 "wheel-only rebuild" + seg);
     if (o.pending_n > pcap_) throw std::runtime_error("pending overflow (" + std::to_string(o.pending_n) + ")" + seg);
     // coverage: every even accounted for, by phase A, phase B or a leftover
     if (uint64_t(o.acct[0]) + o.pending_n != j.gpu_evens)
@@ -814,6 +882,9 @@ RunStats DenseRun::run()
     rs_.count_seconds = t_count_;
     rs_.sync_seconds = std::max(0.0, rs_.gpu_seconds - t_sieve_ - t_large_ - t_check_ - t_count_);
     rs_.start = start_;
+    if (verify_every_ && nlarge_)
+        std::fprintf(stderr, "bucket verify: %llu segment maps rebuilt without the multiplier bitmap, identical\n",
+                     (unsigned long long)verified_);
     if (ridges_ && ridges_->active()) {
         ridges_->save(ridge_path_);
         rs_.ridge_min = ridges_->floor();

@@ -109,7 +109,7 @@ typedef struct {
     uint32_t twins[3];             /* twin pairs (11,13) (17,19) (29,31) */
     uint32_t pfirst, plast;        /* first / last counted prime: position q - q_first (~0u / 0: none) */
     uint32_t gtail_n;              /* tail gaps (g >= 700) inside the segment */
-    uint32_t pad0;
+    uint32_t vdiff;                /* map words that differ from the verification map (gbh_launch_mapcmp) */
     uint64_t gmax;                 /* largest inner gap: g << 32 | ~(upper position) (0: none) */
     uint32_t ghist[GBH_GAP_BINS + 1];   /* inner gaps by g / 2; [GBH_GAP_BINS]: overflow */
     uint32_t pad1;
@@ -161,6 +161,19 @@ typedef struct {
     uint32_t* hit_n;               /* [nseg][8][nwin] fill counts (zeroed before the launch) */
     uint32_t* err;                 /* bit 0: a list overflow; bit 2: t_hl^3 <= q; bit 3: m outside the bitmap */
 } gbh_harvest_args;
+
+/* Map self-check (reference GB_BUCKET_VERIFY compare kernel,
+ * src/verifier.cpp:2059-2085): counts the words of one segment's class map
+ * (bits below nc[c] of each class) that differ from a second map of the
+ * same segment built another way, into *diff. */
+typedef struct {
+    const uint64_t* map;           /* the production map of the segment */
+    const uint64_t* ref;           /* the verification map */
+    const gbh_geom* geom;          /* the segment's geometry (device) */
+    uint32_t cw;                   /* class stride (words) of both maps */
+    uint32_t pad0;
+    uint32_t* diff;
+} gbh_mapcmp_args;
 
 /* Multiplier bitmap build (reference mt_build, goldbach.cl:3181-3242): the
  * class maps of a batch sieved over m (segment s: class arrays of nc bits
@@ -275,6 +288,7 @@ int gbh_launch_window(gbh_ctx* ctx, int stream, const gbh_window_args* a);
 int gbh_launch_check_a(gbh_ctx* ctx, int stream, const gbh_check_args* a);
 int gbh_launch_check_b(gbh_ctx* ctx, int stream, const gbh_checkb_args* a);
 int gbh_launch_count(gbh_ctx* ctx, int stream, const gbh_count_args* a);
+int gbh_launch_mapcmp(gbh_ctx* ctx, int stream, const gbh_mapcmp_args* a);
 
 #ifdef __cplusplus
 }

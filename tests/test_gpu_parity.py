@@ -77,6 +77,10 @@ WINDOWS = [
     # multiplier bitmap), and a small bitmap budget (wheel below ~1.7e7)
     ("w4e18", 2_000_000, {"GB_MT_GB": "0"}),
     ("w44e18", 1_000_000, {"GB_MT_GB": "8"}),
+    # every segment's map rebuilt with all large primes on the wheel walk and
+    # compared word for word (the reference's GB_BUCKET_VERIFY self-check)
+    ("w4e18", 2_000_000, {"GB_BUCKET_VERIFY": "1"}),
+    ("w5e18end", 3_000_000, {"GB_BUCKET_VERIFY": "2"}),
 ]
 
 
