@@ -36,6 +36,17 @@ std::vector<uint32_t> odd_primes_up_to(uint32_t n);
 /// base primes: up to ~2^32, ~200M entries).
 std::vector<uint32_t> base_primes(uint32_t n, unsigned threads);
 
+/// FNV-1a (64-bit) over the 32-bit words of a prime payload: the content
+/// hash of the prime cache (primes.cpp:113-122 of the reference).
+uint64_t prime_payload_hash(const uint32_t* p, size_t cnt);
+
+/// base_primes through the reference's prime cache (GBP2 files,
+/// src/primes.cpp:125-284): <path>.<n> when it exists and passes its checks,
+/// else the smallest <path>.<L> with L > n (its primes <= n), else the sieve,
+/// published as <path>.<n> (temp file, fsync, rename; best effort). A cache
+/// that fails its header, hash or content check is reported and ignored.
+std::vector<uint32_t> base_primes_cached(uint32_t n, const char* cache_path, unsigned threads);
+
 /// Deterministic Miller-Rabin (bases 2..37): exact for every uint64_t.
 bool is_prime_u64(uint64_t n);
 

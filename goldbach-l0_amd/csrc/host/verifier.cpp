@@ -173,7 +173,7 @@ struct Engine::State {
     Event ev[2][EV_N];
     Candidates cand;
 
-    State(uint64_t max_limit_, uint64_t max_span_)
+    State(uint64_t max_limit_, uint64_t max_span_, const char* prime_cache)
     {
         t0 = clock_::now();
         require_supported_limit(max_limit_);
@@ -188,7 +188,7 @@ struct Engine::State {
         // base primes (the reference loads them from its prime cache;
         // here a segmented sieve on the host threads)
         const auto tc = clock_::now();
-        base = base_primes(std::max<uint32_t>(base_prime_bound(max_limit), 64), host_threads());
+        base = base_primes_cached(std::max<uint32_t>(base_prime_bound(max_limit), 64), prime_cache, host_threads());
         cache_s = since(tc);
         win_lo = uint32_t(std::lower_bound(base.begin(), base.end(), 67u) - base.begin());
         const uint64_t win_p = env::u64("GB_WIN_P", WIN_P_DEFAULT, 67, UINT32_MAX);
@@ -930,7 +930,10 @@ uint32_t DenseRun::map_of(uint64_t seg_start, std::vector<uint64_t>& words, uint
 
 }  // namespace
 
-Engine::Engine(uint64_t max_limit, uint64_t max_span) : s_(std::make_unique<State>(max_limit, max_span)) {}
+Engine::Engine(uint64_t max_limit, uint64_t max_span, const char* prime_cache)
+    : s_(std::make_unique<State>(max_limit, max_span, prime_cache))
+{
+}
 Engine::~Engine() = default;
 double Engine::init_seconds() const { return s_->init_s; }
 double Engine::prepare(uint64_t limit, uint64_t start_even)
@@ -969,11 +972,10 @@ RunStats run_gpu(uint64_t limit, uint64_t segment_even, const std::string& spv_p
                  uint32_t sieve_split_arg, const char* prime_cache_path, bool collect_segments)
 {
     (void)spv_path;
-    (void)prime_cache_path;
     refuse_removed_knobs();
     if (limit < 4) throw std::runtime_error("limit < 4: no even n >= 4 to verify");
     require_supported_limit(limit);
-    Engine e(limit & ~uint64_t(1), resolved_span(limit, start_even, segment_even));
+    Engine e(limit & ~uint64_t(1), resolved_span(limit, start_even, segment_even), prime_cache_path);
     RunStats s = e.run(limit, segment_even, verbose, start_even, sieve_split_arg, collect_segments);
     s.init_seconds += e.init_seconds();
     s.total_seconds = s.init_seconds + s.gpu_seconds;

@@ -70,7 +70,8 @@ RunStats run_gpu(uint64_t limit, uint64_t segment_even, const std::string& spv_p
 class Engine {
 public:
     /// @throws std::runtime_error when no gfx950 device is usable.
-    Engine(uint64_t max_limit, uint64_t max_span = 467'500'000ULL);
+    /// @param prime_cache GBP2 base-prime cache path stem (nullptr: none)
+    Engine(uint64_t max_limit, uint64_t max_span = 467'500'000ULL, const char* prime_cache = nullptr);
     ~Engine();
     Engine(const Engine&) = delete;
     Engine& operator=(const Engine&) = delete;

@@ -17,7 +17,9 @@ import pytest
 
 import compare
 import gbamd
-from conftest import GOLDEN
+from conftest import GOLDEN, PKG
+
+PKG_CLI = PKG / "goldbach_amd"
 
 pytestmark = pytest.mark.gpu
 
@@ -163,3 +165,50 @@ def test_ridges_match_oracle(stats_env, monkeypatch):
     (stats_env / "st.log").unlink()
     gbamd.run_gpu(b, 300_000, start_even=a)
     assert (stats_env / "ridges.csv").read_text() == before
+
+
+def _fnv1a_words(data: bytes) -> int:
+    h = 0xcbf29ce484222325
+    for (w,) in __import__("struct").iter_unpack("<I", data):
+        h = ((h ^ w) * 0x100000001b3) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def test_prime_cache_gbp2(stats_env):
+    """The reference's GBP2 base-prime cache (src/primes.cpp:125-284) through the
+    CLI: written as <stem>.<n> with a valid header, FNV-1a payload hash and the
+    odd primes <= n; reused as is; a corrupted cache is ignored and rewritten;
+    a cache for a larger bound serves a smaller one by its prefix."""
+    import struct
+    cli = str(PKG_CLI)
+    stem = stats_env / "pc"
+    a, b = 1_000_000_000_000, 1_000_000_200_000
+    n = int(b ** 0.5) + 16
+
+    def run(lim, start=a):
+        return subprocess.run([cli, "--start", str(start), "--limit", str(lim), "--prime-cache", str(stem)],
+                              capture_output=True, text=True, timeout=120)
+    _say("prime cache: first run")
+    r = run(b)
+    assert r.returncode == 0, r.stderr
+    f = stats_env / f"pc.{n}"
+    raw = f.read_bytes()
+    magic, lim, cnt, h = raw[:4], *struct.unpack("<QQQ", raw[4:28])
+    assert magic == b"GBP2" and lim == n and len(raw) == 28 + 4 * cnt
+    primes = list(struct.unpack(f"<{cnt}I", raw[28:]))
+    assert primes == compare.odd_primes(n)
+    assert h == _fnv1a_words(raw[28:])
+    mtime = f.stat().st_mtime_ns
+    r = run(b)
+    assert r.returncode == 0 and "prime cache" not in r.stderr and f.stat().st_mtime_ns == mtime
+    bad = bytearray(raw)
+    bad[100] ^= 1
+    f.write_bytes(bytes(bad))
+    r = run(b)
+    assert r.returncode == 0 and "payload hash mismatch" in r.stderr
+    assert f.read_bytes() == raw
+    # a smaller bound: served by the prefix of pc.<n>, no new file
+    a2 = 500_000_000_000
+    r = run(a2 + 2_000, a2)
+    n2 = int((a2 + 2_000) ** 0.5) + 16
+    assert r.returncode == 0 and n2 < n and not (stats_env / f"pc.{n2}").exists()
