@@ -7,6 +7,8 @@
 #include <algorithm>
 #include <cstdio>
 #include <ctime>
+#include <mutex>
+#include <unordered_map>
 
 #include "env.hpp"
 #include "primes.hpp"
@@ -16,6 +18,21 @@ namespace {
 
 // larger ridge first, equal ridge: larger even first
 bool before(const RidgeEntry& a, const RidgeEntry& b) { return a.p != b.p ? a.p > b.p : a.n > b.n; }
+
+// The exact checks' verdicts on (n, p) pairs already checked in this process
+// (a store reloaded run after run re-checks the same rows: each costs up to
+// a few hundred Miller-Rabin tests). The verdict depends only on n, p and the
+// candidate list, which is the same fixed prime list for every tracker.
+enum Verdict : uint8_t { kOk, kBadPair, kNotMinimal };
+struct PairHash {
+    size_t operator()(const std::pair<uint64_t, uint64_t>& k) const
+    {
+        return size_t(k.first * 0x9E3779B97F4A7C15ull ^ (k.second + (k.first >> 29)));
+    }
+};
+std::mutex g_memo_mu;
+std::unordered_map<std::pair<uint64_t, uint64_t>, uint32_t, PairHash> g_memo;   // verdict | witness << 8
+constexpr size_t kMemoMax = 1 << 16;
 
 }  // namespace
 
@@ -55,22 +72,44 @@ void RidgeTracker::consider(uint64_t n, uint64_t p, uint64_t seg, uint64_t found
     }
     for (const RidgeEntry& t : top_)
         if (t.n == n) return;                 // already stored: keep its found time
-    if (!is_prime_u64(p) || !is_prime_u64(n - p)) {
+    const std::pair<uint64_t, uint64_t> key{n, p};
+    uint32_t v = UINT32_MAX;
+    if (cand_) {
+        std::lock_guard<std::mutex> lk(g_memo_mu);
+        const auto it = g_memo.find(key);
+        if (it != g_memo.end()) v = it->second;
+    }
+    if (v == UINT32_MAX) {
+        v = kOk;
+        if (!is_prime_u64(p) || !is_prime_u64(n - p)) {
+            v = kBadPair;
+        } else if (cand_) {
+            for (uint32_t c : *cand_) {       // minimality: no smaller candidate works
+                if (c >= p) break;
+                if (is_prime_u64(n - c)) {
+                    v = kNotMinimal | c << 8;
+                    break;
+                }
+            }
+        }
+        if (cand_) {
+            std::lock_guard<std::mutex> lk(g_memo_mu);
+            if (g_memo.size() >= kMemoMax) g_memo.clear();
+            g_memo.emplace(key, v);
+        }
+    }
+    if ((v & 0xFF) == kBadPair) {
         ++dropped_;
         if (debug_) std::fprintf(stderr, "ridge drop (pair): n=%llu p=%llu\n", (unsigned long long)n,
                                  (unsigned long long)p);
         return;
     }
-    if (cand_)
-        for (uint32_t c : *cand_) {           // minimality: no smaller candidate works
-            if (c >= p) break;
-            if (is_prime_u64(n - c)) {
-                ++dropped_;
-                if (debug_) std::fprintf(stderr, "ridge drop (not minimal): n=%llu p=%llu < %u\n",
-                                         (unsigned long long)n, (unsigned long long)p, c);
-                return;
-            }
-        }
+    if ((v & 0xFF) == kNotMinimal) {
+        ++dropped_;
+        if (debug_) std::fprintf(stderr, "ridge drop (not minimal): n=%llu p=%llu < %u\n", (unsigned long long)n,
+                                 (unsigned long long)p, v >> 8);
+        return;
+    }
     if (found == UINT64_MAX) found = uint64_t(std::time(nullptr));
     const RidgeEntry e{n, p, n - p, seg, found};
     top_.insert(std::upper_bound(top_.begin(), top_.end(), e, before), e);

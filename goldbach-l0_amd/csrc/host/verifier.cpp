@@ -508,7 +508,6 @@ void DenseRun::plan_buffers()
                  : j == nwheel_ ? inv_wheel_ : j == nlarge_ ? inv_large_ : E.inv_at(j);
         };
         const auto factor = [&](uint32_t a) { return a < nwheel_ ? 1.3 * kWheelFrac : hit_factor(a); };
-        const auto hits_f = [&](uint32_t a) { return a < nwheel_ ? kWheelFrac : hit_factor(a); };
         double total = 0;
         for (uint32_t k = 0; k < 256; ++k) {
             uint32_t a = uint32_t(uint64_t(nlarge_) * k / 256) & ~255u, b = uint32_t(uint64_t(nlarge_) * (k + 1) / 256) & ~255u;
@@ -525,11 +524,13 @@ void DenseRun::plan_buffers()
         constexpr double kStageHits = 0.7 * GBH_HL_STAGE;
         const double all_hits = hits_unit * hit_density();
         stage_ = all_hits <= kStageHits * 8192 ? GBH_HL_STAGE : 0u;
-        uint32_t j = 0;
+        uint32_t j = 0, prev = 0;   // prev: the last chunk's length (lengths change slowly: start the search there)
         while (j < nlarge_) {
             uint32_t hi = std::min<uint32_t>(nlarge_, j + GBH_HL_CHUNK);
             if (j < nwheel_) hi = std::min(hi, nwheel_);
-            const double f = factor(j), fh = hits_f(j), ij = j == nwheel_ ? inv_wheel_ : E.inv_at(j);
+            const double hf = hit_factor(j);
+            const double f = j < nwheel_ ? 1.3 * kWheelFrac : hf, fh = hf;
+            const double ij = j == nwheel_ ? inv_wheel_ : E.inv_at(j);
             const auto ok = [&](uint32_t e) {   // chunk [j, e) within the targets (the rates barely change inside)
                 const double si = inv(e) - ij;
                 return kPrimeCost * (e - j) + hits_unit * f * si <= target && (!stage_ || hits_unit * fh * si <= kStageHits);
@@ -544,14 +545,17 @@ void DenseRun::plan_buffers()
                 } else {
                     if (lo < j + 256) lo += 256;
                     if (lo > up) lo = up;
-                    while (lo < up) {
-                        const uint32_t mid = ((lo >> 8) + ((up >> 8) - (lo >> 8) + 1) / 2) << 8;
-                        if (ok(mid)) lo = mid;
-                        else up = mid - 256;
+                    // ok() grows with e: walk from the previous length to the
+                    // largest e in [lo, up] with ok(e) (lo when none)
+                    e = std::min(up, std::max(lo, ((j + prev) >> 8) << 8));
+                    if (ok(e)) {
+                        while (e < up && ok(e + 256)) e += 256;
+                    } else {
+                        while (e > lo && !ok(e)) e -= 256;
                     }
-                    e = lo;
                 }
             }
+            prev = e - j;
             ch.push_back(e);
             j = e;
         }
