@@ -272,14 +272,17 @@ int gbh_launch_harvest(gbh_ctx* c, int s, const gbh_harvest_args* a)
 {
     if (const int rc = check_batch(a->b)) return rc;
     if (!a->nchunks) return GBH_OK;
-    const uint64_t nb = uint64_t(a->b.nseg) * 8 * a->b.nwin;
+    if (a->runs && (!a->nruns || !a->run_nseg_max || a->run_nseg_max > a->b.nseg))
+        return fail(GBH_E_ARG, "harvest: run list without runs or with a run longer than the batch");
+    const uint64_t nb = uint64_t(a->runs ? a->run_nseg_max : a->b.nseg) * 8 * a->b.nwin;
     if (nb > GBH_HL_BUCKETS) return fail(GBH_E_ARG, "harvest: more window lists than the work-group counts");
     if (a->m_base % 240) return fail(GBH_E_ARG, "harvest: bitmap base not a multiple of 240");
     if (a->stage > GBH_HL_STAGE) return fail(GBH_E_ARG, "harvest: stage above GBH_HL_STAGE");
     const size_t lds = a->stage ? size_t(nb) * 8 + size_t(a->stage) * 6 + 16 : size_t(nb) * 4;
     if (lds > 144 * 1024) return fail(GBH_E_ARG, "harvest: lists and stage exceed the work-group's LDS");
     hipLaunchKernelGGL(gbh_dev::k_harvest, dim3(a->nchunks), dim3(256), lds, S(c, s), a->b, a->primes, a->chunks,
-                       a->t_hl, a->mbits, a->m_base, a->m_span, a->cap, a->stage, a->hits, a->hit_n, a->err);
+                       a->runs, a->runs ? a->nruns : 1u, uint32_t(nb), a->t_hl, a->mbits, a->m_base, a->m_span,
+                       a->cap, a->stage, a->hits, a->hit_n, a->err);
     GBH_HIP(hipGetLastError());
     return GBH_OK;
 }
@@ -336,7 +339,7 @@ int gbh_launch_check_b(gbh_ctx* c, int s, const gbh_checkb_args* a)
 {
     if (const int rc = check_batch(a->b)) return rc;
     hipLaunchKernelGGL(gbh_dev::k_check_b, dim3(grid_of(a->pcap, 256, 512), a->b.nseg), dim3(256), 0, S(c, s), a->b,
-                       a->coff, a->ka, a->tail_t, a->pending, a->pcap, a->ridge_thresh, a->out);
+                       a->coff, a->ka, a->tail_t, a->pending, a->pcap, a->out);
     GBH_HIP(hipGetLastError());
     return GBH_OK;
 }

@@ -83,8 +83,8 @@ __global__ void __launch_bounds__(256) k_setup(gbh_batch b, const uint32_t* __re
 {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t s = blockIdx.y;
-    if (j >= np) return;
     const gbh_geom& g = b.geoms[s];
+    if (j >= np || j >= g.np_win) return;   // primes above sqrt(q_last) have no multiple to mark (k_window skips them)
     const uint32_t p = primes[j];
     const uint64_t m0 = first_mult(g.q_first, p);
     const uint32_t m0r = (uint32_t)(m0 % 30);
@@ -410,20 +410,24 @@ __device__ __forceinline__ void bm_group(const HlCtx& h, double qb0d, const uint
 // `stage` slots, the counting pass also keeps the entries in LDS; when they
 // all fit, the work-group writes them from there in one burst (its runs of
 // each list complete their lines in L2) and skips the second walk.
+// Batched ranges: the chunk is walked once per run (a range's consecutive
+// segments in the batch), each run with its own q range, lists and prime
+// bound; nullptr runs is one run of the whole batch.
 // A bitmap prime with p^3 <= qb1 breaks the two-factor rule: fail closed.
 __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __restrict__ primes,
-                                                 const uint32_t* __restrict__ chunks, uint32_t t_hl,
-                                                 const uint64_t* __restrict__ mbits, uint64_t m_base, uint64_t m_span,
-                                                 uint32_t cap, uint32_t stage, uint32_t* __restrict__ hits,
-                                                 uint32_t* __restrict__ hit_n, uint32_t* __restrict__ err)
+                                                 const uint32_t* __restrict__ chunks,
+                                                 const gbh_hl_run* __restrict__ runs, uint32_t nruns, uint32_t nbmax,
+                                                 uint32_t t_hl, const uint64_t* __restrict__ mbits, uint64_t m_base,
+                                                 uint64_t m_span, uint32_t cap, uint32_t stage,
+                                                 uint32_t* __restrict__ hits, uint32_t* __restrict__ hit_n,
+                                                 uint32_t* __restrict__ err)
 {
     __shared__ uint32_t df[GBH_MAX_BATCH], dl[GBH_MAX_BATCH], dq[GBH_MAX_BATCH * 8], cmt[8], sn;
     __shared__ uint64_t any_hit[GBH_HL_CHUNK / 64 + 4 * kG];   // bitmap primes with a set bit in range (pass 2 walks those)
     __shared__ WheelLds wl;
-    extern __shared__ uint32_t cnt[];     // [nseg][8][nwin] counts; with a stage: run counters, the stage
-    const uint32_t nb = b.nseg * 8 * b.nwin;
-    uint32_t* runc = cnt + nb;
-    uint32_t* stv = runc + nb;
+    extern __shared__ uint32_t cnt[];     // [nseg][8][nwin] counts of a run; with a stage: run counters, the stage
+    uint32_t* runc = cnt + nbmax;
+    uint32_t* stv = runc + nbmax;
     uint16_t* stb = reinterpret_cast<uint16_t*>(stv + stage);
     const uint32_t tid = threadIdx.x;
     if (tid < 210) { wl.next[tid] = c_w210_next[tid]; wl.idx[tid] = c_w210_idx[tid]; }
@@ -434,36 +438,6 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
         for (uint32_t i = 0; i < 8; ++i) v |= (uint32_t)c_cls_of[(res_of(tid) * res_of(i)) % 30] << (3 * i);
         cmt[tid] = v;
     }
-    const uint64_t qb0 = b.geoms[0].q_first;
-    for (uint32_t s = tid; s < b.nseg; s += 256) {
-        df[s] = (uint32_t)((b.geoms[s].q_first - qb0) >> 1);
-        dl[s] = (uint32_t)((b.geoms[s].q_last - qb0) >> 1);
-    }
-    for (uint32_t i = tid; i < b.nseg * 8; i += 256) dq[i] = (uint32_t)((b.geoms[i / 8].Q[i % 8] - qb0) >> 1);
-    for (uint32_t i = tid; i < nb; i += 256) {
-        cnt[i] = 0;
-        if (stage) runc[i] = 0;
-    }
-    if (tid == 0) sn = 0;
-    __syncthreads();
-    HlCtx h;
-    h.stv = stage ? stv : nullptr;
-    h.stb = stb;
-    h.sn = &sn;
-    h.scap = stage;
-    h.qb0 = qb0;
-    h.dmax = dl[b.nseg - 1];
-    h.hs_magic = (b.nseg > 1 && df[1] > df[0]) ? (uint32_t)((1ull << 32) / (df[1] - df[0] + 1)) : 0u;
-    h.nseg = b.nseg;
-    h.nwin = b.nwin;
-    h.lw = 31 - __clz(b.wbits);
-    h.wmask = b.wbits - 1;
-    h.t_hl = t_hl;
-    h.mbits = mbits;
-    h.m_base = m_base;
-    h.m_span = m_span;
-    const uint64_t qb1 = qb0 + 2ull * h.dmax;
-    const double qb0d = (double)qb0;
     const uint32_t j0 = chunks[blockIdx.x], j1 = chunks[blockIdx.x + 1];
     if (j1 - j0 > GBH_HL_CHUNK) {            // beyond the pass-2 flags: fail closed
         if (tid == 0) atomicOr(err, 1u);
@@ -471,40 +445,85 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
     }
     // a chunk is all wheel primes or all bitmap primes (the host splits at t_hl)
     const bool wheel = j0 < j1 && primes[j0] < t_hl;
-    if (!wheel && j0 < j1) {              // the chunk's smallest prime carries the two-factor rule
-        const uint64_t p = primes[j0], p2 = p * p;
-        if (__umul64hi(p2, p) == 0 && p2 * p <= qb1) {
-            if (tid == 0) atomicOr(err, 4u);
-            return;
+    for (uint32_t r = 0; r < nruns; ++r) {
+        // every condition below is uniform over the work-group
+        const uint32_t seg0 = runs ? runs[r].seg0 : 0u, rn = runs ? runs[r].nseg : b.nseg;
+        const uint32_t je = runs ? min(j1, runs[r].nprimes) : j1;
+        if (j0 >= je) continue;
+        if (rn == 0 || seg0 + rn > b.nseg || rn * 8 * b.nwin > nbmax) {   // a run beyond the contract: fail closed
+            if (tid == 0) atomicOr(err, 1u);
+            continue;
         }
-    }
-    if (wheel) {
-        for (uint32_t j = j0 + tid; j < j1; j += 256) walk_wheel<false>(h, qb0d, primes[j], df, dl, dq, wl, cnt, hits, cap, err);
-    } else {
-        for (uint32_t k = 0; j0 + k * 256 < j1; k += kG)
-            bm_group<false>(h, qb0d, primes, j0, j1, k, tid, df, dl, dq, cmt, cnt, any_hit, hits, cap, err);
-    }
-    __syncthreads();
-    for (uint32_t i = tid; i < nb; i += 256) {
-        const uint32_t c = cnt[i];
-        cnt[i] = c ? atomicAdd(&hit_n[i], c) : 0u;
-    }
-    __syncthreads();
-    if (stage && sn <= stage) {          // every entry is in the stage: write it from there
-        const uint32_t n = sn;
-        for (uint32_t i = tid; i < n; i += 256) {
-            const uint32_t bk = stb[i];
-            const uint32_t pos = cnt[bk] + atomicAdd(&runc[bk], 1u);
-            if (pos < cap) hits[(size_t)bk * cap + pos] = stv[i];
-            else atomicOr(err, 1u);
+        __syncthreads();                      // the previous run is done with the LDS tables
+        const gbh_geom* gs = b.geoms + seg0;
+        const uint64_t qb0 = gs[0].q_first;
+        for (uint32_t s = tid; s < rn; s += 256) {
+            df[s] = (uint32_t)((gs[s].q_first - qb0) >> 1);
+            dl[s] = (uint32_t)((gs[s].q_last - qb0) >> 1);
         }
-        return;
-    }
-    if (wheel) {
-        for (uint32_t j = j0 + tid; j < j1; j += 256) walk_wheel<true>(h, qb0d, primes[j], df, dl, dq, wl, cnt, hits, cap, err);
-    } else {
-        for (uint32_t k = 0; j0 + k * 256 < j1; k += kG)
-            bm_group<true>(h, qb0d, primes, j0, j1, k, tid, df, dl, dq, cmt, cnt, any_hit, hits, cap, err);
+        for (uint32_t i = tid; i < rn * 8; i += 256) dq[i] = (uint32_t)((gs[i / 8].Q[i % 8] - qb0) >> 1);
+        const uint32_t nb = rn * 8 * b.nwin;
+        for (uint32_t i = tid; i < nb; i += 256) {
+            cnt[i] = 0;
+            if (stage) runc[i] = 0;
+        }
+        if (tid == 0) sn = 0;
+        __syncthreads();
+        HlCtx h;
+        h.stv = stage ? stv : nullptr;
+        h.stb = stb;
+        h.sn = &sn;
+        h.scap = stage;
+        h.qb0 = qb0;
+        h.dmax = dl[rn - 1];
+        h.hs_magic = (rn > 1 && df[1] > df[0]) ? (uint32_t)((1ull << 32) / (df[1] - df[0] + 1)) : 0u;
+        h.nseg = rn;
+        h.nwin = b.nwin;
+        h.lw = 31 - __clz(b.wbits);
+        h.wmask = b.wbits - 1;
+        h.t_hl = t_hl;
+        h.mbits = mbits;
+        h.m_base = m_base;
+        h.m_span = m_span;
+        const uint64_t qb1 = qb0 + 2ull * h.dmax;
+        const double qb0d = (double)qb0;
+        if (!wheel) {                         // the chunk's smallest prime carries the two-factor rule
+            const uint64_t p = primes[j0], p2 = p * p;
+            if (__umul64hi(p2, p) == 0 && p2 * p <= qb1) {
+                if (tid == 0) atomicOr(err, 4u);
+                continue;
+            }
+        }
+        uint32_t* rh = hits + (size_t)seg0 * 8 * b.nwin * cap;   // the run's lists
+        uint32_t* rhn = hit_n + (size_t)seg0 * 8 * b.nwin;
+        if (wheel) {
+            for (uint32_t j = j0 + tid; j < je; j += 256) walk_wheel<false>(h, qb0d, primes[j], df, dl, dq, wl, cnt, rh, cap, err);
+        } else {
+            for (uint32_t k = 0; j0 + k * 256 < je; k += kG)
+                bm_group<false>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < nb; i += 256) {
+            const uint32_t c = cnt[i];
+            cnt[i] = c ? atomicAdd(&rhn[i], c) : 0u;
+        }
+        __syncthreads();
+        if (stage && sn <= stage) {          // every entry is in the stage: write it from there
+            const uint32_t n = sn;
+            for (uint32_t i = tid; i < n; i += 256) {
+                const uint32_t bk = stb[i];
+                const uint32_t pos = cnt[bk] + atomicAdd(&runc[bk], 1u);
+                if (pos < cap) rh[(size_t)bk * cap + pos] = stv[i];
+                else atomicOr(err, 1u);
+            }
+            continue;
+        }
+        if (wheel) {
+            for (uint32_t j = j0 + tid; j < je; j += 256) walk_wheel<true>(h, qb0d, primes[j], df, dl, dq, wl, cnt, rh, cap, err);
+        } else {
+            for (uint32_t k = 0; j0 + k * 256 < je; k += kG)
+                bm_group<true>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
+        }
     }
 }
 
@@ -612,6 +631,8 @@ __global__ void __launch_bounds__(256) k_window(gbh_batch b, const uint32_t* __r
     // 2) window primes below wbits / 128 (at least 128 marks each): one
     //    wave per prime, lane l takes marks l, l + 64, ...
     const uint32_t lane = tid & 63, wv = tid >> 6, wbits = b.wbits;
+    np = min(np, g.np_win);                // the segment's primes (larger ones have no multiple in it)
+    ncoop = min(ncoop, np);
     for (uint32_t j = wv; j < ncoop; j += 4) {
         const uint32_t t0 = i0[j];
         if (t0 == 0xFFFFFFFFu) continue;
@@ -860,11 +881,12 @@ __global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_batch b, const uin
 
 __global__ void __launch_bounds__(256) k_check_b(gbh_batch b, const uint32_t* __restrict__ coff, uint32_t ka,
                                                  uint32_t tail_t, const uint32_t* __restrict__ pending, uint32_t pcap,
-                                                 uint32_t ridge_thresh, gbh_segout* __restrict__ outs)
+                                                 gbh_segout* __restrict__ outs)
 {
     const uint32_t s = blockIdx.y;
     const gbh_geom& g = b.geoms[s];
     if (g.n_gpu > g.seg_end) return;
+    const uint32_t ridge_thresh = g.ridge_t;
     gbh_segout* out = outs + s;
     const uint64_t* map = seg_map(b, s);
     const uint32_t* pend = pending + (size_t)s * pcap;

@@ -1,7 +1,11 @@
 // Run-level C ABI (include/goldbach_amd.h) over the C++ driver.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <exception>
+#include <stdexcept>
+#include <string>
+#include <vector>
 
 #include "config.hpp"
 #include "goldbach_amd.h"
@@ -143,5 +147,85 @@ int gba_segment_map(uint64_t seg_start, uint64_t limit, uint64_t span, uint64_t*
         return 1;
     }
 }
+
+struct gba_batcher {
+    gb::Batcher b;
+    std::vector<size_t> finished;   // completed ranges not yet reported
+    gba_batcher(gba_engine* e, uint64_t max_limit, uint64_t min_start, uint64_t span)
+        : b(e->e, max_limit, min_start, span)
+    {
+    }
+};
+
+int gba_batcher_open(gba_engine* e, uint64_t max_limit, uint64_t min_start, uint64_t segment_even, gba_batcher** out,
+                     uint32_t* batch_segments, char* err, size_t err_cap)
+{
+    try {
+        *out = new gba_batcher(e, max_limit, min_start, segment_even);
+        if (batch_segments) *batch_segments = (*out)->b.batch_segments();
+        put(err, err_cap, "");
+        return 0;
+    } catch (const std::exception& ex) {
+        *out = nullptr;
+        put(err, err_cap, ex.what());
+        return 1;
+    }
+}
+
+int gba_batcher_submit(gba_batcher* b, uint64_t start_even, uint64_t limit, uint64_t* id, char* err, size_t err_cap)
+{
+    try {
+        const size_t i = b->b.submit(start_even, limit);
+        if (id) *id = i;
+        put(err, err_cap, "");
+        return 0;
+    } catch (const std::exception& ex) {
+        put(err, err_cap, ex.what());
+        return 1;
+    }
+}
+
+int gba_batcher_step(gba_batcher* b, uint64_t* finished, uint32_t finished_cap, uint32_t* n_finished, int* more,
+                     char* err, size_t err_cap)
+{
+    try {
+        const bool busy = b->b.step(&b->finished);
+        const uint32_t n = uint32_t(std::min<size_t>(finished ? finished_cap : 0, b->finished.size()));
+        for (uint32_t k = 0; k < n; ++k) finished[k] = b->finished[k];
+        b->finished.erase(b->finished.begin(), b->finished.begin() + n);
+        if (n_finished) *n_finished = n;
+        if (more) *more = busy || !b->finished.empty();
+        put(err, err_cap, "");
+        return 0;
+    } catch (const std::exception& ex) {
+        put(err, err_cap, ex.what());
+        return 1;
+    }
+}
+
+int gba_batcher_result(gba_batcher* b, uint64_t id, gba_run_result* out, char* stats_line, size_t stats_cap,
+                       char* state, size_t state_cap, char* err, size_t err_cap)
+{
+    try {
+        const gb::Batcher::Result& r = b->b.result(size_t(id));
+        if (!r.done) throw std::runtime_error("range " + std::to_string(id) + " not finished");
+        fill(r.stats, out);
+        put(stats_line, stats_cap, r.stats.stats_line.c_str());
+        put(state, state_cap, r.state.c_str());
+        put(err, err_cap, "");
+        return 0;
+    } catch (const std::exception& ex) {
+        put(err, err_cap, ex.what());
+        return 1;
+    }
+}
+
+int gba_batcher_totals(gba_batcher* b, gba_run_result* out)
+{
+    fill(b->b.totals(), out);
+    return 0;
+}
+
+void gba_batcher_close(gba_batcher* b) { delete b; }
 
 }  // extern "C"

@@ -90,6 +90,8 @@ inline gbh_geom make_geom(const SegmentGeometry& sg, const ClassGeo& cg, uint32_
         nmax = std::max<uint64_t>(nmax, cg.n[c]);
     }
     g.nwin = uint32_t((nmax + wbits - 1) / wbits);
+    g.np_win = 0xFFFFFFFFu;   // every window prime (callers bound it by the range's sqrt)
+    g.ridge_t = 0xFFFFFFFFu;  // no ridge records
     for (int c = 0; c < 8; ++c) {
         uint64_t restore = 0;
         for (int k = 0; k < 15; ++k) {

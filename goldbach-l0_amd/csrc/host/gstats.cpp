@@ -236,8 +236,8 @@ void GapTotals::merge(const GapTotals& o)
 
 // ---------------------------------------------------------------- GoldbachStats
 
-GoldbachStats::GoldbachStats(const std::vector<uint32_t>& cands, uint32_t tail_t)
-    : cands_(cands), tail_t_(tail_t)
+GoldbachStats::GoldbachStats(const std::vector<uint32_t>& cands, uint32_t tail_t, bool campaign)
+    : cands_(cands), tail_t_(tail_t), campaign_(campaign)
 {
     np_ = cands.size();
     pidx_.assign(cands.back() / 2 + 1, -1);
@@ -256,8 +256,10 @@ GoldbachStats::GoldbachStats(const std::vector<uint32_t>& cands, uint32_t tail_t
     log_path_ = env::path("GB_STATS_LOG", "goldbach_stats.log");
     rec_path_ = env::path("GB_RECORDS_FILE", "goldbach_records.txt");
     live_path_ = env::path("GB_STATS_LIVE", "goldbach_live.json");
-    load_state();
-    (void)load_gaps();
+    if (campaign_) {
+        load_state();
+        (void)load_gaps();
+    }
     set_run_start(~0ull);
 }
 
@@ -762,7 +764,9 @@ void GoldbachStats::load_state()
     merge_ranges(covered_);
 }
 
-void GoldbachStats::save_state() const
+void GoldbachStats::save_state() const { write_durably(state_path_, state_text()); }
+
+std::string GoldbachStats::state_text() const
 {
     std::string out =
         "# goldbach-l0 statistics state v2 (exact; include/gstats.hpp)\n"
@@ -790,11 +794,31 @@ void GoldbachStats::save_state() const
         out += "tailcls " + std::to_string(cr) + " " + dense_list(v) + "\n";
     }
     out += "tailagg " + dense_list(cum_tail_agg_) + "\n";
-    write_durably(state_path_, out);
+    return out;
+}
+
+std::string GoldbachStats::memory_commit(uint64_t sampled_total)
+{
+    if (campaign_) throw std::runtime_error("stats: memory_commit on a campaign run");
+    if (evens_sampled_ != sampled_total)
+        throw std::runtime_error("stats: " + std::to_string(evens_sampled_) + " sampled evens recorded, " +
+                                 std::to_string(sampled_total) + " in the sampled segments");
+    for (size_t i = 0; i < samp_.size(); ++i) { cum_samp_[i] += samp_[i]; cum_tail_[i] += tail_[i]; }
+    for (size_t i = 0; i < np_; ++i) cum_tail_agg_[i] += tail_agg_[i];
+    for (size_t i = 0; i < np_; ++i)
+        if (run_first_[i] && (!first_[i] || run_first_[i] < first_[i])) first_[i] = run_first_[i];
+    for (int i = 0; i < 8; ++i) tot_primes_[i] += primes_res_[i];
+    for (int i = 0; i < 3; ++i) tot_twins_[i] += twins_[i];
+    tot_evens_ += evens_;
+    tot_sampled_ += evens_sampled_;
+    for (auto& b : beyond_) tot_beyond_.push_back(b);
+    covered_.push_back({start_, limit_});
+    return state_text();
 }
 
 void GoldbachStats::commit(const char* version, uint64_t sampled_total)
 {
+    if (!campaign_) throw std::runtime_error("stats: commit on a run without campaign files");
     if (evens_sampled_ != sampled_total)
         throw std::runtime_error("stats: " + std::to_string(evens_sampled_) + " sampled evens recorded, " +
                                  std::to_string(sampled_total) + " in the sampled segments (statistics not written)");
@@ -843,6 +867,7 @@ void GoldbachStats::commit(const char* version, uint64_t sampled_total)
 
 void GoldbachStats::write_live(uint64_t start, uint64_t limit, uint64_t done, uint64_t total, double elapsed, bool force)
 {
+    if (!campaign_) return;
     const long long now = (long long)std::time(nullptr);
     if (!force && now - live_t_ < 2) return;
     live_t_ = now;

@@ -74,7 +74,10 @@ public:
     /// @param tail_t every even with p(n) >= tail_t is recorded.
     /// Loads the campaign state (GB_STATS_STATE, default goldbach_stats.state)
     /// and its gap state; a damaged file fails here, before the run.
-    GoldbachStats(const std::vector<uint32_t>& cands, uint32_t tail_t);
+    /// @param campaign false: a run of its own (batched and served ranges):
+    ///        no campaign files are read or written, the run starts from a
+    ///        fresh state and memory_commit() returns the state text.
+    GoldbachStats(const std::vector<uint32_t>& cands, uint32_t tail_t, bool campaign = true);
 
     void set_run_start(uint64_t start);
     /// Tail prime indices without a campaign first occurrence at or below the
@@ -97,6 +100,9 @@ public:
     /// Merges the run into the campaign state and appends its log line
     /// (nothing when the range overlaps a covered one).
     void commit(const char* version, uint64_t sampled_total);
+    /// A run without campaign files: the state file text a fresh campaign
+    /// holds after committing this run alone (same checks as commit()).
+    std::string memory_commit(uint64_t sampled_total);
     std::string log_line(const char* version) const;
     void write_live(uint64_t start, uint64_t limit, uint64_t done, uint64_t total, double elapsed, bool force);
 
@@ -115,6 +121,7 @@ private:
     std::vector<uint64_t> by_prime(const std::vector<uint64_t>& cls, const std::vector<uint64_t>* agg) const;
     void load_state();
     void save_state() const;
+    std::string state_text() const;
     struct GapCampaign {
         GapTotals g;
         uint64_t evens = 0, walked = 0;
@@ -132,6 +139,7 @@ private:
     size_t np_ = 0;
     std::vector<int32_t> pidx_;                // p >> 1 -> prime index, -1 none
     std::string state_path_, log_path_, rec_path_, live_path_;
+    bool campaign_ = true;
     // campaign
     std::vector<uint64_t> cum_samp_, cum_tail_, cum_tail_agg_;   // [cr * np + i], [cr * np + i], [i]
     std::vector<uint64_t> first_;

@@ -133,11 +133,14 @@ void refuse_removed_knobs()
 // wheel fraction of the harvest: multipliers coprime to 7, 11 and 13
 constexpr double kWheelFrac = (6.0 / 7) * (10.0 / 11) * (12.0 / 13);
 
+struct Range;
+
 // One segment of a batch, host side.
 struct SegJob {
+    Range* r = nullptr;          // the range it belongs to
     SegmentGeometry sg;
     ClassGeo cg;
-    uint64_t index = 0;
+    uint64_t index = 0;          // segment index within its range
     uint64_t n_gpu = 0;
     uint64_t gpu_evens = 0;
     bool sampled = false, walked = false, counted = false;
@@ -168,8 +171,8 @@ struct Engine::State {
     int cus = 256;
     Buf d_win, d_large, d_chunks, d_i0, d_map, d_coff, d_coffa, d_pending, d_scratch, d_hits, d_hit_n;
     Buf d_vmap, d_vhits, d_vhit_n, d_vchunks, h_vchunks;   // the GB_BUCKET_VERIFY rebuild
-    Buf d_geoms, d_cs;
-    Buf d_out[2], h_out[2], h_geoms[2], h_cs[2], h_chunks;
+    Buf d_geoms, d_cs, d_runs;
+    Buf d_out[2], h_out[2], h_geoms[2], h_cs[2], h_chunks[2], h_runs[2];
     Event ev[2][EV_N];
     Candidates cand;
 
@@ -235,178 +238,114 @@ struct Engine::State {
 
 namespace {
 
-// One run over [start, limit] on an engine's resident state (the
-// reference's DenseRun phases: resolve_range, statistics set-up, the
-// segment loop, finish and commit), issued in batches of segments.
-class DenseRun {
-public:
-    DenseRun(Engine::State& e, uint64_t limit, uint64_t segment_even, bool verbose, uint64_t start_even,
-             uint32_t split_arg, bool collect)
-        : E(e), limit_(limit), span_(segment_even), verbose_(verbose), start_(start_even), split_arg_(split_arg),
-          collect_(collect)
-    {
-    }
-    RunStats run();
-    /// Range resolution and the multiplier bitmap only; the build's seconds.
-    double prepare()
-    {
-        const auto t = clock_::now();
-        resolve_range();
-        return since(t);
-    }
-    /// The class map of segment [seg_start, ...] alone (validators).
-    uint32_t map_of(uint64_t seg_start, std::vector<uint64_t>& words, uint64_t Q[8], uint32_t nc[8]);
-
-private:
-    void resolve_range();
-    void plan_bitmap();
-    void configure_stats();
-    void configure_ridges();
-    void plan_buffers();
-    double hit_factor(uint32_t j) const;
-    double hit_density() const;
-    double compute_hit_density() const;
-    double hit_density_ = 0;
-    void enqueue(std::vector<SegJob>& batch, int slot, bool check_and_count);
-    void collect(std::vector<SegJob>& batch, int slot);
-    void collect_segment(const SegJob& j, const gbh_segout& o, double share[4]);
-
-    Engine::State& E;
-    uint64_t limit_, span_;
-    bool verbose_;
-    uint64_t start_;
-    uint32_t split_arg_;
-    bool collect_;
-    RunStats rs_;
-    uint64_t total_evens_ = 0;
-    clock_::time_point t_start_;
-    uint32_t nwin_ = 0, nlarge_ = 0, split_ = 0, ncoop_ = 0;   // this run's window / large prime counts
-    uint32_t wbits_ = 0, nwin_max_ = 0, cw_ = 0, cap_ = 0, pcap_ = 0, swords_ = 0, nchunks_ = 0, kb_ = 1;
-    // large primes: [0, nwheel_) walk the wheel, [nwheel_, nlarge_) read the multiplier bitmap from t_hl_ up
-    uint32_t nwheel_ = 0, t_hl_ = UINT32_MAX;
-    uint32_t stage_ = 0;       // harvest LDS stage per work-group (short batches: one walk per prime)
-    // GB_BUCKET_VERIFY=N: every Nth segment's map is rebuilt with every large
-    // prime on the wheel walk (no multiplier bitmap) and compared word for word
-    uint32_t verify_every_ = 0, cap_v_ = 0, nvchunks_ = 0;
-    uint64_t verified_ = 0;
-    void enqueue_verify(uint32_t s, const gbh_batch& b, gbh_segout* out);
-    double inv_wheel_ = 0, inv_large_ = 0;   // sum of 1/p over the wheel primes, over all large primes
-    std::shared_ptr<const MultBitmap> mb_;
-    bool stats_on_ = false, gaps_on_ = false, gaps_exact_ = false;
-    uint32_t sample_ = 256;
-    std::optional<GoldbachStats> gs_;
-    std::optional<RidgeTracker> ridges_;
-    std::string ridge_path_;
-    uint64_t sampled_total_ = 0, evens_done_ = 0;
-    bool phase_trace_ = false;
-    bool harvested_[2] = {false, false};
-    double t_sieve_ = 0, t_large_ = 0, t_check_ = 0, t_count_ = 0;
+// One range of a pipeline: the reference's DenseRun over [start, limit]
+// (src/verifier.cpp:2615-2642): its resolved segmentation, statistics and
+// ridge set-up, accounting and segment cursor. A plain run is one range; a
+// batched or served workload is many, sharing the pipeline's batches.
+struct Range {
+    size_t id = 0;
+    uint64_t start = 4, limit = 0, span = 0, total_evens = 0;
+    uint64_t q_lo = 3;                 // the first q of its first segment
+    uint32_t nwin = 0, nlarge = 0, split = 0;
+    bool campaign = true;              // statistics and ridge files (plain runs)
+    bool verbose = false, collect = false;
+    bool stats_on = false, gaps_on = false, gaps_exact = false;
+    uint32_t sample = 256;
+    std::optional<GoldbachStats> gs;
+    std::optional<RidgeTracker> ridges;
+    std::string ridge_path;
+    uint64_t sampled_total = 0, evens_done = 0;
+    double density = 0;                // expected harvest hits per class bit over its large primes
+    double inv_large = 0;              // sum of 1/p over its large primes
+    RunStats rs;
+    uint64_t next_start = 0, next_index = 0;
+    bool issued_all = false, last_collected = false, finished = false, started = false;
+    clock_::time_point t_submit, t_first;
+    double t_sieve = 0, t_large = 0, t_check = 0, t_count = 0;
+    std::string state_text;            // without campaign files: the state a fresh campaign holds after it
 };
 
-void DenseRun::resolve_range()
+// Range resolution (the reference's resolve_range): even bounds, the span
+// clamp to at least two segments, the run's window / large prime counts.
+void resolve_range(Engine::State& E, Range& r, uint32_t split_arg)
 {
     refuse_removed_knobs();
-    require_supported_limit(limit_);
-    limit_ &= ~uint64_t(1);
-    rs_.limit = limit_;
-    if (limit_ < 4) throw std::runtime_error("limit < 4: no even n >= 4 to verify");
-    if (limit_ > E.max_limit) throw std::runtime_error("limit above the engine's prime bound");
-    if (start_ < 4) start_ = 4;
-    if (start_ & 1) start_ += 1;
-    if (start_ > limit_) throw std::runtime_error("no even n in [start, limit]");
-    if (span_ < 2) span_ = 2;
-    if (span_ & 1) span_ += 1;
-    total_evens_ = (limit_ - start_) / 2 + 1;
+    require_supported_limit(r.limit);
+    r.limit &= ~uint64_t(1);
+    r.rs.limit = r.limit;
+    if (r.limit < 4) throw std::runtime_error("limit < 4: no even n >= 4 to verify");
+    if (r.limit > E.max_limit) throw std::runtime_error("limit above the engine's prime bound");
+    if (r.start < 4) r.start = 4;
+    if (r.start & 1) r.start += 1;
+    if (r.start > r.limit) throw std::runtime_error("no even n in [start, limit]");
+    uint64_t& span = r.span;
+    if (span < 2) span = 2;
+    if (span & 1) span += 1;
+    r.total_evens = (r.limit - r.start) / 2 + 1;
     // at least two segments (the span is an integer span: span / 2 evens)
-    if (span_ > total_evens_) span_ = std::max<uint64_t>(total_evens_ & ~uint64_t(1), 2);
-    if (start_ == 4) {
-        if (limit_ < 6)
+    if (span > r.total_evens) span = std::max<uint64_t>(r.total_evens & ~uint64_t(1), 2);
+    if (r.start == 4) {
+        if (r.limit < 6)
             throw std::runtime_error("the GPU path needs limit >= 6 (n = 4 = 2 + 2 has no odd-prime partition; "
                                      "--cpu-check 4 verifies it)");
-        span_ = std::max<uint64_t>(span_, 4);
+        span = std::max<uint64_t>(span, 4);
     }
-    if (span_ > E.max_span) throw std::runtime_error("segment span above the engine's buffers");
+    if (span > E.max_span) throw std::runtime_error("segment span above the engine's buffers");
     // a batch's positions are 32-bit half-offsets: one merged segment (below
     // two spans) must stay below 2^33 integers
-    if (span_ >= (uint64_t(1) << 32)) throw std::runtime_error("segment span of 2^32 integers or more");
+    if (span >= (uint64_t(1) << 32)) throw std::runtime_error("segment span of 2^32 integers or more");
     // this run's primes: those <= sqrt(limit) + 16
-    const uint32_t bound = std::max<uint32_t>(base_prime_bound(limit_), 64);
+    const uint32_t bound = std::max<uint32_t>(base_prime_bound(r.limit), 64);
     const uint32_t nb = uint32_t(std::upper_bound(E.base.begin(), E.base.end(), bound) - E.base.begin());
     uint32_t hi = std::min(E.win_hi, nb);
-    if (split_arg_ > 0) {          // --split: the first large prime's index among the base primes
-        if (split_arg_ < E.win_lo || split_arg_ > E.win_hi)
+    if (split_arg > 0) {          // --split: the first large prime's index among the base primes
+        if (split_arg < E.win_lo || split_arg > E.win_hi)
             throw std::runtime_error("--split must lie in [" + std::to_string(E.win_lo) + ", " +
                                      std::to_string(E.win_hi) + "] on this engine");
-        if (split_arg_ < hi)
+        if (split_arg < hi)
             std::fprintf(stderr, "note: --split below the engine's window bound: the windows keep their primes\n");
     }
-    nwin_ = hi > E.win_lo ? hi - E.win_lo : 0;
-    split_ = E.win_lo + nwin_;
-    nlarge_ = nb > E.win_hi ? nb - E.win_hi : 0;
-    rs_.split_used = split_;
-    plan_bitmap();
-}
-
-// The large primes' split between the wheel walk and the multiplier bitmap:
-// a prime p >= t_hl lists p m with m prime, exact while every q < t_hl^3;
-// the bitmap covers m in [max(t_hl, sqrt(q_lo)), q_hi / t_hl] within its
-// budget (t_hl rises until it fits).
-void DenseRun::plan_bitmap()
-{
-    nwheel_ = nlarge_;
-    t_hl_ = UINT32_MAX;
-    mb_.reset();
-    if (!nlarge_) return;
-    const uint64_t budget = mult_bitmap_budget();
-    if (!budget) return;
-    const uint64_t q_lo = start_ >= MAX_GB_PRIME + 2 ? start_ - MAX_GB_PRIME : 3, q_hi = limit_ - 3;
-    const uint64_t m_cap = (budget - std::min<uint64_t>(budget, uint64_t(1) << 24)) * 30;   // bitmap bytes -> m span
-    const uint64_t sq_lo = uint64_t(std::sqrt(double(q_lo)));
-    const uint32_t* lp = E.base.data() + E.win_hi;
-    // smallest bitmap prime: its cube above q_hi, and the m span within the budget
-    const auto cube_le = [](uint64_t x, uint64_t q) { return (unsigned __int128)x * x * x <= q; };
-    uint64_t cb = uint64_t(std::cbrt(double(q_hi)));
-    while (cube_le(cb + 1, q_hi)) ++cb;
-    while (cb > 1 && !cube_le(cb, q_hi)) --cb;
-    const uint64_t t = std::max<uint64_t>({lp[0], cb + 1, q_hi / (m_cap + sq_lo) + 1});
-    nwheel_ = uint32_t(std::lower_bound(lp, lp + nlarge_, uint32_t(std::min<uint64_t>(t, UINT32_MAX))) - lp);
-    if (nwheel_ == nlarge_) return;
-    t_hl_ = lp[nwheel_];
-    const uint64_t m_lo = std::max<uint64_t>(t_hl_, sq_lo > 1 ? sq_lo - 1 : 1), m_hi = q_hi / t_hl_;
-    if (m_hi < m_lo) { nwheel_ = nlarge_; t_hl_ = UINT32_MAX; return; }
-    mb_ = mult_bitmap(m_lo, m_hi, E.base, verbose_ || phase_trace_);
+    r.nwin = hi > E.win_lo ? hi - E.win_lo : 0;
+    r.split = E.win_lo + r.nwin;
+    r.nlarge = nb > E.win_hi ? nb - E.win_hi : 0;
+    r.rs.split_used = r.split;
+    r.q_lo = r.start >= MAX_GB_PRIME + 2 ? r.start - MAX_GB_PRIME : 3;
+    r.inv_large = r.nlarge ? E.inv_at(r.nlarge) : 0.0;
+    r.next_start = r.start;
 }
 
 // Goldbach ridges (the reference's configure_ridges_and_stats,
 // src/verifier.cpp:718-744): the floor is clamped above every phase-A
-// candidate, so above it phase B sees every even; the store is loaded after
-// the clamp.
-void DenseRun::configure_ridges()
+// candidate, so above it phase B sees every even; a campaign run loads the
+// store after the clamp.
+void configure_ridges(Engine::State& E, Range& r)
 {
-    ridges_.emplace(&E.cand.primes);
-    ridge_path_ = env::path("GB_RIDGE_FILE", "goldbach_ridges.csv");
+    r.ridges.emplace(&E.cand.primes);
     uint32_t pa_max = 0;
     for (uint32_t cr = 0; cr < 15; ++cr)
         pa_max = std::max(pa_max, E.cand.primes[E.cand.slot_pi[cr][std::min<size_t>(E.ka, E.cand.slot_pi[cr].size()) - 1]]);
-    if (ridges_->active()) {
-        ridges_->clamp_floor_above(pa_max);
-        ridges_->load(ridge_path_);
+    if (r.ridges->active()) {
+        r.ridges->clamp_floor_above(pa_max);
+        if (r.campaign) {
+            r.ridge_path = env::path("GB_RIDGE_FILE", "goldbach_ridges.csv");
+            r.ridges->load(r.ridge_path);
+        }
     }
 }
 
-void DenseRun::configure_stats()
+void configure_stats(Engine::State& E, Range& r)
 {
     const bool stats_env = env::onoff("GB_STATS", true);
-    stats_on_ = start_ >= STATS_MIN_START && stats_env;
+    r.stats_on = r.start >= STATS_MIN_START && stats_env;
     const std::string gaps_env = env::choice("GB_GAPS", "exact", {"exact", "1", "sampled", "0"});
-    gaps_on_ = stats_on_ && gaps_env != "0";
-    gaps_exact_ = gaps_on_ && gaps_env != "sampled";
-    sample_ = env::u32("GB_STATS_SAMPLE", 256, 1);
-    if (stats_on_) {
-        gs_.emplace(E.cand.primes, CHECK_TAIL);
-        gs_->set_run_start(start_);
-        if (gaps_on_) gs_->begin_gaps(start_);
-    } else if (start_ < STATS_MIN_START && !env::present("GB_STATS")) {
+    r.gaps_on = r.stats_on && gaps_env != "0";
+    r.gaps_exact = r.gaps_on && gaps_env != "sampled";
+    r.sample = env::u32("GB_STATS_SAMPLE", 256, 1);
+    if (r.stats_on) {
+        r.gs.emplace(E.cand.primes, CHECK_TAIL, r.campaign);
+        r.gs->set_run_start(r.start);
+        if (r.gaps_on) r.gs->begin_gaps(r.start);
+    } else if (r.campaign && r.start < STATS_MIN_START && !env::present("GB_STATS")) {
         std::fprintf(stderr, "note: Goldbach statistics off below start %llu\n", (unsigned long long)STATS_MIN_START);
     }
     // every phase-A candidate must lie below the tail bound: evens with
@@ -416,40 +355,177 @@ void DenseRun::configure_stats()
             throw std::runtime_error("GB_KA too large: a phase-A candidate reaches the tail bound");
 }
 
+// What a pipeline is planned for: the largest q and its prime counts, the
+// smallest q whose range has large primes, the largest span, the segments a
+// plain run's batches hold, and the first q of the ranges whose hit density
+// sizes the lists.
+struct PlanBounds {
+    uint64_t q_hi = 0;
+    uint64_t sq_lo = 0;               // floor(sqrt(smallest q_lo with large primes)); 0: no large primes
+    uint64_t span = 0;
+    uint32_t nwin = 0, nlarge = 0;
+    uint64_t nseg = 1;
+    std::vector<std::pair<uint64_t, uint32_t>> q_lo_np;   // (q_lo, large primes) of the ranges to serve
+};
+
+PlanBounds bounds_of(const Range& r)
+{
+    PlanBounds b;
+    b.q_hi = r.limit - 3;
+    b.sq_lo = r.nlarge ? std::max<uint64_t>(uint64_t(std::sqrt(double(r.q_lo))), 1) : 0;
+    b.span = r.span;
+    b.nwin = r.nwin;
+    b.nlarge = r.nlarge;
+    b.nseg = (r.total_evens + r.span / 2 - 1) / std::max<uint64_t>(r.span / 2, 1) + 1;
+    b.q_lo_np.push_back({r.q_lo, r.nlarge});
+    return b;
+}
+
+// Harvest cost of one run of a batch: its large primes, class bits, first q
+// and hit density.
+struct RunCost {
+    uint32_t np = 0;
+    double bits = 0, q_lo = 3, density = 0;
+    bool operator==(const RunCost& o) const { return np == o.np && bits == o.bits && q_lo == o.q_lo && density == o.density; }
+};
+
+// The segment pipeline of one engine: batches of segments (of one range, or
+// of several ranges as runs), each batch one launch per stage on one
+// in-order HIP stream while the host collects the previous batch (two
+// output slots), fail-closed coverage accounting, the leftover CPU fallback
+// and the exact statistics of every range.
+class Pipeline {
+public:
+    Pipeline(Engine::State& e, bool multi) : E(e), multi_(multi) { phase_trace_ = env::present("GB_PHASE"); }
+    /// The large primes' split and the multiplier bitmap for @p b.
+    void plan_bitmap(const PlanBounds& b);
+    /// Bitmap, window size, list capacities and buffers for @p b.
+    void plan(const PlanBounds& b);
+    double density_of(uint64_t q_lo, uint32_t nlarge) const;
+    void add(Range* r) { active_.push_back(r); }
+    /// Issues the next batch of waiting segments and collects the batch
+    /// before it; ranges whose last segment was collected are finished and
+    /// appended to @p done. @return whether anything is waiting or in flight.
+    bool step(std::vector<Range*>* done);
+    /// The class map of one segment of @p r alone (validators).
+    uint32_t map_of(Range& r, uint64_t seg_start, std::vector<uint64_t>& words, uint64_t Q[8], uint32_t nc[8]);
+    uint32_t kb() const { return kb_; }
+    const RunStats& totals() const { return agg_; }
+
+private:
+    double hit_factor(uint32_t j, double q_lo) const;
+    void plan_chunks(const std::vector<RunCost>& runs, int slot);
+    uint32_t run_cap(const Range& r) const;
+    void form(std::vector<SegJob>& bt);
+    void enqueue(std::vector<SegJob>& batch, int slot, bool check_and_count);
+    void enqueue_verify(uint32_t s, const gbh_batch& b, gbh_segout* out);
+    void collect(std::vector<SegJob>& batch, int slot);
+    void collect_segment(const SegJob& j, const gbh_segout& o, const double share[4]);
+    void finish_range(Range& r);
+
+    Engine::State& E;
+    bool multi_;
+    bool phase_trace_ = false;
+    std::vector<Range*> active_;       // ranges with segments not yet issued, in submission order
+    std::vector<SegJob> batches_[2];
+    int cur_ = 0;
+    bool pending_ = false;
+    RunStats agg_;                     // kernel times and bytes over every batch
+    uint32_t nwin_ = 0, nlarge_ = 0, ncoop_ = 0;   // the largest range's window / large prime counts
+    uint32_t wbits_ = 0, nwin_max_ = 0, cw_ = 0, cap_ = 0, pcap_ = 0, swords_ = 0, kb_ = 1;
+    uint64_t span_ = 0, evens_cap_ = UINT64_MAX;
+    // large primes: [0, nwheel_) walk the wheel, [nwheel_, ...) read the multiplier bitmap from t_hl_ up
+    uint32_t nwheel_ = 0, t_hl_ = UINT32_MAX;
+    double inv_wheel_ = 0;             // sum of 1/p over the wheel primes
+    double density_max_ = 0;
+    std::shared_ptr<const MultBitmap> mb_;
+    // harvest chunks of the batch in each slot (re-planned when the runs change)
+    std::vector<RunCost> chunk_key_;
+    std::vector<uint32_t> chunks_;
+    uint32_t nchunks_ = 0, stage_ = 0;
+    bool chunks_valid_ = false;
+    // GB_BUCKET_VERIFY=N: every Nth segment's map is rebuilt with every large
+    // prime on the wheel walk (no multiplier bitmap) and compared word for word
+    uint32_t verify_every_ = 0, cap_v_ = 0, nvchunks_ = 0;
+    uint64_t verified_ = 0;
+    bool harvested_[2] = {false, false};
+};
+
+// The large primes' split between the wheel walk and the multiplier bitmap:
+// a prime p >= t_hl lists p m with m prime, exact while every q < t_hl^3;
+// the bitmap covers m in [max(t_hl, sqrt(q_lo)), q_hi / t_hl] within its
+// budget (t_hl rises until it fits). One split serves every range of the
+// pipeline (each range's multipliers lie inside the bitmap's span).
+void Pipeline::plan_bitmap(const PlanBounds& b)
+{
+    nlarge_ = b.nlarge;
+    nwheel_ = nlarge_;
+    t_hl_ = UINT32_MAX;
+    mb_.reset();
+    if (!nlarge_ || !b.sq_lo) return;
+    const uint64_t budget = mult_bitmap_budget();
+    if (!budget) return;
+    const uint64_t q_hi = b.q_hi, sq_lo = b.sq_lo;
+    const uint64_t m_cap = (budget - std::min<uint64_t>(budget, uint64_t(1) << 24)) * 30;   // bitmap bytes -> m span
+    const uint32_t* lp = E.base.data() + E.win_hi;
+    // smallest bitmap prime: its cube above q_hi, and the m span within the budget
+    const auto cube_le = [](uint64_t x, uint64_t q) { return (unsigned __int128)x * x * x <= q; };
+    uint64_t cb = uint64_t(std::cbrt(double(q_hi)));
+    while (cube_le(cb + 1, q_hi)) ++cb;
+    while (cb > 1 && !cube_le(cb, q_hi)) --cb;
+    uint64_t t = std::max<uint64_t>({lp[0], cb + 1, q_hi / (m_cap + sq_lo) + 1});
+    // a resident bitmap serves this range too when t rises to its span
+    // (m >= its base, q_hi / t below its end): taken when t rises at most
+    // 8-fold (the primes in between walk the wheel instead)
+    if (const auto cur = mult_bitmap_peek()) {
+        const uint64_t m_end = cur->m_base + cur->m_span;   // exclusive
+        uint64_t tc = std::max(t, q_hi / (m_end - 1) + 1);
+        if (sq_lo < cur->m_base + 1) tc = std::max(tc, cur->m_base);
+        if (tc <= 8 * t) t = tc;
+    }
+    nwheel_ = uint32_t(std::lower_bound(lp, lp + nlarge_, uint32_t(std::min<uint64_t>(t, UINT32_MAX))) - lp);
+    if (nwheel_ == nlarge_) return;
+    t_hl_ = lp[nwheel_];
+    const uint64_t m_lo = std::max<uint64_t>(t_hl_, sq_lo > 1 ? sq_lo - 1 : 1), m_hi = q_hi / t_hl_;
+    if (m_hi < m_lo) { nwheel_ = nlarge_; t_hl_ = UINT32_MAX; return; }
+    mb_ = mult_bitmap(m_lo, m_hi, E.base, phase_trace_);
+}
+
 // Expected hits per class bit and unit 1/p of large prime j (the j-th above
 // the window bound): the wheel lists every multiplier coprime to 7 11 13,
 // the bitmap lists prime multipliers, (30/8) / ln m of those coprime to 30
 // (m = q / p >= max(q_lo / p, p), the bound that makes the rate largest).
-double DenseRun::hit_factor(uint32_t j) const
+double Pipeline::hit_factor(uint32_t j, double q_lo) const
 {
     if (j < nwheel_) return kWheelFrac;
     const double p = E.base[E.win_hi + j];
-    const double q_lo = double(start_ >= MAX_GB_PRIME + 2 ? start_ - MAX_GB_PRIME : 3);
     return 3.75 / std::log(std::max({q_lo / p, p, 3.0}));
 }
 
-// Expected hits per class bit over all large primes: the wheel primes
-// exactly, the bitmap primes in 64 bands of the prime index, each at its
-// largest rate.
-double DenseRun::hit_density() const { return hit_density_; }
-
-double DenseRun::compute_hit_density() const
+// Expected hits per class bit over the first nlarge large primes of a range
+// starting at q_lo: the wheel primes exactly, the bitmap primes in 64 bands
+// of the prime index, each at its largest rate.
+double Pipeline::density_of(uint64_t q_lo, uint32_t nlarge) const
 {
-    if (!nlarge_) return 0.0;
-    double d = kWheelFrac * E.inv_at(nwheel_);
-    const uint32_t n = nlarge_ - nwheel_;
+    if (!nlarge) return 0.0;
+    const uint32_t nw = std::min(nwheel_, nlarge);
+    double d = kWheelFrac * E.inv_at(nw);
+    const uint32_t n = nlarge - nw;
     for (uint32_t k = 0; k < 64 && n; ++k) {
-        const uint32_t a = nwheel_ + uint32_t(uint64_t(n) * k / 64), b = nwheel_ + uint32_t(uint64_t(n) * (k + 1) / 64);
+        const uint32_t a = nw + uint32_t(uint64_t(n) * k / 64), b = nw + uint32_t(uint64_t(n) * (k + 1) / 64);
         if (b <= a) continue;
-        d += (E.inv_at(b) - E.inv_at(a)) * std::max(hit_factor(a), hit_factor(b - 1));
+        d += (E.inv_at(b) - E.inv_at(a)) * std::max(hit_factor(a, double(q_lo)), hit_factor(b - 1, double(q_lo)));
     }
     return d;
 }
 
-// Window size, strides, hit-list capacity and buffer sizes of this run.
-void DenseRun::plan_buffers()
+// Window size, strides, hit-list capacity and buffer sizes.
+void Pipeline::plan(const PlanBounds& pb)
 {
-    const uint64_t nseg = std::min<uint64_t>(E.kbatch, (total_evens_ + span_ / 2 - 1) / std::max<uint64_t>(span_ / 2, 1) + 1);
+    plan_bitmap(pb);
+    nwin_ = pb.nwin;
+    span_ = pb.span;
+    const uint64_t nseg = std::min<uint64_t>(multi_ ? GBH_MAX_BATCH : E.kbatch, pb.nseg);
     // the largest segment: the merged last one spans below 2 x span
     const uint64_t qodds_max = (2 * span_ + MAX_GB_PRIME) / 2 + 2;
     const uint64_t nc_max = (qodds_max + 14) / 15 + 1;
@@ -460,7 +536,8 @@ void DenseRun::plan_buffers()
         wbits_ = E.win_kb * 8192;
     } else {
         wbits_ = 1u << 19;
-        while (wbits_ > (1u << 15) && nseg * 8 * ((nc_typ + wbits_ - 1) / wbits_) < 2ull * E.cus) wbits_ >>= 1;
+        const uint64_t nseg_typ = multi_ ? E.kbatch : nseg;
+        while (wbits_ > (1u << 15) && nseg_typ * 8 * ((nc_typ + wbits_ - 1) / wbits_) < 2ull * E.cus) wbits_ >>= 1;
     }
     nwin_max_ = uint32_t((nc_max + wbits_ - 1) / wbits_);
     while (nwin_max_ > 256 && wbits_ < (1u << 20)) {   // the window split's LDS tables
@@ -471,18 +548,25 @@ void DenseRun::plan_buffers()
     cw_ = nwin_max_ * (wbits_ / 64) + 16;
     ncoop_ = 0;
     while (ncoop_ < nwin_ && E.base[E.win_lo + ncoop_] < wbits_ / 128) ++ncoop_;
-    hit_density_ = compute_hit_density();
-    const double lam = wbits_ * hit_density();
-    cap_ = uint32_t(std::min(lam * 1.15 + 8 * std::sqrt(lam) + 1024, 4e9 / (double(E.kbatch) * 8 * nwin_max_)));
-    const uint64_t evens_max = span_ + 2;
-    pcap_ = uint32_t(std::min<uint64_t>(evens_max / 32 + 65536, UINT32_MAX / 2 / E.kbatch));
-    swords_ = uint32_t(2 * (nc_max / 64 + 4));
-    gbh_ctx* c = E.c;
+    inv_wheel_ = E.inv_at(nwheel_);
+    density_max_ = 0;
+    for (const auto& [q_lo, np] : pb.q_lo_np) density_max_ = std::max(density_max_, density_of(q_lo, np));
     // a batch's q range stays below 2^33 (the harvest's 32-bit half-offsets):
     // at most kb_ - 1 spans plus the merged last segment (below 2 spans)
     const uint64_t kmax = std::max<uint64_t>(1, ((uint64_t(1) << 33) - 4 * MAX_GB_PRIME) / span_ - 1);
-    kb_ = uint32_t(std::min<uint64_t>({nseg, kmax, GBH_HL_BUCKETS / (8 * nwin_max_)}));
-    if (!kb_) throw std::runtime_error("segment span too large for the hit-list work-group");
+    if (multi_) {
+        kb_ = GBH_MAX_BATCH;
+        evens_cap_ = uint64_t(E.kbatch) * (span_ / 2);
+    } else {
+        kb_ = uint32_t(std::min<uint64_t>({nseg, kmax, GBH_HL_BUCKETS / (8 * nwin_max_)}));
+    }
+    if (!kb_ || GBH_HL_BUCKETS / (8 * nwin_max_) == 0) throw std::runtime_error("segment span too large for the hit-list work-group");
+    const double lam = wbits_ * density_max_;
+    cap_ = uint32_t(std::min(lam * 1.15 + 8 * std::sqrt(lam) + 1024, 4e9 / (double(kb_) * 8 * nwin_max_)));
+    const uint64_t evens_max = span_ + 2;
+    pcap_ = uint32_t(std::min<uint64_t>(evens_max / 32 + 65536, UINT32_MAX / 2 / kb_));
+    swords_ = uint32_t(2 * (nc_max / 64 + 4));
+    gbh_ctx* c = E.c;
     const size_t K = kb_;
     E.d_map.ensure(c, K * 8 * cw_ * 8, 0);
     E.d_i0.ensure(c, K * 8 * size_t(std::max(nwin_, 1u)) * 4, 0);
@@ -490,50 +574,132 @@ void DenseRun::plan_buffers()
     E.d_scratch.ensure(c, K * size_t(swords_) * 4, 0);
     E.d_hit_n.ensure(c, K * 8 * size_t(nwin_max_) * 4, 0);
     if (nlarge_) E.d_hits.ensure(c, K * 8 * size_t(nwin_max_) * cap_ * 4, 0);
-    const auto tp0 = clock_::now();
-    // harvest chunks: contiguous prime ranges of about equal cost (a fixed
-    // cost per prime plus its expected hits; a wheel step costs about as
-    // much as a hit), enough chunks to fill every CU several times, 256 to
-    // GBH_HL_CHUNK primes each, split where the bitmap primes start
-    std::vector<uint32_t> ch{0};
-    inv_wheel_ = E.inv_at(nwheel_);
-    inv_large_ = E.inv_at(nlarge_);
-    if (nlarge_) {
-        const double hits_unit = double(K) * double(span_) * (8.0 / 30);   // expected hits of prime p: unit * factor / p
-        constexpr double kPrimeCost = 4.0;
-        // sum of 1/p up to j: exact at the split and the end, coarse (every
-        // 256 primes) elsewhere, where chunk ends fall
-        const auto inv = [&](uint32_t j) {
-            return (j & 255) == 0 ? E.inv_coarse[j >> 8]
-                 : j == nwheel_ ? inv_wheel_ : j == nlarge_ ? inv_large_ : E.inv_at(j);
-        };
-        const auto factor = [&](uint32_t a) { return a < nwheel_ ? 1.3 * kWheelFrac : hit_factor(a); };
-        double total = 0;
-        for (uint32_t k = 0; k < 256; ++k) {
-            uint32_t a = uint32_t(uint64_t(nlarge_) * k / 256) & ~255u, b = uint32_t(uint64_t(nlarge_) * (k + 1) / 256) & ~255u;
-            if (k == 255) b = nlarge_;
-            if (a < nwheel_ && b > nwheel_) {
-                total += kPrimeCost * (nwheel_ - a) + hits_unit * factor(a) * (inv(nwheel_) - inv(a));
-                a = nwheel_;
+    E.d_geoms.ensure(c, sizeof(gbh_geom) * K, 0);
+    E.d_cs.ensure(c, sizeof(gbh_countseg) * K, 0);
+    E.d_coff.ensure(c, K * 15 * GBH_NCAND * 4, 0);
+    E.d_coffa.ensure(c, K * 15 * GBH_NCAND_PAD * 4, 0);
+    E.d_runs.ensure(c, sizeof(gbh_hl_run) * K, 0);
+    for (int s = 0; s < 2; ++s) {
+        E.d_out[s].ensure(c, sizeof(gbh_segout) * K, 0);
+        E.h_out[s].ensure(c, sizeof(gbh_segout) * K, 1);
+        E.h_geoms[s].ensure(c, sizeof(gbh_geom) * K, 1);
+        E.h_cs[s].ensure(c, sizeof(gbh_countseg) * K, 1);
+        E.h_runs[s].ensure(c, sizeof(gbh_hl_run) * K, 1);
+    }
+    chunks_valid_ = false;
+    if (!multi_ && nlarge_) {   // a plain run's batches share one chunk plan: make it now
+        const auto tp0 = clock_::now();
+        plan_chunks({RunCost{nlarge_, double(kb_) * double(span_) * (8.0 / 30), double(pb.q_lo_np[0].first),
+                             density_max_}}, 0);
+        if (phase_trace_) std::fprintf(stderr, "plan: chunking %.3f ms\n", since(tp0) * 1e3);
+    }
+    if (phase_trace_)
+        std::fprintf(stderr, "plan: %u segments per batch, window %u bits, %u window primes, %u large (%u on the wheel, "
+                     "bitmap from %u), %u harvest chunks, list cap %u, stage %u\n", kb_, wbits_, nwin_, nlarge_,
+                     nwheel_, t_hl_ == UINT32_MAX ? 0u : t_hl_, nchunks_, cap_, stage_);
+    verify_every_ = env::present("GB_BUCKET_VERIFY") ? env::u32("GB_BUCKET_VERIFY", 0, 0) : 0u;
+    if (verify_every_ && nlarge_) {
+        // one segment, every large prime on the wheel: its own lists and chunks
+        const double inv_large = E.inv_at(nlarge_);
+        const double lam_v = wbits_ * kWheelFrac * inv_large;
+        cap_v_ = uint32_t(lam_v * 1.15 + 8 * std::sqrt(lam_v) + 1024);
+        E.d_vmap.ensure(c, size_t(8) * cw_ * 8, 0);
+        E.d_vhits.ensure(c, size_t(8) * nwin_max_ * cap_v_ * 4, 0);
+        E.d_vhit_n.ensure(c, size_t(8) * nwin_max_ * 4, 0);
+        const double unit = double(span_) * (8.0 / 30) * 1.3 * kWheelFrac;
+        const double target = std::max((4.0 * nlarge_ + unit * inv_large) / (8.0 * E.cus), 2e4);
+        std::vector<uint32_t> vc{0};
+        for (uint32_t j = 0; j < nlarge_;) {
+            const uint32_t hi = std::min<uint32_t>(nlarge_, j + GBH_HL_CHUNK);
+            const double ij = E.inv_at(j);
+            uint32_t e = std::min(j + 256, hi);
+            while (e < hi) {   // grow by 256 primes while within the target
+                const uint32_t nx = std::min(e + 256, hi);
+                const double inx = nx == hi ? E.inv_at(nx) : E.inv_coarse[nx >> 8];
+                if (4.0 * (nx - j) + unit * (inx - ij) > target && (nx & 255) == 0) break;
+                e = nx;
             }
-            if (b > a) total += kPrimeCost * (b - a) + hits_unit * std::max(factor(a), factor(b - 1)) * (inv(b) - inv(a));
+            vc.push_back(e);
+            j = e;
+        }
+        nvchunks_ = uint32_t(vc.size() - 1);
+        E.h_vchunks.ensure(c, vc.size() * 4, 1);
+        E.d_vchunks.ensure(c, vc.size() * 4, 0);
+        std::memcpy(E.h_vchunks.p, vc.data(), vc.size() * 4);
+        check(gbh_copy_h2d(c, 0, E.d_vchunks.p, E.h_vchunks.p, vc.size() * 4), "upload verify chunks");
+    }
+    check(gbh_stream_sync(c, 0), "plan sync");
+}
+
+// Harvest chunks: contiguous prime ranges of about equal cost (a fixed cost
+// per prime and run plus the expected hits; a wheel step costs about as
+// much as a hit), enough chunks to fill every CU several times, 256 to
+// GBH_HL_CHUNK primes each, split where the bitmap primes start and where a
+// run's primes end. Uploaded for the batch in @p slot (stream-ordered after
+// the batch before it).
+void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, int slot)
+{
+    if (chunks_valid_ && runs_in == chunk_key_) return;
+    chunk_key_ = runs_in;
+    chunks_.assign(1, 0u);
+    std::vector<RunCost> rs;
+    for (const RunCost& r : runs_in)
+        if (r.np) rs.push_back(r);
+    std::sort(rs.begin(), rs.end(), [](const RunCost& a, const RunCost& b) { return a.np < b.np; });
+    stage_ = 0;
+    if (!rs.empty()) {
+        constexpr double kPrimeCost = 4.0;
+        const size_t nr = rs.size();
+        const uint32_t npmax = rs.back().np;
+        // runs [k, nr) are the ones serving primes below rs[k].np: their
+        // summed class bits, largest bits and smallest first q
+        std::vector<double> suf_bits(nr + 1, 0.0), suf_max(nr + 1, 0.0), suf_qlo(nr + 1, 1e30);
+        for (size_t k = nr; k-- > 0;) {
+            suf_bits[k] = suf_bits[k + 1] + rs[k].bits;
+            suf_max[k] = std::max(suf_max[k + 1], rs[k].bits);
+            suf_qlo[k] = std::min(suf_qlo[k + 1], rs[k].q_lo);
+        }
+        // sum of 1/p up to j: coarse (every 256 primes) where chunk ends fall, exact elsewhere
+        const auto inv = [&](uint32_t j) { return (j & 255) == 0 ? E.inv_coarse[j >> 8] : j == nwheel_ ? inv_wheel_ : E.inv_at(j); };
+        const auto factor = [&](uint32_t a, double q_lo) { return a < nwheel_ ? 1.3 * kWheelFrac : hit_factor(a, q_lo); };
+        double total = 0;
+        {
+            size_t ai = 0;
+            for (uint32_t k = 0; k < 256; ++k) {
+                uint32_t a = uint32_t(uint64_t(npmax) * k / 256) & ~255u, b = uint32_t(uint64_t(npmax) * (k + 1) / 256) & ~255u;
+                if (k == 255) b = npmax;
+                while (a < b) {
+                    while (rs[ai].np <= a) ++ai;
+                    uint32_t e = std::min(b, rs[ai].np);
+                    if (a < nwheel_) e = std::min(e, nwheel_);
+                    const double q = suf_qlo[ai];
+                    total += kPrimeCost * (e - a) * double(nr - ai) +
+                             suf_bits[ai] * std::max(factor(a, q), factor(e - 1, q)) * (inv(e) - inv(a));
+                    a = e;
+                }
+            }
         }
         const double target = std::max(total / (8.0 * E.cus), 2e4);
-        // a short batch's hits fit the work-groups' LDS stages: chunks of at
-        // most kStageHits expected hits, one walk per prime
+        // short batches' hits fit the work-groups' LDS stages: chunks of at
+        // most kStageHits expected hits per run, one walk per prime
         constexpr double kStageHits = 0.7 * GBH_HL_STAGE;
-        const double all_hits = hits_unit * hit_density();
+        double all_hits = 0;
+        for (const RunCost& r : rs) all_hits += r.bits * r.density;
         stage_ = all_hits <= kStageHits * 8192 ? GBH_HL_STAGE : 0u;
         uint32_t j = 0, prev = 0;   // prev: the last chunk's length (lengths change slowly: start the search there)
-        while (j < nlarge_) {
-            uint32_t hi = std::min<uint32_t>(nlarge_, j + GBH_HL_CHUNK);
+        size_t ai = 0;
+        while (j < npmax) {
+            while (rs[ai].np <= j) ++ai;   // runs [ai, nr) serve prime j
+            uint32_t hi = std::min<uint32_t>(npmax, j + GBH_HL_CHUNK);
             if (j < nwheel_) hi = std::min(hi, nwheel_);
-            const double hf = hit_factor(j);
+            hi = std::min(hi, rs[ai].np);
+            const double nact = double(nr - ai), bits = suf_bits[ai], bmax = suf_max[ai], q = suf_qlo[ai];
+            const double hf = hit_factor(j, q);
             const double f = j < nwheel_ ? 1.3 * kWheelFrac : hf, fh = hf;
             const double ij = j == nwheel_ ? inv_wheel_ : E.inv_at(j);
             const auto ok = [&](uint32_t e) {   // chunk [j, e) within the targets (the rates barely change inside)
                 const double si = inv(e) - ij;
-                return kPrimeCost * (e - j) + hits_unit * f * si <= target && (!stage_ || hits_unit * fh * si <= kStageHits);
+                return kPrimeCost * (e - j) * nact + bits * f * si <= target && (!stage_ || bmax * fh * si <= kStageHits);
             };
             uint32_t e;
             if (ok(hi)) {
@@ -556,65 +722,111 @@ void DenseRun::plan_buffers()
                 }
             }
             prev = e - j;
-            ch.push_back(e);
+            chunks_.push_back(e);
             j = e;
         }
     }
-    const double t_chunks = since(tp0);
-    nchunks_ = uint32_t(ch.size() - 1);
-    if (phase_trace_)
-        std::fprintf(stderr, "plan: %u segments per batch, window %u bits, %u window primes, %u large (%u on the wheel, "
-                     "bitmap from %u), %u harvest chunks, list cap %u, stage %u\n", kb_, wbits_, nwin_, nlarge_,
-                     nwheel_, t_hl_ == UINT32_MAX ? 0u : t_hl_, nchunks_, cap_, stage_);
-    if (phase_trace_) std::fprintf(stderr, "plan: chunking %.3f ms\n", t_chunks * 1e3);
-    E.h_chunks.ensure(c, ch.size() * 4, 1);
-    E.d_chunks.ensure(c, ch.size() * 4, 0);
-    std::memcpy(E.h_chunks.p, ch.data(), ch.size() * 4);
-    check(gbh_copy_h2d(c, 0, E.d_chunks.p, E.h_chunks.p, ch.size() * 4), "upload chunks");
-    verify_every_ = env::present("GB_BUCKET_VERIFY") ? env::u32("GB_BUCKET_VERIFY", 0, 0) : 0u;
-    if (verify_every_ && nlarge_) {
-        // one segment, every large prime on the wheel: its own lists and chunks
-        const double lam_v = wbits_ * kWheelFrac * inv_large_;
-        cap_v_ = uint32_t(lam_v * 1.15 + 8 * std::sqrt(lam_v) + 1024);
-        E.d_vmap.ensure(c, size_t(8) * cw_ * 8, 0);
-        E.d_vhits.ensure(c, size_t(8) * nwin_max_ * cap_v_ * 4, 0);
-        E.d_vhit_n.ensure(c, size_t(8) * nwin_max_ * 4, 0);
-        const double unit = double(span_) * (8.0 / 30) * 1.3 * kWheelFrac;
-        const double target = std::max((4.0 * nlarge_ + unit * inv_large_) / (8.0 * E.cus), 2e4);
-        std::vector<uint32_t> vc{0};
-        for (uint32_t j = 0; j < nlarge_;) {
-            const uint32_t hi = std::min<uint32_t>(nlarge_, j + GBH_HL_CHUNK);
-            const double ij = E.inv_at(j);
-            uint32_t e = std::min(j + 256, hi);
-            while (e < hi) {   // grow by 256 primes while within the target
-                const uint32_t nx = std::min(e + 256, hi);
-                const double inx = nx == hi ? E.inv_at(nx) : E.inv_coarse[nx >> 8];
-                if (4.0 * (nx - j) + unit * (inx - ij) > target && (nx & 255) == 0) break;
-                e = nx;
-            }
-            vc.push_back(e);
-            j = e;
-        }
-        nvchunks_ = uint32_t(vc.size() - 1);
-        E.h_vchunks.ensure(c, vc.size() * 4, 1);
-        E.d_vchunks.ensure(c, vc.size() * 4, 0);
-        std::memcpy(E.h_vchunks.p, vc.data(), vc.size() * 4);
-        check(gbh_copy_h2d(c, 0, E.d_vchunks.p, E.h_vchunks.p, vc.size() * 4), "upload verify chunks");
-    }
-    check(gbh_stream_sync(c, 0), "chunk sync");
+    nchunks_ = uint32_t(chunks_.size() - 1);
+    gbh_ctx* c = E.c;
+    E.h_chunks[slot].ensure(c, chunks_.size() * 4, 1);
+    E.d_chunks.ensure(c, chunks_.size() * 4, 0);
+    std::memcpy(E.h_chunks[slot].p, chunks_.data(), chunks_.size() * 4);
+    check(gbh_copy_h2d(c, 0, E.d_chunks.p, E.h_chunks[slot].p, chunks_.size() * 4), "upload chunks");
+    chunks_valid_ = true;
 }
 
-void DenseRun::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_count)
+// Segments of one range a batch may hold: the run's q range below 2^33 and
+// its window lists within the harvest work-group's counters.
+uint32_t Pipeline::run_cap(const Range& r) const
+{
+    const uint64_t kmax = std::max<uint64_t>(1, ((uint64_t(1) << 33) - 4 * MAX_GB_PRIME) / r.span - 1);
+    return uint32_t(std::min<uint64_t>({kmax, GBH_HL_BUCKETS / (8 * nwin_max_), kb_}));
+}
+
+// The next batch: waiting segments of the ranges in submission order, each
+// range's in order, up to kb_ segments (batched ranges: and about E.kbatch
+// full segments of evens).
+void Pipeline::form(std::vector<SegJob>& bt)
+{
+    bt.clear();
+    uint64_t evens = 0;
+    bool full = false;
+    for (Range* r : active_) {
+        uint32_t taken = 0;
+        const uint32_t cap = run_cap(*r);
+        while (!r->issued_all && bt.size() < kb_ && taken < cap) {
+            const SegmentGeometry sg = segment_geometry(r->next_start, r->limit, r->span);
+            if (multi_ && !bt.empty() && evens + sg.seg_evens > evens_cap_) { full = true; break; }
+            SegJob j;
+            j.r = r;
+            j.sg = sg;
+            j.index = r->next_index;
+            j.sampled = r->stats_on && j.index % r->sample == 0;
+            j.walked = r->gaps_on && (j.sampled || r->gaps_exact);
+            j.counted = r->stats_on;
+            bt.push_back(j);
+            evens += sg.seg_evens;
+            ++taken;
+            ++r->next_index;
+            r->rs.total_large_items += r->nlarge;
+            if (!r->started) {
+                r->started = true;
+                r->t_first = clock_::now();
+            }
+            if (sg.seg_end == r->limit) r->issued_all = true;
+            else r->next_start += r->span;
+        }
+        if (full || bt.size() >= kb_) break;
+    }
+    active_.erase(std::remove_if(active_.begin(), active_.end(), [](const Range* r) { return r->issued_all; }),
+                  active_.end());
+}
+
+bool Pipeline::step(std::vector<Range*>* done)
+{
+    std::vector<SegJob>& bt = batches_[cur_];
+    form(bt);
+    if (!bt.empty()) enqueue(bt, cur_, true);
+    if (pending_) {
+        std::vector<SegJob>& prev = batches_[cur_ ^ 1];
+        collect(prev, cur_ ^ 1);
+        for (const SegJob& j : prev)
+            if (j.r->last_collected && !j.r->finished) {
+                finish_range(*j.r);
+                if (done) done->push_back(j.r);
+            }
+    }
+    pending_ = !bt.empty();
+    cur_ ^= 1;
+    return pending_ || !active_.empty();
+}
+
+void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_count)
 {
     gbh_ctx* c = E.c;
     const uint32_t nseg = uint32_t(batch.size());
     gbh_geom* hg = E.h_geoms[slot].as<gbh_geom>();
     gbh_countseg* hc = E.h_cs[slot].as<gbh_countseg>();
-    uint32_t nwin = 1, nwords = 0;
+    gbh_hl_run* hr = E.h_runs[slot].as<gbh_hl_run>();
+    // batched ranges: a batch of short segments takes a window just above
+    // its longest class array (one window per class); plain runs keep wbits_
+    uint32_t wbits = wbits_;
+    if (multi_ && !E.win_kb) {
+        uint64_t ncm = 1;
+        for (SegJob& j : batch) {
+            j.cg = class_geometry(j.sg.q_first, j.sg.q_odds);
+            for (int k = 0; k < 8; ++k) ncm = std::max<uint64_t>(ncm, j.cg.n[k]);
+        }
+        uint32_t w = 1u << 13;
+        while (w < wbits_ && w < ncm) w <<= 1;
+        wbits = w;
+    }
+    uint32_t nwin = 1, nwords = 0, nruns = 0, run_max = 0;
     for (uint32_t s = 0; s < nseg; ++s) {
         SegJob& j = batch[s];
+        Range& r = *j.r;
         j.cg = class_geometry(j.sg.q_first, j.sg.q_odds);
-        gbh_geom g = make_geom(j.sg, j.cg, wbits_);
+        gbh_geom g = make_geom(j.sg, j.cg, wbits);
         // evens below q_first + 9780 (only at the very bottom of the range)
         // have candidate q outside the map: the CPU resolves them
         uint64_t n_gpu = std::max<uint64_t>(j.sg.seg_start, j.sg.q_first + 9780);
@@ -622,6 +834,11 @@ void DenseRun::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         g.n_gpu = n_gpu;
         j.n_gpu = n_gpu;
         g.hist_on = j.sampled ? 1u : 0u;
+        g.np_win = r.nwin;
+        // ridge records: a campaign run records from its store's current
+        // threshold; a run without campaign files from the floor (the same
+        // records however its segments are batched)
+        g.ridge_t = !r.ridges || !r.ridges->active() ? 0xFFFFFFFFu : r.campaign ? r.ridges->thresh() : r.ridges->floor();
         j.gpu_evens = n_gpu <= j.sg.seg_end ? (j.sg.seg_end - n_gpu) / 2 + 1 : 0;
         if (g.nwin > nwin_max_) throw std::runtime_error("segment longer than the run's buffers");
         nwin = std::max(nwin, g.nwin);
@@ -632,24 +849,43 @@ void DenseRun::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
             nwords = std::max(nwords, cs.nwords);
         }
         hc[s] = cs;
+        r.rs.total_tiles += uint64_t((j.cg.n[0] + wbits - 1) / wbits) * 8;
+        if (s == 0 || batch[s - 1].r != j.r) {
+            hr[nruns++] = gbh_hl_run{s, 0, r.nlarge, 0};
+        }
+        ++hr[nruns - 1].nseg;
+        run_max = std::max(run_max, hr[nruns - 1].nseg);
     }
-    gbh_batch b{E.d_geoms.as<gbh_geom>(), nseg, cw_, wbits_, nwin, E.d_map.as<uint64_t>()};
+    gbh_batch b{E.d_geoms.as<gbh_geom>(), nseg, cw_, wbits, nwin, E.d_map.as<uint64_t>()};
     gbh_segout* out = E.d_out[slot].as<gbh_segout>();
     auto& ev = E.ev[slot];
     check(gbh_event_record(c, 0, ev[EV_BEGIN].e), "event");
     harvested_[slot] = nlarge_ != 0;
-    {   // algorithmic bytes of this batch's harvest and window launches
-        const double lq = double(hg[nseg - 1].q_last - hg[0].q_first);
-        double bits = 0;
-        for (uint32_t s = 0; s < nseg; ++s)
-            for (int k = 0; k < 8; ++k) bits += hg[s].nc[k];
-        const double hits = hit_density() * bits;
-        const double s_bm = nlarge_ ? inv_large_ - inv_wheel_ : 0.0;
-        rs_.harvest_bytes += nlarge_ ? 4.0 * nlarge_ + 8.0 * ((nlarge_ - nwheel_) + lq * s_bm / 240) + 4.0 * hits : 0.0;
-        rs_.window_bytes += bits / 8 + 4.0 * nseg * 8 * nwin_ + 4.0 * hits;
+    std::vector<RunCost> costs;
+    {   // algorithmic bytes of this batch's harvest and window launches, per run
+        double bits_all = 0, hits_all = 0, win_prime_bytes = 0;
+        for (uint32_t k = 0; k < nruns; ++k) {
+            const gbh_hl_run& run = hr[k];
+            const Range& r = *batch[run.seg0].r;
+            const double lq = double(hg[run.seg0 + run.nseg - 1].q_last - hg[run.seg0].q_first);
+            double bits = 0;
+            for (uint32_t s = run.seg0; s < run.seg0 + run.nseg; ++s)
+                for (int x = 0; x < 8; ++x) bits += hg[s].nc[x];
+            const double hits = r.density * bits;
+            const uint32_t np = r.nlarge, nw = std::min(np, nwheel_);
+            const double s_bm = np > nwheel_ ? r.inv_large - inv_wheel_ : 0.0;
+            agg_.harvest_bytes += np ? 4.0 * np + 8.0 * ((np - nw) + lq * s_bm / 240) + 4.0 * hits : 0.0;
+            bits_all += bits;
+            hits_all += hits;
+            win_prime_bytes += 4.0 * run.nseg * 8 * r.nwin;
+            costs.push_back(RunCost{np, bits, double(r.q_lo), r.density});
+        }
+        agg_.window_bytes += bits_all / 8 + win_prime_bytes + 4.0 * hits_all;
     }
+    if (multi_ && nlarge_) plan_chunks(costs, slot);
     check(gbh_copy_h2d(c, 0, E.d_geoms.p, hg, sizeof(gbh_geom) * nseg), "upload geometry");
     check(gbh_copy_h2d(c, 0, E.d_cs.p, hc, sizeof(gbh_countseg) * nseg), "upload count parameters");
+    check(gbh_copy_h2d(c, 0, E.d_runs.p, hr, sizeof(gbh_hl_run) * nruns), "upload runs");
     check(gbh_reset_outputs(c, 0, out, nseg), "reset outputs");
     if (nwin_) {
         gbh_setup_args a{b, E.d_win.as<uint32_t>(), nwin_, 0, E.d_i0.as<uint32_t>()};
@@ -657,15 +893,17 @@ void DenseRun::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     }
     if (nlarge_) {
         check(gbh_zero(c, 0, E.d_hit_n.p, size_t(nseg) * 8 * nwin * 4), "zero hit counts");
-        gbh_harvest_args h{b, E.d_large.as<uint32_t>(), E.d_chunks.as<uint32_t>(), nchunks_, cap_, t_hl_, stage_,
-                           mb_ ? mb_->bits : nullptr, mb_ ? mb_->m_base : 0, mb_ ? mb_->m_span : 0,
-                           E.d_hits.as<uint32_t>(), E.d_hit_n.as<uint32_t>(), &out[0].err};
+        gbh_harvest_args h{b, E.d_large.as<uint32_t>(), E.d_chunks.as<uint32_t>(), E.d_runs.as<gbh_hl_run>(), nruns,
+                           run_max, nchunks_, cap_, t_hl_, stage_, mb_ ? mb_->bits : nullptr, mb_ ? mb_->m_base : 0,
+                           mb_ ? mb_->m_span : 0, E.d_hits.as<uint32_t>(), E.d_hit_n.as<uint32_t>(), &out[0].err};
         check(gbh_event_record(c, 0, ev[EV_HL0].e), "event");
         check(gbh_launch_harvest(c, 0, &h), "harvest");
         check(gbh_event_record(c, 0, ev[EV_HL1].e), "event");
     }
     check(gbh_event_record(c, 0, ev[EV_SETUP].e), "event");
-    gbh_window_args w{b, E.d_win.as<uint32_t>(), nwin_, std::min(ncoop_, nwin_), E.d_i0.as<uint32_t>(),
+    uint32_t ncoop = 0;
+    while (ncoop < nwin_ && E.base[E.win_lo + ncoop] < wbits / 128) ++ncoop;
+    gbh_window_args w{b, E.d_win.as<uint32_t>(), nwin_, ncoop, E.d_i0.as<uint32_t>(),
                       nlarge_ ? E.d_hits.as<uint32_t>() : nullptr, E.d_hit_n.as<uint32_t>(), cap_, 0};
     check(gbh_launch_window(c, 0, &w), "window sieve");
     check(gbh_event_record(c, 0, ev[EV_WINDOW].e), "event");
@@ -678,8 +916,7 @@ void DenseRun::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         gbh_check_args ca{b, E.ka, pcap_, E.d_coff.as<uint32_t>(), E.d_coffa.as<uint32_t>(),
                           E.d_pending.as<uint32_t>(), out, kinds, 0};
         check(gbh_launch_check_a(c, 0, &ca), "check phase A");
-        gbh_checkb_args cb{b, E.ka, CHECK_TAIL, E.d_coff.as<uint32_t>(), E.d_pending.as<uint32_t>(), pcap_,
-                           ridges_ ? ridges_->thresh() : 0xFFFFFFFFu, out};
+        gbh_checkb_args cb{b, E.ka, CHECK_TAIL, E.d_coff.as<uint32_t>(), E.d_pending.as<uint32_t>(), pcap_, 0, out};
         check(gbh_launch_check_b(c, 0, &cb), "check phase B");
     }
     check(gbh_event_record(c, 0, ev[EV_CHECK].e), "event");
@@ -696,19 +933,22 @@ void DenseRun::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
 // enqueue_bucket_verify, src/verifier.cpp:2059-2085): window primes as in
 // production, every large prime by the wheel walk (no bitmap), then a word
 // compare with the production map into out[s].vdiff.
-void DenseRun::enqueue_verify(uint32_t s, const gbh_batch& b, gbh_segout* out)
+void Pipeline::enqueue_verify(uint32_t s, const gbh_batch& b, gbh_segout* out)
 {
     gbh_ctx* c = E.c;
-    const gbh_batch vb{E.d_geoms.as<gbh_geom>() + s, 1, cw_, wbits_, b.nwin, E.d_vmap.as<uint64_t>()};
+    const gbh_batch vb{E.d_geoms.as<gbh_geom>() + s, 1, cw_, b.wbits, b.nwin, E.d_vmap.as<uint64_t>()};
+    uint32_t ncoop = 0;
+    while (ncoop < nwin_ && E.base[E.win_lo + ncoop] < b.wbits / 128) ++ncoop;
     if (nwin_) {
         const gbh_setup_args a{vb, E.d_win.as<uint32_t>(), nwin_, 0, E.d_i0.as<uint32_t>()};
         check(gbh_launch_setup(c, 0, &a), "verify setup");
     }
     check(gbh_zero(c, 0, E.d_vhit_n.p, size_t(8) * b.nwin * 4), "verify zero");
-    const gbh_harvest_args h{vb, E.d_large.as<uint32_t>(), E.d_vchunks.as<uint32_t>(), nvchunks_, cap_v_, UINT32_MAX, 0,
-                             nullptr, 0, 0, E.d_vhits.as<uint32_t>(), E.d_vhit_n.as<uint32_t>(), &out[s].err};
+    const gbh_harvest_args h{vb, E.d_large.as<uint32_t>(), E.d_vchunks.as<uint32_t>(), nullptr, 0, 0, nvchunks_, cap_v_,
+                             UINT32_MAX, 0, nullptr, 0, 0, E.d_vhits.as<uint32_t>(), E.d_vhit_n.as<uint32_t>(),
+                             &out[s].err};
     check(gbh_launch_harvest(c, 0, &h), "verify harvest");
-    const gbh_window_args w{vb, E.d_win.as<uint32_t>(), nwin_, std::min(ncoop_, nwin_), E.d_i0.as<uint32_t>(),
+    const gbh_window_args w{vb, E.d_win.as<uint32_t>(), nwin_, ncoop, E.d_i0.as<uint32_t>(),
                             E.d_vhits.as<uint32_t>(), E.d_vhit_n.as<uint32_t>(), cap_v_, 0};
     check(gbh_launch_window(c, 0, &w), "verify window");
     const gbh_mapcmp_args m{E.d_map.as<uint64_t>() + size_t(s) * 8 * cw_, E.d_vmap.as<uint64_t>(),
@@ -717,7 +957,7 @@ void DenseRun::enqueue_verify(uint32_t s, const gbh_batch& b, gbh_segout* out)
     ++verified_;
 }
 
-void DenseRun::collect(std::vector<SegJob>& batch, int slot)
+void Pipeline::collect(std::vector<SegJob>& batch, int slot)
 {
     gbh_ctx* c = E.c;
     auto& ev = E.ev[slot];
@@ -725,19 +965,22 @@ void DenseRun::collect(std::vector<SegJob>& batch, int slot)
     const double tl = elapsed_s(c, ev[EV_BEGIN], ev[EV_SETUP]);
     const double ts = elapsed_s(c, ev[EV_SETUP], ev[EV_WINDOW]);
     if (harvested_[slot]) {
-        rs_.harvest_kernel_seconds += elapsed_s(c, ev[EV_HL0], ev[EV_HL1]);
-        ++rs_.harvest_launches;
+        agg_.harvest_kernel_seconds += elapsed_s(c, ev[EV_HL0], ev[EV_HL1]);
+        ++agg_.harvest_launches;
     }
-    rs_.window_kernel_seconds += ts;
-    ++rs_.window_launches;
+    agg_.window_kernel_seconds += ts;
+    ++agg_.window_launches;
     const double tc = elapsed_s(c, ev[EV_WINDOW], ev[EV_CHECK]);
     const double tn = elapsed_s(c, ev[EV_CHECK], ev[EV_COUNT]);
-    t_sieve_ += ts; t_large_ += tl; t_check_ += tc; t_count_ += tn;
+    agg_.sieve_seconds += ts;
+    agg_.large_seconds += tl;
+    agg_.verify_seconds += tc + tn;
+    agg_.count_seconds += tn;
     const gbh_segout* outs = E.h_out[slot].as<gbh_segout>();
     if (outs[0].err & 1u) throw std::runtime_error("hit-list bucket overflow (raise the capacity)");
     if (outs[0].err & 4u) throw std::runtime_error("hit lists: a bitmap prime's cube is not above the batch's q range");
     if (outs[0].err & 8u) throw std::runtime_error("hit lists: a multiplier outside the multiplier bitmap");
-    double share[4] = {ts / batch.size(), tl / batch.size(), tc / batch.size(), tn / batch.size()};
+    const double share[4] = {ts / batch.size(), tl / batch.size(), tc / batch.size(), tn / batch.size()};
     for (size_t s = 0; s < batch.size(); ++s) collect_segment(batch[s], outs[s], share);
     if (phase_trace_)
         std::fprintf(stderr, "batch of %zu segments from %llu: harvest+setup %.3f ms window %.3f ms check %.3f ms "
@@ -745,8 +988,13 @@ void DenseRun::collect(std::vector<SegJob>& batch, int slot)
                      tc * 1e3, tn * 1e3);
 }
 
-void DenseRun::collect_segment(const SegJob& j, const gbh_segout& o, double share[4])
+void Pipeline::collect_segment(const SegJob& j, const gbh_segout& o, const double share[4])
 {
+    Range& r = *j.r;
+    r.t_sieve += share[0];
+    r.t_large += share[1];
+    r.t_check += share[2];
+    r.t_count += share[3];
     const std::string seg = " in segment [" + std::to_string(j.sg.seg_start) + ", " + std::to_string(j.sg.seg_end) + "]";
     if (o.err & 1u) throw std::runtime_error("hit-list bucket overflow (raise the capacity)" + seg);
     if (o.err & ~13u) throw std::runtime_error("device error flags " + std::to_string(o.err) + seg);
@@ -777,20 +1025,20 @@ void DenseRun::collect_segment(const SegJob& j, const gbh_segout& o, double shar
         if (p > MAX_GB_PRIME)
             std::fprintf(stderr, "note: n=%llu needs p=%llu > MAX_GB_PRIME (new p_min record candidate)\n",
                          (unsigned long long)n, (unsigned long long)p);
-        if (p > 2 && gs_) gs_->add_cpu(n, p, j.sampled);
-        if (p > 2 && ridges_) ridges_->consider(n, p, j.sg.seg_start);
+        if (p > 2 && r.gs) r.gs->add_cpu(n, p, j.sampled);
+        if (p > 2 && r.ridges) r.ridges->consider(n, p, j.sg.seg_start);
     }
-    rs_.leftovers += o.left_n;
-    if (ridges_) ridges_->merge(o.ridge, o.ridge_n, GBH_RIDGE_CAP, j.sg.seg_start);
-    if (gs_) {
+    r.rs.leftovers += o.left_n;
+    if (r.ridges) r.ridges->merge(o.ridge, o.ridge_n, GBH_RIDGE_CAP, j.sg.seg_start);
+    if (r.gs) {
         const size_t np = E.cand.primes.size();
         std::vector<uint64_t> h(15 * np, 0);
         for (uint32_t cr = 0; cr < 15; ++cr)
             for (size_t k = 0; k < E.cand.slot_pi[cr].size(); ++k)
                 h[cr * np + E.cand.slot_pi[cr][k]] += o.hist[cr * GBH_NCAND + k];
-        gs_->add_hist(j.sampled, h);
-        gs_->add_first(j.sg.seg_start, o.first);
-        if (gaps_on_) {
+        r.gs->add_hist(j.sampled, h);
+        r.gs->add_first(j.sg.seg_start, o.first);
+        if (r.gaps_on) {
             if (o.gtail_n > GBH_GAP_TCAP) throw std::runtime_error("count: gap tail list overflow" + seg);
             SegmentGaps g;
             g.q_first = j.sg.q_first;
@@ -803,125 +1051,95 @@ void DenseRun::collect_segment(const SegJob& j, const gbh_segout& o, double shar
             g.walked = j.walked;
             g.gmax = o.gmax;
             g.hist = o.ghist;
-            gs_->add_gap_segment(g);
+            r.gs->add_gap_segment(g);
         }
-        gs_->add_counts(o.cnt, o.twins);
-        if (j.sampled) sampled_total_ += j.sg.seg_evens;
-        evens_done_ += j.sg.seg_evens;
-        gs_->write_live(start_, limit_, evens_done_, total_evens_, since(t_start_), false);
+        r.gs->add_counts(o.cnt, o.twins);
+        if (j.sampled) r.sampled_total += j.sg.seg_evens;
+        r.evens_done += j.sg.seg_evens;
+        r.gs->write_live(r.start, r.limit, r.evens_done, r.total_evens, since(r.t_submit), false);
     }
-    if (verbose_)
+    if (r.verbose)
         std::printf("  seg %llu: [%llu, %llu] evens=%llu n_left=%u\n", (unsigned long long)j.index,
                     (unsigned long long)j.sg.seg_start, (unsigned long long)j.sg.seg_end,
                     (unsigned long long)j.sg.seg_evens, o.left_n);
-    if (collect_) {
+    if (r.collect) {
         SegmentStats st;
         st.seg_start = j.sg.seg_start;
         st.seg_end = j.sg.seg_end;
         st.evens = j.sg.seg_evens;
         st.tiles = uint64_t((j.cg.n[0] + wbits_ - 1) / wbits_) * 8;
-        st.large_items = nlarge_;
-        st.sieve_split = split_;
+        st.large_items = r.nlarge;
+        st.sieve_split = r.split;
         st.mask_count = E.win_lo;
         st.leftovers = o.left_n;
         st.sieve_seconds = share[0];
         st.large_seconds = share[1];
         st.check_seconds = share[2] + share[3];
-        rs_.per_segment.push_back(st);
+        r.rs.per_segment.push_back(st);
     }
+    if (j.sg.seg_end == r.limit) r.last_collected = true;
 }
 
-RunStats DenseRun::run()
+// A range whose last segment is collected: timings, ridges, the statistics
+// commit (campaign files) or the in-memory state (batched and served ranges).
+void Pipeline::finish_range(Range& r)
 {
-    t_start_ = clock_::now();
-    phase_trace_ = env::present("GB_PHASE");
-    resolve_range();
-    const double t_res = since(t_start_);
-    configure_stats();
-    configure_ridges();
-    const double t_cfg = since(t_start_);
-    plan_buffers();
-    const auto t_gpu = clock_::now();
-    rs_.host_plan_seconds = since(t_start_);
-    if (phase_trace_)
-        std::fprintf(stderr, "host: resolve %.3f ms, statistics set-up %.3f ms, plan %.3f ms\n", t_res * 1e3,
-                     (t_cfg - t_res) * 1e3, (rs_.host_plan_seconds - t_cfg) * 1e3);
-    std::vector<SegJob> batches[2];
-    int cur = 0;
-    bool pending_batch = false;
-    uint64_t k = 0;
-    bool done = false;
-    uint64_t seg_start = start_;
-    while (!done) {
-        std::vector<SegJob>& bt = batches[cur];
-        bt.clear();
-        while (bt.size() < kb_ && !done) {
-            SegJob j;
-            j.sg = segment_geometry(seg_start, limit_, span_);
-            j.index = k;
-            j.sampled = stats_on_ && k % sample_ == 0;
-            j.walked = gaps_on_ && (j.sampled || gaps_exact_);
-            j.counted = stats_on_;
-            bt.push_back(j);
-            ++k;
-            rs_.total_large_items += nlarge_;
-            if (j.sg.seg_end == limit_) done = true;
-            else seg_start += span_;
-        }
-        enqueue(bt, cur, true);
-        for (const SegJob& j : bt) rs_.total_tiles += uint64_t((j.cg.n[0] + wbits_ - 1) / wbits_) * 8;
-        if (pending_batch) collect(batches[cur ^ 1], cur ^ 1);
-        pending_batch = true;
-        cur ^= 1;
+    RunStats& rs = r.rs;
+    rs.segments = r.next_index;
+    rs.gpu_seconds = since(r.t_first);
+    rs.init_seconds = std::chrono::duration<double>(r.t_first - r.t_submit).count();
+    rs.total_seconds = rs.init_seconds + rs.gpu_seconds;
+    rs.evens_per_second = double(r.total_evens) / rs.gpu_seconds;
+    rs.sieve_seconds = r.t_sieve;
+    rs.large_seconds = r.t_large;
+    rs.verify_seconds = r.t_check + r.t_count;
+    rs.count_seconds = r.t_count;
+    rs.sync_seconds = std::max(0.0, rs.gpu_seconds - r.t_sieve - r.t_large - r.t_check - r.t_count);
+    rs.start = r.start;
+    if (!multi_) {   // a plain run owns every launch
+        rs.harvest_kernel_seconds = agg_.harvest_kernel_seconds;
+        rs.window_kernel_seconds = agg_.window_kernel_seconds;
+        rs.harvest_launches = agg_.harvest_launches;
+        rs.window_launches = agg_.window_launches;
+        rs.harvest_bytes = agg_.harvest_bytes;
+        rs.window_bytes = agg_.window_bytes;
+        if (verify_every_ && nlarge_)
+            std::fprintf(stderr, "bucket verify: %llu segment maps rebuilt without the multiplier bitmap, identical\n",
+                         (unsigned long long)verified_);
     }
-    if (pending_batch) collect(batches[cur ^ 1], cur ^ 1);
-    rs_.segments = k;
-    rs_.gpu_seconds = since(t_gpu);
-    rs_.init_seconds = since(t_start_) - rs_.gpu_seconds;
-    rs_.total_seconds = rs_.init_seconds + rs_.gpu_seconds;
-    rs_.evens_per_second = double(total_evens_) / rs_.gpu_seconds;
-    rs_.sieve_seconds = t_sieve_;
-    rs_.large_seconds = t_large_;
-    rs_.verify_seconds = t_check_ + t_count_;
-    rs_.count_seconds = t_count_;
-    rs_.sync_seconds = std::max(0.0, rs_.gpu_seconds - t_sieve_ - t_large_ - t_check_ - t_count_);
-    rs_.start = start_;
-    if (verify_every_ && nlarge_)
-        std::fprintf(stderr, "bucket verify: %llu segment maps rebuilt without the multiplier bitmap, identical\n",
-                     (unsigned long long)verified_);
-    if (ridges_ && ridges_->active()) {
-        ridges_->save(ridge_path_);
-        rs_.ridge_min = ridges_->floor();
-        rs_.ridge_found = ridges_->appends();
-        rs_.ridge_dropped = ridges_->dropped();
-        rs_.ridge_top = ridges_->top();
-        if (ridges_->overflowed())
+    if (r.ridges && r.ridges->active()) {
+        if (r.campaign) r.ridges->save(r.ridge_path);
+        rs.ridge_min = r.ridges->floor();
+        rs.ridge_found = r.ridges->appends();
+        rs.ridge_dropped = r.ridges->dropped();
+        rs.ridge_top = r.ridges->top();
+        if (r.ridges->overflowed())
             std::fprintf(stderr, "warning: a segment recorded more than %u ridges at or above the threshold; the "
                          "excess was not considered\n", GBH_RIDGE_CAP);
     }
-    if (gs_) {
-        gs_->finish_run(start_, limit_, total_evens_);
-        rs_.stats_on = true;
-        rs_.stats_pmax = gs_->run_pmax();
-        rs_.stats_primes = gs_->run_primes();
-        rs_.stats_twins = gs_->run_twins();
-        rs_.stats_sampled = gs_->evens_sampled();
-        rs_.stats_tail = gs_->run_tail();
-        gs_->write_live(start_, limit_, total_evens_, total_evens_, since(t_start_), true);
-        rs_.stats_line = gs_->log_line(VERSION);
+    if (r.gs) {
+        r.gs->finish_run(r.start, r.limit, r.total_evens);
+        rs.stats_on = true;
+        rs.stats_pmax = r.gs->run_pmax();
+        rs.stats_primes = r.gs->run_primes();
+        rs.stats_twins = r.gs->run_twins();
+        rs.stats_sampled = r.gs->evens_sampled();
+        rs.stats_tail = r.gs->run_tail();
+        r.gs->write_live(r.start, r.limit, r.total_evens, r.total_evens, since(r.t_submit), true);
+        rs.stats_line = r.gs->log_line(VERSION);
         const auto tc = clock_::now();
-        gs_->commit(VERSION, sampled_total_);
+        if (r.campaign) r.gs->commit(VERSION, r.sampled_total);
+        else r.state_text = r.gs->memory_commit(r.sampled_total);
         if (phase_trace_) std::fprintf(stderr, "host: statistics commit %.3f ms\n", since(tc) * 1e3);
     }
-    return rs_;
+    r.finished = true;
 }
 
-uint32_t DenseRun::map_of(uint64_t seg_start, std::vector<uint64_t>& words, uint64_t Q[8], uint32_t nc[8])
+uint32_t Pipeline::map_of(Range& r, uint64_t seg_start, std::vector<uint64_t>& words, uint64_t Q[8], uint32_t nc[8])
 {
-    resolve_range();
-    plan_buffers();
     std::vector<SegJob> bt(1);
-    bt[0].sg = segment_geometry(seg_start, limit_, span_);
+    bt[0].r = &r;
+    bt[0].sg = segment_geometry(seg_start, r.limit, r.span);
     enqueue(bt, 0, false);
     check(gbh_event_sync(E.c, E.ev[0][EV_END].e), "sync");
     if (E.h_out[0].as<gbh_segout>()[0].err & 13u) throw std::runtime_error("hit-list device error");
@@ -932,7 +1150,132 @@ uint32_t DenseRun::map_of(uint64_t seg_start, std::vector<uint64_t>& words, uint
     return cw_;
 }
 
+// One plain run over [start, limit] (the reference's run_gpu body).
+RunStats run_plain(Engine::State& E, uint64_t limit, uint64_t segment_even, bool verbose, uint64_t start_even,
+                   uint32_t split_arg, bool collect)
+{
+    const auto t0 = clock_::now();
+    const bool trace = env::present("GB_PHASE");
+    Range r;
+    r.limit = limit;
+    r.span = segment_even;
+    r.start = start_even;
+    r.verbose = verbose;
+    r.collect = collect;
+    r.t_submit = t0;
+    resolve_range(E, r, split_arg);
+    Pipeline P(E, false);
+    const double t_res = since(t0);
+    configure_stats(E, r);
+    configure_ridges(E, r);
+    const double t_cfg = since(t0);
+    PlanBounds pb = bounds_of(r);
+    P.plan(pb);
+    r.density = P.density_of(r.q_lo, r.nlarge);
+    r.rs.host_plan_seconds = since(t0);
+    if (trace)
+        std::fprintf(stderr, "host: resolve %.3f ms, statistics set-up %.3f ms, plan %.3f ms\n", t_res * 1e3,
+                     (t_cfg - t_res) * 1e3, (r.rs.host_plan_seconds - t_cfg) * 1e3);
+    P.add(&r);
+    while (P.step(nullptr)) {}
+    if (!r.finished) throw std::runtime_error("run ended before its last segment was collected");
+    return std::move(r.rs);
+}
+
 }  // namespace
+
+// ---------------------------------------------------------------- batches of ranges
+
+struct Batcher::Impl {
+    Engine::State& E;
+    Pipeline P;
+    uint64_t max_limit, span, min_start;
+    std::vector<std::unique_ptr<Range>> ranges;
+    std::vector<Result> results;
+    Impl(Engine::State& e, uint64_t max_limit_, uint64_t min_start_, uint64_t segment_even)
+        : E(e), P(e, true), max_limit(max_limit_ & ~uint64_t(1)), span(segment_even + (segment_even & 1))
+    {
+        if (max_limit > E.max_limit) throw std::runtime_error("batcher limit above the engine's prime bound");
+        if (span < 4 || span > E.max_span) throw std::runtime_error("batcher segment span outside the engine's buffers");
+        min_start = std::max<uint64_t>(min_start_ + (min_start_ & 1), 4);
+        if (min_start > max_limit) throw std::runtime_error("batcher: no even in [min_start, max_limit]");
+        // the bounds every range must fit: primes of max_limit, the bitmap
+        // from the smallest start up, lists sized for the densest magnitude
+        Range top;
+        top.limit = max_limit;
+        top.start = min_start;
+        top.span = span;
+        resolve_range(E, top, 0);
+        PlanBounds pb = bounds_of(top);
+        pb.span = span;
+        pb.nseg = GBH_MAX_BATCH;
+        pb.q_lo_np.clear();
+        const double l0 = std::log(double(std::max<uint64_t>(min_start, 16))), l1 = std::log(double(max_limit));
+        for (int k = 0; k <= 24; ++k) {
+            const double x = std::exp(l0 + (l1 - l0) * k / 24);
+            const uint64_t q = uint64_t(std::min(x, double(max_limit)));
+            const uint32_t bound = std::max<uint32_t>(base_prime_bound(std::max<uint64_t>(q + 2 * span, 64)), 64);
+            const uint32_t nb = uint32_t(std::upper_bound(E.base.begin(), E.base.end(), bound) - E.base.begin());
+            pb.q_lo_np.push_back({q > MAX_GB_PRIME ? q - MAX_GB_PRIME : 3, nb > E.win_hi ? nb - E.win_hi : 0});
+        }
+        P.plan(pb);
+    }
+};
+
+Batcher::Batcher(Engine& e, uint64_t max_limit, uint64_t min_start, uint64_t segment_even)
+    : p_(std::make_unique<Impl>(*e.state(), max_limit, min_start, segment_even))
+{
+}
+Batcher::~Batcher() = default;
+
+size_t Batcher::submit(uint64_t start_even, uint64_t limit)
+{
+    Impl& I = *p_;
+    auto r = std::make_unique<Range>();
+    r->id = I.ranges.size();
+    r->limit = limit;
+    r->start = start_even;
+    r->span = I.span;
+    r->campaign = false;
+    r->t_submit = clock_::now();
+    if ((limit & ~uint64_t(1)) > I.max_limit) throw std::runtime_error("range limit above the batcher's bound");
+    if (start_even < I.min_start) throw std::runtime_error("range start below the batcher's bound");
+    resolve_range(I.E, *r, 0);
+    configure_stats(I.E, *r);
+    configure_ridges(I.E, *r);
+    r->density = I.P.density_of(r->q_lo, r->nlarge);
+    I.P.add(r.get());
+    I.ranges.push_back(std::move(r));
+    I.results.emplace_back();
+    return I.ranges.size() - 1;
+}
+
+bool Batcher::step(std::vector<size_t>* finished)
+{
+    Impl& I = *p_;
+    std::vector<Range*> done;
+    const bool more = I.P.step(&done);
+    for (Range* r : done) {
+        Result& res = I.results[r->id];
+        res.stats = std::move(r->rs);
+        res.state = std::move(r->state_text);
+        res.done = true;
+        r->gs.reset();
+        r->ridges.reset();
+        if (finished) finished->push_back(r->id);
+    }
+    return more;
+}
+
+const Batcher::Result& Batcher::result(size_t id) const
+{
+    if (id >= p_->results.size()) throw std::runtime_error("no such range");
+    return p_->results[id];
+}
+
+size_t Batcher::size() const { return p_->ranges.size(); }
+uint32_t Batcher::batch_segments() const { return p_->P.kb(); }
+const RunStats& Batcher::totals() const { return p_->P.totals(); }
 
 Engine::Engine(uint64_t max_limit, uint64_t max_span, const char* prime_cache)
     : s_(std::make_unique<State>(max_limit, max_span, prime_cache))
@@ -942,8 +1285,15 @@ Engine::~Engine() = default;
 double Engine::init_seconds() const { return s_->init_s; }
 double Engine::prepare(uint64_t limit, uint64_t start_even)
 {
-    DenseRun r(*s_, limit, s_->max_span, false, start_even, 0, false);
-    return r.prepare();
+    const auto t = clock_::now();
+    Range r;
+    r.limit = limit;
+    r.start = start_even;
+    r.span = s_->max_span;
+    resolve_range(*s_, r, 0);
+    Pipeline P(*s_, false);
+    P.plan_bitmap(bounds_of(r));
+    return since(t);
 }
 uint64_t Engine::max_limit() const { return s_->max_limit; }
 void Engine::init_times(double& cache, double& module, double& upload) const
@@ -956,8 +1306,7 @@ void Engine::init_times(double& cache, double& module, double& upload) const
 RunStats Engine::run(uint64_t limit, uint64_t segment_even, bool verbose, uint64_t start_even, uint32_t sieve_split_arg,
                      bool collect_segments)
 {
-    DenseRun r(*s_, limit, segment_even, verbose, start_even, sieve_split_arg, collect_segments);
-    return r.run();
+    return run_plain(*s_, limit, segment_even, verbose, start_even, sieve_split_arg, collect_segments);
 }
 
 namespace {
@@ -1025,8 +1374,15 @@ uint32_t gpu_segment_map(uint64_t seg_start, uint64_t limit, uint64_t span, std:
 {
     if (seg_start & 1 || limit & 1 || seg_start < 4 || seg_start > limit) throw std::runtime_error("bad segment");
     Engine e(limit, std::max<uint64_t>(span, 4));
-    DenseRun r(*e.state(), limit, span, false, seg_start, 0, false);
-    return r.map_of(seg_start, words, Q, nc);
+    Range r;
+    r.limit = limit;
+    r.span = span;
+    r.start = seg_start;
+    resolve_range(*e.state(), r, 0);
+    Pipeline P(*e.state(), false);
+    P.plan(bounds_of(r));
+    r.density = P.density_of(r.q_lo, r.nlarge);
+    return P.map_of(r, seg_start, words, Q, nc);
 }
 
 }  // namespace gb

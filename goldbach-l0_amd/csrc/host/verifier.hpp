@@ -94,6 +94,50 @@ private:
     std::unique_ptr<State> s_;
 };
 
+/// Many ranges through one engine's pipeline (batched and served
+/// verification; no counterpart in the reference, which verifies one range
+/// per process). Each batch packs the waiting segments of several ranges
+/// (each range split into segments exactly as a plain run splits it), one
+/// launch per stage for the whole batch; the large-prime harvest walks each
+/// range's q range separately. Every range's statistics, ridges and
+/// fail-closed checks are those of a plain run of that range alone, without
+/// campaign files: the result holds the GBSTATS line and the state text a
+/// fresh campaign would hold after the run. One thread per Batcher.
+class Batcher {
+public:
+    struct Result {
+        RunStats stats;          // stats_line: the GBSTATS line
+        std::string state;       // the state file text of a fresh campaign after this range alone
+        bool done = false;
+    };
+    /// Plans the pipeline (large-prime split, multiplier bitmap, buffers)
+    /// for ranges with start >= @p min_start and limit <= @p max_limit.
+    /// @throws std::runtime_error when the bounds exceed the engine's.
+    Batcher(Engine& e, uint64_t max_limit, uint64_t min_start, uint64_t segment_even = 467'500'000ULL);
+    ~Batcher();
+    Batcher(const Batcher&) = delete;
+    Batcher& operator=(const Batcher&) = delete;
+    /// Queues [start_even, limit]; @return its id (0, 1, ...).
+    /// @throws std::runtime_error for a range outside the planned bounds.
+    size_t submit(uint64_t start_even, uint64_t limit);
+    /// Issues the next batch of waiting segments (if any) and collects the
+    /// batch issued before it; appends the ids of ranges completed by that
+    /// collection to @p finished. @return false when nothing waits or is in
+    /// flight. @throws std::runtime_error on any device error or failed
+    /// check (the Batcher is unusable afterwards).
+    bool step(std::vector<size_t>* finished = nullptr);
+    const Result& result(size_t id) const;
+    size_t size() const;
+    /// Segments a batch may hold.
+    uint32_t batch_segments() const;
+    /// Kernel times and algorithmic bytes over every batch so far.
+    const RunStats& totals() const;
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> p_;
+};
+
 void print_device_info();
 
 /// Independent CPU check by trial division for every even n in [4, limit].

@@ -74,6 +74,20 @@ def load() -> ctypes.CDLL:
         lib.gba_engine_prepare.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
                                            ctypes.POINTER(ctypes.c_double), ctypes.c_char_p, ctypes.c_size_t]
         lib.gba_engine_prepare.restype = ctypes.c_int
+        vp, u64, u32, cp, sz = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_size_t
+        lib.gba_batcher_open.argtypes = [vp, u64, u64, u64, ctypes.POINTER(vp), ctypes.POINTER(u32), cp, sz]
+        lib.gba_batcher_open.restype = ctypes.c_int
+        lib.gba_batcher_submit.argtypes = [vp, u64, u64, ctypes.POINTER(u64), cp, sz]
+        lib.gba_batcher_submit.restype = ctypes.c_int
+        lib.gba_batcher_step.argtypes = [vp, ctypes.POINTER(u64), u32, ctypes.POINTER(u32), ctypes.POINTER(ctypes.c_int),
+                                         cp, sz]
+        lib.gba_batcher_step.restype = ctypes.c_int
+        lib.gba_batcher_result.argtypes = [vp, u64, ctypes.POINTER(GbaRunResult), cp, sz, cp, sz, cp, sz]
+        lib.gba_batcher_result.restype = ctypes.c_int
+        lib.gba_batcher_totals.argtypes = [vp, ctypes.POINTER(GbaRunResult)]
+        lib.gba_batcher_totals.restype = ctypes.c_int
+        lib.gba_batcher_close.argtypes = [vp]
+        lib.gba_batcher_close.restype = None
         _lib = lib
     return _lib
 
@@ -140,6 +154,87 @@ class Engine:
     def close(self):
         if self._h:
             self._lib.gba_engine_close(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batcher:
+    """Many ranges through one engine's pipeline (batched and served
+    verification): submit() queues a range, step() issues the next batch of
+    waiting segments and collects the one before it, returning the ids of
+    ranges it completed; result() gives a range's figures, GBSTATS line and
+    the state text of a fresh campaign after it alone."""
+
+    def __init__(self, engine: Engine, max_limit: int, min_start: int, segment_even: int = 467_500_000):
+        self._lib = load()
+        self._engine = engine            # the engine must outlive the batcher
+        self._h = ctypes.c_void_p()
+        kb = ctypes.c_uint32(0)
+        err = ctypes.create_string_buffer(4096)
+        if self._lib.gba_batcher_open(engine._h, int(max_limit), int(min_start), int(segment_even),
+                                      ctypes.byref(self._h), ctypes.byref(kb), err, len(err)):
+            raise VerificationError(err.value.decode())
+        self.batch_segments = kb.value
+        self._ids = (ctypes.c_uint64 * 1024)()
+
+    def submit(self, start_even: int, limit: int) -> int:
+        rid = ctypes.c_uint64(0)
+        err = ctypes.create_string_buffer(4096)
+        if self._lib.gba_batcher_submit(self._h, int(start_even), int(limit), ctypes.byref(rid), err, len(err)):
+            raise VerificationError(err.value.decode())
+        return rid.value
+
+    def step(self):
+        """One pipeline step: (ids of ranges completed, whether work remains)."""
+        n = ctypes.c_uint32(0)
+        more = ctypes.c_int(0)
+        err = ctypes.create_string_buffer(4096)
+        if self._lib.gba_batcher_step(self._h, self._ids, len(self._ids), ctypes.byref(n), ctypes.byref(more), err,
+                                      len(err)):
+            raise VerificationError(err.value.decode())
+        return [self._ids[k] for k in range(n.value)], bool(more.value)
+
+    def result(self, rid: int) -> dict:
+        res = GbaRunResult()
+        line = ctypes.create_string_buffer(1 << 20)
+        state = ctypes.create_string_buffer(1 << 20)
+        err = ctypes.create_string_buffer(4096)
+        if self._lib.gba_batcher_result(self._h, int(rid), ctypes.byref(res), line, len(line), state, len(state),
+                                        err, len(err)):
+            raise VerificationError(err.value.decode())
+        out = {name: getattr(res, name) for name, _ in GbaRunResult._fields_}
+        out["stats_line"] = line.value.decode()
+        out["state"] = state.value.decode()
+        return out
+
+    def totals(self) -> dict:
+        res = GbaRunResult()
+        self._lib.gba_batcher_totals(self._h, ctypes.byref(res))
+        return {name: getattr(res, name) for name, _ in GbaRunResult._fields_}
+
+    def run_all(self) -> list:
+        """Steps until every submitted range is done; returns the completion order."""
+        order = []
+        more = True
+        while more:
+            done, more = self.step()
+            order += done
+        return order
+
+    def close(self):
+        if self._h:
+            self._lib.gba_batcher_close(self._h)
             self._h = ctypes.c_void_p()
 
     def __enter__(self):

@@ -70,6 +70,9 @@ typedef struct {
     uint32_t nc[GBH_CLASSES];      /* bits of each class array */
     uint32_t nwin;                 /* windows of this segment's class arrays */
     uint32_t hist_on;              /* record p(n) of every even (a sampled segment) */
+    uint32_t np_win;               /* window primes [0, np_win) can divide a q of the segment (the rest are
+                                      above sqrt(q_last): skipped) */
+    uint32_t ridge_t;              /* phase B records (n, p(n)) for p(n) >= ridge_t (~0u: none) */
     uint64_t restore[GBH_CLASSES]; /* bits of word 0 set again after the mask stage (q = p <= 61) */
     uint32_t s0[GBH_CLASSES][16];  /* mask primes: class bit of every multiple, residue mod p */
 } gbh_geom;
@@ -87,8 +90,10 @@ typedef struct {
     uint32_t on;                   /* 0: no counting for this segment */
 } gbh_countseg;
 
-/* A batch: nseg consecutive segments, their class maps side by side
- * (map + s * 8 * cw + c * cw), one LDS window size. */
+/* A batch: nseg segments, their class maps side by side (map + s * 8 * cw +
+ * c * cw), one LDS window size. The segments are consecutive segments of one
+ * range, or (batched ranges) runs of consecutive segments of several ranges
+ * (gbh_hl_run). */
 typedef struct {
     const gbh_geom* geoms;         /* device array [nseg] */
     uint32_t nseg;
@@ -146,10 +151,24 @@ typedef struct {
 #define GBH_HL_BUCKETS 12288
 #define GBH_HL_CHUNK 65536     /* primes per harvest work-group, at most */
 #define GBH_HL_STAGE 4096      /* staged entries per harvest work-group, at most */
+/* One run of a batch: segments [seg0, seg0 + nseg), consecutive segments of
+ * one range (their q ranges overlap or touch), served by the large primes
+ * [0, nprimes) of the list (those up to the range's sqrt bound). A batch of
+ * several ranges has one run per range; every work-group walks its chunk's
+ * primes once per run. nseg * 8 * nwin <= GBH_HL_BUCKETS and the run's q
+ * range below 2^33. */
+typedef struct {
+    uint32_t seg0, nseg;
+    uint32_t nprimes;
+    uint32_t pad0;
+} gbh_hl_run;
 typedef struct {
     gbh_batch b;
     const uint32_t* primes;        /* large primes (device), ascending */
     const uint32_t* chunks;        /* [nchunks + 1] prime index boundaries */
+    const gbh_hl_run* runs;        /* device array [nruns]; nullptr: one run of the whole batch, every prime */
+    uint32_t nruns;
+    uint32_t run_nseg_max;         /* the largest run's nseg (sizes the work-group's counters) */
     uint32_t nchunks;
     uint32_t cap;                  /* entries per window list */
     uint32_t t_hl;                 /* first prime served by the bitmap */
@@ -159,7 +178,8 @@ typedef struct {
     uint64_t m_span;               /* the bitmap covers m in [m_base, m_base + m_span) */
     uint32_t* hits;                /* [nseg][8][nwin][cap] window offsets */
     uint32_t* hit_n;               /* [nseg][8][nwin] fill counts (zeroed before the launch) */
-    uint32_t* err;                 /* bit 0: a list overflow; bit 2: t_hl^3 <= q; bit 3: m outside the bitmap */
+    uint32_t* err;                 /* bit 0: a list overflow or a run beyond the contract; bit 2: t_hl^3 <= q;
+                                      bit 3: m outside the bitmap */
 } gbh_harvest_args;
 
 /* Map self-check (reference GB_BUCKET_VERIFY compare kernel,
@@ -224,7 +244,8 @@ typedef struct {
 
 /* Stage 5: phase B (reference check30_b, goldbach.cl:2210-2306): the
  * remaining candidates of every pending even, ascending; the first hit is
- * p(n). */
+ * p(n). Evens with p(n) >= the segment's ridge_t are recorded (Goldbach
+ * ridges). */
 typedef struct {
     gbh_batch b;
     uint32_t ka;                   /* first candidate slot to probe */
@@ -232,7 +253,7 @@ typedef struct {
     const uint32_t* coff;
     const uint32_t* pending;
     uint32_t pcap;
-    uint32_t ridge_thresh;         /* record (n, p(n)) for p(n) >= this (Goldbach ridges; ~0u: none) */
+    uint32_t pad0;
     gbh_segout* out;
 } gbh_checkb_args;
 

@@ -77,6 +77,37 @@ void gba_engine_close(gba_engine* e);
 int gba_engine_prepare(gba_engine* e, uint64_t limit, uint64_t start_even, double* seconds, char* err,
                        size_t err_cap);
 
+/* Batcher: many ranges through one engine's pipeline (batched and served
+ * verification; no counterpart in the reference, which verifies one range
+ * per process). A batch packs waiting segments of several ranges (each range
+ * segmented exactly as a plain run of it), one launch per stage; every
+ * range's statistics are those of a plain run of that range alone, without
+ * campaign files (gba_batcher_result returns its GBSTATS line and the state
+ * text a fresh campaign would hold after it).
+ *   open:   plans for ranges with start >= min_start, limit <= max_limit
+ *           (large-prime split, multiplier bitmap, buffers; the engine must
+ *           cover max_limit); *batch_segments receives the segments a batch
+ *           may hold.
+ *   submit: queues [start_even, limit], *id = 0, 1, ...
+ *   step:   issues the next batch of waiting segments and collects the one
+ *           issued before; returns up to finished_cap ids of completed
+ *           ranges (the rest on later calls); *more = 0 when nothing waits,
+ *           is in flight or is left to report.
+ *   totals: kernel seconds, launches and algorithmic bytes over all batches.
+ * A failed step (device error, failed check) leaves the batcher unusable.
+ * One thread per batcher; the engine must outlive it. All return 0 on
+ * success. */
+typedef struct gba_batcher gba_batcher;
+int gba_batcher_open(gba_engine* e, uint64_t max_limit, uint64_t min_start, uint64_t segment_even, gba_batcher** out,
+                     uint32_t* batch_segments, char* err, size_t err_cap);
+int gba_batcher_submit(gba_batcher* b, uint64_t start_even, uint64_t limit, uint64_t* id, char* err, size_t err_cap);
+int gba_batcher_step(gba_batcher* b, uint64_t* finished, uint32_t finished_cap, uint32_t* n_finished, int* more,
+                     char* err, size_t err_cap);
+int gba_batcher_result(gba_batcher* b, uint64_t id, gba_run_result* out, char* stats_line, size_t stats_cap,
+                       char* state, size_t state_cap, char* err, size_t err_cap);
+int gba_batcher_totals(gba_batcher* b, gba_run_result* out);
+void gba_batcher_close(gba_batcher* b);
+
 /* The library's version string. */
 const char* gba_version(void);
 
