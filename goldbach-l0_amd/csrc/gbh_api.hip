@@ -274,7 +274,10 @@ int gbh_launch_harvest(gbh_ctx* c, int s, const gbh_harvest_args* a)
     if (!a->nchunks) return GBH_OK;
     if (a->runs && (!a->nruns || !a->run_nseg_max || a->run_nseg_max > a->b.nseg))
         return fail(GBH_E_ARG, "harvest: run list without runs or with a run longer than the batch");
-    const uint64_t nb = uint64_t(a->runs ? a->run_nseg_max : a->b.nseg) * 8 * a->b.nwin;
+    const bool direct = a->rdesc != nullptr;
+    if (direct && (!a->region || !a->fills || a->stage))
+        return fail(GBH_E_ARG, "harvest: the direct form needs regions and fill counts and no stage");
+    const uint64_t nb = uint64_t(a->runs ? a->run_nseg_max : a->b.nseg) * 8 * (direct ? 1 : a->b.nwin);
     if (nb > GBH_HL_BUCKETS) return fail(GBH_E_ARG, "harvest: more window lists than the work-group counts");
     if (a->m_base % 240) return fail(GBH_E_ARG, "harvest: bitmap base not a multiple of 240");
     if (a->stage > GBH_HL_STAGE) return fail(GBH_E_ARG, "harvest: stage above GBH_HL_STAGE");
@@ -282,7 +285,21 @@ int gbh_launch_harvest(gbh_ctx* c, int s, const gbh_harvest_args* a)
     if (lds > 144 * 1024) return fail(GBH_E_ARG, "harvest: lists and stage exceed the work-group's LDS");
     hipLaunchKernelGGL(gbh_dev::k_harvest, dim3(a->nchunks), dim3(256), lds, S(c, s), a->b, a->primes, a->chunks,
                        a->runs, a->runs ? a->nruns : 1u, uint32_t(nb), a->t_hl, a->mbits, a->m_base, a->m_span,
-                       a->cap, a->stage, a->hits, a->hit_n, a->err);
+                       a->cap, a->stage, a->hits, a->hit_n, a->rdesc, a->region, a->fills, a->err);
+    GBH_HIP(hipGetLastError());
+    return GBH_OK;
+}
+
+int gbh_launch_partition(gbh_ctx* c, int s, const gbh_partition_args* a)
+{
+    if (const int rc = check_batch(a->b)) return rc;
+    if (!a->ngroups) return GBH_OK;
+    if (!a->rdesc || !a->region || !a->fills || !a->groups || !a->hits || !a->hit_n)
+        return fail(GBH_E_ARG, "partition: null argument");
+    if (a->b.nwin > 256) return fail(GBH_E_ARG, "partition: more than 256 windows per class");
+    hipLaunchKernelGGL(gbh_dev::k_partition, dim3(a->b.nseg * 8, a->ngroups), dim3(512), 0, S(c, s), a->rdesc,
+                       a->region, a->fills, a->groups, a->b.nseg * 8, a->b.nwin, a->b.wbits, a->cap, a->hits,
+                       a->hit_n, a->err);
     GBH_HIP(hipGetLastError());
     return GBH_OK;
 }

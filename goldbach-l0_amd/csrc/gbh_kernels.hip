@@ -143,19 +143,33 @@ struct HlCtx {
     uint16_t* stb;           // their lists
     uint32_t* sn;            // entries offered to the stage
     uint32_t scap;
+    uint32_t* region;        // kDirect: the work-group's region, [bucket][rcap]
+    uint32_t rcap;
 };
+
+// Harvest passes: kCount counts per window list (and stages), kWrite writes
+// the window lists, kDirect (one pass) appends to the work-group's own
+// region of each (segment, class) list.
+enum { kCount = 0, kWrite = 1, kDirect = 2 };
 
 // Files the multiple at half-offset d of class c into the window list of
 // every segment whose q range holds it (consecutive segments overlap by 9776
-// in q); the entry is the multiple's bit inside its window.
-template <bool kWrite>
+// in q); the entry is the multiple's bit inside its window (kDirect: its bit
+// in the class array).
+template <int kMode>
 __device__ __forceinline__ void file_one(const HlCtx& h, uint32_t s, uint32_t c, uint32_t d, const uint32_t* dq,
                                          uint32_t* cnt, uint32_t* __restrict__ hits, uint32_t cap,
                                          uint32_t* __restrict__ err)
 {
     const uint32_t t = __umulhi(d - dq[s * 8 + c], 0x88888889u) >> 3;   // (q - Q) / 30 = (d - dq) / 15
+    if (kMode == kDirect) {
+        const uint32_t pos = atomicAdd(&cnt[s * 8 + c], 1u);
+        if (pos < h.rcap) h.region[(size_t)(s * 8 + c) * h.rcap + pos] = t;
+        else atomicOr(err, 1u);
+        return;
+    }
     const uint32_t bk = (s * 8 + c) * h.nwin + (t >> h.lw);
-    if (!kWrite) {
+    if (kMode == kCount) {
         atomicAdd(&cnt[bk], 1u);
         if (h.stv) {
             const uint32_t pos = atomicAdd(h.sn, 1u);
@@ -171,7 +185,7 @@ __device__ __forceinline__ void file_one(const HlCtx& h, uint32_t s, uint32_t c,
     }
 }
 
-template <bool kWrite>
+template <int kMode>
 __device__ __forceinline__ void file_hit(const HlCtx& h, const uint32_t* df, const uint32_t* dl, const uint32_t* dq,
                                          uint32_t d, uint32_t c, uint32_t* cnt, uint32_t* __restrict__ hits,
                                          uint32_t cap, uint32_t* __restrict__ err)
@@ -183,8 +197,8 @@ __device__ __forceinline__ void file_hit(const HlCtx& h, const uint32_t* df, con
     s += (s + 1 < h.nseg && d >= df[min(s + 1, h.nseg - 1)]);
     while (s + 1 < h.nseg && d >= df[s + 1]) ++s;     // irregular spacing only (a run's first segment below 9783)
     while (s > 0 && d < df[s]) --s;
-    if (d <= dl[s]) file_one<kWrite>(h, s, c, d, dq, cnt, hits, cap, err);
-    if (s > 0 && d <= dl[s - 1]) file_one<kWrite>(h, s - 1, c, d, dq, cnt, hits, cap, err);   // overlap
+    if (d <= dl[s]) file_one<kMode>(h, s, c, d, dq, cnt, hits, cap, err);
+    if (s > 0 && d <= dl[s - 1]) file_one<kMode>(h, s - 1, c, d, dq, cnt, hits, cap, err);   // overlap
 }
 
 // 1 / p in double: the hardware reciprocal and one Newton step
@@ -219,7 +233,7 @@ struct WheelLds {
     uint8_t next[210], idx[210], inc[48], cls[30];
 };
 
-template <bool kWrite>
+template <int kMode>
 __device__ __forceinline__ void walk_wheel(const HlCtx& h, double qb0d, uint32_t p, const uint32_t* df,
                                            const uint32_t* dl, const uint32_t* dq, const WheelLds& wl, uint32_t* cnt,
                                            uint32_t* __restrict__ hits, uint32_t cap, uint32_t* __restrict__ err)
@@ -239,7 +253,7 @@ __device__ __forceinline__ void walk_wheel(const HlCtx& h, double qb0d, uint32_t
     uint32_t idx = wl.idx[(r210 + d0) % 210];
     const uint32_t pr = p % 30;
     for (;;) {
-        if (r11 && r13) file_hit<kWrite>(h, df, dl, dq, d, wl.cls[(r30 * pr) % 30], cnt, hits, cap, err);
+        if (r11 && r13) file_hit<kMode>(h, df, dl, dq, d, wl.cls[(r30 * pr) % 30], cnt, hits, cap, err);
         const uint32_t inc = wl.inc[idx];       // even: the walk stays on odd m
         idx = idx == 47 ? 0 : idx + 1;
         const uint64_t step = (uint64_t)(inc >> 1) * p;
@@ -341,7 +355,7 @@ __device__ __forceinline__ BmWalk bm_setup(const HlCtx& h, double qb0d, uint32_t
 
 // v: word w0 of the bitmap (already loaded). Returns whether any bit of the
 // range was set.
-template <bool kWrite>
+template <int kMode>
 __device__ __forceinline__ bool bm_run(const HlCtx& h, const BmWalk& k, uint64_t v, const uint32_t* df,
                                        const uint32_t* dl, const uint32_t* dq, uint32_t* cnt,
                                        uint32_t* __restrict__ hits, uint32_t cap, uint32_t* __restrict__ err)
@@ -360,12 +374,12 @@ __device__ __forceinline__ bool bm_run(const HlCtx& h, const BmWalk& k, uint64_t
                 const uint32_t i = bit & 7;
                 const uint32_t delta = 30 * (bit >> 3) + res_of(i);
                 const uint32_t d = (uint32_t)((qw + (uint64_t)k.p * delta) >> 1);
-                file_hit<kWrite>(h, df, dl, dq, d, (k.cmap >> (3 * i)) & 7, cnt, hits, cap, err);
+                file_hit<kMode>(h, df, dl, dq, d, (k.cmap >> (3 * i)) & 7, cnt, hits, cap, err);
             } while (v);
         }
         if (w >= k.w1) break;
         ++w;
-        v = h.mbits[w];
+        v = __builtin_nontemporal_load(h.mbits + w);
     }
     return any;
 }
@@ -374,7 +388,7 @@ __device__ __forceinline__ bool bm_run(const HlCtx& h, const BmWalk& k, uint64_t
 // u < kG; on the writing pass only those flagged by the counting pass.
 constexpr uint32_t kG = 4;
 
-template <bool kWrite>
+template <int kMode>
 __device__ __forceinline__ void bm_group(const HlCtx& h, double qb0d, const uint32_t* __restrict__ primes,
                                          uint32_t j0, uint32_t j1, uint32_t k, uint32_t tid, const uint32_t* df,
                                          const uint32_t* dl, const uint32_t* dq, const uint32_t* cmt, uint32_t* cnt,
@@ -386,7 +400,7 @@ __device__ __forceinline__ void bm_group(const HlCtx& h, double qb0d, const uint
     for (uint32_t u = 0; u < kG; ++u) {
         const uint32_t j = j0 + (k + u) * 256 + tid;
         bool on = j < j1;
-        if (kWrite && on) on = (any_hit[(k + u) * 4 + (tid >> 6)] >> (tid & 63)) & 1;
+        if (kMode == kWrite && on) on = (any_hit[(k + u) * 4 + (tid >> 6)] >> (tid & 63)) & 1;
         p[u] = on ? primes[j] : 0u;
     }
     BmWalk wk[kG];
@@ -394,11 +408,13 @@ __device__ __forceinline__ void bm_group(const HlCtx& h, double qb0d, const uint
 #pragma unroll
     for (uint32_t u = 0; u < kG; ++u) wk[u] = bm_setup(h, qb0d, p[u], cmt, err);
 #pragma unroll
-    for (uint32_t u = 0; u < kG; ++u) v[u] = wk[u].live ? h.mbits[wk[u].w0] : 0ull;
+    // bitmap words are read once: streamed past L2, whose lines then keep
+    // the hit lists being written
+    for (uint32_t u = 0; u < kG; ++u) v[u] = wk[u].live ? __builtin_nontemporal_load(h.mbits + wk[u].w0) : 0ull;
 #pragma unroll
     for (uint32_t u = 0; u < kG; ++u) {
-        const bool any = bm_run<kWrite>(h, wk[u], v[u], df, dl, dq, cnt, hits, cap, err);
-        if (!kWrite) {
+        const bool any = bm_run<kMode>(h, wk[u], v[u], df, dl, dq, cnt, hits, cap, err);
+        if (kMode == kCount) {
             const uint64_t bal = __ballot(any);
             if ((tid & 63) == 0) any_hit[(k + u) * 4 + (tid >> 6)] = bal;
         }
@@ -420,7 +436,8 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
                                                  uint32_t t_hl, const uint64_t* __restrict__ mbits, uint64_t m_base,
                                                  uint64_t m_span, uint32_t cap, uint32_t stage,
                                                  uint32_t* __restrict__ hits, uint32_t* __restrict__ hit_n,
-                                                 uint32_t* __restrict__ err)
+                                                 const uint32_t* __restrict__ rdesc, uint32_t* __restrict__ region,
+                                                 uint32_t* __restrict__ fills, uint32_t* __restrict__ err)
 {
     __shared__ uint32_t df[GBH_MAX_BATCH], dl[GBH_MAX_BATCH], dq[GBH_MAX_BATCH * 8], cmt[8], sn;
     __shared__ uint64_t any_hit[GBH_HL_CHUNK / 64 + 4 * kG];   // bitmap primes with a set bit in range (pass 2 walks those)
@@ -450,7 +467,9 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
         const uint32_t seg0 = runs ? runs[r].seg0 : 0u, rn = runs ? runs[r].nseg : b.nseg;
         const uint32_t je = runs ? min(j1, runs[r].nprimes) : j1;
         if (j0 >= je) continue;
-        if (rn == 0 || seg0 + rn > b.nseg || rn * 8 * b.nwin > nbmax) {   // a run beyond the contract: fail closed
+        const bool direct = rdesc != nullptr;
+        const uint32_t nb = direct ? rn * 8 : rn * 8 * b.nwin;   // the run's counters
+        if (rn == 0 || seg0 + rn > b.nseg || nb > nbmax) {       // a run beyond the contract: fail closed
             if (tid == 0) atomicOr(err, 1u);
             continue;
         }
@@ -462,7 +481,6 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
             dl[s] = (uint32_t)((gs[s].q_last - qb0) >> 1);
         }
         for (uint32_t i = tid; i < rn * 8; i += 256) dq[i] = (uint32_t)((gs[i / 8].Q[i % 8] - qb0) >> 1);
-        const uint32_t nb = rn * 8 * b.nwin;
         for (uint32_t i = tid; i < nb; i += 256) {
             cnt[i] = 0;
             if (stage) runc[i] = 0;
@@ -474,6 +492,8 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
         h.stb = stb;
         h.sn = &sn;
         h.scap = stage;
+        h.region = direct ? region + rdesc[2 * blockIdx.x] + (size_t)seg0 * 8 * rdesc[2 * blockIdx.x + 1] : nullptr;
+        h.rcap = direct ? rdesc[2 * blockIdx.x + 1] : 0u;
         h.qb0 = qb0;
         h.dmax = dl[rn - 1];
         h.hs_magic = (rn > 1 && df[1] > df[0]) ? (uint32_t)((1ull << 32) / (df[1] - df[0] + 1)) : 0u;
@@ -496,11 +516,23 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
         }
         uint32_t* rh = hits + (size_t)seg0 * 8 * b.nwin * cap;   // the run's lists
         uint32_t* rhn = hit_n + (size_t)seg0 * 8 * b.nwin;
+        if (direct) {   // one walk into the work-group's regions, then their fill counts
+            if (wheel) {
+                for (uint32_t j = j0 + tid; j < je; j += 256) walk_wheel<kDirect>(h, qb0d, primes[j], df, dl, dq, wl, cnt, rh, cap, err);
+            } else {
+                for (uint32_t k = 0; j0 + k * 256 < je; k += kG)
+                    bm_group<kDirect>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
+            }
+            __syncthreads();
+            uint32_t* fl = fills + (size_t)blockIdx.x * (b.nseg * 8) + seg0 * 8;
+            for (uint32_t i = tid; i < nb; i += 256) fl[i] = min(cnt[i], h.rcap);
+            continue;
+        }
         if (wheel) {
-            for (uint32_t j = j0 + tid; j < je; j += 256) walk_wheel<false>(h, qb0d, primes[j], df, dl, dq, wl, cnt, rh, cap, err);
+            for (uint32_t j = j0 + tid; j < je; j += 256) walk_wheel<kCount>(h, qb0d, primes[j], df, dl, dq, wl, cnt, rh, cap, err);
         } else {
             for (uint32_t k = 0; j0 + k * 256 < je; k += kG)
-                bm_group<false>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
+                bm_group<kCount>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
         }
         __syncthreads();
         for (uint32_t i = tid; i < nb; i += 256) {
@@ -519,11 +551,98 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
             continue;
         }
         if (wheel) {
-            for (uint32_t j = j0 + tid; j < je; j += 256) walk_wheel<true>(h, qb0d, primes[j], df, dl, dq, wl, cnt, rh, cap, err);
+            for (uint32_t j = j0 + tid; j < je; j += 256) walk_wheel<kWrite>(h, qb0d, primes[j], df, dl, dq, wl, cnt, rh, cap, err);
         } else {
             for (uint32_t k = 0; j0 + k * 256 < je; k += kG)
-                bm_group<true>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
+                bm_group<kWrite>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
         }
+    }
+}
+
+// Direct harvest, second step: the (segment, class) regions every chunk's
+// work-group filled are split into the window lists. Work-group (bk, g):
+// bucket bk = s * 8 + c of the batch, the chunks [groups[g], groups[g + 1])
+// (at most GBH_PART_CHUNKS, their regions at most GBH_PART_CAP entries in
+// all: planned by the host). The entries are read once into LDS (the chunks'
+// fill counts scanned so every thread's loads are independent) and counted
+// per window; one reservation per window list; then the writes from LDS
+// (each window list's share lands in consecutive slots).
+__global__ void __launch_bounds__(512) k_partition(const uint32_t* __restrict__ rdesc,
+                                                   const uint32_t* __restrict__ region,
+                                                   const uint32_t* __restrict__ fills,
+                                                   const uint32_t* __restrict__ groups, uint32_t nbb, uint32_t nwin,
+                                                   uint32_t wbits, uint32_t cap, uint32_t* __restrict__ hits,
+                                                   uint32_t* __restrict__ hit_n, uint32_t* __restrict__ err)
+{
+    __shared__ uint32_t ent[GBH_PART_CAP];
+    __shared__ uint32_t wc[256], wbase[256], pre[GBH_PART_CHUNKS + 1], rb[GBH_PART_CHUNKS];
+    const uint32_t bk = blockIdx.x, tid = threadIdx.x;
+    const uint32_t k0 = groups[blockIdx.y], nk = min(groups[blockIdx.y + 1] - k0, (uint32_t)GBH_PART_CHUNKS);
+    const uint32_t lw = 31 - __clz(wbits);
+    for (uint32_t i = tid; i < nwin; i += 512) wc[i] = 0;
+    if (tid < 64) {   // one wave: fill counts, region bases and their exclusive scan
+        uint32_t n = 0;
+        if (tid < nk) {
+            const uint32_t k = k0 + tid;
+            n = fills[(size_t)k * nbb + bk];
+            rb[tid] = rdesc[2 * k] + bk * rdesc[2 * k + 1];
+        }
+        uint32_t tot = 0;
+        const uint32_t x = wave_excl_scan(n, &tot);
+        pre[tid] = x;
+        if (tid == 0) pre[nk] = tot;
+    }
+    __syncthreads();
+    const uint32_t total = pre[nk];
+    if (total > GBH_PART_CAP) {              // beyond the host's plan: fail closed
+        if (tid == 0) atomicOr(err, 1u);
+        return;
+    }
+    // the entries, kPartLoads per thread in flight at a time
+    constexpr uint32_t kPartLoads = 8;
+    for (uint32_t f0 = 0; f0 < total; f0 += 512 * kPartLoads) {
+        uint32_t v[kPartLoads];
+#pragma unroll
+        for (uint32_t u = 0; u < kPartLoads; ++u) {
+            const uint32_t f = f0 + u * 512 + tid;
+            v[u] = 0xFFFFFFFFu;
+            if (f < total) {
+                uint32_t lo = 0, hi = nk;    // the chunk holding f: the last k with pre[k] <= f
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (pre[mid] <= f) lo = mid;
+                    else hi = mid;
+                }
+                v[u] = __builtin_nontemporal_load(region + rb[lo] + (f - pre[lo]));   // read once
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kPartLoads; ++u) {
+            const uint32_t f = f0 + u * 512 + tid;
+            if (f >= total) continue;
+            uint32_t t = v[u];
+            if ((t >> lw) >= nwin) {
+                atomicOr(err, 1u);
+                t = 0xFFFFFFFFu;
+            } else {
+                atomicAdd(&wc[t >> lw], 1u);
+            }
+            ent[f] = t;
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < nwin; i += 512) {   // one reservation per window list
+        const uint32_t c = wc[i];
+        wbase[i] = c ? atomicAdd(&hit_n[(size_t)bk * nwin + i], c) : 0u;
+        wc[i] = 0;
+    }
+    __syncthreads();
+    for (uint32_t f = tid; f < total; f += 512) {
+        const uint32_t t = ent[f];
+        if (t == 0xFFFFFFFFu) continue;
+        const uint32_t w = t >> lw, pos = wbase[w] + atomicAdd(&wc[w], 1u);
+        if (pos < cap) hits[((size_t)bk * nwin + w) * cap + pos] = t & (wbits - 1);
+        else atomicOr(err, 1u);
     }
 }
 
@@ -684,25 +803,31 @@ __device__ __forceinline__ ClassEvens class_evens(const gbh_geom& g, uint32_t cr
 }
 
 // Phase A layout: a work-group stages words [w0, w0 + kRowW) of the eight
-// class arrays as 32-bit halves in LDS. Lane i of wave v takes class-array
-// word w0 + 63 v + i of every even class (64 evens of the class, one per
-// bit); lane 63 only lends its first dword to lane 62. For candidate j the
-// 64 bits the lane needs start kA = (entry >> 5) dwords into the staged rows
-// at bit sh = entry & 31: dwords a, b read at kA + 2 (63 v + i), the third
-// is lane i + 1's a, and two funnel shifts give the 64 bits.
+// class arrays in LDS as two planes of 32-bit halves, the low dwords of
+// every row first, then the high dwords (lane i reads dword x + i of a plane:
+// unit stride across the wave, no bank conflicts). Lane i of wave v takes
+// class-array word w0 + 63 v + i of every even class (64 evens of the class,
+// one per bit); lane 63 only lends its first dword to lane 62. For candidate
+// j the 64 bits the lane needs start at bit sh of dword a, then dwords b and
+// c follow: a and b are read at plane offsets fa + 63 v + i and fb + 63 v + i
+// (a low half then the same word's high half, or a high half then the next
+// word's low half), c is lane i + 1's a, and two funnel shifts give the 64
+// bits.
 constexpr uint32_t kWaveWords = 63;
 constexpr uint32_t kBlockWords = kWaveWords * (GBH_CHECK_WG / 64);   // 252 words of every class per work-group
 constexpr uint32_t kRowW = kBlockWords + 12;                         // staged words per class row (offsets < 12 words)
-constexpr uint32_t kRowDw = 2 * kRowW;
+constexpr uint32_t kPlane = GBH_CLASSES * kRowW;                     // dwords per plane (low / high halves)
 constexpr uint32_t kPbuf = 2048;                                     // pending evens staged per work-group
 // a candidate's offset is below (9781 + 58) / 30 bits (n0 <= n_gpu + 28 <= q_first + 9810)
-static_assert((9781 + 58) / 30 / 32 + 2 * kBlockWords + 2 <= kRowDw, "staged rows too short for the offsets");
+static_assert((9781 + 58) / 30 / 64 + kBlockWords + 2 <= kRowW, "staged rows too short for the offsets");
+static_assert(2 * kPlane < (1u << 13), "plane offsets must fit 13 bits");
 
 // Per segment and even class cr, candidate j:
 //   coff[(s * 15 + cr) * GBH_NCAND + j] = class << 16 | bit offset (phase B)
-//   coffa[(s * 15 + cr) * GBH_NCAND_PAD + j] = staged dword << 5 | shift (phase A)
+//   coffa[(s * 15 + cr) * GBH_NCAND_PAD + j] = fa | fb << 13 | shift << 26 (phase A)
 // where the bit offset is (n0 - p - Q[class]) / 30 for the class's first
-// checked even n0.
+// checked even n0, and fa / fb the staged plane offsets of the dwords a / b
+// that hold it (word off / 64 of the class row, its low or high half).
 __global__ void __launch_bounds__(256) k_coff(gbh_batch b, uint32_t* __restrict__ coff, uint32_t* __restrict__ coffa)
 {
     const uint32_t cr = blockIdx.y, s = blockIdx.z;
@@ -720,7 +845,9 @@ __global__ void __launch_bounds__(256) k_coff(gbh_batch b, uint32_t* __restrict_
     const uint32_t c = c_cand_c[cr * GBH_NCAND + j];
     const uint32_t off = (uint32_t)((e.n0 - p - g.Q[c]) / 30);   // exact and >= 0 (host contract on n_gpu)
     coff[((size_t)s * 15 + cr) * GBH_NCAND + j] = (c << 16) | off;
-    pa[j] = ((c * kRowDw + (off >> 5)) << 5) | (off & 31);
+    const uint32_t x = c * kRowW + (off >> 6);
+    const uint32_t fa = (off & 32) ? kPlane + x : x, fb = (off & 32) ? x + 1 : kPlane + x;
+    pa[j] = fa | (fb << 13) | ((off & 31) << 26);
 }
 
 // Sum over the wave, in lane 63: inclusive row sums by DPP row_shr 1 2 4 8,
@@ -747,7 +874,7 @@ __global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_batch b, const uin
                                                           uint32_t ka, uint32_t* __restrict__ pending, uint32_t pcap,
                                                           gbh_segout* __restrict__ outs)
 {
-    __shared__ uint32_t st[GBH_CLASSES * kRowDw];
+    __shared__ uint32_t st[2 * kPlane];
     __shared__ uint32_t pbuf[kPbuf];
     __shared__ uint32_t hl[kHist ? GBH_EVEN_CLASSES * GBH_KA_MAX : 1];
     __shared__ uint32_t pn, pres, pbase;
@@ -770,9 +897,9 @@ __global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_batch b, const uin
 #pragma unroll
         for (uint32_t k = 0; k < kPer; ++k) {
             const uint32_t i = tid + k * GBH_CHECK_WG;
-            if (i < kN) {
-                const uint32_t c = i / kRowW, x = i % kRowW;
-                *reinterpret_cast<uint2*>(&st[c * kRowDw + 2 * x]) = make_uint2((uint32_t)v[k], (uint32_t)(v[k] >> 32));
+            if (i < kN) {                    // i = c * kRowW + x
+                st[i] = (uint32_t)v[k];
+                st[kPlane + i] = (uint32_t)(v[k] >> 32);
             }
         }
     }
@@ -782,7 +909,7 @@ __global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_batch b, const uin
     __syncthreads();
 
     const uint32_t word = w0 + wv * kWaveWords + lane;     // this lane's word of every class array
-    const uint32_t* sb = st + 2 * (wv * kWaveWords + lane);
+    const uint32_t* sb = st + wv * kWaveWords + lane;
     uint32_t resolved = 0;
     for (uint32_t cr = 0; cr < GBH_EVEN_CLASSES; ++cr) {
         const ClassEvens e = class_evens(g, cr);
@@ -800,11 +927,10 @@ __global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_batch b, const uin
                 uint32_t hc[8];
 #pragma unroll
                 for (uint32_t u = 0; u < 16; ++u) {
-                    const uint32_t ent = ct[j0 + u];
-                    const uint32_t* p = sb + (ent >> 5);
-                    const uint32_t a = p[0], bw = p[1], cw = from_next_lane(a);
-                    const uint32_t x0 = __builtin_amdgcn_alignbit(bw, a, ent & 31);
-                    const uint32_t x1 = __builtin_amdgcn_alignbit(cw, bw, ent & 31);
+                    const uint32_t ent = ct[j0 + u], sh = ent >> 26;
+                    const uint32_t a = sb[ent & 0x1FFF], bw = sb[(ent >> 13) & 0x1FFF], cw = from_next_lane(a);
+                    const uint32_t x0 = __builtin_amdgcn_alignbit(bw, a, sh);
+                    const uint32_t x1 = __builtin_amdgcn_alignbit(cw, bw, sh);
                     const uint32_t h = (uint32_t)__popc(x0 & u0) + (uint32_t)__popc(x1 & u1);
                     hc[u / 2] = (u & 1) ? (hc[u / 2] | (h << 16)) : h;
                     u0 &= ~x0;
@@ -822,11 +948,10 @@ __global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_batch b, const uin
                 uint32_t r0 = 0, r1 = 0;
 #pragma unroll
                 for (uint32_t u = 0; u < 16; ++u) {
-                    const uint32_t ent = ct[j0 + u];
-                    const uint32_t* p = sb + (ent >> 5);
-                    const uint32_t a = p[0], bw = p[1], cw = from_next_lane(a);
-                    r0 |= __builtin_amdgcn_alignbit(bw, a, ent & 31);
-                    r1 |= __builtin_amdgcn_alignbit(cw, bw, ent & 31);
+                    const uint32_t ent = ct[j0 + u], sh = ent >> 26;
+                    const uint32_t a = sb[ent & 0x1FFF], bw = sb[(ent >> 13) & 0x1FFF], cw = from_next_lane(a);
+                    r0 |= __builtin_amdgcn_alignbit(bw, a, sh);
+                    r1 |= __builtin_amdgcn_alignbit(cw, bw, sh);
                 }
                 u0 &= ~r0;
                 u1 &= ~r1;
@@ -955,6 +1080,10 @@ __global__ void __launch_bounds__(256) k_count_a(gbh_batch b, const gbh_countseg
     const uint32_t tid = threadIdx.x, s = blockIdx.y;
     const gbh_countseg& a = css[s];
     if (!a.on || blockIdx.x * 256 >= a.nwords) return;
+    // the walk's phase offsets, read once (not again after every atomic)
+    uint32_t pos0[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) pos0[k] = a.pos0[k];
     gbh_segout* out = outs + s;
     const uint64_t* map = seg_map(b, s);
     uint32_t* fl = scratch + (size_t)s * swords;
@@ -963,21 +1092,22 @@ __global__ void __launch_bounds__(256) k_count_a(gbh_batch b, const gbh_countseg
     __syncthreads();
     const uint32_t x = blockIdx.x * 256 + tid;
     uint64_t best = 0;
+    uint32_t pcs[11];                       // primes per class, twins per pair: reduced per wave below
+#pragma unroll
+    for (int k = 0; k < 11; ++k) pcs[k] = 0;
     if (x < a.nwords) {
         uint64_t w[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             w[k] = map[(size_t)a.cls[k] * b.cw + x] & range_mask(x, a.lo[k], a.hi[k]);
-            const uint32_t pc = (uint32_t)__popcll(w[k]);
-            if (pc) atomicAdd(&cs[k], pc);
+            pcs[k] = (uint32_t)__popcll(w[k]);
         }
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             const uint64_t lw = map[(size_t)a.tk[j] * b.cw + x] & range_mask(x, a.tlo[j], a.thi[j]);
             const uint64_t* p2 = map + (size_t)a.tk2[j] * b.cw + x;
             const uint64_t pw = a.td[j] ? ((p2[0] >> 1) | (p2[1] << 63)) : p2[0];
-            const uint32_t pc = (uint32_t)__popcll(lw & pw);
-            if (pc) atomicAdd(&cs[8 + j], pc);
+            pcs[8 + j] = (uint32_t)__popcll(lw & pw);
         }
         // the walk: primes in position order (block t, then ascending phase)
         uint64_t any = 0;
@@ -992,7 +1122,7 @@ __global__ void __launch_bounds__(256) k_count_a(gbh_batch b, const gbh_countseg
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 if ((w[k] >> bt) & 1ull) {
-                    const uint32_t pos = base + a.pos0[k];
+                    const uint32_t pos = base + pos0[k];
                     if (have) note_gap(pos - prev, pos, hl, out, best);
                     else { first = pos; have = true; }
                     prev = pos;
@@ -1005,6 +1135,11 @@ __global__ void __launch_bounds__(256) k_count_a(gbh_batch b, const gbh_countseg
             atomicMin(&out->pfirst, first);
             atomicMax(&out->plast, prev);
         }
+    }
+#pragma unroll
+    for (int k = 0; k < 11; ++k) {
+        const uint32_t t = wave_total63(pcs[k]);   // the wave's total in lane 63
+        if ((tid & 63) == 63 && t) atomicAdd(&cs[k], t);
     }
     if (best) atomicMax((unsigned long long*)&out->gmax, (unsigned long long)best);
     __syncthreads();

@@ -171,8 +171,8 @@ struct Engine::State {
     int cus = 256;
     Buf d_win, d_large, d_chunks, d_i0, d_map, d_coff, d_coffa, d_pending, d_scratch, d_hits, d_hit_n;
     Buf d_vmap, d_vhits, d_vhit_n, d_vchunks, h_vchunks;   // the GB_BUCKET_VERIFY rebuild
-    Buf d_geoms, d_cs, d_runs;
-    Buf d_out[2], h_out[2], h_geoms[2], h_cs[2], h_chunks[2], h_runs[2];
+    Buf d_geoms, d_cs, d_runs, d_rdesc, d_groups, d_region, d_fills;
+    Buf d_out[2], h_out[2], h_geoms[2], h_cs[2], h_chunks[2], h_runs[2], h_rdesc[2];
     Event ev[2][EV_N];
     Candidates cand;
 
@@ -414,7 +414,7 @@ public:
 
 private:
     double hit_factor(uint32_t j, double q_lo) const;
-    void plan_chunks(const std::vector<RunCost>& runs, int slot);
+    void plan_chunks(const std::vector<RunCost>& runs, uint32_t nbb, double ncb, int slot);
     uint32_t run_cap(const Range& r) const;
     void form(std::vector<SegJob>& bt);
     void enqueue(std::vector<SegJob>& batch, int slot, bool check_and_count);
@@ -444,6 +444,15 @@ private:
     std::vector<uint32_t> chunks_;
     uint32_t nchunks_ = 0, stage_ = 0;
     bool chunks_valid_ = false;
+    uint32_t chunk_nbb_ = 0;
+    double chunk_ncb_ = 0;
+    // the harvest's form: LDS-staged (short batches), direct (one walk into
+    // per-work-group regions, then the partition) or counted (two walks);
+    // GB_HL_MODE = auto | direct | counted
+    std::string hl_mode_ = env::choice("GB_HL_MODE", "auto", {"auto", "direct", "counted"});
+    bool direct_ = false, force_count_ = false, direct_used_[2] = {false, false};
+    uint64_t region_entries_ = 0;
+    uint32_t ngroups_ = 0;
     // GB_BUCKET_VERIFY=N: every Nth segment's map is rebuilt with every large
     // prime on the wheel walk (no multiplier bitmap) and compared word for word
     uint32_t verify_every_ = 0, cap_v_ = 0, nvchunks_ = 0;
@@ -590,7 +599,7 @@ void Pipeline::plan(const PlanBounds& pb)
     if (!multi_ && nlarge_) {   // a plain run's batches share one chunk plan: make it now
         const auto tp0 = clock_::now();
         plan_chunks({RunCost{nlarge_, double(kb_) * double(span_) * (8.0 / 30), double(pb.q_lo_np[0].first),
-                             density_max_}}, 0);
+                             density_max_}}, kb_ * 8, double(nc_max), 0);
         if (phase_trace_) std::fprintf(stderr, "plan: chunking %.3f ms\n", since(tp0) * 1e3);
     }
     if (phase_trace_)
@@ -637,11 +646,14 @@ void Pipeline::plan(const PlanBounds& pb)
 // GBH_HL_CHUNK primes each, split where the bitmap primes start and where a
 // run's primes end. Uploaded for the batch in @p slot (stream-ordered after
 // the batch before it).
-void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, int slot)
+void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, double ncb, int slot)
 {
-    if (chunks_valid_ && runs_in == chunk_key_) return;
+    if (chunks_valid_ && runs_in == chunk_key_ && nbb == chunk_nbb_ && ncb == chunk_ncb_) return;
     chunk_key_ = runs_in;
+    chunk_nbb_ = nbb;
+    chunk_ncb_ = ncb;
     chunks_.assign(1, 0u);
+    std::vector<double> hpb;    // per chunk: expected hits per class bit (the larger rate of its ends)
     std::vector<RunCost> rs;
     for (const RunCost& r : runs_in)
         if (r.np) rs.push_back(r);
@@ -685,7 +697,7 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, int slot)
         constexpr double kStageHits = 0.7 * GBH_HL_STAGE;
         double all_hits = 0;
         for (const RunCost& r : rs) all_hits += r.bits * r.density;
-        stage_ = all_hits <= kStageHits * 8192 ? GBH_HL_STAGE : 0u;
+        stage_ = hl_mode_ != "direct" && all_hits <= kStageHits * 8192 ? GBH_HL_STAGE : 0u;
         uint32_t j = 0, prev = 0;   // prev: the last chunk's length (lengths change slowly: start the search there)
         size_t ai = 0;
         while (j < npmax) {
@@ -722,6 +734,7 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, int slot)
                 }
             }
             prev = e - j;
+            hpb.push_back(std::max(hf, hit_factor(e - 1, q)) * (inv(e) - ij));
             chunks_.push_back(e);
             j = e;
         }
@@ -732,6 +745,54 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, int slot)
     E.d_chunks.ensure(c, chunks_.size() * 4, 0);
     std::memcpy(E.h_chunks[slot].p, chunks_.data(), chunks_.size() * 4);
     check(gbh_copy_h2d(c, 0, E.d_chunks.p, E.h_chunks[slot].p, chunks_.size() * 4), "upload chunks");
+    // the direct harvest (batches whose hits exceed the LDS stages): work-group
+    // k's region holds r_k entries per (segment, class) list, r_k well above
+    // the largest list's expected hits lambda_k (lambda + 8 sqrt(lambda) + 64;
+    // an overflow re-runs the batch with counted lists)
+    direct_ = nchunks_ && !stage_ && hl_mode_ != "counted";
+    if (direct_) {
+        std::vector<uint32_t> rd(2 * size_t(nchunks_));
+        std::vector<uint32_t> groups{0};   // the partition's chunk groups: their regions fit its LDS
+        uint64_t base = 0, gsum = 0, rmax = 0;
+        for (uint32_t k = 0; k < nchunks_; ++k) {
+            const double lam = ncb * hpb[k];
+            const uint64_t r = uint64_t(lam + 8 * std::sqrt(lam) + 64);
+            rd[2 * k] = uint32_t(base);
+            rd[2 * k + 1] = uint32_t(r);
+            base += r * nbb;
+            rmax = std::max(rmax, r);
+            if (base >= UINT32_MAX || r > GBH_PART_CAP) {
+                if (phase_trace_)
+                    std::fprintf(stderr, "plan: direct harvest off (chunk %u of %u: region %llu entries per list, "
+                                 "%llu in all)\n", k, nchunks_, (unsigned long long)r, (unsigned long long)base);
+                direct_ = false;
+                break;
+            }
+            if (k > groups.back() && (k - groups.back() >= GBH_PART_CHUNKS || gsum + r > GBH_PART_CAP)) {
+                groups.push_back(k);
+                gsum = 0;
+            }
+            gsum += r;
+        }
+        groups.push_back(nchunks_);
+        if (direct_) {
+            region_entries_ = base;
+            ngroups_ = uint32_t(groups.size() - 1);
+            if (phase_trace_)
+                std::fprintf(stderr, "plan: direct harvest: %llu region entries, at most %llu per list, %u partition "
+                             "groups\n", (unsigned long long)base, (unsigned long long)rmax, ngroups_);
+            E.h_rdesc[slot].ensure(c, rd.size() * 4 + groups.size() * 4, 1);
+            E.d_rdesc.ensure(c, rd.size() * 4, 0);
+            E.d_groups.ensure(c, groups.size() * 4, 0);
+            E.d_region.ensure(c, base * 4, 0);
+            E.d_fills.ensure(c, size_t(nchunks_) * nbb * 4, 0);
+            uint32_t* hr = E.h_rdesc[slot].as<uint32_t>();
+            std::memcpy(hr, rd.data(), rd.size() * 4);
+            std::memcpy(hr + rd.size(), groups.data(), groups.size() * 4);
+            check(gbh_copy_h2d(c, 0, E.d_rdesc.p, hr, rd.size() * 4), "upload regions");
+            check(gbh_copy_h2d(c, 0, E.d_groups.p, hr + rd.size(), groups.size() * 4), "upload region groups");
+        }
+    }
     chunks_valid_ = true;
 }
 
@@ -882,7 +943,12 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         }
         agg_.window_bytes += bits_all / 8 + win_prime_bytes + 4.0 * hits_all;
     }
-    if (multi_ && nlarge_) plan_chunks(costs, slot);
+    if (multi_ && nlarge_) {
+        double ncb = 0;
+        for (uint32_t s = 0; s < nseg; ++s)
+            for (int x = 0; x < 8; ++x) ncb = std::max<double>(ncb, hg[s].nc[x]);
+        plan_chunks(costs, nseg * 8, ncb, slot);
+    }
     check(gbh_copy_h2d(c, 0, E.d_geoms.p, hg, sizeof(gbh_geom) * nseg), "upload geometry");
     check(gbh_copy_h2d(c, 0, E.d_cs.p, hc, sizeof(gbh_countseg) * nseg), "upload count parameters");
     check(gbh_copy_h2d(c, 0, E.d_runs.p, hr, sizeof(gbh_hl_run) * nruns), "upload runs");
@@ -891,14 +957,26 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         gbh_setup_args a{b, E.d_win.as<uint32_t>(), nwin_, 0, E.d_i0.as<uint32_t>()};
         check(gbh_launch_setup(c, 0, &a), "setup");
     }
+    const bool direct = nlarge_ && direct_ && !force_count_;
+    direct_used_[slot] = direct;
     if (nlarge_) {
         check(gbh_zero(c, 0, E.d_hit_n.p, size_t(nseg) * 8 * nwin * 4), "zero hit counts");
+        if (direct) check(gbh_zero(c, 0, E.d_fills.p, size_t(nchunks_) * nseg * 8 * 4), "zero region fills");
         gbh_harvest_args h{b, E.d_large.as<uint32_t>(), E.d_chunks.as<uint32_t>(), E.d_runs.as<gbh_hl_run>(), nruns,
-                           run_max, nchunks_, cap_, t_hl_, stage_, mb_ ? mb_->bits : nullptr, mb_ ? mb_->m_base : 0,
-                           mb_ ? mb_->m_span : 0, E.d_hits.as<uint32_t>(), E.d_hit_n.as<uint32_t>(), &out[0].err};
+                           run_max, nchunks_, cap_, t_hl_, direct ? 0u : stage_, mb_ ? mb_->bits : nullptr,
+                           mb_ ? mb_->m_base : 0, mb_ ? mb_->m_span : 0, E.d_hits.as<uint32_t>(),
+                           E.d_hit_n.as<uint32_t>(), direct ? E.d_rdesc.as<uint32_t>() : nullptr,
+                           direct ? E.d_region.as<uint32_t>() : nullptr, direct ? E.d_fills.as<uint32_t>() : nullptr,
+                           &out[0].err};
         check(gbh_event_record(c, 0, ev[EV_HL0].e), "event");
         check(gbh_launch_harvest(c, 0, &h), "harvest");
         check(gbh_event_record(c, 0, ev[EV_HL1].e), "event");
+        if (direct) {
+            const gbh_partition_args pa{b, E.d_rdesc.as<uint32_t>(), E.d_region.as<uint32_t>(), E.d_fills.as<uint32_t>(),
+                                        E.d_groups.as<uint32_t>(), ngroups_, cap_, E.d_hits.as<uint32_t>(),
+                                        E.d_hit_n.as<uint32_t>(), &out[0].err};
+            check(gbh_launch_partition(c, 0, &pa), "partition");
+        }
     }
     check(gbh_event_record(c, 0, ev[EV_SETUP].e), "event");
     uint32_t ncoop = 0;
@@ -946,7 +1024,7 @@ void Pipeline::enqueue_verify(uint32_t s, const gbh_batch& b, gbh_segout* out)
     check(gbh_zero(c, 0, E.d_vhit_n.p, size_t(8) * b.nwin * 4), "verify zero");
     const gbh_harvest_args h{vb, E.d_large.as<uint32_t>(), E.d_vchunks.as<uint32_t>(), nullptr, 0, 0, nvchunks_, cap_v_,
                              UINT32_MAX, 0, nullptr, 0, 0, E.d_vhits.as<uint32_t>(), E.d_vhit_n.as<uint32_t>(),
-                             &out[s].err};
+                             nullptr, nullptr, nullptr, &out[s].err};
     check(gbh_launch_harvest(c, 0, &h), "verify harvest");
     const gbh_window_args w{vb, E.d_win.as<uint32_t>(), nwin_, ncoop, E.d_i0.as<uint32_t>(),
                             E.d_vhits.as<uint32_t>(), E.d_vhit_n.as<uint32_t>(), cap_v_, 0};
@@ -977,6 +1055,18 @@ void Pipeline::collect(std::vector<SegJob>& batch, int slot)
     agg_.verify_seconds += tc + tn;
     agg_.count_seconds += tn;
     const gbh_segout* outs = E.h_out[slot].as<gbh_segout>();
+    if ((outs[0].err & 1u) && direct_used_[slot]) {
+        // a region of the direct harvest overflowed: the batch again, with
+        // counted lists (queued behind the batch in flight, which keeps its
+        // own outputs)
+        std::fprintf(stderr, "note: harvest region overflow in the batch from %llu: re-run with counted lists\n",
+                     (unsigned long long)batch[0].sg.seg_start);
+        force_count_ = true;
+        enqueue(batch, slot, true);
+        force_count_ = false;
+        collect(batch, slot);
+        return;
+    }
     if (outs[0].err & 1u) throw std::runtime_error("hit-list bucket overflow (raise the capacity)");
     if (outs[0].err & 4u) throw std::runtime_error("hit lists: a bitmap prime's cube is not above the batch's q range");
     if (outs[0].err & 8u) throw std::runtime_error("hit lists: a multiplier outside the multiplier bitmap");

@@ -178,9 +178,38 @@ typedef struct {
     uint64_t m_span;               /* the bitmap covers m in [m_base, m_base + m_span) */
     uint32_t* hits;                /* [nseg][8][nwin][cap] window offsets */
     uint32_t* hit_n;               /* [nseg][8][nwin] fill counts (zeroed before the launch) */
-    uint32_t* err;                 /* bit 0: a list overflow or a run beyond the contract; bit 2: t_hl^3 <= q;
-                                      bit 3: m outside the bitmap */
+    /* Direct (one-pass) form, when rdesc is not null: work-group k appends
+     * the class bit of every hit to its own region of each (segment, class)
+     * list, region + rdesc[2k] + (s * 8 + c) * rdesc[2k + 1] (rdesc[2k + 1]
+     * entries each), and writes the counts to fills[k * nseg * 8 + s * 8 + c]
+     * (zeroed before the launch); gbh_launch_partition then builds the
+     * window lists. hits / hit_n / cap / stage are unused then (stage 0). */
+    const uint32_t* rdesc;
+    uint32_t* region;
+    uint32_t* fills;
+    uint32_t* err;                 /* bit 0: a list or region overflow or a run beyond the contract;
+                                      bit 2: t_hl^3 <= q; bit 3: m outside the bitmap */
 } gbh_harvest_args;
+
+/* Second step of the direct harvest: the regions of every chunk split into
+ * the window lists hits / hit_n of gbh_harvest_args (hit_n zeroed before).
+ * The chunks go in groups [groups[g], groups[g + 1]) of at most
+ * GBH_PART_CHUNKS chunks whose regions (rdesc[2k + 1] entries per list)
+ * hold at most GBH_PART_CAP entries together. */
+#define GBH_PART_CHUNKS 64
+#define GBH_PART_CAP 16384
+typedef struct {
+    gbh_batch b;
+    const uint32_t* rdesc;
+    const uint32_t* region;
+    const uint32_t* fills;
+    const uint32_t* groups;        /* [ngroups + 1] chunk boundaries */
+    uint32_t ngroups;
+    uint32_t cap;                  /* entries per window list */
+    uint32_t* hits;
+    uint32_t* hit_n;
+    uint32_t* err;                 /* bit 0: a window list overflow */
+} gbh_partition_args;
 
 /* Map self-check (reference GB_BUCKET_VERIFY compare kernel,
  * src/verifier.cpp:2059-2085): counts the words of one segment's class map
@@ -304,6 +333,7 @@ int gbh_event_elapsed_ms(gbh_ctx* ctx, gbh_event* a, gbh_event* b, float* ms);
 /* ---- stage launches (all asynchronous on `stream`) ---- */
 int gbh_launch_setup(gbh_ctx* ctx, int stream, const gbh_setup_args* a);
 int gbh_launch_harvest(gbh_ctx* ctx, int stream, const gbh_harvest_args* a);
+int gbh_launch_partition(gbh_ctx* ctx, int stream, const gbh_partition_args* a);
 int gbh_launch_mtrans(gbh_ctx* ctx, int stream, const gbh_mtrans_args* a);
 int gbh_launch_window(gbh_ctx* ctx, int stream, const gbh_window_args* a);
 int gbh_launch_check_a(gbh_ctx* ctx, int stream, const gbh_check_args* a);

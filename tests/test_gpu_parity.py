@@ -78,6 +78,12 @@ WINDOWS = [
     # the large-prime split: every large prime on the wheel walk (no
     # multiplier bitmap), and a small bitmap budget (wheel below ~1.7e7)
     ("w4e18", 2_000_000, {"GB_MT_GB": "0"}),
+    # the harvest's forms: direct (one walk into work-group regions, then the
+    # partition into window lists) and counted (two walks)
+    ("w4e18", 2_000_000, {"GB_HL_MODE": "direct"}),
+    ("w5e18end", 3_000_000, {"GB_HL_MODE": "direct", "GB_MT_GB": "8"}),
+    ("wtop", 1_000_000, {"GB_HL_MODE": "direct"}),
+    ("w4e18", 2_000_000, {"GB_HL_MODE": "counted"}),
     ("w44e18", 1_000_000, {"GB_MT_GB": "8"}),
     # every segment's map rebuilt with all large primes on the wheel walk and
     # compared word for word (the reference's GB_BUCKET_VERIFY self-check)
@@ -212,3 +218,23 @@ def test_prime_cache_gbp2(stats_env):
     r = run(a2 + 2_000, a2)
     n2 = int((a2 + 2_000) ** 0.5) + 16
     assert r.returncode == 0 and n2 < n and not (stats_env / f"pc.{n2}").exists()
+
+
+def test_long_batches_direct_harvest_maps_verified(stats_env, monkeypatch):
+    """Full 467.5M segments in batches of 8 (the direct harvest's production
+    shape): every segment's class map is rebuilt with every large prime on
+    the wheel walk and compared word for word on the GPU (GB_BUCKET_VERIFY),
+    and the statistics equal the counted harvest's."""
+    a = 4_500_000_000_000_000_000
+    b = a + 8 * 467_500_000 - 2
+    monkeypatch.setenv("GB_BUCKET_VERIFY", "1")
+    _say(f"long direct case [{a}, {b}]")
+    r = gbamd.run_gpu(b, 467_500_000, start_even=a)
+    assert r["segments"] == 8 and r["leftovers"] == 0
+    monkeypatch.delenv("GB_BUCKET_VERIFY")
+    for f in ("st.state", "st.log", "st.state.gaps"):
+        (stats_env / f).unlink(missing_ok=True)
+    monkeypatch.setenv("GB_HL_MODE", "counted")
+    _say("long counted case")
+    r2 = gbamd.run_gpu(b, 467_500_000, start_even=a)
+    assert compare.strip_version(r2["stats_line"]) == compare.strip_version(r["stats_line"])
