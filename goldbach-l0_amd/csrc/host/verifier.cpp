@@ -146,6 +146,40 @@ struct SegJob {
     bool sampled = false, walked = false, counted = false;
 };
 
+// Harvest cost of one run of a batch: its large primes, class bits, first q
+// and hit density.
+struct RunCost {
+    uint32_t np = 0;
+    double bits = 0, q_lo = 3, density = 0;
+};
+
+bool close_rel(double a, double b) { return std::fabs(a - b) <= 1e-3 * std::max(std::fabs(a), std::fabs(b)); }
+
+// Runs of the same shape: the same prime counts, class bits, first q and
+// hit density within 0.1% (a chunk plan balances load; any plan is exact).
+bool same_shape(const std::vector<RunCost>& a, const std::vector<RunCost>& b)
+{
+    if (a.size() != b.size()) return false;
+    for (size_t i = 0; i < a.size(); ++i)
+        if (a[i].np != b[i].np || !close_rel(a[i].bits, b[i].bits) || !close_rel(a[i].q_lo, b[i].q_lo) ||
+            !close_rel(a[i].density, b[i].density))
+            return false;
+    return true;
+}
+
+// The harvest chunk plan last uploaded to an engine's device tables (chunk
+// bounds, direct-harvest regions and partition groups): a batch of the same
+// shape (the next batch of a run, the next run of a campaign) reuses it.
+struct ChunkPlan {
+    bool valid = false;
+    std::vector<RunCost> key;
+    uint32_t nbb = 0, nwheel = 0;
+    double ncb = 0;
+    std::string mode;
+    uint32_t nchunks = 0, stage = 0, ngroups = 0;
+    bool direct = false;
+};
+
 }  // namespace
 
 // ---------------------------------------------------------------- Engine state
@@ -175,6 +209,7 @@ struct Engine::State {
     Buf d_out[2], h_out[2], h_geoms[2], h_cs[2], h_chunks[2], h_runs[2], h_rdesc[2];
     Event ev[2][EV_N];
     Candidates cand;
+    ChunkPlan cplan;
 
     State(uint64_t max_limit_, uint64_t max_span_, const char* prime_cache)
     {
@@ -381,13 +416,6 @@ PlanBounds bounds_of(const Range& r)
     return b;
 }
 
-// Harvest cost of one run of a batch: its large primes, class bits, first q
-// and hit density.
-struct RunCost {
-    uint32_t np = 0;
-    double bits = 0, q_lo = 3, density = 0;
-    bool operator==(const RunCost& o) const { return np == o.np && bits == o.bits && q_lo == o.q_lo && density == o.density; }
-};
 
 // The segment pipeline of one engine: batches of segments (of one range, or
 // of several ranges as runs), each batch one launch per stage on one
@@ -440,12 +468,8 @@ private:
     double density_max_ = 0;
     std::shared_ptr<const MultBitmap> mb_;
     // harvest chunks of the batch in each slot (re-planned when the runs change)
-    std::vector<RunCost> chunk_key_;
     std::vector<uint32_t> chunks_;
     uint32_t nchunks_ = 0, stage_ = 0;
-    bool chunks_valid_ = false;
-    uint32_t chunk_nbb_ = 0;
-    double chunk_ncb_ = 0;
     // the harvest's form: LDS-staged (short batches), direct (one walk into
     // per-work-group regions, then the partition) or counted (two walks);
     // GB_HL_MODE = auto | direct | counted
@@ -595,7 +619,6 @@ void Pipeline::plan(const PlanBounds& pb)
         E.h_cs[s].ensure(c, sizeof(gbh_countseg) * K, 1);
         E.h_runs[s].ensure(c, sizeof(gbh_hl_run) * K, 1);
     }
-    chunks_valid_ = false;
     if (!multi_ && nlarge_) {   // a plain run's batches share one chunk plan: make it now
         const auto tp0 = clock_::now();
         plan_chunks({RunCost{nlarge_, double(kb_) * double(span_) * (8.0 / 30), double(pb.q_lo_np[0].first),
@@ -648,10 +671,16 @@ void Pipeline::plan(const PlanBounds& pb)
 // the batch before it).
 void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, double ncb, int slot)
 {
-    if (chunks_valid_ && runs_in == chunk_key_ && nbb == chunk_nbb_ && ncb == chunk_ncb_) return;
-    chunk_key_ = runs_in;
-    chunk_nbb_ = nbb;
-    chunk_ncb_ = ncb;
+    ChunkPlan& cp = E.cplan;
+    if (cp.valid && cp.nbb == nbb && close_rel(cp.ncb, ncb) && cp.nwheel == nwheel_ && cp.mode == hl_mode_ &&
+        same_shape(cp.key, runs_in)) {   // the device holds this plan already
+        nchunks_ = cp.nchunks;
+        stage_ = cp.stage;
+        direct_ = cp.direct;
+        ngroups_ = cp.ngroups;
+        return;
+    }
+    cp.valid = false;
     chunks_.assign(1, 0u);
     std::vector<double> hpb;    // per chunk: expected hits per class bit (the larger rate of its ends)
     std::vector<RunCost> rs;
@@ -793,7 +822,16 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
             check(gbh_copy_h2d(c, 0, E.d_groups.p, hr + rd.size(), groups.size() * 4), "upload region groups");
         }
     }
-    chunks_valid_ = true;
+    cp.key = runs_in;
+    cp.nbb = nbb;
+    cp.ncb = ncb;
+    cp.nwheel = nwheel_;
+    cp.mode = hl_mode_;
+    cp.nchunks = nchunks_;
+    cp.stage = stage_;
+    cp.direct = direct_;
+    cp.ngroups = ngroups_;
+    cp.valid = true;
 }
 
 // Segments of one range a batch may hold: the run's q range below 2^33 and
