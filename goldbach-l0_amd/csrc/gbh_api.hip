@@ -236,6 +236,12 @@ int gbh_event_elapsed_ms(gbh_ctx*, gbh_event* a, gbh_event* b, float* ms)
     return GBH_OK;
 }
 
+int gbh_stream_wait(gbh_ctx* c, int s, gbh_event* ev)
+{
+    GBH_HIP(hipStreamWaitEvent(S(c, s), ev->ev, 0));
+    return GBH_OK;
+}
+
 namespace {
 
 int check_batch(const gbh_batch& b)
@@ -319,8 +325,11 @@ int gbh_launch_window(gbh_ctx* c, int s, const gbh_window_args* a)
     if (const int rc = check_batch(a->b)) return rc;
     if (a->ncoop > a->nprimes) return fail(GBH_E_ARG, "ncoop > nprimes");
     const size_t lds = a->b.wbits / 8;
+    const uint32_t phases = a->phases ? a->phases : 3u;
+    if (phases > 3) return fail(GBH_E_ARG, "window: phases beyond 3");
+    if (phases == 2 && !a->hits) return GBH_OK;    // the hit-list phase alone, without lists: nothing to do
     hipLaunchKernelGGL(gbh_dev::k_window, dim3(a->b.nwin, GBH_CLASSES, a->b.nseg), dim3(256), lds, S(c, s), a->b,
-                       a->primes, a->nprimes, a->ncoop, a->i0, a->hits, a->hit_n, a->cap);
+                       a->primes, a->nprimes, a->ncoop, a->i0, a->hits, a->hit_n, a->cap, phases);
     GBH_HIP(hipGetLastError());
     return GBH_OK;
 }

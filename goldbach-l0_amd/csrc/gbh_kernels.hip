@@ -708,7 +708,7 @@ __device__ __forceinline__ uint32_t first_off_big(uint32_t t, uint32_t wb0, uint
 __global__ void __launch_bounds__(256) k_window(gbh_batch b, const uint32_t* __restrict__ primes, uint32_t np,
                                                 uint32_t ncoop, const uint32_t* __restrict__ i0tab,
                                                 const uint32_t* __restrict__ hits, const uint32_t* __restrict__ hit_n,
-                                                uint32_t cap)
+                                                uint32_t cap, uint32_t phases)
 {
     extern __shared__ uint64_t win[];
     uint32_t* win32 = reinterpret_cast<uint32_t*>(win);
@@ -721,9 +721,12 @@ __global__ void __launch_bounds__(256) k_window(gbh_batch b, const uint32_t* __r
     const uint32_t wb1 = wb0 + b.wbits;
     const uint32_t tid = threadIdx.x;
     const uint32_t* i0 = i0tab + ((size_t)s * 8 + c) * np;
+    const bool sieve = phases & 1, apply = (phases & 2) && hits;
+    uint64_t* dst = seg_map(b, s) + (size_t)c * b.cw + gw0;
 
-    // 1) mask stage: every multiple of 7..61 cleared by whole-word masks
-    {
+    if (!sieve) {   // the hit lists only: the window as the sieve phase stored it
+        for (uint32_t x = tid; x < nwords; x += 256) win[x] = dst[x];
+    } else {        // 1) mask stage: every multiple of 7..61 cleared by whole-word masks
         uint32_t ph[GBH_NSMALL];
 #pragma unroll
         for (int k = 0; k < GBH_NSMALL; ++k) {
@@ -750,7 +753,7 @@ __global__ void __launch_bounds__(256) k_window(gbh_batch b, const uint32_t* __r
     // 2) window primes below wbits / 128 (at least 128 marks each): one
     //    wave per prime, lane l takes marks l, l + 64, ...
     const uint32_t lane = tid & 63, wv = tid >> 6, wbits = b.wbits;
-    np = min(np, g.np_win);                // the segment's primes (larger ones have no multiple in it)
+    np = sieve ? min(np, g.np_win) : 0u;   // the segment's primes (larger ones have no multiple in it)
     ncoop = min(ncoop, np);
     for (uint32_t j = wv; j < ncoop; j += 4) {
         const uint32_t t0 = i0[j];
@@ -768,7 +771,7 @@ __global__ void __launch_bounds__(256) k_window(gbh_batch b, const uint32_t* __r
             atomicAnd(&win32[bt >> 5], ~(1u << (bt & 31)));
     }
     // 4) the large primes' hits of this window
-    if (hits) {
+    if (apply) {
         const uint32_t bucket = (s * 8 + c) * b.nwin + blockIdx.x;
         const uint32_t n = min(hit_n[bucket], cap);
         const uint32_t* hl = hits + (size_t)bucket * cap;
@@ -780,7 +783,6 @@ __global__ void __launch_bounds__(256) k_window(gbh_batch b, const uint32_t* __r
     __syncthreads();
 
     // 5) coalesced store of the window
-    uint64_t* dst = seg_map(b, s) + (size_t)c * b.cw + gw0;
     for (uint32_t x = tid; x < nwords; x += 256) dst[x] = win[x];
 }
 

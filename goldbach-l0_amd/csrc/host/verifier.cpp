@@ -114,7 +114,10 @@ struct Candidates {
     }
 };
 
-enum { EV_BEGIN, EV_HL0, EV_HL1, EV_SETUP, EV_WINDOW, EV_CHECK, EV_COUNT, EV_END, EV_N };
+// Batch events: stream 0 carries the uploads, the harvest, the hit-list
+// phase of the window stage, the checker and the counter; stream 1 the
+// set-up and the sieve phase of the window stage, beside the harvest.
+enum { EV_BEGIN, EV_HL0, EV_HL1, EV_SETUP, EV_WINDOW, EV_CHECK, EV_COUNT, EV_END, EV_GEOM, EV_S0, EV_S1, EV_N };
 
 // Knobs of removed reference pipelines are refused, not ignored: a run that
 // asks for one would otherwise silently run something else
@@ -475,6 +478,8 @@ private:
     // GB_HL_MODE = auto | direct | counted
     std::string hl_mode_ = env::choice("GB_HL_MODE", "auto", {"auto", "direct", "counted"});
     bool direct_ = false, force_count_ = false, direct_used_[2] = {false, false};
+    // the window stage's sieve phase beside the harvest on a second stream (GB_OVERLAP=0: one stream)
+    bool overlap_ = env::onoff("GB_OVERLAP", true);
     uint64_t region_entries_ = 0;
     uint32_t ngroups_ = 0;
     // GB_BUCKET_VERIFY=N: every Nth segment's map is rebuilt with every large
@@ -991,10 +996,24 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     check(gbh_copy_h2d(c, 0, E.d_cs.p, hc, sizeof(gbh_countseg) * nseg), "upload count parameters");
     check(gbh_copy_h2d(c, 0, E.d_runs.p, hr, sizeof(gbh_hl_run) * nruns), "upload runs");
     check(gbh_reset_outputs(c, 0, out, nseg), "reset outputs");
+    uint32_t ncoop = 0;
+    while (ncoop < nwin_ && E.base[E.win_lo + ncoop] < wbits / 128) ++ncoop;
+    // the window stage's sieve phase (masks, window primes) on stream 1
+    // beside the harvest; it starts once this batch's tables are uploaded
+    // (and so every kernel of the batch before has finished on stream 0)
+    const int ss = overlap_ ? 1 : 0;
+    check(gbh_event_record(c, 0, ev[EV_GEOM].e), "event");
+    if (ss) check(gbh_stream_wait(c, ss, ev[EV_GEOM].e), "stream wait");
     if (nwin_) {
         gbh_setup_args a{b, E.d_win.as<uint32_t>(), nwin_, 0, E.d_i0.as<uint32_t>()};
-        check(gbh_launch_setup(c, 0, &a), "setup");
+        check(gbh_launch_setup(c, ss, &a), "setup");
     }
+    check(gbh_event_record(c, ss, ev[EV_S0].e), "event");
+    {
+        const gbh_window_args w{b, E.d_win.as<uint32_t>(), nwin_, ncoop, E.d_i0.as<uint32_t>(), nullptr, nullptr, 0, 1};
+        check(gbh_launch_window(c, ss, &w), "window sieve");
+    }
+    check(gbh_event_record(c, ss, ev[EV_S1].e), "event");
     const bool direct = nlarge_ && direct_ && !force_count_;
     direct_used_[slot] = direct;
     if (nlarge_) {
@@ -1017,11 +1036,12 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         }
     }
     check(gbh_event_record(c, 0, ev[EV_SETUP].e), "event");
-    uint32_t ncoop = 0;
-    while (ncoop < nwin_ && E.base[E.win_lo + ncoop] < wbits / 128) ++ncoop;
-    gbh_window_args w{b, E.d_win.as<uint32_t>(), nwin_, ncoop, E.d_i0.as<uint32_t>(),
-                      nlarge_ ? E.d_hits.as<uint32_t>() : nullptr, E.d_hit_n.as<uint32_t>(), cap_, 0};
-    check(gbh_launch_window(c, 0, &w), "window sieve");
+    if (ss) check(gbh_stream_wait(c, 0, ev[EV_S1].e), "stream wait");
+    if (nlarge_) {   // the hit-list phase: every window's list applied to the sieved map
+        const gbh_window_args w{b, E.d_win.as<uint32_t>(), nwin_, ncoop, E.d_i0.as<uint32_t>(),
+                                E.d_hits.as<uint32_t>(), E.d_hit_n.as<uint32_t>(), cap_, 2};
+        check(gbh_launch_window(c, 0, &w), "window hit lists");
+    }
     check(gbh_event_record(c, 0, ev[EV_WINDOW].e), "event");
     if (verify_every_ && nlarge_ && check_and_count)
         for (uint32_t s = 0; s < nseg; ++s)
@@ -1078,8 +1098,10 @@ void Pipeline::collect(std::vector<SegJob>& batch, int slot)
     gbh_ctx* c = E.c;
     auto& ev = E.ev[slot];
     check(gbh_event_sync(c, ev[EV_END].e), "batch sync");
+    // the sieve phase ran beside the harvest: its own time plus the hit-list
+    // phase count as the window stage
     const double tl = elapsed_s(c, ev[EV_BEGIN], ev[EV_SETUP]);
-    const double ts = elapsed_s(c, ev[EV_SETUP], ev[EV_WINDOW]);
+    const double ts = elapsed_s(c, ev[EV_S0], ev[EV_S1]) + elapsed_s(c, ev[EV_SETUP], ev[EV_WINDOW]);
     if (harvested_[slot]) {
         agg_.harvest_kernel_seconds += elapsed_s(c, ev[EV_HL0], ev[EV_HL1]);
         ++agg_.harvest_launches;

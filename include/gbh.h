@@ -243,7 +243,10 @@ typedef struct {
 /* Stage 3: LDS window sieve of every class array of every segment
  * (reference make_prime_map + mid_window + hl_apply, goldbach.cl:1092-1608,
  * 3434-3457, 3600-3701): masks for 7..61, marks of the window primes, the
- * window's hit list, one coalesced store. */
+ * window's hit list, one coalesced store. phases: bit 0 the masks and window
+ * primes (the window starts from the masks), bit 1 the hit lists (without
+ * bit 0 the window starts from the map as stored); 0 means both. The sieve
+ * phase does not read the hit lists, so it can run beside the harvest. */
 typedef struct {
     gbh_batch b;
     const uint32_t* primes;        /* window primes (device) */
@@ -253,7 +256,7 @@ typedef struct {
     const uint32_t* hits;          /* stage-2 lists (nullptr: no large primes) */
     const uint32_t* hit_n;
     uint32_t cap;
-    uint32_t pad0;
+    uint32_t phases;               /* 1: sieve, 2: hit lists, 0 or 3: both */
 } gbh_window_args;
 
 /* Stage 4: checker phase A (reference check30_a, goldbach.cl:2028-2189):
@@ -329,6 +332,9 @@ int gbh_event_destroy(gbh_ctx* ctx, gbh_event* ev);
 int gbh_event_record(gbh_ctx* ctx, int stream, gbh_event* ev);
 int gbh_event_sync(gbh_ctx* ctx, gbh_event* ev);
 int gbh_event_elapsed_ms(gbh_ctx* ctx, gbh_event* a, gbh_event* b, float* ms);
+/* Work queued on `stream` after this call waits for `ev` (recorded on the
+ * other stream). */
+int gbh_stream_wait(gbh_ctx* ctx, int stream, gbh_event* ev);
 
 /* ---- stage launches (all asynchronous on `stream`) ---- */
 int gbh_launch_setup(gbh_ctx* ctx, int stream, const gbh_setup_args* a);

@@ -84,6 +84,8 @@ WINDOWS = [
     ("w5e18end", 3_000_000, {"GB_HL_MODE": "direct", "GB_MT_GB": "8"}),
     ("wtop", 1_000_000, {"GB_HL_MODE": "direct"}),
     ("w4e18", 2_000_000, {"GB_HL_MODE": "counted"}),
+    # one stream: the window stage's sieve phase after the harvest
+    ("w4e18", 2_000_000, {"GB_OVERLAP": "0"}),
     ("w44e18", 1_000_000, {"GB_MT_GB": "8"}),
     # every segment's map rebuilt with all large primes on the wheel walk and
     # compared word for word (the reference's GB_BUCKET_VERIFY self-check)
