@@ -245,9 +245,11 @@ GoldbachStats::GoldbachStats(const std::vector<uint32_t>& cands, uint32_t tail_t
     samp_.assign(15 * np_, 0);
     tail_.assign(15 * np_, 0);
     tail_agg_.assign(np_, 0);
-    cum_samp_.assign(15 * np_, 0);
-    cum_tail_.assign(15 * np_, 0);
-    cum_tail_agg_.assign(np_, 0);
+    if (campaign_) {   // a run of its own fills these at its commit
+        cum_samp_.assign(15 * np_, 0);
+        cum_tail_.assign(15 * np_, 0);
+        cum_tail_agg_.assign(np_, 0);
+    }
     first_.assign(np_, 0);
     run_first_.assign(np_, 0);
     want_.assign(np_, 0);
@@ -289,6 +291,31 @@ uint64_t GoldbachStats::add_hist(bool sampled, const std::vector<uint64_t>& h)
     return n;
 }
 
+uint64_t GoldbachStats::add_hist_slots(bool sampled, const uint32_t* counts, size_t stride,
+                                       const std::vector<uint16_t>* slot_pi)
+{
+    uint64_t n = 0;
+    for (uint32_t cr = 0; cr < 15; ++cr) {
+        const uint32_t* row = counts + cr * stride;
+        const std::vector<uint16_t>& sp = slot_pi[cr];
+        for (size_t k = 0; k < sp.size(); ++k) {
+            const uint32_t c = row[k];
+            if (!c) continue;
+            const uint32_t i = sp[k];
+            const bool tail = cands_[i] >= tail_t_;
+            if (sampled) {
+                samp_[cr * np_ + i] += c;
+                if (tail) tail_[cr * np_ + i] += c;
+                n += c;
+            } else if (tail) {
+                tail_agg_[i] += c;
+            }
+        }
+    }
+    evens_sampled_ += n;
+    return n;
+}
+
 void GoldbachStats::add_first(uint64_t seg_start, const uint32_t* first)
 {
     for (size_t i = 0; i < np_; ++i)
@@ -303,7 +330,7 @@ void GoldbachStats::add_counts(const uint32_t cnt[8], const uint32_t tw[3])
 
 void GoldbachStats::begin_gaps(uint64_t start)
 {
-    gap_last_ = prev_prime(start - 3);
+    gap_last_ = 0;             // prev_prime(start - 3), found when the first segment arrives
     gap_start_ = start;
 }
 
@@ -322,6 +349,7 @@ void GoldbachStats::add_gap_segment(const SegmentGaps& s)
         return;
     }
     const uint64_t fq = s.q_first + s.pfirst, lq = s.q_first + s.plast;
+    if (!gap_last_) gap_last_ = prev_prime(gap_start_ - 3);
     if (s.walked) {
         std::map<uint32_t, uint64_t> by_g;
         for (uint32_t i = 0; i < s.ntail; ++i) by_g[s.tail[2 * i + 1]] += 1;
@@ -396,6 +424,7 @@ void GoldbachStats::finish_run(uint64_t start, uint64_t limit, uint64_t evens)
         if (is_prime_u64(q) && is_prime_u64(q + 2)) twins_[twin_idx(q)] -= 1;
     if (is_prime_u64(limit - 3) && is_prime_u64(limit - 1)) twins_[twin_idx(limit - 3)] += 1;
     if (gap_start_ == start) {
+        if (!gap_last_) gap_last_ = prev_prime(gap_start_ - 3);   // no segment had a prime
         gaps_.n = run_primes();
         if (limit >= start + 2) {
             if (is_prime_u64(start - 3) && is_prime_u64(start - 1)) gaps_.walk_remove(2);
@@ -803,8 +832,9 @@ std::string GoldbachStats::memory_commit(uint64_t sampled_total)
     if (evens_sampled_ != sampled_total)
         throw std::runtime_error("stats: " + std::to_string(evens_sampled_) + " sampled evens recorded, " +
                                  std::to_string(sampled_total) + " in the sampled segments");
-    for (size_t i = 0; i < samp_.size(); ++i) { cum_samp_[i] += samp_[i]; cum_tail_[i] += tail_[i]; }
-    for (size_t i = 0; i < np_; ++i) cum_tail_agg_[i] += tail_agg_[i];
+    cum_samp_ = samp_;     // a fresh campaign: its totals are this run's
+    cum_tail_ = tail_;
+    cum_tail_agg_ = tail_agg_;
     for (size_t i = 0; i < np_; ++i)
         if (run_first_[i] && (!first_[i] || run_first_[i] < first_[i])) first_[i] = run_first_[i];
     for (int i = 0; i < 8; ++i) tot_primes_[i] += primes_res_[i];

@@ -88,6 +88,10 @@ public:
     /// ([15][ncand]); sampled: all of them count, else only p >= tail_t.
     /// @return evens counted into the sampled histogram.
     uint64_t add_hist(bool sampled, const std::vector<uint64_t>& by_class_prime);
+    /// The same from the device's per-slot counts: counts[cr * stride + k]
+    /// evens of class cr resolved by slot k, slot_pi[cr][k] its prime index.
+    uint64_t add_hist_slots(bool sampled, const uint32_t* counts, size_t stride,
+                            const std::vector<uint16_t>* slot_pi);
     /// First even index (from seg_start) by prime index (~0u: none).
     void add_first(uint64_t seg_start, const uint32_t* first_by_prime);
     void add_counts(const uint32_t cnt[8], const uint32_t twins[3]);

@@ -547,12 +547,18 @@ double Pipeline::density_of(uint64_t q_lo, uint32_t nlarge) const
 {
     if (!nlarge) return 0.0;
     const uint32_t nw = std::min(nwheel_, nlarge);
-    double d = kWheelFrac * E.inv_at(nw);
+    const double inv_nw = E.inv_at(nw);
+    double d = kWheelFrac * inv_nw;
     const uint32_t n = nlarge - nw;
-    for (uint32_t k = 0; k < 64 && n; ++k) {
-        const uint32_t a = nw + uint32_t(uint64_t(n) * k / 64), b = nw + uint32_t(uint64_t(n) * (k + 1) / 64);
+    double ia = inv_nw;
+    uint32_t a = nw;
+    for (uint32_t k = 1; k <= 64 && a < nlarge; ++k) {   // band ends on the 1/p table's 256-prime grid
+        uint32_t b = k == 64 ? nlarge : (nw + uint32_t(uint64_t(n) * k / 64)) & ~255u;
         if (b <= a) continue;
-        d += (E.inv_at(b) - E.inv_at(a)) * std::max(hit_factor(a, double(q_lo)), hit_factor(b - 1, double(q_lo)));
+        const double ib = b == nlarge ? E.inv_at(b) : E.inv_coarse[b >> 8];
+        d += (ib - ia) * std::max(hit_factor(a, double(q_lo)), hit_factor(b - 1, double(q_lo)));
+        a = b;
+        ia = ib;
     }
     return d;
 }
@@ -1181,12 +1187,7 @@ void Pipeline::collect_segment(const SegJob& j, const gbh_segout& o, const doubl
     r.rs.leftovers += o.left_n;
     if (r.ridges) r.ridges->merge(o.ridge, o.ridge_n, GBH_RIDGE_CAP, j.sg.seg_start);
     if (r.gs) {
-        const size_t np = E.cand.primes.size();
-        std::vector<uint64_t> h(15 * np, 0);
-        for (uint32_t cr = 0; cr < 15; ++cr)
-            for (size_t k = 0; k < E.cand.slot_pi[cr].size(); ++k)
-                h[cr * np + E.cand.slot_pi[cr][k]] += o.hist[cr * GBH_NCAND + k];
-        r.gs->add_hist(j.sampled, h);
+        r.gs->add_hist_slots(j.sampled, o.hist, GBH_NCAND, E.cand.slot_pi);
         r.gs->add_first(j.sg.seg_start, o.first);
         if (r.gaps_on) {
             if (o.gtail_n > GBH_GAP_TCAP) throw std::runtime_error("count: gap tail list overflow" + seg);
