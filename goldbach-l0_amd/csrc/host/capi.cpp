@@ -9,6 +9,7 @@
 
 #include "config.hpp"
 #include "goldbach_amd.h"
+#include "tune.hpp"
 #include "verifier.hpp"
 
 namespace {
@@ -227,5 +228,18 @@ int gba_batcher_totals(gba_batcher* b, gba_run_result* out)
 }
 
 void gba_batcher_close(gba_batcher* b) { delete b; }
+
+int gba_tune_load(const char* path, uint64_t limit, char* applied, size_t applied_cap, char* err, size_t err_cap)
+{
+    try {
+        const std::string a = path ? gb::tune_load(path, limit) : (gb::tune_clear(), std::string());
+        put(applied, applied_cap, a.c_str());
+        put(err, err_cap, "");
+        return 0;
+    } catch (const std::exception& ex) {
+        put(err, err_cap, ex.what());
+        return 1;
+    }
+}
 
 }  // extern "C"

@@ -10,11 +10,22 @@
 
 namespace gb::env {
 
-inline const char* raw(const char* name) { return std::getenv(name); }
-inline bool present(const char* name) { return std::getenv(name) != nullptr; }
-inline bool is_set(const char* name)
+/// A tuned default of a pipeline knob (tune.cpp: the GB_TUNE_FILE band of
+/// the engine's range), or nullptr. The environment always wins.
+const char* tuned(const char* name);
+
+// the knob's value: the environment's, else the tuned default
+inline const char* value(const char* name)
 {
     const char* v = std::getenv(name);
+    return v ? v : tuned(name);
+}
+
+inline const char* raw(const char* name) { return value(name); }
+inline bool present(const char* name) { return value(name) != nullptr; }
+inline bool is_set(const char* name)
+{
+    const char* v = value(name);
     return v && *v;
 }
 
@@ -33,7 +44,7 @@ inline uint64_t parse_u64(const char* name, const char* v)
 
 inline uint64_t u64(const char* name, uint64_t dflt, uint64_t lo = 0, uint64_t hi = UINT64_MAX)
 {
-    const char* v = std::getenv(name);
+    const char* v = value(name);
     if (!v) return dflt;
     const uint64_t x = parse_u64(name, v);
     if (x < lo || x > hi)
@@ -49,7 +60,7 @@ inline uint32_t u32(const char* name, uint32_t dflt, uint32_t lo = 0, uint32_t h
 
 inline bool onoff(const char* name, bool dflt)
 {
-    const char* v = std::getenv(name);
+    const char* v = value(name);
     if (!v) return dflt;
     const std::string s = v;
     if (s == "1" || s == "on" || s == "yes" || s == "true") return true;
@@ -59,7 +70,7 @@ inline bool onoff(const char* name, bool dflt)
 
 inline std::string choice(const char* name, const char* dflt, std::initializer_list<const char*> ok)
 {
-    const char* v = std::getenv(name);
+    const char* v = value(name);
     if (!v) return dflt;
     for (const char* o : ok)
         if (std::string(v) == o) return v;

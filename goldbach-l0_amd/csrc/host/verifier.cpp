@@ -25,6 +25,7 @@
 #include "mbits.hpp"
 #include "primes.hpp"
 #include "ridge.hpp"
+#include "tune.hpp"
 
 namespace gb {
 namespace {
@@ -213,11 +214,15 @@ struct Engine::State {
     Event ev[2][EV_N];
     Candidates cand;
     ChunkPlan cplan;
+    std::string tuned;                   // knob defaults taken from GB_TUNE_FILE
 
     State(uint64_t max_limit_, uint64_t max_span_, const char* prime_cache)
     {
         t0 = clock_::now();
         require_supported_limit(max_limit_);
+        // the autotuner's knob defaults for this engine's range (before any knob is read)
+        if (const char* tf = std::getenv("GB_TUNE_FILE")) tuned = tune_load(tf, max_limit_);
+        else tune_clear();
         max_limit = max_limit_;
         max_span = std::max<uint64_t>(max_span_ + (max_span_ & 1), 4);
         if (max_span > (uint64_t(1) << 35)) throw std::runtime_error("segment span above 2^35 integers");

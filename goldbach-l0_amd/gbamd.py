@@ -88,6 +88,8 @@ def load() -> ctypes.CDLL:
         lib.gba_batcher_totals.restype = ctypes.c_int
         lib.gba_batcher_close.argtypes = [vp]
         lib.gba_batcher_close.restype = None
+        lib.gba_tune_load.argtypes = [cp, u64, cp, sz, cp, sz]
+        lib.gba_tune_load.restype = ctypes.c_int
         _lib = lib
     return _lib
 
@@ -248,6 +250,17 @@ class Batcher:
             self.close()
         except Exception:
             pass
+
+
+def tune_load(path, limit: int) -> str:
+    """Installs the tuned knob defaults of `path`'s band holding `limit`
+    (path None: drops them); returns the installed "KNOB=value ..." text."""
+    lib = load()
+    out = ctypes.create_string_buffer(4096)
+    err = ctypes.create_string_buffer(4096)
+    if lib.gba_tune_load(None if path is None else str(path).encode(), int(limit), out, len(out), err, len(err)):
+        raise VerificationError(err.value.decode())
+    return out.value.decode()
 
 
 def segment_map(seg_start: int, limit: int, span: int):

@@ -108,6 +108,14 @@ int gba_batcher_result(gba_batcher* b, uint64_t id, gba_run_result* out, char* s
 int gba_batcher_totals(gba_batcher* b, gba_run_result* out);
 void gba_batcher_close(gba_batcher* b);
 
+/* Tuned knob defaults (the autotuner's file, goldbach-l0_amd/autotune.py):
+ * installs the knobs of the first band of `path` that holds `limit` as the
+ * defaults of knobs the environment does not set (path NULL: drops them);
+ * `applied` receives "KNOB=value ...". An engine does this itself for its
+ * limit when GB_TUNE_FILE is set. No counterpart in the reference (its
+ * knobs are environment variables only). Returns 0 on success. */
+int gba_tune_load(const char* path, uint64_t limit, char* applied, size_t applied_cap, char* err, size_t err_cap);
+
 /* The library's version string. */
 const char* gba_version(void);
 
