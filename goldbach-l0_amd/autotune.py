@@ -9,6 +9,8 @@ passes), the choices that shape the pipeline on a sample workload:
     GB_BATCH    segments per batch (ranges of many segments)
     GB_HL_MODE  harvest form of long batches (direct or counted)
     GB_OVERLAP  sieve phase beside the harvest on a second stream
+    GB_HL_WHEEL_HITS  wheel walk instead of random bitmap reads below this
+                many hits per prime
 
 Each candidate runs the sample range on a fresh engine (one warmup, then
 the best of `--reps` timed runs; evens per second of wall time). Every choice
@@ -40,9 +42,10 @@ SPACE = {
     "GB_BATCH": ["8", "16"],
     "GB_HL_MODE": ["auto", "counted"],
     "GB_OVERLAP": ["1", "0"],
+    "GB_HL_WHEEL_HITS": ["0", "2", "4", "8"],
 }
 DEFAULT = {"GB_WIN_KB": "0", "GB_WIN_P": "524288", "GB_KA": "96", "GB_BATCH": "16", "GB_HL_MODE": "auto",
-           "GB_OVERLAP": "1"}
+           "GB_OVERLAP": "1", "GB_HL_WHEEL_HITS": "2"}
 
 
 def band_of(limit: int):

@@ -290,7 +290,7 @@ int gbh_launch_harvest(gbh_ctx* c, int s, const gbh_harvest_args* a)
     const size_t lds = a->stage ? size_t(nb) * 8 + size_t(a->stage) * 6 + 16 : size_t(nb) * 4;
     if (lds > 144 * 1024) return fail(GBH_E_ARG, "harvest: lists and stage exceed the work-group's LDS");
     hipLaunchKernelGGL(gbh_dev::k_harvest, dim3(a->nchunks), dim3(256), lds, S(c, s), a->b, a->primes, a->chunks,
-                       a->runs, a->runs ? a->nruns : 1u, uint32_t(nb), a->t_hl, a->mbits, a->m_base, a->m_span,
+                       a->runs, a->runs ? a->nruns : 1u, uint32_t(nb), a->t_hl, a->t_hi, a->t_hi_end, a->mbits, a->m_base, a->m_span,
                        a->cap, a->stage, a->hits, a->hit_n, a->rdesc, a->region, a->fills, a->err);
     GBH_HIP(hipGetLastError());
     return GBH_OK;

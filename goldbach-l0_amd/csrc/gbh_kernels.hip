@@ -433,7 +433,9 @@ __device__ __forceinline__ void bm_group(const HlCtx& h, double qb0d, const uint
 __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __restrict__ primes,
                                                  const uint32_t* __restrict__ chunks,
                                                  const gbh_hl_run* __restrict__ runs, uint32_t nruns, uint32_t nbmax,
-                                                 uint32_t t_hl, const uint64_t* __restrict__ mbits, uint64_t m_base,
+                                                 uint32_t t_hl, uint32_t t_hi, uint32_t t_hi_end,
+                                                 const uint64_t* __restrict__ mbits,
+                                                 uint64_t m_base,
                                                  uint64_t m_span, uint32_t cap, uint32_t stage,
                                                  uint32_t* __restrict__ hits, uint32_t* __restrict__ hit_n,
                                                  const uint32_t* __restrict__ rdesc, uint32_t* __restrict__ region,
@@ -460,8 +462,8 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
         if (tid == 0) atomicOr(err, 1u);
         return;
     }
-    // a chunk is all wheel primes or all bitmap primes (the host splits at t_hl)
-    const bool wheel = j0 < j1 && primes[j0] < t_hl;
+    // a chunk is all wheel primes or all bitmap primes (the host splits at t_hl, t_hi and t_hi_end)
+    const bool wheel = j0 < j1 && (primes[j0] < t_hl || (primes[j0] >= t_hi && primes[j0] < t_hi_end));
     for (uint32_t r = 0; r < nruns; ++r) {
         // every condition below is uniform over the work-group
         const uint32_t seg0 = runs ? runs[r].seg0 : 0u, rn = runs ? runs[r].nseg : b.nseg;

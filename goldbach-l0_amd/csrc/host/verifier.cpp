@@ -177,7 +177,7 @@ bool same_shape(const std::vector<RunCost>& a, const std::vector<RunCost>& b)
 struct ChunkPlan {
     bool valid = false;
     std::vector<RunCost> key;
-    uint32_t nbb = 0, nwheel = 0;
+    uint32_t nbb = 0, nwheel = 0, nwheel_hi = 0, nwheel_hi2 = 0;
     double ncb = 0;
     std::string mode;
     uint32_t nchunks = 0, stage = 0, ngroups = 0;
@@ -404,6 +404,7 @@ void configure_stats(Engine::State& E, Range& r)
 // sizes the lists.
 struct PlanBounds {
     uint64_t q_hi = 0;
+    uint64_t qspan = 0;               // q span of a batch's run (0: a full batch of spans)
     uint64_t sq_lo = 0;               // floor(sqrt(smallest q_lo with large primes)); 0: no large primes
     uint64_t span = 0;
     uint32_t nwin = 0, nlarge = 0;
@@ -415,6 +416,7 @@ PlanBounds bounds_of(const Range& r)
 {
     PlanBounds b;
     b.q_hi = r.limit - 3;
+    b.qspan = r.limit - r.start + 2;
     b.sq_lo = r.nlarge ? std::max<uint64_t>(uint64_t(std::sqrt(double(r.q_lo))), 1) : 0;
     b.span = r.span;
     b.nwin = r.nwin;
@@ -470,8 +472,12 @@ private:
     uint32_t nwin_ = 0, nlarge_ = 0, ncoop_ = 0;   // the largest range's window / large prime counts
     uint32_t wbits_ = 0, nwin_max_ = 0, cw_ = 0, cap_ = 0, pcap_ = 0, swords_ = 0, kb_ = 1;
     uint64_t span_ = 0, evens_cap_ = UINT64_MAX;
-    // large primes: [0, nwheel_) walk the wheel, [nwheel_, ...) read the multiplier bitmap from t_hl_ up
+    // large primes: [0, nwheel_) and the band [nwheel_hi_, nwheel_hi2_) walk the wheel, the others read
+    // the multiplier bitmap (primes from t_hl_ on, except the band's t_hi_ .. below t_hi2_)
     uint32_t nwheel_ = 0, t_hl_ = UINT32_MAX;
+    uint32_t nwheel_hi_ = UINT32_MAX, nwheel_hi2_ = UINT32_MAX, t_hi_ = UINT32_MAX, t_hi2_ = UINT32_MAX;
+    double inv_hi_ = 0, inv_hi2_ = 0;   // sum of 1/p over the large primes below t_hi_, below t_hi2_
+    bool on_wheel(uint32_t j) const { return j < nwheel_ || (j >= nwheel_hi_ && j < nwheel_hi2_); }
     double inv_wheel_ = 0;             // sum of 1/p over the wheel primes
     double density_max_ = 0;
     std::shared_ptr<const MultBitmap> mb_;
@@ -504,6 +510,8 @@ void Pipeline::plan_bitmap(const PlanBounds& b)
     nlarge_ = b.nlarge;
     nwheel_ = nlarge_;
     t_hl_ = UINT32_MAX;
+    nwheel_hi_ = nwheel_hi2_ = nlarge_;
+    t_hi_ = t_hi2_ = UINT32_MAX;
     mb_.reset();
     if (!nlarge_ || !b.sq_lo) return;
     const uint64_t budget = mult_bitmap_budget();
@@ -528,9 +536,41 @@ void Pipeline::plan_bitmap(const PlanBounds& b)
     }
     nwheel_ = uint32_t(std::lower_bound(lp, lp + nlarge_, uint32_t(std::min<uint64_t>(t, UINT32_MAX))) - lp);
     if (nwheel_ == nlarge_) return;
+    // a band of primes walks the wheel instead of reading the bitmap: those
+    // whose walk lists fewer than GB_HL_WHEEL_HITS multiples per batch run
+    // (0.192 qspan / p: the multipliers coprime to 30030) and whose bitmap
+    // words are a random read each (consecutive primes' words further apart
+    // than a 64-byte sector: p below sqrt(q ln p / 1920))
+    {
+        const double hits = double(env::u32("GB_HL_WHEEL_HITS", 2, 0, 1u << 20));
+        const double qspan = double(b.qspan ? std::min<uint64_t>(b.qspan, uint64_t(E.kbatch) * b.span)
+                                            : uint64_t(E.kbatch) * b.span);
+        const double q = double(sq_lo) * double(sq_lo);
+        const double th = hits > 0 ? 0.1918 * qspan / hits : 1e300;
+        const double th2 = std::sqrt(q * std::log(std::sqrt(std::max(q, 4.0))) / 1920.0);
+        if (th < th2 && th < double(UINT32_MAX)) {
+            const auto at = [&](double x) {
+                return uint32_t(std::lower_bound(lp + nwheel_, lp + nlarge_,
+                                                 uint32_t(std::min(std::max(x, 1.0), double(UINT32_MAX)))) - lp);
+            };
+            nwheel_hi_ = at(th);
+            nwheel_hi2_ = at(th2);
+            if (nwheel_hi2_ <= nwheel_hi_) {
+                nwheel_hi_ = nwheel_hi2_ = nlarge_;
+            } else {
+                t_hi_ = lp[nwheel_hi_];
+                t_hi2_ = nwheel_hi2_ < nlarge_ ? lp[nwheel_hi2_] : UINT32_MAX;
+            }
+        }
+        if (nwheel_hi_ == nwheel_ && nwheel_hi2_ == nlarge_) {   // no prime left for the bitmap
+            nwheel_ = nwheel_hi_ = nwheel_hi2_ = nlarge_;
+            t_hi_ = t_hi2_ = UINT32_MAX;
+            return;
+        }
+    }
     t_hl_ = lp[nwheel_];
     const uint64_t m_lo = std::max<uint64_t>(t_hl_, sq_lo > 1 ? sq_lo - 1 : 1), m_hi = q_hi / t_hl_;
-    if (m_hi < m_lo) { nwheel_ = nlarge_; t_hl_ = UINT32_MAX; return; }
+    if (m_hi < m_lo) { nwheel_ = nwheel_hi_ = nwheel_hi2_ = nlarge_; t_hl_ = t_hi_ = t_hi2_ = UINT32_MAX; return; }
     mb_ = mult_bitmap(m_lo, m_hi, E.base, phase_trace_);
 }
 
@@ -540,7 +580,7 @@ void Pipeline::plan_bitmap(const PlanBounds& b)
 // (m = q / p >= max(q_lo / p, p), the bound that makes the rate largest).
 double Pipeline::hit_factor(uint32_t j, double q_lo) const
 {
-    if (j < nwheel_) return kWheelFrac;
+    if (on_wheel(j)) return kWheelFrac;
     const double p = E.base[E.win_hi + j];
     return 3.75 / std::log(std::max({q_lo / p, p, 3.0}));
 }
@@ -551,19 +591,25 @@ double Pipeline::hit_factor(uint32_t j, double q_lo) const
 double Pipeline::density_of(uint64_t q_lo, uint32_t nlarge) const
 {
     if (!nlarge) return 0.0;
-    const uint32_t nw = std::min(nwheel_, nlarge);
-    const double inv_nw = E.inv_at(nw);
-    double d = kWheelFrac * inv_nw;
-    const uint32_t n = nlarge - nw;
-    double ia = inv_nw;
-    uint32_t a = nw;
-    for (uint32_t k = 1; k <= 64 && a < nlarge; ++k) {   // band ends on the 1/p table's 256-prime grid
-        uint32_t b = k == 64 ? nlarge : (nw + uint32_t(uint64_t(n) * k / 64)) & ~255u;
-        if (b <= a) continue;
-        const double ib = b == nlarge ? E.inv_at(b) : E.inv_coarse[b >> 8];
-        d += (ib - ia) * std::max(hit_factor(a, double(q_lo)), hit_factor(b - 1, double(q_lo)));
-        a = b;
-        ia = ib;
+    double d = kWheelFrac * E.inv_at(nlarge);     // every prime at the wheel's rate ...
+    // ... except the bitmap's intervals, in 32 bands each at the band's largest rate
+    const uint32_t iv[2][2] = {{std::min(nwheel_, nlarge), std::min(nwheel_hi_, nlarge)},
+                               {std::min(nwheel_hi2_, nlarge), nlarge}};
+    for (const auto& r : iv) {
+        const uint32_t a0 = r[0], a1 = r[1];
+        if (a1 <= a0) continue;
+        double ia = E.inv_at(a0);
+        const double i1 = E.inv_at(a1);
+        d -= kWheelFrac * (i1 - ia);
+        uint32_t a = a0;
+        for (uint32_t k = 1; k <= 32 && a < a1; ++k) {   // band ends on the 1/p table's 256-prime grid
+            const uint32_t b = k == 32 ? a1 : (a0 + uint32_t(uint64_t(a1 - a0) * k / 32)) & ~255u;
+            if (b <= a) continue;
+            const double ib = b == a1 ? i1 : E.inv_coarse[b >> 8];
+            d += (ib - ia) * std::max(hit_factor(a, double(q_lo)), hit_factor(b - 1, double(q_lo)));
+            a = b;
+            ia = ib;
+        }
     }
     return d;
 }
@@ -598,6 +644,8 @@ void Pipeline::plan(const PlanBounds& pb)
     ncoop_ = 0;
     while (ncoop_ < nwin_ && E.base[E.win_lo + ncoop_] < wbits_ / 128) ++ncoop_;
     inv_wheel_ = E.inv_at(nwheel_);
+    inv_hi_ = E.inv_at(std::min(nwheel_hi_, nlarge_));
+    inv_hi2_ = E.inv_at(std::min(nwheel_hi2_, nlarge_));
     density_max_ = 0;
     for (const auto& [q_lo, np] : pb.q_lo_np) density_max_ = std::max(density_max_, density_of(q_lo, np));
     // a batch's q range stays below 2^33 (the harvest's 32-bit half-offsets):
@@ -643,8 +691,10 @@ void Pipeline::plan(const PlanBounds& pb)
     }
     if (phase_trace_)
         std::fprintf(stderr, "plan: %u segments per batch, window %u bits, %u window primes, %u large (%u on the wheel, "
-                     "bitmap from %u), %u harvest chunks, list cap %u, stage %u\n", kb_, wbits_, nwin_, nlarge_,
-                     nwheel_, t_hl_ == UINT32_MAX ? 0u : t_hl_, nchunks_, cap_, stage_);
+                     "bitmap from %u, the wheel again in [%u, %u)), %u harvest chunks, list cap %u, stage %u\n", kb_,
+                     wbits_, nwin_, nlarge_, nwheel_ + (std::min(nwheel_hi2_, nlarge_) - std::min(nwheel_hi_, nlarge_)),
+                     t_hl_ == UINT32_MAX ? 0u : t_hl_, t_hi_ == UINT32_MAX ? 0u : t_hi_,
+                     t_hi2_ == UINT32_MAX ? 0u : t_hi2_, nchunks_, cap_, stage_);
     verify_every_ = env::present("GB_BUCKET_VERIFY") ? env::u32("GB_BUCKET_VERIFY", 0, 0) : 0u;
     if (verify_every_ && nlarge_) {
         // one segment, every large prime on the wheel: its own lists and chunks
@@ -688,7 +738,9 @@ void Pipeline::plan(const PlanBounds& pb)
 void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, double ncb, int slot)
 {
     ChunkPlan& cp = E.cplan;
-    if (cp.valid && cp.nbb == nbb && close_rel(cp.ncb, ncb) && cp.nwheel == nwheel_ && cp.mode == hl_mode_ &&
+    if (cp.valid && cp.nbb == nbb && close_rel(cp.ncb, ncb) && cp.nwheel == nwheel_ && cp.nwheel_hi == nwheel_hi_ &&
+        cp.nwheel_hi2 == nwheel_hi2_ &&
+        cp.mode == hl_mode_ &&
         same_shape(cp.key, runs_in)) {   // the device holds this plan already
         nchunks_ = cp.nchunks;
         stage_ = cp.stage;
@@ -717,8 +769,11 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
             suf_qlo[k] = std::min(suf_qlo[k + 1], rs[k].q_lo);
         }
         // sum of 1/p up to j: coarse (every 256 primes) where chunk ends fall, exact elsewhere
-        const auto inv = [&](uint32_t j) { return (j & 255) == 0 ? E.inv_coarse[j >> 8] : j == nwheel_ ? inv_wheel_ : E.inv_at(j); };
-        const auto factor = [&](uint32_t a, double q_lo) { return a < nwheel_ ? 1.3 * kWheelFrac : hit_factor(a, q_lo); };
+        const auto inv = [&](uint32_t j) {
+            return (j & 255) == 0 ? E.inv_coarse[j >> 8]
+                 : j == nwheel_ ? inv_wheel_ : j == nwheel_hi_ ? inv_hi_ : j == nwheel_hi2_ ? inv_hi2_ : E.inv_at(j);
+        };
+        const auto factor = [&](uint32_t a, double q_lo) { return on_wheel(a) ? 1.3 * kWheelFrac : hit_factor(a, q_lo); };
         double total = 0;
         {
             size_t ai = 0;
@@ -729,6 +784,8 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
                     while (rs[ai].np <= a) ++ai;
                     uint32_t e = std::min(b, rs[ai].np);
                     if (a < nwheel_) e = std::min(e, nwheel_);
+                    else if (a < nwheel_hi_) e = std::min(e, nwheel_hi_);
+                    else if (a < nwheel_hi2_) e = std::min(e, nwheel_hi2_);
                     const double q = suf_qlo[ai];
                     total += kPrimeCost * (e - a) * double(nr - ai) +
                              suf_bits[ai] * std::max(factor(a, q), factor(e - 1, q)) * (inv(e) - inv(a));
@@ -749,10 +806,12 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
             while (rs[ai].np <= j) ++ai;   // runs [ai, nr) serve prime j
             uint32_t hi = std::min<uint32_t>(npmax, j + GBH_HL_CHUNK);
             if (j < nwheel_) hi = std::min(hi, nwheel_);
+            else if (j < nwheel_hi_) hi = std::min(hi, nwheel_hi_);
+            else if (j < nwheel_hi2_) hi = std::min(hi, nwheel_hi2_);
             hi = std::min(hi, rs[ai].np);
             const double nact = double(nr - ai), bits = suf_bits[ai], bmax = suf_max[ai], q = suf_qlo[ai];
             const double hf = hit_factor(j, q);
-            const double f = j < nwheel_ ? 1.3 * kWheelFrac : hf, fh = hf;
+            const double f = on_wheel(j) ? 1.3 * kWheelFrac : hf, fh = hf;
             const double ij = j == nwheel_ ? inv_wheel_ : E.inv_at(j);
             const auto ok = [&](uint32_t e) {   // chunk [j, e) within the targets (the rates barely change inside)
                 const double si = inv(e) - ij;
@@ -842,6 +901,8 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
     cp.nbb = nbb;
     cp.ncb = ncb;
     cp.nwheel = nwheel_;
+    cp.nwheel_hi = nwheel_hi_;
+    cp.nwheel_hi2 = nwheel_hi2_;
     cp.mode = hl_mode_;
     cp.nchunks = nchunks_;
     cp.stage = stage_;
@@ -987,9 +1048,14 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
             for (uint32_t s = run.seg0; s < run.seg0 + run.nseg; ++s)
                 for (int x = 0; x < 8; ++x) bits += hg[s].nc[x];
             const double hits = r.density * bits;
-            const uint32_t np = r.nlarge, nw = std::min(np, nwheel_);
-            const double s_bm = np > nwheel_ ? r.inv_large - inv_wheel_ : 0.0;
-            agg_.harvest_bytes += np ? 4.0 * np + 8.0 * ((np - nw) + lq * s_bm / 240) + 4.0 * hits : 0.0;
+            // the run's bitmap primes: [nwheel_, nwheel_hi_) and [nwheel_hi2_, np)
+            const uint32_t np = r.nlarge, a0 = std::min(np, nwheel_), a1 = std::min(np, nwheel_hi_),
+                           b0 = std::min(np, nwheel_hi2_);
+            const double inv_np = r.inv_large;
+            const double s_bm = (a1 > a0 ? (a1 == np ? inv_np : inv_hi_) - inv_wheel_ : 0.0) +
+                                (np > b0 ? inv_np - inv_hi2_ : 0.0);
+            const double nbm = double(a1 - a0) + double(np - b0);
+            agg_.harvest_bytes += np ? 4.0 * np + 8.0 * (nbm + lq * s_bm / 240) + 4.0 * hits : 0.0;
             bits_all += bits;
             hits_all += hits;
             win_prime_bytes += 4.0 * run.nseg * 8 * r.nwin;
@@ -1031,7 +1097,8 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         check(gbh_zero(c, 0, E.d_hit_n.p, size_t(nseg) * 8 * nwin * 4), "zero hit counts");
         if (direct) check(gbh_zero(c, 0, E.d_fills.p, size_t(nchunks_) * nseg * 8 * 4), "zero region fills");
         gbh_harvest_args h{b, E.d_large.as<uint32_t>(), E.d_chunks.as<uint32_t>(), E.d_runs.as<gbh_hl_run>(), nruns,
-                           run_max, nchunks_, cap_, t_hl_, direct ? 0u : stage_, mb_ ? mb_->bits : nullptr,
+                           run_max, nchunks_, cap_, t_hl_, t_hi_, t_hi2_, direct ? 0u : stage_,
+                           mb_ ? mb_->bits : nullptr,
                            mb_ ? mb_->m_base : 0, mb_ ? mb_->m_span : 0, E.d_hits.as<uint32_t>(),
                            E.d_hit_n.as<uint32_t>(), direct ? E.d_rdesc.as<uint32_t>() : nullptr,
                            direct ? E.d_region.as<uint32_t>() : nullptr, direct ? E.d_fills.as<uint32_t>() : nullptr,
@@ -1092,7 +1159,7 @@ void Pipeline::enqueue_verify(uint32_t s, const gbh_batch& b, gbh_segout* out)
     }
     check(gbh_zero(c, 0, E.d_vhit_n.p, size_t(8) * b.nwin * 4), "verify zero");
     const gbh_harvest_args h{vb, E.d_large.as<uint32_t>(), E.d_vchunks.as<uint32_t>(), nullptr, 0, 0, nvchunks_, cap_v_,
-                             UINT32_MAX, 0, nullptr, 0, 0, E.d_vhits.as<uint32_t>(), E.d_vhit_n.as<uint32_t>(),
+                             UINT32_MAX, UINT32_MAX, UINT32_MAX, 0, nullptr, 0, 0, E.d_vhits.as<uint32_t>(), E.d_vhit_n.as<uint32_t>(),
                              nullptr, nullptr, nullptr, &out[s].err};
     check(gbh_launch_harvest(c, 0, &h), "verify harvest");
     const gbh_window_args w{vb, E.d_win.as<uint32_t>(), nwin_, ncoop, E.d_i0.as<uint32_t>(),

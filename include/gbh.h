@@ -172,6 +172,8 @@ typedef struct {
     uint32_t nchunks;
     uint32_t cap;                  /* entries per window list */
     uint32_t t_hl;                 /* first prime served by the bitmap */
+    uint32_t t_hi, t_hi_end;       /* primes in [t_hi, t_hi_end) walk the wheel too (UINT32_MAX: none);
+                                      chunks never straddle t_hl, t_hi or t_hi_end */
     uint32_t stage;                /* LDS entries a work-group may stage (0: always two walks), <= GBH_HL_STAGE */
     const uint64_t* mbits;         /* multiplier bitmap (gbh_launch_mtrans layout) */
     uint64_t m_base;               /* a multiple of 240 */

@@ -86,6 +86,9 @@ WINDOWS = [
     ("w4e18", 2_000_000, {"GB_HL_MODE": "counted"}),
     # one stream: the window stage's sieve phase after the harvest
     ("w4e18", 2_000_000, {"GB_OVERLAP": "0"}),
+    # no band of wheel primes among the bitmap primes, and a wide one
+    ("w4e18", 2_000_000, {"GB_HL_WHEEL_HITS": "0"}),
+    ("w2p63s", 400_000, {"GB_HL_WHEEL_HITS": "64"}),
     ("w44e18", 1_000_000, {"GB_MT_GB": "8"}),
     # every segment's map rebuilt with all large primes on the wheel walk and
     # compared word for word (the reference's GB_BUCKET_VERIFY self-check)
