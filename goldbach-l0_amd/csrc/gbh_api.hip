@@ -16,7 +16,7 @@
 
 struct gbh_ctx {
     int device = 0;
-    hipStream_t stream[2] = {nullptr, nullptr};
+    hipStream_t stream[GBH_STREAMS] = {};
     gbh_device_info_t info{};
 };
 struct gbh_event {
@@ -45,7 +45,7 @@ int hip_fail(hipError_t e, const char* what)
         if (e_ != hipSuccess) return hip_fail(e_, #call);          \
     } while (0)
 
-hipStream_t S(gbh_ctx* c, int s) { return c->stream[s & 1]; }
+hipStream_t S(gbh_ctx* c, int s) { return c->stream[unsigned(s) % GBH_STREAMS]; }
 
 // Candidate tables of the checker (static: they depend on the even class
 // only). Per even class r = 2 cr: the odd primes p <= 9781 with r - p a unit

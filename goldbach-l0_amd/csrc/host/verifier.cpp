@@ -180,6 +180,8 @@ struct ChunkPlan {
     uint32_t nbb = 0, nwheel = 0, nwheel_hi = 0, nwheel_hi2 = 0;
     double ncb = 0;
     std::string mode;
+    std::vector<uint32_t> chunks, rdesc, groups;   // the tables uploaded to a slot
+    uint64_t region_entries = 0;
     uint32_t nchunks = 0, stage = 0, ngroups = 0;
     bool direct = false;
 };
@@ -207,13 +209,19 @@ struct Engine::State {
     std::optional<Ctx> ctx;
     gbh_ctx* c = nullptr;
     int cus = 256;
-    Buf d_win, d_large, d_chunks, d_i0, d_map, d_coff, d_coffa, d_pending, d_scratch, d_hits, d_hit_n;
-    Buf d_vmap, d_vhits, d_vhit_n, d_vchunks, h_vchunks;   // the GB_BUCKET_VERIFY rebuild
-    Buf d_geoms, d_cs, d_runs, d_rdesc, d_groups, d_region, d_fills;
+    Buf d_win, d_large;
+    // the device buffers of one batch in flight: two sets, so a batch's
+    // sieve and harvest run while the batch before it is checked and counted
+    struct Slot {
+        Buf map, i0, coff, coffa, pending, scratch, hits, hit_n, geoms, cs, runs, chunks, rdesc, groups, region, fills;
+        Buf vmap, vhits, vhit_n, vchunks;   // the GB_BUCKET_VERIFY rebuild
+    } sl[2];
+    Buf h_vchunks;
     Buf d_out[2], h_out[2], h_geoms[2], h_cs[2], h_chunks[2], h_runs[2], h_rdesc[2];
     Event ev[2][EV_N];
     Candidates cand;
-    ChunkPlan cplan;
+    ChunkPlan cplan;                     // the last harvest chunk plan made (host side)
+    uint64_t cplan_version = 0, slot_plan[2] = {~0ull, ~0ull};   // the plan each slot's device tables hold
     std::string tuned;                   // knob defaults taken from GB_TUNE_FILE
 
     State(uint64_t max_limit_, uint64_t max_span_, const char* prime_cache)
@@ -269,10 +277,12 @@ struct Engine::State {
             h_cs[s].ensure(c, sizeof(gbh_countseg) * kbatch, 1);
             for (auto& e : ev[s]) e.create(c);
         }
-        d_geoms.ensure(c, sizeof(gbh_geom) * kbatch, 0);
-        d_cs.ensure(c, sizeof(gbh_countseg) * kbatch, 0);
-        d_coff.ensure(c, size_t(kbatch) * 15 * GBH_NCAND * 4, 0);
-        d_coffa.ensure(c, size_t(kbatch) * 15 * GBH_NCAND_PAD * 4, 0);
+        for (auto& B : sl) {
+            B.geoms.ensure(c, sizeof(gbh_geom) * kbatch, 0);
+            B.cs.ensure(c, sizeof(gbh_countseg) * kbatch, 0);
+            B.coff.ensure(c, size_t(kbatch) * 15 * GBH_NCAND * 4, 0);
+            B.coffa.ensure(c, size_t(kbatch) * 15 * GBH_NCAND_PAD * 4, 0);
+        }
         check(gbh_stream_sync(c, 0), "upload sync");
         upload_s = since(tu);
         init_s = since(t0);
@@ -453,10 +463,11 @@ public:
 private:
     double hit_factor(uint32_t j, double q_lo) const;
     void plan_chunks(const std::vector<RunCost>& runs, uint32_t nbb, double ncb, int slot);
+    void upload_chunks(int slot);
     uint32_t run_cap(const Range& r) const;
     void form(std::vector<SegJob>& bt);
     void enqueue(std::vector<SegJob>& batch, int slot, bool check_and_count);
-    void enqueue_verify(uint32_t s, const gbh_batch& b, gbh_segout* out);
+    void enqueue_verify(uint32_t s, const gbh_batch& b, gbh_segout* out, int slot);
     void collect(std::vector<SegJob>& batch, int slot);
     void collect_segment(const SegJob& j, const gbh_segout& o, const double share[4]);
     void finish_range(Range& r);
@@ -665,17 +676,19 @@ void Pipeline::plan(const PlanBounds& pb)
     swords_ = uint32_t(2 * (nc_max / 64 + 4));
     gbh_ctx* c = E.c;
     const size_t K = kb_;
-    E.d_map.ensure(c, K * 8 * cw_ * 8, 0);
-    E.d_i0.ensure(c, K * 8 * size_t(std::max(nwin_, 1u)) * 4, 0);
-    E.d_pending.ensure(c, K * size_t(pcap_) * 4, 0);
-    E.d_scratch.ensure(c, K * size_t(swords_) * 4, 0);
-    E.d_hit_n.ensure(c, K * 8 * size_t(nwin_max_) * 4, 0);
-    if (nlarge_) E.d_hits.ensure(c, K * 8 * size_t(nwin_max_) * cap_ * 4, 0);
-    E.d_geoms.ensure(c, sizeof(gbh_geom) * K, 0);
-    E.d_cs.ensure(c, sizeof(gbh_countseg) * K, 0);
-    E.d_coff.ensure(c, K * 15 * GBH_NCAND * 4, 0);
-    E.d_coffa.ensure(c, K * 15 * GBH_NCAND_PAD * 4, 0);
-    E.d_runs.ensure(c, sizeof(gbh_hl_run) * K, 0);
+    for (auto& B : E.sl) {
+        B.map.ensure(c, K * 8 * cw_ * 8, 0);
+        B.i0.ensure(c, K * 8 * size_t(std::max(nwin_, 1u)) * 4, 0);
+        B.pending.ensure(c, K * size_t(pcap_) * 4, 0);
+        B.scratch.ensure(c, K * size_t(swords_) * 4, 0);
+        B.hit_n.ensure(c, K * 8 * size_t(nwin_max_) * 4, 0);
+        if (nlarge_) B.hits.ensure(c, K * 8 * size_t(nwin_max_) * cap_ * 4, 0);
+        B.geoms.ensure(c, sizeof(gbh_geom) * K, 0);
+        B.cs.ensure(c, sizeof(gbh_countseg) * K, 0);
+        B.coff.ensure(c, K * 15 * GBH_NCAND * 4, 0);
+        B.coffa.ensure(c, K * 15 * GBH_NCAND_PAD * 4, 0);
+        B.runs.ensure(c, sizeof(gbh_hl_run) * K, 0);
+    }
     for (int s = 0; s < 2; ++s) {
         E.d_out[s].ensure(c, sizeof(gbh_segout) * K, 0);
         E.h_out[s].ensure(c, sizeof(gbh_segout) * K, 1);
@@ -701,9 +714,11 @@ void Pipeline::plan(const PlanBounds& pb)
         const double inv_large = E.inv_at(nlarge_);
         const double lam_v = wbits_ * kWheelFrac * inv_large;
         cap_v_ = uint32_t(lam_v * 1.15 + 8 * std::sqrt(lam_v) + 1024);
-        E.d_vmap.ensure(c, size_t(8) * cw_ * 8, 0);
-        E.d_vhits.ensure(c, size_t(8) * nwin_max_ * cap_v_ * 4, 0);
-        E.d_vhit_n.ensure(c, size_t(8) * nwin_max_ * 4, 0);
+        for (auto& B : E.sl) {
+            B.vmap.ensure(c, size_t(8) * cw_ * 8, 0);
+            B.vhits.ensure(c, size_t(8) * nwin_max_ * cap_v_ * 4, 0);
+            B.vhit_n.ensure(c, size_t(8) * nwin_max_ * 4, 0);
+        }
         const double unit = double(span_) * (8.0 / 30) * 1.3 * kWheelFrac;
         const double target = std::max((4.0 * nlarge_ + unit * inv_large) / (8.0 * E.cus), 2e4);
         std::vector<uint32_t> vc{0};
@@ -722,9 +737,11 @@ void Pipeline::plan(const PlanBounds& pb)
         }
         nvchunks_ = uint32_t(vc.size() - 1);
         E.h_vchunks.ensure(c, vc.size() * 4, 1);
-        E.d_vchunks.ensure(c, vc.size() * 4, 0);
         std::memcpy(E.h_vchunks.p, vc.data(), vc.size() * 4);
-        check(gbh_copy_h2d(c, 0, E.d_vchunks.p, E.h_vchunks.p, vc.size() * 4), "upload verify chunks");
+        for (auto& B : E.sl) {
+            B.vchunks.ensure(c, vc.size() * 4, 0);
+            check(gbh_copy_h2d(c, 0, B.vchunks.p, E.h_vchunks.p, vc.size() * 4), "upload verify chunks");
+        }
     }
     check(gbh_stream_sync(c, 0), "plan sync");
 }
@@ -739,13 +756,12 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
 {
     ChunkPlan& cp = E.cplan;
     if (cp.valid && cp.nbb == nbb && close_rel(cp.ncb, ncb) && cp.nwheel == nwheel_ && cp.nwheel_hi == nwheel_hi_ &&
-        cp.nwheel_hi2 == nwheel_hi2_ &&
-        cp.mode == hl_mode_ &&
-        same_shape(cp.key, runs_in)) {   // the device holds this plan already
+        cp.nwheel_hi2 == nwheel_hi2_ && cp.mode == hl_mode_ && same_shape(cp.key, runs_in)) {
         nchunks_ = cp.nchunks;
         stage_ = cp.stage;
         direct_ = cp.direct;
         ngroups_ = cp.ngroups;
+        upload_chunks(slot);   // unless the slot's tables hold it already
         return;
     }
     cp.valid = false;
@@ -844,19 +860,18 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
         }
     }
     nchunks_ = uint32_t(chunks_.size() - 1);
-    gbh_ctx* c = E.c;
-    E.h_chunks[slot].ensure(c, chunks_.size() * 4, 1);
-    E.d_chunks.ensure(c, chunks_.size() * 4, 0);
-    std::memcpy(E.h_chunks[slot].p, chunks_.data(), chunks_.size() * 4);
-    check(gbh_copy_h2d(c, 0, E.d_chunks.p, E.h_chunks[slot].p, chunks_.size() * 4), "upload chunks");
     // the direct harvest (batches whose hits exceed the LDS stages): work-group
     // k's region holds r_k entries per (segment, class) list, r_k well above
     // the largest list's expected hits lambda_k (lambda + 8 sqrt(lambda) + 64;
     // an overflow re-runs the batch with counted lists)
     direct_ = nchunks_ && !stage_ && hl_mode_ != "counted";
+    std::vector<uint32_t>& rd = cp.rdesc;
+    std::vector<uint32_t>& groups = cp.groups;   // the partition's chunk groups: their regions fit its LDS
+    rd.clear();
+    groups.clear();
     if (direct_) {
-        std::vector<uint32_t> rd(2 * size_t(nchunks_));
-        std::vector<uint32_t> groups{0};   // the partition's chunk groups: their regions fit its LDS
+        rd.assign(2 * size_t(nchunks_), 0u);
+        groups.assign(1, 0u);
         uint64_t base = 0, gsum = 0, rmax = 0;
         for (uint32_t k = 0; k < nchunks_; ++k) {
             const double lam = ncb * hpb[k];
@@ -885,18 +900,13 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
             if (phase_trace_)
                 std::fprintf(stderr, "plan: direct harvest: %llu region entries, at most %llu per list, %u partition "
                              "groups\n", (unsigned long long)base, (unsigned long long)rmax, ngroups_);
-            E.h_rdesc[slot].ensure(c, rd.size() * 4 + groups.size() * 4, 1);
-            E.d_rdesc.ensure(c, rd.size() * 4, 0);
-            E.d_groups.ensure(c, groups.size() * 4, 0);
-            E.d_region.ensure(c, base * 4, 0);
-            E.d_fills.ensure(c, size_t(nchunks_) * nbb * 4, 0);
-            uint32_t* hr = E.h_rdesc[slot].as<uint32_t>();
-            std::memcpy(hr, rd.data(), rd.size() * 4);
-            std::memcpy(hr + rd.size(), groups.data(), groups.size() * 4);
-            check(gbh_copy_h2d(c, 0, E.d_rdesc.p, hr, rd.size() * 4), "upload regions");
-            check(gbh_copy_h2d(c, 0, E.d_groups.p, hr + rd.size(), groups.size() * 4), "upload region groups");
+        } else {
+            rd.clear();
+            groups.clear();
         }
     }
+    cp.chunks = chunks_;
+    cp.region_entries = region_entries_;
     cp.key = runs_in;
     cp.nbb = nbb;
     cp.ncb = ncb;
@@ -909,6 +919,35 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
     cp.direct = direct_;
     cp.ngroups = ngroups_;
     cp.valid = true;
+    ++E.cplan_version;
+    upload_chunks(slot);
+}
+
+// The current chunk plan into the device tables of @p slot (on the slot's
+// stream, after the slot's batch before, which the host has collected).
+void Pipeline::upload_chunks(int slot)
+{
+    if (E.slot_plan[slot] == E.cplan_version) return;
+    const ChunkPlan& cp = E.cplan;
+    gbh_ctx* c = E.c;
+    auto& B = E.sl[slot];
+    const size_t nc = cp.chunks.size(), nr = cp.rdesc.size(), ng = cp.groups.size();
+    E.h_chunks[slot].ensure(c, (nc + nr + ng) * 4, 1);
+    uint32_t* h = E.h_chunks[slot].as<uint32_t>();
+    std::memcpy(h, cp.chunks.data(), nc * 4);
+    B.chunks.ensure(c, nc * 4, 0);
+    check(gbh_copy_h2d(c, slot, B.chunks.p, h, nc * 4), "upload chunks");
+    if (cp.direct) {
+        std::memcpy(h + nc, cp.rdesc.data(), nr * 4);
+        std::memcpy(h + nc + nr, cp.groups.data(), ng * 4);
+        B.rdesc.ensure(c, nr * 4, 0);
+        B.groups.ensure(c, ng * 4, 0);
+        B.region.ensure(c, cp.region_entries * 4, 0);
+        B.fills.ensure(c, size_t(cp.nchunks) * cp.nbb * 4, 0);
+        check(gbh_copy_h2d(c, slot, B.rdesc.p, h + nc, nr * 4), "upload regions");
+        check(gbh_copy_h2d(c, slot, B.groups.p, h + nc + nr, ng * 4), "upload region groups");
+    }
+    E.slot_plan[slot] = E.cplan_version;
 }
 
 // Segments of one range a batch may hold: the run's q range below 2^33 and
@@ -980,6 +1019,10 @@ bool Pipeline::step(std::vector<Range*>* done)
 void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_count)
 {
     gbh_ctx* c = E.c;
+    auto& B = E.sl[slot];
+    // streams: the slot's main stream (uploads, harvest, hit lists, checker,
+    // counter) and its side stream (set-up and the window sieve phase)
+    const int ms = slot;
     const uint32_t nseg = uint32_t(batch.size());
     gbh_geom* hg = E.h_geoms[slot].as<gbh_geom>();
     gbh_countseg* hc = E.h_cs[slot].as<gbh_countseg>();
@@ -1032,10 +1075,10 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         ++hr[nruns - 1].nseg;
         run_max = std::max(run_max, hr[nruns - 1].nseg);
     }
-    gbh_batch b{E.d_geoms.as<gbh_geom>(), nseg, cw_, wbits, nwin, E.d_map.as<uint64_t>()};
+    gbh_batch b{B.geoms.as<gbh_geom>(), nseg, cw_, wbits, nwin, B.map.as<uint64_t>()};
     gbh_segout* out = E.d_out[slot].as<gbh_segout>();
     auto& ev = E.ev[slot];
-    check(gbh_event_record(c, 0, ev[EV_BEGIN].e), "event");
+    check(gbh_event_record(c, ms, ev[EV_BEGIN].e), "event");
     harvested_[slot] = nlarge_ != 0;
     std::vector<RunCost> costs;
     {   // algorithmic bytes of this batch's harvest and window launches, per run
@@ -1068,106 +1111,110 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         for (uint32_t s = 0; s < nseg; ++s)
             for (int x = 0; x < 8; ++x) ncb = std::max<double>(ncb, hg[s].nc[x]);
         plan_chunks(costs, nseg * 8, ncb, slot);
+    } else if (nlarge_) {
+        upload_chunks(slot);
     }
-    check(gbh_copy_h2d(c, 0, E.d_geoms.p, hg, sizeof(gbh_geom) * nseg), "upload geometry");
-    check(gbh_copy_h2d(c, 0, E.d_cs.p, hc, sizeof(gbh_countseg) * nseg), "upload count parameters");
-    check(gbh_copy_h2d(c, 0, E.d_runs.p, hr, sizeof(gbh_hl_run) * nruns), "upload runs");
-    check(gbh_reset_outputs(c, 0, out, nseg), "reset outputs");
+    check(gbh_copy_h2d(c, ms, B.geoms.p, hg, sizeof(gbh_geom) * nseg), "upload geometry");
+    check(gbh_copy_h2d(c, ms, B.cs.p, hc, sizeof(gbh_countseg) * nseg), "upload count parameters");
+    check(gbh_copy_h2d(c, ms, B.runs.p, hr, sizeof(gbh_hl_run) * nruns), "upload runs");
+    check(gbh_reset_outputs(c, ms, out, nseg), "reset outputs");
     uint32_t ncoop = 0;
     while (ncoop < nwin_ && E.base[E.win_lo + ncoop] < wbits / 128) ++ncoop;
     // the window stage's sieve phase (masks, window primes) on stream 1
     // beside the harvest; it starts once this batch's tables are uploaded
     // (and so every kernel of the batch before has finished on stream 0)
-    const int ss = overlap_ ? 1 : 0;
-    check(gbh_event_record(c, 0, ev[EV_GEOM].e), "event");
-    if (ss) check(gbh_stream_wait(c, ss, ev[EV_GEOM].e), "stream wait");
+    const int ss = overlap_ ? 2 + slot : ms;
+    check(gbh_event_record(c, ms, ev[EV_GEOM].e), "event");
+    if (ss != ms) check(gbh_stream_wait(c, ss, ev[EV_GEOM].e), "stream wait");
     if (nwin_) {
-        gbh_setup_args a{b, E.d_win.as<uint32_t>(), nwin_, 0, E.d_i0.as<uint32_t>()};
+        gbh_setup_args a{b, E.d_win.as<uint32_t>(), nwin_, 0, B.i0.as<uint32_t>()};
         check(gbh_launch_setup(c, ss, &a), "setup");
     }
     check(gbh_event_record(c, ss, ev[EV_S0].e), "event");
     {
-        const gbh_window_args w{b, E.d_win.as<uint32_t>(), nwin_, ncoop, E.d_i0.as<uint32_t>(), nullptr, nullptr, 0, 1};
+        const gbh_window_args w{b, E.d_win.as<uint32_t>(), nwin_, ncoop, B.i0.as<uint32_t>(), nullptr, nullptr, 0, 1};
         check(gbh_launch_window(c, ss, &w), "window sieve");
     }
     check(gbh_event_record(c, ss, ev[EV_S1].e), "event");
     const bool direct = nlarge_ && direct_ && !force_count_;
     direct_used_[slot] = direct;
     if (nlarge_) {
-        check(gbh_zero(c, 0, E.d_hit_n.p, size_t(nseg) * 8 * nwin * 4), "zero hit counts");
-        if (direct) check(gbh_zero(c, 0, E.d_fills.p, size_t(nchunks_) * nseg * 8 * 4), "zero region fills");
-        gbh_harvest_args h{b, E.d_large.as<uint32_t>(), E.d_chunks.as<uint32_t>(), E.d_runs.as<gbh_hl_run>(), nruns,
+        check(gbh_zero(c, ms, B.hit_n.p, size_t(nseg) * 8 * nwin * 4), "zero hit counts");
+        if (direct) check(gbh_zero(c, ms, B.fills.p, size_t(nchunks_) * nseg * 8 * 4), "zero region fills");
+        gbh_harvest_args h{b, E.d_large.as<uint32_t>(), B.chunks.as<uint32_t>(), B.runs.as<gbh_hl_run>(), nruns,
                            run_max, nchunks_, cap_, t_hl_, t_hi_, t_hi2_, direct ? 0u : stage_,
                            mb_ ? mb_->bits : nullptr,
-                           mb_ ? mb_->m_base : 0, mb_ ? mb_->m_span : 0, E.d_hits.as<uint32_t>(),
-                           E.d_hit_n.as<uint32_t>(), direct ? E.d_rdesc.as<uint32_t>() : nullptr,
-                           direct ? E.d_region.as<uint32_t>() : nullptr, direct ? E.d_fills.as<uint32_t>() : nullptr,
+                           mb_ ? mb_->m_base : 0, mb_ ? mb_->m_span : 0, B.hits.as<uint32_t>(),
+                           B.hit_n.as<uint32_t>(), direct ? B.rdesc.as<uint32_t>() : nullptr,
+                           direct ? B.region.as<uint32_t>() : nullptr, direct ? B.fills.as<uint32_t>() : nullptr,
                            &out[0].err};
-        check(gbh_event_record(c, 0, ev[EV_HL0].e), "event");
-        check(gbh_launch_harvest(c, 0, &h), "harvest");
-        check(gbh_event_record(c, 0, ev[EV_HL1].e), "event");
+        check(gbh_event_record(c, ms, ev[EV_HL0].e), "event");
+        check(gbh_launch_harvest(c, ms, &h), "harvest");
+        check(gbh_event_record(c, ms, ev[EV_HL1].e), "event");
         if (direct) {
-            const gbh_partition_args pa{b, E.d_rdesc.as<uint32_t>(), E.d_region.as<uint32_t>(), E.d_fills.as<uint32_t>(),
-                                        E.d_groups.as<uint32_t>(), ngroups_, cap_, E.d_hits.as<uint32_t>(),
-                                        E.d_hit_n.as<uint32_t>(), &out[0].err};
-            check(gbh_launch_partition(c, 0, &pa), "partition");
+            const gbh_partition_args pa{b, B.rdesc.as<uint32_t>(), B.region.as<uint32_t>(), B.fills.as<uint32_t>(),
+                                        B.groups.as<uint32_t>(), ngroups_, cap_, B.hits.as<uint32_t>(),
+                                        B.hit_n.as<uint32_t>(), &out[0].err};
+            check(gbh_launch_partition(c, ms, &pa), "partition");
         }
     }
-    check(gbh_event_record(c, 0, ev[EV_SETUP].e), "event");
-    if (ss) check(gbh_stream_wait(c, 0, ev[EV_S1].e), "stream wait");
+    check(gbh_event_record(c, ms, ev[EV_SETUP].e), "event");
+    if (ss != ms) check(gbh_stream_wait(c, ms, ev[EV_S1].e), "stream wait");
     if (nlarge_) {   // the hit-list phase: every window's list applied to the sieved map
-        const gbh_window_args w{b, E.d_win.as<uint32_t>(), nwin_, ncoop, E.d_i0.as<uint32_t>(),
-                                E.d_hits.as<uint32_t>(), E.d_hit_n.as<uint32_t>(), cap_, 2};
-        check(gbh_launch_window(c, 0, &w), "window hit lists");
+        const gbh_window_args w{b, E.d_win.as<uint32_t>(), nwin_, ncoop, B.i0.as<uint32_t>(),
+                                B.hits.as<uint32_t>(), B.hit_n.as<uint32_t>(), cap_, 2};
+        check(gbh_launch_window(c, ms, &w), "window hit lists");
     }
-    check(gbh_event_record(c, 0, ev[EV_WINDOW].e), "event");
+    check(gbh_event_record(c, ms, ev[EV_WINDOW].e), "event");
     if (verify_every_ && nlarge_ && check_and_count)
         for (uint32_t s = 0; s < nseg; ++s)
-            if (batch[s].index % verify_every_ == 0) enqueue_verify(s, b, out);
+            if (batch[s].index % verify_every_ == 0) enqueue_verify(s, b, out, slot);
     if (check_and_count) {
         uint32_t kinds = 0;
         for (const SegJob& j : batch) kinds |= j.sampled ? 2u : 1u;
-        gbh_check_args ca{b, E.ka, pcap_, E.d_coff.as<uint32_t>(), E.d_coffa.as<uint32_t>(),
-                          E.d_pending.as<uint32_t>(), out, kinds, 0};
-        check(gbh_launch_check_a(c, 0, &ca), "check phase A");
-        gbh_checkb_args cb{b, E.ka, CHECK_TAIL, E.d_coff.as<uint32_t>(), E.d_pending.as<uint32_t>(), pcap_, 0, out};
-        check(gbh_launch_check_b(c, 0, &cb), "check phase B");
+        gbh_check_args ca{b, E.ka, pcap_, B.coff.as<uint32_t>(), B.coffa.as<uint32_t>(),
+                          B.pending.as<uint32_t>(), out, kinds, 0};
+        check(gbh_launch_check_a(c, ms, &ca), "check phase A");
+        gbh_checkb_args cb{b, E.ka, CHECK_TAIL, B.coff.as<uint32_t>(), B.pending.as<uint32_t>(), pcap_, 0, out};
+        check(gbh_launch_check_b(c, ms, &cb), "check phase B");
     }
-    check(gbh_event_record(c, 0, ev[EV_CHECK].e), "event");
+    check(gbh_event_record(c, ms, ev[EV_CHECK].e), "event");
     if (check_and_count && nwords) {
-        gbh_count_args ct{b, E.d_cs.as<gbh_countseg>(), nwords, swords_, E.d_scratch.as<uint32_t>(), out};
-        check(gbh_launch_count(c, 0, &ct), "count");
+        gbh_count_args ct{b, B.cs.as<gbh_countseg>(), nwords, swords_, B.scratch.as<uint32_t>(), out};
+        check(gbh_launch_count(c, ms, &ct), "count");
     }
-    check(gbh_event_record(c, 0, ev[EV_COUNT].e), "event");
-    check(gbh_copy_d2h(c, 0, E.h_out[slot].p, out, sizeof(gbh_segout) * nseg), "copy outputs");
-    check(gbh_event_record(c, 0, ev[EV_END].e), "event");
+    check(gbh_event_record(c, ms, ev[EV_COUNT].e), "event");
+    check(gbh_copy_d2h(c, ms, E.h_out[slot].p, out, sizeof(gbh_segout) * nseg), "copy outputs");
+    check(gbh_event_record(c, ms, ev[EV_END].e), "event");
 }
 
 // The GB_BUCKET_VERIFY rebuild of segment s (the reference's
 // enqueue_bucket_verify, src/verifier.cpp:2059-2085): window primes as in
 // production, every large prime by the wheel walk (no bitmap), then a word
 // compare with the production map into out[s].vdiff.
-void Pipeline::enqueue_verify(uint32_t s, const gbh_batch& b, gbh_segout* out)
+void Pipeline::enqueue_verify(uint32_t s, const gbh_batch& b, gbh_segout* out, int slot)
 {
     gbh_ctx* c = E.c;
-    const gbh_batch vb{E.d_geoms.as<gbh_geom>() + s, 1, cw_, b.wbits, b.nwin, E.d_vmap.as<uint64_t>()};
+    auto& B = E.sl[slot];
+    const int ms = slot;
+    const gbh_batch vb{B.geoms.as<gbh_geom>() + s, 1, cw_, b.wbits, b.nwin, B.vmap.as<uint64_t>()};
     uint32_t ncoop = 0;
     while (ncoop < nwin_ && E.base[E.win_lo + ncoop] < b.wbits / 128) ++ncoop;
     if (nwin_) {
-        const gbh_setup_args a{vb, E.d_win.as<uint32_t>(), nwin_, 0, E.d_i0.as<uint32_t>()};
-        check(gbh_launch_setup(c, 0, &a), "verify setup");
+        const gbh_setup_args a{vb, E.d_win.as<uint32_t>(), nwin_, 0, B.i0.as<uint32_t>()};
+        check(gbh_launch_setup(c, ms, &a), "verify setup");
     }
-    check(gbh_zero(c, 0, E.d_vhit_n.p, size_t(8) * b.nwin * 4), "verify zero");
-    const gbh_harvest_args h{vb, E.d_large.as<uint32_t>(), E.d_vchunks.as<uint32_t>(), nullptr, 0, 0, nvchunks_, cap_v_,
-                             UINT32_MAX, UINT32_MAX, UINT32_MAX, 0, nullptr, 0, 0, E.d_vhits.as<uint32_t>(), E.d_vhit_n.as<uint32_t>(),
+    check(gbh_zero(c, ms, B.vhit_n.p, size_t(8) * b.nwin * 4), "verify zero");
+    const gbh_harvest_args h{vb, E.d_large.as<uint32_t>(), B.vchunks.as<uint32_t>(), nullptr, 0, 0, nvchunks_, cap_v_,
+                             UINT32_MAX, UINT32_MAX, UINT32_MAX, 0, nullptr, 0, 0, B.vhits.as<uint32_t>(), B.vhit_n.as<uint32_t>(),
                              nullptr, nullptr, nullptr, &out[s].err};
-    check(gbh_launch_harvest(c, 0, &h), "verify harvest");
-    const gbh_window_args w{vb, E.d_win.as<uint32_t>(), nwin_, ncoop, E.d_i0.as<uint32_t>(),
-                            E.d_vhits.as<uint32_t>(), E.d_vhit_n.as<uint32_t>(), cap_v_, 0};
-    check(gbh_launch_window(c, 0, &w), "verify window");
-    const gbh_mapcmp_args m{E.d_map.as<uint64_t>() + size_t(s) * 8 * cw_, E.d_vmap.as<uint64_t>(),
-                            E.d_geoms.as<gbh_geom>() + s, cw_, 0, &out[s].vdiff};
-    check(gbh_launch_mapcmp(c, 0, &m), "verify compare");
+    check(gbh_launch_harvest(c, ms, &h), "verify harvest");
+    const gbh_window_args w{vb, E.d_win.as<uint32_t>(), nwin_, ncoop, B.i0.as<uint32_t>(),
+                            B.vhits.as<uint32_t>(), B.vhit_n.as<uint32_t>(), cap_v_, 0};
+    check(gbh_launch_window(c, ms, &w), "verify window");
+    const gbh_mapcmp_args m{B.map.as<uint64_t>() + size_t(s) * 8 * cw_, B.vmap.as<uint64_t>(),
+                            B.geoms.as<gbh_geom>() + s, cw_, 0, &out[s].vdiff};
+    check(gbh_launch_mapcmp(c, ms, &m), "verify compare");
     ++verified_;
 }
 
@@ -1367,7 +1414,7 @@ uint32_t Pipeline::map_of(Range& r, uint64_t seg_start, std::vector<uint64_t>& w
     check(gbh_event_sync(E.c, E.ev[0][EV_END].e), "sync");
     if (E.h_out[0].as<gbh_segout>()[0].err & 13u) throw std::runtime_error("hit-list device error");
     words.assign(size_t(8) * cw_, 0);
-    check(gbh_copy_d2h(E.c, 0, words.data(), E.d_map.p, words.size() * 8), "copy map");
+    check(gbh_copy_d2h(E.c, 0, words.data(), E.sl[0].map.p, words.size() * 8), "copy map");
     check(gbh_stream_sync(E.c, 0), "sync");
     for (int c = 0; c < 8; ++c) { Q[c] = bt[0].cg.Q[c]; nc[c] = uint32_t(bt[0].cg.n[c]); }
     return cw_;

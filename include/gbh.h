@@ -22,13 +22,17 @@
  *
  * Threading: one driver thread per gbh_ctx (as the reference's single
  * immediate command list). Streams are in-order HIP streams at the default
- * priority.
+ * priority; a context holds GBH_STREAMS of them, named 0 .. GBH_STREAMS - 1
+ * in every `stream` argument (the C++ driver runs a batch's main chain on
+ * stream k and its window sieve on stream 2 + k, k the batch's slot).
  */
 #ifndef GBH_H
 #define GBH_H
 
 #include <stddef.h>
 #include <stdint.h>
+
+#define GBH_STREAMS 4
 
 #ifdef __cplusplus
 extern "C" {
