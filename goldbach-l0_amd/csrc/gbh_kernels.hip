@@ -940,13 +940,17 @@ __global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_batch b, const uin
                     u0 &= ~x0;
                     u1 &= ~x1;
                 }
+                // the eight wave totals to lanes 0..7, then two LDS adds
+                // with per-lane addresses (not sixteen single-lane adds)
+                uint32_t tv = 0;
 #pragma unroll
                 for (uint32_t k = 0; k < 8; ++k) {
-                    const uint32_t t = wave_total63(hc[k]);
-                    if (lane == 63 && t) {
-                        atomicAdd(&hl[cr * GBH_KA_MAX + j0 + 2 * k], t & 0xFFFFu);
-                        atomicAdd(&hl[cr * GBH_KA_MAX + j0 + 2 * k + 1], t >> 16);
-                    }
+                    const uint32_t t = __builtin_amdgcn_readlane(wave_total63(hc[k]), 63);
+                    tv = lane == k ? t : tv;
+                }
+                if (lane < 8 && tv) {
+                    atomicAdd(&hl[cr * GBH_KA_MAX + j0 + 2 * lane], tv & 0xFFFFu);
+                    atomicAdd(&hl[cr * GBH_KA_MAX + j0 + 2 * lane + 1], tv >> 16);
                 }
             } else {
                 uint32_t r0 = 0, r1 = 0;
@@ -1084,10 +1088,11 @@ __global__ void __launch_bounds__(256) k_count_a(gbh_batch b, const gbh_countseg
     const uint32_t tid = threadIdx.x, s = blockIdx.y;
     const gbh_countseg& a = css[s];
     if (!a.on || blockIdx.x * 256 >= a.nwords) return;
-    // the walk's phase offsets, read once (not again after every atomic)
-    uint32_t pos0[8];
+    // the walk's phase offsets, read once: pos0[k] / 2 (below 15) in 4-bit
+    // field k
+    uint32_t ph4 = 0;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) pos0[k] = a.pos0[k];
+    for (int k = 0; k < 8; ++k) ph4 |= (a.pos0[k] >> 1) << (4 * k);
     gbh_segout* out = outs + s;
     const uint64_t* map = seg_map(b, s);
     uint32_t* fl = scratch + (size_t)s * swords;
@@ -1113,24 +1118,34 @@ __global__ void __launch_bounds__(256) k_count_a(gbh_batch b, const gbh_countseg
             const uint64_t pw = a.td[j] ? ((p2[0] >> 1) | (p2[1] << 63)) : p2[0];
             pcs[8 + j] = (uint32_t)__popcll(lw & pw);
         }
-        // the walk: primes in position order (block t, then ascending phase)
-        uint64_t any = 0;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) any |= w[k];
+        // the walk: primes in position order (block t, then ascending
+        // phase), over the 32-bit halves of the words; per block the eight
+        // class bits gathered into m8, then one step per prime
         uint32_t first = 0xFFFFFFFFu, prev = 0;
         bool have = false;
-        while (any) {
-            const uint32_t bt = (uint32_t)__ffsll((long long)any) - 1;
-            any &= any - 1;
-            const uint32_t base = 30u * (x * 64 + bt);
+#pragma unroll
+        for (int hf = 0; hf < 2; ++hf) {
+            uint32_t h32[8], any = 0;
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                if ((w[k] >> bt) & 1ull) {
-                    const uint32_t pos = base + pos0[k];
+                h32[k] = (uint32_t)(w[k] >> (32 * hf));
+                any |= h32[k];
+            }
+            while (any) {
+                const uint32_t bt = (uint32_t)__ffs(any) - 1;
+                any &= any - 1;
+                uint32_t m8 = 0;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) m8 |= ((h32[k] >> bt) & 1u) << k;
+                const uint32_t base = 30u * (x * 64 + 32 * hf + bt);
+                do {
+                    const uint32_t k = (uint32_t)__ffs(m8) - 1;
+                    m8 &= m8 - 1;
+                    const uint32_t pos = base + 2 * ((ph4 >> (4 * k)) & 15u);
                     if (have) note_gap(pos - prev, pos, hl, out, best);
                     else { first = pos; have = true; }
                     prev = pos;
-                }
+                } while (m8);
             }
         }
         fl[2 * x] = first;

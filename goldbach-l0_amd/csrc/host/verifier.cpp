@@ -500,6 +500,7 @@ private:
     // GB_HL_MODE = auto | direct | counted
     std::string hl_mode_ = env::choice("GB_HL_MODE", "auto", {"auto", "direct", "counted"});
     bool direct_ = false, force_count_ = false, direct_used_[2] = {false, false};
+    bool count_side_[2] = {false, false};   // the slot's counter ran on its side stream
     // the window stage's sieve phase beside the harvest on a second stream (GB_OVERLAP=0: one stream)
     bool overlap_ = env::onoff("GB_OVERLAP", true);
     uint64_t region_entries_ = 0;
@@ -1166,6 +1167,8 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         check(gbh_launch_window(c, ms, &w), "window hit lists");
     }
     check(gbh_event_record(c, ms, ev[EV_WINDOW].e), "event");
+    count_side_[slot] = ss != ms && check_and_count && nwords;
+    if (count_side_[slot]) check(gbh_stream_wait(c, ss, ev[EV_WINDOW].e), "stream wait");
     if (verify_every_ && nlarge_ && check_and_count)
         for (uint32_t s = 0; s < nseg; ++s)
             if (batch[s].index % verify_every_ == 0) enqueue_verify(s, b, out, slot);
@@ -1180,10 +1183,16 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     }
     check(gbh_event_record(c, ms, ev[EV_CHECK].e), "event");
     if (check_and_count && nwords) {
+        // the counter only reads the map and fills its own output fields:
+        // on the side stream it runs beside the checker
+        const int cs = count_side_[slot] ? ss : ms;
         gbh_count_args ct{b, B.cs.as<gbh_countseg>(), nwords, swords_, B.scratch.as<uint32_t>(), out};
-        check(gbh_launch_count(c, ms, &ct), "count");
+        check(gbh_launch_count(c, cs, &ct), "count");
+        check(gbh_event_record(c, cs, ev[EV_COUNT].e), "event");
+        if (cs != ms) check(gbh_stream_wait(c, ms, ev[EV_COUNT].e), "stream wait");
+    } else {
+        check(gbh_event_record(c, ms, ev[EV_COUNT].e), "event");
     }
-    check(gbh_event_record(c, ms, ev[EV_COUNT].e), "event");
     check(gbh_copy_d2h(c, ms, E.h_out[slot].p, out, sizeof(gbh_segout) * nseg), "copy outputs");
     check(gbh_event_record(c, ms, ev[EV_END].e), "event");
 }
@@ -1234,7 +1243,8 @@ void Pipeline::collect(std::vector<SegJob>& batch, int slot)
     agg_.window_kernel_seconds += ts;
     ++agg_.window_launches;
     const double tc = elapsed_s(c, ev[EV_WINDOW], ev[EV_CHECK]);
-    const double tn = elapsed_s(c, ev[EV_CHECK], ev[EV_COUNT]);
+    // the counter ran beside the checker (from the window's end) or after it
+    const double tn = elapsed_s(c, ev[count_side_[slot] ? EV_WINDOW : EV_CHECK], ev[EV_COUNT]);
     agg_.sieve_seconds += ts;
     agg_.large_seconds += tl;
     agg_.verify_seconds += tc + tn;
