@@ -210,18 +210,21 @@ __device__ __forceinline__ double inv_of(uint32_t p)
     return __builtin_fma(r, e, r);
 }
 
-// floor(x / p) for p >= 2^12 and its remainder: a double-precision quotient
-// (within a few units here: x rounds to within 2^11, inv and the product to
-// about 2^-50 relative) made exact by the integer remainder.
+// floor(x / p) for x < 2^63, p >= 2^12 and its remainder: a double-precision
+// quotient (x rounds to within 2^10, inv and the product to about 2^-50
+// relative, so the quotient, below 2^51, is within one) made exact by one
+// remainder step each way. Returns *rem >= p if the estimate was further off
+// (the caller fails closed). Written without correction loops: a loop of
+// r += p would be compiled into a 64-bit division.
 __device__ __forceinline__ uint64_t div_big(uint64_t x, double xd, uint32_t p, double inv, uint32_t* rem)
 {
     const double qd = xd * inv;
     const uint64_t hi = (uint64_t)(uint32_t)(qd * (1.0 / 4294967296.0));
     uint64_t q = (hi << 32) + (uint64_t)(uint32_t)(qd - (double)hi * 4294967296.0);
     int64_t r = (int64_t)(x - q * p);
-    while (r < 0) { --q; r += p; }
-    while (r >= (int64_t)p) { ++q; r -= p; }
-    *rem = (uint32_t)r;
+    if (r < 0) { --q; r += p; }
+    else if (r >= (int64_t)p) { ++q; r -= p; }
+    *rem = r >= 0 && r < (int64_t)p ? (uint32_t)r : p;
     return q;
 }
 
@@ -240,6 +243,7 @@ __device__ __forceinline__ void walk_wheel(const HlCtx& h, double qb0d, uint32_t
 {
     uint32_t rem;
     uint64_t m = div_big(h.qb0, qb0d, p, inv_of(p), &rem);
+    if (rem >= p) { atomicOr(err, 16u); return; }
     m += rem != 0;
     if (m < p) m = p;
     const uint32_t r = (uint32_t)(m % 30030);
@@ -305,9 +309,13 @@ __device__ __forceinline__ BmWalk bm_setup(const HlCtx& h, double qb0d, uint32_t
     const double fd = __builtin_floor(qb0d * inv);
     const uint32_t fhi = (uint32_t)(fd * (1.0 / 4294967296.0));
     uint64_t m = ((uint64_t)fhi << 32) | (uint32_t)(fd - (double)fhi * 4294967296.0);
+    // within one of floor(qb0 / p) (relative error below 2^-45 of a
+    // quotient below 2^44): one step each way (no correction loop, which
+    // would be compiled into a 64-bit division), then the guard
     int64_t r = (int64_t)(h.qb0 - m * p);
-    while (r < 0) { --m; r += p; }
-    while (r >= (int64_t)p) { ++m; r -= p; }
+    if (r < 0) { --m; r += p; }
+    else if (r >= (int64_t)p) { ++m; r -= p; }
+    if (r < 0 || r >= (int64_t)p) { atomicOr(err, 16u); return k; }
     uint32_t r0 = 0;                       // m0 p = qb0 + r0
     if (r) { ++m; r0 = p - (uint32_t)r; }
     // further multipliers up to qb1: t = floor((span - r0) / p)
@@ -316,9 +324,11 @@ __device__ __forceinline__ BmWalk bm_setup(const HlCtx& h, double qb0d, uint32_t
     const uint64_t y = span - r0;
     uint32_t t = 0;
     if (y >= p) {
+        // y < 2^33 and p >= 2^12: the float quotient is within one
         t = (uint32_t)((float)(uint32_t)(y >> 1) * 2.0f * __builtin_amdgcn_rcpf((float)p));
-        while ((uint64_t)t * p > y) --t;
-        while ((uint64_t)(t + 1) * p <= y) ++t;
+        if ((uint64_t)t * p > y) --t;
+        else if ((uint64_t)(t + 1) * p <= y) ++t;
+        if ((uint64_t)t * p > y || (uint64_t)(t + 1) * p <= y) { atomicOr(err, 16u); return k; }
     } else if (m >= p) {
         // one multiplier only (the common case of a short batch): it must be
         // a unit mod 30 (2^32 = 16 mod 30) to be in the bitmap at all
@@ -421,6 +431,76 @@ __device__ __forceinline__ void bm_group(const HlCtx& h, double qb0d, const uint
     }
 }
 
+// Bitmap primes above the run's q span (p > qb1 - qb0) have at most one
+// multiple in it, q = m0 p with m0 = ceil(qb0 / p): a lean form of
+// bm_setup + bm_run for them (one quotient, one bit of one word; most
+// primes leave after the span or unit test, before any load).
+template <int kMode>
+__device__ __forceinline__ void bm_single(const HlCtx& h, double qb0d, const uint32_t* __restrict__ primes,
+                                          uint32_t j0, uint32_t j1, uint32_t k, uint32_t tid, const uint32_t* df,
+                                          const uint32_t* dl, const uint32_t* dq, const uint32_t* cmt, uint32_t* cnt,
+                                          uint64_t* any_hit, uint32_t* __restrict__ hits, uint32_t cap,
+                                          uint32_t* __restrict__ err)
+{
+    uint32_t p[kG];
+#pragma unroll
+    for (uint32_t u = 0; u < kG; ++u) {
+        const uint32_t j = j0 + (k + u) * 256 + tid;
+        bool on = j < j1;
+        if (kMode == kWrite && on) on = (any_hit[(k + u) * 4 + (tid >> 6)] >> (tid & 63)) & 1;
+        p[u] = on ? primes[j] : 0u;
+    }
+    const uint64_t span = 2ull * h.dmax;
+    uint64_t w[kG];
+    uint32_t bit[kG], d[kG], cls[kG];
+#pragma unroll
+    for (uint32_t u = 0; u < kG; ++u) {
+        const uint32_t pu = p[u];
+        bit[u] = 64;                       // 64: no multiple to test
+        w[u] = 0;
+        d[u] = cls[u] = 0;
+        if (!pu) continue;
+        // m = floor(qb0 / p) as in bm_setup: within one, one step each way
+        const double pd = (double)pu;
+        double inv = (double)__builtin_amdgcn_rcpf((float)pu);
+        inv = __builtin_fma(inv, __builtin_fma(-pd, inv, 1.0), inv);
+        const double fd = __builtin_floor(qb0d * inv);
+        const uint32_t fhi = (uint32_t)(fd * (1.0 / 4294967296.0));
+        uint64_t m = ((uint64_t)fhi << 32) | (uint32_t)(fd - (double)fhi * 4294967296.0);
+        int64_t r = (int64_t)(h.qb0 - m * pu);
+        if (r < 0) { --m; r += pu; }
+        else if (r >= (int64_t)pu) { ++m; r -= pu; }
+        if (r < 0 || r >= (int64_t)pu) { atomicOr(err, 16u); continue; }
+        const uint32_t r0 = r ? pu - (uint32_t)r : 0u;   // m0 p = qb0 + r0
+        const uint64_t m0 = m + (r != 0);
+        if (r0 > span || m0 < pu) continue;             // no multiple in the span, or m0 below p
+        const uint32_t r30 = ((uint32_t)(m0 >> 32) * 16u + (uint32_t)m0 % 30u) % 30u;
+        if (!((kUnitMask >> r30) & 1u)) continue;       // not in the bitmap: m0 shares a factor with 30
+        if (m0 < h.m_base || m0 - h.m_base >= h.m_span) {   // outside the bitmap: fail closed
+            atomicOr(err, 8u);
+            continue;
+        }
+        const uint64_t y0 = ((m0 - r30 - h.m_base) >> 1) * 0xEEEEEEEEEEEEEEEFull;   // byte of m0
+        const uint32_t i = units_below(r30);
+        w[u] = y0 >> 3;
+        bit[u] = 8 * (uint32_t)(y0 & 7) + i;
+        d[u] = r0 >> 1;
+        cls[u] = (cmt[units_below(pu % 30)] >> (3 * i)) & 7;
+    }
+    uint64_t v[kG];
+#pragma unroll
+    for (uint32_t u = 0; u < kG; ++u) v[u] = bit[u] < 64 ? __builtin_nontemporal_load(h.mbits + w[u]) : 0ull;
+#pragma unroll
+    for (uint32_t u = 0; u < kG; ++u) {
+        const bool any = bit[u] < 64 && ((v[u] >> bit[u]) & 1ull);
+        if (any) file_hit<kMode>(h, df, dl, dq, d[u], cls[u], cnt, hits, cap, err);
+        if (kMode == kCount) {
+            const uint64_t bal = __ballot(any);
+            if ((tid & 63) == 0) any_hit[(k + u) * 4 + (tid >> 6)] = bal;
+        }
+    }
+}
+
 // One work-group per chunk of primes; two passes over the chunk: count per
 // window list, reserve each list's share with one atomic, then write. With
 // `stage` slots, the counting pass also keeps the entries in LDS; when they
@@ -516,14 +596,18 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
                 continue;
             }
         }
+        // every prime of the chunk above the run's q span: the lean form
+        const bool single = !wheel && (uint64_t)primes[j0] > qb1 - qb0;
         uint32_t* rh = hits + (size_t)seg0 * 8 * b.nwin * cap;   // the run's lists
         uint32_t* rhn = hit_n + (size_t)seg0 * 8 * b.nwin;
         if (direct) {   // one walk into the work-group's regions, then their fill counts
             if (wheel) {
                 for (uint32_t j = j0 + tid; j < je; j += 256) walk_wheel<kDirect>(h, qb0d, primes[j], df, dl, dq, wl, cnt, rh, cap, err);
             } else {
-                for (uint32_t k = 0; j0 + k * 256 < je; k += kG)
-                    bm_group<kDirect>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
+                for (uint32_t k = 0; j0 + k * 256 < je; k += kG) {
+                    if (single) bm_single<kDirect>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
+                    else bm_group<kDirect>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
+                }
             }
             __syncthreads();
             uint32_t* fl = fills + (size_t)blockIdx.x * (b.nseg * 8) + seg0 * 8;
@@ -533,8 +617,10 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
         if (wheel) {
             for (uint32_t j = j0 + tid; j < je; j += 256) walk_wheel<kCount>(h, qb0d, primes[j], df, dl, dq, wl, cnt, rh, cap, err);
         } else {
-            for (uint32_t k = 0; j0 + k * 256 < je; k += kG)
-                bm_group<kCount>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
+            for (uint32_t k = 0; j0 + k * 256 < je; k += kG) {
+                if (single) bm_single<kCount>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
+                else bm_group<kCount>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
+            }
         }
         __syncthreads();
         for (uint32_t i = tid; i < nb; i += 256) {
@@ -555,8 +641,10 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
         if (wheel) {
             for (uint32_t j = j0 + tid; j < je; j += 256) walk_wheel<kWrite>(h, qb0d, primes[j], df, dl, dq, wl, cnt, rh, cap, err);
         } else {
-            for (uint32_t k = 0; j0 + k * 256 < je; k += kG)
-                bm_group<kWrite>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
+            for (uint32_t k = 0; j0 + k * 256 < je; k += kG) {
+                if (single) bm_single<kWrite>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
+                else bm_group<kWrite>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
+            }
         }
     }
 }

@@ -245,7 +245,9 @@ struct Engine::State {
         base = base_primes_cached(std::max<uint32_t>(base_prime_bound(max_limit), 64), prime_cache, host_threads());
         cache_s = since(tc);
         win_lo = uint32_t(std::lower_bound(base.begin(), base.end(), 67u) - base.begin());
-        const uint64_t win_p = env::u64("GB_WIN_P", WIN_P_DEFAULT, 67, UINT32_MAX);
+        // at least 2^16: the hit lists' double-precision quotients q / p
+        // stay within one of the integer quotient only for p >= 2^16
+        const uint64_t win_p = env::u64("GB_WIN_P", WIN_P_DEFAULT, uint64_t(1) << 16, UINT32_MAX);
         win_hi = std::max(win_lo, uint32_t(std::lower_bound(base.begin(), base.end(), win_p) - base.begin()));
         {
             const size_t nl = base.size() - win_hi;
@@ -1195,6 +1197,8 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     }
     check(gbh_copy_d2h(c, ms, E.h_out[slot].p, out, sizeof(gbh_segout) * nseg), "copy outputs");
     check(gbh_event_record(c, ms, ev[EV_END].e), "event");
+    if (phase_trace_) std::fprintf(stderr, "host: batch queued %.3f ms after its first segment was formed\n",
+                                   since(batch[0].r->t_first) * 1e3);
 }
 
 // The GB_BUCKET_VERIFY rebuild of segment s (the reference's
@@ -1231,7 +1235,10 @@ void Pipeline::collect(std::vector<SegJob>& batch, int slot)
 {
     gbh_ctx* c = E.c;
     auto& ev = E.ev[slot];
+    const auto tw = clock_::now();
     check(gbh_event_sync(c, ev[EV_END].e), "batch sync");
+    const auto tcol = clock_::now();
+    if (phase_trace_) std::fprintf(stderr, "host: waited %.3f ms for the batch\n", since(tw) * 1e3);
     // the sieve phase ran beside the harvest: its own time plus the hit-list
     // phase count as the window stage
     const double tl = elapsed_s(c, ev[EV_BEGIN], ev[EV_SETUP]);
@@ -1265,8 +1272,10 @@ void Pipeline::collect(std::vector<SegJob>& batch, int slot)
     if (outs[0].err & 1u) throw std::runtime_error("hit-list bucket overflow (raise the capacity)");
     if (outs[0].err & 4u) throw std::runtime_error("hit lists: a bitmap prime's cube is not above the batch's q range");
     if (outs[0].err & 8u) throw std::runtime_error("hit lists: a multiplier outside the multiplier bitmap");
+    if (outs[0].err & 16u) throw std::runtime_error("hit lists: a quotient estimate more than one off");
     const double share[4] = {ts / batch.size(), tl / batch.size(), tc / batch.size(), tn / batch.size()};
     for (size_t s = 0; s < batch.size(); ++s) collect_segment(batch[s], outs[s], share);
+    if (phase_trace_) std::fprintf(stderr, "host: collected in %.3f ms\n", since(tcol) * 1e3);
     if (phase_trace_)
         std::fprintf(stderr, "batch of %zu segments from %llu: harvest+setup %.3f ms window %.3f ms check %.3f ms "
                      "count %.3f ms\n", batch.size(), (unsigned long long)batch[0].sg.seg_start, tl * 1e3, ts * 1e3,
