@@ -127,7 +127,7 @@ int gbh_open(int device, gbh_ctx** out)
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&gbh_dev::k_window),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) (void)hipGetLastError();   // optional: windows up to the default limit still work
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&gbh_dev::k_harvest),
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&gbh_dev::k_harvest<false>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024) != hipSuccess)
         (void)hipGetLastError();
     *out = c;
@@ -289,9 +289,16 @@ int gbh_launch_harvest(gbh_ctx* c, int s, const gbh_harvest_args* a)
     if (a->stage > GBH_HL_STAGE) return fail(GBH_E_ARG, "harvest: stage above GBH_HL_STAGE");
     const size_t lds = a->stage ? size_t(nb) * 8 + size_t(a->stage) * 6 + 16 : size_t(nb) * 4;
     if (lds > 144 * 1024) return fail(GBH_E_ARG, "harvest: lists and stage exceed the work-group's LDS");
-    hipLaunchKernelGGL(gbh_dev::k_harvest, dim3(a->nchunks), dim3(256), lds, S(c, s), a->b, a->primes, a->chunks,
-                       a->runs, a->runs ? a->nruns : 1u, uint32_t(nb), a->t_hl, a->t_hi, a->t_hi_end, a->mbits, a->m_base, a->m_span,
-                       a->cap, a->stage, a->hits, a->hit_n, a->rdesc, a->region, a->fills, a->err);
+    if (direct)
+        hipLaunchKernelGGL(gbh_dev::k_harvest<true>, dim3(a->nchunks), dim3(256), lds, S(c, s), a->b, a->primes,
+                           a->chunks, a->runs, a->runs ? a->nruns : 1u, uint32_t(nb), a->t_hl, a->t_hi, a->t_hi_end,
+                           a->mbits, a->m_base, a->m_span, a->cap, a->stage, a->hits, a->hit_n, a->rdesc, a->region,
+                           a->fills, a->err);
+    else
+        hipLaunchKernelGGL(gbh_dev::k_harvest<false>, dim3(a->nchunks), dim3(256), lds, S(c, s), a->b, a->primes,
+                           a->chunks, a->runs, a->runs ? a->nruns : 1u, uint32_t(nb), a->t_hl, a->t_hi, a->t_hi_end,
+                           a->mbits, a->m_base, a->m_span, a->cap, a->stage, a->hits, a->hit_n, a->rdesc, a->region,
+                           a->fills, a->err);
     GBH_HIP(hipGetLastError());
     return GBH_OK;
 }

@@ -510,6 +510,10 @@ __device__ __forceinline__ void bm_single(const HlCtx& h, double qb0d, const uin
 // segments in the batch), each run with its own q range, lists and prime
 // bound; nullptr runs is one run of the whole batch.
 // A bitmap prime with p^3 <= qb1 breaks the two-factor rule: fail closed.
+// kDirectForm: the direct form (rdesc, region and fills given) or the
+// counted / staged forms; one instance each, so neither carries the other's
+// registers.
+template <bool kDirectForm>
 __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __restrict__ primes,
                                                  const uint32_t* __restrict__ chunks,
                                                  const gbh_hl_run* __restrict__ runs, uint32_t nruns, uint32_t nbmax,
@@ -549,7 +553,7 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
         const uint32_t seg0 = runs ? runs[r].seg0 : 0u, rn = runs ? runs[r].nseg : b.nseg;
         const uint32_t je = runs ? min(j1, runs[r].nprimes) : j1;
         if (j0 >= je) continue;
-        const bool direct = rdesc != nullptr;
+        constexpr bool direct = kDirectForm;
         const uint32_t nb = direct ? rn * 8 : rn * 8 * b.nwin;   // the run's counters
         if (rn == 0 || seg0 + rn > b.nseg || nb > nbmax) {       // a run beyond the contract: fail closed
             if (tid == 0) atomicOr(err, 1u);
