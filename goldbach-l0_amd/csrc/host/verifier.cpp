@@ -118,7 +118,24 @@ struct Candidates {
 // Batch events: stream 0 carries the uploads, the harvest, the hit-list
 // phase of the window stage, the checker and the counter; stream 1 the
 // set-up and the sieve phase of the window stage, beside the harvest.
-enum { EV_BEGIN, EV_HL0, EV_HL1, EV_SETUP, EV_WINDOW, EV_CHECK, EV_COUNT, EV_END, EV_GEOM, EV_S0, EV_S1, EV_N };
+// A batch's host-built tables, uploaded in one copy: the geometry of its
+// segments, its runs and its counter parameters, each section 256-byte
+// aligned.
+struct TabLayout {
+    size_t runs, cs, bytes;
+};
+inline TabLayout tab_layout(size_t nseg)
+{
+    const auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    TabLayout t;
+    t.runs = al(sizeof(gbh_geom) * nseg);
+    t.cs = al(t.runs + sizeof(gbh_hl_run) * nseg);
+    t.bytes = t.cs + sizeof(gbh_countseg) * nseg;
+    return t;
+}
+
+enum { EV_BEGIN, EV_HL0, EV_HL1, EV_SETUP, EV_WINDOW, EV_CHECK, EV_COUNT, EV_END, EV_GEOM, EV_S0, EV_S1, EV_COFF, EV_CA,
+       EV_N };
 
 // Knobs of removed reference pipelines are refused, not ignored: a run that
 // asks for one would otherwise silently run something else
@@ -213,11 +230,12 @@ struct Engine::State {
     // the device buffers of one batch in flight: two sets, so a batch's
     // sieve and harvest run while the batch before it is checked and counted
     struct Slot {
-        Buf map, i0, coff, coffa, pending, scratch, hits, hit_n, geoms, cs, runs, chunks, rdesc, groups, region, fills;
+        Buf map, i0, coff, coffa, pending, scratch, hits, hit_n, chunks, rdesc, groups, region, fills;
+        Buf tab;                            // the batch tables (tab_layout)
         Buf vmap, vhits, vhit_n, vchunks;   // the GB_BUCKET_VERIFY rebuild
     } sl[2];
     Buf h_vchunks;
-    Buf d_out[2], h_out[2], h_geoms[2], h_cs[2], h_chunks[2], h_runs[2], h_rdesc[2];
+    Buf d_out[2], h_out[2], h_tab[2], h_chunks[2], h_rdesc[2];
     Event ev[2][EV_N];
     Candidates cand;
     ChunkPlan cplan;                     // the last harvest chunk plan made (host side)
@@ -275,13 +293,11 @@ struct Engine::State {
         for (int s = 0; s < 2; ++s) {
             d_out[s].ensure(c, sizeof(gbh_segout) * kbatch, 0);
             h_out[s].ensure(c, sizeof(gbh_segout) * kbatch, 1);
-            h_geoms[s].ensure(c, sizeof(gbh_geom) * kbatch, 1);
-            h_cs[s].ensure(c, sizeof(gbh_countseg) * kbatch, 1);
+            h_tab[s].ensure(c, tab_layout(kbatch).bytes, 1);
             for (auto& e : ev[s]) e.create(c);
         }
         for (auto& B : sl) {
-            B.geoms.ensure(c, sizeof(gbh_geom) * kbatch, 0);
-            B.cs.ensure(c, sizeof(gbh_countseg) * kbatch, 0);
+            B.tab.ensure(c, tab_layout(kbatch).bytes, 0);
             B.coff.ensure(c, size_t(kbatch) * 15 * GBH_NCAND * 4, 0);
             B.coffa.ensure(c, size_t(kbatch) * 15 * GBH_NCAND_PAD * 4, 0);
         }
@@ -686,18 +702,14 @@ void Pipeline::plan(const PlanBounds& pb)
         B.scratch.ensure(c, K * size_t(swords_) * 4, 0);
         B.hit_n.ensure(c, K * 8 * size_t(nwin_max_) * 4, 0);
         if (nlarge_) B.hits.ensure(c, K * 8 * size_t(nwin_max_) * cap_ * 4, 0);
-        B.geoms.ensure(c, sizeof(gbh_geom) * K, 0);
-        B.cs.ensure(c, sizeof(gbh_countseg) * K, 0);
+        B.tab.ensure(c, tab_layout(K).bytes, 0);
         B.coff.ensure(c, K * 15 * GBH_NCAND * 4, 0);
         B.coffa.ensure(c, K * 15 * GBH_NCAND_PAD * 4, 0);
-        B.runs.ensure(c, sizeof(gbh_hl_run) * K, 0);
     }
     for (int s = 0; s < 2; ++s) {
         E.d_out[s].ensure(c, sizeof(gbh_segout) * K, 0);
         E.h_out[s].ensure(c, sizeof(gbh_segout) * K, 1);
-        E.h_geoms[s].ensure(c, sizeof(gbh_geom) * K, 1);
-        E.h_cs[s].ensure(c, sizeof(gbh_countseg) * K, 1);
-        E.h_runs[s].ensure(c, sizeof(gbh_hl_run) * K, 1);
+        E.h_tab[s].ensure(c, tab_layout(K).bytes, 1);
     }
     if (!multi_ && nlarge_) {   // a plain run's batches share one chunk plan: make it now
         const auto tp0 = clock_::now();
@@ -1027,9 +1039,14 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     // counter) and its side stream (set-up and the window sieve phase)
     const int ms = slot;
     const uint32_t nseg = uint32_t(batch.size());
-    gbh_geom* hg = E.h_geoms[slot].as<gbh_geom>();
-    gbh_countseg* hc = E.h_cs[slot].as<gbh_countseg>();
-    gbh_hl_run* hr = E.h_runs[slot].as<gbh_hl_run>();
+    const TabLayout tl = tab_layout(nseg);
+    uint8_t* ht = E.h_tab[slot].as<uint8_t>();
+    gbh_geom* hg = reinterpret_cast<gbh_geom*>(ht);
+    gbh_hl_run* hr = reinterpret_cast<gbh_hl_run*>(ht + tl.runs);
+    gbh_countseg* hc = reinterpret_cast<gbh_countseg*>(ht + tl.cs);
+    gbh_geom* dg = B.tab.as<gbh_geom>();
+    gbh_hl_run* dr = reinterpret_cast<gbh_hl_run*>(B.tab.as<uint8_t>() + tl.runs);
+    gbh_countseg* dcs = reinterpret_cast<gbh_countseg*>(B.tab.as<uint8_t>() + tl.cs);
     // batched ranges: a batch of short segments takes a window just above
     // its longest class array (one window per class); plain runs keep wbits_
     uint32_t wbits = wbits_;
@@ -1078,7 +1095,7 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         ++hr[nruns - 1].nseg;
         run_max = std::max(run_max, hr[nruns - 1].nseg);
     }
-    gbh_batch b{B.geoms.as<gbh_geom>(), nseg, cw_, wbits, nwin, B.map.as<uint64_t>()};
+    gbh_batch b{dg, nseg, cw_, wbits, nwin, B.map.as<uint64_t>()};
     gbh_segout* out = E.d_out[slot].as<gbh_segout>();
     auto& ev = E.ev[slot];
     check(gbh_event_record(c, ms, ev[EV_BEGIN].e), "event");
@@ -1117,17 +1134,35 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     } else if (nlarge_) {
         upload_chunks(slot);
     }
-    check(gbh_copy_h2d(c, ms, B.geoms.p, hg, sizeof(gbh_geom) * nseg), "upload geometry");
-    check(gbh_copy_h2d(c, ms, B.cs.p, hc, sizeof(gbh_countseg) * nseg), "upload count parameters");
-    check(gbh_copy_h2d(c, ms, B.runs.p, hr, sizeof(gbh_hl_run) * nruns), "upload runs");
+    // the harvest first (the longest stage): the tables, the error word and
+    // the fill counts it needs, then its launch; the sieve phase and the
+    // rest of the set-up are queued behind it
+    check(gbh_copy_h2d(c, ms, B.tab.p, ht, tl.bytes), "upload batch tables");
     check(gbh_reset_outputs(c, ms, out, nseg), "reset outputs");
     uint32_t ncoop = 0;
     while (ncoop < nwin_ && E.base[E.win_lo + ncoop] < wbits / 128) ++ncoop;
-    // the window stage's sieve phase (masks, window primes) on stream 1
-    // beside the harvest; it starts once this batch's tables are uploaded
-    // (and so every kernel of the batch before has finished on stream 0)
+    // the window stage's sieve phase (masks, window primes) on the slot's
+    // side stream beside the harvest, once the batch tables are uploaded
     const int ss = overlap_ ? 2 + slot : ms;
     check(gbh_event_record(c, ms, ev[EV_GEOM].e), "event");
+    const bool direct = nlarge_ && direct_ && !force_count_;
+    direct_used_[slot] = direct;
+    if (nlarge_) {
+        // the counted and staged forms reserve list slots in hit_n; the
+        // direct form needs it only for the partition (zeroed below)
+        if (direct) check(gbh_zero(c, ms, B.fills.p, size_t(nchunks_) * nseg * 8 * 4), "zero region fills");
+        else check(gbh_zero(c, ms, B.hit_n.p, size_t(nseg) * 8 * nwin * 4), "zero hit counts");
+        gbh_harvest_args h{b, E.d_large.as<uint32_t>(), B.chunks.as<uint32_t>(), dr, nruns,
+                           run_max, nchunks_, cap_, t_hl_, t_hi_, t_hi2_, direct ? 0u : stage_,
+                           mb_ ? mb_->bits : nullptr,
+                           mb_ ? mb_->m_base : 0, mb_ ? mb_->m_span : 0, B.hits.as<uint32_t>(),
+                           B.hit_n.as<uint32_t>(), direct ? B.rdesc.as<uint32_t>() : nullptr,
+                           direct ? B.region.as<uint32_t>() : nullptr, direct ? B.fills.as<uint32_t>() : nullptr,
+                           &out[0].err};
+        check(gbh_event_record(c, ms, ev[EV_HL0].e), "event");
+        check(gbh_launch_harvest(c, ms, &h), "harvest");
+        check(gbh_event_record(c, ms, ev[EV_HL1].e), "event");
+    }
     if (ss != ms) check(gbh_stream_wait(c, ss, ev[EV_GEOM].e), "stream wait");
     if (nwin_) {
         gbh_setup_args a{b, E.d_win.as<uint32_t>(), nwin_, 0, B.i0.as<uint32_t>()};
@@ -1139,27 +1174,12 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         check(gbh_launch_window(c, ss, &w), "window sieve");
     }
     check(gbh_event_record(c, ss, ev[EV_S1].e), "event");
-    const bool direct = nlarge_ && direct_ && !force_count_;
-    direct_used_[slot] = direct;
-    if (nlarge_) {
+    if (direct) {
         check(gbh_zero(c, ms, B.hit_n.p, size_t(nseg) * 8 * nwin * 4), "zero hit counts");
-        if (direct) check(gbh_zero(c, ms, B.fills.p, size_t(nchunks_) * nseg * 8 * 4), "zero region fills");
-        gbh_harvest_args h{b, E.d_large.as<uint32_t>(), B.chunks.as<uint32_t>(), B.runs.as<gbh_hl_run>(), nruns,
-                           run_max, nchunks_, cap_, t_hl_, t_hi_, t_hi2_, direct ? 0u : stage_,
-                           mb_ ? mb_->bits : nullptr,
-                           mb_ ? mb_->m_base : 0, mb_ ? mb_->m_span : 0, B.hits.as<uint32_t>(),
-                           B.hit_n.as<uint32_t>(), direct ? B.rdesc.as<uint32_t>() : nullptr,
-                           direct ? B.region.as<uint32_t>() : nullptr, direct ? B.fills.as<uint32_t>() : nullptr,
-                           &out[0].err};
-        check(gbh_event_record(c, ms, ev[EV_HL0].e), "event");
-        check(gbh_launch_harvest(c, ms, &h), "harvest");
-        check(gbh_event_record(c, ms, ev[EV_HL1].e), "event");
-        if (direct) {
-            const gbh_partition_args pa{b, B.rdesc.as<uint32_t>(), B.region.as<uint32_t>(), B.fills.as<uint32_t>(),
-                                        B.groups.as<uint32_t>(), ngroups_, cap_, B.hits.as<uint32_t>(),
-                                        B.hit_n.as<uint32_t>(), &out[0].err};
-            check(gbh_launch_partition(c, ms, &pa), "partition");
-        }
+        const gbh_partition_args pa{b, B.rdesc.as<uint32_t>(), B.region.as<uint32_t>(), B.fills.as<uint32_t>(),
+                                    B.groups.as<uint32_t>(), ngroups_, cap_, B.hits.as<uint32_t>(),
+                                    B.hit_n.as<uint32_t>(), &out[0].err};
+        check(gbh_launch_partition(c, ms, &pa), "partition");
     }
     check(gbh_event_record(c, ms, ev[EV_SETUP].e), "event");
     if (ss != ms) check(gbh_stream_wait(c, ms, ev[EV_S1].e), "stream wait");
@@ -1179,7 +1199,22 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         for (const SegJob& j : batch) kinds |= j.sampled ? 2u : 1u;
         gbh_check_args ca{b, E.ka, pcap_, B.coff.as<uint32_t>(), B.coffa.as<uint32_t>(),
                           B.pending.as<uint32_t>(), out, kinds, 0};
-        check(gbh_launch_check_a(c, ms, &ca), "check phase A");
+        if (kinds == 3 && count_side_[slot]) {
+            // both kinds: the histogram segments on the side stream (ahead
+            // of the counter), the others here, phase B after both
+            ca.kinds = 0;                    // the candidate tables only
+            check(gbh_launch_check_a(c, ms, &ca), "check candidate tables");
+            check(gbh_event_record(c, ms, ev[EV_COFF].e), "event");
+            check(gbh_stream_wait(c, ss, ev[EV_COFF].e), "stream wait");
+            ca.kinds = 1 | 4;
+            check(gbh_launch_check_a(c, ms, &ca), "check phase A");
+            ca.kinds = 2 | 4;
+            check(gbh_launch_check_a(c, ss, &ca), "check phase A (histogram)");
+            check(gbh_event_record(c, ss, ev[EV_CA].e), "event");
+            check(gbh_stream_wait(c, ms, ev[EV_CA].e), "stream wait");
+        } else {
+            check(gbh_launch_check_a(c, ms, &ca), "check phase A");
+        }
         gbh_checkb_args cb{b, E.ka, CHECK_TAIL, B.coff.as<uint32_t>(), B.pending.as<uint32_t>(), pcap_, 0, out};
         check(gbh_launch_check_b(c, ms, &cb), "check phase B");
     }
@@ -1188,7 +1223,7 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         // the counter only reads the map and fills its own output fields:
         // on the side stream it runs beside the checker
         const int cs = count_side_[slot] ? ss : ms;
-        gbh_count_args ct{b, B.cs.as<gbh_countseg>(), nwords, swords_, B.scratch.as<uint32_t>(), out};
+        gbh_count_args ct{b, dcs, nwords, swords_, B.scratch.as<uint32_t>(), out};
         check(gbh_launch_count(c, cs, &ct), "count");
         check(gbh_event_record(c, cs, ev[EV_COUNT].e), "event");
         if (cs != ms) check(gbh_stream_wait(c, ms, ev[EV_COUNT].e), "stream wait");
@@ -1210,7 +1245,7 @@ void Pipeline::enqueue_verify(uint32_t s, const gbh_batch& b, gbh_segout* out, i
     gbh_ctx* c = E.c;
     auto& B = E.sl[slot];
     const int ms = slot;
-    const gbh_batch vb{B.geoms.as<gbh_geom>() + s, 1, cw_, b.wbits, b.nwin, B.vmap.as<uint64_t>()};
+    const gbh_batch vb{b.geoms + s, 1, cw_, b.wbits, b.nwin, B.vmap.as<uint64_t>()};
     uint32_t ncoop = 0;
     while (ncoop < nwin_ && E.base[E.win_lo + ncoop] < b.wbits / 128) ++ncoop;
     if (nwin_) {
@@ -1226,7 +1261,7 @@ void Pipeline::enqueue_verify(uint32_t s, const gbh_batch& b, gbh_segout* out, i
                             B.vhits.as<uint32_t>(), B.vhit_n.as<uint32_t>(), cap_v_, 0};
     check(gbh_launch_window(c, ms, &w), "verify window");
     const gbh_mapcmp_args m{B.map.as<uint64_t>() + size_t(s) * 8 * cw_, B.vmap.as<uint64_t>(),
-                            B.geoms.as<gbh_geom>() + s, cw_, 0, &out[s].vdiff};
+                            b.geoms + s, cw_, 0, &out[s].vdiff};
     check(gbh_launch_mapcmp(c, ms, &m), "verify compare");
     ++verified_;
 }

@@ -276,7 +276,9 @@ typedef struct {
     uint32_t* coffa;               /* [nseg][15][GBH_NCAND_PAD] the same as staged-word offsets (written here) */
     uint32_t* pending;             /* [nseg][pcap] even indices left for phase B */
     gbh_segout* out;               /* [nseg] */
-    uint32_t kinds;                /* bit 0: the batch has segments without histogram, bit 1: with */
+    uint32_t kinds;                /* bit 0: the batch has segments without histogram, bit 1: with;
+                                      bit 2: the candidate tables coff / coffa are already built
+                                      for this batch (an earlier call), skip k_coff */
     uint32_t pad0;
 } gbh_check_args;
 
