@@ -397,6 +397,8 @@ __device__ __forceinline__ bool bm_run(const HlCtx& h, const BmWalk& k, uint64_t
 // kG bitmap primes per lane at a time: primes j0 + (k + u) 256 + tid for
 // u < kG; on the writing pass only those flagged by the counting pass.
 constexpr uint32_t kG = 4;
+// bm_single's primes per lane at a time (its per-prime state is small)
+constexpr uint32_t kS = 8;
 
 template <int kMode>
 __device__ __forceinline__ void bm_group(const HlCtx& h, double qb0d, const uint32_t* __restrict__ primes,
@@ -442,19 +444,19 @@ __device__ __forceinline__ void bm_single(const HlCtx& h, double qb0d, const uin
                                           uint64_t* any_hit, uint32_t* __restrict__ hits, uint32_t cap,
                                           uint32_t* __restrict__ err)
 {
-    uint32_t p[kG];
+    uint32_t p[kS];
 #pragma unroll
-    for (uint32_t u = 0; u < kG; ++u) {
+    for (uint32_t u = 0; u < kS; ++u) {
         const uint32_t j = j0 + (k + u) * 256 + tid;
         bool on = j < j1;
         if (kMode == kWrite && on) on = (any_hit[(k + u) * 4 + (tid >> 6)] >> (tid & 63)) & 1;
         p[u] = on ? primes[j] : 0u;
     }
     const uint64_t span = 2ull * h.dmax;
-    uint64_t w[kG];
-    uint32_t bit[kG], d[kG], cls[kG];
+    uint64_t w[kS];
+    uint32_t bit[kS], d[kS], cls[kS];
 #pragma unroll
-    for (uint32_t u = 0; u < kG; ++u) {
+    for (uint32_t u = 0; u < kS; ++u) {
         const uint32_t pu = p[u];
         bit[u] = 64;                       // 64: no multiple to test
         w[u] = 0;
@@ -487,11 +489,11 @@ __device__ __forceinline__ void bm_single(const HlCtx& h, double qb0d, const uin
         d[u] = r0 >> 1;
         cls[u] = (cmt[units_below(pu % 30)] >> (3 * i)) & 7;
     }
-    uint64_t v[kG];
+    uint64_t v[kS];
 #pragma unroll
-    for (uint32_t u = 0; u < kG; ++u) v[u] = bit[u] < 64 ? __builtin_nontemporal_load(h.mbits + w[u]) : 0ull;
+    for (uint32_t u = 0; u < kS; ++u) v[u] = bit[u] < 64 ? __builtin_nontemporal_load(h.mbits + w[u]) : 0ull;
 #pragma unroll
-    for (uint32_t u = 0; u < kG; ++u) {
+    for (uint32_t u = 0; u < kS; ++u) {
         const bool any = bit[u] < 64 && ((v[u] >> bit[u]) & 1ull);
         if (any) file_hit<kMode>(h, df, dl, dq, d[u], cls[u], cnt, hits, cap, err);
         if (kMode == kCount) {
@@ -526,7 +528,7 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
                                                  uint32_t* __restrict__ fills, uint32_t* __restrict__ err)
 {
     __shared__ uint32_t df[GBH_MAX_BATCH], dl[GBH_MAX_BATCH], dq[GBH_MAX_BATCH * 8], cmt[8], sn;
-    __shared__ uint64_t any_hit[GBH_HL_CHUNK / 64 + 4 * kG];   // bitmap primes with a set bit in range (pass 2 walks those)
+    __shared__ uint64_t any_hit[GBH_HL_CHUNK / 64 + 4 * (kG > kS ? kG : kS)];   // bitmap primes with a set bit in range (pass 2 walks those)
     __shared__ WheelLds wl;
     extern __shared__ uint32_t cnt[];     // [nseg][8][nwin] counts of a run; with a stage: run counters, the stage
     uint32_t* runc = cnt + nbmax;
@@ -608,7 +610,7 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
             if (wheel) {
                 for (uint32_t j = j0 + tid; j < je; j += 256) walk_wheel<kDirect>(h, qb0d, primes[j], df, dl, dq, wl, cnt, rh, cap, err);
             } else {
-                for (uint32_t k = 0; j0 + k * 256 < je; k += kG) {
+                for (uint32_t k = 0; j0 + k * 256 < je; k += single ? kS : kG) {
                     if (single) bm_single<kDirect>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
                     else bm_group<kDirect>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
                 }
@@ -621,7 +623,7 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
         if (wheel) {
             for (uint32_t j = j0 + tid; j < je; j += 256) walk_wheel<kCount>(h, qb0d, primes[j], df, dl, dq, wl, cnt, rh, cap, err);
         } else {
-            for (uint32_t k = 0; j0 + k * 256 < je; k += kG) {
+            for (uint32_t k = 0; j0 + k * 256 < je; k += single ? kS : kG) {
                 if (single) bm_single<kCount>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
                 else bm_group<kCount>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
             }
@@ -645,7 +647,7 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
         if (wheel) {
             for (uint32_t j = j0 + tid; j < je; j += 256) walk_wheel<kWrite>(h, qb0d, primes[j], df, dl, dq, wl, cnt, rh, cap, err);
         } else {
-            for (uint32_t k = 0; j0 + k * 256 < je; k += kG) {
+            for (uint32_t k = 0; j0 + k * 256 < je; k += single ? kS : kG) {
                 if (single) bm_single<kWrite>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
                 else bm_group<kWrite>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
             }
@@ -792,10 +794,14 @@ __device__ __forceinline__ uint32_t first_off_big(uint32_t t, uint32_t wb0, uint
 {
     if (t >= wb0) return t - wb0;
     const uint32_t d = wb0 - t;
-    const uint32_t q = (uint32_t)((float)d * rp);
-    int32_t r = (int32_t)(d - q * p);
-    while (r < 0) r += (int32_t)p;
-    while (r >= (int32_t)p) r -= (int32_t)p;
+    // d < 2^31, 67 <= p < 2^19: the float quotient is within 7 of d / p, so
+    // |r| < 9 p (exact in float); a second float quotient of r is within one
+    // of r / p; then one step each way. No correction loop: such a loop is
+    // compiled into an integer division.
+    int32_t r = (int32_t)(d - (uint32_t)((float)d * rp) * p);
+    r -= (int32_t)((float)r * rp) * (int32_t)p;
+    if (r < 0) r += (int32_t)p;
+    else if (r >= (int32_t)p) r -= (int32_t)p;
     return r ? p - (uint32_t)r : 0u;
 }
 
@@ -818,9 +824,11 @@ __global__ void __launch_bounds__(256) k_window(gbh_batch b, const uint32_t* __r
     const bool sieve = phases & 1, apply = (phases & 2) && hits;
     uint64_t* dst = seg_map(b, s) + (size_t)c * b.cw + gw0;
 
+    // In LDS a window holds the composites as set bits (marks are ORs; the
+    // store inverts it back to the map's primes-as-ones).
     if (!sieve) {   // the hit lists only: the window as the sieve phase stored it
-        for (uint32_t x = tid; x < nwords; x += 256) win[x] = dst[x];
-    } else {        // 1) mask stage: every multiple of 7..61 cleared by whole-word masks
+        for (uint32_t x = tid; x < nwords; x += 256) win[x] = ~dst[x];
+    } else {        // 1) mask stage: every multiple of 7..61 set by whole-word masks
         uint32_t ph[GBH_NSMALL];
 #pragma unroll
         for (int k = 0; k < GBH_NSMALL; ++k) {
@@ -837,9 +845,8 @@ __global__ void __launch_bounds__(256) k_window(gbh_batch b, const uint32_t* __r
                 const uint32_t st = kStep256[k];                 // (64 * 256) mod p
                 ph[k] = ph[k] >= st ? ph[k] - st : ph[k] + p - st;
             }
-            uint64_t w = ~m;
-            if (gw0 + x == 0) w |= g.restore[c];
-            win[x] = w;
+            if (gw0 + x == 0) m &= ~g.restore[c];
+            win[x] = m;
         }
     }
     __syncthreads();
@@ -849,20 +856,39 @@ __global__ void __launch_bounds__(256) k_window(gbh_batch b, const uint32_t* __r
     const uint32_t lane = tid & 63, wv = tid >> 6, wbits = b.wbits;
     np = sieve ? min(np, g.np_win) : 0u;   // the segment's primes (larger ones have no multiple in it)
     ncoop = min(ncoop, np);
-    for (uint32_t j = wv; j < ncoop; j += 4) {
-        const uint32_t t0 = i0[j];
-        if (t0 == 0xFFFFFFFFu) continue;
-        const uint32_t p = primes[j];
-        for (uint32_t bt = first_off(t0, wb0, p) + lane * p; bt < wbits; bt += 64 * p)
-            atomicAnd(&win32[bt >> 5], ~(1u << (bt & 31)));
+    {   // the wave's primes are uniform: scalar loads, the next one's issued
+        // before the current one's marks
+        const uint32_t w0 = __builtin_amdgcn_readfirstlane(wv);
+        uint32_t t0n = w0 < ncoop ? i0[w0] : 0xFFFFFFFFu, pn = w0 < ncoop ? primes[w0] : 0u;
+        for (uint32_t j = w0; j < ncoop; j += 4) {
+            const uint32_t t0 = t0n, p = pn;
+            if (j + 4 < ncoop) {
+                t0n = i0[j + 4];
+                pn = primes[j + 4];
+            }
+            if (t0 == 0xFFFFFFFFu) continue;
+            for (uint32_t bt = first_off(t0, wb0, p) + lane * p; bt < wbits; bt += 64 * p)
+                atomicOr(&win32[bt >> 5], 1u << (bt & 31));
+        }
     }
-    // 3) the other window primes: one prime per lane
-    for (uint32_t j = ncoop + tid; j < np; j += 256) {
-        const uint32_t t0 = i0[j];
-        if (t0 == 0xFFFFFFFFu || t0 >= wb1) continue;
-        const uint32_t p = primes[j];
-        for (uint32_t bt = first_off_big(t0, wb0, p, __builtin_amdgcn_rcpf((float)p)); bt < wbits; bt += p)
-            atomicAnd(&win32[bt >> 5], ~(1u << (bt & 31)));
+    // 3) the other window primes: one prime per lane, kWL of them loaded at
+    //    a time
+    constexpr uint32_t kWL = 4;
+    for (uint32_t j0 = ncoop + tid; j0 < np; j0 += 256 * kWL) {
+        uint32_t t0[kWL], pr[kWL];
+#pragma unroll
+        for (uint32_t u = 0; u < kWL; ++u) {
+            const uint32_t j = j0 + 256 * u;
+            t0[u] = j < np ? i0[j] : 0xFFFFFFFFu;
+            pr[u] = j < np ? primes[j] : 1u;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kWL; ++u) {
+            if (t0[u] == 0xFFFFFFFFu || t0[u] >= wb1) continue;
+            const uint32_t p = pr[u];
+            for (uint32_t bt = first_off_big(t0[u], wb0, p, __builtin_amdgcn_rcpf((float)p)); bt < wbits; bt += p)
+                atomicOr(&win32[bt >> 5], 1u << (bt & 31));
+        }
     }
     // 4) the large primes' hits of this window
     if (apply) {
@@ -871,13 +897,13 @@ __global__ void __launch_bounds__(256) k_window(gbh_batch b, const uint32_t* __r
         const uint32_t* hl = hits + (size_t)bucket * cap;
         for (uint32_t i = tid; i < n; i += 256) {
             const uint32_t bt = hl[i];
-            atomicAnd(&win32[bt >> 5], ~(1u << (bt & 31)));
+            atomicOr(&win32[bt >> 5], 1u << (bt & 31));
         }
     }
     __syncthreads();
 
-    // 5) coalesced store of the window
-    for (uint32_t x = tid; x < nwords; x += 256) dst[x] = win[x];
+    // 5) coalesced store of the window (primes as set bits again)
+    for (uint32_t x = tid; x < nwords; x += 256) dst[x] = ~win[x];
 }
 
 // ---------------------------------------------------------------- checker
