@@ -520,7 +520,10 @@ private:
     bool direct_ = false, force_count_ = false, direct_used_[2] = {false, false};
     bool count_side_[2] = {false, false};   // the slot's counter ran on its side stream
     // the window stage's sieve phase beside the harvest on a second stream (GB_OVERLAP=0: one stream)
+    // GB_OVERLAP=0: one stream for everything (no stage or batch runs
+    // beside another; for profiling and as a fallback)
     bool overlap_ = env::onoff("GB_OVERLAP", true);
+    int main_stream(int slot) const { return overlap_ ? slot : 0; }
     uint64_t region_entries_ = 0;
     uint32_t ngroups_ = 0;
     // GB_BUCKET_VERIFY=N: every Nth segment's map is rebuilt with every large
@@ -951,7 +954,8 @@ void Pipeline::upload_chunks(int slot)
     uint32_t* h = E.h_chunks[slot].as<uint32_t>();
     std::memcpy(h, cp.chunks.data(), nc * 4);
     B.chunks.ensure(c, nc * 4, 0);
-    check(gbh_copy_h2d(c, slot, B.chunks.p, h, nc * 4), "upload chunks");
+    const int ms = main_stream(slot);
+    check(gbh_copy_h2d(c, ms, B.chunks.p, h, nc * 4), "upload chunks");
     if (cp.direct) {
         std::memcpy(h + nc, cp.rdesc.data(), nr * 4);
         std::memcpy(h + nc + nr, cp.groups.data(), ng * 4);
@@ -959,8 +963,8 @@ void Pipeline::upload_chunks(int slot)
         B.groups.ensure(c, ng * 4, 0);
         B.region.ensure(c, cp.region_entries * 4, 0);
         B.fills.ensure(c, size_t(cp.nchunks) * cp.nbb * 4, 0);
-        check(gbh_copy_h2d(c, slot, B.rdesc.p, h + nc, nr * 4), "upload regions");
-        check(gbh_copy_h2d(c, slot, B.groups.p, h + nc + nr, ng * 4), "upload region groups");
+        check(gbh_copy_h2d(c, ms, B.rdesc.p, h + nc, nr * 4), "upload regions");
+        check(gbh_copy_h2d(c, ms, B.groups.p, h + nc + nr, ng * 4), "upload region groups");
     }
     E.slot_plan[slot] = E.cplan_version;
 }
@@ -1037,7 +1041,7 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     auto& B = E.sl[slot];
     // streams: the slot's main stream (uploads, harvest, hit lists, checker,
     // counter) and its side stream (set-up and the window sieve phase)
-    const int ms = slot;
+    const int ms = main_stream(slot);
     const uint32_t nseg = uint32_t(batch.size());
     const TabLayout tl = tab_layout(nseg);
     uint8_t* ht = E.h_tab[slot].as<uint8_t>();
@@ -1244,7 +1248,7 @@ void Pipeline::enqueue_verify(uint32_t s, const gbh_batch& b, gbh_segout* out, i
 {
     gbh_ctx* c = E.c;
     auto& B = E.sl[slot];
-    const int ms = slot;
+    const int ms = main_stream(slot);
     const gbh_batch vb{b.geoms + s, 1, cw_, b.wbits, b.nwin, B.vmap.as<uint64_t>()};
     uint32_t ncoop = 0;
     while (ncoop < nwin_ && E.base[E.win_lo + ncoop] < b.wbits / 128) ++ncoop;
