@@ -1115,14 +1115,23 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
             for (uint32_t s = run.seg0; s < run.seg0 + run.nseg; ++s)
                 for (int x = 0; x < 8; ++x) bits += hg[s].nc[x];
             const double hits = r.density * bits;
-            // the run's bitmap primes: [nwheel_, nwheel_hi_) and [nwheel_hi2_, np)
+            // the run's bitmap primes: [nwheel_, nwheel_hi_) and [nwheel_hi2_, np).
+            // Words a prime must read: below the run's q span one per prime
+            // plus its multiplier range (q span / p / 240); above it, one
+            // only when its single multiple lies in the span and its
+            // multiplier is a unit mod 30 ((8/30) q span / p).
             const uint32_t np = r.nlarge, a0 = std::min(np, nwheel_), a1 = std::min(np, nwheel_hi_),
                            b0 = std::min(np, nwheel_hi2_);
-            const double inv_np = r.inv_large;
-            const double s_bm = (a1 > a0 ? (a1 == np ? inv_np : inv_hi_) - inv_wheel_ : 0.0) +
-                                (np > b0 ? inv_np - inv_hi2_ : 0.0);
-            const double nbm = double(a1 - a0) + double(np - b0);
-            agg_.harvest_bytes += np ? 4.0 * np + 8.0 * (nbm + lq * s_bm / 240) + 4.0 * hits : 0.0;
+            const uint32_t* lp = E.base.data() + E.win_hi;
+            const uint32_t js = uint32_t(std::upper_bound(lp, lp + np, uint32_t(std::min(lq, 4294967295.0))) - lp);
+            double words = 0;
+            for (const auto& [x, y] : {std::pair<uint32_t, uint32_t>{a0, a1}, {b0, np}}) {
+                if (x >= y) continue;
+                const uint32_t m = std::clamp(js, x, y);
+                const double ix = E.inv_at(x), im = E.inv_at(m), iy = E.inv_at(y);
+                words += double(m - x) + lq * (im - ix) / 240 + (8.0 / 30) * lq * (iy - im);
+            }
+            agg_.harvest_bytes += np ? 4.0 * np + 8.0 * words + 4.0 * hits : 0.0;
             bits_all += bits;
             hits_all += hits;
             win_prime_bytes += 4.0 * run.nseg * 8 * r.nwin;
