@@ -194,7 +194,7 @@ bool same_shape(const std::vector<RunCost>& a, const std::vector<RunCost>& b)
 struct ChunkPlan {
     bool valid = false;
     std::vector<RunCost> key;
-    uint32_t nbb = 0, nwheel = 0, nwheel_hi = 0, nwheel_hi2 = 0;
+    uint32_t nbb = 0, nwheel = 0, nwheel_hi = 0, nwheel_hi2 = 0, nrand = 0;
     double ncb = 0;
     std::string mode;
     std::vector<uint32_t> chunks, rdesc, groups;   // the tables uploaded to a slot
@@ -507,6 +507,18 @@ private:
     uint32_t nwheel_hi_ = UINT32_MAX, nwheel_hi2_ = UINT32_MAX, t_hi_ = UINT32_MAX, t_hi2_ = UINT32_MAX;
     double inv_hi_ = 0, inv_hi2_ = 0;   // sum of 1/p over the large primes below t_hi_, below t_hi2_
     bool on_wheel(uint32_t j) const { return j < nwheel_ || (j >= nwheel_hi_ && j < nwheel_hi2_); }
+    // bitmap primes below nrand_ read their words at random (consecutive
+    // primes' words more than a sector apart)
+    uint32_t nrand_ = 0;
+    bool random_reads(uint32_t j) const { return j >= nwheel_ && j < nrand_ && !on_wheel(j); }
+    // the first chunk boundary above a: the wheel / bitmap / band /
+    // random-read boundaries (chunks are all of one kind)
+    uint32_t next_cut(uint32_t a, uint32_t end) const
+    {
+        for (uint32_t c : {nwheel_, nwheel_hi_, nwheel_hi2_, nrand_})
+            if (c > a && c < end) end = c;
+        return end;
+    }
     double inv_wheel_ = 0;             // sum of 1/p over the wheel primes
     double density_max_ = 0;
     std::shared_ptr<const MultBitmap> mb_;
@@ -545,6 +557,7 @@ void Pipeline::plan_bitmap(const PlanBounds& b)
     t_hl_ = UINT32_MAX;
     nwheel_hi_ = nwheel_hi2_ = nlarge_;
     t_hi_ = t_hi2_ = UINT32_MAX;
+    nrand_ = 0;
     mb_.reset();
     if (!nlarge_ || !b.sq_lo) return;
     const uint64_t budget = mult_bitmap_budget();
@@ -581,11 +594,12 @@ void Pipeline::plan_bitmap(const PlanBounds& b)
         const double q = double(sq_lo) * double(sq_lo);
         const double th = hits > 0 ? 0.1918 * qspan / hits : 1e300;
         const double th2 = std::sqrt(q * std::log(std::sqrt(std::max(q, 4.0))) / 1920.0);
+        const auto at = [&](double x) {
+            return uint32_t(std::lower_bound(lp + nwheel_, lp + nlarge_,
+                                             uint32_t(std::min(std::max(x, 1.0), double(UINT32_MAX)))) - lp);
+        };
+        nrand_ = at(th2);
         if (th < th2 && th < double(UINT32_MAX)) {
-            const auto at = [&](double x) {
-                return uint32_t(std::lower_bound(lp + nwheel_, lp + nlarge_,
-                                                 uint32_t(std::min(std::max(x, 1.0), double(UINT32_MAX)))) - lp);
-            };
             nwheel_hi_ = at(th);
             nwheel_hi2_ = at(th2);
             if (nwheel_hi2_ <= nwheel_hi_) {
@@ -774,7 +788,7 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
 {
     ChunkPlan& cp = E.cplan;
     if (cp.valid && cp.nbb == nbb && close_rel(cp.ncb, ncb) && cp.nwheel == nwheel_ && cp.nwheel_hi == nwheel_hi_ &&
-        cp.nwheel_hi2 == nwheel_hi2_ && cp.mode == hl_mode_ && same_shape(cp.key, runs_in)) {
+        cp.nwheel_hi2 == nwheel_hi2_ && cp.nrand == nrand_ && cp.mode == hl_mode_ && same_shape(cp.key, runs_in)) {
         nchunks_ = cp.nchunks;
         stage_ = cp.stage;
         direct_ = cp.direct;
@@ -792,6 +806,9 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
     stage_ = 0;
     if (!rs.empty()) {
         constexpr double kPrimeCost = 4.0;
+        // a bitmap prime whose words are a random read (likely a TLB miss
+        // across the bitmap) costs as much as that many hits
+        constexpr double kRandCost = 24.0;
         const size_t nr = rs.size();
         const uint32_t npmax = rs.back().np;
         // runs [k, nr) are the ones serving primes below rs[k].np: their
@@ -816,12 +833,9 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
                 if (k == 255) b = npmax;
                 while (a < b) {
                     while (rs[ai].np <= a) ++ai;
-                    uint32_t e = std::min(b, rs[ai].np);
-                    if (a < nwheel_) e = std::min(e, nwheel_);
-                    else if (a < nwheel_hi_) e = std::min(e, nwheel_hi_);
-                    else if (a < nwheel_hi2_) e = std::min(e, nwheel_hi2_);
+                    const uint32_t e = next_cut(a, std::min(b, rs[ai].np));
                     const double q = suf_qlo[ai];
-                    total += kPrimeCost * (e - a) * double(nr - ai) +
+                    total += (kPrimeCost + (random_reads(a) ? kRandCost : 0.0)) * (e - a) * double(nr - ai) +
                              suf_bits[ai] * std::max(factor(a, q), factor(e - 1, q)) * (inv(e) - inv(a));
                     a = e;
                 }
@@ -838,18 +852,15 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
         size_t ai = 0;
         while (j < npmax) {
             while (rs[ai].np <= j) ++ai;   // runs [ai, nr) serve prime j
-            uint32_t hi = std::min<uint32_t>(npmax, j + GBH_HL_CHUNK);
-            if (j < nwheel_) hi = std::min(hi, nwheel_);
-            else if (j < nwheel_hi_) hi = std::min(hi, nwheel_hi_);
-            else if (j < nwheel_hi2_) hi = std::min(hi, nwheel_hi2_);
-            hi = std::min(hi, rs[ai].np);
+            const uint32_t hi = next_cut(j, std::min({npmax, j + GBH_HL_CHUNK, rs[ai].np}));
             const double nact = double(nr - ai), bits = suf_bits[ai], bmax = suf_max[ai], q = suf_qlo[ai];
+            const double pc = kPrimeCost + (random_reads(j) ? kRandCost : 0.0);
             const double hf = hit_factor(j, q);
             const double f = on_wheel(j) ? 1.3 * kWheelFrac : hf, fh = hf;
             const double ij = j == nwheel_ ? inv_wheel_ : E.inv_at(j);
             const auto ok = [&](uint32_t e) {   // chunk [j, e) within the targets (the rates barely change inside)
                 const double si = inv(e) - ij;
-                return kPrimeCost * (e - j) * nact + bits * f * si <= target && (!stage_ || bmax * fh * si <= kStageHits);
+                return pc * (e - j) * nact + bits * f * si <= target && (!stage_ || bmax * fh * si <= kStageHits);
             };
             uint32_t e;
             if (ok(hi)) {
@@ -931,6 +942,7 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
     cp.nwheel = nwheel_;
     cp.nwheel_hi = nwheel_hi_;
     cp.nwheel_hi2 = nwheel_hi2_;
+    cp.nrand = nrand_;
     cp.mode = hl_mode_;
     cp.nchunks = nchunks_;
     cp.stage = stage_;
