@@ -845,6 +845,11 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
         // short batches' hits fit the work-groups' LDS stages: chunks of at
         // most kStageHits expected hits per run, one walk per prime
         constexpr double kStageHits = 0.7 * GBH_HL_STAGE;
+        // long batches' direct regions: a chunk's largest (segment, class)
+        // list well inside the partition's LDS (lambda + 8 sqrt(lambda) + 64
+        // <= GBH_PART_CAP at lambda = 0.9 x 15330)
+        constexpr double kRegionHits = 0.9 * 15330;
+        static_assert(GBH_PART_CAP == 16384, "kRegionHits is sized for a 16384-entry partition");
         double all_hits = 0;
         for (const RunCost& r : rs) all_hits += r.bits * r.density;
         stage_ = hl_mode_ != "direct" && all_hits <= kStageHits * 8192 ? GBH_HL_STAGE : 0u;
@@ -860,7 +865,8 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
             const double ij = j == nwheel_ ? inv_wheel_ : E.inv_at(j);
             const auto ok = [&](uint32_t e) {   // chunk [j, e) within the targets (the rates barely change inside)
                 const double si = inv(e) - ij;
-                return pc * (e - j) * nact + bits * f * si <= target && (!stage_ || bmax * fh * si <= kStageHits);
+                return pc * (e - j) * nact + bits * f * si <= target && (!stage_ || bmax * fh * si <= kStageHits) &&
+                       (stage_ || ncb * fh * si <= kRegionHits);
             };
             uint32_t e;
             if (ok(hi)) {
