@@ -5,6 +5,8 @@ passes), the choices that shape the pipeline on a sample workload:
 
     GB_WIN_KB   LDS window of the sieve (0: sized per run; 32, 64, 128 KB)
     GB_WIN_P    bound between the window primes and the large primes
+    GB_WIN_P_LONG  long runs' bound: their large wheel primes below it are
+                sieved in the windows too
     GB_KA       phase-A candidates per even class
     GB_BATCH    segments per batch (ranges of many segments)
     GB_HL_MODE  harvest form of long batches (direct or counted)
@@ -38,14 +40,15 @@ import gbamd  # noqa: E402
 SPACE = {
     "GB_WIN_KB": ["0", "32", "64", "128"],
     "GB_WIN_P": ["262144", "524288", "1048576"],
+    "GB_WIN_P_LONG": ["524288", "1048576", "2097152"],
     "GB_KA": ["64", "96", "128"],
     "GB_BATCH": ["8", "16"],
     "GB_HL_MODE": ["auto", "counted"],
     "GB_OVERLAP": ["1", "0"],
     "GB_HL_WHEEL_HITS": ["0", "2", "4", "8"],
 }
-DEFAULT = {"GB_WIN_KB": "0", "GB_WIN_P": "524288", "GB_KA": "96", "GB_BATCH": "16", "GB_HL_MODE": "auto",
-           "GB_OVERLAP": "1", "GB_HL_WHEEL_HITS": "2"}
+DEFAULT = {"GB_WIN_KB": "0", "GB_WIN_P": "524288", "GB_WIN_P_LONG": "1048576", "GB_KA": "96", "GB_BATCH": "16",
+           "GB_HL_MODE": "auto", "GB_OVERLAP": "1", "GB_HL_WHEEL_HITS": "2"}
 
 
 def band_of(limit: int):

@@ -794,10 +794,10 @@ __device__ __forceinline__ uint32_t first_off_big(uint32_t t, uint32_t wb0, uint
 {
     if (t >= wb0) return t - wb0;
     const uint32_t d = wb0 - t;
-    // d < 2^31, 67 <= p < 2^19: the float quotient is within 7 of d / p, so
-    // |r| < 9 p (exact in float); a second float quotient of r is within one
-    // of r / p; then one step each way. No correction loop: such a loop is
-    // compiled into an integer division.
+    // d < 2^31, 67 <= p < 2^21: the float quotient is within 7 of d / p (within
+    // one and a bit for p >= 2^16), so |r| < 2^24 (exact in float); a second
+    // float quotient of r is within one of r / p; then one step each way. No
+    // correction loop: such a loop is compiled into an integer division.
     int32_t r = (int32_t)(d - (uint32_t)((float)d * rp) * p);
     r -= (int32_t)((float)r * rp) * (int32_t)p;
     if (r < 0) r += (int32_t)p;

@@ -23,6 +23,10 @@ inline constexpr uint32_t CHECK_TAIL = 3000;                    // check30.hpp:4
 inline constexpr uint32_t WIN_KB_DEFAULT = 64;
 // Window primes are [67, WIN_P_DEFAULT); larger primes take the large stage.
 inline constexpr uint64_t WIN_P_DEFAULT = uint64_t(1) << 19;   // window primes below this bound
+// Runs of at least kLongBatch segments per batch also sieve the large wheel
+// primes below WIN_P_LONG_DEFAULT in the windows.
+inline constexpr uint64_t WIN_P_LONG_DEFAULT = uint64_t(1) << 20;
+inline constexpr uint64_t kLongBatch = 8;
 // Phase-A candidates per even class (the rest go to phase B).
 inline constexpr uint32_t KA_DEFAULT = 96;
 

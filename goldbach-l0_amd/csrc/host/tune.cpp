@@ -18,8 +18,8 @@ std::mutex g_mu;
 std::map<std::string, const std::string*>* g_tuned = new std::map<std::string, const std::string*>();
 std::set<std::string>* g_values = new std::set<std::string>();
 
-constexpr const char* kKnobs[] = {"GB_WIN_KB", "GB_WIN_P", "GB_KA", "GB_BATCH", "GB_HL_MODE", "GB_OVERLAP",
-                                  "GB_HL_WHEEL_HITS"};
+constexpr const char* kKnobs[] = {"GB_WIN_KB", "GB_WIN_P", "GB_WIN_P_LONG", "GB_KA", "GB_BATCH", "GB_HL_MODE",
+                                  "GB_OVERLAP", "GB_HL_WHEEL_HITS"};
 
 bool known(const std::string& k)
 {

@@ -4,8 +4,9 @@
 // A tune file holds one band per line:
 //     band <lo> <hi> KNOB=value ...
 // for ranges whose largest limit lies in [lo, hi); '#' starts a comment.
-// The knobs are the pipeline's shape choices (GB_WIN_KB, GB_WIN_P, GB_KA,
-// GB_BATCH, GB_HL_MODE, GB_OVERLAP, GB_HL_WHEEL_HITS); a value set in the
+// The knobs are the pipeline's shape choices (GB_WIN_KB, GB_WIN_P,
+// GB_WIN_P_LONG, GB_KA, GB_BATCH, GB_HL_MODE, GB_OVERLAP, GB_HL_WHEEL_HITS);
+// a value set in the
 // environment always wins over the file, and the values pass the knobs' own strict checks when
 // read. An engine loads GB_TUNE_FILE, when set, for its limit.
 #include <cstdint>

@@ -194,7 +194,7 @@ bool same_shape(const std::vector<RunCost>& a, const std::vector<RunCost>& b)
 struct ChunkPlan {
     bool valid = false;
     std::vector<RunCost> key;
-    uint32_t nbb = 0, nwheel = 0, nwheel_hi = 0, nwheel_hi2 = 0, nrand = 0;
+    uint32_t nbb = 0, nwheel = 0, nwheel_hi = 0, nwheel_hi2 = 0, nrand = 0, nx = 0;
     double ncb = 0;
     std::string mode;
     std::vector<uint32_t> chunks, rdesc, groups;   // the tables uploaded to a slot
@@ -226,7 +226,9 @@ struct Engine::State {
     std::optional<Ctx> ctx;
     gbh_ctx* c = nullptr;
     int cus = 256;
-    Buf d_win, d_large;
+    Buf d_win;                           // window primes, then the large primes (one table)
+    uint32_t* d_large = nullptr;         // the large primes inside d_win
+    uint64_t win_p_long = 0;             // long runs' window bound (GB_WIN_P_LONG)
     // the device buffers of one batch in flight: two sets, so a batch's
     // sieve and harvest run while the batch before it is checked and counted
     struct Slot {
@@ -263,9 +265,11 @@ struct Engine::State {
         base = base_primes_cached(std::max<uint32_t>(base_prime_bound(max_limit), 64), prime_cache, host_threads());
         cache_s = since(tc);
         win_lo = uint32_t(std::lower_bound(base.begin(), base.end(), 67u) - base.begin());
-        // at least 2^16: the hit lists' double-precision quotients q / p
-        // stay within one of the integer quotient only for p >= 2^16
-        const uint64_t win_p = env::u64("GB_WIN_P", WIN_P_DEFAULT, uint64_t(1) << 16, UINT32_MAX);
+        // within [2^16, 2^21]: the hit lists' double-precision quotients q / p
+        // stay within one of the integer quotient only for p >= 2^16, the
+        // window's float remainders are exact for p < 2^21
+        const uint64_t win_p = env::u64("GB_WIN_P", WIN_P_DEFAULT, uint64_t(1) << 16, uint64_t(1) << 21);
+        win_p_long = env::u64("GB_WIN_P_LONG", WIN_P_LONG_DEFAULT, win_p, uint64_t(1) << 21);
         win_hi = std::max(win_lo, uint32_t(std::lower_bound(base.begin(), base.end(), win_p) - base.begin()));
         {
             const size_t nl = base.size() - win_hi;
@@ -286,10 +290,10 @@ struct Engine::State {
         module_s = since(tm);
         const auto tu = clock_::now();
         const uint32_t nwin = win_hi - win_lo, nlarge = uint32_t(base.size()) - win_hi;
-        d_win.ensure(c, size_t(nwin) * 4, 0);
-        d_large.ensure(c, size_t(nlarge) * 4, 0);
-        if (nwin) check(gbh_copy_h2d(c, 0, d_win.p, base.data() + win_lo, size_t(nwin) * 4), "upload window primes");
-        if (nlarge) check(gbh_copy_h2d(c, 0, d_large.p, base.data() + win_hi, size_t(nlarge) * 4), "upload large primes");
+        d_win.ensure(c, size_t(nwin + nlarge) * 4 + 4, 0);
+        d_large = d_win.as<uint32_t>() + nwin;
+        if (nwin + nlarge)
+            check(gbh_copy_h2d(c, 0, d_win.p, base.data() + win_lo, size_t(nwin + nlarge) * 4), "upload primes");
         for (int s = 0; s < 2; ++s) {
             d_out[s].ensure(c, sizeof(gbh_segout) * kbatch, 0);
             h_out[s].ensure(c, sizeof(gbh_segout) * kbatch, 1);
@@ -499,6 +503,11 @@ private:
     bool pending_ = false;
     RunStats agg_;                     // kernel times and bytes over every batch
     uint32_t nwin_ = 0, nlarge_ = 0, ncoop_ = 0;   // the largest range's window / large prime counts
+    // long runs: the first nx_ large primes (wheel primes below
+    // GB_WIN_P_LONG) are sieved in the windows beside the window primes
+    // (their multiples marked in LDS instead of listed); the harvest starts
+    // at large prime nx_
+    uint32_t nx_ = 0;
     uint32_t wbits_ = 0, nwin_max_ = 0, cw_ = 0, cap_ = 0, pcap_ = 0, swords_ = 0, kb_ = 1;
     uint64_t span_ = 0, evens_cap_ = UINT64_MAX;
     // large primes: [0, nwheel_) and the band [nwheel_hi_, nwheel_hi2_) walk the wheel, the others read
@@ -637,8 +646,9 @@ double Pipeline::hit_factor(uint32_t j, double q_lo) const
 // of the prime index, each at its largest rate.
 double Pipeline::density_of(uint64_t q_lo, uint32_t nlarge) const
 {
-    if (!nlarge) return 0.0;
-    double d = kWheelFrac * E.inv_at(nlarge);     // every prime at the wheel's rate ...
+    if (nlarge <= nx_) return 0.0;
+    // every listed prime (from nx_ on) at the wheel's rate ...
+    double d = kWheelFrac * (E.inv_at(nlarge) - E.inv_at(nx_));
     // ... except the bitmap's intervals, in 32 bands each at the band's largest rate
     const uint32_t iv[2][2] = {{std::min(nwheel_, nlarge), std::min(nwheel_hi_, nlarge)},
                                {std::min(nwheel_hi2_, nlarge), nlarge}};
@@ -668,6 +678,12 @@ void Pipeline::plan(const PlanBounds& pb)
     nwin_ = pb.nwin;
     span_ = pb.span;
     const uint64_t nseg = std::min<uint64_t>(multi_ ? GBH_MAX_BATCH : E.kbatch, pb.nseg);
+    nx_ = 0;
+    if (!multi_ && nseg >= kLongBatch && nwin_ == E.win_hi - E.win_lo && nlarge_) {
+        const uint32_t* lp = E.base.data() + E.win_hi;
+        const uint32_t n = uint32_t(std::lower_bound(lp, lp + nlarge_, uint32_t(E.win_p_long)) - lp);
+        nx_ = std::min(n, nwheel_);
+    }
     // the largest segment: the merged last one spans below 2 x span
     const uint64_t qodds_max = (2 * span_ + MAX_GB_PRIME) / 2 + 2;
     const uint64_t nc_max = (qodds_max + 14) / 15 + 1;
@@ -714,7 +730,7 @@ void Pipeline::plan(const PlanBounds& pb)
     const size_t K = kb_;
     for (auto& B : E.sl) {
         B.map.ensure(c, K * 8 * cw_ * 8, 0);
-        B.i0.ensure(c, K * 8 * size_t(std::max(nwin_, 1u)) * 4, 0);
+        B.i0.ensure(c, K * 8 * size_t(std::max(nwin_ + nx_, 1u)) * 4, 0);
         B.pending.ensure(c, K * size_t(pcap_) * 4, 0);
         B.scratch.ensure(c, K * size_t(swords_) * 4, 0);
         B.hit_n.ensure(c, K * 8 * size_t(nwin_max_) * 4, 0);
@@ -735,9 +751,10 @@ void Pipeline::plan(const PlanBounds& pb)
         if (phase_trace_) std::fprintf(stderr, "plan: chunking %.3f ms\n", since(tp0) * 1e3);
     }
     if (phase_trace_)
-        std::fprintf(stderr, "plan: %u segments per batch, window %u bits, %u window primes, %u large (%u on the wheel, "
-                     "bitmap from %u, the wheel again in [%u, %u)), %u harvest chunks, list cap %u, stage %u\n", kb_,
-                     wbits_, nwin_, nlarge_, nwheel_ + (std::min(nwheel_hi2_, nlarge_) - std::min(nwheel_hi_, nlarge_)),
+        std::fprintf(stderr, "plan: %u segments per batch, window %u bits, %u window primes, %u large (%u in the "
+                     "windows, %u on the wheel, bitmap from %u, the wheel again in [%u, %u)), %u harvest chunks, list "
+                     "cap %u, stage %u\n", kb_, wbits_, nwin_, nlarge_, nx_,
+                     nwheel_ - std::min(nx_, nwheel_) + (std::min(nwheel_hi2_, nlarge_) - std::min(nwheel_hi_, nlarge_)),
                      t_hl_ == UINT32_MAX ? 0u : t_hl_, t_hi_ == UINT32_MAX ? 0u : t_hi_,
                      t_hi2_ == UINT32_MAX ? 0u : t_hi2_, nchunks_, cap_, stage_);
     verify_every_ = env::present("GB_BUCKET_VERIFY") ? env::u32("GB_BUCKET_VERIFY", 0, 0) : 0u;
@@ -788,7 +805,8 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
 {
     ChunkPlan& cp = E.cplan;
     if (cp.valid && cp.nbb == nbb && close_rel(cp.ncb, ncb) && cp.nwheel == nwheel_ && cp.nwheel_hi == nwheel_hi_ &&
-        cp.nwheel_hi2 == nwheel_hi2_ && cp.nrand == nrand_ && cp.mode == hl_mode_ && same_shape(cp.key, runs_in)) {
+        cp.nwheel_hi2 == nwheel_hi2_ && cp.nrand == nrand_ && cp.nx == nx_ && cp.mode == hl_mode_ &&
+        same_shape(cp.key, runs_in)) {
         nchunks_ = cp.nchunks;
         stage_ = cp.stage;
         direct_ = cp.direct;
@@ -797,7 +815,7 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
         return;
     }
     cp.valid = false;
-    chunks_.assign(1, 0u);
+    chunks_.assign(1, nx_);   // large primes below nx_ are sieved in the windows
     std::vector<double> hpb;    // per chunk: expected hits per class bit (the larger rate of its ends)
     std::vector<RunCost> rs;
     for (const RunCost& r : runs_in)
@@ -831,6 +849,7 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
             for (uint32_t k = 0; k < 256; ++k) {
                 uint32_t a = uint32_t(uint64_t(npmax) * k / 256) & ~255u, b = uint32_t(uint64_t(npmax) * (k + 1) / 256) & ~255u;
                 if (k == 255) b = npmax;
+                a = std::max(a, nx_);
                 while (a < b) {
                     while (rs[ai].np <= a) ++ai;
                     const uint32_t e = next_cut(a, std::min(b, rs[ai].np));
@@ -853,7 +872,7 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
         double all_hits = 0;
         for (const RunCost& r : rs) all_hits += r.bits * r.density;
         stage_ = hl_mode_ != "direct" && all_hits <= kStageHits * 8192 ? GBH_HL_STAGE : 0u;
-        uint32_t j = 0, prev = 0;   // prev: the last chunk's length (lengths change slowly: start the search there)
+        uint32_t j = nx_, prev = 0;   // prev: the last chunk's length (lengths change slowly: start the search there)
         size_t ai = 0;
         while (j < npmax) {
             while (rs[ai].np <= j) ++ai;   // runs [ai, nr) serve prime j
@@ -949,6 +968,7 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
     cp.nwheel_hi = nwheel_hi_;
     cp.nwheel_hi2 = nwheel_hi2_;
     cp.nrand = nrand_;
+    cp.nx = nx_;
     cp.mode = hl_mode_;
     cp.nchunks = nchunks_;
     cp.stage = stage_;
@@ -1095,7 +1115,7 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         g.n_gpu = n_gpu;
         j.n_gpu = n_gpu;
         g.hist_on = j.sampled ? 1u : 0u;
-        g.np_win = r.nwin;
+        g.np_win = r.nwin + std::min(nx_, r.nlarge);
         // ridge records: a campaign run records from its store's current
         // threshold; a run without campaign files from the floor (the same
         // records however its segments are batched)
@@ -1149,10 +1169,10 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
                 const double ix = E.inv_at(x), im = E.inv_at(m), iy = E.inv_at(y);
                 words += double(m - x) + lq * (im - ix) / 240 + (8.0 / 30) * lq * (iy - im);
             }
-            agg_.harvest_bytes += np ? 4.0 * np + 8.0 * words + 4.0 * hits : 0.0;
+            agg_.harvest_bytes += np > nx_ ? 4.0 * (np - nx_) + 8.0 * words + 4.0 * hits : 0.0;
             bits_all += bits;
             hits_all += hits;
-            win_prime_bytes += 4.0 * run.nseg * 8 * r.nwin;
+            win_prime_bytes += 4.0 * run.nseg * 8 * (r.nwin + std::min(nx_, r.nlarge));
             costs.push_back(RunCost{np, bits, double(r.q_lo), r.density});
         }
         agg_.window_bytes += bits_all / 8 + win_prime_bytes + 4.0 * hits_all;
@@ -1183,7 +1203,7 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         // direct form needs it only for the partition (zeroed below)
         if (direct) check(gbh_zero(c, ms, B.fills.p, size_t(nchunks_) * nseg * 8 * 4), "zero region fills");
         else check(gbh_zero(c, ms, B.hit_n.p, size_t(nseg) * 8 * nwin * 4), "zero hit counts");
-        gbh_harvest_args h{b, E.d_large.as<uint32_t>(), B.chunks.as<uint32_t>(), dr, nruns,
+        gbh_harvest_args h{b, E.d_large, B.chunks.as<uint32_t>(), dr, nruns,
                            run_max, nchunks_, cap_, t_hl_, t_hi_, t_hi2_, direct ? 0u : stage_,
                            mb_ ? mb_->bits : nullptr,
                            mb_ ? mb_->m_base : 0, mb_ ? mb_->m_span : 0, B.hits.as<uint32_t>(),
@@ -1195,13 +1215,14 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         check(gbh_event_record(c, ms, ev[EV_HL1].e), "event");
     }
     if (ss != ms) check(gbh_stream_wait(c, ss, ev[EV_GEOM].e), "stream wait");
-    if (nwin_) {
-        gbh_setup_args a{b, E.d_win.as<uint32_t>(), nwin_, 0, B.i0.as<uint32_t>()};
+    const uint32_t nwp = nwin_ + nx_;   // the windows' primes
+    if (nwp) {
+        gbh_setup_args a{b, E.d_win.as<uint32_t>(), nwp, 0, B.i0.as<uint32_t>()};
         check(gbh_launch_setup(c, ss, &a), "setup");
     }
     check(gbh_event_record(c, ss, ev[EV_S0].e), "event");
     {
-        const gbh_window_args w{b, E.d_win.as<uint32_t>(), nwin_, ncoop, B.i0.as<uint32_t>(), nullptr, nullptr, 0, 1};
+        const gbh_window_args w{b, E.d_win.as<uint32_t>(), nwp, ncoop, B.i0.as<uint32_t>(), nullptr, nullptr, 0, 1};
         check(gbh_launch_window(c, ss, &w), "window sieve");
     }
     check(gbh_event_record(c, ss, ev[EV_S1].e), "event");
@@ -1215,7 +1236,7 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     check(gbh_event_record(c, ms, ev[EV_SETUP].e), "event");
     if (ss != ms) check(gbh_stream_wait(c, ms, ev[EV_S1].e), "stream wait");
     if (nlarge_) {   // the hit-list phase: every window's list applied to the sieved map
-        const gbh_window_args w{b, E.d_win.as<uint32_t>(), nwin_, ncoop, B.i0.as<uint32_t>(),
+        const gbh_window_args w{b, E.d_win.as<uint32_t>(), nwp, ncoop, B.i0.as<uint32_t>(),
                                 B.hits.as<uint32_t>(), B.hit_n.as<uint32_t>(), cap_, 2};
         check(gbh_launch_window(c, ms, &w), "window hit lists");
     }
@@ -1284,7 +1305,7 @@ void Pipeline::enqueue_verify(uint32_t s, const gbh_batch& b, gbh_segout* out, i
         check(gbh_launch_setup(c, ms, &a), "verify setup");
     }
     check(gbh_zero(c, ms, B.vhit_n.p, size_t(8) * b.nwin * 4), "verify zero");
-    const gbh_harvest_args h{vb, E.d_large.as<uint32_t>(), B.vchunks.as<uint32_t>(), nullptr, 0, 0, nvchunks_, cap_v_,
+    const gbh_harvest_args h{vb, E.d_large, B.vchunks.as<uint32_t>(), nullptr, 0, 0, nvchunks_, cap_v_,
                              UINT32_MAX, UINT32_MAX, UINT32_MAX, 0, nullptr, 0, 0, B.vhits.as<uint32_t>(), B.vhit_n.as<uint32_t>(),
                              nullptr, nullptr, nullptr, &out[s].err};
     check(gbh_launch_harvest(c, ms, &h), "verify harvest");

@@ -25,7 +25,7 @@ def stats_env(tmp_path, monkeypatch):
     monkeypatch.setenv("GB_STATS_LIVE", str(tmp_path / "live.json"))
     monkeypatch.setenv("GB_RIDGE_FILE", str(tmp_path / "ridges.csv"))
     monkeypatch.setenv("GB_RECORD_HOOK", "")
-    for k in ("GB_STATS", "GB_GAPS", "GB_STATS_SAMPLE", "GB_WIN_KB", "GB_WIN_P", "GB_KA", "GB_BATCH", "GB_HL_MODE",
-              "GB_OVERLAP", "GB_TUNE_FILE", "GB_HL_WHEEL_HITS"):
+    for k in ("GB_STATS", "GB_GAPS", "GB_STATS_SAMPLE", "GB_WIN_KB", "GB_WIN_P", "GB_WIN_P_LONG", "GB_KA", "GB_BATCH",
+              "GB_HL_MODE", "GB_OVERLAP", "GB_TUNE_FILE", "GB_HL_WHEEL_HITS"):
         monkeypatch.delenv(k, raising=False)
     return tmp_path

@@ -1,7 +1,7 @@
 """The kernels' loop-free quotient corrections, restated in numpy with the
 device's float formats, against exact integer arithmetic.
 
-* Window sieve (`first_off_big`): d < 2^31, 67 <= p < 2^19, a float32
+* Window sieve (`first_off_big`): d < 2^31, 67 <= p < 2^21, a float32
   quotient d * rcp(p), the residue's own float32 quotient, then one step each
   way, must give d mod p exactly for a reciprocal within one ulp of 1/p.
 * Harvest (`bm_setup`, `bm_single`): floor(qb0 / p) from a double quotient
@@ -37,7 +37,7 @@ def _first_off_residue(d, p, rp, second=True):
 
 def test_window_remainder_random():
     n = 2_000_000
-    p = RNG.integers(67, 1 << 19, n, dtype=np.uint64).astype(np.uint32)
+    p = RNG.integers(67, 1 << 21, n, dtype=np.uint64).astype(np.uint32)
     p[::2] = RNG.integers(67, 2000, n // 2, dtype=np.uint64).astype(np.uint32)
     d = RNG.integers(0, 1 << 31, n, dtype=np.uint64).astype(np.uint32)
     for k in (-1, 0, 1):
@@ -51,7 +51,7 @@ def test_window_remainder_random():
 
 
 def test_window_remainder_edges():
-    ps = np.array([67, 71, 73, 79, 97, 4093, 4099, 131071, 524269, 524287], dtype=np.uint32)
+    ps = np.array([67, 71, 73, 79, 97, 4093, 4099, 131071, 524269, 524287, 1048573, 2097143], dtype=np.uint32)
     d_hi = np.arange((1 << 31) - 200_000, 1 << 31, dtype=np.uint64).astype(np.uint32)
     d_lo = np.arange(0, 200_000, dtype=np.uint32)
     for p0 in ps:

@@ -243,3 +243,14 @@ def test_long_batches_direct_harvest_maps_verified(stats_env, monkeypatch):
     _say("long counted case")
     r2 = gbamd.run_gpu(b, 467_500_000, start_even=a)
     assert compare.strip_version(r2["stats_line"]) == compare.strip_version(r["stats_line"])
+    # the long runs' window extension (large wheel primes below 2^20 sieved
+    # in the windows) off: the same statistics
+    monkeypatch.delenv("GB_HL_MODE")
+    monkeypatch.setenv("GB_WIN_P_LONG", "524288")
+    state = (stats_env / "st.state").read_text()
+    for f in ("st.state", "st.log", "st.state.gaps"):
+        (stats_env / f).unlink(missing_ok=True)
+    _say("long case without the window extension")
+    r3 = gbamd.run_gpu(b, 467_500_000, start_even=a)
+    assert compare.strip_version(r3["stats_line"]) == compare.strip_version(r["stats_line"])
+    assert (stats_env / "st.state").read_text() == state
