@@ -865,10 +865,10 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
         // most kStageHits expected hits per run, one walk per prime
         constexpr double kStageHits = 0.7 * GBH_HL_STAGE;
         // long batches' direct regions: a chunk's largest (segment, class)
-        // list well inside the partition's LDS (lambda + 8 sqrt(lambda) + 64
-        // <= GBH_PART_CAP at lambda = 0.9 x 15330)
-        constexpr double kRegionHits = 0.9 * 15330;
-        static_assert(GBH_PART_CAP == 16384, "kRegionHits is sized for a 16384-entry partition");
+        // list well inside the partition's LDS (0.9 of the lambda with
+        // lambda + 8 sqrt(lambda) + 64 = GBH_PART_CAP)
+        const double x_cap = (-8.0 + std::sqrt(64.0 + 4.0 * (GBH_PART_CAP - 64))) / 2;
+        const double kRegionHits = 0.9 * x_cap * x_cap;
         double all_hits = 0;
         for (const RunCost& r : rs) all_hits += r.bits * r.density;
         stage_ = hl_mode_ != "direct" && all_hits <= kStageHits * 8192 ? GBH_HL_STAGE : 0u;

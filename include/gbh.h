@@ -203,7 +203,7 @@ typedef struct {
  * GBH_PART_CHUNKS chunks whose regions (rdesc[2k + 1] entries per list)
  * hold at most GBH_PART_CAP entries together. */
 #define GBH_PART_CHUNKS 64
-#define GBH_PART_CAP 16384
+#define GBH_PART_CAP 8192
 typedef struct {
     gbh_batch b;
     const uint32_t* rdesc;
