@@ -310,9 +310,16 @@ int gbh_launch_partition(gbh_ctx* c, int s, const gbh_partition_args* a)
     if (!a->rdesc || !a->region || !a->fills || !a->groups || !a->hits || !a->hit_n)
         return fail(GBH_E_ARG, "partition: null argument");
     if (a->b.nwin > 256) return fail(GBH_E_ARG, "partition: more than 256 windows per class");
-    hipLaunchKernelGGL(gbh_dev::k_partition, dim3(a->b.nseg * 8, a->ngroups), dim3(512), 0, S(c, s), a->rdesc,
-                       a->region, a->fills, a->groups, a->b.nseg * 8, a->b.nwin, a->b.wbits, a->cap, a->hits,
-                       a->hit_n, a->err);
+    if (a->part_cap == GBH_PART_CAP_SMALL)
+        hipLaunchKernelGGL(gbh_dev::k_partition<GBH_PART_CAP_SMALL>, dim3(a->b.nseg * 8, a->ngroups), dim3(512), 0,
+                           S(c, s), a->rdesc, a->region, a->fills, a->groups, a->b.nseg * 8, a->b.nwin, a->b.wbits,
+                           a->cap, a->hits, a->hit_n, a->err);
+    else if (a->part_cap == GBH_PART_CAP)
+        hipLaunchKernelGGL(gbh_dev::k_partition<GBH_PART_CAP>, dim3(a->b.nseg * 8, a->ngroups), dim3(512), 0, S(c, s),
+                           a->rdesc, a->region, a->fills, a->groups, a->b.nseg * 8, a->b.nwin, a->b.wbits, a->cap,
+                           a->hits, a->hit_n, a->err);
+    else
+        return fail(GBH_E_ARG, "partition: part_cap must be GBH_PART_CAP_SMALL or GBH_PART_CAP");
     GBH_HIP(hipGetLastError());
     return GBH_OK;
 }

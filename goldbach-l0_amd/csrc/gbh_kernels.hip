@@ -658,11 +658,12 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
 // Direct harvest, second step: the (segment, class) regions every chunk's
 // work-group filled are split into the window lists. Work-group (bk, g):
 // bucket bk = s * 8 + c of the batch, the chunks [groups[g], groups[g + 1])
-// (at most GBH_PART_CHUNKS, their regions at most GBH_PART_CAP entries in
-// all: planned by the host). The entries are read once into LDS (the chunks'
+// (at most GBH_PART_CHUNKS, their regions at most kCap entries in all:
+// planned by the host). The entries are read once into LDS (the chunks'
 // fill counts scanned so every thread's loads are independent) and counted
 // per window; one reservation per window list; then the writes from LDS
 // (each window list's share lands in consecutive slots).
+template <uint32_t kCap>
 __global__ void __launch_bounds__(512) k_partition(const uint32_t* __restrict__ rdesc,
                                                    const uint32_t* __restrict__ region,
                                                    const uint32_t* __restrict__ fills,
@@ -670,7 +671,7 @@ __global__ void __launch_bounds__(512) k_partition(const uint32_t* __restrict__ 
                                                    uint32_t wbits, uint32_t cap, uint32_t* __restrict__ hits,
                                                    uint32_t* __restrict__ hit_n, uint32_t* __restrict__ err)
 {
-    __shared__ uint32_t ent[GBH_PART_CAP];
+    __shared__ uint32_t ent[kCap];
     __shared__ uint32_t wc[256], wbase[256], pre[GBH_PART_CHUNKS + 1], rb[GBH_PART_CHUNKS];
     const uint32_t bk = blockIdx.x, tid = threadIdx.x;
     const uint32_t k0 = groups[blockIdx.y], nk = min(groups[blockIdx.y + 1] - k0, (uint32_t)GBH_PART_CHUNKS);
@@ -690,7 +691,7 @@ __global__ void __launch_bounds__(512) k_partition(const uint32_t* __restrict__ 
     }
     __syncthreads();
     const uint32_t total = pre[nk];
-    if (total > GBH_PART_CAP) {              // beyond the host's plan: fail closed
+    if (total > kCap) {                      // beyond the host's plan: fail closed
         if (tid == 0) atomicOr(err, 1u);
         return;
     }

@@ -199,7 +199,7 @@ struct ChunkPlan {
     std::string mode;
     std::vector<uint32_t> chunks, rdesc, groups;   // the tables uploaded to a slot
     uint64_t region_entries = 0;
-    uint32_t nchunks = 0, stage = 0, ngroups = 0;
+    uint32_t nchunks = 0, stage = 0, ngroups = 0, part_cap = GBH_PART_CAP;
     bool direct = false;
 };
 
@@ -546,7 +546,7 @@ private:
     bool overlap_ = env::onoff("GB_OVERLAP", true);
     int main_stream(int slot) const { return overlap_ ? slot : 0; }
     uint64_t region_entries_ = 0;
-    uint32_t ngroups_ = 0;
+    uint32_t ngroups_ = 0, part_cap_ = GBH_PART_CAP;   // the partition: groups and their LDS entries
     // GB_BUCKET_VERIFY=N: every Nth segment's map is rebuilt with every large
     // prime on the wheel walk (no multiplier bitmap) and compared word for word
     uint32_t verify_every_ = 0, cap_v_ = 0, nvchunks_ = 0;
@@ -811,6 +811,7 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
         stage_ = cp.stage;
         direct_ = cp.direct;
         ngroups_ = cp.ngroups;
+        part_cap_ = cp.part_cap;
         upload_chunks(slot);   // unless the slot's tables hold it already
         return;
     }
@@ -865,9 +866,10 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
         // most kStageHits expected hits per run, one walk per prime
         constexpr double kStageHits = 0.7 * GBH_HL_STAGE;
         // long batches' direct regions: a chunk's largest (segment, class)
-        // list well inside the partition's LDS (0.9 of the lambda with
-        // lambda + 8 sqrt(lambda) + 64 = GBH_PART_CAP)
-        const double x_cap = (-8.0 + std::sqrt(64.0 + 4.0 * (GBH_PART_CAP - 64))) / 2;
+        // list well inside the small partition's LDS (0.9 of the lambda with
+        // lambda + 8 sqrt(lambda) + 64 = GBH_PART_CAP_SMALL; chunks at their
+        // 256-prime minimum may exceed it: then the large partition)
+        const double x_cap = (-8.0 + std::sqrt(64.0 + 4.0 * (GBH_PART_CAP_SMALL - 64))) / 2;
         const double kRegionHits = 0.9 * x_cap * x_cap;
         double all_hits = 0;
         for (const RunCost& r : rs) all_hits += r.bits * r.density;
@@ -925,8 +927,7 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
     groups.clear();
     if (direct_) {
         rd.assign(2 * size_t(nchunks_), 0u);
-        groups.assign(1, 0u);
-        uint64_t base = 0, gsum = 0, rmax = 0;
+        uint64_t base = 0, rmax = 0;
         for (uint32_t k = 0; k < nchunks_; ++k) {
             const double lam = ncb * hpb[k];
             const uint64_t r = uint64_t(lam + 8 * std::sqrt(lam) + 64);
@@ -941,7 +942,14 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
                 direct_ = false;
                 break;
             }
-            if (k > groups.back() && (k - groups.back() >= GBH_PART_CHUNKS || gsum + r > GBH_PART_CAP)) {
+        }
+        // the partition's groups: the small work-groups when every region fits
+        part_cap_ = rmax <= GBH_PART_CAP_SMALL ? GBH_PART_CAP_SMALL : GBH_PART_CAP;
+        groups.assign(1, 0u);
+        uint64_t gsum = 0;
+        for (uint32_t k = 0; direct_ && k < nchunks_; ++k) {
+            const uint64_t r = rd[2 * k + 1];
+            if (k > groups.back() && (k - groups.back() >= GBH_PART_CHUNKS || gsum + r > part_cap_)) {
                 groups.push_back(k);
                 gsum = 0;
             }
@@ -953,7 +961,8 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
             ngroups_ = uint32_t(groups.size() - 1);
             if (phase_trace_)
                 std::fprintf(stderr, "plan: direct harvest: %llu region entries, at most %llu per list, %u partition "
-                             "groups\n", (unsigned long long)base, (unsigned long long)rmax, ngroups_);
+                             "groups of up to %u entries\n", (unsigned long long)base, (unsigned long long)rmax,
+                             ngroups_, part_cap_);
         } else {
             rd.clear();
             groups.clear();
@@ -974,6 +983,7 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
     cp.stage = stage_;
     cp.direct = direct_;
     cp.ngroups = ngroups_;
+    cp.part_cap = part_cap_;
     cp.valid = true;
     ++E.cplan_version;
     upload_chunks(slot);
@@ -1230,7 +1240,7 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         check(gbh_zero(c, ms, B.hit_n.p, size_t(nseg) * 8 * nwin * 4), "zero hit counts");
         const gbh_partition_args pa{b, B.rdesc.as<uint32_t>(), B.region.as<uint32_t>(), B.fills.as<uint32_t>(),
                                     B.groups.as<uint32_t>(), ngroups_, cap_, B.hits.as<uint32_t>(),
-                                    B.hit_n.as<uint32_t>(), &out[0].err};
+                                    B.hit_n.as<uint32_t>(), &out[0].err, part_cap_, 0};
         check(gbh_launch_partition(c, ms, &pa), "partition");
     }
     check(gbh_event_record(c, ms, ev[EV_SETUP].e), "event");

@@ -201,9 +201,11 @@ typedef struct {
  * the window lists hits / hit_n of gbh_harvest_args (hit_n zeroed before).
  * The chunks go in groups [groups[g], groups[g + 1]) of at most
  * GBH_PART_CHUNKS chunks whose regions (rdesc[2k + 1] entries per list)
- * hold at most GBH_PART_CAP entries together. */
+ * hold at most part_cap entries together: GBH_PART_CAP_SMALL (32 KB of LDS
+ * per work-group, more of them per CU) or GBH_PART_CAP. */
 #define GBH_PART_CHUNKS 64
-#define GBH_PART_CAP 8192
+#define GBH_PART_CAP_SMALL 8192
+#define GBH_PART_CAP 16384
 typedef struct {
     gbh_batch b;
     const uint32_t* rdesc;
@@ -215,6 +217,8 @@ typedef struct {
     uint32_t* hits;
     uint32_t* hit_n;
     uint32_t* err;                 /* bit 0: a window list overflow */
+    uint32_t part_cap;             /* GBH_PART_CAP_SMALL or GBH_PART_CAP */
+    uint32_t pad0;
 } gbh_partition_args;
 
 /* Map self-check (reference GB_BUCKET_VERIFY compare kernel,
