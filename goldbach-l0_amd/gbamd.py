@@ -132,11 +132,13 @@ class Engine:
                                      len(err)):
             raise VerificationError(err.value.decode())
         self.init_seconds = init.value
+        # the result text buffers, allocated once (a run writes a terminated string)
+        self._line = ctypes.create_string_buffer(1 << 20)
+        self._err = ctypes.create_string_buffer(4096)
 
     def run(self, limit: int, segment_even: int = 467_500_000, start_even: int = 4) -> dict:
         res = GbaRunResult()
-        line = ctypes.create_string_buffer(1 << 20)
-        err = ctypes.create_string_buffer(4096)
+        line, err = self._line, self._err
         if self._lib.gba_engine_run(self._h, int(limit), int(segment_even), int(start_even), ctypes.byref(res), line,
                                     len(line), err, len(err)):
             raise VerificationError(err.value.decode())
@@ -189,6 +191,9 @@ class Batcher:
             raise VerificationError(err.value.decode())
         self.batch_segments = kb.value
         self._ids = (ctypes.c_uint64 * 1024)()
+        self._line = ctypes.create_string_buffer(1 << 20)
+        self._state = ctypes.create_string_buffer(1 << 20)
+        self._err = ctypes.create_string_buffer(4096)
 
     def submit(self, start_even: int, limit: int) -> int:
         rid = ctypes.c_uint64(0)
@@ -201,7 +206,7 @@ class Batcher:
         """One pipeline step: (ids of ranges completed, whether work remains)."""
         n = ctypes.c_uint32(0)
         more = ctypes.c_int(0)
-        err = ctypes.create_string_buffer(4096)
+        err = self._err
         if self._lib.gba_batcher_step(self._h, self._ids, len(self._ids), ctypes.byref(n), ctypes.byref(more), err,
                                       len(err)):
             raise VerificationError(err.value.decode())
@@ -209,9 +214,7 @@ class Batcher:
 
     def result(self, rid: int) -> dict:
         res = GbaRunResult()
-        line = ctypes.create_string_buffer(1 << 20)
-        state = ctypes.create_string_buffer(1 << 20)
-        err = ctypes.create_string_buffer(4096)
+        line, state, err = self._line, self._state, self._err
         if self._lib.gba_batcher_result(self._h, int(rid), ctypes.byref(res), line, len(line), state, len(state),
                                         err, len(err)):
             raise VerificationError(err.value.decode())
