@@ -363,7 +363,9 @@ int gbh_launch_check_a(gbh_ctx* c, int s, const gbh_check_args* a)
     }
     // sampled segments (histogram on) and the others: one launch each, every
     // work-group leaves at once on the other kind of segment
-    const dim3 grid(grid_of(words, gbh_dev::kBlockWords, 1u << 30), a->b.nseg);
+    const uint32_t cg = a->class_groups ? a->class_groups : 1u;
+    if (GBH_EVEN_CLASSES % cg) return fail(GBH_E_ARG, "check: class_groups must divide the 15 even classes");
+    const dim3 grid(grid_of(words, gbh_dev::kBlockWords, 1u << 30), a->b.nseg, cg);
     if (a->kinds & 1u) {
         hipLaunchKernelGGL(gbh_dev::k_check_a<false>, grid, dim3(GBH_CHECK_WG), 0, S(c, s), a->b, a->coffa, a->ka,
                            a->pending, a->pcap, a->out);

@@ -1034,7 +1034,9 @@ __global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_batch b, const uin
     const uint32_t word = w0 + wv * kWaveWords + lane;     // this lane's word of every class array
     const uint32_t* sb = st + wv * kWaveWords + lane;
     uint32_t resolved = 0;
-    for (uint32_t cr = 0; cr < GBH_EVEN_CLASSES; ++cr) {
+    // the work-group's even classes: all 15, or its share of a split
+    const uint32_t ncr = GBH_EVEN_CLASSES / gridDim.z, cr0 = blockIdx.z * ncr;
+    for (uint32_t cr = cr0; cr < cr0 + ncr; ++cr) {
         const ClassEvens e = class_evens(g, cr);
         const uint32_t k0 = word * 64;
         uint64_t valid = 0;

@@ -1259,8 +1259,11 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     if (check_and_count) {
         uint32_t kinds = 0;
         for (const SegJob& j : batch) kinds |= j.sampled ? 2u : 1u;
+        // short batches: the 15 even classes over 3 work-groups (three times
+        // the waves; each stages the same rows)
+        const uint32_t cgroups = nseg < 4 ? 3u : 1u;
         gbh_check_args ca{b, E.ka, pcap_, B.coff.as<uint32_t>(), B.coffa.as<uint32_t>(),
-                          B.pending.as<uint32_t>(), out, kinds, 0};
+                          B.pending.as<uint32_t>(), out, kinds, cgroups};
         if (kinds == 3 && count_side_[slot]) {
             // both kinds: the histogram segments on the side stream (ahead
             // of the counter), the others here, phase B after both

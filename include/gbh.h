@@ -283,7 +283,9 @@ typedef struct {
     uint32_t kinds;                /* bit 0: the batch has segments without histogram, bit 1: with;
                                       bit 2: the candidate tables coff / coffa are already built
                                       for this batch (an earlier call), skip k_coff */
-    uint32_t pad0;
+    uint32_t class_groups;         /* 0 or 1: a work-group checks all 15 even classes of its words;
+                                      3 or 5: the classes split over that many work-groups (more
+                                      waves for short batches) */
 } gbh_check_args;
 
 /* Stage 5: phase B (reference check30_b, goldbach.cl:2210-2306): the
