@@ -403,7 +403,10 @@ int gbh_launch_count(gbh_ctx* c, int s, const gbh_count_args* a)
     if (a->swords < 2ull * a->nwords) return fail(GBH_E_ARG, "count scratch too small");
     if (uint64_t(a->nwords) + 1 > a->b.cw) return fail(GBH_E_ARG, "count words exceed the class stride");
     if (!a->nwords) return GBH_OK;
-    const unsigned grid = grid_of(a->nwords, 256, 1u << 30);
+    // about eight work-groups per CU over the batch, each over several words
+    // (one histogram flush per work-group, not per 256 words)
+    const unsigned grid = std::max(1u, std::min(grid_of(a->nwords, 256, 1u << 30),
+                                                (8u * uint32_t(std::max(c->info.cus, 1)) + a->b.nseg - 1) / a->b.nseg));
     hipLaunchKernelGGL(gbh_dev::k_count_a, dim3(grid, a->b.nseg), dim3(256), 0, S(c, s), a->b, a->cs, a->scratch,
                        a->swords, a->out);
     GBH_HIP(hipGetLastError());

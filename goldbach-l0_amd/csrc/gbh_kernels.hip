@@ -1220,24 +1220,25 @@ __global__ void __launch_bounds__(256) k_count_a(gbh_batch b, const gbh_countseg
     for (uint32_t i = tid; i <= GBH_GAP_BINS; i += 256) hl[i] = 0;
     if (tid < 11) cs[tid] = 0;
     __syncthreads();
-    const uint32_t x = blockIdx.x * 256 + tid;
     uint64_t best = 0;
     uint32_t pcs[11];                       // primes per class, twins per pair: reduced per wave below
 #pragma unroll
     for (int k = 0; k < 11; ++k) pcs[k] = 0;
-    if (x < a.nwords) {
+    // words x, x + grid, ...: the work-group's histogram and counters are
+    // flushed once for all of them
+    for (uint32_t x = blockIdx.x * 256 + tid; x < a.nwords; x += gridDim.x * 256) {
         uint64_t w[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             w[k] = map[(size_t)a.cls[k] * b.cw + x] & range_mask(x, a.lo[k], a.hi[k]);
-            pcs[k] = (uint32_t)__popcll(w[k]);
+            pcs[k] += (uint32_t)__popcll(w[k]);
         }
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             const uint64_t lw = map[(size_t)a.tk[j] * b.cw + x] & range_mask(x, a.tlo[j], a.thi[j]);
             const uint64_t* p2 = map + (size_t)a.tk2[j] * b.cw + x;
             const uint64_t pw = a.td[j] ? ((p2[0] >> 1) | (p2[1] << 63)) : p2[0];
-            pcs[8 + j] = (uint32_t)__popcll(lw & pw);
+            pcs[8 + j] += (uint32_t)__popcll(lw & pw);
         }
         // the walk: primes in position order (block t, then ascending
         // phase), over the 32-bit halves of the words; per block the eight
@@ -1303,9 +1304,8 @@ __global__ void __launch_bounds__(256) k_count_b(const gbh_countseg* __restrict_
     const uint32_t* fl = scratch + (size_t)s * swords;
     for (uint32_t i = tid; i <= GBH_GAP_BINS; i += 256) hl[i] = 0;
     __syncthreads();
-    const uint32_t x = blockIdx.x * 256 + tid;
     uint64_t best = 0;
-    if (x < a.nwords) {
+    for (uint32_t x = blockIdx.x * 256 + tid; x < a.nwords; x += gridDim.x * 256) {
         const uint32_t last = fl[2 * x + 1];
         if (last != 0xFFFFFFFFu) {
             for (uint32_t y = x + 1; y < a.nwords; ++y) {
