@@ -455,39 +455,39 @@ __device__ __forceinline__ void bm_single(const HlCtx& h, double qb0d, const uin
     const uint64_t span = 2ull * h.dmax;
     uint64_t w[kS];
     uint32_t bit[kS], d[kS], cls[kS];
+    // branch-free per prime (every lane computes every step; the flags
+    // decide): a wave almost always has a live lane, so branches only added
+    // mask bookkeeping
 #pragma unroll
     for (uint32_t u = 0; u < kS; ++u) {
-        const uint32_t pu = p[u];
-        bit[u] = 64;                       // 64: no multiple to test
-        w[u] = 0;
-        d[u] = cls[u] = 0;
-        if (!pu) continue;
+        const uint32_t pu = p[u], pc = pu | (pu == 0);   // pc: a divisor for the idle lanes
         // m = floor(qb0 / p) as in bm_setup: within one, one step each way
-        const double pd = (double)pu;
-        double inv = (double)__builtin_amdgcn_rcpf((float)pu);
+        const double pd = (double)pc;
+        double inv = (double)__builtin_amdgcn_rcpf((float)pc);
         inv = __builtin_fma(inv, __builtin_fma(-pd, inv, 1.0), inv);
         const double fd = __builtin_floor(qb0d * inv);
         const uint32_t fhi = (uint32_t)(fd * (1.0 / 4294967296.0));
         uint64_t m = ((uint64_t)fhi << 32) | (uint32_t)(fd - (double)fhi * 4294967296.0);
-        int64_t r = (int64_t)(h.qb0 - m * pu);
-        if (r < 0) { --m; r += pu; }
-        else if (r >= (int64_t)pu) { ++m; r -= pu; }
-        if (r < 0 || r >= (int64_t)pu) { atomicOr(err, 16u); continue; }
-        const uint32_t r0 = r ? pu - (uint32_t)r : 0u;   // m0 p = qb0 + r0
+        int64_t r = (int64_t)(h.qb0 - m * pc);
+        const bool lo = r < 0, hi = r >= (int64_t)pc;
+        m = lo ? m - 1 : (hi ? m + 1 : m);
+        r = lo ? r + pc : (hi ? r - (int64_t)pc : r);
+        const bool bad = r < 0 || r >= (int64_t)pc;
+        const uint32_t r0 = r ? pc - (uint32_t)r : 0u;   // m0 p = qb0 + r0
         const uint64_t m0 = m + (r != 0);
-        if (r0 > span || m0 < pu) continue;             // no multiple in the span, or m0 below p
         const uint32_t r30 = ((uint32_t)(m0 >> 32) * 16u + (uint32_t)m0 % 30u) % 30u;
-        if (!((kUnitMask >> r30) & 1u)) continue;       // not in the bitmap: m0 shares a factor with 30
-        if (m0 < h.m_base || m0 - h.m_base >= h.m_span) {   // outside the bitmap: fail closed
-            atomicOr(err, 8u);
-            continue;
-        }
+        // a multiple in the span, m0 >= p (smaller ones belong to smaller
+        // primes) and a unit mod 30 (else not in the bitmap)
+        const bool cand = pu && !bad && r0 <= span && m0 >= pc && ((kUnitMask >> r30) & 1u);
+        const bool inb = m0 >= h.m_base && m0 - h.m_base < h.m_span;
+        if (pu && bad) atomicOr(err, 16u);               // the quotient more than one off: fail closed
+        if (cand && !inb) atomicOr(err, 8u);             // outside the bitmap: fail closed
         const uint64_t y0 = ((m0 - r30 - h.m_base) >> 1) * 0xEEEEEEEEEEEEEEEFull;   // byte of m0
         const uint32_t i = units_below(r30);
         w[u] = y0 >> 3;
-        bit[u] = 8 * (uint32_t)(y0 & 7) + i;
+        bit[u] = cand && inb ? 8 * (uint32_t)(y0 & 7) + i : 64u;   // 64: no multiple to test
         d[u] = r0 >> 1;
-        cls[u] = (cmt[units_below(pu % 30)] >> (3 * i)) & 7;
+        cls[u] = (cmt[units_below(pc % 30)] >> (3 * i)) & 7;
     }
     uint64_t v[kS];
 #pragma unroll
