@@ -356,6 +356,9 @@ int gbh_launch_check_a(gbh_ctx* c, int s, const gbh_check_args* a)
     // every class array holds nc <= nwin * wbits bits; the checker's words
     // stay below that (host contract: n_gpu >= q_first + 9780)
     const uint32_t words = uint32_t((uint64_t(a->b.nwin) * a->b.wbits + 63) / 64);
+    const uint32_t cg = a->class_groups ? a->class_groups : 1u;
+    if (GBH_EVEN_CLASSES % cg) return fail(GBH_E_ARG, "check: class_groups must divide the 15 even classes");
+    if (!a->coff || !a->coffa || !a->pending || !a->out) return fail(GBH_E_ARG, "check: null argument");
     if (!(a->kinds & 4u)) {
         hipLaunchKernelGGL(gbh_dev::k_coff, dim3((GBH_NCAND_PAD + 255) / 256, GBH_EVEN_CLASSES, a->b.nseg),
                            dim3(256), 0, S(c, s), a->b, a->coff, a->coffa);
@@ -363,8 +366,6 @@ int gbh_launch_check_a(gbh_ctx* c, int s, const gbh_check_args* a)
     }
     // sampled segments (histogram on) and the others: one launch each, every
     // work-group leaves at once on the other kind of segment
-    const uint32_t cg = a->class_groups ? a->class_groups : 1u;
-    if (GBH_EVEN_CLASSES % cg) return fail(GBH_E_ARG, "check: class_groups must divide the 15 even classes");
     const dim3 grid(grid_of(words, gbh_dev::kBlockWords, 1u << 30), a->b.nseg, cg);
     if (a->kinds & 1u) {
         hipLaunchKernelGGL(gbh_dev::k_check_a<false>, grid, dim3(GBH_CHECK_WG), 0, S(c, s), a->b, a->coffa, a->ka,
