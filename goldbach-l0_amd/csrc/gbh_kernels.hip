@@ -806,6 +806,18 @@ __device__ __forceinline__ uint32_t first_off_big(uint32_t t, uint32_t wb0, uint
     return r ? p - (uint32_t)r : 0u;
 }
 
+// Sets the bits bt, bt + step, ... below wbits of an LDS window (composites
+// as set bits), two per iteration.
+__device__ __forceinline__ void mark_run(uint32_t* win32, uint32_t bt, uint32_t step, uint32_t wbits)
+{
+    for (; bt + step < wbits; bt += 2 * step) {
+        atomicOr(&win32[bt >> 5], 1u << (bt & 31));
+        const uint32_t b2 = bt + step;
+        atomicOr(&win32[b2 >> 5], 1u << (b2 & 31));
+    }
+    if (bt < wbits) atomicOr(&win32[bt >> 5], 1u << (bt & 31));
+}
+
 __global__ void __launch_bounds__(256) k_window(gbh_batch b, const uint32_t* __restrict__ primes, uint32_t np,
                                                 uint32_t ncoop, const uint32_t* __restrict__ i0tab,
                                                 const uint32_t* __restrict__ hits, const uint32_t* __restrict__ hit_n,
@@ -868,8 +880,7 @@ __global__ void __launch_bounds__(256) k_window(gbh_batch b, const uint32_t* __r
                 pn = primes[j + 4];
             }
             if (t0 == 0xFFFFFFFFu) continue;
-            for (uint32_t bt = first_off(t0, wb0, p) + lane * p; bt < wbits; bt += 64 * p)
-                atomicOr(&win32[bt >> 5], 1u << (bt & 31));
+            mark_run(win32, first_off(t0, wb0, p) + lane * p, 64 * p, wbits);
         }
     }
     // 3) the other window primes: one prime per lane, kWL of them loaded at
@@ -887,8 +898,7 @@ __global__ void __launch_bounds__(256) k_window(gbh_batch b, const uint32_t* __r
         for (uint32_t u = 0; u < kWL; ++u) {
             if (t0[u] == 0xFFFFFFFFu || t0[u] >= wb1) continue;
             const uint32_t p = pr[u];
-            for (uint32_t bt = first_off_big(t0[u], wb0, p, __builtin_amdgcn_rcpf((float)p)); bt < wbits; bt += p)
-                atomicOr(&win32[bt >> 5], 1u << (bt & 31));
+            mark_run(win32, first_off_big(t0[u], wb0, p, __builtin_amdgcn_rcpf((float)p)), p, wbits);
         }
     }
     // 4) the large primes' hits of this window
