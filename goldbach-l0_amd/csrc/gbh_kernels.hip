@@ -906,7 +906,15 @@ __global__ void __launch_bounds__(256) k_window(gbh_batch b, const uint32_t* __r
         const uint32_t bucket = (s * 8 + c) * b.nwin + blockIdx.x;
         const uint32_t n = min(hit_n[bucket], cap);
         const uint32_t* hl = hits + (size_t)bucket * cap;
-        for (uint32_t i = tid; i < n; i += 256) {
+        uint32_t i = tid;
+        for (; i + 3 * 256 < n; i += 4 * 256) {   // four loads in flight per thread
+            uint32_t bt[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) bt[u] = hl[i + u * 256];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) atomicOr(&win32[bt[u] >> 5], 1u << (bt[u] & 31));
+        }
+        for (; i < n; i += 256) {
             const uint32_t bt = hl[i];
             atomicOr(&win32[bt >> 5], 1u << (bt & 31));
         }
