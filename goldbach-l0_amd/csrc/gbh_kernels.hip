@@ -840,7 +840,15 @@ __global__ void __launch_bounds__(256) k_window(gbh_batch b, const uint32_t* __r
     // In LDS a window holds the composites as set bits (marks are ORs; the
     // store inverts it back to the map's primes-as-ones).
     if (!sieve) {   // the hit lists only: the window as the sieve phase stored it
-        for (uint32_t x = tid; x < nwords; x += 256) win[x] = ~dst[x];
+        uint32_t x = tid;
+        for (; x + 3 * 256 < nwords; x += 4 * 256) {   // four loads in flight per thread
+            uint64_t v[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) v[u] = dst[x + u * 256];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) win[x + u * 256] = ~v[u];
+        }
+        for (; x < nwords; x += 256) win[x] = ~dst[x];
     } else {        // 1) mask stage: every multiple of 7..61 set by whole-word masks
         uint32_t ph[GBH_NSMALL];
 #pragma unroll
@@ -1171,11 +1179,26 @@ __global__ void __launch_bounds__(256) k_check_b(gbh_batch b, const uint32_t* __
         const uint32_t k = (uint32_t)((n - e.n0) / 30);
         const uint32_t nj = c_cand_n[cr];
         const uint32_t* ct = coff + ((size_t)s * 15 + cr) * GBH_NCAND;
+        // ascending candidates, four probes in flight at a time; the first
+        // set bit in candidate order is p(n)
         uint32_t jf = 0xFFFFFFFFu;
-        for (uint32_t j = ka; j < nj; ++j) {
-            const uint32_t ent = ct[j];
-            const uint32_t c = ent >> 16, t = k + (ent & 0xFFFF);
-            if ((map[(size_t)c * b.cw + (t >> 6)] >> (t & 63)) & 1ull) { jf = j; break; }
+        for (uint32_t j = ka; j < nj && jf == 0xFFFFFFFFu; j += 4) {
+            uint32_t sh[4];
+            uint64_t wd[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) {
+                wd[u] = 0;
+                sh[u] = 0;
+                if (j + u < nj) {
+                    const uint32_t ent = ct[j + u];
+                    const uint32_t c = ent >> 16, t = k + (ent & 0xFFFF);
+                    sh[u] = t & 63;
+                    wd[u] = map[(size_t)c * b.cw + (t >> 6)];
+                }
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u)
+                if (jf == 0xFFFFFFFFu && j + u < nj && ((wd[u] >> sh[u]) & 1ull)) jf = j + u;
         }
         if (jf != 0xFFFFFFFFu) {
             atomicAdd(&out->hist[cr * GBH_NCAND + jf], 1u);
