@@ -454,7 +454,7 @@ __device__ __forceinline__ void bm_single(const HlCtx& h, double qb0d, const uin
     }
     const uint64_t span = 2ull * h.dmax;
     uint64_t w[kS];
-    uint32_t bit[kS], d[kS], cls[kS];
+    uint32_t bit[kS], d[kS], cls[kS], ef = 0;
     // branch-free per prime (every lane computes every step; the flags
     // decide): a wave almost always has a live lane, so branches only added
     // mask bookkeeping
@@ -480,8 +480,8 @@ __device__ __forceinline__ void bm_single(const HlCtx& h, double qb0d, const uin
         // primes) and a unit mod 30 (else not in the bitmap)
         const bool cand = pu && !bad && r0 <= span && m0 >= pc && ((kUnitMask >> r30) & 1u);
         const bool inb = m0 >= h.m_base && m0 - h.m_base < h.m_span;
-        if (pu && bad) atomicOr(err, 16u);               // the quotient more than one off: fail closed
-        if (cand && !inb) atomicOr(err, 8u);             // outside the bitmap: fail closed
+        ef |= pu && bad ? 16u : 0u;                      // the quotient more than one off: fail closed
+        ef |= cand && !inb ? 8u : 0u;                    // outside the bitmap: fail closed
         const uint64_t y0 = ((m0 - r30 - h.m_base) >> 1) * 0xEEEEEEEEEEEEEEEFull;   // byte of m0
         const uint32_t i = units_below(r30);
         w[u] = y0 >> 3;
@@ -489,6 +489,7 @@ __device__ __forceinline__ void bm_single(const HlCtx& h, double qb0d, const uin
         d[u] = r0 >> 1;
         cls[u] = (cmt[units_below(pc % 30)] >> (3 * i)) & 7;
     }
+    if (ef) atomicOr(err, ef);
     uint64_t v[kS];
 #pragma unroll
     for (uint32_t u = 0; u < kS; ++u) v[u] = bit[u] < 64 ? __builtin_nontemporal_load(h.mbits + w[u]) : 0ull;
@@ -1087,18 +1088,31 @@ __global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_batch b, const uin
                     u0 &= ~x0;
                     u1 &= ~x1;
                 }
-                // the eight wave totals to lanes 0..7, then two LDS adds
-                // with per-lane addresses (not sixteen single-lane adds)
-                uint32_t tv = 0;
+                // the sixteen wave totals by a halving exchange: each step
+                // a lane keeps half of its counts, adds its partner's copy
+                // of that half and hands over the other half. Partners:
+                // lane ^ 32 and ^ 16 (bpermute), then inside a row of 16
+                // lane ^ 15 (DPP row_mirror) and ^ 7 (row_half_mirror),
+                // each differing in the bit that picks the half. After
+                // four steps lane l holds count (l >> 2) & 15 summed over
+                // eight lanes of its row; quad_perm ^ 1 and ^ 2 add the
+                // rest. Fields stay below 2^10 before the unpacking (64
+                // per lane, doubling each step).
+                const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4;
+                uint32_t h4[4], h2[2];
 #pragma unroll
-                for (uint32_t k = 0; k < 8; ++k) {
-                    const uint32_t t = __builtin_amdgcn_readlane(wave_total63(hc[k]), 63);
-                    tv = lane == k ? t : tv;
-                }
-                if (lane < 8 && tv) {
-                    atomicAdd(&hl[cr * GBH_KA_MAX + j0 + 2 * lane], tv & 0xFFFFu);
-                    atomicAdd(&hl[cr * GBH_KA_MAX + j0 + 2 * lane + 1], tv >> 16);
-                }
+                for (uint32_t k = 0; k < 4; ++k)
+                    h4[k] = (b5 ? hc[k + 4] : hc[k]) + (uint32_t)__shfl_xor((int)(b5 ? hc[k] : hc[k + 4]), 32, 64);
+#pragma unroll
+                for (uint32_t k = 0; k < 2; ++k)
+                    h2[k] = (b4 ? h4[k + 2] : h4[k]) + (uint32_t)__shfl_xor((int)(b4 ? h4[k] : h4[k + 2]), 16, 64);
+                const uint32_t h1 = (b3 ? h2[1] : h2[0]) +
+                                    (uint32_t)__builtin_amdgcn_mov_dpp((int)(b3 ? h2[0] : h2[1]), 0x140, 0xF, 0xF, false);
+                uint32_t tv = (b2 ? h1 >> 16 : h1 & 0xFFFFu) +
+                              (uint32_t)__builtin_amdgcn_mov_dpp((int)(b2 ? h1 & 0xFFFFu : h1 >> 16), 0x141, 0xF, 0xF, false);
+                tv += (uint32_t)__builtin_amdgcn_mov_dpp((int)tv, 0xB1, 0xF, 0xF, false);
+                tv += (uint32_t)__builtin_amdgcn_mov_dpp((int)tv, 0x4E, 0xF, 0xF, false);
+                if ((lane & 3) == 0 && tv) atomicAdd(&hl[cr * GBH_KA_MAX + j0 + (lane >> 2)], tv);
             } else {
                 uint32_t r0 = 0, r1 = 0;
 #pragma unroll
