@@ -555,11 +555,17 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
         // every condition below is uniform over the work-group
         const uint32_t seg0 = runs ? runs[r].seg0 : 0u, rn = runs ? runs[r].nseg : b.nseg;
         const uint32_t je = runs ? min(j1, runs[r].nprimes) : j1;
-        if (j0 >= je) continue;
         constexpr bool direct = kDirectForm;
         const uint32_t nb = direct ? rn * 8 : rn * 8 * b.nwin;   // the run's counters
         if (rn == 0 || seg0 + rn > b.nseg || nb > nbmax) {       // a run beyond the contract: fail closed
             if (tid == 0) atomicOr(err, 1u);
+            continue;
+        }
+        if (j0 >= je) {                       // no prime of the chunk in this run: empty regions
+            if (direct) {
+                uint32_t* fl = fills + (size_t)blockIdx.x * (b.nseg * 8) + seg0 * 8;
+                for (uint32_t i = tid; i < nb; i += 256) fl[i] = 0;
+            }
             continue;
         }
         __syncthreads();                      // the previous run is done with the LDS tables
@@ -682,7 +688,7 @@ __global__ void __launch_bounds__(512) k_partition(const uint32_t* __restrict__ 
         uint32_t n = 0;
         if (tid < nk) {
             const uint32_t k = k0 + tid;
-            n = fills[(size_t)k * nbb + bk];
+            n = min(fills[(size_t)k * nbb + bk], rdesc[2 * k + 1]);   // within the region whatever happened
             rb[tid] = rdesc[2 * k] + bk * rdesc[2 * k + 1];
         }
         uint32_t tot = 0;

@@ -1210,9 +1210,9 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     direct_used_[slot] = direct;
     if (nlarge_) {
         // the counted and staged forms reserve list slots in hit_n; the
-        // direct form needs it only for the partition (zeroed below)
-        if (direct) check(gbh_zero(c, ms, B.fills.p, size_t(nchunks_) * nseg * 8 * 4), "zero region fills");
-        else check(gbh_zero(c, ms, B.hit_n.p, size_t(nseg) * 8 * nwin * 4), "zero hit counts");
+        // direct form needs it only for the partition (zeroed on the side
+        // stream, below) and writes every region fill count itself
+        if (!direct) check(gbh_zero(c, ms, B.hit_n.p, size_t(nseg) * 8 * nwin * 4), "zero hit counts");
         gbh_harvest_args h{b, E.d_large, B.chunks.as<uint32_t>(), dr, nruns,
                            run_max, nchunks_, cap_, t_hl_, t_hi_, t_hi2_, direct ? 0u : stage_,
                            mb_ ? mb_->bits : nullptr,
@@ -1225,6 +1225,9 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         check(gbh_event_record(c, ms, ev[EV_HL1].e), "event");
     }
     if (ss != ms) check(gbh_stream_wait(c, ss, ev[EV_GEOM].e), "stream wait");
+    // the partition's list counts, zeroed beside the harvest (the main
+    // stream waits for EV_S0 before the partition)
+    if (direct) check(gbh_zero(c, ss, B.hit_n.p, size_t(nseg) * 8 * nwin * 4), "zero hit counts");
     const uint32_t nwp = nwin_ + nx_;   // the windows' primes
     if (nwp) {
         gbh_setup_args a{b, E.d_win.as<uint32_t>(), nwp, 0, B.i0.as<uint32_t>()};
@@ -1237,7 +1240,7 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     }
     check(gbh_event_record(c, ss, ev[EV_S1].e), "event");
     if (direct) {
-        check(gbh_zero(c, ms, B.hit_n.p, size_t(nseg) * 8 * nwin * 4), "zero hit counts");
+        if (ss != ms) check(gbh_stream_wait(c, ms, ev[EV_S0].e), "stream wait");
         const gbh_partition_args pa{b, B.rdesc.as<uint32_t>(), B.region.as<uint32_t>(), B.fills.as<uint32_t>(),
                                     B.groups.as<uint32_t>(), ngroups_, cap_, B.hits.as<uint32_t>(),
                                     B.hit_n.as<uint32_t>(), &out[0].err, part_cap_, 0};

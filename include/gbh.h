@@ -188,8 +188,10 @@ typedef struct {
      * the class bit of every hit to its own region of each (segment, class)
      * list, region + rdesc[2k] + (s * 8 + c) * rdesc[2k + 1] (rdesc[2k + 1]
      * entries each), and writes the counts to fills[k * nseg * 8 + s * 8 + c]
-     * (zeroed before the launch); gbh_launch_partition then builds the
-     * window lists. hits / hit_n / cap / stage are unused then (stage 0). */
+     * (every entry written, 0 for a run without primes in the chunk: no
+     * zeroing needed); gbh_launch_partition then builds the window lists
+     * (each count taken at most rdesc[2k + 1]). hits / hit_n / cap / stage
+     * are unused then (stage 0). */
     const uint32_t* rdesc;
     uint32_t* region;
     uint32_t* fills;
