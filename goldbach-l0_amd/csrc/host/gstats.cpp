@@ -236,18 +236,41 @@ void GapTotals::merge(const GapTotals& o)
 
 // ---------------------------------------------------------------- GoldbachStats
 
+namespace {
+// The [15][np] count vectors (~145 KB each) of one statistics object come
+// from a small per-thread pool and go back to it: allocated afresh for every
+// run, the allocator returned their pages to the system at each free and
+// every run faulted them in again (~50 us per vector); reused, they are only
+// cleared.
+thread_local std::vector<std::vector<uint64_t>> t_counts_pool;
+
+void take_counts(std::vector<uint64_t>& v, size_t n)
+{
+    if (!t_counts_pool.empty()) {
+        v = std::move(t_counts_pool.back());
+        t_counts_pool.pop_back();
+    }
+    v.assign(n, 0);
+}
+
+void give_counts(std::vector<uint64_t>& v)
+{
+    if (v.capacity() != 0 && t_counts_pool.size() < 16) t_counts_pool.push_back(std::move(v));
+}
+}  // namespace
+
 GoldbachStats::GoldbachStats(const std::vector<uint32_t>& cands, uint32_t tail_t, bool campaign)
     : cands_(cands), tail_t_(tail_t), campaign_(campaign)
 {
     np_ = cands.size();
     pidx_.assign(cands.back() / 2 + 1, -1);
     for (size_t i = 0; i < np_; ++i) pidx_[cands[i] / 2] = int32_t(i);
-    samp_.assign(15 * np_, 0);
-    tail_.assign(15 * np_, 0);
+    take_counts(samp_, 15 * np_);
+    take_counts(tail_, 15 * np_);
     tail_agg_.assign(np_, 0);
     if (campaign_) {   // a run of its own fills these at its commit
-        cum_samp_.assign(15 * np_, 0);
-        cum_tail_.assign(15 * np_, 0);
+        take_counts(cum_samp_, 15 * np_);
+        take_counts(cum_tail_, 15 * np_);
         cum_tail_agg_.assign(np_, 0);
     }
     first_.assign(np_, 0);
@@ -263,6 +286,14 @@ GoldbachStats::GoldbachStats(const std::vector<uint32_t>& cands, uint32_t tail_t
         (void)load_gaps();
     }
     set_run_start(~0ull);
+}
+
+GoldbachStats::~GoldbachStats()
+{
+    give_counts(samp_);
+    give_counts(tail_);
+    give_counts(cum_samp_);
+    give_counts(cum_tail_);
 }
 
 void GoldbachStats::set_run_start(uint64_t start)
@@ -846,15 +877,20 @@ std::string GoldbachStats::memory_commit(uint64_t sampled_total)
     return state_text();
 }
 
-void GoldbachStats::commit(const char* version, uint64_t sampled_total)
+void GoldbachStats::commit(const char* version, uint64_t sampled_total, const std::string* run_line)
 {
     if (!campaign_) throw std::runtime_error("stats: commit on a run without campaign files");
     if (evens_sampled_ != sampled_total)
         throw std::runtime_error("stats: " + std::to_string(evens_sampled_) + " sampled evens recorded, " +
                                  std::to_string(sampled_total) + " in the sampled segments (statistics not written)");
     const FileLock lock(state_path_ + ".lock");
+    // the log line names first occurrences new to the campaign: the caller's
+    // copy holds only if the state read here has the first occurrences the
+    // run started from
+    const std::vector<uint64_t> first_before = run_line ? first_ : std::vector<uint64_t>();
     reset_campaign();
     load_state();
+    if (run_line && first_ != first_before) run_line = nullptr;
     for (auto& c : covered_)
         if (start_ <= c.second && c.first <= limit_) {
             std::fprintf(stderr, "stats: [%llu, %llu] overlaps the covered range [%llu, %llu] of %s -- not recorded again\n",
@@ -878,7 +914,7 @@ void GoldbachStats::commit(const char* version, uint64_t sampled_total)
             }
             return;
         }
-    const std::string line = log_line(version);
+    const std::string line = run_line ? *run_line : log_line(version);
     for (size_t i = 0; i < samp_.size(); ++i) { cum_samp_[i] += samp_[i]; cum_tail_[i] += tail_[i]; }
     for (size_t i = 0; i < np_; ++i) cum_tail_agg_[i] += tail_agg_[i];
     for (size_t i = 0; i < np_; ++i)

@@ -78,6 +78,8 @@ public:
     ///        no campaign files are read or written, the run starts from a
     ///        fresh state and memory_commit() returns the state text.
     GoldbachStats(const std::vector<uint32_t>& cands, uint32_t tail_t, bool campaign = true);
+    GoldbachStats(const GoldbachStats&) = default;
+    ~GoldbachStats();
 
     void set_run_start(uint64_t start);
     /// Tail prime indices without a campaign first occurrence at or below the
@@ -103,7 +105,9 @@ public:
     void finish_run(uint64_t start, uint64_t limit, uint64_t evens);
     /// Merges the run into the campaign state and appends its log line
     /// (nothing when the range overlaps a covered one).
-    void commit(const char* version, uint64_t sampled_total);
+    /// run_line: log_line(version) if the caller already has it (the run's
+    /// data is unchanged since).
+    void commit(const char* version, uint64_t sampled_total, const std::string* run_line = nullptr);
     /// A run without campaign files: the state file text a fresh campaign
     /// holds after committing this run alone (same checks as commit()).
     std::string memory_commit(uint64_t sampled_total);

@@ -1520,7 +1520,7 @@ void Pipeline::finish_range(Range& r)
         r.gs->write_live(r.start, r.limit, r.total_evens, r.total_evens, since(r.t_submit), true);
         rs.stats_line = r.gs->log_line(VERSION);
         const auto tc = clock_::now();
-        if (r.campaign) r.gs->commit(VERSION, r.sampled_total);
+        if (r.campaign) r.gs->commit(VERSION, r.sampled_total, &rs.stats_line);
         else r.state_text = r.gs->memory_commit(r.sampled_total);
         if (phase_trace_) std::fprintf(stderr, "host: statistics commit %.3f ms\n", since(tc) * 1e3);
     }
