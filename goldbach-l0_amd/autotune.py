@@ -22,7 +22,11 @@ speed. The best setting is written as the band of the sample's magnitude
 for that band; an engine then takes the band of its limit from
 GB_TUNE_FILE for every knob the environment does not set.
 
-    python goldbach-l0_amd/autotune.py --start 4e18 --length 7.48e9 --out tune.txt
+The sample should hold several batches (the default, 160 full segments, is
+ten of 16): a sample of one batch cannot show the overlap of consecutive
+batches and favours smaller batches (16 segments: GB_BATCH=8 first; 160:
+GB_BATCH=16 by 7%).
+    python goldbach-l0_amd/autotune.py --start 4.5e18 --out tune.txt
 """
 from __future__ import annotations
 
@@ -88,7 +92,7 @@ def write_band(path: pathlib.Path, lo: int, hi: int, knobs: dict, note: str):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--start", type=float, required=True, help="first even of the sample range")
-    ap.add_argument("--length", type=float, default=7.48e9, help="integers in the sample range")
+    ap.add_argument("--length", type=float, default=7.48e10, help="integers in the sample range")
     ap.add_argument("--segment", type=int, default=467_500_000)
     ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--passes", type=int, default=2)
