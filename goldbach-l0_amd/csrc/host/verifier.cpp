@@ -1277,6 +1277,7 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
             ca.kinds = 1 | 4;
             check(gbh_launch_check_a(c, ms, &ca), "check phase A");
             ca.kinds = 2 | 4;
+            if (cgroups == 3) ca.class_groups = 5;   // the slower histogram form: more waves for it
             check(gbh_launch_check_a(c, ss, &ca), "check phase A (histogram)");
             check(gbh_event_record(c, ss, ev[EV_CA].e), "event");
             check(gbh_stream_wait(c, ms, ev[EV_CA].e), "stream wait");
