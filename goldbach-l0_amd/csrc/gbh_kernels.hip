@@ -681,7 +681,14 @@ __global__ void __launch_bounds__(512) k_partition(const uint32_t* __restrict__ 
     __shared__ uint32_t ent[kCap];
     __shared__ uint32_t wc[256], wbase[256], pre[GBH_PART_CHUNKS + 1], rb[GBH_PART_CHUNKS];
     const uint32_t bk = blockIdx.x, tid = threadIdx.x;
-    const uint32_t k0 = groups[blockIdx.y], nk = min(groups[blockIdx.y + 1] - k0, (uint32_t)GBH_PART_CHUNKS);
+    const uint32_t k0 = groups[blockIdx.y], k1 = groups[blockIdx.y + 1];
+    // a group that is empty, reversed or longer than the LDS tables is
+    // beyond the host's plan: its regions would go unread, so fail closed
+    if (k1 <= k0 || k1 - k0 > GBH_PART_CHUNKS) {
+        if (tid == 0) atomicOr(err, 1u);
+        return;
+    }
+    const uint32_t nk = k1 - k0;
     const uint32_t lw = 31 - __clz(wbits);
     for (uint32_t i = tid; i < nwin; i += 512) wc[i] = 0;
     if (tid < 64) {   // one wave: fill counts, region bases and their exclusive scan
@@ -1180,7 +1187,7 @@ __global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_batch b, const uin
 }
 
 __global__ void __launch_bounds__(256) k_check_b(gbh_batch b, const uint32_t* __restrict__ coff, uint32_t ka,
-                                                 uint32_t tail_t, const uint32_t* __restrict__ pending, uint32_t pcap,
+                                                 uint32_t kend, uint32_t tail_t, const uint32_t* __restrict__ pending, uint32_t pcap,
                                                  gbh_segout* __restrict__ outs)
 {
     const uint32_t s = blockIdx.y;
@@ -1197,7 +1204,7 @@ __global__ void __launch_bounds__(256) k_check_b(gbh_batch b, const uint32_t* __
         const uint32_t cr = (uint32_t)(n % 30) >> 1;
         const ClassEvens e = class_evens(g, cr);
         const uint32_t k = (uint32_t)((n - e.n0) / 30);
-        const uint32_t nj = c_cand_n[cr];
+        const uint32_t nj = min(c_cand_n[cr], kend);
         const uint32_t* ct = coff + ((size_t)s * 15 + cr) * GBH_NCAND;
         // ascending candidates, four probes in flight at a time; the first
         // set bit in candidate order is p(n)

@@ -58,6 +58,7 @@ void fill(const gb::RunStats& s, gba_run_result* out)
             out->harvest_bytes = s.harvest_bytes;
             out->window_bytes = s.window_bytes;
             out->host_plan_seconds = s.host_plan_seconds;
+            out->harvest_reruns = s.harvest_reruns;
         }
 }
 }  // namespace
@@ -112,12 +113,34 @@ int gba_engine_prepare(gba_engine* e, uint64_t limit, uint64_t start_even, doubl
     }
 }
 
-int gba_run_gpu(uint64_t limit, uint64_t segment_even, int verbose, uint64_t start_even, uint32_t sieve_split,
-                gba_run_result* out, char* stats_line, size_t stats_cap, char* err, size_t err_cap)
+int gba_run_gpu(uint64_t limit, uint64_t segment_even, const char* spv_path, int verbose, uint64_t start_even,
+                uint32_t sieve_split, const char* prime_cache_path, int collect_segments, gba_run_result* out,
+                gba_segment_stats* segs, size_t seg_cap, size_t* n_segs, char* stats_line, size_t stats_cap,
+                char* err, size_t err_cap)
 {
     try {
-        const gb::RunStats s = gb::run_gpu(limit, segment_even, "", verbose != 0, start_even, sieve_split);
+        const gb::RunStats s = gb::run_gpu(limit, segment_even, spv_path ? spv_path : "", verbose != 0, start_even,
+                                           sieve_split, prime_cache_path, collect_segments != 0);
         fill(s, out);
+        if (n_segs) *n_segs = s.per_segment.size();
+        for (size_t k = 0; segs && k < std::min(seg_cap, s.per_segment.size()); ++k) {
+            const gb::SegmentStats& g = s.per_segment[k];
+            gba_segment_stats& d = segs[k];
+            std::memset(&d, 0, sizeof d);
+            d.seg_start = g.seg_start;
+            d.seg_end = g.seg_end;
+            d.evens = g.evens;
+            d.tiles = g.tiles;
+            d.large_items = g.large_items;
+            d.sieve_split = g.sieve_split;
+            d.mask_count = g.mask_count;
+            d.leftovers = g.leftovers;
+            d.sieve_seconds = g.sieve_seconds;
+            d.large_seconds = g.large_seconds;
+            d.check_seconds = g.check_seconds;
+            d.offset_seconds = g.offset_seconds;
+            d.wall_seconds = g.wall_seconds;
+        }
         put(stats_line, stats_cap, s.stats_line.c_str());
         put(err, err_cap, "");
         return 0;

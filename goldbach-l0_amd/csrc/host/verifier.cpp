@@ -197,6 +197,7 @@ struct ChunkPlan {
     uint32_t nbb = 0, nwheel = 0, nwheel_hi = 0, nwheel_hi2 = 0, nrand = 0, nx = 0;
     double ncb = 0;
     std::string mode;
+    uint32_t region_fix = 0;
     std::vector<uint32_t> chunks, rdesc, groups;   // the tables uploaded to a slot
     uint64_t region_entries = 0;
     uint32_t nchunks = 0, stage = 0, ngroups = 0, part_cap = GBH_PART_CAP;
@@ -366,9 +367,11 @@ void resolve_range(Engine::State& E, Range& r, uint32_t split_arg)
         span = std::max<uint64_t>(span, 4);
     }
     if (span > E.max_span) throw std::runtime_error("segment span above the engine's buffers");
-    // a batch's positions are 32-bit half-offsets: one merged segment (below
-    // two spans) must stay below 2^33 integers
-    if (span >= (uint64_t(1) << 32)) throw std::runtime_error("segment span of 2^32 integers or more");
+    // a segment's prime positions (q - q_first, the counter's gap walk) are
+    // 32-bit: the merged last segment (below two spans plus the 9781 below
+    // its first even) must stay below 2^32 integers
+    if (span >= (uint64_t(1) << 31) - (uint64_t(1) << 20))
+        throw std::runtime_error("segment span of 2^31 - 2^20 integers or more (32-bit prime positions)");
     // this run's primes: those <= sqrt(limit) + 16
     const uint32_t bound = std::max<uint32_t>(base_prime_bound(r.limit), 64);
     const uint32_t nb = uint32_t(std::upper_bound(E.base.begin(), E.base.end(), bound) - E.base.begin());
@@ -466,7 +469,11 @@ PlanBounds bounds_of(const Range& r)
 // and the exact statistics of every range.
 class Pipeline {
 public:
-    Pipeline(Engine::State& e, bool multi) : E(e), multi_(multi) { phase_trace_ = env::present("GB_PHASE"); }
+    Pipeline(Engine::State& e, bool multi) : E(e), multi_(multi)
+    {
+        phase_trace_ = env::present("GB_PHASE");
+        if (kend_ && kend_ <= E.ka) throw std::runtime_error("GB_PHASEB_SLOTS must lie above GB_KA (0: every slot)");
+    }
     /// The large primes' split and the multiplier bitmap for @p b.
     void plan_bitmap(const PlanBounds& b);
     /// Bitmap, window size, list capacities and buffers for @p b.
@@ -550,6 +557,14 @@ private:
     // GB_BUCKET_VERIFY=N: every Nth segment's map is rebuilt with every large
     // prime on the wheel walk (no multiplier bitmap) and compared word for word
     uint32_t verify_every_ = 0, cap_v_ = 0, nvchunks_ = 0;
+    // GB_PHASEB_SLOTS=K: phase B probes candidate slots below K only, so the
+    // evens whose p(n) lies beyond become leftovers for the CPU (the role of
+    // the reference's GB_C30_PLAN override: it exercises the fallback)
+    uint32_t kend_ = env::u32("GB_PHASEB_SLOTS", 0, 0, GBH_NCAND);
+    // GB_HL_REGION=N: every direct-harvest region holds N entries per list
+    // instead of the size planned from the expected hits (a small N forces
+    // the region overflow and the counted re-run of the batch)
+    uint32_t region_fix_ = env::u32("GB_HL_REGION", 0, 0, GBH_PART_CAP);
     uint64_t verified_ = 0;
     bool harvested_[2] = {false, false};
 };
@@ -806,7 +821,7 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
     ChunkPlan& cp = E.cplan;
     if (cp.valid && cp.nbb == nbb && close_rel(cp.ncb, ncb) && cp.nwheel == nwheel_ && cp.nwheel_hi == nwheel_hi_ &&
         cp.nwheel_hi2 == nwheel_hi2_ && cp.nrand == nrand_ && cp.nx == nx_ && cp.mode == hl_mode_ &&
-        same_shape(cp.key, runs_in)) {
+        cp.region_fix == region_fix_ && same_shape(cp.key, runs_in)) {
         nchunks_ = cp.nchunks;
         stage_ = cp.stage;
         direct_ = cp.direct;
@@ -930,7 +945,7 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
         uint64_t base = 0, rmax = 0;
         for (uint32_t k = 0; k < nchunks_; ++k) {
             const double lam = ncb * hpb[k];
-            const uint64_t r = uint64_t(lam + 8 * std::sqrt(lam) + 64);
+            const uint64_t r = region_fix_ ? region_fix_ : uint64_t(lam + 8 * std::sqrt(lam) + 64);
             rd[2 * k] = uint32_t(base);
             rd[2 * k + 1] = uint32_t(r);
             base += r * nbb;
@@ -979,6 +994,7 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
     cp.nrand = nrand_;
     cp.nx = nx_;
     cp.mode = hl_mode_;
+    cp.region_fix = region_fix_;
     cp.nchunks = nchunks_;
     cp.stage = stage_;
     cp.direct = direct_;
@@ -1140,7 +1156,7 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
             nwords = std::max(nwords, cs.nwords);
         }
         hc[s] = cs;
-        r.rs.total_tiles += uint64_t((j.cg.n[0] + wbits - 1) / wbits) * 8;
+        if (!force_count_) r.rs.total_tiles += uint64_t((j.cg.n[0] + wbits - 1) / wbits) * 8;   // once per batch
         if (s == 0 || batch[s - 1].r != j.r) {
             hr[nruns++] = gbh_hl_run{s, 0, r.nlarge, 0};
         }
@@ -1179,13 +1195,13 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
                 const double ix = E.inv_at(x), im = E.inv_at(m), iy = E.inv_at(y);
                 words += double(m - x) + lq * (im - ix) / 240 + (8.0 / 30) * lq * (iy - im);
             }
-            agg_.harvest_bytes += np > nx_ ? 4.0 * (np - nx_) + 8.0 * words + 4.0 * hits : 0.0;
+            if (!force_count_) agg_.harvest_bytes += np > nx_ ? 4.0 * (np - nx_) + 8.0 * words + 4.0 * hits : 0.0;
             bits_all += bits;
             hits_all += hits;
             win_prime_bytes += 4.0 * run.nseg * 8 * (r.nwin + std::min(nx_, r.nlarge));
             costs.push_back(RunCost{np, bits, double(r.q_lo), r.density});
         }
-        agg_.window_bytes += bits_all / 8 + win_prime_bytes + 4.0 * hits_all;
+        if (!force_count_) agg_.window_bytes += bits_all / 8 + win_prime_bytes + 4.0 * hits_all;
     }
     if (multi_ && nlarge_) {
         double ncb = 0;
@@ -1284,7 +1300,7 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         } else {
             check(gbh_launch_check_a(c, ms, &ca), "check phase A");
         }
-        gbh_checkb_args cb{b, E.ka, CHECK_TAIL, B.coff.as<uint32_t>(), B.pending.as<uint32_t>(), pcap_, 0, out};
+        gbh_checkb_args cb{b, E.ka, CHECK_TAIL, B.coff.as<uint32_t>(), B.pending.as<uint32_t>(), pcap_, kend_, out};
         check(gbh_launch_check_b(c, ms, &cb), "check phase B");
     }
     check(gbh_event_record(c, ms, ev[EV_CHECK].e), "event");
@@ -1343,6 +1359,20 @@ void Pipeline::collect(std::vector<SegJob>& batch, int slot)
     check(gbh_event_sync(c, ev[EV_END].e), "batch sync");
     const auto tcol = clock_::now();
     if (phase_trace_) std::fprintf(stderr, "host: waited %.3f ms for the batch\n", since(tw) * 1e3);
+    const gbh_segout* outs = E.h_out[slot].as<gbh_segout>();
+    if ((outs[0].err & 1u) && direct_used_[slot]) {
+        // a region of the direct harvest overflowed: the batch again, with
+        // counted lists (queued behind the batch in flight, which keeps its
+        // own outputs); the failed attempt's times and bytes are not counted
+        std::fprintf(stderr, "note: harvest region overflow in the batch from %llu: re-run with counted lists\n",
+                     (unsigned long long)batch[0].sg.seg_start);
+        ++agg_.harvest_reruns;
+        force_count_ = true;
+        enqueue(batch, slot, true);
+        force_count_ = false;
+        collect(batch, slot);
+        return;
+    }
     // the sieve phase ran beside the harvest: its own time plus the hit-list
     // phase count as the window stage
     const double tl = elapsed_s(c, ev[EV_BEGIN], ev[EV_SETUP]);
@@ -1360,19 +1390,6 @@ void Pipeline::collect(std::vector<SegJob>& batch, int slot)
     agg_.large_seconds += tl;
     agg_.verify_seconds += tc + tn;
     agg_.count_seconds += tn;
-    const gbh_segout* outs = E.h_out[slot].as<gbh_segout>();
-    if ((outs[0].err & 1u) && direct_used_[slot]) {
-        // a region of the direct harvest overflowed: the batch again, with
-        // counted lists (queued behind the batch in flight, which keeps its
-        // own outputs)
-        std::fprintf(stderr, "note: harvest region overflow in the batch from %llu: re-run with counted lists\n",
-                     (unsigned long long)batch[0].sg.seg_start);
-        force_count_ = true;
-        enqueue(batch, slot, true);
-        force_count_ = false;
-        collect(batch, slot);
-        return;
-    }
     if (outs[0].err & 1u) throw std::runtime_error("hit-list bucket overflow (raise the capacity)");
     if (outs[0].err & 4u) throw std::runtime_error("hit lists: a bitmap prime's cube is not above the batch's q range");
     if (outs[0].err & 8u) throw std::runtime_error("hit lists: a multiplier outside the multiplier bitmap");
@@ -1493,6 +1510,7 @@ void Pipeline::finish_range(Range& r)
         rs.harvest_kernel_seconds = agg_.harvest_kernel_seconds;
         rs.window_kernel_seconds = agg_.window_kernel_seconds;
         rs.harvest_launches = agg_.harvest_launches;
+        rs.harvest_reruns = agg_.harvest_reruns;
         rs.window_launches = agg_.window_launches;
         rs.harvest_bytes = agg_.harvest_bytes;
         rs.window_bytes = agg_.window_bytes;

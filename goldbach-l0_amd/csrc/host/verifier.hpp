@@ -48,6 +48,7 @@ struct RunStats {
     uint64_t harvest_launches = 0, window_launches = 0;
     double harvest_bytes = 0, window_bytes = 0;
     double host_plan_seconds = 0;   // range resolution, statistics set-up and buffer planning of this run
+    uint64_t harvest_reruns = 0;    // batches re-run with counted lists after a direct-harvest region overflow
     std::vector<SegmentStats> per_segment;
 };
 

@@ -34,7 +34,18 @@ class GbaRunResult(ctypes.Structure):
         ("harvest_kernel_seconds", ctypes.c_double), ("window_kernel_seconds", ctypes.c_double),
         ("harvest_launches", ctypes.c_uint64), ("window_launches", ctypes.c_uint64),
         ("harvest_bytes", ctypes.c_double), ("window_bytes", ctypes.c_double),
-        ("host_plan_seconds", ctypes.c_double),
+        ("host_plan_seconds", ctypes.c_double), ("harvest_reruns", ctypes.c_uint64),
+    ]
+
+
+class GbaSegmentStats(ctypes.Structure):
+    _fields_ = [
+        ("seg_start", ctypes.c_uint64), ("seg_end", ctypes.c_uint64), ("evens", ctypes.c_uint64),
+        ("tiles", ctypes.c_uint64), ("large_items", ctypes.c_uint64),
+        ("sieve_split", ctypes.c_uint32), ("mask_count", ctypes.c_uint32), ("leftovers", ctypes.c_uint32),
+        ("pad0", ctypes.c_uint32),
+        ("sieve_seconds", ctypes.c_double), ("large_seconds", ctypes.c_double), ("check_seconds", ctypes.c_double),
+        ("offset_seconds", ctypes.c_double), ("wall_seconds", ctypes.c_double),
     ]
 
 
@@ -52,9 +63,11 @@ def load() -> ctypes.CDLL:
         if not LIB_PATH.exists():
             raise FileNotFoundError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
         lib = ctypes.CDLL(str(LIB_PATH))
-        lib.gba_run_gpu.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32,
-                                    ctypes.POINTER(GbaRunResult), ctypes.c_char_p, ctypes.c_size_t, ctypes.c_char_p,
-                                    ctypes.c_size_t]
+        lib.gba_run_gpu.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_char_p, ctypes.c_int, ctypes.c_uint64,
+                                    ctypes.c_uint32, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(GbaRunResult),
+                                    ctypes.POINTER(GbaSegmentStats), ctypes.c_size_t,
+                                    ctypes.POINTER(ctypes.c_size_t), ctypes.c_char_p, ctypes.c_size_t,
+                                    ctypes.c_char_p, ctypes.c_size_t]
         lib.gba_run_gpu.restype = ctypes.c_int
         lib.gba_segment_map.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p,
                                         ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint32),
@@ -98,23 +111,37 @@ def version() -> str:
     return load().gba_version().decode()
 
 
-def run_gpu(limit: int, segment_even: int = 467_500_000, verbose: bool = False, start_even: int = 4,
-            sieve_split: int = 0) -> dict:
-    """Verifies every even n in [start_even, limit] on the GPU.
+def run_gpu(limit: int, segment_even: int = 467_500_000, spv_path: str = "", verbose: bool = False,
+            start_even: int = 4, sieve_split: int = 0, prime_cache_path: str | None = None,
+            collect_segments: bool = False) -> dict:
+    """Verifies every even n in [start_even, limit] on the GPU; the
+    reference's eight arguments in its order (spv_path is ignored: the
+    device code is linked into the library).
 
-    Returns the run's figures and its GBSTATS line ("" when statistics are
-    off). Raises VerificationError when the run does not pass.
+    Returns the run's figures, its GBSTATS line ("" when statistics are off)
+    and, with collect_segments, "per_segment": one dict per segment. Raises
+    VerificationError when the run does not pass.
     """
     lib = load()
     res = GbaRunResult()
     line = ctypes.create_string_buffer(1 << 20)
     err = ctypes.create_string_buffer(4096)
-    rc = lib.gba_run_gpu(int(limit), int(segment_even), int(verbose), int(start_even), int(sieve_split),
-                         ctypes.byref(res), line, len(line), err, len(err))
+    cap = 1 << 16 if collect_segments else 0
+    segs = (GbaSegmentStats * max(cap, 1))()
+    nseg = ctypes.c_size_t(0)
+    cache = prime_cache_path.encode() if prime_cache_path else None
+    rc = lib.gba_run_gpu(int(limit), int(segment_even), spv_path.encode(), int(verbose), int(start_even),
+                         int(sieve_split), cache, int(bool(collect_segments)), ctypes.byref(res), segs, cap,
+                         ctypes.byref(nseg), line, len(line), err, len(err))
     if rc:
         raise VerificationError(err.value.decode())
     out = {name: getattr(res, name) for name, _ in GbaRunResult._fields_}
     out["stats_line"] = line.value.decode()
+    if collect_segments:
+        if nseg.value > cap:
+            raise VerificationError(f"{nseg.value} segments: more than the {cap} collected")
+        out["per_segment"] = [{n: getattr(segs[k], n) for n, _ in GbaSegmentStats._fields_ if n != "pad0"}
+                              for k in range(nseg.value)]
     return out
 
 

@@ -301,7 +301,8 @@ typedef struct {
     const uint32_t* coff;
     const uint32_t* pending;
     uint32_t pcap;
-    uint32_t pad0;
+    uint32_t kend;                 /* probe candidate slots [ka, kend) only (0: every slot); an even whose p(n)
+                                      lies beyond is a leftover for the CPU (a test of the fallback) */
     gbh_segout* out;
 } gbh_checkb_args;
 

@@ -41,12 +41,28 @@ typedef struct {
     uint64_t harvest_launches, window_launches;
     double harvest_bytes, window_bytes;
     double host_plan_seconds;
+    uint64_t harvest_reruns;       /* batches re-run with counted lists after a harvest region overflow */
 } gba_run_result;
 
-/* Returns 0 on PASS. stats_line (if non-null, capacity stats_cap) receives
- * the run's GBSTATS line when statistics were on. */
-int gba_run_gpu(uint64_t limit, uint64_t segment_even, int verbose, uint64_t start_even, uint32_t sieve_split,
-                gba_run_result* out, char* stats_line, size_t stats_cap, char* err, size_t err_cap);
+/* One segment of a run (the reference's SegmentStats, include/verifier.hpp:
+ * 14-27), filled when collect_segments is set. */
+typedef struct {
+    uint64_t seg_start, seg_end, evens, tiles, large_items;
+    uint32_t sieve_split, mask_count, leftovers, pad0;
+    double sieve_seconds, large_seconds, check_seconds, offset_seconds, wall_seconds;
+} gba_segment_stats;
+
+/* The reference's eight arguments in its order (verifier.hpp:103-110):
+ * spv_path is accepted and ignored (the device code is linked into this
+ * library), prime_cache_path names the GBP2 base-prime cache stem (NULL:
+ * none), collect_segments asks for one gba_segment_stats per segment: the
+ * first seg_cap of them are written to segs and *n_segs receives the run's
+ * segment count. Returns 0 on PASS. stats_line (if non-null, capacity
+ * stats_cap) receives the run's GBSTATS line when statistics were on. */
+int gba_run_gpu(uint64_t limit, uint64_t segment_even, const char* spv_path, int verbose, uint64_t start_even,
+                uint32_t sieve_split, const char* prime_cache_path, int collect_segments, gba_run_result* out,
+                gba_segment_stats* segs, size_t seg_cap, size_t* n_segs, char* stats_line, size_t stats_cap,
+                char* err, size_t err_cap);
 
 /* One segment's class map: words [8 * cw] (cw returned in *cw_out, at most
  * cap words are written), Q[8] first q per class, nc[8] bits per class.

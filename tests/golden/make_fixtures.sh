@@ -26,6 +26,7 @@ windows=(
   "wgapcross   1425172824436700150   1425172824438700150"
   "w4e18l      4000000000000000000   4000000000200000000"
   "cfg1        1000000000000         1000467499998"
+  "cfg2        4000000000000000000   4000000000467499998"
 )
 want=" $* "
 for w in "${windows[@]}"; do

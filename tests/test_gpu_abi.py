@@ -1,8 +1,11 @@
 """The device-level C ABI (include/gbh.h) refuses arguments outside a
 stage's contract before it launches anything: a non-zero GBH_E_ARG status
 and a message in gbh_last_error(), the reference's fail-closed ZE_CHECK
-behaviour (include/l0_context.hpp:14-20) as a status code. The pointers
-below are never dereferenced: every call fails its checks first."""
+behaviour (include/l0_context.hpp:14-20) as a status code. Every call
+below fails its checks before it launches; the pointers it passes still
+point into one real, zeroed 4 MiB device buffer, so a check that stopped
+refusing would launch on mapped memory of a harmless size, not on a wild
+address."""
 import ctypes
 
 import pytest
@@ -13,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 GBH_E_ARG = 2
 P, U32 = ctypes.c_void_p, ctypes.c_uint32
-DUMMY = 0x10000          # a non-null device address no call reaches
+DUMMY = None             # the device buffer of the `dev` fixture (set there)
 
 
 class Batch(ctypes.Structure):
@@ -30,6 +33,11 @@ class PartitionArgs(ctypes.Structure):
                 ("cap", U32), ("hits", P), ("hit_n", P), ("err", P), ("part_cap", U32), ("pad0", U32)]
 
 
+class CheckBArgs(ctypes.Structure):
+    _fields_ = [("b", Batch), ("ka", U32), ("tail_t", U32), ("coff", P), ("pending", P), ("pcap", U32),
+                ("kend", U32), ("out", P)]
+
+
 class WindowArgs(ctypes.Structure):
     _fields_ = [("b", Batch), ("primes", P), ("nprimes", U32), ("ncoop", U32), ("i0", P), ("hits", P),
                 ("hit_n", P), ("cap", U32), ("phases", U32)]
@@ -41,7 +49,16 @@ def dev():
     lib.gbh_last_error.restype = ctypes.c_char_p
     ctx = P()
     assert lib.gbh_open(0, ctypes.byref(ctx)) == 0, lib.gbh_last_error()
+    global DUMMY
+    buf = P()
+    nbytes = 4 << 20
+    assert lib.gbh_alloc(ctx, ctypes.c_size_t(nbytes), 0, ctypes.byref(buf)) == 0, lib.gbh_last_error()
+    assert lib.gbh_zero(ctx, 0, buf, ctypes.c_size_t(nbytes)) == 0
+    assert lib.gbh_stream_sync(ctx, 0) == 0
+    DUMMY = buf.value
     yield lib, ctx
+    assert lib.gbh_stream_sync(ctx, 0) == 0
+    lib.gbh_free(ctx, buf, 0)
     lib.gbh_close(ctx)
 
 
@@ -91,3 +108,9 @@ def test_partition_size(dev):
     _refused(lib, lib.gbh_launch_partition(ctx, 0, ctypes.byref(a)), "part_cap")
     a.part_cap, a.hits = 8192, None
     _refused(lib, lib.gbh_launch_partition(ctx, 0, ctypes.byref(a)), "null")
+
+
+def test_check_b_slot_limit(dev):
+    lib, ctx = dev
+    a = CheckBArgs(b=_batch(), ka=96, tail_t=3000, coff=DUMMY, pending=DUMMY, pcap=1024, kend=96, out=DUMMY)
+    _refused(lib, lib.gbh_launch_check_b(ctx, 0, ctypes.byref(a)), "kend")
