@@ -14,10 +14,17 @@
 #include "gbh.h"
 #include "gbh_kernels.hip"
 
+// The allocation-free counter's workspace (gbh_launch_count_noalloc): one
+// copy per device, at the device address gbh_open looks up. In host code
+// this name is a host-side stand-in, never a device address: launches take
+// the looked-up pointer.
+static __device__ __align__(256) uint32_t scratch[GBH_NOALLOC_SCRATCH_WORDS];
+
 struct gbh_ctx {
     int device = 0;
     hipStream_t stream[GBH_STREAMS] = {};
     gbh_device_info_t info{};
+    uint32_t* scratch_dev = nullptr;   // `scratch` on this context's device
 };
 struct gbh_event {
     hipEvent_t ev = nullptr;
@@ -124,6 +131,12 @@ int gbh_open(int device, gbh_ctx** out)
     c->info.clock_khz = pr.clockRate;
     c->info.mem_clock_khz = pr.memoryClockRate;
     if (const int rc = upload_candidates()) { gbh_close(c); return rc; }
+    // the allocation-free counter's workspace on this device (the device is current)
+    const hipError_t es = hipGetSymbolAddress(reinterpret_cast<void**>(&c->scratch_dev), HIP_SYMBOL(scratch));
+    if (es != hipSuccess || !c->scratch_dev) {
+        gbh_close(c);
+        return hip_fail(es != hipSuccess ? es : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+    }
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&gbh_dev::k_window),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) (void)hipGetLastError();   // optional: windows up to the default limit still work
@@ -399,23 +412,48 @@ int gbh_launch_mapcmp(gbh_ctx* c, int s, const gbh_mapcmp_args* a)
     return GBH_OK;
 }
 
-int gbh_launch_count(gbh_ctx* c, int s, const gbh_count_args* a)
+}  // extern "C"
+
+namespace {
+
+// The counter and its gap walk across words, with the per-word scratch at
+// `sc` (the caller's workspace or the library's static array).
+int launch_count(gbh_ctx* c, int s, const gbh_count_args* a, uint32_t* sc)
 {
     if (const int rc = check_batch(a->b)) return rc;
     if (a->swords < 2ull * a->nwords) return fail(GBH_E_ARG, "count scratch too small");
     if (uint64_t(a->nwords) + 1 > a->b.cw) return fail(GBH_E_ARG, "count words exceed the class stride");
+    if (!a->cs || !a->out) return fail(GBH_E_ARG, "count: null argument");
     if (!a->nwords) return GBH_OK;
+    if (!sc) return fail(GBH_E_ARG, "count: null scratch");
     // about eight work-groups per CU over the batch, each over several words
     // (one histogram flush per work-group, not per 256 words)
     const unsigned grid = std::max(1u, std::min(grid_of(a->nwords, 256, 1u << 30),
                                                 (8u * uint32_t(std::max(c->info.cus, 1)) + a->b.nseg - 1) / a->b.nseg));
-    hipLaunchKernelGGL(gbh_dev::k_count_a, dim3(grid, a->b.nseg), dim3(256), 0, S(c, s), a->b, a->cs, a->scratch,
+    hipLaunchKernelGGL(gbh_dev::k_count_a, dim3(grid, a->b.nseg), dim3(256), 0, S(c, s), a->b, a->cs, sc,
                        a->swords, a->out);
     GBH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(gbh_dev::k_count_b, dim3(grid, a->b.nseg), dim3(256), 0, S(c, s), a->cs, a->scratch, a->swords,
+    hipLaunchKernelGGL(gbh_dev::k_count_b, dim3(grid, a->b.nseg), dim3(256), 0, S(c, s), a->cs, sc, a->swords,
                        a->out);
     GBH_HIP(hipGetLastError());
     return GBH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gbh_launch_count(gbh_ctx* c, int s, const gbh_count_args* a)
+{
+    return launch_count(c, s, a, a->scratch);
+}
+
+int gbh_launch_count_noalloc(gbh_ctx* c, int s, const gbh_count_args* a)
+{
+    if (a->scratch) return fail(GBH_E_ARG, "count_noalloc: scratch must be NULL (the library's array is used)");
+    if (uint64_t(a->swords) * a->b.nseg > GBH_NOALLOC_SCRATCH_WORDS)
+        return fail(GBH_E_ARG, "count_noalloc: nseg * swords above GBH_NOALLOC_SCRATCH_WORDS");
+    return launch_count(c, s, a, c->scratch_dev);
 }
 
 }  // extern "C"

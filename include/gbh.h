@@ -364,6 +364,23 @@ int gbh_launch_check_b(gbh_ctx* ctx, int stream, const gbh_checkb_args* a);
 int gbh_launch_count(gbh_ctx* ctx, int stream, const gbh_count_args* a);
 int gbh_launch_mapcmp(gbh_ctx* ctx, int stream, const gbh_mapcmp_args* a);
 
+/* ---- allocation-free form ----
+ * gbh_launch_count without a workspace: the counter's per-word scratch (the
+ * first and last prime position of every map word, read back by the gap
+ * walk across words) is the library's own static device array of
+ * GBH_NOALLOC_SCRATCH_WORDS u32 (16 MiB per device, held until the process
+ * ends; gbh_open looks up its address on the context's device). No
+ * counterpart in the reference: for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The caller passes inputs and outputs as for gbh_launch_count; a->scratch
+ * must be NULL and nseg * swords <= GBH_NOALLOC_SCRATCH_WORDS, else
+ * GBH_E_ARG. The call allocates and frees nothing. Every call on one device
+ * shares that device's array, which keeps the last call's contents (each
+ * call writes every entry it reads, so nothing needs clearing): calls that
+ * overlap on one device (two streams, contexts or threads) race. */
+#define GBH_NOALLOC_SCRATCH_WORDS (4u << 20)
+int gbh_launch_count_noalloc(gbh_ctx* ctx, int stream, const gbh_count_args* a);
+
 #ifdef __cplusplus
 }
 #endif
