@@ -398,7 +398,8 @@ int gbh_launch_check_b(gbh_ctx* c, int s, const gbh_checkb_args* a)
     if (const int rc = check_batch(a->b)) return rc;
     if (a->kend && a->kend <= a->ka) return fail(GBH_E_ARG, "check_b: kend must lie above ka (0: every slot)");
     hipLaunchKernelGGL(gbh_dev::k_check_b, dim3(grid_of(a->pcap, 256, 512), a->b.nseg), dim3(256), 0, S(c, s), a->b,
-                       a->coff, a->ka, a->kend ? a->kend : 0xFFFFFFFFu, a->tail_t, a->pending, a->pcap, a->out);
+                       a->coff, a->ka, a->kend ? a->kend : 0xFFFFFFFFu, a->tail_t, a->pending, a->pcap, a->out,
+                       a->ridge_more);
     GBH_HIP(hipGetLastError());
     return GBH_OK;
 }

@@ -1188,7 +1188,7 @@ __global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_batch b, const uin
 
 __global__ void __launch_bounds__(256) k_check_b(gbh_batch b, const uint32_t* __restrict__ coff, uint32_t ka,
                                                  uint32_t kend, uint32_t tail_t, const uint32_t* __restrict__ pending, uint32_t pcap,
-                                                 gbh_segout* __restrict__ outs)
+                                                 gbh_segout* __restrict__ outs, uint64_t* __restrict__ ridge_more)
 {
     const uint32_t s = blockIdx.y;
     const gbh_geom& g = b.geoms[s];
@@ -1236,6 +1236,10 @@ __global__ void __launch_bounds__(256) k_check_b(gbh_batch b, const uint32_t* __
                 if (sl < GBH_RIDGE_CAP) {
                     out->ridge[2 * sl] = n;
                     out->ridge[2 * sl + 1] = pj;
+                } else if (ridge_more && sl - GBH_RIDGE_CAP < GBH_RIDGE_MORE) {
+                    uint64_t* m = ridge_more + 2 * ((size_t)s * GBH_RIDGE_MORE + (sl - GBH_RIDGE_CAP));
+                    m[0] = n;
+                    m[1] = pj;
                 }
             }
             atomicAdd(&out->acct[1], 1u);

@@ -1,41 +1,35 @@
-// Exact statistics and campaign files (contracts and field semantics in
-// gstats.hpp; formats as the reference's include/gstats.hpp).
+// Exact statistics and campaign files (contracts, field meanings and the
+// design in gstats.hpp).
 #include "gstats.hpp"
-
-#include <fcntl.h>
-#include <sys/file.h>
-#include <unistd.h>
 
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
-#include <cstring>
 #include <ctime>
+#include <functional>
 #include <numeric>
 #include <stdexcept>
+#include <string_view>
 
 #include "env.hpp"
 #include "primes.hpp"
+#include "textio.hpp"
 
 namespace gb {
 
-std::string u128_str(u128 v)
-{
-    char b[48];
-    int n = 0;
-    do { b[n++] = char('0' + int(v % 10)); v /= 10; } while (v);
-    std::string s;
-    while (n) s += b[--n];
-    return s;
-}
+std::string u128_str(u128 v) { return textio::dec(v); }
 
 namespace {
 
-// Primes p <= 9781 with no even n <= 4e18 having p(n) = p (T. Oliveira e
-// Silva's published S(p) table, the '?' entries; the reference lists them at
-// include/gstats.hpp:127-136). A run's first even with such a p is reported
-// as an S(p) candidate.
-constexpr uint16_t kUnknownS[] = {
+using Ranges = std::vector<std::pair<uint64_t, uint64_t>>;
+using textio::count_list;
+using textio::dec;
+
+// Primes p <= 9781 for which no even n <= 4e18 has p(n) = p (the unknown
+// entries of T. Oliveira e Silva's published S(p) table, as the reference
+// lists them at include/gstats.hpp:127-136): the first even of a campaign
+// with such a p(n) is announced as an S(p) candidate.
+constexpr uint16_t kSpUnknown[] = {
     8423, 8429, 8537, 8627, 8669, 8713, 8741, 8753, 8783, 8803, 8807, 8837, 8839, 8849, 8861,
     8863, 8867, 8887, 8893, 8923, 8963, 8969, 8999, 9007, 9011, 9013, 9029, 9041, 9043, 9049,
     9059, 9067, 9103, 9109, 9127, 9133, 9137, 9151, 9157, 9173, 9181, 9187, 9199, 9209, 9221,
@@ -44,933 +38,863 @@ constexpr uint16_t kUnknownS[] = {
     9479, 9491, 9497, 9511, 9521, 9533, 9539, 9547, 9551, 9587, 9601, 9613, 9619, 9623, 9631,
     9643, 9649, 9661, 9677, 9679, 9689, 9697, 9719, 9721, 9733, 9739, 9743, 9749, 9767, 9769};
 
-int res_idx(uint64_t q)
+bool sp_unknown(uint32_t p) { return std::binary_search(std::begin(kSpUnknown), std::end(kSpUnknown), uint16_t(p)); }
+
+// position of q mod 30 among the eight unit residues 1 7 11 13 17 19 23 29
+// (the primes30 order); -1 for the other residues
+int unit_slot(uint64_t q)
 {
-    static const int idx[30] = {-1, 0, -1, -1, -1, -1, -1, 1, -1, -1, -1, 2, -1, 3, -1,
-                                -1, -1, 4, -1, 5, -1, -1, -1, 6, -1, -1, -1, -1, -1, 7};
-    const int i = idx[q % 30];
-    if (i < 0) throw std::runtime_error("stats: prime " + std::to_string(q) + " outside the classes");
-    return i;
+    constexpr uint32_t kUnits = (1u << 1) | (1u << 7) | (1u << 11) | (1u << 13) | (1u << 17) | (1u << 19) |
+                                (1u << 23) | (1u << 29);
+    const uint32_t r = uint32_t(q % 30);
+    if (!((kUnits >> r) & 1u)) return -1;
+    return __builtin_popcount(kUnits & ((1u << r) - 1));
 }
 
-int twin_idx(uint64_t q)
+// unit_slot of a prime the counts hold (every prime above 5)
+size_t prime_slot(uint64_t q)
 {
-    const uint32_t m = uint32_t(q % 30);
-    return m == 11 ? 0 : m == 17 ? 1 : 2;
+    const int s = unit_slot(q);
+    if (s < 0) throw std::runtime_error("stats: prime " + std::to_string(q) + " outside the eight classes");
+    return size_t(s);
 }
 
-void merge_ranges(std::vector<std::pair<uint64_t, uint64_t>>& r)
+// twin pair kind of its lower prime: 11 mod 30 -> 0, 17 -> 1, 29 -> 2
+int twin_slot(uint64_t q)
+{
+    switch (q % 30) {
+    case 11: return 0;
+    case 17: return 1;
+    default: return 2;
+    }
+}
+
+// sorted, and ranges that overlap or touch (the next start at most 2 above
+// the previous limit: consecutive evens) joined
+void normalize(Ranges& r)
 {
     std::sort(r.begin(), r.end());
-    std::vector<std::pair<uint64_t, uint64_t>> m;
-    for (auto& c : r) {
-        if (!m.empty() && c.first <= m.back().second + 2) m.back().second = std::max(m.back().second, c.second);
-        else m.push_back(c);
+    size_t w = 0;
+    for (size_t i = 0; i < r.size(); ++i) {
+        if (w && r[i].first <= r[w - 1].second + 2) {
+            r[w - 1].second = std::max(r[w - 1].second, r[i].second);
+        } else {
+            r[w++] = r[i];
+        }
     }
-    r = m;
+    r.resize(w);
 }
 
-// comma list by index, zero entries empty, trailing zeros cut
-std::string dense_list(const std::vector<uint64_t>& v, size_t from = 0)
+bool intersects(const Ranges& r, uint64_t a, uint64_t b, std::pair<uint64_t, uint64_t>* hit = nullptr)
 {
-    size_t last = from;
-    for (size_t i = from; i < v.size(); ++i)
-        if (v[i]) last = i + 1;
-    std::string s;
-    for (size_t i = from; i < last; ++i) {
-        if (i > from) s += ',';
-        if (v[i]) s += std::to_string(v[i]);
-    }
-    return s;
+    for (const auto& c : r)
+        if (a <= c.second && c.first <= b) {
+            if (hit) *hit = c;
+            return true;
+        }
+    return false;
 }
 
-std::vector<std::string> split_spaces(const std::string& line)
-{
-    std::vector<std::string> f;
-    size_t pos = 0;
-    for (;;) {
-        const size_t e = line.find(' ', pos);
-        f.push_back(line.substr(pos, e == std::string::npos ? std::string::npos : e - pos));
-        if (e == std::string::npos) return f;
-        pos = e + 1;
-    }
-}
+// ---- line-rule reader of the state and gap files -------------------------
 
-bool parse_u128(const std::string& t, u128& v)
-{
-    if (t.empty()) return false;
-    v = 0;
-    const u128 mx = ~u128(0);
-    for (char ch : t) {
-        if (ch < '0' || ch > '9') return false;
-        const unsigned d = unsigned(ch - '0');
-        if (v > (mx - d) / 10) return false;
-        v = v * 10 + d;
-    }
-    return true;
-}
-bool parse_u64(const std::string& t, uint64_t& v)
-{
-    u128 w = 0;
-    if (!parse_u128(t, w) || w > u128(UINT64_MAX)) return false;
-    v = uint64_t(w);
-    return true;
-}
-bool parse_u32(const std::string& t, uint32_t& v)
-{
-    uint64_t w = 0;
-    if (!parse_u64(t, w) || w > UINT32_MAX) return false;
-    v = uint32_t(w);
-    return true;
-}
-bool parse_dense(const std::string& t, size_t n, std::vector<uint64_t>& v)
-{
-    v.assign(n, 0);
-    if (t.empty()) return true;
-    size_t i = 0, pos = 0;
-    for (;;) {
-        const size_t e = t.find(',', pos);
-        const std::string x = t.substr(pos, e == std::string::npos ? std::string::npos : e - pos);
-        if (i >= n || (!x.empty() && !parse_u64(x, v[i]))) return false;
-        ++i;
-        if (e == std::string::npos) return true;
-        pos = e + 1;
-    }
-}
+using Words = std::vector<std::string_view>;
 
-std::string read_all(const std::string& path, bool& exists)
-{
-    std::FILE* f = std::fopen(path.c_str(), "rb");
-    exists = f != nullptr;
-    std::string all;
-    if (!f) return all;
-    char chunk[1 << 14];
-    size_t r;
-    while ((r = std::fread(chunk, 1, sizeof chunk, f)) > 0) all.append(chunk, r);
-    std::fclose(f);
-    return all;
-}
-
-// write + fsync a temp file, rename over path, fsync the directory
-void write_durably(const std::string& path, const std::string& out)
-{
-    const std::string tmp = path + ".tmp";
-    std::FILE* f = std::fopen(tmp.c_str(), "wb");
-    if (!f) throw std::runtime_error("stats: cannot write " + tmp);
-    bool ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
-    ok = std::fflush(f) == 0 && ok;
-    ok = ::fsync(::fileno(f)) == 0 && ok;
-    ok = std::fclose(f) == 0 && ok;
-    if (!ok || std::rename(tmp.c_str(), path.c_str()) != 0) {
-        std::remove(tmp.c_str());
-        throw std::runtime_error("stats: cannot write " + path + " (state unchanged)");
-    }
-    const size_t sl = path.find_last_of('/');
-    const std::string dir = sl == std::string::npos ? "." : (sl == 0 ? "/" : path.substr(0, sl));
-    const int dfd = ::open(dir.c_str(), O_RDONLY | O_DIRECTORY | O_CLOEXEC);
-    if (dfd >= 0) { ::fsync(dfd); ::close(dfd); }
-}
-
-struct FileLock {
-    int fd = -1;
-    explicit FileLock(const std::string& path)
-    {
-        fd = ::open(path.c_str(), O_RDWR | O_CREAT | O_CLOEXEC, 0644);
-        if (fd < 0 || ::flock(fd, LOCK_EX) != 0) throw std::runtime_error("stats: cannot lock " + path);
-    }
-    ~FileLock() { if (fd >= 0) ::close(fd); }
-    FileLock(const FileLock&) = delete;
-    FileLock& operator=(const FileLock&) = delete;
+// One record kind of a line-oriented file: lines whose first word is `key`
+// and that have between min_words and max_words words (the key included).
+// `times`: 1 = exactly once, 0 = any number, 15 = once per even class (the
+// class is the second word). `take` stores the record, false if malformed.
+struct Rule {
+    std::string_view key;
+    size_t min_words, max_words;
+    int times;
+    std::function<bool(const Words&)> take;
 };
+
+// Applies `rules` to every line of `text` (blank lines and '#' comments
+// skipped); a line no rule takes, a malformed record or a repeated one is a
+// damaged file, a missing record an incomplete one: both throw with the
+// file's name, nothing having been merged.
+void read_by_rules(const std::string& text, const std::string& what, const std::vector<Rule>& rules)
+{
+    const auto damaged = [&](std::string_view line) {
+        return std::runtime_error("stats: " + what + " has a malformed record '" +
+                                  std::string(line.substr(0, 80)) + "'; refusing to merge into it");
+    };
+    std::vector<uint32_t> seen(rules.size(), 0);
+    size_t at = 0;
+    while (at < text.size()) {
+        const size_t nl = text.find('\n', at);
+        if (nl == std::string::npos)
+            throw std::runtime_error("stats: " + what + " ends in a cut-off line; refusing to merge into it");
+        const std::string_view line(text.data() + at, nl - at);
+        at = nl + 1;
+        if (line.empty() || line.front() == '#') continue;
+        const Words w = textio::words(line);
+        size_t k = 0;
+        while (k < rules.size() && rules[k].key != w[0]) ++k;
+        if (k == rules.size() || w.size() < rules[k].min_words || w.size() > rules[k].max_words) throw damaged(line);
+        const Rule& r = rules[k];
+        uint32_t bit = 1;
+        if (r.times == 15) {
+            uint32_t cr = 0;
+            if (!textio::parse(w[1], cr) || cr >= 15) throw damaged(line);
+            bit = 1u << cr;
+        }
+        if (r.times && (seen[k] & bit)) throw damaged(line);
+        seen[k] |= bit;
+        if (!r.take(w)) throw damaged(line);
+    }
+    for (size_t k = 0; k < rules.size(); ++k) {
+        const uint32_t want = rules[k].times == 15 ? 0x7FFFu : rules[k].times == 1 ? 1u : 0u;
+        if ((seen[k] & want) != want)
+            throw std::runtime_error("stats: " + what + " lacks its '" + std::string(rules[k].key) +
+                                     "' records (cut off or damaged); refusing to merge into it");
+    }
+}
+
+// ---- count vectors --------------------------------------------------------
+// The [15][np] count vectors (~145 KB each) of a statistics object come from
+// a small per-thread free list: fresh for every run, the allocator gave their
+// pages back to the system at each free and every run faulted them in again.
+struct VectorShelf {
+    std::vector<std::vector<uint64_t>> free;
+    ~VectorShelf() = default;
+};
+thread_local VectorShelf t_shelf;
+
+void borrow(std::vector<uint64_t>& v, size_t n)
+{
+    if (!t_shelf.free.empty()) {
+        v.swap(t_shelf.free.back());
+        t_shelf.free.pop_back();
+    }
+    v.assign(n, 0);
+}
+
+void give_back(std::vector<uint64_t>& v)
+{
+    if (v.capacity() && t_shelf.free.size() < 16) {
+        t_shelf.free.emplace_back();
+        t_shelf.free.back().swap(v);
+    }
+}
 
 }  // namespace
 
-// ---------------------------------------------------------------- GapTotals
+// ---------------------------------------------------------------- GapBook
 
-void GapTotals::walk_add(uint64_t g)
+void GapBook::walk(uint64_t g, int sign)
 {
-    const u128 g1 = g, g2 = g1 * g1;
-    ++wn;
-    wd[0] += g1; wd[1] += g2; wd[2] += g2 * g1; wd[3] += g2 * g2;
-    samp.at(g / 2) += 1;
+    const size_t bin = g / 2;
+    if (bin >= walk_hist.size()) throw std::runtime_error("gaps: walked gap " + std::to_string(g) + " beyond the bins");
+    if (sign < 0 && (!walk_count || !walk_hist[bin] || g >= GAP_TAIL))
+        throw std::runtime_error("gaps: taking out a gap the walk does not hold");
+    const u128 p1 = g, p2 = p1 * p1, terms[4] = {p1, p2, p2 * p1, p2 * p2};
+    for (int k = 0; k < 4; ++k) moment[k] = sign > 0 ? moment[k] + terms[k] : moment[k] - terms[k];
+    walk_count += sign > 0 ? 1 : uint64_t(-1);
+    walk_hist[bin] += sign > 0 ? 1 : uint64_t(-1);
 }
 
-void GapTotals::walk_remove(uint64_t g)
+void GapBook::offer_top(uint64_t g, uint64_t lower)
 {
-    if (g >= GAP_TAIL || !wn || !samp.at(g / 2)) throw std::runtime_error("gaps: bad walk removal");
-    const u128 g1 = g, g2 = g1 * g1;
-    --wn;
-    wd[0] -= g1; wd[1] -= g2; wd[2] -= g2 * g1; wd[3] -= g2 * g2;
-    samp.at(g / 2) -= 1;
+    const bool larger = g > top_g, earlier_tie = g == top_g && g != 0 && lower < top_lower;
+    if (larger || earlier_tie) {
+        top_g = g;
+        top_lower = lower;
+    }
 }
 
-void GapTotals::exact_gap(uint64_t g, uint64_t lower)
+void GapBook::add_tail(uint32_t g, uint64_t n, uint64_t lower)
 {
-    if (g >= GAP_TAIL) note_tail(uint32_t(g), 1, lower);
-    note_max(g, lower);
+    Tail& t = tail[g];
+    t.first_lower = t.count ? std::min(t.first_lower, lower) : lower;
+    t.count += n;
 }
 
-void GapTotals::note_max(uint64_t g, uint64_t lower)
+void GapBook::exact(uint64_t g, uint64_t lower)
 {
-    if (g > gmax || (g == gmax && g && lower < gmax_lo)) { gmax = g; gmax_lo = lower; }
+    if (g >= GAP_TAIL) add_tail(uint32_t(g), 1, lower);
+    offer_top(g, lower);
 }
 
-void GapTotals::note_tail(uint32_t g, uint64_t count, uint64_t lower)
+void GapBook::absorb(const GapBook& o)
 {
-    auto& t = tail[g];
-    if (!t.first || lower < t.second) t.second = lower;
-    t.first += count;
-}
-
-void GapTotals::merge(const GapTotals& o)
-{
-    n += o.n;
-    d1 += o.d1;
-    for (auto& t : o.tail) note_tail(t.first, t.second.first, t.second.second);
-    note_max(o.gmax, o.gmax_lo);
+    count += o.count;
+    span += o.span;
+    for (const auto& [g, t] : o.tail) add_tail(g, t.count, t.first_lower);
+    offer_top(o.top_g, o.top_lower);
     all_walked = all_walked && o.all_walked;
-    wn += o.wn;
-    for (int k = 0; k < 4; ++k) wd[k] += o.wd[k];
-    for (size_t i = 0; i < samp.size(); ++i) samp[i] += o.samp[i];
+    walk_count += o.walk_count;
+    for (int k = 0; k < 4; ++k) moment[k] += o.moment[k];
+    for (size_t i = 0; i < walk_hist.size(); ++i) walk_hist[i] += o.walk_hist[i];
+}
+
+// ---------------------------------------------------------------- Tally
+
+void Tally::size_for(size_t np)
+{
+    borrow(hist, 15 * np);
+    borrow(tail_cls, 15 * np);
+    tail_flat.assign(np, 0);
+    first.assign(np, 0);
+}
+
+void Tally::clear()
+{
+    evens = sampled = 0;
+    std::fill(std::begin(primes), std::end(primes), 0);
+    std::fill(std::begin(twins), std::end(twins), 0);
+    for (auto* v : {&hist, &tail_cls, &tail_flat, &first}) std::fill(v->begin(), v->end(), 0);
+    beyond.clear();
+    ranges.clear();
+}
+
+void Tally::absorb(const Tally& run)
+{
+    evens += run.evens;
+    sampled += run.sampled;
+    for (int i = 0; i < 8; ++i) primes[i] += run.primes[i];
+    for (int i = 0; i < 3; ++i) twins[i] += run.twins[i];
+    std::transform(hist.begin(), hist.end(), run.hist.begin(), hist.begin(), std::plus<>());
+    std::transform(tail_cls.begin(), tail_cls.end(), run.tail_cls.begin(), tail_cls.begin(), std::plus<>());
+    std::transform(tail_flat.begin(), tail_flat.end(), run.tail_flat.begin(), tail_flat.begin(), std::plus<>());
+    for (size_t i = 0; i < first.size(); ++i)
+        if (run.first[i] && (!first[i] || run.first[i] < first[i])) first[i] = run.first[i];
+    beyond.insert(beyond.end(), run.beyond.begin(), run.beyond.end());
+    ranges.insert(ranges.end(), run.ranges.begin(), run.ranges.end());
+    normalize(ranges);
 }
 
 // ---------------------------------------------------------------- GoldbachStats
-
-namespace {
-// The [15][np] count vectors (~145 KB each) of one statistics object come
-// from a small per-thread pool and go back to it: allocated afresh for every
-// run, the allocator returned their pages to the system at each free and
-// every run faulted them in again (~50 us per vector); reused, they are only
-// cleared.
-thread_local std::vector<std::vector<uint64_t>> t_counts_pool;
-
-void take_counts(std::vector<uint64_t>& v, size_t n)
-{
-    if (!t_counts_pool.empty()) {
-        v = std::move(t_counts_pool.back());
-        t_counts_pool.pop_back();
-    }
-    v.assign(n, 0);
-}
-
-void give_counts(std::vector<uint64_t>& v)
-{
-    if (v.capacity() != 0 && t_counts_pool.size() < 16) t_counts_pool.push_back(std::move(v));
-}
-}  // namespace
 
 GoldbachStats::GoldbachStats(const std::vector<uint32_t>& cands, uint32_t tail_t, bool campaign)
     : cands_(cands), tail_t_(tail_t), campaign_(campaign)
 {
     np_ = cands.size();
-    pidx_.assign(cands.back() / 2 + 1, -1);
-    for (size_t i = 0; i < np_; ++i) pidx_[cands[i] / 2] = int32_t(i);
-    take_counts(samp_, 15 * np_);
-    take_counts(tail_, 15 * np_);
-    tail_agg_.assign(np_, 0);
-    if (campaign_) {   // a run of its own fills these at its commit
-        take_counts(cum_samp_, 15 * np_);
-        take_counts(cum_tail_, 15 * np_);
-        cum_tail_agg_.assign(np_, 0);
-    }
-    first_.assign(np_, 0);
-    run_first_.assign(np_, 0);
+    tail0_ = size_t(std::lower_bound(cands.begin(), cands.end(), tail_t) - cands.begin());
+    index_of_.assign(cands.back() / 2 + 1, -1);
+    for (size_t i = 0; i < np_; ++i) index_of_[cands[i] / 2] = int32_t(i);
+    run_.size_for(np_);
+    camp_.size_for(np_);
     want_.assign(np_, 0);
-    unknown_.assign(std::begin(kUnknownS), std::end(kUnknownS));
     state_path_ = env::path("GB_STATS_STATE", "goldbach_stats.state");
     log_path_ = env::path("GB_STATS_LOG", "goldbach_stats.log");
     rec_path_ = env::path("GB_RECORDS_FILE", "goldbach_records.txt");
     live_path_ = env::path("GB_STATS_LIVE", "goldbach_live.json");
-    if (campaign_) {
-        load_state();
-        (void)load_gaps();
+    if (campaign_) {   // damaged campaign files fail here, before any GPU work
+        read_state(camp_);
+        (void)read_gap_file();
     }
     set_run_start(~0ull);
 }
 
 GoldbachStats::~GoldbachStats()
 {
-    give_counts(samp_);
-    give_counts(tail_);
-    give_counts(cum_samp_);
-    give_counts(cum_tail_);
+    for (Tally* t : {&run_, &camp_}) {
+        give_back(t->hist);
+        give_back(t->tail_cls);
+    }
 }
 
 void GoldbachStats::set_run_start(uint64_t start)
 {
-    for (size_t i = 0; i < np_; ++i)
-        want_[i] = cands_[i] >= tail_t_ && (!first_[i] || first_[i] > start);
+    for (size_t i = 0; i < np_; ++i) {
+        const uint64_t f = camp_.first[i];
+        want_[i] = i >= tail0_ && (f == 0 || f > start);
+    }
 }
 
-uint64_t GoldbachStats::add_hist(bool sampled, const std::vector<uint64_t>& h)
+void GoldbachStats::count_hit(bool sampled, uint32_t cr, uint32_t pi, uint64_t c)
 {
-    uint64_t n = 0;
-    for (uint32_t cr = 0; cr < 15; ++cr)
-        for (size_t i = 0; i < np_; ++i) {
-            const uint64_t c = h[cr * np_ + i];
-            if (!c) continue;
-            const bool tail = cands_[i] >= tail_t_;
-            if (sampled) {
-                samp_[cr * np_ + i] += c;
-                if (tail) tail_[cr * np_ + i] += c;
-                n += c;
-            } else if (tail) {
-                tail_agg_[i] += c;
-            }
-        }
-    evens_sampled_ += n;
-    return n;
+    const size_t at = size_t(cr) * np_ + pi;
+    const bool in_tail = pi >= tail0_;
+    if (sampled) {
+        run_.hist[at] += c;
+        run_.sampled += c;
+        if (in_tail) run_.tail_cls[at] += c;
+    } else if (in_tail) {
+        run_.tail_flat[pi] += c;
+    }
 }
 
 uint64_t GoldbachStats::add_hist_slots(bool sampled, const uint32_t* counts, size_t stride,
                                        const std::vector<uint16_t>* slot_pi)
 {
-    uint64_t n = 0;
+    const uint64_t before = run_.sampled;
     for (uint32_t cr = 0; cr < 15; ++cr) {
-        const uint32_t* row = counts + cr * stride;
-        const std::vector<uint16_t>& sp = slot_pi[cr];
-        for (size_t k = 0; k < sp.size(); ++k) {
-            const uint32_t c = row[k];
-            if (!c) continue;
-            const uint32_t i = sp[k];
-            const bool tail = cands_[i] >= tail_t_;
-            if (sampled) {
-                samp_[cr * np_ + i] += c;
-                if (tail) tail_[cr * np_ + i] += c;
-                n += c;
-            } else if (tail) {
-                tail_agg_[i] += c;
-            }
-        }
+        const uint32_t* row = counts + size_t(cr) * stride;
+        const std::vector<uint16_t>& pis = slot_pi[cr];
+        for (size_t k = 0; k < pis.size(); ++k)
+            if (row[k]) count_hit(sampled, cr, pis[k], row[k]);
     }
-    evens_sampled_ += n;
-    return n;
+    return run_.sampled - before;
 }
 
 void GoldbachStats::add_first(uint64_t seg_start, const uint32_t* first)
 {
-    for (size_t i = 0; i < np_; ++i)
-        if (first[i] != ~0u && want_[i]) note_first(uint32_t(i), seg_start + 2ull * first[i]);
+    for (size_t i = tail0_; i < np_; ++i)
+        if (first[i] != ~0u && want_[i]) note_first(uint32_t(i), seg_start + 2 * uint64_t(first[i]));
 }
 
 void GoldbachStats::add_counts(const uint32_t cnt[8], const uint32_t tw[3])
 {
-    for (int i = 0; i < 8; ++i) primes_res_[i] += cnt[i];
-    for (int i = 0; i < 3; ++i) twins_[i] += tw[i];
+    for (int i = 0; i < 8; ++i) run_.primes[i] += cnt[i];
+    for (int i = 0; i < 3; ++i) run_.twins[i] += tw[i];
 }
 
 void GoldbachStats::begin_gaps(uint64_t start)
 {
-    gap_last_ = 0;             // prev_prime(start - 3), found when the first segment arrives
-    gap_start_ = start;
+    gaps_from_ = start;
+    last_prime_ = 0;   // prev_prime(start - 3), looked up when the first segment arrives
 }
 
 void GoldbachStats::add_gap_segment(const SegmentGaps& s)
 {
-    GapTotals& r = gaps_;
-    gap_last_walked_ = s.walked;
-    r.all_walked = r.all_walked && s.walked;
-    if (s.walked) walk_evens_ += s.evens;
+    GapBook& b = gaps_;
+    last_walked_ = s.walked;
+    b.all_walked = b.all_walked && s.walked;
+    if (s.walked) walked_evens_ += s.evens;
+    // the device's inner tail gaps, those whose upper prime is in the run
     for (uint32_t i = 0; i < s.ntail; ++i) {
-        const uint64_t up = s.q_first + s.tail[2 * i], g = s.tail[2 * i + 1];
-        if (up >= gap_start_ + 1) r.exact_gap(g, up - g);
+        const uint64_t upper = s.q_first + s.tail[2 * i], g = s.tail[2 * i + 1];
+        if (upper > gaps_from_) b.exact(g, upper - g);
     }
-    if (!s.primes) {
-        if (s.pfirst != ~0u || s.gmax) throw std::runtime_error("gaps: a prime position in a segment without primes");
+    if (s.primes == 0) {
+        if (s.pfirst != ~0u || s.gmax != 0)
+            throw std::runtime_error("gaps: a segment without primes reports a prime position");
         return;
     }
-    const uint64_t fq = s.q_first + s.pfirst, lq = s.q_first + s.plast;
-    if (!gap_last_) gap_last_ = prev_prime(gap_start_ - 3);
+    const uint64_t first_q = s.q_first + s.pfirst, last_q = s.q_first + s.plast;
+    if (last_prime_ == 0) last_prime_ = prev_prime(gaps_from_ - 3);
     if (s.walked) {
-        std::map<uint32_t, uint64_t> by_g;
-        for (uint32_t i = 0; i < s.ntail; ++i) by_g[s.tail[2 * i + 1]] += 1;
-        u128 m[4] = {};
-        uint64_t hs = 0, ht = 0;
-        bool tail_ok = true;
-        for (uint32_t i = 1; i + 1 < GAP_BINS; ++i) {
-            const u128 c = s.hist[i], g = 2 * i, g2 = g * g;
-            hs += s.hist[i];
-            m[0] += c * g; m[1] += c * g2; m[2] += c * g2 * g; m[3] += c * g2 * g2;
-            if (2 * i >= GAP_TAIL) {
-                ht += s.hist[i];
-                const auto it = by_g.find(2 * i);
-                tail_ok = tail_ok && s.hist[i] == (it == by_g.end() ? 0 : it->second);
+        // the walked histogram must agree with the segment's own figures:
+        // one gap fewer than primes, the gaps summing to the span between
+        // its first and last prime, no gap beyond the bins, and every tail
+        // bin matching the listed tail gaps
+        std::map<uint32_t, uint64_t> listed;
+        for (uint32_t i = 0; i < s.ntail; ++i) ++listed[s.tail[2 * i + 1]];
+        uint64_t gaps = 0, tails = 0;
+        u128 sum[4] = {};
+        bool tails_agree = true;
+        for (uint32_t bin = 1; bin + 1 < GAP_BINS; ++bin) {
+            const uint64_t c = s.hist[bin];
+            const u128 g = 2 * bin, g2 = g * g;
+            gaps += c;
+            sum[0] += c * g;
+            sum[1] += c * g2;
+            sum[2] += c * g2 * g;
+            sum[3] += c * g2 * g2;
+            if (2 * bin >= GAP_TAIL) {
+                tails += c;
+                const auto it = listed.find(2 * bin);
+                tails_agree = tails_agree && c == (it == listed.end() ? 0 : it->second);
             }
         }
-        const uint64_t over = uint64_t(s.hist[GAP_BINS - 1]) + s.hist[GAP_BINS];
-        const bool ok = s.hist[0] == 0 && over == 0 && hs + 1 == s.primes && m[0] == lq - fq && tail_ok &&
-                        ht == s.ntail;
-        if (!ok)
-            throw std::runtime_error("gaps: cross-check failed (walk gaps " + std::to_string(hs) + ", primes " +
-                                     std::to_string(s.primes) + ", sum " + u128_str(m[0]) + ", span " +
-                                     std::to_string(lq - fq) + ", overflow " + std::to_string(over) + ", tails " +
-                                     std::to_string(ht) + "/" + std::to_string(s.ntail) + (tail_ok ? "" : " by g") + ")");
-        r.wn += hs;
-        for (int k = 0; k < 4; ++k) r.wd[k] += m[k];
-        for (uint32_t i = 1; i < GAP_BINS; ++i) r.samp[i] += s.hist[i];
-        if (hs) {
-            const uint64_t g = s.gmax >> 32, up = s.q_first + uint32_t(~uint32_t(s.gmax));
-            if (!g || g >= 2 * GAP_BINS) throw std::runtime_error("gaps: walk maximum out of range");
-            r.note_max(g, up - g);
+        const uint64_t beyond_bins = uint64_t(s.hist[GAP_BINS - 1]) + s.hist[GAP_BINS];
+        if (s.hist[0] || beyond_bins || gaps + 1 != s.primes || sum[0] != last_q - first_q || !tails_agree ||
+            tails != s.ntail)
+            throw std::runtime_error("gaps: the walk does not add up (gaps " + std::to_string(gaps) + " for " +
+                                     std::to_string(s.primes) + " primes, sum " + dec(sum[0]) + " over a span of " +
+                                     std::to_string(last_q - first_q) + ", " + std::to_string(beyond_bins) +
+                                     " beyond the bins, " + std::to_string(tails) + " tail gaps of " +
+                                     std::to_string(s.ntail) + " listed" + (tails_agree ? "" : ", by size") + ")");
+        b.walk_count += gaps;
+        for (int k = 0; k < 4; ++k) b.moment[k] += sum[k];
+        for (uint32_t bin = 1; bin < GAP_BINS; ++bin) b.walk_hist[bin] += s.hist[bin];
+        if (gaps) {
+            const uint64_t g = s.gmax >> 32, upper = s.q_first + uint32_t(~uint32_t(s.gmax));
+            if (g == 0 || g >= 2 * GAP_BINS) throw std::runtime_error("gaps: the walk's largest gap is out of range");
+            b.offer_top(g, upper - g);
         }
     }
-    if (fq >= gap_start_ + 1) {
-        r.exact_gap(fq - gap_last_, gap_last_);
-        if (s.walked) r.walk_add(fq - gap_last_);
+    // the gap into the segment's first prime, from the last prime before it
+    if (first_q > gaps_from_) {
+        const uint64_t g = first_q - last_prime_;
+        b.exact(g, last_prime_);
+        if (s.walked) b.walk(g, +1);
     }
-    gap_last_ = lq;
+    last_prime_ = last_q;
 }
 
 void GoldbachStats::add_cpu(uint64_t n, uint64_t p, bool sampled)
 {
-    const uint32_t cr = uint32_t(n % 30) / 2;
-    if (p <= cands_.back() && (p & 1) && pidx_[p / 2] >= 0) {
-        const uint32_t pi = uint32_t(pidx_[p / 2]);
-        if (sampled) { samp_[cr * np_ + pi] += 1; ++evens_sampled_; }
-        if (p >= tail_t_) {
-            if (sampled) tail_[cr * np_ + pi] += 1;
-            else tail_agg_[pi] += 1;
-            note_first(pi, n);
-        }
-    } else {
-        beyond_.push_back({n, p});
-        if (sampled) ++evens_sampled_;
-        const bool rec = p > RECORD_P;
-        record(n, p, rec ? "RECORD" : "BEYOND",
-               rec ? "NEW RECORD: p(n) > 9781, the largest p(n) known below 4e18" : "p(n) beyond the candidate list");
+    const int32_t pi = (p & 1) && p <= cands_.back() ? index_of_[p / 2] : -1;
+    if (pi >= 0) {
+        count_hit(sampled, uint32_t(n % 30) / 2, uint32_t(pi), 1);
+        if (uint32_t(pi) >= tail0_) note_first(uint32_t(pi), n);
+        return;
     }
+    run_.beyond.push_back({n, p});
+    if (sampled) ++run_.sampled;
+    if (p > RECORD_P)
+        announce(n, p, "RECORD", "NEW RECORD: p(n) > 9781, the largest p(n) known below 4e18");
+    else
+        announce(n, p, "BEYOND", "p(n) beyond the candidate list");
 }
 
 void GoldbachStats::finish_run(uint64_t start, uint64_t limit, uint64_t evens)
 {
     start_ = start;
     limit_ = limit;
-    evens_ = evens;
-    // counted over [start - 3, limit - 3]; reported over [start + 1, limit - 1]
+    run_.evens = evens;
+    run_.ranges = {{start, limit}};
+    const auto prime = [](uint64_t q) { return is_prime_u64(q); };
+    // the device counted primes q in [start - 3, limit - 3]: the run's are
+    // [start + 1, limit - 1]
     for (uint64_t q : {start - 3, start - 1})
-        if (is_prime_u64(q)) primes_res_[res_idx(q)] -= 1;
-    if (is_prime_u64(limit - 1)) primes_res_[res_idx(limit - 1)] += 1;
-    // twins counted for q in [start - 5, limit - 5]; reported q in [start + 1, limit - 3]
+        if (prime(q)) run_.primes[prime_slot(q)] -= 1;
+    if (prime(limit - 1)) run_.primes[prime_slot(limit - 1)] += 1;
+    // and twin pairs with the lower prime in [start - 5, limit - 5]: the
+    // run's have it in [start + 1, limit - 3]
     for (uint64_t q : {start - 5, start - 3, start - 1})
-        if (is_prime_u64(q) && is_prime_u64(q + 2)) twins_[twin_idx(q)] -= 1;
-    if (is_prime_u64(limit - 3) && is_prime_u64(limit - 1)) twins_[twin_idx(limit - 3)] += 1;
-    if (gap_start_ == start) {
-        if (!gap_last_) gap_last_ = prev_prime(gap_start_ - 3);   // no segment had a prime
-        gaps_.n = run_primes();
-        if (limit >= start + 2) {
-            if (is_prime_u64(start - 3) && is_prime_u64(start - 1)) gaps_.walk_remove(2);
-            if (is_prime_u64(limit - 1)) {
-                gaps_.exact_gap(limit - 1 - gap_last_, gap_last_);
-                if (gap_last_walked_) gaps_.walk_add(limit - 1 - gap_last_);
-            }
-            uint64_t f = (start + 1) | 1;
-            while (f <= limit - 1 && !is_prime_u64(f)) f += 2;
-            if (f <= limit - 1) gaps_.d1 = prev_prime(limit) - prev_prime(f);
-        }
+        if (prime(q) && prime(q + 2)) run_.twins[twin_slot(q)] -= 1;
+    if (prime(limit - 3) && prime(limit - 1)) run_.twins[twin_slot(limit - 3)] += 1;
+    for (uint64_t& v : run_.primes)
+        if (int64_t(v) < 0) throw std::runtime_error("stats: negative prime count after the edge corrections");
+    if (gaps_from_ != start) return;
+    GapBook& b = gaps_;
+    if (last_prime_ == 0) last_prime_ = prev_prime(start - 3);   // no segment held a prime
+    b.count = run_primes();
+    if (limit < start + 2) return;
+    // start - 1 lies outside the run: the walked twin gap start - 3 -> start - 1 goes
+    if (prime(start - 3) && prime(start - 1)) b.walk(2, -1);
+    // limit - 1 lies inside it, beyond the device's counting range
+    if (prime(limit - 1)) {
+        const uint64_t g = limit - 1 - last_prime_;
+        b.exact(g, last_prime_);
+        if (last_walked_) b.walk(g, +1);
     }
+    // the sum of the gaps telescopes: last prime - the prime before the first
+    uint64_t f = (start + 1) | 1;
+    while (f <= limit - 1 && !prime(f)) f += 2;
+    if (f <= limit - 1) b.span = prev_prime(limit) - prev_prime(f);
 }
 
-uint64_t GoldbachStats::run_primes() const { return std::accumulate(primes_res_, primes_res_ + 8, uint64_t(0)); }
+uint64_t GoldbachStats::run_primes() const { return std::accumulate(std::begin(run_.primes), std::end(run_.primes), uint64_t(0)); }
 
 uint64_t GoldbachStats::run_tail() const
 {
-    uint64_t t = beyond_.size();
-    for (uint64_t v : tail_) t += v;
-    for (uint64_t v : tail_agg_) t += v;
+    uint64_t t = run_.beyond.size();
+    for (size_t cr = 0; cr < 15; ++cr)
+        for (size_t i = tail0_; i < np_; ++i) t += run_.tail_cls[cr * np_ + i];
+    for (size_t i = tail0_; i < np_; ++i) t += run_.tail_flat[i];
     return t;
 }
 
 uint64_t GoldbachStats::run_pmax() const
 {
-    uint64_t m = 0;
-    for (auto& b : beyond_) m = std::max(m, b.second);
-    if (m) return m;
-    for (size_t i = np_; i-- > 0;) {
-        if (tail_agg_[i]) return cands_[i];
-        for (uint32_t cr = 0; cr < 15; ++cr)
-            if (tail_[cr * np_ + i]) return cands_[i];
+    uint64_t top = 0;
+    for (const auto& [n, p] : run_.beyond) top = std::max(top, p);
+    if (top) return top;
+    for (size_t i = np_; i-- > tail0_;) {
+        uint64_t c = run_.tail_flat[i];
+        for (size_t cr = 0; cr < 15 && !c; ++cr) c = run_.tail_cls[cr * np_ + i];
+        if (c) return cands_[i];
     }
     return 0;
 }
 
 void GoldbachStats::note_first(uint32_t pi, uint64_t n)
 {
-    const bool fresh = !run_first_[pi];
-    if (fresh || n < run_first_[pi]) run_first_[pi] = n;
-    const uint32_t p = cands_[pi];
-    if (fresh && !first_[pi] && std::binary_search(unknown_.begin(), unknown_.end(), uint16_t(p)))
-        record(n, p, "SP",
-               "S(p) candidate: first even in this campaign with p(n) = p; S(p) was unknown (> 4e18) -- this IS "
-               "S(p) if the campaign covers [4e18, n] without a gap");
+    uint64_t& f = run_.first[pi];
+    const bool new_to_run = f == 0;
+    if (new_to_run || n < f) f = n;
+    if (new_to_run && camp_.first[pi] == 0 && sp_unknown(cands_[pi]))
+        announce(n, cands_[pi], "SP",
+                 "S(p) candidate: first even in this campaign with p(n) = p; S(p) was unknown (> 4e18) -- this IS "
+                 "S(p) if the campaign covers [4e18, n] without a gap");
 }
 
-void GoldbachStats::record(uint64_t n, uint64_t p, const char* kind, const char* what)
+void GoldbachStats::announce(uint64_t n, uint64_t p, const char* kind, const char* what)
 {
-    // re-proof by the independent ascending scan before anything is written
-    uint64_t ex = 0;
-    for (uint64_t c = 3; c <= n / 2; c += 2)
-        if (is_prime_u64(c) && is_prime_u64(n - c)) { ex = c; break; }
-    char ts[32];
-    const std::time_t t = std::time(nullptr);
-    std::strftime(ts, sizeof ts, "%Y-%m-%dT%H:%M:%S", std::localtime(&t));
-    char line[512];
-    std::snprintf(line, sizeof line, "%s p(n)=%llu n=%llu q=%llu cpu_check=%s  %s\n", ts, (unsigned long long)p,
-                  (unsigned long long)n, (unsigned long long)(n - p), ex == p ? "OK" : "MISMATCH", what);
-    std::fprintf(stderr, "\n*** %s", line);
-    if (std::FILE* f = std::fopen(rec_path_.c_str(), "ab")) { std::fputs(line, f); std::fclose(f); }
+    // re-proved by an independent ascending scan before anything is written
+    uint64_t witness = 0;
+    for (uint64_t c = 3; c <= n / 2 && !witness; c += 2)
+        if (is_prime_u64(c) && is_prime_u64(n - c)) witness = c;
+    const char* verdict = witness == p ? "OK" : "MISMATCH";
+    char stamp[32];
+    const std::time_t now = std::time(nullptr);
+    std::tm local{};
+    localtime_r(&now, &local);
+    std::strftime(stamp, sizeof stamp, "%Y-%m-%dT%H:%M:%S", &local);
+    const std::string line = std::string(stamp) + " p(n)=" + std::to_string(p) + " n=" + std::to_string(n) +
+                             " q=" + std::to_string(n - p) + " cpu_check=" + verdict + "  " + what + "\n";
+    std::fprintf(stderr, "\n*** %s", line.c_str());
+    if (!textio::append_file(rec_path_, line)) std::fprintf(stderr, "stats: cannot append to %s\n", rec_path_.c_str());
     events_.push_back({n, p});
-    const char* hk = std::getenv("GB_RECORD_HOOK");
-    std::string hook = hk ? hk : "";
-    if (!hk)
-        if (std::FILE* sf = std::fopen("scripts/notify_record.sh", "rb")) {
-            std::fclose(sf);
-            hook = "bash scripts/notify_record.sh";
-        }
+    // the notification hook: GB_RECORD_HOOK, else the campaign's own script
+    // when the working directory has one (run detached, output dropped)
+    std::string hook;
+    if (const char* h = std::getenv("GB_RECORD_HOOK")) hook = h;
+    else if (textio::read_file("scripts/notify_record.sh")) hook = "bash scripts/notify_record.sh";
     if (!hook.empty()) {
-        char cmd[1024];
-        std::snprintf(cmd, sizeof cmd, "%s %s %llu %llu %llu %s >/dev/null 2>&1 &", hook.c_str(), kind,
-                      (unsigned long long)p, (unsigned long long)n, (unsigned long long)(n - p),
-                      ex == p ? "OK" : "MISMATCH");
-        if (std::system(cmd) != 0) std::fprintf(stderr, "record hook failed to start: %s\n", hook.c_str());
+        const std::string cmd = hook + " " + kind + " " + std::to_string(p) + " " + std::to_string(n) + " " +
+                                std::to_string(n - p) + " " + verdict + " >/dev/null 2>&1 &";
+        if (std::system(cmd.c_str()) != 0) std::fprintf(stderr, "record hook failed to start: %s\n", hook.c_str());
     }
 }
 
-std::vector<uint64_t> GoldbachStats::by_prime(const std::vector<uint64_t>& cls, const std::vector<uint64_t>* agg) const
+std::vector<uint64_t> GoldbachStats::per_prime(const std::vector<uint64_t>& by_class,
+                                               const std::vector<uint64_t>* flat) const
 {
-    std::vector<uint64_t> v(np_, 0);
-    for (uint32_t cr = 0; cr < 15; ++cr)
-        for (size_t i = 0; i < np_; ++i) v[i] += cls[cr * np_ + i];
-    if (agg)
-        for (size_t i = 0; i < np_; ++i) v[i] += (*agg)[i];
+    std::vector<uint64_t> v = flat ? *flat : std::vector<uint64_t>(np_, 0);
+    for (size_t cr = 0; cr < 15; ++cr)
+        for (size_t i = 0; i < np_; ++i) v[i] += by_class[cr * np_ + i];
     return v;
 }
 
 std::string GoldbachStats::gap_fields() const
 {
-    const GapTotals& g = gaps_;
-    std::string o = " gaps=" + std::to_string(g.n) + " gd1=" + u128_str(g.d1) + " gmax=" +
-                    (g.gmax_exact() && g.gmax ? std::to_string(g.gmax) + ":" + std::to_string(g.gmax_lo) : "-") +
-                    " gtail=";
-    std::string t;
-    for (auto& e : g.tail)
-        t += (t.empty() ? "" : ";") + std::to_string(e.first) + ":" + std::to_string(e.second.first) + ":" +
-             std::to_string(e.second.second);
-    o += t.empty() ? "-" : t;
-    o += " gwalk=" + std::to_string(walk_evens_) + ":" + std::to_string(g.wn) + " gsmom=";
-    for (int k = 0; k < 4; ++k) o += (k ? "," : "") + u128_str(g.wd[k]);
-    const std::string h = dense_list(g.samp);
-    return o + " gsamp=" + (h.empty() ? std::string("-") : h);
+    const GapBook& b = gaps_;
+    std::string s = " gaps=" + std::to_string(b.count) + " gd1=" + dec(b.span) + " gmax=";
+    s += b.top_exact() && b.top_g ? std::to_string(b.top_g) + ":" + std::to_string(b.top_lower) : "-";
+    std::string tails;
+    for (const auto& [g, t] : b.tail) {
+        if (!tails.empty()) tails += ';';
+        tails += std::to_string(g) + ":" + std::to_string(t.count) + ":" + std::to_string(t.first_lower);
+    }
+    s += " gtail=" + (tails.empty() ? "-" : tails);
+    s += " gwalk=" + std::to_string(walked_evens_) + ":" + std::to_string(b.walk_count) + " gsmom=" + dec(b.moment[0]) +
+         "," + dec(b.moment[1]) + "," + dec(b.moment[2]) + "," + dec(b.moment[3]);
+    const std::string h = count_list(b.walk_hist);
+    return s + " gsamp=" + (h.empty() ? "-" : h);
 }
 
 std::string GoldbachStats::log_line(const char* version) const
 {
-    std::string line = "GBSTATS v2 version=" + std::string(version);
-    line += " start=" + std::to_string(start_) + " limit=" + std::to_string(limit_) + " evens=" +
-            std::to_string(evens_) + " evens_sampled=" + std::to_string(evens_sampled_) + " tail_t=" +
-            std::to_string(tail_t_) + " tail_evens=" + std::to_string(run_tail()) + " pmax=" +
-            std::to_string(run_pmax()) + " primes=" + std::to_string(run_primes()) + " twins=" +
-            std::to_string(run_twins());
-    line += " primes30=";
-    for (int i = 0; i < 8; ++i) line += (i ? "," : "") + std::to_string(primes_res_[i]);
-    line += " twins30=";
-    for (int i = 0; i < 3; ++i) line += (i ? "," : "") + std::to_string(twins_[i]);
-    std::string nf;
-    for (size_t i = 0; i < np_; ++i)
-        if (run_first_[i] && (!first_[i] || run_first_[i] < first_[i]))
-            nf += (nf.empty() ? "" : ";") + std::to_string(cands_[i]) + ":" + std::to_string(run_first_[i]);
-    line += " new_first=" + (nf.empty() ? std::string("-") : nf);
-    std::string bey;
-    for (auto& e : beyond_) bey += (bey.empty() ? "" : ";") + std::to_string(e.second) + ":" + std::to_string(e.first);
-    line += " beyond=" + (bey.empty() ? std::string("-") : bey);
-    if (gap_start_ == start_) line += gap_fields();
-    size_t t0 = 0;
-    while (t0 < np_ && cands_[t0] < tail_t_) ++t0;
-    const std::string tl = t0 < np_ ? dense_list(by_prime(tail_, &tail_agg_), t0) : "";
-    line += " tail=" + (tl.empty() ? std::string("-") : std::to_string(cands_[t0]) + ":" + tl) + "\n";
-    return line;
-}
-
-void GoldbachStats::append_log(const std::string& line) const
-{
-    std::FILE* f = std::fopen(log_path_.c_str(), "ab");
-    if (!f) { std::fprintf(stderr, "stats: cannot append to %s\n", log_path_.c_str()); return; }
-    bool ok = std::fwrite(line.data(), 1, line.size(), f) == line.size();
-    ok = std::fflush(f) == 0 && ok;
-    ok = ::fsync(::fileno(f)) == 0 && ok;
-    if (std::fclose(f) != 0 || !ok) std::fprintf(stderr, "stats: incomplete append to %s\n", log_path_.c_str());
-}
-
-bool GoldbachStats::log_has_gaps_in(uint64_t a, uint64_t b) const
-{
-    bool ex = false;
-    const std::string all = read_all(log_path_, ex);
-    size_t pos = 0;
-    while (pos < all.size()) {
-        size_t e = all.find('\n', pos);
-        if (e == std::string::npos) e = all.size();
-        const std::string ln = all.substr(pos, e - pos);
-        pos = e + 1;
-        if (ln.compare(0, 2, "GB") != 0 || ln.find(" gd1=") == std::string::npos) continue;
-        const size_t s = ln.find(" start="), l = ln.find(" limit=");
-        if (s != std::string::npos && l != std::string::npos &&
-            std::strtoull(ln.c_str() + s + 7, nullptr, 10) <= b && a <= std::strtoull(ln.c_str() + l + 7, nullptr, 10))
-            return true;
-    }
-    return false;
-}
-
-GoldbachStats::GapCampaign GoldbachStats::load_gaps() const
-{
-    GapCampaign gc;
-    const std::string gp = state_path_ + ".gaps";
-    bool ex = false;
-    const std::string all = read_all(gp, ex);
-    if (!ex) return gc;
-    unsigned seen = 0;
-    size_t pos = 0;
-    const auto damaged = [&](const std::string& line) {
-        return std::runtime_error("stats: gap state " + gp + " has a malformed record '" + line.substr(0, 80) +
-                                  "'; refusing to merge into it");
+    const auto field = [](const char* k, const std::string& v) { return std::string(" ") + k + "=" + v; };
+    const auto csv = [](const uint64_t* v, size_t n) {
+        std::string s;
+        for (size_t i = 0; i < n; ++i) s += (i ? "," : "") + std::to_string(v[i]);
+        return s;
     };
-    while (pos < all.size()) {
-        const size_t e = all.find('\n', pos);
-        if (e == std::string::npos) break;
-        const std::string buf = all.substr(pos, e - pos);
-        pos = e + 1;
-        if (buf.empty() || buf[0] == '#') continue;
-        const std::vector<std::string> fl = split_spaces(buf);
-        const std::string& k = fl[0];
-        const size_t nf = fl.size();
-        const auto once = [&](unsigned bit) {
-            if (seen & bit) throw damaged(buf);
-            seen |= bit;
-        };
-        bool ok = true;
-        uint64_t a = 0, c = 0;
-        if (k == "gaps_range" && nf == 3) {
-            ok = parse_u64(fl[1], a) && parse_u64(fl[2], c) && a <= c;
-            gc.covered.push_back({a, c});
-            seen |= 1;
-        } else if (k == "gaps_evens" && nf == 3) {
-            once(2);
-            ok = parse_u64(fl[1], gc.evens) && parse_u64(fl[2], gc.walked);
-        } else if (k == "gaps" && nf == 3) {
-            once(4);
-            ok = parse_u64(fl[1], gc.g.n) && parse_u128(fl[2], gc.g.d1);
-        } else if (k == "gaps_samp" && nf <= 2) {
-            once(8);
-            std::vector<uint64_t> v;
-            ok = parse_dense(nf == 2 ? fl[1] : std::string(), GAP_BINS, v);
-            std::copy(v.begin(), v.end(), gc.g.samp.begin());
-        } else if (k == "gaps_max" && nf == 4) {
-            once(16);
-            ok = parse_u64(fl[1], a) && parse_u64(fl[2], c) && (fl[3] == "0" || fl[3] == "1");
-            gc.g.gmax = a;
-            gc.g.gmax_lo = c;
-            gc.g.all_walked = fl[3] == "1" && a < GAP_TAIL;
-        } else if (k == "gaps_walk" && nf == 6) {
-            once(64);
-            ok = parse_u64(fl[1], gc.g.wn);
-            for (int i = 0; i < 4 && ok; ++i) ok = parse_u128(fl[2 + size_t(i)], gc.g.wd[i]);
-        } else if (k == "gaps_tail" && nf == 4) {
-            uint32_t g = 0;
-            ok = parse_u32(fl[1], g) && parse_u64(fl[2], a) && parse_u64(fl[3], c);
-            if (ok) gc.g.note_tail(g, a, c);
-        } else if (k == "end" && nf == 1) {
-            once(32);
-        } else {
-            ok = false;
-        }
-        if (!ok) throw damaged(buf);
+    std::string line = "GBSTATS v2";
+    line += field("version", version);
+    line += field("start", std::to_string(start_)) + field("limit", std::to_string(limit_));
+    line += field("evens", std::to_string(run_.evens)) + field("evens_sampled", std::to_string(run_.sampled));
+    line += field("tail_t", std::to_string(tail_t_)) + field("tail_evens", std::to_string(run_tail()));
+    line += field("pmax", std::to_string(run_pmax())) + field("primes", std::to_string(run_primes()));
+    line += field("twins", std::to_string(run_twins()));
+    line += field("primes30", csv(run_.primes, 8)) + field("twins30", csv(run_.twins, 3));
+    std::string firsts;
+    for (size_t i = 0; i < np_; ++i) {
+        const uint64_t f = run_.first[i], c = camp_.first[i];
+        if (f && (!c || f < c)) firsts += (firsts.empty() ? "" : ";") + std::to_string(cands_[i]) + ":" + std::to_string(f);
     }
-    if (seen != 127)
-        throw std::runtime_error("stats: gap state " + gp + " is cut off or missing records; refusing to merge into it");
-    merge_ranges(gc.covered);
-    return gc;
+    line += field("new_first", firsts.empty() ? "-" : firsts);
+    std::string bey;
+    for (const auto& [n, p] : run_.beyond) bey += (bey.empty() ? "" : ";") + std::to_string(p) + ":" + std::to_string(n);
+    line += field("beyond", bey.empty() ? "-" : bey);
+    if (gaps_from_ == start_) line += gap_fields();
+    const std::vector<uint64_t> t = per_prime(run_.tail_cls, &run_.tail_flat);
+    const std::string tl = tail0_ < np_ ? count_list(t.data() + tail0_, np_ - tail0_) : "";
+    line += field("tail", tl.empty() ? "-" : std::to_string(cands_[tail0_]) + ":" + tl);
+    return line + "\n";
 }
 
-bool GoldbachStats::gaps_cover(uint64_t a, uint64_t b) const
+void GoldbachStats::log_append(const std::string& line) const
 {
-    for (auto& c : load_gaps().covered)
-        if (c.first <= a && b <= c.second) return true;
+    if (!textio::append_file(log_path_, line)) std::fprintf(stderr, "stats: cannot append to %s\n", log_path_.c_str());
+}
+
+bool GoldbachStats::log_has_gap_row(uint64_t a, uint64_t b) const
+{
+    const std::optional<std::string> all = textio::read_file(log_path_);
+    if (!all) return false;
+    size_t at = 0;
+    while (at < all->size()) {
+        size_t nl = all->find('\n', at);
+        if (nl == std::string::npos) nl = all->size();
+        const std::string_view line(all->data() + at, nl - at);
+        at = nl + 1;
+        // rows with gap fields: GBSTATS lines of gap runs and GBGAPS rows
+        if (line.substr(0, 2) != "GB" || line.find(" gd1=") == std::string_view::npos) continue;
+        uint64_t s = 0, l = 0;
+        bool has_s = false, has_l = false;
+        for (std::string_view w : textio::words(line)) {
+            if (w.substr(0, 6) == "start=") has_s = textio::parse(w.substr(6), s);
+            else if (w.substr(0, 6) == "limit=") has_l = textio::parse(w.substr(6), l);
+        }
+        if (has_s && has_l && s <= b && a <= l) return true;
+    }
     return false;
 }
 
-bool GoldbachStats::commit_gaps()
+// ---- the state file v2 ------------------------------------------------------
+
+std::string GoldbachStats::state_text(const Tally& t) const
 {
-    GapCampaign gc = load_gaps();
-    for (auto& c : gc.covered)
-        if (start_ <= c.second && c.first <= limit_) {
-            std::fprintf(stderr, "stats: gaps of [%llu, %llu] already in %s -- not recorded again\n",
-                         (unsigned long long)start_, (unsigned long long)limit_, (state_path_ + ".gaps").c_str());
-            return false;
-        }
-    gc.g.merge(gaps_);
-    gc.evens += evens_;
-    gc.walked += walk_evens_;
-    gc.covered.push_back({start_, limit_});
-    merge_ranges(gc.covered);
-    std::string out =
+    std::string s =
+        "# goldbach-l0 statistics state v2 (exact; include/gstats.hpp)\n"
+        "# one record per line; count lists are indexed by odd prime (3, 5, 7, ...), an empty entry is 0\n"
+        "# sampled <n mod 30 / 2> <list>: p(n) of every even of the sampled segments\n"
+        "# tailcls <n mod 30 / 2> <list>, tailagg <list>: p(n) >= tail_t (sampled evens by class, the rest)\n"
+        "# first <p> <n>: the campaign's first even with p(n) = p, for p >= tail_t\n";
+    s += "tail_t " + std::to_string(tail_t_) + "\n";
+    for (const auto& [a, b] : t.ranges) s += "range " + std::to_string(a) + " " + std::to_string(b) + "\n";
+    s += "evens " + std::to_string(t.evens) + " " + std::to_string(t.sampled) + "\n";
+    s += "primes30";
+    for (uint64_t v : t.primes) s += " " + std::to_string(v);
+    s += "\ntwins30";
+    for (uint64_t v : t.twins) s += " " + std::to_string(v);
+    s += "\n";
+    for (size_t i = 0; i < np_; ++i)
+        if (t.first[i]) s += "first " + std::to_string(cands_[i]) + " " + std::to_string(t.first[i]) + "\n";
+    for (const auto& [n, p] : t.beyond) s += "beyond " + std::to_string(p) + " " + std::to_string(n) + "\n";
+    for (size_t cr = 0; cr < 15; ++cr)
+        s += "sampled " + std::to_string(cr) + " " + count_list(t.hist.data() + cr * np_, np_) + "\n";
+    for (size_t cr = 0; cr < 15; ++cr)
+        s += "tailcls " + std::to_string(cr) + " " + count_list(t.tail_cls.data() + cr * np_, np_) + "\n";
+    return s + "tailagg " + count_list(t.tail_flat) + "\n";
+}
+
+void GoldbachStats::read_state(Tally& t) const
+{
+    t.clear();
+    const std::optional<std::string> text = textio::read_file(state_path_);
+    if (!text) return;   // a fresh campaign
+    const std::string what = "state " + state_path_;
+    if (text->empty()) throw std::runtime_error("stats: " + what + " is empty (truncated write?); refusing to merge into it");
+    const auto u64 = [](std::string_view w, uint64_t& v) { return textio::parse(w, v); };
+    const std::vector<Rule> rules = {
+        {"tail_t", 2, 2, 1, [&](const Words& w) {
+             uint32_t v = 0;
+             if (!textio::parse(w[1], v)) return false;
+             if (v != tail_t_)
+                 throw std::runtime_error("stats: " + what + " was recorded with tail_t " + std::to_string(v) +
+                                          ", this run uses " + std::to_string(tail_t_));
+             return true;
+         }},
+        {"range", 3, 3, 0, [&](const Words& w) {
+             uint64_t a = 0, b = 0;
+             if (!u64(w[1], a) || !u64(w[2], b) || a > b) return false;
+             t.ranges.push_back({a, b});
+             return true;
+         }},
+        {"evens", 3, 3, 1, [&](const Words& w) { return u64(w[1], t.evens) && u64(w[2], t.sampled); }},
+        {"primes30", 9, 9, 1, [&](const Words& w) {
+             for (size_t i = 0; i < 8; ++i)
+                 if (!u64(w[1 + i], t.primes[i])) return false;
+             return true;
+         }},
+        {"twins30", 4, 4, 1, [&](const Words& w) {
+             for (size_t i = 0; i < 3; ++i)
+                 if (!u64(w[1 + i], t.twins[i])) return false;
+             return true;
+         }},
+        {"first", 3, 3, 0, [&](const Words& w) {
+             uint32_t p = 0;
+             uint64_t n = 0;
+             if (!textio::parse(w[1], p) || !u64(w[2], n) || !(p & 1) || p > cands_.back() || index_of_[p / 2] < 0)
+                 return false;
+             t.first[size_t(index_of_[p / 2])] = n;
+             return true;
+         }},
+        {"beyond", 3, 3, 0, [&](const Words& w) {
+             uint64_t p = 0, n = 0;
+             if (!u64(w[1], p) || !u64(w[2], n)) return false;
+             t.beyond.push_back({n, p});
+             return true;
+         }},
+        {"sampled", 3, 3, 15, [&](const Words& w) {
+             uint32_t cr = 0;
+             return textio::parse(w[1], cr) && textio::parse_count_list(w[2], t.hist.data() + cr * np_, np_);
+         }},
+        {"tailcls", 3, 3, 15, [&](const Words& w) {
+             uint32_t cr = 0;
+             return textio::parse(w[1], cr) && textio::parse_count_list(w[2], t.tail_cls.data() + cr * np_, np_);
+         }},
+        {"tailagg", 2, 2, 1, [&](const Words& w) { return textio::parse_count_list(w[1], t.tail_flat.data(), np_); }},
+    };
+    read_by_rules(*text, what, rules);
+    normalize(t.ranges);
+}
+
+// ---- the gap state file v1 ---------------------------------------------------
+
+GoldbachStats::GapFile GoldbachStats::read_gap_file() const
+{
+    GapFile f;
+    const std::string path = state_path_ + ".gaps";
+    const std::optional<std::string> text = textio::read_file(path);
+    if (!text) return f;
+    GapBook& b = f.book;
+    const auto u64 = [](std::string_view w, uint64_t& v) { return textio::parse(w, v); };
+    const std::vector<Rule> rules = {
+        {"gaps_range", 3, 3, 0, [&](const Words& w) {
+             uint64_t a = 0, c = 0;
+             if (!u64(w[1], a) || !u64(w[2], c) || a > c) return false;
+             f.ranges.push_back({a, c});
+             return true;
+         }},
+        {"gaps_evens", 3, 3, 1, [&](const Words& w) { return u64(w[1], f.evens) && u64(w[2], f.walked); }},
+        {"gaps", 3, 3, 1, [&](const Words& w) { return u64(w[1], b.count) && textio::parse(w[2], b.span); }},
+        {"gaps_walk", 6, 6, 1, [&](const Words& w) {
+             if (!u64(w[1], b.walk_count)) return false;
+             for (size_t k = 0; k < 4; ++k)
+                 if (!textio::parse(w[2 + k], b.moment[k])) return false;
+             return true;
+         }},
+        {"gaps_samp", 1, 2, 1, [&](const Words& w) {
+             return textio::parse_count_list(w.size() == 2 ? w[1] : std::string_view(), b.walk_hist.data(), GAP_BINS);
+         }},
+        {"gaps_max", 4, 4, 1, [&](const Words& w) {
+             if (!u64(w[1], b.top_g) || !u64(w[2], b.top_lower) || (w[3] != "0" && w[3] != "1")) return false;
+             b.all_walked = w[3] == "1" && b.top_g < GAP_TAIL;
+             return true;
+         }},
+        {"gaps_tail", 4, 4, 0, [&](const Words& w) {
+             uint32_t g = 0;
+             uint64_t n = 0, lower = 0;
+             if (!textio::parse(w[1], g) || !u64(w[2], n) || !u64(w[3], lower)) return false;
+             b.add_tail(g, n, lower);
+             return true;
+         }},
+        {"end", 1, 1, 1, [](const Words&) { return true; }},
+    };
+    read_by_rules(*text, "gap state " + path, rules);
+    if (f.ranges.empty())
+        throw std::runtime_error("stats: gap state " + path + " lacks its 'gaps_range' records (cut off or damaged); "
+                                 "refusing to merge into it");
+    normalize(f.ranges);
+    return f;
+}
+
+bool GoldbachStats::commit_gap_file()
+{
+    GapFile f = read_gap_file();
+    const std::string path = state_path_ + ".gaps";
+    if (intersects(f.ranges, start_, limit_)) {
+        std::fprintf(stderr, "stats: gaps of [%llu, %llu] already in %s -- not recorded again\n",
+                     (unsigned long long)start_, (unsigned long long)limit_, path.c_str());
+        return false;
+    }
+    f.book.absorb(gaps_);
+    f.evens += run_.evens;
+    f.walked += walked_evens_;
+    f.ranges.push_back({start_, limit_});
+    normalize(f.ranges);
+    const GapBook& b = f.book;
+    std::string s =
         "# goldbach-l0 prime gap state v1 (exact; include/gstats.hpp GapStats)\n"
-        "# gaps_range: lo hi with prime gaps recorded (upper prime in [lo + 1, hi - 1]);\n"
-        "# gaps_evens <evens> <walked evens>; gaps <count> <sum g> (exact);\n"
-        "# gaps_walk <count> <sum g> <sum g^2> <sum g^3> <sum g^4> and gaps_samp (histogram by g / 2): walked "
-        "segments;\n"
-        "# gaps_max <g> <lower> <1: exact, 0: largest seen>; gaps_tail <g> <count> <first lower> (g >= 700, exact)\n";
-    for (auto& c : gc.covered) out += "gaps_range " + std::to_string(c.first) + " " + std::to_string(c.second) + "\n";
-    out += "gaps_evens " + std::to_string(gc.evens) + " " + std::to_string(gc.walked) + "\n";
-    out += "gaps " + std::to_string(gc.g.n) + " " + u128_str(gc.g.d1) + "\ngaps_walk " + std::to_string(gc.g.wn);
-    for (int k = 0; k < 4; ++k) out += " " + u128_str(gc.g.wd[k]);
-    out += "\ngaps_samp " + dense_list(gc.g.samp) + "\n";
-    out += "gaps_max " + std::to_string(gc.g.gmax) + " " + std::to_string(gc.g.gmax_lo) + " " +
-           (gc.g.gmax_exact() ? "1" : "0") + "\n";
-    for (auto& t : gc.g.tail)
-        out += "gaps_tail " + std::to_string(t.first) + " " + std::to_string(t.second.first) + " " +
-               std::to_string(t.second.second) + "\n";
-    out += "end\n";
-    write_durably(state_path_ + ".gaps", out);
+        "# gaps_range <lo> <hi>: prime gaps recorded for the upper prime in [lo + 1, hi - 1]\n"
+        "# gaps_evens <evens> <walked evens>; gaps <count> <sum of g> (every gap, exact)\n"
+        "# gaps_walk <count> <sums of g, g^2, g^3, g^4>, gaps_samp <histogram by g / 2>: the walked segments\n"
+        "# gaps_max <g> <lower> <1: the largest, 0: the largest seen>; gaps_tail <g> <count> <first lower>: g >= 700\n";
+    for (const auto& [a, c] : f.ranges) s += "gaps_range " + std::to_string(a) + " " + std::to_string(c) + "\n";
+    s += "gaps_evens " + std::to_string(f.evens) + " " + std::to_string(f.walked) + "\n";
+    s += "gaps " + std::to_string(b.count) + " " + dec(b.span) + "\n";
+    s += "gaps_walk " + std::to_string(b.walk_count);
+    for (const u128& m : b.moment) s += " " + dec(m);
+    s += "\ngaps_samp " + count_list(b.walk_hist) + "\n";
+    s += "gaps_max " + std::to_string(b.top_g) + " " + std::to_string(b.top_lower) + (b.top_exact() ? " 1\n" : " 0\n");
+    for (const auto& [g, t] : b.tail)
+        s += "gaps_tail " + std::to_string(g) + " " + std::to_string(t.count) + " " + std::to_string(t.first_lower) + "\n";
+    textio::replace_file(path, s + "end\n");
     return true;
 }
 
-void GoldbachStats::reset_campaign()
-{
-    std::fill(cum_samp_.begin(), cum_samp_.end(), 0);
-    std::fill(cum_tail_.begin(), cum_tail_.end(), 0);
-    std::fill(cum_tail_agg_.begin(), cum_tail_agg_.end(), 0);
-    std::fill(first_.begin(), first_.end(), 0);
-    covered_.clear();
-    tot_beyond_.clear();
-    tot_evens_ = tot_sampled_ = 0;
-    for (auto& v : tot_primes_) v = 0;
-    for (auto& v : tot_twins_) v = 0;
-}
-
-void GoldbachStats::load_state()
-{
-    bool ex = false;
-    const std::string all = read_all(state_path_, ex);
-    if (!ex) return;
-    if (all.empty() || all.back() != '\n')
-        throw std::runtime_error("stats: state " + state_path_ +
-                                 " is empty or cut off (truncated write?); refusing to merge into it");
-    uint32_t seen_samp = 0, seen_tail = 0;
-    bool seen_t = false, seen_ev = false, seen_p = false, seen_tw = false, seen_agg = false;
-    const auto damaged = [&](const std::string& line) {
-        return std::runtime_error("stats: state " + state_path_ + " has a malformed record '" + line.substr(0, 80) +
-                                  "' (damaged); refusing to merge into it");
-    };
-    const auto once = [&](bool& seen, const std::string& line) {
-        if (seen) throw damaged(line);
-        seen = true;
-    };
-    size_t pos = 0;
-    while (pos < all.size()) {
-        size_t e = all.find('\n', pos);
-        if (e == std::string::npos) e = all.size();
-        const std::string buf = all.substr(pos, e - pos);
-        pos = e + 1;
-        if (buf.empty() || buf[0] == '#') continue;
-        const std::vector<std::string> fl = split_spaces(buf);
-        const std::string& k = fl[0];
-        const size_t nf = fl.size();
-        bool ok = true;
-        uint64_t a = 0, c = 0;
-        uint32_t p = 0, cr = 0;
-        if (k == "range" && nf == 3) {
-            ok = parse_u64(fl[1], a) && parse_u64(fl[2], c) && a <= c;
-            covered_.push_back({a, c});
-        } else if (k == "tail_t" && nf == 2) {
-            once(seen_t, buf);
-            ok = parse_u32(fl[1], p);
-            if (ok && p != tail_t_)
-                throw std::runtime_error("stats: state " + state_path_ + " was recorded with tail_t " +
-                                         std::to_string(p) + ", this run uses " + std::to_string(tail_t_));
-        } else if (k == "evens" && nf == 3) {
-            once(seen_ev, buf);
-            ok = parse_u64(fl[1], tot_evens_) && parse_u64(fl[2], tot_sampled_);
-        } else if (k == "first" && nf == 3) {
-            ok = parse_u32(fl[1], p) && parse_u64(fl[2], a) && p <= cands_.back() && (p & 1u) && pidx_[p / 2] >= 0;
-            if (ok) first_[size_t(pidx_[p / 2])] = a;
-        } else if (k == "beyond" && nf == 3) {
-            ok = parse_u64(fl[1], c) && parse_u64(fl[2], a);
-            tot_beyond_.push_back({a, c});
-        } else if (k == "primes30" && nf == 9) {
-            once(seen_p, buf);
-            for (size_t i = 0; i < 8 && ok; ++i) ok = parse_u64(fl[1 + i], tot_primes_[i]);
-        } else if (k == "twins30" && nf == 4) {
-            once(seen_tw, buf);
-            for (size_t i = 0; i < 3 && ok; ++i) ok = parse_u64(fl[1 + i], tot_twins_[i]);
-        } else if ((k == "sampled" || k == "tailcls") && nf == 3) {
-            uint32_t& seen = k == "sampled" ? seen_samp : seen_tail;
-            std::vector<uint64_t> v;
-            ok = parse_u32(fl[1], cr) && cr < 15 && !((seen >> cr) & 1u) && parse_dense(fl[2], np_, v);
-            if (ok) {
-                std::copy(v.begin(), v.end(), (k == "sampled" ? cum_samp_ : cum_tail_).begin() + cr * np_);
-                seen |= 1u << cr;
-            }
-        } else if (k == "tailagg" && nf == 2) {
-            once(seen_agg, buf);
-            std::vector<uint64_t> v;
-            ok = parse_dense(fl[1], np_, v);
-            if (ok) cum_tail_agg_ = v;
-        } else {
-            ok = false;
-        }
-        if (!ok) throw damaged(buf);
-    }
-    if (!(seen_t && seen_ev && seen_p && seen_tw && seen_agg && seen_samp == 0x7FFFu && seen_tail == 0x7FFFu))
-        throw std::runtime_error("stats: state " + state_path_ +
-                                 " is missing records (truncated or damaged); refusing to merge into it");
-    merge_ranges(covered_);
-}
-
-void GoldbachStats::save_state() const { write_durably(state_path_, state_text()); }
-
-std::string GoldbachStats::state_text() const
-{
-    std::string out =
-        "# goldbach-l0 statistics state v2 (exact; include/gstats.hpp)\n"
-        "# lists: counts by odd-prime index (3, 5, 7, 11, ...), empty = 0\n"
-        "# sampled <n mod 30 / 2>: p(n) histogram of the sampled segments\n"
-        "# tailcls <n mod 30 / 2> / tailagg: p(n) >= tail_t, all evens (tailagg: class not kept)\n"
-        "# first <p> <n>: first even of the campaign with p(n) = p (tail p only)\n";
-    out += "tail_t " + std::to_string(tail_t_) + "\n";
-    for (auto& c : covered_) out += "range " + std::to_string(c.first) + " " + std::to_string(c.second) + "\n";
-    out += "evens " + std::to_string(tot_evens_) + " " + std::to_string(tot_sampled_) + "\n";
-    out += "primes30";
-    for (int i = 0; i < 8; ++i) out += " " + std::to_string(tot_primes_[i]);
-    out += "\ntwins30";
-    for (int i = 0; i < 3; ++i) out += " " + std::to_string(tot_twins_[i]);
-    out += "\n";
-    for (size_t i = 0; i < np_; ++i)
-        if (first_[i]) out += "first " + std::to_string(cands_[i]) + " " + std::to_string(first_[i]) + "\n";
-    for (auto& e : tot_beyond_) out += "beyond " + std::to_string(e.second) + " " + std::to_string(e.first) + "\n";
-    for (uint32_t cr = 0; cr < 15; ++cr) {
-        std::vector<uint64_t> v(cum_samp_.begin() + cr * np_, cum_samp_.begin() + (cr + 1) * np_);
-        out += "sampled " + std::to_string(cr) + " " + dense_list(v) + "\n";
-    }
-    for (uint32_t cr = 0; cr < 15; ++cr) {
-        std::vector<uint64_t> v(cum_tail_.begin() + cr * np_, cum_tail_.begin() + (cr + 1) * np_);
-        out += "tailcls " + std::to_string(cr) + " " + dense_list(v) + "\n";
-    }
-    out += "tailagg " + dense_list(cum_tail_agg_) + "\n";
-    return out;
-}
+// ---- commit -------------------------------------------------------------------
 
 std::string GoldbachStats::memory_commit(uint64_t sampled_total)
 {
     if (campaign_) throw std::runtime_error("stats: memory_commit on a campaign run");
-    if (evens_sampled_ != sampled_total)
-        throw std::runtime_error("stats: " + std::to_string(evens_sampled_) + " sampled evens recorded, " +
+    if (run_.sampled != sampled_total)
+        throw std::runtime_error("stats: " + std::to_string(run_.sampled) + " sampled evens recorded, " +
                                  std::to_string(sampled_total) + " in the sampled segments");
-    cum_samp_ = samp_;     // a fresh campaign: its totals are this run's
-    cum_tail_ = tail_;
-    cum_tail_agg_ = tail_agg_;
-    for (size_t i = 0; i < np_; ++i)
-        if (run_first_[i] && (!first_[i] || run_first_[i] < first_[i])) first_[i] = run_first_[i];
-    for (int i = 0; i < 8; ++i) tot_primes_[i] += primes_res_[i];
-    for (int i = 0; i < 3; ++i) tot_twins_[i] += twins_[i];
-    tot_evens_ += evens_;
-    tot_sampled_ += evens_sampled_;
-    for (auto& b : beyond_) tot_beyond_.push_back(b);
-    covered_.push_back({start_, limit_});
-    return state_text();
+    camp_.clear();          // a fresh campaign that then holds this run alone
+    camp_.absorb(run_);
+    return state_text(camp_);
 }
 
 void GoldbachStats::commit(const char* version, uint64_t sampled_total, const std::string* run_line)
 {
     if (!campaign_) throw std::runtime_error("stats: commit on a run without campaign files");
-    if (evens_sampled_ != sampled_total)
-        throw std::runtime_error("stats: " + std::to_string(evens_sampled_) + " sampled evens recorded, " +
+    if (run_.sampled != sampled_total)
+        throw std::runtime_error("stats: " + std::to_string(run_.sampled) + " sampled evens recorded, " +
                                  std::to_string(sampled_total) + " in the sampled segments (statistics not written)");
-    const FileLock lock(state_path_ + ".lock");
-    // the log line names first occurrences new to the campaign: the caller's
-    // copy holds only if the state read here has the first occurrences the
-    // run started from
-    const std::vector<uint64_t> first_before = run_line ? first_ : std::vector<uint64_t>();
-    reset_campaign();
-    load_state();
-    if (run_line && first_ != first_before) run_line = nullptr;
-    for (auto& c : covered_)
-        if (start_ <= c.second && c.first <= limit_) {
-            std::fprintf(stderr, "stats: [%llu, %llu] overlaps the covered range [%llu, %llu] of %s -- not recorded again\n",
-                         (unsigned long long)start_, (unsigned long long)limit_, (unsigned long long)c.first,
-                         (unsigned long long)c.second, state_path_.c_str());
-            if (gap_start_ == start_) {
-                append_log("GBPSAMP v1 version=" + std::string(version) + " start=" + std::to_string(start_) +
-                           " limit=" + std::to_string(limit_) + " evens=" + std::to_string(evens_) +
-                           " psev=" + std::to_string(evens_sampled_) + " psamp=" + dense_list(by_prime(samp_, nullptr)) +
-                           "\n");
-                const std::string row = "GBGAPS v1 version=" + std::string(version) + " start=" + std::to_string(start_) +
-                                        " limit=" + std::to_string(limit_) + " evens=" + std::to_string(evens_) +
-                                        gap_fields() + "\n";
-                if (commit_gaps())
-                    append_log(row);
-                else if (gaps_cover(start_, limit_) && !log_has_gaps_in(start_, limit_)) {
-                    std::fprintf(stderr, "stats: %s has no gap row for [%llu, %llu] -- written now\n",
-                                 log_path_.c_str(), (unsigned long long)start_, (unsigned long long)limit_);
-                    append_log(row);
-                }
+    const textio::FileLock lock(state_path_ + ".lock");
+    // re-read under the lock: another process may have committed meanwhile;
+    // the caller's log line names first occurrences new to the campaign, so
+    // it holds only while those are unchanged
+    const std::vector<uint64_t> firsts_seen = camp_.first;
+    read_state(camp_);
+    const std::string line = run_line && camp_.first == firsts_seen ? *run_line : log_line(version);
+    std::pair<uint64_t, uint64_t> covered;
+    if (intersects(camp_.ranges, start_, limit_, &covered)) {
+        std::fprintf(stderr, "stats: [%llu, %llu] overlaps the covered range [%llu, %llu] of %s -- not recorded again\n",
+                     (unsigned long long)start_, (unsigned long long)limit_, (unsigned long long)covered.first,
+                     (unsigned long long)covered.second, state_path_.c_str());
+        if (gaps_from_ != start_) return;
+        // what the campaign may still lack of an overlapping gap run: the
+        // run's sampled histogram by prime, and its gap row
+        const std::string head = " version=" + std::string(version) + " start=" + std::to_string(start_) +
+                                 " limit=" + std::to_string(limit_) + " evens=" + std::to_string(run_.evens);
+        log_append("GBPSAMP v1" + head + " psev=" + std::to_string(run_.sampled) +
+                   " psamp=" + count_list(per_prime(run_.hist, nullptr)) + "\n");
+        const std::string gap_row = "GBGAPS v1" + head + gap_fields() + "\n";
+        if (commit_gap_file()) {
+            log_append(gap_row);
+        } else {
+            const GapFile f = read_gap_file();
+            bool inside = false;
+            for (const auto& [a, b] : f.ranges) inside = inside || (a <= start_ && limit_ <= b);
+            if (inside && !log_has_gap_row(start_, limit_)) {
+                std::fprintf(stderr, "stats: %s has no gap row for [%llu, %llu] -- written now\n", log_path_.c_str(),
+                             (unsigned long long)start_, (unsigned long long)limit_);
+                log_append(gap_row);
             }
-            return;
         }
-    const std::string line = run_line ? *run_line : log_line(version);
-    for (size_t i = 0; i < samp_.size(); ++i) { cum_samp_[i] += samp_[i]; cum_tail_[i] += tail_[i]; }
-    for (size_t i = 0; i < np_; ++i) cum_tail_agg_[i] += tail_agg_[i];
-    for (size_t i = 0; i < np_; ++i)
-        if (run_first_[i] && (!first_[i] || run_first_[i] < first_[i])) first_[i] = run_first_[i];
-    for (int i = 0; i < 8; ++i) tot_primes_[i] += primes_res_[i];
-    for (int i = 0; i < 3; ++i) tot_twins_[i] += twins_[i];
-    tot_evens_ += evens_;
-    tot_sampled_ += evens_sampled_;
-    if (gap_start_ == start_) commit_gaps();
-    for (auto& b : beyond_) tot_beyond_.push_back(b);
-    covered_.push_back({start_, limit_});
-    merge_ranges(covered_);
-    save_state();
-    append_log(line);
+        return;
+    }
+    camp_.absorb(run_);
+    if (gaps_from_ == start_) commit_gap_file();
+    textio::replace_file(state_path_, state_text(camp_));
+    log_append(line);
 }
 
 void GoldbachStats::write_live(uint64_t start, uint64_t limit, uint64_t done, uint64_t total, double elapsed, bool force)
 {
     if (!campaign_) return;
     const long long now = (long long)std::time(nullptr);
-    if (!force && now - live_t_ < 2) return;
-    live_t_ = now;
-    uint64_t pm = 0, pmn = 0, ge[3] = {};
+    if (!force && now - live_stamp_ < 2) return;
+    live_stamp_ = now;
+    // the run's largest p(n) so far and its first even; evens at or above 8000, 9000 and beyond 9781
+    uint64_t pm = 0, pm_n = 0, at_least[3] = {};
+    for (size_t i = tail0_; i < np_; ++i)
+        if (run_.first[i] && cands_[i] > pm) {
+            pm = cands_[i];
+            pm_n = run_.first[i];
+        }
+    for (const auto& [n, p] : run_.beyond)
+        if (p > pm) {
+            pm = p;
+            pm_n = n;
+        }
+    const uint64_t bounds[3] = {8000, 9000, RECORD_P + 1};
+    const std::vector<uint64_t> t = per_prime(run_.tail_cls, &run_.tail_flat);
     for (size_t i = 0; i < np_; ++i)
-        if (run_first_[i] && cands_[i] > pm) { pm = cands_[i]; pmn = run_first_[i]; }
-    for (auto& b : beyond_)
-        if (b.second > pm) { pm = b.second; pmn = b.first; }
-    const uint64_t lims[3] = {8000, 9000, RECORD_P + 1};
-    for (size_t i = 0; i < np_; ++i) {
-        uint64_t c = tail_agg_[i];
-        for (uint32_t cr = 0; cr < 15; ++cr) c += tail_[cr * np_ + i];
         for (int k = 0; k < 3; ++k)
-            if (cands_[i] >= lims[k]) ge[k] += c;
-    }
-    ge[2] += beyond_.size();
-    char b[512];
-    std::snprintf(b, sizeof b,
+            if (cands_[i] >= bounds[k]) at_least[k] += t[i];
+    at_least[2] += run_.beyond.size();
+    char head[512];
+    std::snprintf(head, sizeof head,
                   "{\"time\": %lld, \"start\": %llu, \"limit\": %llu, \"evens_done\": %llu, \"evens_total\": %llu, "
                   "\"elapsed_s\": %.1f, \"tail_t\": %u, \"tail_evens\": %llu, \"pmax\": %llu, \"pmax_n\": %llu, "
                   "\"ge8000\": %llu, \"ge9000\": %llu, \"gt9781\": %llu, \"primes\": %llu, \"twins\": %llu, "
                   "\"events\": [",
                   now, (unsigned long long)start, (unsigned long long)limit, (unsigned long long)done,
                   (unsigned long long)total, elapsed, tail_t_, (unsigned long long)run_tail(), (unsigned long long)pm,
-                  (unsigned long long)pmn, (unsigned long long)ge[0], (unsigned long long)ge[1],
-                  (unsigned long long)ge[2], (unsigned long long)run_primes(), (unsigned long long)run_twins());
-    std::string o = b;
-    for (size_t i = 0; i < events_.size(); ++i) {
-        std::snprintf(b, sizeof b, "%s[%llu, \"%llu\"]", i ? ", " : "", (unsigned long long)events_[i].second,
-                      (unsigned long long)events_[i].first);
-        o += b;
-    }
-    o += "]}\n";
-    const std::string tmp = live_path_ + ".tmp";
-    if (std::FILE* f = std::fopen(tmp.c_str(), "wb")) {
-        const bool ok = std::fwrite(o.data(), 1, o.size(), f) == o.size();
-        if (std::fclose(f) == 0 && ok) std::rename(tmp.c_str(), live_path_.c_str());
+                  (unsigned long long)pm_n, (unsigned long long)at_least[0], (unsigned long long)at_least[1],
+                  (unsigned long long)at_least[2], (unsigned long long)run_primes(), (unsigned long long)run_twins());
+    std::string json = head;
+    for (size_t i = 0; i < events_.size(); ++i)
+        json += std::string(i ? ", " : "") + "[" + std::to_string(events_[i].second) + ", \"" +
+                std::to_string(events_[i].first) + "\"]";
+    json += "]}\n";
+    try {   // advisory: replaced atomically but not synced; a failed write leaves the last one
+        textio::replace_file(live_path_, json, ".tmp", false);
+    } catch (const std::exception&) {
     }
 }
 

@@ -8,7 +8,6 @@
 #include <atomic>
 #include <cstdio>
 #include <cstring>
-#include <filesystem>
 #include <thread>
 
 namespace gb {
@@ -141,118 +140,6 @@ uint64_t prev_prime(uint64_t x)
     if (q >= x) q -= 2;
     while (!is_prime_u64(q)) q -= 2;
     return q;
-}
-
-// ---------------------------------------------------------------- prime cache
-// GBP2 file: "GBP2", u64 limit, u64 count, u64 payload hash, then count u32
-// odd primes ascending (the reference's format, so caches are shared).
-
-uint64_t prime_payload_hash(const uint32_t* p, size_t cnt)
-{
-    uint64_t h = 0xcbf29ce484222325ull;
-    for (size_t i = 0; i < cnt; ++i) {
-        h ^= p[i];
-        h *= 0x100000001b3ull;
-    }
-    return h;
-}
-
-namespace {
-
-constexpr char kCacheMagic[4] = {'G', 'B', 'P', '2'};
-constexpr size_t kCacheHeader = 28;
-
-// the odd primes below 64 first, then strictly ascending odd values <= lim
-bool payload_ok(const uint32_t* p, size_t cnt, uint64_t lim)
-{
-    static constexpr uint32_t head[] = {3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61};
-    for (size_t i = 0; i < cnt && i < sizeof head / sizeof head[0]; ++i)
-        if (p[i] != head[i]) return false;
-    for (size_t i = 1; i < cnt; ++i)
-        if (p[i] <= p[i - 1] || !(p[i] & 1u)) return false;
-    return cnt == 0 || p[cnt - 1] <= lim;
-}
-
-// The primes <= n of the cache file, when its limit is n (or larger, with
-// allow_larger) and it passes every check.
-bool read_cache(const std::string& file, uint64_t n, bool allow_larger, std::vector<uint32_t>& out)
-{
-    std::FILE* f = std::fopen(file.c_str(), "rb");
-    if (!f) return false;
-    char magic[4] = {};
-    uint64_t lim = 0, cnt = 0, hash = 0;
-    const char* why = nullptr;
-    if (std::fread(magic, 1, 4, f) != 4 || std::fread(&lim, 8, 1, f) != 1 || std::fread(&cnt, 8, 1, f) != 1 ||
-        std::fread(&hash, 8, 1, f) != 1)
-        why = "short header";
-    else if (std::memcmp(magic, kCacheMagic, 4) != 0)
-        why = "unknown format";
-    else if (!(lim == n || (allow_larger && lim > n))) {
-        std::fclose(f);                  // another limit: not this run's file
-        return false;
-    } else if (cnt == 0 || cnt >= (uint64_t(1) << 32))
-        why = "size does not match the header";
-    if (!why) {
-        std::vector<uint32_t> v(cnt);
-        if (std::fread(v.data(), 4, cnt, f) != cnt || std::fgetc(f) != EOF) why = "size does not match the header";
-        else if (prime_payload_hash(v.data(), cnt) != hash) why = "payload hash mismatch";
-        else if (!payload_ok(v.data(), cnt, lim)) why = "payload content check failed";
-        if (!why) {
-            std::fclose(f);
-            if (lim > n) v.resize(size_t(std::upper_bound(v.begin(), v.end(), uint32_t(std::min<uint64_t>(n, UINT32_MAX))) -
-                                         v.begin()));
-            out.swap(v);
-            return true;
-        }
-    }
-    std::fclose(f);
-    std::fprintf(stderr, "warning: prime cache %s ignored (%s); regenerating\n", file.c_str(), why);
-    return false;
-}
-
-void write_cache(const std::string& path, uint64_t n, const std::vector<uint32_t>& primes)
-{
-    const std::string tmp = path + ".tmp." + std::to_string(::getpid());
-    std::FILE* f = std::fopen(tmp.c_str(), "wb");
-    if (!f) return;
-    const uint64_t cnt = primes.size(), h = prime_payload_hash(primes.data(), primes.size());
-    bool ok = std::fwrite(kCacheMagic, 1, 4, f) == 4 && std::fwrite(&n, 8, 1, f) == 1 &&
-              std::fwrite(&cnt, 8, 1, f) == 1 && std::fwrite(&h, 8, 1, f) == 1 &&
-              std::fwrite(primes.data(), 4, primes.size(), f) == primes.size();
-    ok = std::fflush(f) == 0 && ok;
-    ok = ::fsync(::fileno(f)) == 0 && ok;
-    ok = std::fclose(f) == 0 && ok;
-    if (!ok || std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
-}
-
-}  // namespace
-
-std::vector<uint32_t> base_primes_cached(uint32_t n, const char* cache_path, unsigned threads)
-{
-    if (!cache_path || !*cache_path) return base_primes(n, threads);
-    std::vector<uint32_t> v;
-    const std::string keyed = std::string(cache_path) + "." + std::to_string(n);
-    if (read_cache(keyed, n, false, v)) return v;
-    {   // the smallest keyed cache above n: its prefix
-        namespace fs = std::filesystem;
-        const fs::path p(cache_path);
-        const fs::path dir = p.has_parent_path() ? p.parent_path() : fs::path(".");
-        const std::string stem = p.filename().string() + ".";
-        uint64_t best = 0;
-        std::error_code ec;
-        for (const auto& de : fs::directory_iterator(dir, ec)) {
-            const std::string name = de.path().filename().string();
-            if (name.size() <= stem.size() || name.compare(0, stem.size(), stem) != 0) continue;
-            const std::string num = name.substr(stem.size());
-            if (num.find_first_not_of("0123456789") != std::string::npos) continue;
-            const uint64_t L = std::strtoull(num.c_str(), nullptr, 10);
-            if (L > n && (best == 0 || L < best)) best = L;
-        }
-        if (best && read_cache((dir / (stem + std::to_string(best))).string(), n, true, v)) return v;
-    }
-    v = base_primes(n, threads);
-    write_cache(keyed, n, v);
-    return v;
 }
 
 }  // namespace gb

@@ -1,168 +1,188 @@
-// Goldbach ridge tracker and store (contracts in ridge.hpp). The store
-// format, validation rules and ordering follow the reference's
-// include/ridge.hpp:52-236 (an existing campaign's goldbach_ridges.csv keeps
-// loading).
+// Goldbach ridge board and store (contracts and format in ridge.hpp).
 #include "ridge.hpp"
 
 #include <algorithm>
 #include <cstdio>
 #include <ctime>
 #include <mutex>
+#include <string_view>
 #include <unordered_map>
 
 #include "env.hpp"
 #include "primes.hpp"
+#include "textio.hpp"
 
 namespace gb {
 namespace {
 
-// larger ridge first, equal ridge: larger even first
-bool before(const RidgeEntry& a, const RidgeEntry& b) { return a.p != b.p ? a.p > b.p : a.n > b.n; }
+// ordering of the board: the larger ridge first; equal ridges by the
+// larger even first
+bool ranks_above(const RidgeEntry& a, const RidgeEntry& b) { return a.p > b.p || (a.p == b.p && a.n > b.n); }
 
-// The exact checks' verdicts on (n, p) pairs already checked in this process
-// (a store reloaded run after run re-checks the same rows: each costs up to
-// a few hundred Miller-Rabin tests). The verdict depends only on n, p and the
-// candidate list, which is the same fixed prime list for every tracker.
-enum Verdict : uint8_t { kOk, kBadPair, kNotMinimal };
-struct PairHash {
-    size_t operator()(const std::pair<uint64_t, uint64_t>& k) const
-    {
-        return size_t(k.first * 0x9E3779B97F4A7C15ull ^ (k.second + (k.first >> 29)));
-    }
+// Outcome of the exact check of a pair.
+struct Check {
+    enum Kind : uint8_t { Ok, NotPrime, Smaller } kind = Ok;
+    uint32_t smaller = 0;   // a smaller candidate with n - p' prime (Smaller)
 };
-std::mutex g_memo_mu;
-std::unordered_map<std::pair<uint64_t, uint64_t>, uint32_t, PairHash> g_memo;   // verdict | witness << 8
-constexpr size_t kMemoMax = 1 << 16;
+
+// The exact check: n even, p and n - p prime, and p the smallest candidate
+// with n - p prime. A store reloaded run after run checks the same rows
+// again (each up to a few hundred Miller-Rabin tests), so the outcomes are
+// remembered for the process: they depend only on (n, p) and the candidate
+// list, the same fixed list for every board.
+class CheckMemo {
+public:
+    Check get(uint64_t n, uint64_t p, const std::vector<uint32_t>* cand)
+    {
+        const Key k{n, p};
+        if (cand) {
+            std::lock_guard<std::mutex> g(mu_);
+            if (auto it = seen_.find(k); it != seen_.end()) return it->second;
+        }
+        Check c;
+        if (!is_prime_u64(p) || !is_prime_u64(n - p)) {
+            c.kind = Check::NotPrime;
+        } else if (cand) {
+            const auto end = std::lower_bound(cand->begin(), cand->end(), p);
+            const auto hit = std::find_if(cand->begin(), end, [n](uint32_t q) { return is_prime_u64(n - q); });
+            if (hit != end) c = Check{Check::Smaller, *hit};
+        }
+        if (cand) {
+            std::lock_guard<std::mutex> g(mu_);
+            if (seen_.size() >= kCap) seen_.clear();
+            seen_.emplace(k, c);
+        }
+        return c;
+    }
+
+private:
+    struct Key {
+        uint64_t n, p;
+        bool operator==(const Key& o) const { return n == o.n && p == o.p; }
+    };
+    struct KeyHash {
+        size_t operator()(const Key& k) const { return size_t((k.n ^ (k.p << 40) ^ (k.p >> 24)) * 0xD6E8FEB86659FD93ull); }
+    };
+    static constexpr size_t kCap = 1 << 16;
+    std::mutex mu_;
+    std::unordered_map<Key, Check, KeyHash> seen_;
+};
+
+CheckMemo& memo()
+{
+    static CheckMemo m;
+    return m;
+}
 
 }  // namespace
 
 RidgeTracker::RidgeTracker(const std::vector<uint32_t>* cand) : cand_(cand)
 {
     if (!env::onoff("GB_RIDGE", true)) return;
-    floor_ = env::u32("GB_RIDGE_MIN", RIDGE_MIN_DEFAULT);
-    if (floor_ < 3) floor_ = 3;
-    debug_ = env::present("GB_RIDGE_DEBUG");
+    floor_ = std::max<uint32_t>(env::u32("GB_RIDGE_MIN", RIDGE_MIN_DEFAULT), 3);
+    trace_ = env::present("GB_RIDGE_DEBUG");
 }
 
 void RidgeTracker::clamp_floor_above(uint32_t bound)
 {
-    if (floor_ != 0 && floor_ <= bound) {
-        floor_ = bound + 1;
-        std::fprintf(stderr, "note: ridge floor raised to %u (phase-A candidates reach %u)\n", floor_, bound);
-    }
+    if (!active() || floor_ > bound) return;
+    floor_ = bound + 1;
+    std::fprintf(stderr, "note: ridge floor raised to %u (phase-A candidates reach %u)\n", floor_, bound);
 }
 
 uint32_t RidgeTracker::thresh() const
 {
     if (!active()) return 0xFFFFFFFFu;
-    return top_.size() >= RIDGE_KEEP ? std::max<uint32_t>(floor_, uint32_t(top_.back().p)) : floor_;
+    if (board_.size() < RIDGE_KEEP) return floor_;
+    return std::max<uint32_t>(floor_, uint32_t(board_.back().p));
+}
+
+bool RidgeTracker::admit(const RidgeEntry& e)
+{
+    // a full board takes only a ridge that ranks above its last one
+    if (board_.size() >= RIDGE_KEEP && !ranks_above(e, board_.back())) return false;
+    if (std::any_of(board_.begin(), board_.end(), [&](const RidgeEntry& b) { return b.n == e.n; }))
+        return false;   // already on the board: its found time stays
+    const Check c = memo().get(e.n, e.p, cand_);
+    if (c.kind != Check::Ok) {
+        ++dropped_;
+        if (trace_)
+            std::fprintf(stderr, "ridge drop (%s): n=%llu p=%llu%s\n", c.kind == Check::NotPrime ? "pair" : "not minimal",
+                         (unsigned long long)e.n, (unsigned long long)e.p,
+                         c.kind == Check::Smaller ? (" > " + std::to_string(c.smaller)).c_str() : "");
+        return false;
+    }
+    board_.insert(std::upper_bound(board_.begin(), board_.end(), e, ranks_above), e);
+    if (board_.size() > RIDGE_KEEP) board_.resize(RIDGE_KEEP);
+    dirty_ = true;
+    return true;
 }
 
 void RidgeTracker::consider(uint64_t n, uint64_t p, uint64_t seg, uint64_t found)
 {
     if (!active() || p < floor_) return;
     ++appends_;
-    if (n < 4 || (n & 1) || p < 3 || !(p & 1) || p >= n) {
+    const bool shape_ok = n >= 4 && n % 2 == 0 && p >= 3 && p % 2 == 1 && p < n;
+    if (!shape_ok) {
         ++dropped_;
         return;
     }
-    if (top_.size() >= RIDGE_KEEP) {          // cheap qualification before the exact checks
-        const RidgeEntry& last = top_.back();
-        if (p < last.p || (p == last.p && n <= last.n)) return;
-    }
-    for (const RidgeEntry& t : top_)
-        if (t.n == n) return;                 // already stored: keep its found time
-    const std::pair<uint64_t, uint64_t> key{n, p};
-    uint32_t v = UINT32_MAX;
-    if (cand_) {
-        std::lock_guard<std::mutex> lk(g_memo_mu);
-        const auto it = g_memo.find(key);
-        if (it != g_memo.end()) v = it->second;
-    }
-    if (v == UINT32_MAX) {
-        v = kOk;
-        if (!is_prime_u64(p) || !is_prime_u64(n - p)) {
-            v = kBadPair;
-        } else if (cand_) {
-            for (uint32_t c : *cand_) {       // minimality: no smaller candidate works
-                if (c >= p) break;
-                if (is_prime_u64(n - c)) {
-                    v = kNotMinimal | c << 8;
-                    break;
-                }
-            }
-        }
-        if (cand_) {
-            std::lock_guard<std::mutex> lk(g_memo_mu);
-            if (g_memo.size() >= kMemoMax) g_memo.clear();
-            g_memo.emplace(key, v);
-        }
-    }
-    if ((v & 0xFF) == kBadPair) {
-        ++dropped_;
-        if (debug_) std::fprintf(stderr, "ridge drop (pair): n=%llu p=%llu\n", (unsigned long long)n,
-                                 (unsigned long long)p);
-        return;
-    }
-    if ((v & 0xFF) == kNotMinimal) {
-        ++dropped_;
-        if (debug_) std::fprintf(stderr, "ridge drop (not minimal): n=%llu p=%llu < %u\n", (unsigned long long)n,
-                                 (unsigned long long)p, v >> 8);
-        return;
-    }
-    if (found == UINT64_MAX) found = uint64_t(std::time(nullptr));
-    const RidgeEntry e{n, p, n - p, seg, found};
-    top_.insert(std::upper_bound(top_.begin(), top_.end(), e, before), e);
-    if (top_.size() > RIDGE_KEEP) top_.pop_back();
-    changed_ = true;
+    (void)admit(RidgeEntry{n, p, n - p, seg, found == UINT64_MAX ? uint64_t(std::time(nullptr)) : found});
 }
 
-void RidgeTracker::merge(const uint64_t* pairs, uint32_t cnt, uint32_t cap, uint64_t seg)
+void RidgeTracker::merge(const uint64_t* pairs, size_t stored, uint64_t claimed, uint64_t seg)
 {
     if (!active()) return;
-    if (cnt > cap) overflow_ = true;
-    const uint32_t n = std::min(cnt, cap);
-    for (uint32_t i = 0; i < n; ++i) consider(pairs[2 * size_t(i)], pairs[2 * size_t(i) + 1], seg);
+    overflowed_ = overflowed_ || claimed > stored;
+    const size_t n = size_t(std::min<uint64_t>(claimed, stored));
+    for (size_t i = 0; i < n; ++i) consider(pairs[2 * i], pairs[2 * i + 1], seg);
 }
 
 bool RidgeTracker::load(const std::string& path)
 {
-    std::FILE* f = std::fopen(path.c_str(), "rb");
-    if (!f) return false;
-    char line[256];
-    while (std::fgets(line, sizeof line, f)) {
-        if (line[0] == '#' || line[0] == '\n') continue;
-        unsigned long long r, n, p, q, s, t = 0;   // t is absent in v1 rows
-        if (std::sscanf(line, "%llu,%llu,%llu,%llu,%llu,%llu", &r, &n, &p, &q, &s, &t) >= 5) consider(n, p, s, t);
+    const std::optional<std::string> text = textio::read_file(path);
+    if (!text) return false;
+    std::string_view rest(*text);
+    while (!rest.empty()) {
+        const size_t nl = rest.find('\n');
+        std::string_view line = rest.substr(0, nl);
+        rest = nl == std::string_view::npos ? std::string_view() : rest.substr(nl + 1);
+        if (!line.empty() && line.back() == '\r') line.remove_suffix(1);
+        if (line.empty() || line.front() == '#') continue;
+        // rank,n,p,q,segment_start[,found_unix]: the rank and q are
+        // recomputed, the rest is checked again before it counts
+        uint64_t v[6] = {};
+        size_t cols = 0;
+        bool ok = true;
+        while (ok && cols < 6) {
+            const size_t comma = line.find(',');
+            ok = textio::parse(line.substr(0, comma), v[cols]);
+            cols += ok;
+            if (comma == std::string_view::npos) break;
+            line.remove_prefix(comma + 1);
+        }
+        if (cols >= 5) consider(v[1], v[2], v[4], cols == 6 ? v[5] : 0);
     }
-    std::fclose(f);
-    changed_ = false;
+    dirty_ = false;
     return true;
 }
 
 void RidgeTracker::save(const std::string& path)
 {
-    if (!changed_) return;
+    if (!dirty_) return;
     std::string out = "# goldbach-l0 ridge store v2 - top 30 largest Goldbach ridges\n"
                       "# rank,even,small_prime_p,large_prime_q,segment_start,found_unix\n";
-    char row[192];
-    for (size_t i = 0; i < top_.size(); ++i) {
-        const RidgeEntry& e = top_[i];
-        std::snprintf(row, sizeof row, "%zu,%llu,%llu,%llu,%llu,%llu\n", i + 1, (unsigned long long)e.n,
-                      (unsigned long long)e.p, (unsigned long long)e.q, (unsigned long long)e.seg,
-                      (unsigned long long)e.found);
-        out += row;
+    for (size_t i = 0; i < board_.size(); ++i) {
+        const RidgeEntry& e = board_[i];
+        out += std::to_string(i + 1) + "," + std::to_string(e.n) + "," + std::to_string(e.p) + "," +
+               std::to_string(e.q) + "," + std::to_string(e.seg) + "," + std::to_string(e.found) + "\n";
     }
-    const std::string tmp = path + ".tmp";
-    std::FILE* f = std::fopen(tmp.c_str(), "wb");
-    if (!f) {
-        std::fprintf(stderr, "ridge store: cannot write %s\n", tmp.c_str());
-        return;
+    try {
+        textio::replace_file(path, out);
+        dirty_ = false;
+    } catch (const std::exception& ex) {
+        std::fprintf(stderr, "ridge store: %s\n", ex.what());
     }
-    const bool ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
-    if (std::fclose(f) == 0 && ok) std::rename(tmp.c_str(), path.c_str());
-    else std::remove(tmp.c_str());
-    changed_ = false;
 }
 
 }  // namespace gb

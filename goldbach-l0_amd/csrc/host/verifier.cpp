@@ -234,6 +234,7 @@ struct Engine::State {
     // sieve and harvest run while the batch before it is checked and counted
     struct Slot {
         Buf map, i0, coff, coffa, pending, scratch, hits, hit_n, chunks, rdesc, groups, region, fills;
+        Buf ridge_more;                     // ridge records beyond the output block's GBH_RIDGE_CAP
         Buf tab;                            // the batch tables (tab_layout)
         Buf vmap, vhits, vhit_n, vchunks;   // the GB_BUCKET_VERIFY rebuild
     } sl[2];
@@ -498,7 +499,7 @@ private:
     void enqueue(std::vector<SegJob>& batch, int slot, bool check_and_count);
     void enqueue_verify(uint32_t s, const gbh_batch& b, gbh_segout* out, int slot);
     void collect(std::vector<SegJob>& batch, int slot);
-    void collect_segment(const SegJob& j, const gbh_segout& o, const double share[4]);
+    void collect_segment(const SegJob& j, const gbh_segout& o, const double share[4], int slot, uint32_t s);
     void finish_range(Range& r);
 
     Engine::State& E;
@@ -748,6 +749,7 @@ void Pipeline::plan(const PlanBounds& pb)
         B.i0.ensure(c, K * 8 * size_t(std::max(nwin_ + nx_, 1u)) * 4, 0);
         B.pending.ensure(c, K * size_t(pcap_) * 4, 0);
         B.scratch.ensure(c, K * size_t(swords_) * 4, 0);
+        B.ridge_more.ensure(c, K * size_t(GBH_RIDGE_MORE) * 16, 0);
         B.hit_n.ensure(c, K * 8 * size_t(nwin_max_) * 4, 0);
         if (nlarge_) B.hits.ensure(c, K * 8 * size_t(nwin_max_) * cap_ * 4, 0);
         B.tab.ensure(c, tab_layout(K).bytes, 0);
@@ -1300,7 +1302,8 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         } else {
             check(gbh_launch_check_a(c, ms, &ca), "check phase A");
         }
-        gbh_checkb_args cb{b, E.ka, CHECK_TAIL, B.coff.as<uint32_t>(), B.pending.as<uint32_t>(), pcap_, kend_, out};
+        gbh_checkb_args cb{b, E.ka, CHECK_TAIL, B.coff.as<uint32_t>(), B.pending.as<uint32_t>(), pcap_, kend_, out,
+                           B.ridge_more.as<uint64_t>()};
         check(gbh_launch_check_b(c, ms, &cb), "check phase B");
     }
     check(gbh_event_record(c, ms, ev[EV_CHECK].e), "event");
@@ -1395,7 +1398,7 @@ void Pipeline::collect(std::vector<SegJob>& batch, int slot)
     if (outs[0].err & 8u) throw std::runtime_error("hit lists: a multiplier outside the multiplier bitmap");
     if (outs[0].err & 16u) throw std::runtime_error("hit lists: a quotient estimate more than one off");
     const double share[4] = {ts / batch.size(), tl / batch.size(), tc / batch.size(), tn / batch.size()};
-    for (size_t s = 0; s < batch.size(); ++s) collect_segment(batch[s], outs[s], share);
+    for (size_t s = 0; s < batch.size(); ++s) collect_segment(batch[s], outs[s], share, slot, uint32_t(s));
     if (phase_trace_) std::fprintf(stderr, "host: collected in %.3f ms\n", since(tcol) * 1e3);
     if (phase_trace_)
         std::fprintf(stderr, "batch of %zu segments from %llu: harvest+setup %.3f ms window %.3f ms check %.3f ms "
@@ -1403,7 +1406,7 @@ void Pipeline::collect(std::vector<SegJob>& batch, int slot)
                      tc * 1e3, tn * 1e3);
 }
 
-void Pipeline::collect_segment(const SegJob& j, const gbh_segout& o, const double share[4])
+void Pipeline::collect_segment(const SegJob& j, const gbh_segout& o, const double share[4], int slot, uint32_t s)
 {
     Range& r = *j.r;
     r.t_sieve += share[0];
@@ -1444,7 +1447,23 @@ void Pipeline::collect_segment(const SegJob& j, const gbh_segout& o, const doubl
         if (p > 2 && r.ridges) r.ridges->consider(n, p, j.sg.seg_start);
     }
     r.rs.leftovers += o.left_n;
-    if (r.ridges) r.ridges->merge(o.ridge, o.ridge_n, GBH_RIDGE_CAP, j.sg.seg_start);
+    if (r.ridges) {
+        // the first GBH_RIDGE_CAP records came with the output block; more
+        // (a low GB_RIDGE_MIN) wait in the slot's overflow buffer
+        const uint32_t kept = std::min<uint32_t>(o.ridge_n, GBH_RIDGE_CAP + GBH_RIDGE_MORE);
+        if (kept > GBH_RIDGE_CAP) {
+            std::vector<uint64_t> all(o.ridge, o.ridge + 2 * GBH_RIDGE_CAP);
+            all.resize(2 * size_t(kept));
+            const uint64_t* src = E.sl[slot].ridge_more.as<uint64_t>() + 2 * size_t(s) * GBH_RIDGE_MORE;
+            const int ms = main_stream(slot);
+            check(gbh_copy_d2h(E.c, ms, all.data() + 2 * GBH_RIDGE_CAP, src, size_t(kept - GBH_RIDGE_CAP) * 16),
+                  "copy ridge records");
+            check(gbh_stream_sync(E.c, ms), "ridge records sync");
+            r.ridges->merge(all.data(), kept, o.ridge_n, j.sg.seg_start);
+        } else {
+            r.ridges->merge(o.ridge, kept, o.ridge_n, j.sg.seg_start);
+        }
+    }
     if (r.gs) {
         r.gs->add_hist_slots(j.sampled, o.hist, GBH_NCAND, E.cand.slot_pi);
         r.gs->add_first(j.sg.seg_start, o.first);
@@ -1526,7 +1545,7 @@ void Pipeline::finish_range(Range& r)
         rs.ridge_top = r.ridges->top();
         if (r.ridges->overflowed())
             std::fprintf(stderr, "warning: a segment recorded more than %u ridges at or above the threshold; the "
-                         "excess was not considered\n", GBH_RIDGE_CAP);
+                         "excess was not considered\n", GBH_RIDGE_CAP + GBH_RIDGE_MORE);
     }
     if (r.gs) {
         r.gs->finish_run(r.start, r.limit, r.total_evens);

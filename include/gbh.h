@@ -58,7 +58,9 @@ enum {
 #define GBH_LEFT_CAP 65536       /* leftovers per segment copied back */
 #define GBH_CHECK_WG 256         /* check_a work-group: 256 lanes x 64 evens */
 #define GBH_KA_MAX 128           /* phase-A candidates per even class, at most */
-#define GBH_RIDGE_CAP 1024       /* ridge records per segment (reference RIDGE_CAP role) */
+#define GBH_RIDGE_CAP 1024       /* ridge records per segment kept in the output block */
+#define GBH_RIDGE_MORE 64512     /* further ridge records per segment in gbh_checkb_args.ridge_more: 65536 in
+                                    all, the reference's RIDGE_CAP (config.hpp:103) */
 #define GBH_MAX_BATCH 64         /* segments per batch, at most */
 
 typedef struct gbh_ctx gbh_ctx;
@@ -304,6 +306,8 @@ typedef struct {
     uint32_t kend;                 /* probe candidate slots [ka, kend) only (0: every slot); an even whose p(n)
                                       lies beyond is a leftover for the CPU (a test of the fallback) */
     gbh_segout* out;
+    uint64_t* ridge_more;          /* [nseg][GBH_RIDGE_MORE][2] (n, p(n)) of the ridge records beyond the first
+                                      GBH_RIDGE_CAP (nullptr: not kept; ridge_n still counts them) */
 } gbh_checkb_args;
 
 /* Stage 6: prime / twin counts and the gap walk of the counting range

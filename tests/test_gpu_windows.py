@@ -165,3 +165,25 @@ def test_region_overflow_rerun_counted(stats_env, monkeypatch):
     assert r["harvest_launches"] >= 1
     bad = compare.compare_ref(r["stats_line"], (stats_env / "st.state").read_text(), ref)
     assert not bad, bad
+
+
+def test_ridge_records_beyond_the_output_block(stats_env):
+    """A low ridge floor: a segment records far more than the 1024 ridges its
+    output block holds; the rest come from the overflow buffer (65536 per
+    segment in all, the reference's RIDGE_CAP), every record is considered
+    (ridge_found above 1024 per segment), and the store's top 30 equals the
+    oracle's 30 largest p(n) of the window."""
+    a, b = E18, E18 + 2_000_000
+    _say(f"ridge overflow [{a}, {b}] floor 700")
+    c = _cli("--start", str(a), "--limit", str(b), "--segment", "1000000", "--csv",
+             env={"GB_KA": "32", "GB_RIDGE_MIN": "700"})
+    assert c.returncode == 0 and "result=PASS" in c.stdout, c.stderr
+    row = compare.kv(c.stdout.strip().splitlines()[-1])
+    assert int(row["segments"]) == 2 and int(row["ridge_min"]) == 700
+    assert int(row["ridge_found"]) > 2 * 1024, row["ridge_found"]
+    assert "more than" not in c.stderr          # no segment beyond the 65536 records
+    rows = (stats_env / "ridges.csv").read_text().splitlines()
+    got = [tuple(int(x) for x in r.split(",")[1:3]) for r in rows if not r.startswith("#")]
+    out = subprocess.run([str(compare.ORACLE), "ridges", str(a), str(b), "700", "30", "8"], capture_output=True,
+                         text=True, check=True).stdout
+    assert got == [tuple(int(x) for x in line.split()) for line in out.splitlines()]
