@@ -6,45 +6,35 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <optional>
 #include <string>
+#include <string_view>
 
 #include "config.hpp"
 #include "env.hpp"
+#include "textio.hpp"
 #include "verifier.hpp"
 
 namespace {
 
-/// Full decimal or e-notation ("4e18"), whole string only, exact integer
-/// math with overflow checks (main.cpp:28-57 of the reference).
-uint64_t parse_count(const char* s, bool& ok)
+/// A count on the command line: full decimal or e-notation ("4e18"), the
+/// whole string, exact; nullopt for anything else or a value above 2^64 - 1
+/// (the reference's number contract, src/main.cpp:28-57 of the reference:
+/// a typo is a usage error, never a silently truncated limit).
+std::optional<uint64_t> parse_count(std::string_view s)
 {
-    ok = false;
-    if (!s || !*s) return 0;
-    uint64_t m = 0, e10 = 0;
-    int digits = 0;
-    const char* p = s;
-    for (; *p >= '0' && *p <= '9'; ++p) {
-        const uint64_t d = uint64_t(*p - '0');
-        if (m > (~0ull - d) / 10) return 0;
-        m = m * 10 + d;
-        ++digits;
+    const size_t e = s.find_first_of("eE");
+    const std::string_view mantissa = s.substr(0, e);
+    gb::textio::u128 v = 0;
+    if (!gb::textio::parse(mantissa, v)) return std::nullopt;
+    if (e != std::string_view::npos) {
+        uint32_t exponent = 0;
+        if (!gb::textio::parse(s.substr(e + 1), exponent) || exponent > 20) return std::nullopt;
+        // stop once above 2^64 - 1: each product then stays far below 2^128
+        while (exponent-- && v <= UINT64_MAX) v *= 10;
     }
-    if (!digits) return 0;
-    if (*p == 'e' || *p == 'E') {
-        ++p;
-        if (*p < '0' || *p > '9') return 0;
-        for (; *p >= '0' && *p <= '9'; ++p) {
-            if (e10 > 19) return 0;
-            e10 = e10 * 10 + uint64_t(*p - '0');
-        }
-    }
-    if (*p) return 0;
-    for (uint64_t i = 0; i < e10; ++i) {
-        if (m > ~0ull / 10) return 0;
-        m *= 10;
-    }
-    ok = true;
-    return m;
+    if (v > UINT64_MAX) return std::nullopt;
+    return uint64_t(v);
 }
 
 void usage()
@@ -64,7 +54,7 @@ void usage()
         "  --segment N    even integer span per GPU segment  (default %llu)\n"
         "  --split N      base-prime split index (0 = auto)  (default 0)\n"
         "  --spv FILE     accepted for compatibility (the device code is linked in)\n"
-        "  --prime-cache FILE  accepted for compatibility (base primes are generated)\n"
+        "  --prime-cache FILE  GBP2 base-prime cache stem: FILE.<bound> (default base_primes.cache)\n"
         "  --cpu-check N  run the independent CPU check up to N and exit\n"
         "  --json         machine-readable JSON report on stdout\n"
         "  --csv          one summary line: key=value pairs (grep/awk friendly)\n"
@@ -184,12 +174,12 @@ int main(int argc, char** argv)
         const std::string a = argv[i];
         const auto next = [&](uint64_t& out) {
             if (i + 1 >= argc) { usage(); std::exit(2); }
-            bool ok = false;
-            out = parse_count(argv[++i], ok);
-            if (!ok) {
+            const std::optional<uint64_t> v = parse_count(argv[++i]);
+            if (!v) {
                 std::fprintf(stderr, "error: invalid number '%s'\n", argv[i]);
                 std::exit(2);
             }
+            out = *v;
         };
         const auto next_str = [&]() -> const char* {
             if (i + 1 >= argc) { usage(); std::exit(2); }

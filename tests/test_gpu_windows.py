@@ -187,3 +187,25 @@ def test_ridge_records_beyond_the_output_block(stats_env):
     out = subprocess.run([str(compare.ORACLE), "ridges", str(a), str(b), "700", "30", "8"], capture_output=True,
                          text=True, check=True).stdout
     assert got == [tuple(int(x) for x in line.split()) for line in out.splitlines()]
+
+
+# the reference's --csv row: 35 keys in this order, then sysman= (and
+# optional telemetry keys), src/main.cpp:192-226 of the reference
+CSV_KEYS = ["version", "limit", "start", "segment", "split", "segments", "leftovers", "result", "init_s",
+            "init_cache_s", "init_module_s", "init_upload_s", "gpu_s", "total_s", "sieve_s", "large_s", "check_s",
+            "offsets_s", "table_s", "sync_s", "evens_per_sec", "tiles", "large_items", "ridge_min", "ridge_found",
+            "ridge_dropped", "ridge_max", "stats", "stats_tail", "stats_sampled", "stats_pmax", "primes", "twins",
+            "split_used", "sched"]
+
+
+def test_csv_row_keys_in_reference_order(stats_env):
+    _say("csv row [4e18, 4e18 + 2e6]")
+    c = _cli("--start", str(E18), "--limit", str(E18 + 2_000_000), "--segment", "1000000", "--csv")
+    assert c.returncode == 0, c.stderr
+    row = c.stdout.strip().splitlines()[-1]
+    keys = [w.split("=", 1)[0] for w in row.split()]
+    assert keys[:len(CSV_KEYS)] == CSV_KEYS
+    assert keys[len(CSV_KEYS)] == "sysman" and row.split()[len(CSV_KEYS)] == "sysman=off"
+    v = compare.kv(row)
+    assert v["result"] == "PASS" and v["sched"] == "driver" and int(v["segments"]) == 2
+    assert int(v["primes"]) == 46640 and int(v["stats_pmax"]) == 2221   # the window's stats_ref figures
