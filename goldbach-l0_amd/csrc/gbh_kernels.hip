@@ -504,6 +504,178 @@ __device__ __forceinline__ void bm_single(const HlCtx& h, double qb0d, const uin
     }
 }
 
+// Bitmap primes with few multipliers in the run's q span (direct form): a
+// lean exact scan of every prime finds its multiples q = p m in the span
+// with q (so m) a unit mod 30; these candidates are compacted into a
+// wave-private LDS queue and resolved together (multiplier, bitmap word and
+// bit, class, filing) with every lane busy. Most primes have no candidate
+// (about one in eight has one), so the bitmap walk's address work is spent
+// on those only. Each wave runs its own rounds: it resolves the previous
+// round's queue while the loads of that round's bitmap words and of the next
+// round's primes are in flight; no work-group barrier is involved.
+//
+// The quotient: m_est = round(qb0 / p - 1) from one fused multiply-add
+// into the low mantissa bits of 2^52 + m_est (2^52 - 1 is exact, and the sum
+// is rounded once, to an integer). With 1 / p from the float reciprocal and
+// one Newton step in double (relative error below 2^-44) the computed
+// qb0 / p is within Q 2^-44 + 2^11 / p of Q = qb0 / p; for Q < 2^41 and
+// p >= 2^20 that is below 0.13, so m_est is floor(Q) or one less and
+// r = qb0 - m_est p lies in [0, 2p): one conditional step makes it exact (a
+// remainder still out of range sets err bit 4, the run fails). Below
+// p = 2^31 the remainder fits 32 bits (kWide: above, 64-bit arithmetic).
+constexpr uint32_t kFewPminLog2 = 20;  // primes of the compacted path are >= 2^20
+constexpr uint32_t kFewQLog2 = 41;     // and their quotients qb1 / p below 2^41
+constexpr uint32_t kFewWave = 256;     // queue slots per wave and round
+
+struct FewCand {
+    uint32_t p, d;                     // the prime; half-offset (q - qb0) / 2 of q = p m
+    uint32_t m_lo, m_hi_adj;           // m_est: low word, high bits | (m - m_est) << 20
+};
+
+// A work-group's queues: per wave two rounds' worth (one being filled, the
+// previous one being resolved).
+struct FewLds {
+    FewCand q[4][2][kFewWave];
+};
+
+// The per-run constants of the scan.
+struct FewRun {
+    double qb0d;
+    uint32_t qb0_lo, qb0_30;           // low word and residue mod 30 of qb0
+    uint64_t span;                     // qb1 - qb0
+    uint32_t span32;                   // min(span, 2^32 - 1)
+};
+
+// Scans one prime (pu = 0: an idle lane): queues the multiples in the span
+// whose q is a unit mod 30, at queue[*n ...]; *n is wave-uniform.
+template <uint32_t kM, bool kWide>
+__device__ __forceinline__ void few_scan(const HlCtx& h, const FewRun& f, uint32_t pu, FewCand* queue, uint32_t& n,
+                                         bool& bad)
+{
+    const uint32_t pc = pu | (pu == 0);
+    const double pd = (double)pc;
+    double inv = (double)__builtin_amdgcn_rcpf((float)pc);
+    inv = __builtin_fma(inv, __builtin_fma(-pd, inv, 1.0), inv);
+    const uint64_t tb = (uint64_t)__double_as_longlong(__builtin_fma(f.qb0d, inv, 0x1p52 - 1.0));
+    uint32_t r;                        // qb0 mod p
+    bool step;                         // m_est was floor - 1
+    if constexpr (kWide) {
+        const uint64_t m_est = tb & ((1ull << 52) - 1);
+        uint64_t r64 = h.qb0 - m_est * pc;
+        step = r64 >= pc;
+        r64 -= step ? pc : 0u;
+        bad = bad || (pu && r64 >= pc);   // an idle lane's quotient is meaningless
+        r = (uint32_t)r64;
+    } else {
+        r = f.qb0_lo - (uint32_t)tb * pc;   // exact mod 2^32: the true value is below 2p < 2^32
+        step = r >= pc;
+        r -= step ? pc : 0u;
+        bad = bad || (pu && r >= pc);
+    }
+    const uint32_t r0 = r ? pc - r : 0u;   // the first multiple at or above qb0 is qb0 + r0
+    const uint32_t adj = (uint32_t)step + (r != 0);
+    uint32_t q30 = r0 % 30u + f.qb0_30;     // q mod 30 of that multiple
+    q30 -= q30 >= 30 ? 30u : 0u;
+    const uint32_t p30 = kM > 1 ? pc % 30u : 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < kM; ++k) {
+        const uint64_t off = (uint64_t)r0 + (uint64_t)k * pc;
+        const bool in_span = kM == 1 ? r0 <= f.span32 : off <= f.span;
+        const bool cand = pu && in_span && ((kUnitMask >> q30) & 1u);
+        const uint64_t bal = __ballot(cand);
+        if (bal) {                             // wave-uniform
+            if (cand) {
+                const uint32_t at = n + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                queue[at] = FewCand{pc, (uint32_t)(off >> 1), (uint32_t)tb,
+                                    ((uint32_t)(tb >> 32) & 0xFFFFFu) | ((adj + k) << 20)};
+            }
+            n += (uint32_t)__popcll(bal);
+        }
+        if (kM > 1) {
+            q30 += p30;
+            q30 -= q30 >= 30 ? 30u : 0u;
+        }
+    }
+}
+
+// kM: multipliers per prime at most (1: primes above the span). A round
+// scans kR primes per lane: at most 64 kR kM' candidates per wave (kM' =
+// 1 unit among 1 consecutive multiplier, 2 among 4), the wave's queue.
+template <uint32_t kM, bool kWide>
+__device__ __forceinline__ void few_chunk(const HlCtx& h, const FewRun& f, const uint32_t* __restrict__ primes,
+                                          uint32_t j0, uint32_t je, uint32_t tid, const uint32_t* df,
+                                          const uint32_t* dl, const uint32_t* dq, const uint32_t* cmt, uint32_t* cnt,
+                                          FewLds& L, uint32_t* __restrict__ err)
+{
+    constexpr uint32_t kR = kM == 1 ? 4 : 2;
+    constexpr uint32_t kD = kFewWave / 64;    // queued candidates a lane resolves per round, at most
+    static_assert(64 * kR * (kM == 1 ? 1 : 2) <= kFewWave, "a round must fit the wave's queue");
+    const uint32_t lane = tid & 63, wv = tid >> 6;
+    const uint32_t rounds = (je - j0 + 256 * kR - 1) / (256 * kR);
+    bool bad = false, outside = false;
+    uint32_t p[kR];
+#pragma unroll
+    for (uint32_t u = 0; u < kR; ++u) {
+        const uint32_t j = j0 + u * 256 + tid;
+        p[u] = j < je ? primes[j] : 0u;
+    }
+    uint32_t prev_n = 0;
+    // the last pass only resolves the last round's queue
+    for (uint32_t r = 0; r <= rounds; ++r) {
+        const FewCand* pq = L.q[wv][(r + 1) & 1];
+        FewCand c[kD];
+        uint64_t w[kD];
+        bool live[kD];
+#pragma unroll
+        for (uint32_t x = 0; x < kD; ++x) {
+            const uint32_t e = lane + 64 * x;
+            live[x] = e < prev_n;
+            c[x] = pq[e < prev_n ? e : 0];
+            const uint64_t m = ((((uint64_t)c[x].m_hi_adj & 0xFFFFFu) << 32) | c[x].m_lo) + (c[x].m_hi_adj >> 20);
+            // multipliers below p belong to smaller primes
+            live[x] = live[x] && m >= c[x].p;
+            const bool inb = m >= h.m_base && m - h.m_base < h.m_span;
+            outside = outside || (live[x] && !inb);
+            live[x] = live[x] && inb;
+            const uint32_t r30 = ((uint32_t)(m >> 32) * 16u + (uint32_t)m % 30u) % 30u;   // 2^32 = 16 mod 30
+            const uint64_t y = ((m - r30 - h.m_base) >> 1) * 0xEEEEEEEEEEEEEEEFull;   // byte of m: exact / 30
+            // one load per slot whether or not it is live (word 0 then): the
+            // loads stay unconditional, so waiting for them does not wait for
+            // later ones
+            w[x] = __builtin_nontemporal_load(h.mbits + (live[x] ? (y >> 3) : 0ull));
+            c[x].m_lo = 8 * ((uint32_t)y & 7u) + units_below(r30);   // the bit of m in its word
+            c[x].m_hi_adj = r30;
+        }
+        uint32_t n = 0;
+        if (r < rounds) {
+            FewCand* q = L.q[wv][r & 1];
+#pragma unroll
+            for (uint32_t u = 0; u < kR; ++u) few_scan<kM, kWide>(h, f, p[u], q, n, bad);
+#pragma unroll
+            for (uint32_t u = 0; u < kR; ++u) {
+                const uint32_t j = j0 + (r + 1) * 256 * kR + u * 256 + tid;
+                p[u] = j < je ? primes[j] : 0u;
+            }
+        }
+#pragma unroll
+        for (uint32_t x = 0; x < kD; ++x) {
+            if (live[x] && ((w[x] >> c[x].m_lo) & 1ull)) {
+                const uint32_t i = units_below(c[x].m_hi_adj);
+                file_hit<kDirect>(h, df, dl, dq, c[x].d, (cmt[units_below(c[x].p % 30)] >> (3 * i)) & 7, cnt, nullptr,
+                                  0, err);
+            }
+        }
+        // this wave's queue writes and reads of the round stay in order
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        prev_n = n;
+    }
+    const uint32_t ef = (bad ? 16u : 0u) | (outside ? 8u : 0u);
+    if (ef) atomicOr(err, ef);
+}
+
 // One work-group per chunk of primes; two passes over the chunk: count per
 // window list, reserve each list's share with one atomic, then write. With
 // `stage` slots, the counting pass also keeps the entries in LDS; when they
@@ -613,9 +785,24 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
         const bool single = !wheel && (uint64_t)primes[j0] > qb1 - qb0;
         uint32_t* rh = hits + (size_t)seg0 * 8 * b.nwin * cap;   // the run's lists
         uint32_t* rhn = hit_n + (size_t)seg0 * 8 * b.nwin;
-        if (direct) {   // one walk into the work-group's regions, then their fill counts
+        if constexpr (kDirectForm) {   // one walk into the work-group's regions, then their fill counts
+            // bitmap primes with at most 4 multipliers in the span, inside the
+            // compacted path's quotient bounds (uniform over the chunk)
+            const uint64_t pmin = primes[j0];
+            const bool few = !wheel && pmin >= (1u << kFewPminLog2) && (qb1 >> kFewQLog2) < pmin &&
+                             qb1 - qb0 < 4 * pmin;
+            const bool wide = primes[je - 1] >= (1u << 31);
             if (wheel) {
                 for (uint32_t j = j0 + tid; j < je; j += 256) walk_wheel<kDirect>(h, qb0d, primes[j], df, dl, dq, wl, cnt, rh, cap, err);
+            } else if (few) {
+                __shared__ FewLds few_lds;
+                const uint64_t span = qb1 - qb0;
+                const FewRun f{qb0d, (uint32_t)qb0, (uint32_t)(qb0 % 30), span,
+                               (uint32_t)min(span, (uint64_t)0xFFFFFFFFu)};
+                if (single && wide) few_chunk<1, true>(h, f, primes, j0, je, tid, df, dl, dq, cmt, cnt, few_lds, err);
+                else if (single) few_chunk<1, false>(h, f, primes, j0, je, tid, df, dl, dq, cmt, cnt, few_lds, err);
+                else if (wide) few_chunk<4, true>(h, f, primes, j0, je, tid, df, dl, dq, cmt, cnt, few_lds, err);
+                else few_chunk<4, false>(h, f, primes, j0, je, tid, df, dl, dq, cmt, cnt, few_lds, err);
             } else {
                 for (uint32_t k = 0; j0 + k * 256 < je; k += single ? kS : kG) {
                     if (single) bm_single<kDirect>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);

@@ -208,4 +208,4 @@ def test_csv_row_keys_in_reference_order(stats_env):
     assert keys[len(CSV_KEYS)] == "sysman" and row.split()[len(CSV_KEYS)] == "sysman=off"
     v = compare.kv(row)
     assert v["result"] == "PASS" and v["sched"] == "driver" and int(v["segments"]) == 2
-    assert int(v["primes"]) == 46640 and int(v["stats_pmax"]) == 2221   # the window's stats_ref figures
+    assert int(v["primes"]) == 46640 and int(v["twins"]) == 1396   # the window's stats_ref figures
