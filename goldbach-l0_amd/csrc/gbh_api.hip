@@ -140,7 +140,7 @@ int gbh_open(int device, gbh_ctx** out)
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&gbh_dev::k_window),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) (void)hipGetLastError();   // optional: windows up to the default limit still work
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&gbh_dev::k_harvest<false>),
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&gbh_dev::k_harvest<false, false>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024) != hipSuccess)
         (void)hipGetLastError();
     *out = c;
@@ -300,18 +300,40 @@ int gbh_launch_harvest(gbh_ctx* c, int s, const gbh_harvest_args* a)
     if (nb > GBH_HL_BUCKETS) return fail(GBH_E_ARG, "harvest: more window lists than the work-group counts");
     if (a->m_base % 240) return fail(GBH_E_ARG, "harvest: bitmap base not a multiple of 240");
     if (a->stage > GBH_HL_STAGE) return fail(GBH_E_ARG, "harvest: stage above GBH_HL_STAGE");
-    const size_t lds = a->stage ? size_t(nb) * 8 + size_t(a->stage) * 6 + 16 : size_t(nb) * 4;
+    size_t lds = a->stage ? size_t(nb) * 8 + size_t(a->stage) * 6 + 16 : size_t(nb) * 4;
+    uint32_t few_off = 0;
+    if (a->few && direct) {                   // the compacted path's queues after the counters
+        few_off = uint32_t((lds + 15) & ~size_t(15));
+        lds = few_off + sizeof(gbh_dev::FewLds);
+    }
     if (lds > 144 * 1024) return fail(GBH_E_ARG, "harvest: lists and stage exceed the work-group's LDS");
-    if (direct)
-        hipLaunchKernelGGL(gbh_dev::k_harvest<true>, dim3(a->nchunks), dim3(256), lds, S(c, s), a->b, a->primes,
-                           a->chunks, a->runs, a->runs ? a->nruns : 1u, uint32_t(nb), a->t_hl, a->t_hi, a->t_hi_end,
-                           a->mbits, a->m_base, a->m_span, a->cap, a->stage, a->hits, a->hit_n, a->rdesc, a->region,
-                           a->fills, a->err);
+    const bool few = a->few && direct;
+    const void* fn = few ? reinterpret_cast<const void*>(&gbh_dev::k_harvest<true, true>)
+                   : direct ? reinterpret_cast<const void*>(&gbh_dev::k_harvest<true, false>)
+                            : reinterpret_cast<const void*>(&gbh_dev::k_harvest<false, false>);
+    uint32_t grid = a->nchunks;
+    if (a->queue) {                           // persistent: the work-groups the GPU holds at once
+        int per_cu = 0;
+        GBH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, lds));
+        if (a->queue_wg) per_cu = std::min(per_cu, int(a->queue_wg));
+        grid = std::min<uint32_t>(a->nchunks, uint32_t(std::max(per_cu, 1)) * uint32_t(std::max(c->info.cus, 1)));
+        hipLaunchKernelGGL(gbh_dev::k_fill, dim3(1), dim3(256), 0, S(c, s), a->queue, size_t(1), 0u);
+        GBH_HIP(hipGetLastError());
+    }
+#define GBH_HARVEST_ARGS                                                                                           \
+    a->b, a->primes, a->chunks, a->runs, a->runs ? a->nruns : 1u, uint32_t(nb), a->t_hl, a->t_hi, a->t_hi_end,      \
+        a->mbits, a->m_base, a->m_span, a->cap, a->stage, a->hits, a->hit_n, a->rdesc, a->region, a->fills, a->err, \
+        a->nchunks, a->queue, few_off
+    if (few)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(gbh_dev::k_harvest<true, true>), dim3(grid), dim3(256), lds, S(c, s),
+                           GBH_HARVEST_ARGS);
+    else if (direct)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(gbh_dev::k_harvest<true, false>), dim3(grid), dim3(256), lds, S(c, s),
+                           GBH_HARVEST_ARGS);
     else
-        hipLaunchKernelGGL(gbh_dev::k_harvest<false>, dim3(a->nchunks), dim3(256), lds, S(c, s), a->b, a->primes,
-                           a->chunks, a->runs, a->runs ? a->nruns : 1u, uint32_t(nb), a->t_hl, a->t_hi, a->t_hi_end,
-                           a->mbits, a->m_base, a->m_span, a->cap, a->stage, a->hits, a->hit_n, a->rdesc, a->region,
-                           a->fills, a->err);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(gbh_dev::k_harvest<false, false>), dim3(grid), dim3(256), lds, S(c, s),
+                           GBH_HARVEST_ARGS);
+#undef GBH_HARVEST_ARGS
     GBH_HIP(hipGetLastError());
     return GBH_OK;
 }

@@ -532,10 +532,10 @@ struct FewCand {
     uint32_t m_lo, m_hi_adj;           // m_est: low word, high bits | (m - m_est) << 20
 };
 
-// A work-group's queues: per wave two rounds' worth (one being filled, the
-// previous one being resolved).
+// A work-group's queues, one per wave: a round first reads the previous
+// round's entries into registers, then scans its primes into the same slots.
 struct FewLds {
-    FewCand q[4][2][kFewWave];
+    FewCand q[4][kFewWave];
 };
 
 // The per-run constants of the scan.
@@ -623,7 +623,7 @@ __device__ __forceinline__ void few_chunk(const HlCtx& h, const FewRun& f, const
     uint32_t prev_n = 0;
     // the last pass only resolves the last round's queue
     for (uint32_t r = 0; r <= rounds; ++r) {
-        const FewCand* pq = L.q[wv][(r + 1) & 1];
+        const FewCand* pq = L.q[wv];
         FewCand c[kD];
         uint64_t w[kD];
         bool live[kD];
@@ -649,7 +649,7 @@ __device__ __forceinline__ void few_chunk(const HlCtx& h, const FewRun& f, const
         }
         uint32_t n = 0;
         if (r < rounds) {
-            FewCand* q = L.q[wv][r & 1];
+            FewCand* q = L.q[wv];
 #pragma unroll
             for (uint32_t u = 0; u < kR; ++u) few_scan<kM, kWide>(h, f, p[u], q, n, bad);
 #pragma unroll
@@ -686,9 +686,11 @@ __device__ __forceinline__ void few_chunk(const HlCtx& h, const FewRun& f, const
 // bound; nullptr runs is one run of the whole batch.
 // A bitmap prime with p^3 <= qb1 breaks the two-factor rule: fail closed.
 // kDirectForm: the direct form (rdesc, region and fills given) or the
-// counted / staged forms; one instance each, so neither carries the other's
-// registers.
-template <bool kDirectForm>
+// counted / staged forms; kFew: the direct form with the compacted path of
+// bitmap primes with few multipliers. One instance each, so none carries
+// another's registers (the compacted path's would crowd out the kernels
+// that run beside the harvest).
+template <bool kDirectForm, bool kFew>
 __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __restrict__ primes,
                                                  const uint32_t* __restrict__ chunks,
                                                  const gbh_hl_run* __restrict__ runs, uint32_t nruns, uint32_t nbmax,
@@ -698,9 +700,10 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
                                                  uint64_t m_span, uint32_t cap, uint32_t stage,
                                                  uint32_t* __restrict__ hits, uint32_t* __restrict__ hit_n,
                                                  const uint32_t* __restrict__ rdesc, uint32_t* __restrict__ region,
-                                                 uint32_t* __restrict__ fills, uint32_t* __restrict__ err)
+                                                 uint32_t* __restrict__ fills, uint32_t* __restrict__ err,
+                                                 uint32_t nchunks, uint32_t* __restrict__ next, uint32_t few_off)
 {
-    __shared__ uint32_t df[GBH_MAX_BATCH], dl[GBH_MAX_BATCH], dq[GBH_MAX_BATCH * 8], cmt[8], sn;
+    __shared__ uint32_t df[GBH_MAX_BATCH], dl[GBH_MAX_BATCH], dq[GBH_MAX_BATCH * 8], cmt[8], sn, chunk_at;
     __shared__ uint64_t any_hit[GBH_HL_CHUNK / 64 + 4 * (kG > kS ? kG : kS)];   // bitmap primes with a set bit in range (pass 2 walks those)
     __shared__ WheelLds wl;
     extern __shared__ uint32_t cnt[];     // [nseg][8][nwin] counts of a run; with a stage: run counters, the stage
@@ -716,10 +719,26 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
         for (uint32_t i = 0; i < 8; ++i) v |= (uint32_t)c_cls_of[(res_of(tid) * res_of(i)) % 30] << (3 * i);
         cmt[tid] = v;
     }
-    const uint32_t j0 = chunks[blockIdx.x], j1 = chunks[blockIdx.x + 1];
+    // persistent form (next given): as many work-groups as the GPU holds at
+    // once, each taking chunk after chunk from the device counter, so no
+    // chunk waits for a dispatch slot behind a longer one; else chunk =
+    // work-group
+    for (uint32_t pass = 0;; ++pass) {
+    uint32_t kc;
+    if (next) {
+        __syncthreads();                      // the previous chunk is done with chunk_at and the LDS tables
+        if (tid == 0) chunk_at = atomicAdd(next, 1u);
+        __syncthreads();
+        kc = chunk_at;
+        if (kc >= nchunks) break;
+    } else {
+        if (pass) break;
+        kc = blockIdx.x;
+    }
+    const uint32_t j0 = chunks[kc], j1 = chunks[kc + 1];
     if (j1 - j0 > GBH_HL_CHUNK) {            // beyond the pass-2 flags: fail closed
         if (tid == 0) atomicOr(err, 1u);
-        return;
+        continue;
     }
     // a chunk is all wheel primes or all bitmap primes (the host splits at t_hl, t_hi and t_hi_end)
     const bool wheel = j0 < j1 && (primes[j0] < t_hl || (primes[j0] >= t_hi && primes[j0] < t_hi_end));
@@ -735,7 +754,7 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
         }
         if (j0 >= je) {                       // no prime of the chunk in this run: empty regions
             if (direct) {
-                uint32_t* fl = fills + (size_t)blockIdx.x * (b.nseg * 8) + seg0 * 8;
+                uint32_t* fl = fills + (size_t)kc * (b.nseg * 8) + seg0 * 8;
                 for (uint32_t i = tid; i < nb; i += 256) fl[i] = 0;
             }
             continue;
@@ -759,8 +778,8 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
         h.stb = stb;
         h.sn = &sn;
         h.scap = stage;
-        h.region = direct ? region + rdesc[2 * blockIdx.x] + (size_t)seg0 * 8 * rdesc[2 * blockIdx.x + 1] : nullptr;
-        h.rcap = direct ? rdesc[2 * blockIdx.x + 1] : 0u;
+        h.region = direct ? region + rdesc[2 * kc] + (size_t)seg0 * 8 * rdesc[2 * kc + 1] : nullptr;
+        h.rcap = direct ? rdesc[2 * kc + 1] : 0u;
         h.qb0 = qb0;
         h.dmax = dl[rn - 1];
         h.hs_magic = (rn > 1 && df[1] > df[0]) ? (uint32_t)((1ull << 32) / (df[1] - df[0] + 1)) : 0u;
@@ -789,13 +808,14 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
             // bitmap primes with at most 4 multipliers in the span, inside the
             // compacted path's quotient bounds (uniform over the chunk)
             const uint64_t pmin = primes[j0];
-            const bool few = !wheel && pmin >= (1u << kFewPminLog2) && (qb1 >> kFewQLog2) < pmin &&
+            const bool few = kFew && few_off && !wheel && pmin >= (1u << kFewPminLog2) && (qb1 >> kFewQLog2) < pmin &&
                              qb1 - qb0 < 4 * pmin;
             const bool wide = primes[je - 1] >= (1u << 31);
             if (wheel) {
                 for (uint32_t j = j0 + tid; j < je; j += 256) walk_wheel<kDirect>(h, qb0d, primes[j], df, dl, dq, wl, cnt, rh, cap, err);
-            } else if (few) {
-                __shared__ FewLds few_lds;
+            } else if (kFew && few) {
+                // the queues: the dynamic LDS from byte few_off (given only when the batch has such chunks)
+                FewLds& few_lds = *reinterpret_cast<FewLds*>(reinterpret_cast<char*>(cnt) + few_off);
                 const uint64_t span = qb1 - qb0;
                 const FewRun f{qb0d, (uint32_t)qb0, (uint32_t)(qb0 % 30), span,
                                (uint32_t)min(span, (uint64_t)0xFFFFFFFFu)};
@@ -810,7 +830,7 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
                 }
             }
             __syncthreads();
-            uint32_t* fl = fills + (size_t)blockIdx.x * (b.nseg * 8) + seg0 * 8;
+            uint32_t* fl = fills + (size_t)kc * (b.nseg * 8) + seg0 * 8;
             for (uint32_t i = tid; i < nb; i += 256) fl[i] = min(cnt[i], h.rcap);
             continue;
         }
@@ -846,6 +866,7 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
                 else bm_group<kWrite>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
             }
         }
+    }
     }
 }
 

@@ -197,7 +197,8 @@ struct ChunkPlan {
     uint32_t nbb = 0, nwheel = 0, nwheel_hi = 0, nwheel_hi2 = 0, nrand = 0, nx = 0;
     double ncb = 0;
     std::string mode;
-    uint32_t region_fix = 0;
+    uint32_t region_fix = 0, chunks_cu = 0;
+    uint32_t nfew = UINT32_MAX, nsingle = UINT32_MAX;
     std::vector<uint32_t> chunks, rdesc, groups;   // the tables uploaded to a slot
     uint64_t region_entries = 0;
     uint32_t nchunks = 0, stage = 0, ngroups = 0, part_cap = GBH_PART_CAP;
@@ -235,6 +236,7 @@ struct Engine::State {
     struct Slot {
         Buf map, i0, coff, coffa, pending, scratch, hits, hit_n, chunks, rdesc, groups, region, fills;
         Buf ridge_more;                     // ridge records beyond the output block's GBH_RIDGE_CAP
+        Buf hq;                             // the persistent harvest's chunk counter
         Buf tab;                            // the batch tables (tab_layout)
         Buf vmap, vhits, vhit_n, vchunks;   // the GB_BUCKET_VERIFY rebuild
     } sl[2];
@@ -528,11 +530,18 @@ private:
     // primes' words more than a sector apart)
     uint32_t nrand_ = 0;
     bool random_reads(uint32_t j) const { return j >= nwheel_ && j < nrand_ && !on_wheel(j); }
+    // bitmap primes from nfew_ on have at most four multipliers in a batch
+    // run's q span, from nsingle_ on at most one (the harvest's compacted
+    // path; set by the chunk plan)
+    uint32_t nfew_ = UINT32_MAX, nsingle_ = UINT32_MAX;
+    // the compacted path pays for its registers only when it serves a
+    // quarter of the large primes or more
+    bool few_path() const { return nfew_ < nlarge_ && 4ull * (nlarge_ - nfew_) >= nlarge_; }
     // the first chunk boundary above a: the wheel / bitmap / band /
     // random-read boundaries (chunks are all of one kind)
     uint32_t next_cut(uint32_t a, uint32_t end) const
     {
-        for (uint32_t c : {nwheel_, nwheel_hi_, nwheel_hi2_, nrand_})
+        for (uint32_t c : {nwheel_, nwheel_hi_, nwheel_hi2_, nrand_, nfew_, nsingle_})
             if (c > a && c < end) end = c;
         return end;
     }
@@ -566,6 +575,16 @@ private:
     // instead of the size planned from the expected hits (a small N forces
     // the region overflow and the counted re-run of the batch)
     uint32_t region_fix_ = env::u32("GB_HL_REGION", 0, 0, GBH_PART_CAP);
+    // GB_HL_PERSIST: the harvest as persistent work-groups taking chunks from
+    // a device counter (1), or one work-group per chunk (0)
+    bool persist_ = env::onoff("GB_HL_PERSIST", true);
+    // GB_HL_PERSIST_WG: its work-groups per CU at most (0: as many as fit;
+    // fewer leave room for the window sieve running beside it)
+    uint32_t persist_wg_ = env::u32("GB_HL_PERSIST_WG", 0, 0, 64);
+    // GB_HL_CHUNKS_CU: harvest chunks per CU the chunk plan aims at (0:
+    // 16 for batches of at most 32 (segment, class) lists, else 8: the
+    // partition's work grows with the chunks times the lists)
+    uint32_t chunks_cu_ = env::u32("GB_HL_CHUNKS_CU", 0, 0, 256);
     uint64_t verified_ = 0;
     bool harvested_[2] = {false, false};
 };
@@ -618,7 +637,9 @@ void Pipeline::plan_bitmap(const PlanBounds& b)
                                             : uint64_t(E.kbatch) * b.span);
         const double q = double(sq_lo) * double(sq_lo);
         const double th = hits > 0 ? 0.1918 * qspan / hits : 1e300;
-        const double th2 = std::sqrt(q * std::log(std::sqrt(std::max(q, 4.0))) / 1920.0);
+        // ... and below a quarter of the span (from there the harvest's
+        // compacted path scans them: at most four multipliers each)
+        const double th2 = std::min(std::sqrt(q * std::log(std::sqrt(std::max(q, 4.0))) / 1920.0), qspan / 4);
         const auto at = [&](double x) {
             return uint32_t(std::lower_bound(lp + nwheel_, lp + nlarge_,
                                              uint32_t(std::min(std::max(x, 1.0), double(UINT32_MAX)))) - lp);
@@ -750,6 +771,7 @@ void Pipeline::plan(const PlanBounds& pb)
         B.pending.ensure(c, K * size_t(pcap_) * 4, 0);
         B.scratch.ensure(c, K * size_t(swords_) * 4, 0);
         B.ridge_more.ensure(c, K * size_t(GBH_RIDGE_MORE) * 16, 0);
+        B.hq.ensure(c, 256, 0);
         B.hit_n.ensure(c, K * 8 * size_t(nwin_max_) * 4, 0);
         if (nlarge_) B.hits.ensure(c, K * 8 * size_t(nwin_max_) * cap_ * 4, 0);
         B.tab.ensure(c, tab_layout(K).bytes, 0);
@@ -823,8 +845,10 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
     ChunkPlan& cp = E.cplan;
     if (cp.valid && cp.nbb == nbb && close_rel(cp.ncb, ncb) && cp.nwheel == nwheel_ && cp.nwheel_hi == nwheel_hi_ &&
         cp.nwheel_hi2 == nwheel_hi2_ && cp.nrand == nrand_ && cp.nx == nx_ && cp.mode == hl_mode_ &&
-        cp.region_fix == region_fix_ && same_shape(cp.key, runs_in)) {
+        cp.region_fix == region_fix_ && cp.chunks_cu == chunks_cu_ && same_shape(cp.key, runs_in)) {
         nchunks_ = cp.nchunks;
+        nfew_ = cp.nfew;
+        nsingle_ = cp.nsingle;
         stage_ = cp.stage;
         direct_ = cp.direct;
         ngroups_ = cp.ngroups;
@@ -841,12 +865,31 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
     std::sort(rs.begin(), rs.end(), [](const RunCost& a, const RunCost& b) { return a.np < b.np; });
     stage_ = 0;
     if (!rs.empty()) {
-        constexpr double kPrimeCost = 4.0;
-        // a bitmap prime whose words are a random read (likely a TLB miss
-        // across the bitmap) costs as much as that many hits
-        constexpr double kRandCost = 24.0;
+        // Cost units per prime and per hit, by the way a prime is walked (a
+        // prime of the single-multiple scan = 4; measured per-work-group
+        // times on one MI355X, bench workload): bitmap primes with at most
+        // four multipliers 7.5, other bitmap primes 8, those whose words are
+        // a random read (likely a TLB miss across the bitmap) 32, wheel
+        // primes 11 plus 2.5 per listed multiple; every hit 1 more.
+        constexpr double kPrimeCost = 4.0, kFewCost = 7.5, kBitmapCost = 8.0, kRandCost = 32.0;
+        constexpr double kWheelCost = 11.0, kWheelHitCost = 2.5;
         const size_t nr = rs.size();
         const uint32_t npmax = rs.back().np;
+        {   // the compacted path's classes, for the largest run's q span (3.75 q-units per class bit)
+            double span_q = 0;
+            for (const RunCost& r : rs) span_q = std::max(span_q, 3.75 * r.bits);
+            const uint32_t* lp = E.base.data() + E.win_hi;
+            const auto first_above = [&](double x) {
+                return uint32_t(std::upper_bound(lp, lp + npmax, uint32_t(std::min(x, 4294967295.0))) - lp);
+            };
+            nfew_ = first_above(span_q / 4);
+            nsingle_ = first_above(span_q);
+        }
+        const auto prime_cost = [&](uint32_t j) {
+            return on_wheel(j) ? kWheelCost : random_reads(j) ? kRandCost : j >= nsingle_ ? kPrimeCost
+                 : j >= nfew_ ? kFewCost : kBitmapCost;
+        };
+        const auto hit_cost = [&](uint32_t j) { return on_wheel(j) ? 1.0 + kWheelHitCost : 1.0; };
         // runs [k, nr) are the ones serving primes below rs[k].np: their
         // summed class bits, largest bits and smallest first q
         std::vector<double> suf_bits(nr + 1, 0.0), suf_max(nr + 1, 0.0), suf_qlo(nr + 1, 1e30);
@@ -872,13 +915,16 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
                     while (rs[ai].np <= a) ++ai;
                     const uint32_t e = next_cut(a, std::min(b, rs[ai].np));
                     const double q = suf_qlo[ai];
-                    total += (kPrimeCost + (random_reads(a) ? kRandCost : 0.0)) * (e - a) * double(nr - ai) +
-                             suf_bits[ai] * std::max(factor(a, q), factor(e - 1, q)) * (inv(e) - inv(a));
+                    total += prime_cost(a) * (e - a) * double(nr - ai) +
+                             hit_cost(a) * suf_bits[ai] * std::max(factor(a, q), factor(e - 1, q)) * (inv(e) - inv(a));
                     a = e;
                 }
             }
         }
-        const double target = std::max(total / (8.0 * E.cus), 2e4);
+        // several chunks per CU (four at a time resident): the last ones to
+        // start are short, so the kernel ends close to its total work / slots
+        const double per_cu = chunks_cu_ ? chunks_cu_ : nbb <= 32 ? 16.0 : 8.0;
+        const double target = std::max(total / (per_cu * E.cus), 2e4);
         // short batches' hits fit the work-groups' LDS stages: chunks of at
         // most kStageHits expected hits per run, one walk per prime
         constexpr double kStageHits = 0.7 * GBH_HL_STAGE;
@@ -897,9 +943,9 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
             while (rs[ai].np <= j) ++ai;   // runs [ai, nr) serve prime j
             const uint32_t hi = next_cut(j, std::min({npmax, j + GBH_HL_CHUNK, rs[ai].np}));
             const double nact = double(nr - ai), bits = suf_bits[ai], bmax = suf_max[ai], q = suf_qlo[ai];
-            const double pc = kPrimeCost + (random_reads(j) ? kRandCost : 0.0);
+            const double pc = prime_cost(j);
             const double hf = hit_factor(j, q);
-            const double f = on_wheel(j) ? 1.3 * kWheelFrac : hf, fh = hf;
+            const double f = (on_wheel(j) ? 1.3 * kWheelFrac : hf) * hit_cost(j), fh = hf;
             const double ij = j == nwheel_ ? inv_wheel_ : E.inv_at(j);
             const auto ok = [&](uint32_t e) {   // chunk [j, e) within the targets (the rates barely change inside)
                 const double si = inv(e) - ij;
@@ -997,6 +1043,9 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
     cp.nx = nx_;
     cp.mode = hl_mode_;
     cp.region_fix = region_fix_;
+    cp.chunks_cu = chunks_cu_;
+    cp.nfew = nfew_;
+    cp.nsingle = nsingle_;
     cp.nchunks = nchunks_;
     cp.stage = stage_;
     cp.direct = direct_;
@@ -1237,7 +1286,8 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
                            mb_ ? mb_->m_base : 0, mb_ ? mb_->m_span : 0, B.hits.as<uint32_t>(),
                            B.hit_n.as<uint32_t>(), direct ? B.rdesc.as<uint32_t>() : nullptr,
                            direct ? B.region.as<uint32_t>() : nullptr, direct ? B.fills.as<uint32_t>() : nullptr,
-                           &out[0].err};
+                           &out[0].err, persist_ ? B.hq.as<uint32_t>() : nullptr, persist_wg_,
+                           few_path() ? 1u : 0u};
         check(gbh_event_record(c, ms, ev[EV_HL0].e), "event");
         check(gbh_launch_harvest(c, ms, &h), "harvest");
         check(gbh_event_record(c, ms, ev[EV_HL1].e), "event");
@@ -1343,7 +1393,7 @@ void Pipeline::enqueue_verify(uint32_t s, const gbh_batch& b, gbh_segout* out, i
     check(gbh_zero(c, ms, B.vhit_n.p, size_t(8) * b.nwin * 4), "verify zero");
     const gbh_harvest_args h{vb, E.d_large, B.vchunks.as<uint32_t>(), nullptr, 0, 0, nvchunks_, cap_v_,
                              UINT32_MAX, UINT32_MAX, UINT32_MAX, 0, nullptr, 0, 0, B.vhits.as<uint32_t>(), B.vhit_n.as<uint32_t>(),
-                             nullptr, nullptr, nullptr, &out[s].err};
+                             nullptr, nullptr, nullptr, &out[s].err, nullptr, 0, 0};
     check(gbh_launch_harvest(c, ms, &h), "verify harvest");
     const gbh_window_args w{vb, E.d_win.as<uint32_t>(), nwin_, ncoop, B.i0.as<uint32_t>(),
                             B.vhits.as<uint32_t>(), B.vhit_n.as<uint32_t>(), cap_v_, 0};
