@@ -1,27 +1,50 @@
 """Throughput benchmark of the MI355X verifier on one GPU.
 
-Workload (one step): one production segment range at 4e18, every even n in
-[4e18, 4e18 + 467,499,998] (233,750,000 evens), through the whole pipeline
-with exact statistics and gaps, run as the reference runs that range (two
-half segments by its at-least-two-segments rule). The engine (base primes,
-tables, buffers) and the multiplier bitmap of the range are built before the
-timed steps, as the reference builds its tables at start-up; each step is one
-complete run (segmentation, GPU pipeline, CPU fallback, statistics commit).
-value = evens verified per second over the timed steps.
+Headline (value; one step): one production segment range at 4e18, every even
+n in [4e18, 4e18 + 467,499,998] (233,750,000 evens), through the whole
+pipeline with exact statistics and gaps, run as the reference runs that range
+(two half segments by its at-least-two-segments rule). The engine (base
+primes, tables, buffers) and the multiplier bitmap of the range are built
+before the timed steps, as the reference builds its tables at start-up; each
+step is one complete run (segmentation, GPU pipeline, CPU fallback,
+statistics commit). value = evens verified per second over the timed steps.
 
-roofline: the kernel with the most device time per step (HIP events around
-each launch on the engine's stream), its algorithmic HBM bytes per launch
-(the primes, bitmap words and hit lists it must move once) over its average
-launch time, against the MI355X HBM peak; traffic: HBM bytes per launch
-measured by rocprofv3 FETCH_SIZE / WRITE_SIZE on this same command, scaled by
-the counters' reading on a plain device copy of known size (the committed
-profile summary), or null without one.
+The same JSON line carries the rest of the metric, each measured in this run:
+  full_segments_4e18  [4e18, 4e18 + 1e12]: 2140 segments (2139 full and a
+                      17.5M-integer tail), G evens/s, and
+                      the window's published prime count, largest p(n) and
+                      its first n checked against the run's statistics.
+  batched             the 256 seeded uneven ranges of
+                      tests/golden/cfg4_ranges.txt: plain per-range runs one
+                      after another against one Batcher over all of them;
+                      both rates, the ratio, and whether every range's
+                      statistics (GBSTATS line and state text) are identical.
+  serving             100 seeded requests arriving as Poisson streams of 10,
+                      30, 100, 300 and 1000 requests/s into the request loop
+                      of goldbach-l0_amd/serve.py: p50 / p99 latency and
+                      throughput per rate, beside sequential plain execution
+                      of the same requests (measured service times, FIFO).
+
+roofline: for the kernel with the most device time per step (HIP events
+around each launch on the engine's stream), its algorithmic HBM bytes per
+launch (the primes, bitmap words and hit lists it must move once) over its
+average launch time, against the MI355X HBM peak; traffic: HBM bytes per
+launch from rocprofv3 FETCH_SIZE / WRITE_SIZE on this same workload, scaled
+by the counters' reading on a plain device copy of known size. "binding"
+names the resource that binds each main kernel, from the committed counter
+summaries of this workload (profiles/*_counters.json): the busier of VALU
+issue (busy VALU cycles of its waves) and LDS (cycles the LDS serves indexed
+operations), each over the CU cycles of the kernel's live launch time.
 
 cpu_baseline: the oracle's restatement of the reference's only CPU path for
-these statistics (stats_ref: per-even ascending Miller-Rabin) on a bounded
-sample of the same range, on the host's cores (at most 16).
+these statistics (stats_ref: per-even ascending Miller-Rabin) over the whole
+headline range, on the host cores this process may use (at most 16, the
+host share of one GPU); its output is compared with the GPU run's statistics
+(parity) and with the reference's own stats_ref output of the range
+(tests/golden/ref_cfg2.txt).
 
 Prints one JSON line. --gpus above 1 is refused: this build runs on one GPU.
+--headline-only skips the sub-objects (for profiling), --no-cpu the CPU leg.
 --copy-calibration runs only a 1 GiB device-to-device copy (three times), the
 reference kernel for the traffic counters.
 
@@ -29,19 +52,8 @@ Other workloads (--workload; each prints its own JSON line):
   slice    the throughput slice: 1e12 consecutive integers from 4.5e18
            (2140 segments: 2139 full and a 17.5M-integer tail), one run per
            step; value = evens per second of wall time.
-  batched  the 256 seeded ranges of tests/golden/cfg4_ranges.txt (starts
-           log-uniform over [1e9, 5e18], lengths log-uniform over
-           [1e5, 4.675e8]): plain per-range runs one after another against
-           one Batcher over all of them (every range's statistics identical
-           either way); value = batched evens per second, with the plain rate
-           and the ratio.
-  serving  100 seeded requests (start uniform in [1e12, 5e18], length
-           log-uniform in [1e4, 1e9]) arriving as Poisson streams of 10, 30,
-           100, 300 and 1000 requests/s into the request loop of
-           goldbach-l0_amd/serve.py (continuous batching, 64 GB admission
-           budget, resident tables): p50 / p99 latency and throughput per
-           rate, beside sequential plain execution of the same requests
-           (measured service times, FIFO).
+  batched  the batched sub-object above as a line of its own.
+  serving  the serving sub-object above as a line of its own.
 """
 import argparse
 import ctypes
@@ -60,9 +72,18 @@ START = 4_000_000_000_000_000_000
 LIMIT = START + 467_499_998
 SEGMENT = 467_500_000
 EVENS = (LIMIT - START) // 2 + 1
+FULL_LIMIT = START + 10**12
+# the window's published figures: primes in (4e18, 4e18 + 1e12], the largest
+# smallest-Goldbach-prime p(n) and the first n attaining it
+FULL_PUBLISHED = {"primes": 23_346_564_662, "pmax": 6073, "pmax_first_n": START + 402_622_391_632}
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = ROOT / "profiles" / "k_harvest_4e18_segment_pair_traffic.json"
-CPU_SAMPLE_PER_CORE = 2_000_000
+CUS = 256
+CLOCK_HZ = 2.4e9
+PROFILES = ROOT / "profiles"
+TRAFFIC_FILE = PROFILES / "k_harvest_4e18_segment_pair_traffic.json"
+COUNTERS_SEGMENT = PROFILES / "pipeline_4e18_segment_pair_counters.json"
+COUNTERS_SLICE = PROFILES / "pipeline_4p5e18_slice_counters.json"
+CPU_THREADS_MAX = 16
 SLICE_START = 4_500_000_000_000_000_000
 
 
@@ -94,21 +115,42 @@ def copy_calibration():
     print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
 
 
-def cpu_baseline():
-    """The oracle's stats_ref restatement on a bounded sample of the range."""
+def say(**kv):
+    """A progress line on stderr (the JSON result is the only stdout line)."""
+    print(json.dumps({"progress": kv}), file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in pathlib.Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(gpu_line, gpu_state):
+    """The oracle's stats_ref restatement over the whole headline range, and
+    its parity with the GPU run's statistics and with the golden fixture."""
     import compare
 
-    cores = min(16, os.cpu_count() or 1)
-    n_evens = CPU_SAMPLE_PER_CORE * cores
-    a = START
-    b = START + 2 * (n_evens - 1)
+    allowed = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cores = max(1, min(CPU_THREADS_MAX, allowed))
+    say(stage="cpu_baseline", evens=EVENS, threads=cores)
     t0 = time.perf_counter()
-    compare.oracle_ref(a, b, cores)
+    line = compare.oracle_ref(START, LIMIT, cores)
     dt = time.perf_counter() - t0
-    return {"value": n_evens / dt / 1e9, "unit": "G evens/s", "cores": cores, "kind": "port",
-            "sample": f"stats_ref restatement (oracle/gb_oracle.c ref: per-even Miller-Rabin, exact histogram, "
-                      f"primes, twins, gaps) over [{a}, {b}]: {n_evens} evens of the same range, {dt:.1f} s wall "
-                      f"on {cores} threads"}
+    _, _, golden = compare.load_fixture(ROOT / "tests" / "golden" / "ref_cfg2.txt")
+    bad = compare.compare_ref(gpu_line, gpu_state, line)
+    bad_golden = compare.compare_ref(gpu_line, gpu_state, golden)
+    return {"value": EVENS / dt / 1e9, "unit": "G evens/s", "cores": cores, "kind": "port",
+            "sample": f"the whole headline range [{START}, {LIMIT}] ({EVENS} evens) through the stats_ref "
+                      f"restatement (oracle/gb_oracle.c ref: per-even ascending Miller-Rabin, exact histogram, "
+                      f"primes, twins, gaps), {dt:.1f} s wall on {cores} threads",
+            "seconds": dt, "nproc": os.cpu_count(), "affinity_cpus": allowed, "cpu_model": cpu_model(),
+            "parity": not bad, "parity_mismatches": bad[:4],
+            "parity_reference_fixture": not bad_golden, "reference_fixture_mismatches": bad_golden[:4]}
 
 
 def traffic():
@@ -119,6 +161,36 @@ def traffic():
                 + t["write_kib_per_dispatch"] * 1024 / t["copy_write_factor"])
     except (OSError, KeyError, ValueError, ZeroDivisionError):
         return None
+
+
+def binding(counter_file, kern, k):
+    """The binding resource of k_harvest and k_window from the committed
+    counters of the workload (per harvest launch; per batch for the window
+    stage's two launches), over each kernel's live launch time in this run:
+    VALU = SQ_ACTIVE_INST_VALU (quad-cycles of VALU work summed over waves;
+    each CU's 4 SIMDs take one wave64 instruction per 4 cycles) over the CU
+    cycles; LDS = SQ_LDS_IDX_ACTIVE over the CU cycles. The resource named is
+    the busier one."""
+    try:
+        c = json.loads(counter_file.read_text())
+    except (OSError, ValueError):
+        return None
+    out = {"counters": str(counter_file.relative_to(ROOT)), "cus": CUS, "clock_ghz": CLOCK_HZ / 1e9}
+    for name in ("k_harvest", "k_window"):
+        per = c.get("kernels", {}).get(name)
+        sec, launches = kern[name][0], kern[name][1]
+        if not per or not launches:
+            continue
+        launch_s = sec / launches
+        cu_cycles = CUS * CLOCK_HZ * launch_s
+        valu = per["SQ_ACTIVE_INST_VALU"] / cu_cycles
+        lds = per["SQ_LDS_IDX_ACTIVE"] / cu_cycles
+        out[name] = {"resource": "VALU issue" if valu >= lds else "LDS", "frac": max(valu, lds),
+                     "valu_frac": valu, "lds_frac": lds,
+                     "lds_bank_conflict_frac": per["SQ_LDS_BANK_CONFLICT"] / cu_cycles,
+                     "valu_insts": per["SQ_INSTS_VALU"], "lds_insts": per["SQ_INSTS_LDS"],
+                     "per": per.get("per", "launch"), "launch_ms": 1e3 * launch_s}
+    return out
 
 
 def stats_files(td):
@@ -138,62 +210,121 @@ def tmpfs():
     return "/dev/shm" if os.path.isdir("/dev/shm") and os.access("/dev/shm", os.W_OK) else None
 
 
-def harvest_roofline(runs):
-    """The dominant kernel's roofline over a list of run results."""
-    kern = {"k_harvest": (sum(r["harvest_kernel_seconds"] for r in runs), sum(r["harvest_launches"] for r in runs),
+def kernel_totals(runs):
+    return {"k_harvest": (sum(r["harvest_kernel_seconds"] for r in runs), sum(r["harvest_launches"] for r in runs),
                           sum(r["harvest_bytes"] for r in runs)),
             "k_window": (sum(r["window_kernel_seconds"] for r in runs), sum(r["window_launches"] for r in runs),
                          sum(r["window_bytes"] for r in runs))}
+
+
+def hbm_roofline(kern):
+    """The dominant kernel's HBM roofline over kernel totals."""
     dom = max(kern, key=lambda n: kern[n][0])
     sec, launches, nbytes = kern[dom]
     launch_s = sec / max(launches, 1)
     bytes_per_launch = nbytes / max(launches, 1)
     achieved = bytes_per_launch / launch_s / 1e9 if launch_s > 0 else 0.0
-    return kern, {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                  "frac": achieved / HBM_PEAK_GBS, "kernel": dom, "bytes_per_launch": bytes_per_launch,
-                  "launch_ms": 1e3 * launch_s}
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "kernel": dom, "bytes_per_launch": bytes_per_launch,
+            "launch_ms": 1e3 * launch_s}
 
 
-def bench_slice(args):
+def per_step(runs, key):
+    return sum(r[key] for r in runs) / len(runs)
+
+
+def stage_ms(runs):
+    return {"large": 1e3 * per_step(runs, "large_seconds"), "sieve": 1e3 * per_step(runs, "sieve_seconds"),
+            "check": 1e3 * (per_step(runs, "verify_seconds") - per_step(runs, "count_seconds")),
+            "count": 1e3 * per_step(runs, "count_seconds")}
+
+
+def timed_runs(eng, td, lib, limit, start, steps, warmup):
+    """warmup + steps runs of [start, limit] (fresh statistics each); the
+    timed runs' results, their wall time, and the last run's state text."""
+    for _ in range(warmup):
+        fresh(td)
+        eng.run(limit, SEGMENT, start)
+    device_sync(lib)
+    runs = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fresh(td)
+        runs.append(eng.run(limit, SEGMENT, start))   # synchronous: every segment collected and committed
+    device_sync(lib)
+    wall = time.perf_counter() - t0
+    return runs, wall, pathlib.Path(td, "st.state").read_text()
+
+
+def measure_headline(args, lib):
     import gbamd
 
-    start, limit = SLICE_START, SLICE_START + 10**12 - 2
-    evens = (limit - start) // 2 + 1
-    lib = hip()
     with tempfile.TemporaryDirectory(dir=tmpfs()) as td, gbamd.env_scope(**stats_files(td)):
-        eng = gbamd.Engine(limit, SEGMENT)
-        prep_s = eng.prepare(limit, start)
-        for _ in range(args.warmup):
-            fresh(td)
-            eng.run(limit, SEGMENT, start)
-        device_sync(lib)
-        runs = []
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            fresh(td)
-            runs.append(eng.run(limit, SEGMENT, start))
-        device_sync(lib)
-        wall = time.perf_counter() - t0
+        eng = gbamd.Engine(LIMIT, SEGMENT)
+        prep_s = eng.prepare(LIMIT, START)
+        runs, wall, state = timed_runs(eng, td, lib, LIMIT, START, args.steps, args.warmup)
+        init_s = eng.init_seconds
         eng.close()
     k = len(runs)
-    kern, roof = harvest_roofline(runs)
-    per_step = lambda key: sum(r[key] for r in runs) / k
-    print(json.dumps({
-        "metric": "G evens/s at 4.5e18 w/ exact stats, 1e12-integer slice", "value": evens * k / wall / 1e9,
-        "unit": "G evens/s", "n_gpus": 1, "steps": k, "warmup": args.warmup, "ms_per_step": 1e3 * wall / k,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+    kern = kernel_totals(runs)
+    roof = hbm_roofline(kern)
+    out = {
+        "metric": "G evens/s at 4e18 w/ exact stats",
+        "value": EVENS * k / wall / 1e9,
+        "unit": "G evens/s",
+        "n_gpus": 1,
+        "steps": k,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / k,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
         "data": "synthetic: the integer range itself (no dataset)",
-        "config": {"workload": "throughput slice: 1e12 consecutive integers from 4.5e18, 2140 segments (2139 full, "
-                               "a 17.5M-integer tail), exact statistics and gaps",
-                   "start": start, "limit": limit, "segment": SEGMENT, "segments": runs[-1]["segments"],
-                   "gpu_loop_evens_per_s": per_step("evens_per_second"),
-                   "stage_ms": {"large": 1e3 * per_step("large_seconds"), "sieve": 1e3 * per_step("sieve_seconds"),
-                                "check": 1e3 * (per_step("verify_seconds") - per_step("count_seconds")),
-                                "count": 1e3 * per_step("count_seconds")},
+        "config": {"workload": "one production segment range at 4e18: evens in [4e18, 4e18 + 467,499,998], "
+                               "233,750,000 evens, 2 half segments (the reference's at-least-two-segments rule), "
+                               "exact statistics and gaps",
+                   "start": START, "limit": LIMIT, "segment": SEGMENT, "segments": runs[-1]["segments"],
+                   "stage_ms": stage_ms(runs),
                    "kernel_ms": {n: 1e3 * v[0] / k for n, v in kern.items()},
-                   "kernel_launches": {n: v[1] / k for n, v in kern.items()},
-                   "host_plan_ms": 1e3 * per_step("host_plan_seconds"), "bitmap_prepare_s": prep_s},
-        "roofline": roof}), flush=True)
+                   "gpu_loop_ms": 1e3 * per_step(runs, "gpu_seconds"),
+                   "host_plan_ms": 1e3 * per_step(runs, "host_plan_seconds"),
+                   "engine_init_s": init_s, "bitmap_prepare_s": prep_s},
+        "roofline": {**{n: roof[n] for n in ("bound", "achieved", "peak", "unit", "frac")},
+                     "traffic": traffic() if roof["kernel"] == "k_harvest" else None,
+                     **{n: roof[n] for n in ("kernel", "bytes_per_launch", "launch_ms")},
+                     "binding": binding(COUNTERS_SEGMENT, kern, k)},
+    }
+    return out, runs[-1]["stats_line"], state
+
+
+def measure_full(lib, steps=2, warmup=1):
+    """[4e18, 4e18 + 1e12] in full segments, checked against the published figures."""
+    import compare
+    import gbamd
+
+    evens = (FULL_LIMIT - START) // 2 + 1
+    say(stage="full_segments_4e18", limit=FULL_LIMIT)
+    with tempfile.TemporaryDirectory(dir=tmpfs()) as td, gbamd.env_scope(**stats_files(td)):
+        eng = gbamd.Engine(FULL_LIMIT, SEGMENT)
+        prep_s = eng.prepare(FULL_LIMIT, START)
+        runs, wall, _ = timed_runs(eng, td, lib, FULL_LIMIT, START, steps, warmup)
+        eng.close()
+    k = len(runs)
+    last = runs[-1]
+    first = compare.kv(last["stats_line"]).get("new_first", "-")
+    firsts = {} if first in ("", "-") else {int(p): int(n) for p, n in (e.split(":") for e in first.split(";"))}
+    got = {"primes": last["stats_primes"], "pmax": last["stats_pmax"],
+           "pmax_first_n": firsts.get(last["stats_pmax"])}
+    every = all((r["stats_primes"], r["stats_pmax"], r["leftovers"]) == (last["stats_primes"], last["stats_pmax"], 0)
+                for r in runs)
+    kern = kernel_totals(runs)
+    return {"value": evens * k / wall / 1e9, "unit": "G evens/s", "start": START, "limit": FULL_LIMIT,
+            "evens": evens, "segments": last["segments"], "steps": k, "warmup": warmup,
+            "ms_per_step": 1e3 * wall / k, "published": FULL_PUBLISHED, "got": got,
+            "check": "pass" if got == FULL_PUBLISHED and every else "fail",
+            "stage_ms": stage_ms(runs), "kernel_ms": {n: 1e3 * v[0] / k for n, v in kern.items()},
+            "bitmap_prepare_s": prep_s}
 
 
 def load_cfg4():
@@ -202,63 +333,78 @@ def load_cfg4():
     return [(int(a), int(b)) for a, b, _ in rows]
 
 
-def bench_batched(args):
+def measure_batched(lib, steps=3, warmup=1):
+    """The 256 uneven ranges: plain per-range runs against one Batcher."""
     import gbamd
 
     ranges = load_cfg4()
     lo, hi = min(a for a, _ in ranges), max(b for _, b in ranges)
     evens = sum((b - a) // 2 + 1 for a, b in ranges)
-    lib = hip()
+    say(stage="batched", ranges=len(ranges))
     with tempfile.TemporaryDirectory(dir=tmpfs()) as td, gbamd.env_scope(**stats_files(td)):
         eng = gbamd.Engine(hi, SEGMENT)
         prep_s = eng.prepare(hi, lo)
 
-        def plain():
+        def plain(keep=False):
+            got = []
             for a, b in ranges:
                 fresh(td)
-                eng.run(b, SEGMENT, a)
+                r = eng.run(b, SEGMENT, a)
+                if keep:
+                    got.append((r["stats_line"], pathlib.Path(td, "st.state").read_text(), r["segments"],
+                                r["leftovers"]))
+            return got
 
-        def batched():
-            with gbamd.Batcher(eng, hi, lo, SEGMENT) as bt:
-                for a, b in ranges:
-                    bt.submit(a, b)
-                bt.run_all()
-                return bt.totals(), bt.batch_segments
+        # the batcher and its batch buffers are set up once, before the timed
+        # steps (as the engine is); each step submits every range and runs
+        # them all
+        bt = gbamd.Batcher(eng, hi, lo, SEGMENT)
 
-        for _ in range(args.warmup):
+        def batched(keep=False):
+            ids = [bt.submit(a, b) for a, b in ranges]
+            bt.run_all()
+            got = []
+            if keep:
+                for i in ids:
+                    r = bt.result(i)
+                    got.append((r["stats_line"], r["state"], r["segments"], r["leftovers"]))
+            return bt.totals(), bt.batch_segments, got
+
+        # the untimed first pass also checks every range's statistics
+        ref = plain(keep=True)
+        _, _, mine = batched(keep=True)
+        identical = sum(1 for x, y in zip(ref, mine) if x == y)
+        for _ in range(warmup - 1):
             plain()
             batched()
         device_sync(lib)
         t_plain, t_batch, tot = [], [], []
-        for _ in range(args.steps):
+        for _ in range(steps):
             t0 = time.perf_counter()
             plain()
             device_sync(lib)
             t_plain.append(time.perf_counter() - t0)
             t0 = time.perf_counter()
-            totals, kb = batched()
+            totals, kb, _ = batched()
             device_sync(lib)
             t_batch.append(time.perf_counter() - t0)
             tot.append(totals)
+        bt.close()
         eng.close()
     k = len(t_batch)
     wall_b, wall_p = sum(t_batch) / k, sum(t_plain) / k
-    kern, roof = harvest_roofline(tot)
-    print(json.dumps({
-        "metric": "batched vs plain per-range runs, G evens/s", "value": evens / wall_b / 1e9, "unit": "G evens/s",
-        "n_gpus": 1, "steps": k, "warmup": args.warmup, "ms_per_step": 1e3 * wall_b, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "u64",
-        "data": "synthetic: 256 seeded ranges (tests/golden/cfg4_ranges.txt)",
-        "config": {"workload": "batched uneven ranges: 256 ranges, seed 1, starts log-uniform over [1e9, 5e18], "
-                               "lengths log-uniform over [1e5, 4.675e8] integers",
-                   "ranges": len(ranges), "evens": evens, "segment": SEGMENT, "batch_segments": kb,
-                   "plain_value": evens / wall_p / 1e9, "plain_ms": 1e3 * wall_p, "batched_ms": 1e3 * wall_b,
-                   "ratio": wall_p / wall_b, "kernel_ms": {n: 1e3 * v[0] / k for n, v in kern.items()},
-                   "kernel_launches": {n: v[1] / k for n, v in kern.items()}, "bitmap_prepare_s": prep_s},
-        "roofline": roof}), flush=True)
+    kern = kernel_totals(tot)
+    return {"value": evens / wall_b / 1e9, "unit": "G evens/s", "plain_value": evens / wall_p / 1e9,
+            "ratio": wall_p / wall_b, "ranges": len(ranges), "evens": evens, "steps": k, "warmup": warmup,
+            "plain_ms": 1e3 * wall_p, "batched_ms": 1e3 * wall_b, "batch_segments": kb,
+            "identical_to_plain": identical, "parity": identical == len(ranges),
+            "kernel_ms": {n: 1e3 * v[0] / k for n, v in kern.items()},
+            "kernel_launches": {n: v[1] / k for n, v in kern.items()}, "bitmap_prepare_s": prep_s,
+            "workload": "256 ranges, seed 1, starts log-uniform over [1e9, 5e18], lengths log-uniform over "
+                        "[1e5, 4.675e8] integers (tests/golden/cfg4_ranges.txt)"}
 
 
-def bench_serving(args):
+def measure_serving():
     import gbamd
     import numpy as np
     import serve
@@ -268,6 +414,7 @@ def bench_serving(args):
     max_limit = serve.REQ_START_HI + 2 * serve.REQ_LEN_HI
     rates = (10, 30, 100, 300, 1000)
     out = []
+    say(stage="serving", requests=len(reqs))
     with tempfile.TemporaryDirectory(dir=tmpfs()) as td, gbamd.env_scope(**stats_files(td)):
         eng = gbamd.Engine(max_limit, serve.SEGMENT)
         prep_s = eng.prepare(max_limit, serve.REQ_START_LO)
@@ -288,6 +435,7 @@ def bench_serving(args):
             arrivals = serve.poisson_arrivals(len(reqs), float(rate), 100 + k)
             srv = serve.Server(eng, max_limit, serve.REQ_START_LO)
             rec = srv.serve(reqs, arrivals)
+            peak, held, slots = srv.peak_reserved, srv.device_bytes(), srv.slots
             srv.close()
             lat = np.array([r["completed"] - r["arrival"] for r in rec])
             span = max(r["completed"] for r in rec) - min(arrivals)
@@ -297,20 +445,50 @@ def bench_serving(args):
                         "p99_ms": 1e3 * float(np.percentile(lat, 99)), "throughput_gevens_s": evens / span / 1e9,
                         "plain_p50_ms": 1e3 * float(np.percentile(seq, 50)),
                         "plain_p99_ms": 1e3 * float(np.percentile(seq, 99)),
-                        "plain_throughput_gevens_s": evens / seq_span / 1e9})
-            print(json.dumps({"progress": "serving", **out[-1]}), file=sys.stderr, flush=True)
+                        "plain_throughput_gevens_s": evens / seq_span / 1e9,
+                        "peak_reserved_gib": peak / 2**30, "batch_buffers_gib": held / 2**30,
+                        "inflight_slots": slots})
+            say(stage="serving", **out[-1])
         eng.close()
-    top = out[-1]
+    return {"rates": out, "requests": len(reqs), "evens": evens, "budget_gib": serve.BUDGET / 2**30,
+            "plain_service_ms_mean": 1e3 * sum(service) / len(service), "bitmap_prepare_s": prep_s,
+            "workload": "serving: 100 requests (serve.draw_requests seed 5), Poisson arrivals at 10-1000 "
+                        "requests/s, start uniform in [1e12, 5e18], length log-uniform in [1e4, 1e9] integers, "
+                        "64 GiB device-memory admission budget"}
+
+
+def bench_slice(args, lib):
+    import gbamd
+
+    start, limit = SLICE_START, SLICE_START + 10**12 - 2
+    evens = (limit - start) // 2 + 1
+    with tempfile.TemporaryDirectory(dir=tmpfs()) as td, gbamd.env_scope(**stats_files(td)):
+        eng = gbamd.Engine(limit, SEGMENT)
+        prep_s = eng.prepare(limit, start)
+        runs, wall, _ = timed_runs(eng, td, lib, limit, start, args.steps, args.warmup)
+        eng.close()
+    k = len(runs)
+    kern = kernel_totals(runs)
+    roof = hbm_roofline(kern)
     print(json.dumps({
-        "metric": "served p50/p99 latency (ms) and throughput", "value": top["throughput_gevens_s"],
-        "unit": "G evens/s at the highest rate", "n_gpus": 1, "steps": len(rates), "warmup": 1,
-        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
-        "data": "synthetic: 100 seeded requests (goldbach-l0_amd/serve.py draw_requests, seed 5)",
-        "config": {"workload": "serving: Poisson arrivals at 10-1000 requests/s, start uniform in [1e12, 5e18], "
-                               "length log-uniform in [1e4, 1e9] integers, 64 GB admission budget",
-                   "requests": len(reqs), "evens": evens, "rates": out,
-                   "plain_service_ms_mean": 1e3 * sum(service) / len(service), "bitmap_prepare_s": prep_s}}),
-          flush=True)
+        "metric": "G evens/s at 4.5e18 w/ exact stats, 1e12-integer slice", "value": evens * k / wall / 1e9,
+        "unit": "G evens/s", "n_gpus": 1, "steps": k, "warmup": args.warmup, "ms_per_step": 1e3 * wall / k,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic: the integer range itself (no dataset)",
+        "config": {"workload": "throughput slice: 1e12 consecutive integers from 4.5e18, 2140 segments (2139 full, "
+                               "a 17.5M-integer tail), exact statistics and gaps",
+                   "start": start, "limit": limit, "segment": SEGMENT, "segments": runs[-1]["segments"],
+                   "gpu_loop_evens_per_s": per_step(runs, "evens_per_second"), "stage_ms": stage_ms(runs),
+                   "kernel_ms": {n: 1e3 * v[0] / k for n, v in kern.items()},
+                   "kernel_launches": {n: v[1] / k for n, v in kern.items()},
+                   "host_plan_ms": 1e3 * per_step(runs, "host_plan_seconds"), "bitmap_prepare_s": prep_s},
+        "roofline": {**roof, "binding": binding(COUNTERS_SLICE, kern, k)}}), flush=True)
+
+
+def line_of(metric, value, unit, sub, steps, warmup, ms, data):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+            "data": data, "config": sub}
 
 
 def main():
@@ -320,6 +498,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", choices=("segment", "slice", "batched", "serving"), default="segment")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--headline-only", action="store_true", help="skip the full-segment, batched and serving parts")
     ap.add_argument("--copy-calibration", action="store_true", help="only the counter-calibration copy")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -332,65 +511,31 @@ def main():
     steps_default, warm_default = {"segment": (20, 2), "slice": (2, 1), "batched": (3, 1), "serving": (1, 1)}[args.workload]
     args.steps = steps_default if args.steps is None else args.steps
     args.warmup = warm_default if args.warmup is None else args.warmup
-    if args.workload != "segment":
-        {"slice": bench_slice, "batched": bench_batched, "serving": bench_serving}[args.workload](args)
-        return
-    import gbamd
-
     lib = hip()
-    with tempfile.TemporaryDirectory(dir=tmpfs()) as td, gbamd.env_scope(**stats_files(td)):
-        eng = gbamd.Engine(LIMIT, SEGMENT)
-        prep_s = eng.prepare(LIMIT, START)
-
-        def step():
-            fresh(td)
-            return eng.run(LIMIT, SEGMENT, START)
-
-        for _ in range(args.warmup):
-            step()
-        device_sync(lib)
-        runs = []
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            runs.append(step())          # synchronous: every segment collected and committed before it returns
-        device_sync(lib)
-        wall = time.perf_counter() - t0
-        init_s = eng.init_seconds
-        eng.close()
-
-    k = len(runs)
-    per_step = lambda key: sum(r[key] for r in runs) / k
-    stage = {"large": per_step("large_seconds"), "sieve": per_step("sieve_seconds"),
-             "check": per_step("verify_seconds") - per_step("count_seconds"), "count": per_step("count_seconds")}
-    kern, roof = harvest_roofline(runs)
-    dom = roof["kernel"]
-    out = {
-        "metric": "G evens/s at 4e18 w/ exact stats",
-        "value": EVENS * k / wall / 1e9,
-        "unit": "G evens/s",
-        "n_gpus": 1,
-        "steps": k,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / k,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u64",
-        "data": "synthetic: the integer range itself (no dataset)",
-        "config": {"workload": "one production segment range at 4e18: evens in [4e18, 4e18 + 467,499,998], "
-                               "233,750,000 evens, 2 half segments (the reference's at-least-two-segments rule), "
-                               "exact statistics and gaps",
-                   "start": START, "limit": LIMIT, "segment": SEGMENT, "segments": runs[-1]["segments"],
-                   "stage_ms": {n: 1e3 * v for n, v in stage.items()},
-                   "kernel_ms": {n: 1e3 * v[0] / k for n, v in kern.items()},
-                   "gpu_loop_ms": 1e3 * per_step("gpu_seconds"),
-                   "host_plan_ms": 1e3 * per_step("host_plan_seconds"),
-                   "engine_init_s": init_s, "bitmap_prepare_s": prep_s},
-        "roofline": {**{k: roof[k] for k in ("bound", "achieved", "peak", "unit", "frac")},
-                     "traffic": traffic() if dom == "k_harvest" else None,
-                     **{k: roof[k] for k in ("kernel", "bytes_per_launch", "launch_ms")}},
-        "cpu_baseline": None if args.no_cpu else cpu_baseline(),
-    }
+    if args.workload == "slice":
+        bench_slice(args, lib)
+        return
+    if args.workload == "batched":
+        b = measure_batched(lib, args.steps, max(args.warmup, 1))
+        print(json.dumps(line_of("batched vs plain per-range runs, G evens/s", b["value"], "G evens/s", b,
+                                 b["steps"], b["warmup"], b["batched_ms"],
+                                 "synthetic: 256 seeded ranges (tests/golden/cfg4_ranges.txt)")), flush=True)
+        return
+    if args.workload == "serving":
+        s = measure_serving()
+        top = s["rates"][-1]
+        print(json.dumps(line_of("served p50/p99 latency (ms) and throughput", top["throughput_gevens_s"],
+                                 "G evens/s at the highest rate", s, len(s["rates"]), 1, None,
+                                 "synthetic: 100 seeded requests (goldbach-l0_amd/serve.py draw_requests, seed 5)")),
+              flush=True)
+        return
+    out, gpu_line, gpu_state = measure_headline(args, lib)
+    say(stage="headline", value=out["value"])
+    if not args.headline_only:
+        out["full_segments_4e18"] = measure_full(lib)
+        out["batched"] = measure_batched(lib)
+        out["serving"] = measure_serving()
+    out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(gpu_line, gpu_state)
     print(json.dumps(out), flush=True)
 
 

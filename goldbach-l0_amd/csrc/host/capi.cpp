@@ -175,17 +175,17 @@ int gba_segment_map(uint64_t seg_start, uint64_t limit, uint64_t span, uint64_t*
 struct gba_batcher {
     gb::Batcher b;
     std::vector<size_t> finished;   // completed ranges not yet reported
-    gba_batcher(gba_engine* e, uint64_t max_limit, uint64_t min_start, uint64_t span)
-        : b(e->e, max_limit, min_start, span)
+    gba_batcher(gba_engine* e, uint64_t max_limit, uint64_t min_start, uint64_t span, uint64_t budget)
+        : b(e->e, max_limit, min_start, span, budget)
     {
     }
 };
 
-int gba_batcher_open(gba_engine* e, uint64_t max_limit, uint64_t min_start, uint64_t segment_even, gba_batcher** out,
-                     uint32_t* batch_segments, char* err, size_t err_cap)
+int gba_batcher_open(gba_engine* e, uint64_t max_limit, uint64_t min_start, uint64_t segment_even,
+                     uint64_t device_budget, gba_batcher** out, uint32_t* batch_segments, char* err, size_t err_cap)
 {
     try {
-        *out = new gba_batcher(e, max_limit, min_start, segment_even);
+        *out = new gba_batcher(e, max_limit, min_start, segment_even, device_budget);
         if (batch_segments) *batch_segments = (*out)->b.batch_segments();
         put(err, err_cap, "");
         return 0;
@@ -248,6 +248,26 @@ int gba_batcher_totals(gba_batcher* b, gba_run_result* out)
 {
     fill(b->b.totals(), out);
     return 0;
+}
+
+int gba_batcher_memory(gba_batcher* b, uint64_t* device_bytes, uint32_t* inflight_segments)
+{
+    if (device_bytes) *device_bytes = b->b.device_bytes();
+    if (inflight_segments) *inflight_segments = 2 * b->b.batch_segments();
+    return 0;
+}
+
+int gba_batcher_segments(gba_batcher* b, uint64_t start_even, uint64_t limit, uint64_t* segments, char* err,
+                         size_t err_cap)
+{
+    try {
+        *segments = b->b.segments_of(start_even, limit);
+        put(err, err_cap, "");
+        return 0;
+    } catch (const std::exception& ex) {
+        put(err, err_cap, ex.what());
+        return 1;
+    }
 }
 
 void gba_batcher_close(gba_batcher* b) { delete b; }

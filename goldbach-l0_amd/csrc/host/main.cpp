@@ -13,6 +13,7 @@
 #include "config.hpp"
 #include "env.hpp"
 #include "textio.hpp"
+#include "cli_api.hpp"
 #include "verifier.hpp"
 
 namespace {
@@ -64,7 +65,6 @@ void usage()
         (unsigned long long)gb::DEFAULT_LIMIT, (unsigned long long)gb::DEFAULT_SEGMENT_EVEN);
 }
 
-const char* sched_name() { return "driver"; }
 
 void print_csv(const gb::RunStats& s, uint64_t segment, uint32_t split)
 {
@@ -82,7 +82,7 @@ void print_csv(const gb::RunStats& s, uint64_t segment, uint32_t split)
         (unsigned long long)s.ridge_found, (unsigned long long)s.ridge_dropped,
         (unsigned long long)(s.ridge_top.empty() ? 0 : s.ridge_top.front().p), s.stats_on ? 1 : 0,
         (unsigned long long)s.stats_tail, (unsigned long long)s.stats_sampled, (unsigned long long)s.stats_pmax,
-        (unsigned long long)s.stats_primes, (unsigned long long)s.stats_twins, s.split_used, sched_name());
+        (unsigned long long)s.stats_primes, (unsigned long long)s.stats_twins, s.split_used, gb::sched_policy_name());
 }
 
 void print_json(const gb::RunStats& s, uint64_t segment, uint32_t split)
@@ -92,7 +92,7 @@ void print_json(const gb::RunStats& s, uint64_t segment, uint32_t split)
                 (unsigned long long)segment, split);
     if (s.split_used) std::printf("  \"split_used\": %u,\n", s.split_used);
     std::printf("  \"sched\": \"%s\",\n  \"spv\": \"\",\n  \"segments\": %llu,\n  \"leftovers\": %llu,\n"
-                "  \"result\": \"PASS\",\n", sched_name(), (unsigned long long)s.segments,
+                "  \"result\": \"PASS\",\n", gb::sched_policy_name(), (unsigned long long)s.segments,
                 (unsigned long long)s.leftovers);
     std::printf("  \"init_seconds\": %.4f,\n  \"gpu_seconds\": %.4f,\n  \"total_seconds\": %.4f,\n"
                 "  \"sieve_seconds\": %.4f,\n  \"large_seconds\": %.4f,\n  \"verify_seconds\": %.4f,\n"
@@ -250,7 +250,7 @@ int main(int argc, char** argv)
                         gb::VERSION, (unsigned long long)s.limit, (unsigned long long)s.start,
                         (unsigned long long)s.segments, (unsigned long long)s.leftovers, s.init_seconds,
                         s.gpu_seconds, s.sieve_seconds, s.large_seconds, s.verify_seconds, s.offset_seconds,
-                        s.sync_seconds, s.total_seconds, s.evens_per_second, sched_name());
+                        s.sync_seconds, s.total_seconds, s.evens_per_second, gb::sched_policy_name());
         if (s.stats_on && !csv && !json)
             std::printf("\nGoldbach statistics (exact; log %s, state %s)\n  largest p(n)            : %llu\n"
                         "  evens with p(n) >= tail : %llu\n  evens with full p(n)    : %llu (sampled segments)\n"

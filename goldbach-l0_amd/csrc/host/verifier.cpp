@@ -205,6 +205,52 @@ struct ChunkPlan {
     bool direct = false;
 };
 
+// The device buffers of the batches in flight: two sets, so a batch's sieve
+// and harvest run while the batch before it is checked and counted. A plain
+// run uses its engine's set; a Batcher owns a set of its own, sized within
+// its device budget (0: no budget).
+struct Slots {
+    struct Slot {
+        Buf map, i0, coff, coffa, pending, scratch, hits, hit_n, chunks, rdesc, groups, region, fills;
+        Buf ridge_more;                     // ridge records beyond the output block's GBH_RIDGE_CAP
+        Buf hq;                             // the persistent harvest's chunk counter
+        Buf tab;                            // the batch tables (tab_layout)
+        Buf vmap, vhits, vhit_n, vchunks;   // the GB_BUCKET_VERIFY rebuild
+    } sl[2];
+    Buf h_vchunks;
+    Buf d_out[2], h_out[2], h_tab[2], h_chunks[2], h_rdesc[2];
+    Event ev[2][EV_N];
+    uint64_t slot_plan[2] = {~0ull, ~0ull};   // the chunk plan each slot's device tables hold
+    uint64_t budget = 0;                      // device bytes the set may hold (0: unbounded)
+
+    void init(gbh_ctx* c, uint32_t kbatch)
+    {
+        for (int s = 0; s < 2; ++s) {
+            d_out[s].ensure(c, sizeof(gbh_segout) * kbatch, 0);
+            h_out[s].ensure(c, sizeof(gbh_segout) * kbatch, 1);
+            h_tab[s].ensure(c, tab_layout(kbatch).bytes, 1);
+            for (auto& e : ev[s]) e.create(c);
+        }
+        for (auto& B : sl) {
+            B.tab.ensure(c, tab_layout(kbatch).bytes, 0);
+            B.coff.ensure(c, size_t(kbatch) * 15 * GBH_NCAND * 4, 0);
+            B.coffa.ensure(c, size_t(kbatch) * 15 * GBH_NCAND_PAD * 4, 0);
+        }
+    }
+    /// Device bytes held now.
+    uint64_t device_bytes() const
+    {
+        uint64_t n = 0;
+        for (const Slot& B : sl)
+            for (const Buf* b : {&B.map, &B.i0, &B.coff, &B.coffa, &B.pending, &B.scratch, &B.hits, &B.hit_n,
+                                 &B.chunks, &B.rdesc, &B.groups, &B.region, &B.fills, &B.ridge_more, &B.hq, &B.tab,
+                                 &B.vmap, &B.vhits, &B.vhit_n, &B.vchunks})
+                n += b->kind == 0 ? b->bytes : 0;
+        for (const Buf& b : d_out) n += b.bytes;
+        return n;
+    }
+};
+
 }  // namespace
 
 // ---------------------------------------------------------------- Engine state
@@ -231,21 +277,10 @@ struct Engine::State {
     Buf d_win;                           // window primes, then the large primes (one table)
     uint32_t* d_large = nullptr;         // the large primes inside d_win
     uint64_t win_p_long = 0;             // long runs' window bound (GB_WIN_P_LONG)
-    // the device buffers of one batch in flight: two sets, so a batch's
-    // sieve and harvest run while the batch before it is checked and counted
-    struct Slot {
-        Buf map, i0, coff, coffa, pending, scratch, hits, hit_n, chunks, rdesc, groups, region, fills;
-        Buf ridge_more;                     // ridge records beyond the output block's GBH_RIDGE_CAP
-        Buf hq;                             // the persistent harvest's chunk counter
-        Buf tab;                            // the batch tables (tab_layout)
-        Buf vmap, vhits, vhit_n, vchunks;   // the GB_BUCKET_VERIFY rebuild
-    } sl[2];
-    Buf h_vchunks;
-    Buf d_out[2], h_out[2], h_tab[2], h_chunks[2], h_rdesc[2];
-    Event ev[2][EV_N];
+    Slots own;                           // the batch buffers of plain runs
     Candidates cand;
     ChunkPlan cplan;                     // the last harvest chunk plan made (host side)
-    uint64_t cplan_version = 0, slot_plan[2] = {~0ull, ~0ull};   // the plan each slot's device tables hold
+    uint64_t cplan_version = 0;          // bumped by every new chunk plan
     std::string tuned;                   // knob defaults taken from GB_TUNE_FILE
 
     State(uint64_t max_limit_, uint64_t max_span_, const char* prime_cache)
@@ -298,17 +333,7 @@ struct Engine::State {
         d_large = d_win.as<uint32_t>() + nwin;
         if (nwin + nlarge)
             check(gbh_copy_h2d(c, 0, d_win.p, base.data() + win_lo, size_t(nwin + nlarge) * 4), "upload primes");
-        for (int s = 0; s < 2; ++s) {
-            d_out[s].ensure(c, sizeof(gbh_segout) * kbatch, 0);
-            h_out[s].ensure(c, sizeof(gbh_segout) * kbatch, 1);
-            h_tab[s].ensure(c, tab_layout(kbatch).bytes, 1);
-            for (auto& e : ev[s]) e.create(c);
-        }
-        for (auto& B : sl) {
-            B.tab.ensure(c, tab_layout(kbatch).bytes, 0);
-            B.coff.ensure(c, size_t(kbatch) * 15 * GBH_NCAND * 4, 0);
-            B.coffa.ensure(c, size_t(kbatch) * 15 * GBH_NCAND_PAD * 4, 0);
-        }
+        own.init(c, kbatch);
         check(gbh_stream_sync(c, 0), "upload sync");
         upload_s = since(tu);
         init_s = since(t0);
@@ -472,7 +497,7 @@ PlanBounds bounds_of(const Range& r)
 // and the exact statistics of every range.
 class Pipeline {
 public:
-    Pipeline(Engine::State& e, bool multi) : E(e), multi_(multi)
+    Pipeline(Engine::State& e, bool multi, Slots* slots = nullptr) : E(e), S(slots ? *slots : e.own), multi_(multi)
     {
         phase_trace_ = env::present("GB_PHASE");
         if (kend_ && kend_ <= E.ka) throw std::runtime_error("GB_PHASEB_SLOTS must lie above GB_KA (0: every slot)");
@@ -491,6 +516,7 @@ public:
     uint32_t map_of(Range& r, uint64_t seg_start, std::vector<uint64_t>& words, uint64_t Q[8], uint32_t nc[8]);
     uint32_t kb() const { return kb_; }
     const RunStats& totals() const { return agg_; }
+    const Slots& slots() const { return S; }
 
 private:
     double hit_factor(uint32_t j, double q_lo) const;
@@ -505,6 +531,7 @@ private:
     void finish_range(Range& r);
 
     Engine::State& E;
+    Slots& S;                          // the batch buffers this pipeline's batches use
     bool multi_;
     bool phase_trace_ = false;
     std::vector<Range*> active_;       // ranges with segments not yet issued, in submission order
@@ -563,6 +590,7 @@ private:
     bool overlap_ = env::onoff("GB_OVERLAP", true);
     int main_stream(int slot) const { return overlap_ ? slot : 0; }
     uint64_t region_entries_ = 0;
+    uint64_t planned_bytes_ = 0;       // the batch buffers' device bytes planned by plan()
     uint32_t ngroups_ = 0, part_cap_ = GBH_PART_CAP;   // the partition: groups and their LDS entries
     // GB_BUCKET_VERIFY=N: every Nth segment's map is rebuilt with every large
     // prime on the wheel walk (no multiplier bitmap) and compared word for word
@@ -751,21 +779,49 @@ void Pipeline::plan(const PlanBounds& pb)
     // a batch's q range stays below 2^33 (the harvest's 32-bit half-offsets):
     // at most kb_ - 1 spans plus the merged last segment (below 2 spans)
     const uint64_t kmax = std::max<uint64_t>(1, ((uint64_t(1) << 33) - 4 * MAX_GB_PRIME) / span_ - 1);
+    const double lam = wbits_ * density_max_;
+    const uint64_t evens_max = span_ + 2;
+    swords_ = uint32_t(2 * (nc_max / 64 + 4));
+    // list and pending capacities of batches of K segments
+    auto caps = [&](uint64_t K) {
+        return std::pair<uint32_t, uint32_t>{
+            uint32_t(std::min(lam * 1.15 + 8 * std::sqrt(lam) + 1024, 4e9 / (double(K) * 8 * nwin_max_))),
+            uint32_t(std::min<uint64_t>(evens_max / 32 + 65536, UINT32_MAX / 2 / K))};
+    };
+    // device bytes of the two slots' batch buffers for batches of K segments
+    // (the ensure calls below)
+    auto slot_bytes = [&](uint64_t K) {
+        const auto [cap, pcap] = caps(K);
+        auto b = [](uint64_t n) { return std::max<uint64_t>(n, 256); };
+        const uint64_t one = b(K * 8 * cw_ * 8) + b(K * 8 * uint64_t(std::max(nwin_ + nx_, 1u)) * 4) +
+                             b(K * uint64_t(pcap) * 4) + b(K * uint64_t(swords_) * 4) +
+                             b(K * uint64_t(GBH_RIDGE_MORE) * 16) + 256 + b(K * 8 * uint64_t(nwin_max_) * 4) +
+                             (nlarge_ ? b(K * 8 * uint64_t(nwin_max_) * cap * 4) : 0) + b(tab_layout(K).bytes) +
+                             b(K * 15 * GBH_NCAND * 4) + b(K * 15 * GBH_NCAND_PAD * 4) + b(sizeof(gbh_segout) * K);
+        return 2 * one;
+    };
     if (multi_) {
         kb_ = GBH_MAX_BATCH;
-        evens_cap_ = uint64_t(E.kbatch) * (span_ / 2);
+        if (S.budget) {
+            // the batch buffers within 7/8 of the budget; the rest holds the
+            // chunk tables and the direct harvest's regions (else counted)
+            const uint64_t room = S.budget / 8 * 7;
+            while (kb_ > 1 && slot_bytes(kb_) > room) --kb_;
+            if (slot_bytes(kb_) > room)
+                throw std::runtime_error("device budget of " + std::to_string(S.budget) + " bytes below the buffers "
+                                         "of one-segment batches (" + std::to_string(slot_bytes(1)) + " bytes, 7/8 "
+                                         "of the budget at most)");
+        }
+        evens_cap_ = uint64_t(std::min(E.kbatch, kb_)) * (span_ / 2);
     } else {
         kb_ = uint32_t(std::min<uint64_t>({nseg, kmax, GBH_HL_BUCKETS / (8 * nwin_max_)}));
     }
     if (!kb_ || GBH_HL_BUCKETS / (8 * nwin_max_) == 0) throw std::runtime_error("segment span too large for the hit-list work-group");
-    const double lam = wbits_ * density_max_;
-    cap_ = uint32_t(std::min(lam * 1.15 + 8 * std::sqrt(lam) + 1024, 4e9 / (double(kb_) * 8 * nwin_max_)));
-    const uint64_t evens_max = span_ + 2;
-    pcap_ = uint32_t(std::min<uint64_t>(evens_max / 32 + 65536, UINT32_MAX / 2 / kb_));
-    swords_ = uint32_t(2 * (nc_max / 64 + 4));
+    std::tie(cap_, pcap_) = caps(kb_);
+    planned_bytes_ = slot_bytes(kb_);
     gbh_ctx* c = E.c;
     const size_t K = kb_;
-    for (auto& B : E.sl) {
+    for (auto& B : S.sl) {
         B.map.ensure(c, K * 8 * cw_ * 8, 0);
         B.i0.ensure(c, K * 8 * size_t(std::max(nwin_ + nx_, 1u)) * 4, 0);
         B.pending.ensure(c, K * size_t(pcap_) * 4, 0);
@@ -779,9 +835,9 @@ void Pipeline::plan(const PlanBounds& pb)
         B.coffa.ensure(c, K * 15 * GBH_NCAND_PAD * 4, 0);
     }
     for (int s = 0; s < 2; ++s) {
-        E.d_out[s].ensure(c, sizeof(gbh_segout) * K, 0);
-        E.h_out[s].ensure(c, sizeof(gbh_segout) * K, 1);
-        E.h_tab[s].ensure(c, tab_layout(K).bytes, 1);
+        S.d_out[s].ensure(c, sizeof(gbh_segout) * K, 0);
+        S.h_out[s].ensure(c, sizeof(gbh_segout) * K, 1);
+        S.h_tab[s].ensure(c, tab_layout(K).bytes, 1);
     }
     if (!multi_ && nlarge_) {   // a plain run's batches share one chunk plan: make it now
         const auto tp0 = clock_::now();
@@ -802,7 +858,7 @@ void Pipeline::plan(const PlanBounds& pb)
         const double inv_large = E.inv_at(nlarge_);
         const double lam_v = wbits_ * kWheelFrac * inv_large;
         cap_v_ = uint32_t(lam_v * 1.15 + 8 * std::sqrt(lam_v) + 1024);
-        for (auto& B : E.sl) {
+        for (auto& B : S.sl) {
             B.vmap.ensure(c, size_t(8) * cw_ * 8, 0);
             B.vhits.ensure(c, size_t(8) * nwin_max_ * cap_v_ * 4, 0);
             B.vhit_n.ensure(c, size_t(8) * nwin_max_ * 4, 0);
@@ -824,11 +880,11 @@ void Pipeline::plan(const PlanBounds& pb)
             j = e;
         }
         nvchunks_ = uint32_t(vc.size() - 1);
-        E.h_vchunks.ensure(c, vc.size() * 4, 1);
-        std::memcpy(E.h_vchunks.p, vc.data(), vc.size() * 4);
-        for (auto& B : E.sl) {
+        S.h_vchunks.ensure(c, vc.size() * 4, 1);
+        std::memcpy(S.h_vchunks.p, vc.data(), vc.size() * 4);
+        for (auto& B : S.sl) {
             B.vchunks.ensure(c, vc.size() * 4, 0);
-            check(gbh_copy_h2d(c, 0, B.vchunks.p, E.h_vchunks.p, vc.size() * 4), "upload verify chunks");
+            check(gbh_copy_h2d(c, 0, B.vchunks.p, S.h_vchunks.p, vc.size() * 4), "upload verify chunks");
         }
     }
     check(gbh_stream_sync(c, 0), "plan sync");
@@ -1019,6 +1075,25 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
             gsum += r;
         }
         groups.push_back(nchunks_);
+        if (direct_ && S.budget) {
+            // a budgeted slot set: the regions (each slot's grown to the
+            // larger of its present and its new size) must fit beside the
+            // buffers held, else the counted form (no regions)
+            uint64_t held = S.device_bytes(), after = held;
+            for (const auto& B : S.sl) {
+                const std::pair<const Buf*, uint64_t> grow[] = {
+                    {&B.region, base * 4}, {&B.fills, uint64_t(nchunks_) * nbb * 4}, {&B.rdesc, rd.size() * 4},
+                    {&B.groups, groups.size() * 4}, {&B.chunks, chunks_.size() * 4}};
+                for (const auto& [buf, n] : grow) after += std::max<uint64_t>(n, 256) > buf->bytes ?
+                                                               std::max<uint64_t>(n, 256) - buf->bytes : 0;
+            }
+            if (after > S.budget) {
+                if (phase_trace_)
+                    std::fprintf(stderr, "plan: direct harvest off (regions of %llu bytes beyond the device budget)\n",
+                                 (unsigned long long)(after - held));
+                direct_ = false;
+            }
+        }
         if (direct_) {
             region_entries_ = base;
             ngroups_ = uint32_t(groups.size() - 1);
@@ -1060,13 +1135,13 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
 // stream, after the slot's batch before, which the host has collected).
 void Pipeline::upload_chunks(int slot)
 {
-    if (E.slot_plan[slot] == E.cplan_version) return;
+    if (S.slot_plan[slot] == E.cplan_version) return;
     const ChunkPlan& cp = E.cplan;
     gbh_ctx* c = E.c;
-    auto& B = E.sl[slot];
+    auto& B = S.sl[slot];
     const size_t nc = cp.chunks.size(), nr = cp.rdesc.size(), ng = cp.groups.size();
-    E.h_chunks[slot].ensure(c, (nc + nr + ng) * 4, 1);
-    uint32_t* h = E.h_chunks[slot].as<uint32_t>();
+    S.h_chunks[slot].ensure(c, (nc + nr + ng) * 4, 1);
+    uint32_t* h = S.h_chunks[slot].as<uint32_t>();
     std::memcpy(h, cp.chunks.data(), nc * 4);
     B.chunks.ensure(c, nc * 4, 0);
     const int ms = main_stream(slot);
@@ -1081,7 +1156,7 @@ void Pipeline::upload_chunks(int slot)
         check(gbh_copy_h2d(c, ms, B.rdesc.p, h + nc, nr * 4), "upload regions");
         check(gbh_copy_h2d(c, ms, B.groups.p, h + nc + nr, ng * 4), "upload region groups");
     }
-    E.slot_plan[slot] = E.cplan_version;
+    S.slot_plan[slot] = E.cplan_version;
 }
 
 // Segments of one range a batch may hold: the run's q range below 2^33 and
@@ -1153,13 +1228,13 @@ bool Pipeline::step(std::vector<Range*>* done)
 void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_count)
 {
     gbh_ctx* c = E.c;
-    auto& B = E.sl[slot];
+    auto& B = S.sl[slot];
     // streams: the slot's main stream (uploads, harvest, hit lists, checker,
     // counter) and its side stream (set-up and the window sieve phase)
     const int ms = main_stream(slot);
     const uint32_t nseg = uint32_t(batch.size());
     const TabLayout tl = tab_layout(nseg);
-    uint8_t* ht = E.h_tab[slot].as<uint8_t>();
+    uint8_t* ht = S.h_tab[slot].as<uint8_t>();
     gbh_geom* hg = reinterpret_cast<gbh_geom*>(ht);
     gbh_hl_run* hr = reinterpret_cast<gbh_hl_run*>(ht + tl.runs);
     gbh_countseg* hc = reinterpret_cast<gbh_countseg*>(ht + tl.cs);
@@ -1215,8 +1290,8 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         run_max = std::max(run_max, hr[nruns - 1].nseg);
     }
     gbh_batch b{dg, nseg, cw_, wbits, nwin, B.map.as<uint64_t>()};
-    gbh_segout* out = E.d_out[slot].as<gbh_segout>();
-    auto& ev = E.ev[slot];
+    gbh_segout* out = S.d_out[slot].as<gbh_segout>();
+    auto& ev = S.ev[slot];
     check(gbh_event_record(c, ms, ev[EV_BEGIN].e), "event");
     harvested_[slot] = nlarge_ != 0;
     std::vector<RunCost> costs;
@@ -1368,7 +1443,7 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     } else {
         check(gbh_event_record(c, ms, ev[EV_COUNT].e), "event");
     }
-    check(gbh_copy_d2h(c, ms, E.h_out[slot].p, out, sizeof(gbh_segout) * nseg), "copy outputs");
+    check(gbh_copy_d2h(c, ms, S.h_out[slot].p, out, sizeof(gbh_segout) * nseg), "copy outputs");
     check(gbh_event_record(c, ms, ev[EV_END].e), "event");
     if (phase_trace_) std::fprintf(stderr, "host: batch queued %.3f ms after its first segment was formed\n",
                                    since(batch[0].r->t_first) * 1e3);
@@ -1381,7 +1456,7 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
 void Pipeline::enqueue_verify(uint32_t s, const gbh_batch& b, gbh_segout* out, int slot)
 {
     gbh_ctx* c = E.c;
-    auto& B = E.sl[slot];
+    auto& B = S.sl[slot];
     const int ms = main_stream(slot);
     const gbh_batch vb{b.geoms + s, 1, cw_, b.wbits, b.nwin, B.vmap.as<uint64_t>()};
     uint32_t ncoop = 0;
@@ -1407,12 +1482,12 @@ void Pipeline::enqueue_verify(uint32_t s, const gbh_batch& b, gbh_segout* out, i
 void Pipeline::collect(std::vector<SegJob>& batch, int slot)
 {
     gbh_ctx* c = E.c;
-    auto& ev = E.ev[slot];
+    auto& ev = S.ev[slot];
     const auto tw = clock_::now();
     check(gbh_event_sync(c, ev[EV_END].e), "batch sync");
     const auto tcol = clock_::now();
     if (phase_trace_) std::fprintf(stderr, "host: waited %.3f ms for the batch\n", since(tw) * 1e3);
-    const gbh_segout* outs = E.h_out[slot].as<gbh_segout>();
+    const gbh_segout* outs = S.h_out[slot].as<gbh_segout>();
     if ((outs[0].err & 1u) && direct_used_[slot]) {
         // a region of the direct harvest overflowed: the batch again, with
         // counted lists (queued behind the batch in flight, which keeps its
@@ -1504,7 +1579,7 @@ void Pipeline::collect_segment(const SegJob& j, const gbh_segout& o, const doubl
         if (kept > GBH_RIDGE_CAP) {
             std::vector<uint64_t> all(o.ridge, o.ridge + 2 * GBH_RIDGE_CAP);
             all.resize(2 * size_t(kept));
-            const uint64_t* src = E.sl[slot].ridge_more.as<uint64_t>() + 2 * size_t(s) * GBH_RIDGE_MORE;
+            const uint64_t* src = S.sl[slot].ridge_more.as<uint64_t>() + 2 * size_t(s) * GBH_RIDGE_MORE;
             const int ms = main_stream(slot);
             check(gbh_copy_d2h(E.c, ms, all.data() + 2 * GBH_RIDGE_CAP, src, size_t(kept - GBH_RIDGE_CAP) * 16),
                   "copy ridge records");
@@ -1621,10 +1696,10 @@ uint32_t Pipeline::map_of(Range& r, uint64_t seg_start, std::vector<uint64_t>& w
     bt[0].r = &r;
     bt[0].sg = segment_geometry(seg_start, r.limit, r.span);
     enqueue(bt, 0, false);
-    check(gbh_event_sync(E.c, E.ev[0][EV_END].e), "sync");
-    if (E.h_out[0].as<gbh_segout>()[0].err & 13u) throw std::runtime_error("hit-list device error");
+    check(gbh_event_sync(E.c, S.ev[0][EV_END].e), "sync");
+    if (S.h_out[0].as<gbh_segout>()[0].err & 13u) throw std::runtime_error("hit-list device error");
     words.assign(size_t(8) * cw_, 0);
-    check(gbh_copy_d2h(E.c, 0, words.data(), E.sl[0].map.p, words.size() * 8), "copy map");
+    check(gbh_copy_d2h(E.c, 0, words.data(), S.sl[0].map.p, words.size() * 8), "copy map");
     check(gbh_stream_sync(E.c, 0), "sync");
     for (int c = 0; c < 8; ++c) { Q[c] = bt[0].cg.Q[c]; nc[c] = uint32_t(bt[0].cg.n[c]); }
     return cw_;
@@ -1668,13 +1743,16 @@ RunStats run_plain(Engine::State& E, uint64_t limit, uint64_t segment_even, bool
 
 struct Batcher::Impl {
     Engine::State& E;
+    Slots slots;                       // this batcher's batch buffers (within its device budget)
     Pipeline P;
     uint64_t max_limit, span, min_start;
     std::vector<std::unique_ptr<Range>> ranges;
     std::vector<Result> results;
-    Impl(Engine::State& e, uint64_t max_limit_, uint64_t min_start_, uint64_t segment_even)
-        : E(e), P(e, true), max_limit(max_limit_ & ~uint64_t(1)), span(segment_even + (segment_even & 1))
+    Impl(Engine::State& e, uint64_t max_limit_, uint64_t min_start_, uint64_t segment_even, uint64_t budget)
+        : E(e), P(e, true, &slots), max_limit(max_limit_ & ~uint64_t(1)), span(segment_even + (segment_even & 1))
     {
+        slots.init(E.c, 1);
+        slots.budget = budget;
         if (max_limit > E.max_limit) throw std::runtime_error("batcher limit above the engine's prime bound");
         if (span < 4 || span > E.max_span) throw std::runtime_error("batcher segment span outside the engine's buffers");
         min_start = std::max<uint64_t>(min_start_ + (min_start_ & 1), 4);
@@ -1700,10 +1778,30 @@ struct Batcher::Impl {
         }
         P.plan(pb);
     }
+    ~Impl()
+    {
+        // nothing of this batcher's may still run when its buffers go
+        for (int s = 0; s < 4; ++s) (void)gbh_stream_sync(E.c, s);
+    }
+    // the segments [start_even, limit] splits into (as a plain run splits it)
+    uint64_t segments_of(uint64_t start_even, uint64_t limit) const
+    {
+        Range r;
+        r.limit = limit;
+        r.start = start_even;
+        r.span = span;
+        resolve_range(E, r, 0);
+        uint64_t n = 0;
+        for (uint64_t s = r.start;; s += r.span) {
+            ++n;
+            if (segment_geometry(s, r.limit, r.span).seg_end == r.limit) break;
+        }
+        return n;
+    }
 };
 
-Batcher::Batcher(Engine& e, uint64_t max_limit, uint64_t min_start, uint64_t segment_even)
-    : p_(std::make_unique<Impl>(*e.state(), max_limit, min_start, segment_even))
+Batcher::Batcher(Engine& e, uint64_t max_limit, uint64_t min_start, uint64_t segment_even, uint64_t device_budget)
+    : p_(std::make_unique<Impl>(*e.state(), max_limit, min_start, segment_even, device_budget))
 {
 }
 Batcher::~Batcher() = default;
@@ -1755,6 +1853,8 @@ const Batcher::Result& Batcher::result(size_t id) const
 
 size_t Batcher::size() const { return p_->ranges.size(); }
 uint32_t Batcher::batch_segments() const { return p_->P.kb(); }
+uint64_t Batcher::device_bytes() const { return p_->slots.device_bytes(); }
+uint64_t Batcher::segments_of(uint64_t start_even, uint64_t limit) const { return p_->segments_of(start_even, limit); }
 const RunStats& Batcher::totals() const { return p_->P.totals(); }
 
 Engine::Engine(uint64_t max_limit, uint64_t max_span, const char* prime_cache)

@@ -112,9 +112,16 @@ public:
         bool done = false;
     };
     /// Plans the pipeline (large-prime split, multiplier bitmap, buffers)
-    /// for ranges with start >= @p min_start and limit <= @p max_limit.
-    /// @throws std::runtime_error when the bounds exceed the engine's.
-    Batcher(Engine& e, uint64_t max_limit, uint64_t min_start, uint64_t segment_even = 467'500'000ULL);
+    /// for ranges with start >= @p min_start and limit <= @p max_limit. The
+    /// batcher's batch buffers are its own, allocated here: with a
+    /// @p device_budget (bytes, 0: none) the segments per batch are the most
+    /// whose buffers fit in 7/8 of it, and the direct harvest's regions are
+    /// allocated only where they fit in the rest (else the counted form), so
+    /// device_bytes() never exceeds the budget.
+    /// @throws std::runtime_error when the bounds exceed the engine's, or
+    /// one-segment batches do not fit the budget.
+    Batcher(Engine& e, uint64_t max_limit, uint64_t min_start, uint64_t segment_even = 467'500'000ULL,
+            uint64_t device_budget = 0);
     ~Batcher();
     Batcher(const Batcher&) = delete;
     Batcher& operator=(const Batcher&) = delete;
@@ -129,8 +136,13 @@ public:
     bool step(std::vector<size_t>* finished = nullptr);
     const Result& result(size_t id) const;
     size_t size() const;
-    /// Segments a batch may hold.
+    /// Segments a batch may hold (two batches are in flight at once).
     uint32_t batch_segments() const;
+    /// Device bytes this batcher's batch buffers hold now.
+    uint64_t device_bytes() const;
+    /// Segments [start_even, limit] splits into (as a plain run splits it).
+    /// @throws std::runtime_error for a range outside the engine's bounds.
+    uint64_t segments_of(uint64_t start_even, uint64_t limit) const;
     /// Kernel times and algorithmic bytes over every batch so far.
     const RunStats& totals() const;
 

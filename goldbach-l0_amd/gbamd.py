@@ -88,7 +88,7 @@ def load() -> ctypes.CDLL:
                                            ctypes.POINTER(ctypes.c_double), ctypes.c_char_p, ctypes.c_size_t]
         lib.gba_engine_prepare.restype = ctypes.c_int
         vp, u64, u32, cp, sz = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_size_t
-        lib.gba_batcher_open.argtypes = [vp, u64, u64, u64, ctypes.POINTER(vp), ctypes.POINTER(u32), cp, sz]
+        lib.gba_batcher_open.argtypes = [vp, u64, u64, u64, u64, ctypes.POINTER(vp), ctypes.POINTER(u32), cp, sz]
         lib.gba_batcher_open.restype = ctypes.c_int
         lib.gba_batcher_submit.argtypes = [vp, u64, u64, ctypes.POINTER(u64), cp, sz]
         lib.gba_batcher_submit.restype = ctypes.c_int
@@ -99,6 +99,10 @@ def load() -> ctypes.CDLL:
         lib.gba_batcher_result.restype = ctypes.c_int
         lib.gba_batcher_totals.argtypes = [vp, ctypes.POINTER(GbaRunResult)]
         lib.gba_batcher_totals.restype = ctypes.c_int
+        lib.gba_batcher_memory.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u32)]
+        lib.gba_batcher_memory.restype = ctypes.c_int
+        lib.gba_batcher_segments.argtypes = [vp, u64, u64, ctypes.POINTER(u64), cp, sz]
+        lib.gba_batcher_segments.restype = ctypes.c_int
         lib.gba_batcher_close.argtypes = [vp]
         lib.gba_batcher_close.restype = None
         lib.gba_tune_load.argtypes = [cp, u64, cp, sz, cp, sz]
@@ -207,14 +211,15 @@ class Batcher:
     ranges it completed; result() gives a range's figures, GBSTATS line and
     the state text of a fresh campaign after it alone."""
 
-    def __init__(self, engine: Engine, max_limit: int, min_start: int, segment_even: int = 467_500_000):
+    def __init__(self, engine: Engine, max_limit: int, min_start: int, segment_even: int = 467_500_000,
+                 device_budget: int = 0):
         self._lib = load()
         self._engine = engine            # the engine must outlive the batcher
         self._h = ctypes.c_void_p()
         kb = ctypes.c_uint32(0)
         err = ctypes.create_string_buffer(4096)
         if self._lib.gba_batcher_open(engine._h, int(max_limit), int(min_start), int(segment_even),
-                                      ctypes.byref(self._h), ctypes.byref(kb), err, len(err)):
+                                      int(device_budget), ctypes.byref(self._h), ctypes.byref(kb), err, len(err)):
             raise VerificationError(err.value.decode())
         self.batch_segments = kb.value
         self._ids = (ctypes.c_uint64 * 1024)()
@@ -254,6 +259,20 @@ class Batcher:
         res = GbaRunResult()
         self._lib.gba_batcher_totals(self._h, ctypes.byref(res))
         return {name: getattr(res, name) for name, _ in GbaRunResult._fields_}
+
+    def memory(self) -> tuple:
+        """(device bytes the batch buffers hold now, segments in flight at once)."""
+        nbytes, inflight = ctypes.c_uint64(0), ctypes.c_uint32(0)
+        self._lib.gba_batcher_memory(self._h, ctypes.byref(nbytes), ctypes.byref(inflight))
+        return nbytes.value, inflight.value
+
+    def segments_of(self, start_even: int, limit: int) -> int:
+        """The segments [start_even, limit] splits into."""
+        n = ctypes.c_uint64(0)
+        err = ctypes.create_string_buffer(4096)
+        if self._lib.gba_batcher_segments(self._h, int(start_even), int(limit), ctypes.byref(n), err, len(err)):
+            raise VerificationError(err.value.decode())
+        return n.value
 
     def run_all(self) -> list:
         """Steps until every submitted range is done; returns the completion order."""

@@ -1,14 +1,24 @@
 """Served verification: a request loop over one resident engine.
 
 Requests (range verifications of any size) arrive over time; the loop admits
-them by device-memory footprint, feeds every admitted request's segments to
-one Batcher (continuous batching: a request that arrives while a batch runs
+them by device memory, feeds every admitted request's segments to one
+Batcher (continuous batching: a request that arrives while a batch runs
 joins the next batch, and each step issues a batch before it collects the
 previous one, so the host's collection overlaps the device's work), and
 records each request's arrival, admission and completion times. The base
 primes, mask tables and multiplier bitmap stay resident in the engine for the
 whole loop. Every response's statistics are those of the request run alone
 (the Batcher's contract).
+
+Admission is by the device memory that requests occupy: the batch buffers.
+The Batcher allocates them once, within the device budget (its segments per
+batch are the most whose buffers fit); two batches are in flight at once, so
+the buffers hold 2 x batch_segments segment slots, each slot costing the
+allocation's bytes over its slots. An admitted request reserves slots for its
+segments (all of them, or every slot for a request with more segments than
+slots) until it completes; a request is admitted only when its reservation
+fits beside those of the requests in flight (or nothing is in flight), so it
+waits otherwise.
 
 The reference has no serving mode (it verifies one range per process); the
 request shape follows the served workload: start uniform over the even
@@ -25,6 +35,7 @@ import numpy as np
 import gbamd
 
 SEGMENT = 467_500_000
+BUDGET = 64 << 30                       # device bytes the batch buffers may hold
 REQ_START_LO, REQ_START_HI = 10**12, 5 * 10**18
 REQ_LEN_LO, REQ_LEN_HI = 10**4, 10**9
 
@@ -49,41 +60,35 @@ def poisson_arrivals(n: int, rate: float, seed: int):
     return np.cumsum(rng.exponential(1.0 / rate, size=n)).tolist()
 
 
-def footprint_bytes(start: int, limit: int, span: int = SEGMENT) -> int:
-    """Device bytes a request's segments occupy while they are in a batch:
-    per segment its class map (8 class arrays of about (evens + 4891) / 15
-    bits), its hit lists (about 0.06 hits of 4 bytes per class bit at 4e18)
-    and its pending list (1/32 of its evens plus 65536 entries of 4 bytes)."""
-    evens = (limit - start) // 2 + 1
-    seg_evens = max(1, min(evens // 2, span // 2))
-    nseg = max(2, -(-evens // seg_evens))
-    bits = (seg_evens + 4891) // 15 + 1          # per class array
-    per_seg = bits + 2 * bits + (seg_evens // 32 + 65536) * 4
-    return nseg * per_seg
-
-
 class Server:
-    """The request loop. `budget` bounds the summed footprint of admitted,
-    unfinished requests (a request larger than the budget is admitted alone)."""
+    """The request loop over one engine, its batch buffers allocated within
+    `budget` device bytes."""
 
-    def __init__(self, engine: gbamd.Engine, max_limit: int, min_start: int, budget: int = 64 << 30,
+    def __init__(self, engine: gbamd.Engine, max_limit: int, min_start: int, budget: int = BUDGET,
                  span: int = SEGMENT):
         self.engine = engine
-        self.batcher = gbamd.Batcher(engine, max_limit, min_start, span)
+        self.batcher = gbamd.Batcher(engine, max_limit, min_start, span, device_budget=budget)
         self.budget = budget
         self.span = span
+        nbytes, self.slots = self.batcher.memory()
+        self.slot_bytes = nbytes / self.slots
+        self.peak_reserved = 0                # bytes (slots x slot bytes) reserved at once, at most
+
+    def reservation(self, start: int, limit: int) -> int:
+        """Segment slots a request reserves while it is admitted."""
+        return min(self.batcher.segments_of(start, limit), self.slots)
 
     def serve(self, requests, arrivals, clock=time.perf_counter):
         """Runs the loop in real time over requests[i] arriving at
         arrivals[i] (seconds from the start). Returns, per request, a dict:
-        arrival, admitted, completed (seconds), batch id and result."""
+        arrival, admitted, completed (seconds), id, slots reserved and result."""
         n = len(requests)
         order = sorted(range(n), key=lambda i: arrivals[i])
+        need = [self.reservation(a, b) for a, b in requests]
         t0 = clock()
-        waiting = collections.deque()      # arrived, not admitted (memory)
+        waiting = collections.deque()      # arrived, not admitted (no room)
         inflight = {}                      # batcher id -> request index
-        used = 0
-        foot = [footprint_bytes(a, b, self.span) for a, b in requests]
+        used = 0                           # slots reserved by the requests in flight
         rec = [None] * n
         nxt = 0
         done = 0
@@ -92,18 +97,20 @@ class Server:
             while nxt < n and arrivals[order[nxt]] <= now:
                 waiting.append(order[nxt])
                 nxt += 1
-            while waiting and (not inflight or used + foot[waiting[0]] <= self.budget):
+            while waiting and (not inflight or used + need[waiting[0]] <= self.slots):
                 i = waiting.popleft()
                 a, b = requests[i]
                 bid = self.batcher.submit(a, b)
                 inflight[bid] = i
-                used += foot[i]
-                rec[i] = {"arrival": arrivals[i], "admitted": clock() - t0, "id": bid}
+                used += need[i]
+                self.peak_reserved = max(self.peak_reserved, int(used * self.slot_bytes))
+                rec[i] = {"arrival": arrivals[i], "admitted": clock() - t0, "id": bid, "slots": need[i],
+                          "reserved_after": used}
             fin, more = self.batcher.step()
             tnow = clock() - t0
             for bid in fin:
                 i = inflight.pop(bid)
-                used -= foot[i]
+                used -= need[i]
                 rec[i]["completed"] = tnow
                 rec[i]["result"] = self.batcher.result(bid)
                 done += 1
@@ -112,6 +119,10 @@ class Server:
                 if wait > 0:
                     time.sleep(wait)
         return rec
+
+    def device_bytes(self) -> int:
+        """Device bytes the batch buffers hold now."""
+        return self.batcher.memory()[0]
 
     def close(self):
         self.batcher.close()

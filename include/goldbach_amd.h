@@ -103,7 +103,16 @@ int gba_engine_prepare(gba_engine* e, uint64_t limit, uint64_t start_even, doubl
  *   open:   plans for ranges with start >= min_start, limit <= max_limit
  *           (large-prime split, multiplier bitmap, buffers; the engine must
  *           cover max_limit); *batch_segments receives the segments a batch
- *           may hold.
+ *           may hold. The batch buffers are the batcher's own, allocated
+ *           here; a nonzero device_budget (bytes) bounds them: the segments
+ *           per batch are the most whose buffers fit in 7/8 of it, and the
+ *           direct harvest's regions are allocated only where they fit in
+ *           the rest (else the counted form). Fails when one-segment batches
+ *           do not fit.
+ *   memory: the device bytes the batch buffers hold now, and the segments
+ *           that can be in flight at once (two batches).
+ *   segments: the segments [start_even, limit] splits into (as a plain run
+ *           splits it); an admission policy reserves in-flight segments by it.
  *   submit: queues [start_even, limit], *id = 0, 1, ...
  *   step:   issues the next batch of waiting segments and collects the one
  *           issued before; returns up to finished_cap ids of completed
@@ -114,14 +123,17 @@ int gba_engine_prepare(gba_engine* e, uint64_t limit, uint64_t start_even, doubl
  * One thread per batcher; the engine must outlive it. All return 0 on
  * success. */
 typedef struct gba_batcher gba_batcher;
-int gba_batcher_open(gba_engine* e, uint64_t max_limit, uint64_t min_start, uint64_t segment_even, gba_batcher** out,
-                     uint32_t* batch_segments, char* err, size_t err_cap);
+int gba_batcher_open(gba_engine* e, uint64_t max_limit, uint64_t min_start, uint64_t segment_even,
+                     uint64_t device_budget, gba_batcher** out, uint32_t* batch_segments, char* err, size_t err_cap);
 int gba_batcher_submit(gba_batcher* b, uint64_t start_even, uint64_t limit, uint64_t* id, char* err, size_t err_cap);
 int gba_batcher_step(gba_batcher* b, uint64_t* finished, uint32_t finished_cap, uint32_t* n_finished, int* more,
                      char* err, size_t err_cap);
 int gba_batcher_result(gba_batcher* b, uint64_t id, gba_run_result* out, char* stats_line, size_t stats_cap,
                        char* state, size_t state_cap, char* err, size_t err_cap);
 int gba_batcher_totals(gba_batcher* b, gba_run_result* out);
+int gba_batcher_memory(gba_batcher* b, uint64_t* device_bytes, uint32_t* inflight_segments);
+int gba_batcher_segments(gba_batcher* b, uint64_t start_even, uint64_t limit, uint64_t* segments, char* err,
+                         size_t err_cap);
 void gba_batcher_close(gba_batcher* b);
 
 /* Tuned knob defaults (the autotuner's file, goldbach-l0_amd/autotune.py):

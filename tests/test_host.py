@@ -1,7 +1,9 @@
 """CPU-side checks of the product: the C-ABI library loads and exports every
-entry point its headers declare, the CLI keeps the reference's usage
-contract (tests/cli_tests.sh: exit 2 on usage errors), and a run without a
-GPU fails closed."""
+entry point its headers declare, the host API the reference's command-line
+driver calls (src/main.cpp of the reference) compiles and links against
+this build's headers and library (tests/cli_api_check.cpp), the CLI keeps
+the reference's usage contract (tests/cli_tests.sh: exit 2 on usage errors),
+and a run without a GPU fails closed."""
 import ctypes
 import pathlib
 import re
@@ -29,6 +31,18 @@ def test_library_exports_every_declared_symbol(header):
 
 def test_library_loads_and_reports_version():
     assert gbamd.version().startswith("2.8.0")
+
+
+def test_reference_cli_host_api_links(tmp_path):
+    pkg = ROOT / "goldbach-l0_amd"
+    exe = tmp_path / "cli_api_check"
+    r = subprocess.run(["g++", "-std=c++23", "-O1", "-Wall", "-Wextra", f"-I{ROOT / 'include'}",
+                        f"-I{pkg / 'csrc' / 'host'}", "-o", str(exe), str(ROOT / "tests" / "cli_api_check.cpp"),
+                        f"-L{pkg}", "-lgoldbach_amd", f"-Wl,-rpath,{pkg}", "-pthread"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "0 failed" in r.stdout, r.stdout + r.stderr
 
 
 def _cli(*args):
