@@ -317,8 +317,10 @@ int gbh_launch_harvest(gbh_ctx* c, int s, const gbh_harvest_args* a)
         GBH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, lds));
         if (a->queue_wg) per_cu = std::min(per_cu, int(a->queue_wg));
         grid = std::min<uint32_t>(a->nchunks, uint32_t(std::max(per_cu, 1)) * uint32_t(std::max(c->info.cus, 1)));
-        hipLaunchKernelGGL(gbh_dev::k_fill, dim3(1), dim3(256), 0, S(c, s), a->queue, size_t(1), 0u);
-        GBH_HIP(hipGetLastError());
+        if (!a->queue_zeroed) {
+            hipLaunchKernelGGL(gbh_dev::k_fill, dim3(1), dim3(256), 0, S(c, s), a->queue, size_t(1), 0u);
+            GBH_HIP(hipGetLastError());
+        }
     }
 #define GBH_HARVEST_ARGS                                                                                           \
     a->b, a->primes, a->chunks, a->runs, a->runs ? a->nruns : 1u, uint32_t(nb), a->t_hl, a->t_hi, a->t_hi_end,      \

@@ -51,6 +51,7 @@ struct Buf {
     void* p = nullptr;
     int kind = 0;
     size_t bytes = 0;
+    uint64_t gen = 0;                  // bumped by every allocation (a new buffer's contents are unknown)
     Buf() = default;
     Buf(const Buf&) = delete;
     Buf& operator=(const Buf&) = delete;
@@ -71,6 +72,7 @@ struct Buf {
         kind = k;
         check(gbh_alloc(c, n, k, &p), "gbh_alloc");
         bytes = n;
+        ++gen;
     }
     template <class T> T* as() const { return static_cast<T*>(p); }
 };
@@ -218,15 +220,26 @@ struct Slots {
         Buf vmap, vhits, vhit_n, vchunks;   // the GB_BUCKET_VERIFY rebuild
     } sl[2];
     Buf h_vchunks;
-    Buf d_out[2], h_out[2], h_tab[2], h_chunks[2], h_rdesc[2];
+    // two device output blocks per slot, used by its batches in turn: the
+    // block the next batch takes is cleared while the batch before it runs
+    Buf d_out[2][2];
+    uint32_t out_sel[2] = {0, 0};             // the block the slot's next batch takes
+    Buf h_out[2], h_tab[2], h_chunks[2], h_rdesc[2];
     Event ev[2][EV_N];
     uint64_t slot_plan[2] = {~0ull, ~0ull};   // the chunk plan each slot's device tables hold
     uint64_t budget = 0;                      // device bytes the set may hold (0: unbounded)
+    // output block k of a slot (its first clean_n segments) and its harvest
+    // chunk counter (word 32 k of the slot's hq) are zero, cleared in stream
+    // order by the slot's batch before, while the buffers are the
+    // allocations clean_gen names: the batch that takes the block then need
+    // not wait for the clears
+    uint32_t clean_n[2][2] = {};
+    uint64_t clean_gen[2][2][2] = {};
 
     void init(gbh_ctx* c, uint32_t kbatch)
     {
         for (int s = 0; s < 2; ++s) {
-            d_out[s].ensure(c, sizeof(gbh_segout) * kbatch, 0);
+            for (Buf& o : d_out[s]) o.ensure(c, sizeof(gbh_segout) * kbatch, 0);
             h_out[s].ensure(c, sizeof(gbh_segout) * kbatch, 1);
             h_tab[s].ensure(c, tab_layout(kbatch).bytes, 1);
             for (auto& e : ev[s]) e.create(c);
@@ -246,7 +259,8 @@ struct Slots {
                                  &B.chunks, &B.rdesc, &B.groups, &B.region, &B.fills, &B.ridge_more, &B.hq, &B.tab,
                                  &B.vmap, &B.vhits, &B.vhit_n, &B.vchunks})
                 n += b->kind == 0 ? b->bytes : 0;
-        for (const Buf& b : d_out) n += b.bytes;
+        for (const auto& pair : d_out)
+            for (const Buf& b : pair) n += b.bytes;
         return n;
     }
 };
@@ -797,7 +811,7 @@ void Pipeline::plan(const PlanBounds& pb)
                              b(K * uint64_t(pcap) * 4) + b(K * uint64_t(swords_) * 4) +
                              b(K * uint64_t(GBH_RIDGE_MORE) * 16) + 256 + b(K * 8 * uint64_t(nwin_max_) * 4) +
                              (nlarge_ ? b(K * 8 * uint64_t(nwin_max_) * cap * 4) : 0) + b(tab_layout(K).bytes) +
-                             b(K * 15 * GBH_NCAND * 4) + b(K * 15 * GBH_NCAND_PAD * 4) + b(sizeof(gbh_segout) * K);
+                             b(K * 15 * GBH_NCAND * 4) + b(K * 15 * GBH_NCAND_PAD * 4) + 2 * b(sizeof(gbh_segout) * K);
         return 2 * one;
     };
     if (multi_) {
@@ -835,7 +849,7 @@ void Pipeline::plan(const PlanBounds& pb)
         B.coffa.ensure(c, K * 15 * GBH_NCAND_PAD * 4, 0);
     }
     for (int s = 0; s < 2; ++s) {
-        S.d_out[s].ensure(c, sizeof(gbh_segout) * K, 0);
+        for (Buf& o : S.d_out[s]) o.ensure(c, sizeof(gbh_segout) * K, 0);
         S.h_out[s].ensure(c, sizeof(gbh_segout) * K, 1);
         S.h_tab[s].ensure(c, tab_layout(K).bytes, 1);
     }
@@ -1290,7 +1304,12 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         run_max = std::max(run_max, hr[nruns - 1].nseg);
     }
     gbh_batch b{dg, nseg, cw_, wbits, nwin, B.map.as<uint64_t>()};
-    gbh_segout* out = S.d_out[slot].as<gbh_segout>();
+    // this batch's output block and chunk counter; the other pair is cleared
+    // for the slot's next batch beside this one's harvest
+    const uint32_t sel = S.out_sel[slot];
+    S.out_sel[slot] = sel ^ 1u;
+    gbh_segout* out = S.d_out[slot][sel].as<gbh_segout>();
+    uint32_t* hq = B.hq.as<uint32_t>() + 32 * sel;
     auto& ev = S.ev[slot];
     check(gbh_event_record(c, ms, ev[EV_BEGIN].e), "event");
     harvested_[slot] = nlarge_ != 0;
@@ -1341,7 +1360,10 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     // the fill counts it needs, then its launch; the sieve phase and the
     // rest of the set-up are queued behind it
     check(gbh_copy_h2d(c, ms, B.tab.p, ht, tl.bytes), "upload batch tables");
-    check(gbh_reset_outputs(c, ms, out, nseg), "reset outputs");
+    const bool zeroed = S.clean_n[slot][sel] >= nseg && S.clean_gen[slot][sel][0] == S.d_out[slot][sel].gen &&
+                        S.clean_gen[slot][sel][1] == B.hq.gen;
+    S.clean_n[slot][sel] = 0;
+    if (!zeroed) check(gbh_reset_outputs(c, ms, out, nseg), "reset outputs");
     uint32_t ncoop = 0;
     while (ncoop < nwin_ && E.base[E.win_lo + ncoop] < wbits / 128) ++ncoop;
     // the window stage's sieve phase (masks, window primes) on the slot's
@@ -1361,8 +1383,8 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
                            mb_ ? mb_->m_base : 0, mb_ ? mb_->m_span : 0, B.hits.as<uint32_t>(),
                            B.hit_n.as<uint32_t>(), direct ? B.rdesc.as<uint32_t>() : nullptr,
                            direct ? B.region.as<uint32_t>() : nullptr, direct ? B.fills.as<uint32_t>() : nullptr,
-                           &out[0].err, persist_ ? B.hq.as<uint32_t>() : nullptr, persist_wg_,
-                           few_path() ? 1u : 0u};
+                           &out[0].err, persist_ ? hq : nullptr, persist_wg_,
+                           few_path() ? 1u : 0u, zeroed ? 1u : 0u};
         check(gbh_event_record(c, ms, ev[EV_HL0].e), "event");
         check(gbh_launch_harvest(c, ms, &h), "harvest");
         check(gbh_event_record(c, ms, ev[EV_HL1].e), "event");
@@ -1371,6 +1393,13 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     // the partition's list counts, zeroed beside the harvest (the main
     // stream waits for EV_S0 before the partition)
     if (direct) check(gbh_zero(c, ss, B.hit_n.p, size_t(nseg) * 8 * nwin * 4), "zero hit counts");
+    // the checker's candidate tables depend on the geometry alone: built here
+    // on the side stream, ahead of the set-up (the main stream's wait for the
+    // set-up before the partition covers them)
+    const uint32_t cgroups = nseg < 4 ? 3u : 1u;   // short batches: the 15 even classes over 3 work-groups
+    gbh_check_args ca{b, E.ka, pcap_, B.coff.as<uint32_t>(), B.coffa.as<uint32_t>(), B.pending.as<uint32_t>(), out,
+                      0u, cgroups};
+    if (check_and_count) check(gbh_launch_check_a(c, ss, &ca), "check candidate tables");
     const uint32_t nwp = nwin_ + nx_;   // the windows' primes
     if (nwp) {
         gbh_setup_args a{b, E.d_win.as<uint32_t>(), nwp, 0, B.i0.as<uint32_t>()};
@@ -1380,6 +1409,16 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     {
         const gbh_window_args w{b, E.d_win.as<uint32_t>(), nwp, ncoop, B.i0.as<uint32_t>(), nullptr, nullptr, 0, 1};
         check(gbh_launch_window(c, ss, &w), "window sieve");
+    }
+    {   // the slot's next batch: its output block and chunk counter, cleared
+        // here beside the harvest (EV_S1, which the hit-list phase waits for,
+        // covers them, so they are done before this batch ends)
+        const uint32_t nx = sel ^ 1u;
+        check(gbh_reset_outputs(c, ss, S.d_out[slot][nx].as<gbh_segout>(), kb_), "reset outputs");
+        check(gbh_fill32(c, ss, B.hq.as<uint32_t>() + 32 * nx, 1, 0u), "clear the harvest chunk counter");
+        S.clean_n[slot][nx] = kb_;
+        S.clean_gen[slot][nx][0] = S.d_out[slot][nx].gen;
+        S.clean_gen[slot][nx][1] = B.hq.gen;
     }
     check(gbh_event_record(c, ss, ev[EV_S1].e), "event");
     if (direct) {
@@ -1405,18 +1444,10 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     if (check_and_count) {
         uint32_t kinds = 0;
         for (const SegJob& j : batch) kinds |= j.sampled ? 2u : 1u;
-        // short batches: the 15 even classes over 3 work-groups (three times
-        // the waves; each stages the same rows)
-        const uint32_t cgroups = nseg < 4 ? 3u : 1u;
-        gbh_check_args ca{b, E.ka, pcap_, B.coff.as<uint32_t>(), B.coffa.as<uint32_t>(),
-                          B.pending.as<uint32_t>(), out, kinds, cgroups};
+        ca.kinds = kinds | 4;                // the candidate tables are built
         if (kinds == 3 && count_side_[slot]) {
             // both kinds: the histogram segments on the side stream (ahead
             // of the counter), the others here, phase B after both
-            ca.kinds = 0;                    // the candidate tables only
-            check(gbh_launch_check_a(c, ms, &ca), "check candidate tables");
-            check(gbh_event_record(c, ms, ev[EV_COFF].e), "event");
-            check(gbh_stream_wait(c, ss, ev[EV_COFF].e), "stream wait");
             ca.kinds = 1 | 4;
             check(gbh_launch_check_a(c, ms, &ca), "check phase A");
             ca.kinds = 2 | 4;
@@ -1468,7 +1499,7 @@ void Pipeline::enqueue_verify(uint32_t s, const gbh_batch& b, gbh_segout* out, i
     check(gbh_zero(c, ms, B.vhit_n.p, size_t(8) * b.nwin * 4), "verify zero");
     const gbh_harvest_args h{vb, E.d_large, B.vchunks.as<uint32_t>(), nullptr, 0, 0, nvchunks_, cap_v_,
                              UINT32_MAX, UINT32_MAX, UINT32_MAX, 0, nullptr, 0, 0, B.vhits.as<uint32_t>(), B.vhit_n.as<uint32_t>(),
-                             nullptr, nullptr, nullptr, &out[s].err, nullptr, 0, 0};
+                             nullptr, nullptr, nullptr, &out[s].err, nullptr, 0, 0, 0};
     check(gbh_launch_harvest(c, ms, &h), "verify harvest");
     const gbh_window_args w{vb, E.d_win.as<uint32_t>(), nwin_, ncoop, B.i0.as<uint32_t>(),
                             B.vhits.as<uint32_t>(), B.vhit_n.as<uint32_t>(), cap_v_, 0};

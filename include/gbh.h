@@ -200,9 +200,10 @@ typedef struct {
     uint32_t* err;                 /* bit 0: a list or region overflow or a run beyond the contract;
                                       bit 2: t_hl^3 <= q; bit 3: m outside the bitmap */
     /* Persistent form, when queue is not null: a device word (cleared by the
-     * launch, in stream order) from which as many work-groups as the GPU holds
-     * at once take chunk indices until all are done; null: one work-group per
-     * chunk. Either way chunk k files into region k and fills row k. */
+     * launch, in stream order, unless queue_zeroed) from which as many
+     * work-groups as the GPU holds at once take chunk indices until all are
+     * done; null: one work-group per chunk. Either way chunk k files into
+     * region k and fills row k. */
     uint32_t* queue;
     uint32_t queue_wg;             /* persistent form: work-groups per CU at most (0: as many as fit) */
     /* Nonzero: the batch has chunks of bitmap primes with at most four
@@ -210,6 +211,9 @@ typedef struct {
      * queues of the compacted path for them (zero: such chunks take the
      * bitmap walk). */
     uint32_t few;
+    /* Nonzero: *queue is already zero in stream order (the caller cleared it
+     * after the queue's previous launch), so the launch does not clear it. */
+    uint32_t queue_zeroed;
 } gbh_harvest_args;
 
 /* Second step of the direct harvest: the regions of every chunk split into
