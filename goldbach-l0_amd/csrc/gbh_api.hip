@@ -446,14 +446,14 @@ namespace {
 int launch_count(gbh_ctx* c, int s, const gbh_count_args* a, uint32_t* sc)
 {
     if (const int rc = check_batch(a->b)) return rc;
-    if (a->swords < 2ull * a->nwords) return fail(GBH_E_ARG, "count scratch too small");
+    if (a->swords < 2ull * GBH_COUNT_SUB * a->nwords) return fail(GBH_E_ARG, "count scratch too small");
     if (uint64_t(a->nwords) + 1 > a->b.cw) return fail(GBH_E_ARG, "count words exceed the class stride");
     if (!a->cs || !a->out) return fail(GBH_E_ARG, "count: null argument");
     if (!a->nwords) return GBH_OK;
     if (!sc) return fail(GBH_E_ARG, "count: null scratch");
-    // about eight work-groups per CU over the batch, each over several words
-    // (one histogram flush per work-group, not per 256 words)
-    const unsigned grid = std::max(1u, std::min(grid_of(a->nwords, 256, 1u << 30),
+    // about eight work-groups per CU over the batch, each over several word
+    // pieces (one histogram flush per work-group, not per 256 pieces)
+    const unsigned grid = std::max(1u, std::min(grid_of(uint64_t(a->nwords) * GBH_COUNT_SUB, 256, 1u << 30),
                                                 (8u * uint32_t(std::max(c->info.cus, 1)) + a->b.nseg - 1) / a->b.nseg));
     hipLaunchKernelGGL(gbh_dev::k_count_a, dim3(grid, a->b.nseg), dim3(256), 0, S(c, s), a->b, a->cs, sc,
                        a->swords, a->out);

@@ -1468,6 +1468,56 @@ __device__ __forceinline__ uint64_t range_mask(uint32_t x, uint32_t lo, uint32_t
     return upto & ~((1ull << a) - 1);
 }
 
+// Wave-wide minimum / maximum (every lane gets the result).
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
+{
+#pragma unroll
+    for (int o = 32; o; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o));
+    return v;
+}
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v)
+{
+#pragma unroll
+    for (int o = 32; o; o >>= 1) {
+        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, o), hi = (uint32_t)__shfl_xor((int)(v >> 32), o);
+        const uint64_t w = ((uint64_t)hi << 32) | lo;
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+// The work-group's smallest first prime position, largest last one and
+// largest gap key into the segment's outputs: one global atomic each per
+// work-group (one per lane, all on the same three words, serialised at the
+// cache and cost more than the walk itself). first = ~0u: no prime; last
+// counts only with a prime; key 0: no gap.
+__device__ __forceinline__ void flush_extremes(uint32_t first, uint32_t last, bool any, uint64_t key,
+                                               gbh_segout* out, uint32_t* s_first, uint32_t* s_last,
+                                               unsigned long long* s_key)
+{
+    const uint32_t tid = threadIdx.x;
+    first = wave_min_u32(first);
+    last = ~wave_min_u32(any ? ~last : 0xFFFFFFFFu);   // the largest last among lanes with a prime
+    const bool wany = __ballot(any) != 0;
+    key = wave_max_u64(key);
+    if ((tid & 63) == 0) {
+        if (wany) {
+            atomicMin(s_first, first);
+            atomicMax(s_last, last);
+            atomicOr(s_last + 1, 1u);
+        }
+        if (key) atomicMax(s_key, (unsigned long long)key);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        if (s_last[1]) {
+            atomicMin(&out->pfirst, *s_first);
+            atomicMax(&out->plast, *s_last);
+        }
+        if (*s_key) atomicMax((unsigned long long*)&out->gmax, *s_key);
+    }
+}
+
 __device__ __forceinline__ void note_gap(uint32_t g, uint32_t up, uint32_t* hl, gbh_segout* out, uint64_t& best)
 {
     const uint32_t bin = g >> 1;
@@ -1484,11 +1534,14 @@ __global__ void __launch_bounds__(256) k_count_a(gbh_batch b, const gbh_countseg
                                                  uint32_t* __restrict__ scratch, uint32_t swords,
                                                  gbh_segout* __restrict__ outs)
 {
+    constexpr uint32_t kSub = GBH_COUNT_SUB, kBits = 64 / kSub;   // pieces per word, blocks per piece
     __shared__ uint32_t hl[GBH_GAP_BINS + 1];
-    __shared__ uint32_t cs[11];
+    __shared__ uint32_t cs[11], s_first, s_last[2];
+    __shared__ unsigned long long s_key;
     const uint32_t tid = threadIdx.x, s = blockIdx.y;
     const gbh_countseg& a = css[s];
-    if (!a.on || blockIdx.x * 256 >= a.nwords) return;
+    const uint32_t npieces = a.nwords * kSub;
+    if (!a.on || blockIdx.x * 256 >= npieces) return;
     // the walk's phase offsets, read once: pos0[k] / 2 (below 15) in 4-bit
     // field k
     uint32_t ph4 = 0;
@@ -1499,62 +1552,66 @@ __global__ void __launch_bounds__(256) k_count_a(gbh_batch b, const gbh_countseg
     uint32_t* fl = scratch + (size_t)s * swords;
     for (uint32_t i = tid; i <= GBH_GAP_BINS; i += 256) hl[i] = 0;
     if (tid < 11) cs[tid] = 0;
+    if (tid == 0) {
+        s_first = 0xFFFFFFFFu;
+        s_last[0] = s_last[1] = 0;
+        s_key = 0;
+    }
     __syncthreads();
     uint64_t best = 0;
+    uint32_t t_first = 0xFFFFFFFFu, t_last = 0;
+    bool t_any = false;
     uint32_t pcs[11];                       // primes per class, twins per pair: reduced per wave below
 #pragma unroll
     for (int k = 0; k < 11; ++k) pcs[k] = 0;
-    // words x, x + grid, ...: the work-group's histogram and counters are
-    // flushed once for all of them
-    for (uint32_t x = blockIdx.x * 256 + tid; x < a.nwords; x += gridDim.x * 256) {
-        uint64_t w[8];
+    // pieces y, y + grid, ...: piece y = word y / kSub, blocks
+    // [kBits (y % kSub), + kBits) of it (the kSub lanes of a word read the
+    // same words); the work-group's histogram and counters are flushed once
+    for (uint32_t y = blockIdx.x * 256 + tid; y < npieces; y += gridDim.x * 256) {
+        const uint32_t x = y / kSub, sh = kBits * (y % kSub);
+        const uint64_t pm = ((1ull << kBits) - 1) << sh;   // the piece's bits of the word
+        uint32_t h[8], any = 0;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            w[k] = map[(size_t)a.cls[k] * b.cw + x] & range_mask(x, a.lo[k], a.hi[k]);
-            pcs[k] += (uint32_t)__popcll(w[k]);
+            const uint64_t w = map[(size_t)a.cls[k] * b.cw + x] & range_mask(x, a.lo[k], a.hi[k]) & pm;
+            pcs[k] += (uint32_t)__popcll(w);
+            h[k] = (uint32_t)(w >> sh);
+            any |= h[k];
         }
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            const uint64_t lw = map[(size_t)a.tk[j] * b.cw + x] & range_mask(x, a.tlo[j], a.thi[j]);
+            const uint64_t lw = map[(size_t)a.tk[j] * b.cw + x] & range_mask(x, a.tlo[j], a.thi[j]) & pm;
             const uint64_t* p2 = map + (size_t)a.tk2[j] * b.cw + x;
             const uint64_t pw = a.td[j] ? ((p2[0] >> 1) | (p2[1] << 63)) : p2[0];
             pcs[8 + j] += (uint32_t)__popcll(lw & pw);
         }
         // the walk: primes in position order (block t, then ascending
-        // phase), over the 32-bit halves of the words; per block the eight
-        // class bits gathered into m8, then one step per prime
+        // phase); per block the eight class bits gathered into m8, then one
+        // step per prime
         uint32_t first = 0xFFFFFFFFu, prev = 0;
         bool have = false;
+        while (any) {
+            const uint32_t bt = (uint32_t)__ffs(any) - 1;
+            any &= any - 1;
+            uint32_t m8 = 0;
 #pragma unroll
-        for (int hf = 0; hf < 2; ++hf) {
-            uint32_t h32[8], any = 0;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                h32[k] = (uint32_t)(w[k] >> (32 * hf));
-                any |= h32[k];
-            }
-            while (any) {
-                const uint32_t bt = (uint32_t)__ffs(any) - 1;
-                any &= any - 1;
-                uint32_t m8 = 0;
-#pragma unroll
-                for (int k = 0; k < 8; ++k) m8 |= ((h32[k] >> bt) & 1u) << k;
-                const uint32_t base = 30u * (x * 64 + 32 * hf + bt);
-                do {
-                    const uint32_t k = (uint32_t)__ffs(m8) - 1;
-                    m8 &= m8 - 1;
-                    const uint32_t pos = base + 2 * ((ph4 >> (4 * k)) & 15u);
-                    if (have) note_gap(pos - prev, pos, hl, out, best);
-                    else { first = pos; have = true; }
-                    prev = pos;
-                } while (m8);
-            }
+            for (int k = 0; k < 8; ++k) m8 |= ((h[k] >> bt) & 1u) << k;
+            const uint32_t base = 30u * (x * 64 + sh + bt);
+            do {
+                const uint32_t k = (uint32_t)__ffs(m8) - 1;
+                m8 &= m8 - 1;
+                const uint32_t pos = base + 2 * ((ph4 >> (4 * k)) & 15u);
+                if (have) note_gap(pos - prev, pos, hl, out, best);
+                else { first = pos; have = true; }
+                prev = pos;
+            } while (m8);
         }
-        fl[2 * x] = first;
-        fl[2 * x + 1] = have ? prev : 0xFFFFFFFFu;
+        fl[2 * y] = first;
+        fl[2 * y + 1] = have ? prev : 0xFFFFFFFFu;
         if (have) {
-            atomicMin(&out->pfirst, first);
-            atomicMax(&out->plast, prev);
+            t_first = min(t_first, first);
+            t_last = max(t_last, prev);
+            t_any = true;
         }
     }
 #pragma unroll
@@ -1562,40 +1619,46 @@ __global__ void __launch_bounds__(256) k_count_a(gbh_batch b, const gbh_countseg
         const uint32_t t = wave_total63(pcs[k]);   // the wave's total in lane 63
         if ((tid & 63) == 63 && t) atomicAdd(&cs[k], t);
     }
-    if (best) atomicMax((unsigned long long*)&out->gmax, (unsigned long long)best);
-    __syncthreads();
+    flush_extremes(t_first, t_last, t_any, best, out, &s_first, s_last, &s_key);   // synchronises
     for (uint32_t i = tid; i <= GBH_GAP_BINS; i += 256)
         if (hl[i]) atomicAdd(&out->ghist[i], hl[i]);
     if (tid < 8 && cs[tid]) atomicAdd(&out->cnt[a.cls[tid]], cs[tid]);
     if (tid >= 8 && tid < 11 && cs[tid]) atomicAdd(&out->twins[tid - 8], cs[tid]);
 }
 
-// Gap from the last prime of word x to the first prime of the next word that
-// has one (the walk across word boundaries).
+// Gap from the last prime of piece y to the first prime of the next piece
+// that has one (the walk across piece and word boundaries).
 __global__ void __launch_bounds__(256) k_count_b(const gbh_countseg* __restrict__ css,
                                                  const uint32_t* __restrict__ scratch, uint32_t swords,
                                                  gbh_segout* __restrict__ outs)
 {
     __shared__ uint32_t hl[GBH_GAP_BINS + 1];
+    __shared__ uint32_t s_first, s_last[2];
+    __shared__ unsigned long long s_key;
     const uint32_t tid = threadIdx.x, s = blockIdx.y;
     const gbh_countseg& a = css[s];
-    if (!a.on || blockIdx.x * 256 >= a.nwords) return;
+    const uint32_t npieces = a.nwords * GBH_COUNT_SUB;
+    if (!a.on || blockIdx.x * 256 >= npieces) return;
     gbh_segout* out = outs + s;
     const uint32_t* fl = scratch + (size_t)s * swords;
     for (uint32_t i = tid; i <= GBH_GAP_BINS; i += 256) hl[i] = 0;
+    if (tid == 0) {
+        s_first = 0xFFFFFFFFu;
+        s_last[0] = s_last[1] = 0;
+        s_key = 0;
+    }
     __syncthreads();
     uint64_t best = 0;
-    for (uint32_t x = blockIdx.x * 256 + tid; x < a.nwords; x += gridDim.x * 256) {
-        const uint32_t last = fl[2 * x + 1];
+    for (uint32_t y = blockIdx.x * 256 + tid; y < npieces; y += gridDim.x * 256) {
+        const uint32_t last = fl[2 * y + 1];
         if (last != 0xFFFFFFFFu) {
-            for (uint32_t y = x + 1; y < a.nwords; ++y) {
-                const uint32_t f = fl[2 * y];
+            for (uint32_t z = y + 1; z < npieces; ++z) {
+                const uint32_t f = fl[2 * z];
                 if (f != 0xFFFFFFFFu) { note_gap(f - last, f, hl, out, best); break; }
             }
         }
     }
-    if (best) atomicMax((unsigned long long*)&out->gmax, (unsigned long long)best);
-    __syncthreads();
+    flush_extremes(0xFFFFFFFFu, 0, false, best, out, &s_first, s_last, &s_key);   // synchronises
     for (uint32_t i = tid; i <= GBH_GAP_BINS; i += 256)
         if (hl[i]) atomicAdd(&out->ghist[i], hl[i]);
 }

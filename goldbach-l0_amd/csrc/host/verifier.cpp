@@ -795,7 +795,7 @@ void Pipeline::plan(const PlanBounds& pb)
     const uint64_t kmax = std::max<uint64_t>(1, ((uint64_t(1) << 33) - 4 * MAX_GB_PRIME) / span_ - 1);
     const double lam = wbits_ * density_max_;
     const uint64_t evens_max = span_ + 2;
-    swords_ = uint32_t(2 * (nc_max / 64 + 4));
+    swords_ = uint32_t(2 * GBH_COUNT_SUB * (nc_max / 64 + 4));
     // list and pending capacities of batches of K segments
     auto caps = [&](uint64_t K) {
         return std::pair<uint32_t, uint32_t>{

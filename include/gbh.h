@@ -326,13 +326,17 @@ typedef struct {
 } gbh_checkb_args;
 
 /* Stage 6: prime / twin counts and the gap walk of the counting range
- * (reference count30 + gap_tail_exact, goldbach.cl:2499-2825). */
+ * (reference count30 + gap_tail_exact, goldbach.cl:2499-2825). The walk of
+ * each map word position runs as GBH_COUNT_SUB pieces of 64 / GBH_COUNT_SUB
+ * blocks of 30 (one lane each; the gaps across pieces come from their first
+ * and last prime positions, kept in the scratch). */
+#define GBH_COUNT_SUB 2
 typedef struct {
     gbh_batch b;
     const gbh_countseg* cs;        /* device array [nseg] */
     uint32_t nwords;               /* the batch's largest nwords */
-    uint32_t swords;               /* scratch u32 per segment (>= 2 * nwords) */
-    uint32_t* scratch;             /* [nseg][swords] per-word first / last prime positions */
+    uint32_t swords;               /* scratch u32 per segment (>= 2 * GBH_COUNT_SUB * nwords) */
+    uint32_t* scratch;             /* [nseg][swords] per-piece first / last prime positions */
     gbh_segout* out;
 } gbh_count_args;
 
@@ -384,9 +388,9 @@ int gbh_launch_count(gbh_ctx* ctx, int stream, const gbh_count_args* a);
 int gbh_launch_mapcmp(gbh_ctx* ctx, int stream, const gbh_mapcmp_args* a);
 
 /* ---- allocation-free form ----
- * gbh_launch_count without a workspace: the counter's per-word scratch (the
- * first and last prime position of every map word, read back by the gap
- * walk across words) is the library's own static device array of
+ * gbh_launch_count without a workspace: the counter's scratch (the first
+ * and last prime position of every piece of every map word, read back by
+ * the gap walk across pieces) is the library's own static device array of
  * GBH_NOALLOC_SCRATCH_WORDS u32 (16 MiB per device, held until the process
  * ends; gbh_open looks up its address on the context's device). No
  * counterpart in the reference: for callers that cannot allocate (code under

@@ -6,17 +6,21 @@ contents), and both pinned by numpy (prime counts per class, twin pairs,
 every gap between consecutive primes: histogram, largest gap and its place,
 the tail gaps). Out-of-contract calls are refused before any launch."""
 import ctypes
+import re
 
 import numpy as np
 import pytest
 
 import gbamd
+from conftest import ROOT
 
 pytestmark = pytest.mark.gpu
 
 U32, U64, P = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_void_p
 GBH_E_ARG = 2
 NOALLOC_WORDS = 4 << 20
+# pieces per map word of the counter's gap walk (include/gbh.h)
+COUNT_SUB = int(re.search(r"#define GBH_COUNT_SUB (\d+)", (ROOT / "include" / "gbh.h").read_text()).group(1))
 WBITS = 4096
 
 
@@ -159,7 +163,7 @@ def _call(dev, maps, nwin, cw, noalloc):
     assert lib.gbh_copy_h2d(ctx, 0, d_cs, ctypes.addressof(cs), ctypes.sizeof(cs)) == 0
     d_out = alloc(ctypes.sizeof(SegOut) * nseg)
     assert lib.gbh_reset_outputs(ctx, 0, d_out, nseg) == 0
-    swords = 2 * nwords
+    swords = 2 * COUNT_SUB * nwords
     a = CountArgs(b=Batch(geoms=d_geo, nseg=nseg, cw=cw, wbits=WBITS, nwin=nwin, map=d_map), cs=d_cs,
                   nwords=nwords, swords=swords, out=d_out)
     if noalloc:
