@@ -189,6 +189,15 @@ int gbh_copy_d2h(gbh_ctx* c, int s, void* dst, const void* src, size_t bytes)
     return GBH_OK;
 }
 
+int gbh_copy_d2h_rows(gbh_ctx* c, int s, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width,
+                      size_t rows)
+{
+    if (width > dpitch || width > spitch) return fail(GBH_E_ARG, "gbh_copy_d2h_rows: width above a pitch");
+    if (!rows || !width) return GBH_OK;
+    GBH_HIP(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, rows, hipMemcpyDeviceToHost, S(c, s)));
+    return GBH_OK;
+}
+
 int gbh_zero(gbh_ctx* c, int s, void* dst, size_t bytes)
 {
     if (bytes % 4) return fail(GBH_E_ARG, "gbh_zero: size not a multiple of 4");

@@ -1474,7 +1474,9 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     } else {
         check(gbh_event_record(c, ms, ev[EV_COUNT].e), "event");
     }
-    check(gbh_copy_d2h(c, ms, S.h_out[slot].p, out, sizeof(gbh_segout) * nseg), "copy outputs");
+    // the blocks without their leftover lists (fetched in collect when used)
+    check(gbh_copy_d2h_rows(c, ms, S.h_out[slot].p, sizeof(gbh_segout), out, sizeof(gbh_segout), GBH_SEGOUT_HEAD,
+                            nseg), "copy outputs");
     check(gbh_event_record(c, ms, ev[EV_END].e), "event");
     if (phase_trace_) std::fprintf(stderr, "host: batch queued %.3f ms after its first segment was formed\n",
                                    since(batch[0].r->t_first) * 1e3);
@@ -1518,7 +1520,18 @@ void Pipeline::collect(std::vector<SegJob>& batch, int slot)
     check(gbh_event_sync(c, ev[EV_END].e), "batch sync");
     const auto tcol = clock_::now();
     if (phase_trace_) std::fprintf(stderr, "host: waited %.3f ms for the batch\n", since(tw) * 1e3);
-    const gbh_segout* outs = S.h_out[slot].as<gbh_segout>();
+    gbh_segout* outs = S.h_out[slot].as<gbh_segout>();
+    {   // the leftover lists of the segments that have any (the CPU fallback)
+        const gbh_segout* dev = S.d_out[slot][S.out_sel[slot] ^ 1u].as<gbh_segout>();   // this batch's block
+        bool any = false;
+        for (size_t s = 0; s < batch.size(); ++s) {
+            const uint32_t n = std::min<uint32_t>(outs[s].left_n, GBH_LEFT_CAP);
+            if (!n) continue;
+            check(gbh_copy_d2h(c, main_stream(slot), outs[s].left, dev[s].left, size_t(n) * 4), "copy leftovers");
+            any = true;
+        }
+        if (any) check(gbh_stream_sync(c, main_stream(slot)), "leftover sync");
+    }
     if ((outs[0].err & 1u) && direct_used_[slot]) {
         // a region of the direct harvest overflowed: the batch again, with
         // counted lists (queued behind the batch in flight, which keeps its

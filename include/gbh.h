@@ -109,8 +109,10 @@ typedef struct {
     uint64_t* map;
 } gbh_batch;
 
-/* Per-segment outputs of the checker and the counter, copied back as one
- * block (reference: the slot's shared USM counters, verifier.cpp:42-87). */
+/* Per-segment outputs of the checker and the counter (reference: the slot's
+ * shared USM counters, verifier.cpp:42-87). The leftover list comes last:
+ * a caller copies back the first GBH_SEGOUT_HEAD bytes of each block and
+ * the list only when left_n > 0 (it is almost always empty). */
 typedef struct {
     uint32_t acct[2];              /* evens resolved by phase A / phase B */
     uint32_t pending_n;            /* evens phase A left */
@@ -129,9 +131,10 @@ typedef struct {
     uint32_t first[GBH_NCAND];     /* smallest even index with p(n) = candidate (by prime index; ~0u none) */
     uint32_t ridge_n;              /* phase-B evens with p(n) >= the ridge threshold */
     uint32_t pad2;
-    uint32_t left[GBH_LEFT_CAP];   /* even indices of the leftovers */
     uint64_t ridge[2 * GBH_RIDGE_CAP];   /* (n, p(n)) of the first GBH_RIDGE_CAP of them */
+    uint32_t left[GBH_LEFT_CAP];   /* even indices of the leftovers */
 } gbh_segout;
+#define GBH_SEGOUT_HEAD offsetof(gbh_segout, left)
 
 /* Stage 1: first class bit of every window prime in every segment
  * (reference mid_setup, goldbach.cl:3528-3563, and the host's
@@ -361,6 +364,9 @@ int gbh_alloc(gbh_ctx* ctx, size_t bytes, int kind, void** out);
 int gbh_free(gbh_ctx* ctx, void* ptr, int kind);
 int gbh_copy_h2d(gbh_ctx* ctx, int stream, void* dst, const void* src, size_t bytes);
 int gbh_copy_d2h(gbh_ctx* ctx, int stream, void* dst, const void* src, size_t bytes);
+/* rows pieces of width bytes, at src + k * spitch, to dst + k * dpitch */
+int gbh_copy_d2h_rows(gbh_ctx* ctx, int stream, void* dst, size_t dpitch, const void* src, size_t spitch,
+                      size_t width, size_t rows);
 int gbh_zero(gbh_ctx* ctx, int stream, void* dst, size_t bytes);  /* fill kernel (graph-safe) */
 int gbh_fill32(gbh_ctx* ctx, int stream, void* dst, size_t count, uint32_t value);
 /* out[s] of every segment of a batch reset for a new run: zeros, with

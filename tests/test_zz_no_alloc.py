@@ -42,7 +42,7 @@ class SegOut(ctypes.Structure):
                 ("twins", U32 * 3), ("pfirst", U32), ("plast", U32), ("gtail_n", U32), ("vdiff", U32),
                 ("gmax", U64), ("ghist", U32 * 1025), ("pad1", U32), ("gtail", U32 * 8192),
                 ("hist", U32 * (15 * 1205)), ("first", U32 * 1205), ("ridge_n", U32), ("pad2", U32),
-                ("left", U32 * 65536), ("ridge", U64 * 2048)]
+                ("ridge", U64 * 2048), ("left", U32 * 65536)]
 
 
 # a segment whose q_first = 1 mod 30: class k holds q = q_first + pos0[k] + 30 t
