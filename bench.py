@@ -200,8 +200,12 @@ def stats_files(td):
 
 
 def fresh(td):
+    """A fresh campaign for the next run: its statistics files removed."""
     for f in ("st.state", "st.log", "st.state.gaps"):
-        pathlib.Path(td, f).unlink(missing_ok=True)
+        try:
+            os.unlink(os.path.join(td, f))
+        except FileNotFoundError:
+            pass
 
 
 def tmpfs():

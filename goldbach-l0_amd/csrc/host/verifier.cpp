@@ -605,6 +605,7 @@ private:
     int main_stream(int slot) const { return overlap_ ? slot : 0; }
     uint64_t region_entries_ = 0;
     uint64_t planned_bytes_ = 0;       // the batch buffers' device bytes planned by plan()
+    bool plan_copies_ = false;         // uploads issued since plan() last synchronised
     uint32_t ngroups_ = 0, part_cap_ = GBH_PART_CAP;   // the partition: groups and their LDS entries
     // GB_BUCKET_VERIFY=N: every Nth segment's map is rebuilt with every large
     // prime on the wheel walk (no multiplier bitmap) and compared word for word
@@ -899,9 +900,12 @@ void Pipeline::plan(const PlanBounds& pb)
         for (auto& B : S.sl) {
             B.vchunks.ensure(c, vc.size() * 4, 0);
             check(gbh_copy_h2d(c, 0, B.vchunks.p, S.h_vchunks.p, vc.size() * 4), "upload verify chunks");
+            plan_copies_ = true;
         }
     }
-    check(gbh_stream_sync(c, 0), "plan sync");
+    // the plan's uploads read pinned host tables that later plans rewrite
+    if (plan_copies_) check(gbh_stream_sync(c, 0), "plan sync");
+    plan_copies_ = false;
 }
 
 // Harvest chunks: contiguous prime ranges of about equal cost (a fixed cost
@@ -1160,6 +1164,7 @@ void Pipeline::upload_chunks(int slot)
     B.chunks.ensure(c, nc * 4, 0);
     const int ms = main_stream(slot);
     check(gbh_copy_h2d(c, ms, B.chunks.p, h, nc * 4), "upload chunks");
+    plan_copies_ = true;
     if (cp.direct) {
         std::memcpy(h + nc, cp.rdesc.data(), nr * 4);
         std::memcpy(h + nc + nr, cp.groups.data(), ng * 4);
