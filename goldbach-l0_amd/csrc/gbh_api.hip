@@ -310,10 +310,12 @@ int gbh_launch_harvest(gbh_ctx* c, int s, const gbh_harvest_args* a)
     if (a->m_base % 240) return fail(GBH_E_ARG, "harvest: bitmap base not a multiple of 240");
     if (a->stage > GBH_HL_STAGE) return fail(GBH_E_ARG, "harvest: stage above GBH_HL_STAGE");
     size_t lds = a->stage ? size_t(nb) * 8 + size_t(a->stage) * 6 + 16 : size_t(nb) * 4;
-    uint32_t few_off = 0;
-    if (a->few && direct) {                   // the compacted path's queues after the counters
-        few_off = uint32_t((lds + 15) & ~size_t(15));
-        lds = few_off + sizeof(gbh_dev::FewLds);
+    uint32_t q_off = 0;
+    const bool queues = direct && (a->few || a->hit_queue);
+    if (queues) {                             // the per-wave queues after the counters
+        q_off = uint32_t((lds + 15) & ~size_t(15));
+        lds = q_off + std::max(a->few ? sizeof(gbh_dev::FewLds) : size_t(0),
+                               a->hit_queue ? sizeof(gbh_dev::HitQLds) : size_t(0));
     }
     if (lds > 144 * 1024) return fail(GBH_E_ARG, "harvest: lists and stage exceed the work-group's LDS");
     const bool few = a->few && direct;
@@ -334,7 +336,7 @@ int gbh_launch_harvest(gbh_ctx* c, int s, const gbh_harvest_args* a)
 #define GBH_HARVEST_ARGS                                                                                           \
     a->b, a->primes, a->chunks, a->runs, a->runs ? a->nruns : 1u, uint32_t(nb), a->t_hl, a->t_hi, a->t_hi_end,      \
         a->mbits, a->m_base, a->m_span, a->cap, a->stage, a->hits, a->hit_n, a->rdesc, a->region, a->fills, a->err, \
-        a->nchunks, a->queue, few_off
+        a->nchunks, a->queue, q_off, a->hit_queue
     if (few)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(gbh_dev::k_harvest<true, true>), dim3(grid), dim3(256), lds, S(c, s),
                            GBH_HARVEST_ARGS);

@@ -433,6 +433,112 @@ __device__ __forceinline__ void bm_group(const HlCtx& h, double qb0d, const uint
     }
 }
 
+// Direct form, bitmap primes with several multipliers: the bitmap walk of
+// bm_group with each wave's hits queued in LDS and filed by all its lanes
+// at once. Filed where they are found, every lane of a wave waits for the
+// lane with the most hits (their number varies from prime to prime) while
+// its neighbours idle; queued, a word step costs one wave scan and the
+// filing of ceil(hits / 64) hits per lane.
+constexpr uint32_t kHitQ = 512;          // queued hits per wave at most
+struct HitQLds {
+    uint32_t d[4][kHitQ];                // half-offset of each queued hit
+    uint8_t c[4][kHitQ];                 // its class
+};
+
+// This wave's LDS writes visible to its own later reads, in order.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Files the wave's n queued hits (every lane takes every 64th) and empties
+// the queue; n is wave-uniform.
+__device__ __forceinline__ void hitq_flush(const HlCtx& h, const uint32_t* df, const uint32_t* dl, const uint32_t* dq,
+                                           uint32_t* cnt, uint32_t* __restrict__ err, const uint32_t* qd,
+                                           const uint8_t* qc, uint32_t& n, uint32_t lane)
+{
+    wave_lds_sync();
+    for (uint32_t e = lane; e < n; e += 64) file_hit<kDirect>(h, df, dl, dq, qd[e], qc[e], cnt, nullptr, 0, err);
+    wave_lds_sync();
+    n = 0;
+}
+
+// kGQ primes per lane at a time (primes j0 + (k + u) 256 + tid, u < kGQ);
+// *n: the wave's queued hits (flushed by the caller at the end of the chunk).
+constexpr uint32_t kGQ = 4;
+__device__ __forceinline__ void bm_group_q(const HlCtx& h, double qb0d, const uint32_t* __restrict__ primes,
+                                           uint32_t j0, uint32_t j1, uint32_t k, uint32_t tid, const uint32_t* df,
+                                           const uint32_t* dl, const uint32_t* dq, const uint32_t* cmt, uint32_t* cnt,
+                                           HitQLds& Q, uint32_t& n, uint32_t* __restrict__ err)
+{
+    const uint32_t lane = tid & 63, wv = tid >> 6;
+    uint32_t* qd = Q.d[wv];
+    uint8_t* qc = Q.c[wv];
+    uint32_t p[kGQ];
+#pragma unroll
+    for (uint32_t u = 0; u < kGQ; ++u) {
+        const uint32_t j = j0 + (k + u) * 256 + tid;
+        p[u] = j < j1 ? primes[j] : 0u;
+    }
+    BmWalk wk[kGQ];
+    uint64_t v[kGQ];
+#pragma unroll
+    for (uint32_t u = 0; u < kGQ; ++u) wk[u] = bm_setup(h, qb0d, p[u], cmt, err);
+#pragma unroll
+    for (uint32_t u = 0; u < kGQ; ++u) v[u] = wk[u].live ? __builtin_nontemporal_load(h.mbits + wk[u].w0) : 0ull;
+    for (uint32_t u = 0; u < kGQ; ++u) {
+        const BmWalk& kk = wk[u];
+        uint64_t w = kk.w0, vv = v[u];
+        bool more = kk.live;
+        for (;;) {                            // one bitmap word per lane and step; wave-uniform exit
+            uint64_t x = 0;
+            if (more) {
+                x = vv;
+                if (w == kk.w0) x &= kk.lo_mask;
+                if (w == kk.w1) x &= kk.hi_mask;
+            }
+            uint32_t tot;
+            const uint32_t off = wave_excl_scan((uint32_t)__popcll(x), &tot);
+            if (tot) {
+                if (n + tot > kHitQ) hitq_flush(h, df, dl, dq, cnt, err, qd, qc, n, lane);
+                if (x) {
+                    const uint64_t qw = (uint64_t)kk.p * (h.m_base + 240 * w) - h.qb0;   // mod 2^64
+                    if (tot <= kHitQ) {
+                        uint32_t at = n + off;
+                        do {
+                            const uint32_t bit = (uint32_t)__ffsll((long long)x) - 1;
+                            x &= x - 1;
+                            const uint32_t i = bit & 7;
+                            const uint32_t delta = 30 * (bit >> 3) + res_of(i);
+                            qd[at] = (uint32_t)((qw + (uint64_t)kk.p * delta) >> 1);
+                            qc[at] = (uint8_t)((kk.cmap >> (3 * i)) & 7);
+                            ++at;
+                        } while (x);
+                    } else {                  // more than a queue holds (dense words): filed in place
+                        do {
+                            const uint32_t bit = (uint32_t)__ffsll((long long)x) - 1;
+                            x &= x - 1;
+                            const uint32_t i = bit & 7;
+                            const uint32_t delta = 30 * (bit >> 3) + res_of(i);
+                            const uint32_t d = (uint32_t)((qw + (uint64_t)kk.p * delta) >> 1);
+                            file_hit<kDirect>(h, df, dl, dq, d, (kk.cmap >> (3 * i)) & 7, cnt, nullptr, 0, err);
+                        } while (x);
+                    }
+                }
+                if (tot <= kHitQ) n += tot;
+            }
+            more = more && w < kk.w1;
+            if (!__ballot(more)) break;
+            if (more) {
+                ++w;
+                vv = __builtin_nontemporal_load(h.mbits + w);
+            }
+        }
+    }
+}
+
 // Bitmap primes above the run's q span (p > qb1 - qb0) have at most one
 // multiple in it, q = m0 p with m0 = ceil(qb0 / p): a lean form of
 // bm_setup + bm_run for them (one quotient, one bit of one word; most
@@ -701,7 +807,8 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
                                                  uint32_t* __restrict__ hits, uint32_t* __restrict__ hit_n,
                                                  const uint32_t* __restrict__ rdesc, uint32_t* __restrict__ region,
                                                  uint32_t* __restrict__ fills, uint32_t* __restrict__ err,
-                                                 uint32_t nchunks, uint32_t* __restrict__ next, uint32_t few_off)
+                                                 uint32_t nchunks, uint32_t* __restrict__ next, uint32_t q_off,
+                                                 uint32_t hit_queue)
 {
     __shared__ uint32_t df[GBH_MAX_BATCH], dl[GBH_MAX_BATCH], dq[GBH_MAX_BATCH * 8], cmt[8], sn, chunk_at;
     __shared__ uint64_t any_hit[GBH_HL_CHUNK / 64 + 4 * (kG > kS ? kG : kS)];   // bitmap primes with a set bit in range (pass 2 walks those)
@@ -808,14 +915,14 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
             // bitmap primes with at most 4 multipliers in the span, inside the
             // compacted path's quotient bounds (uniform over the chunk)
             const uint64_t pmin = primes[j0];
-            const bool few = kFew && few_off && !wheel && pmin >= (1u << kFewPminLog2) && (qb1 >> kFewQLog2) < pmin &&
+            const bool few = kFew && q_off && !wheel && pmin >= (1u << kFewPminLog2) && (qb1 >> kFewQLog2) < pmin &&
                              qb1 - qb0 < 4 * pmin;
             const bool wide = primes[je - 1] >= (1u << 31);
             if (wheel) {
                 for (uint32_t j = j0 + tid; j < je; j += 256) walk_wheel<kDirect>(h, qb0d, primes[j], df, dl, dq, wl, cnt, rh, cap, err);
             } else if (kFew && few) {
-                // the queues: the dynamic LDS from byte few_off (given only when the batch has such chunks)
-                FewLds& few_lds = *reinterpret_cast<FewLds*>(reinterpret_cast<char*>(cnt) + few_off);
+                // the queues: the dynamic LDS from byte q_off
+                FewLds& few_lds = *reinterpret_cast<FewLds*>(reinterpret_cast<char*>(cnt) + q_off);
                 const uint64_t span = qb1 - qb0;
                 const FewRun f{qb0d, (uint32_t)qb0, (uint32_t)(qb0 % 30), span,
                                (uint32_t)min(span, (uint64_t)0xFFFFFFFFu)};
@@ -823,6 +930,14 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
                 else if (single) few_chunk<1, false>(h, f, primes, j0, je, tid, df, dl, dq, cmt, cnt, few_lds, err);
                 else if (wide) few_chunk<4, true>(h, f, primes, j0, je, tid, df, dl, dq, cmt, cnt, few_lds, err);
                 else few_chunk<4, false>(h, f, primes, j0, je, tid, df, dl, dq, cmt, cnt, few_lds, err);
+            } else if (kFew && !single && q_off && hit_queue) {
+                // (the instance with the compacted path only: its registers
+                // are spent already, the other instance keeps its waves)
+                HitQLds& hq = *reinterpret_cast<HitQLds*>(reinterpret_cast<char*>(cnt) + q_off);
+                uint32_t n = 0;               // this wave's queued hits
+                for (uint32_t k = 0; j0 + k * 256 < je; k += kGQ)
+                    bm_group_q(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, hq, n, err);
+                if (n) hitq_flush(h, df, dl, dq, cnt, err, hq.d[tid >> 6], hq.c[tid >> 6], n, tid & 63);
             } else {
                 for (uint32_t k = 0; j0 + k * 256 < je; k += single ? kS : kG) {
                     if (single) bm_single<kDirect>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);

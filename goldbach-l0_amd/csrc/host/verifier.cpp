@@ -606,6 +606,11 @@ private:
     uint64_t region_entries_ = 0;
     uint64_t planned_bytes_ = 0;       // the batch buffers' device bytes planned by plan()
     bool plan_copies_ = false;         // uploads issued since plan() last synchronised
+    // GB_HL_HITQ: the direct harvest's bitmap walk files its hits through a
+    // per-wave LDS queue (1) or where it finds them (0); the queue comes with
+    // the compacted path's kernel instance (short batches), whose registers
+    // it shares
+    bool hit_queue_ = env::onoff("GB_HL_HITQ", true);
     uint32_t ngroups_ = 0, part_cap_ = GBH_PART_CAP;   // the partition: groups and their LDS entries
     // GB_BUCKET_VERIFY=N: every Nth segment's map is rebuilt with every large
     // prime on the wheel walk (no multiplier bitmap) and compared word for word
@@ -1389,7 +1394,7 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
                            B.hit_n.as<uint32_t>(), direct ? B.rdesc.as<uint32_t>() : nullptr,
                            direct ? B.region.as<uint32_t>() : nullptr, direct ? B.fills.as<uint32_t>() : nullptr,
                            &out[0].err, persist_ ? hq : nullptr, persist_wg_,
-                           few_path() ? 1u : 0u, zeroed ? 1u : 0u};
+                           few_path() ? 1u : 0u, zeroed ? 1u : 0u, hit_queue_ && few_path() ? 1u : 0u};
         check(gbh_event_record(c, ms, ev[EV_HL0].e), "event");
         check(gbh_launch_harvest(c, ms, &h), "harvest");
         check(gbh_event_record(c, ms, ev[EV_HL1].e), "event");
@@ -1506,7 +1511,7 @@ void Pipeline::enqueue_verify(uint32_t s, const gbh_batch& b, gbh_segout* out, i
     check(gbh_zero(c, ms, B.vhit_n.p, size_t(8) * b.nwin * 4), "verify zero");
     const gbh_harvest_args h{vb, E.d_large, B.vchunks.as<uint32_t>(), nullptr, 0, 0, nvchunks_, cap_v_,
                              UINT32_MAX, UINT32_MAX, UINT32_MAX, 0, nullptr, 0, 0, B.vhits.as<uint32_t>(), B.vhit_n.as<uint32_t>(),
-                             nullptr, nullptr, nullptr, &out[s].err, nullptr, 0, 0, 0};
+                             nullptr, nullptr, nullptr, &out[s].err, nullptr, 0, 0, 0, 0};
     check(gbh_launch_harvest(c, ms, &h), "verify harvest");
     const gbh_window_args w{vb, E.d_win.as<uint32_t>(), nwin_, ncoop, B.i0.as<uint32_t>(),
                             B.vhits.as<uint32_t>(), B.vhit_n.as<uint32_t>(), cap_v_, 0};

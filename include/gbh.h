@@ -217,6 +217,10 @@ typedef struct {
     /* Nonzero: *queue is already zero in stream order (the caller cleared it
      * after the queue's previous launch), so the launch does not clear it. */
     uint32_t queue_zeroed;
+    /* Nonzero (direct form with few): the bitmap walk of primes with several
+     * multipliers queues each wave's hits in LDS and files them with every
+     * lane at once (10 KB more LDS per work-group). */
+    uint32_t hit_queue;
 } gbh_harvest_args;
 
 /* Second step of the direct harvest: the regions of every chunk split into
