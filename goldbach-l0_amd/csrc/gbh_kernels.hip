@@ -735,6 +735,10 @@ __device__ __forceinline__ void few_chunk(const HlCtx& h, const FewRun& f, const
         bool live[kD];
 #pragma unroll
         for (uint32_t x = 0; x < kD; ++x) {
+            live[x] = false;
+            w[x] = 0;
+            c[x] = FewCand{1u, 0u, 0u, 0u};
+            if (64 * x >= prev_n) continue;  // the whole wave's slot x is empty (uniform): nothing to prepare
             const uint32_t e = lane + 64 * x;
             live[x] = e < prev_n;
             c[x] = pq[e < prev_n ? e : 0];
@@ -744,8 +748,17 @@ __device__ __forceinline__ void few_chunk(const HlCtx& h, const FewRun& f, const
             const bool inb = m >= h.m_base && m - h.m_base < h.m_span;
             outside = outside || (live[x] && !inb);
             live[x] = live[x] && inb;
-            const uint32_t r30 = ((uint32_t)(m >> 32) * 16u + (uint32_t)m % 30u) % 30u;   // 2^32 = 16 mod 30
-            const uint64_t y = ((m - r30 - h.m_base) >> 1) * 0xEEEEEEEEEEEEEEEFull;   // byte of m: exact / 30
+            // byte y = floor((m - m_base) / 30) of m (m_base is a multiple of
+            // 30, so the remainder is m mod 30): m - m_base < 2^42 is exact
+            // in double and its fused quotient by 30 within 2^-15 of the true
+            // one, whose fractional parts are multiples of 1/30: the floor of
+            // the quotient plus 2^-7 is exact
+            const uint64_t dm = live[x] ? m - h.m_base : 0ull;
+            const double dq = __builtin_fma((double)(uint32_t)(dm >> 32), 4294967296.0, (double)(uint32_t)dm);
+            const double fq = __builtin_floor(__builtin_fma(dq, 1.0 / 30.0, 0x1p-7));
+            const uint32_t fq_hi = (uint32_t)(fq * (1.0 / 4294967296.0));
+            const uint64_t y = ((uint64_t)fq_hi << 32) | (uint32_t)(fq - (double)fq_hi * 4294967296.0);
+            const uint32_t r30 = (uint32_t)dm - 30u * (uint32_t)y;
             // one load per slot whether or not it is live (word 0 then): the
             // loads stay unconditional, so waiting for them does not wait for
             // later ones
