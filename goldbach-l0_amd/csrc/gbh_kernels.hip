@@ -345,9 +345,17 @@ __device__ __forceinline__ BmWalk bm_setup(const HlCtx& h, double qb0d, uint32_t
         atomicOr(err, 8u);
         return k;
     }
-    // byte and residue of m0 (2^32 = 16 mod 30), then of m1 = m0 + t
-    const uint32_t r30 = ((uint32_t)(m0 >> 32) * 16u + (uint32_t)m0 % 30u) % 30u;
-    const uint64_t y0 = ((m0 - r30 - h.m_base) >> 1) * 0xEEEEEEEEEEEEEEEFull;   // exact / 30: 15^-1 mod 2^64
+    // byte y0 = floor((m0 - m_base) / 30) and residue of m0 (m_base is a
+    // multiple of 30; m0 - m_base < 2^42 is exact in double and its fused
+    // quotient by 30 within 2^-15 of the true one, whose fractional parts
+    // are multiples of 1/30: the floor of the quotient plus 2^-7 is exact),
+    // then of m1 = m0 + t
+    const uint64_t dm = m0 - h.m_base;
+    const double dmd = __builtin_fma((double)(uint32_t)(dm >> 32), 4294967296.0, (double)(uint32_t)dm);
+    const double fq = __builtin_floor(__builtin_fma(dmd, 1.0 / 30.0, 0x1p-7));
+    const uint32_t fq_hi = (uint32_t)(fq * (1.0 / 4294967296.0));
+    const uint64_t y0 = ((uint64_t)fq_hi << 32) | (uint32_t)(fq - (double)fq_hi * 4294967296.0);
+    const uint32_t r30 = (uint32_t)dm - 30u * (uint32_t)y0;
     const uint32_t e = r30 + t;
     const uint64_t y1 = y0 + e / 30;
     const uint32_t r1 = e % 30;
@@ -644,6 +652,17 @@ struct FewLds {
     FewCand q[4][kFewWave];
 };
 
+// x mod 30 for a 32-bit x: a fused double quotient (within 2^-26 of x / 30,
+// whose fractional parts are multiples of 1/30) floored after adding 2^-7,
+// then the remainder exactly in double (no 32-bit integer multiply or
+// division)
+__device__ __forceinline__ uint32_t mod30_u32(uint32_t x)
+{
+    const double xd = (double)x;
+    const double fq = __builtin_floor(__builtin_fma(xd, 1.0 / 30.0, 0x1p-7));
+    return (uint32_t)__builtin_fma(-30.0, fq, xd);
+}
+
 // The per-run constants of the scan.
 struct FewRun {
     double qb0d;
@@ -680,9 +699,9 @@ __device__ __forceinline__ void few_scan(const HlCtx& h, const FewRun& f, uint32
     }
     const uint32_t r0 = r ? pc - r : 0u;   // the first multiple at or above qb0 is qb0 + r0
     const uint32_t adj = (uint32_t)step + (r != 0);
-    uint32_t q30 = r0 % 30u + f.qb0_30;     // q mod 30 of that multiple
+    uint32_t q30 = mod30_u32(r0) + f.qb0_30;   // q mod 30 of that multiple
     q30 -= q30 >= 30 ? 30u : 0u;
-    const uint32_t p30 = kM > 1 ? pc % 30u : 0u;
+    const uint32_t p30 = kM > 1 ? mod30_u32(pc) : 0u;
 #pragma unroll
     for (uint32_t k = 0; k < kM; ++k) {
         const uint64_t off = (uint64_t)r0 + (uint64_t)k * pc;
