@@ -308,6 +308,8 @@ int gbh_launch_harvest(gbh_ctx* c, int s, const gbh_harvest_args* a)
     const uint64_t nb = uint64_t(a->runs ? a->run_nseg_max : a->b.nseg) * 8 * (direct ? 1 : a->b.nwin);
     if (nb > GBH_HL_BUCKETS) return fail(GBH_E_ARG, "harvest: more window lists than the work-group counts");
     if (a->m_base % 240) return fail(GBH_E_ARG, "harvest: bitmap base not a multiple of 240");
+    if (!a->primes.base || !a->primes.gap || !a->primes.n)
+        return fail(GBH_E_ARG, "harvest: no packed prime stream");
     if (a->stage > GBH_HL_STAGE) return fail(GBH_E_ARG, "harvest: stage above GBH_HL_STAGE");
     size_t lds = a->stage ? size_t(nb) * 8 + size_t(a->stage) * 6 + 16 : size_t(nb) * 4;
     uint32_t q_off = 0;

@@ -63,6 +63,42 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
     return v;
 }
 
+// Inclusive prefix sum over the wave's 64 lanes (every lane active): DPP
+// shifts by 1, 2, 4 and 8 inside each row of 16 lanes, then the row totals
+// carried by row_bcast:15 (into rows 1 and 3) and row_bcast:31 (into rows 2
+// and 3). Two 16-bit halves scan independently while no half's total
+// reaches 2^16.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);    // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);    // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);    // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);    // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);   // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);   // row_bcast:31
+    return v;
+}
+
+// The packed large-prime stream (gbh_pstream): the primes of whole groups,
+// one per lane. jA, jB: the first index of a group each (multiples of 64,
+// uniform over the wave, which must be whole). A group's half-gaps sum
+// below 63 * 256 < 2^16, so both groups scan in one word.
+__device__ __forceinline__ void ps_decode2(const gbh_pstream& ps, uint32_t jA, uint32_t jB, uint32_t lane,
+                                           uint32_t& pA, uint32_t& pB)
+{
+    const uint32_t gA = __builtin_amdgcn_readfirstlane(jA >> 6), gB = __builtin_amdgcn_readfirstlane(jB >> 6);
+    const uint32_t v = wave_incl_scan((uint32_t)ps.gap[(size_t)gA * 64 + lane] |
+                                      ((uint32_t)ps.gap[(size_t)gB * 64 + lane] << 16));
+    pA = ps.base[gA] + 2 * (v & 0xFFFFu);
+    pB = ps.base[gB] + 2 * (v >> 16);
+}
+
+__device__ __forceinline__ uint32_t ps_decode1(const gbh_pstream& ps, uint32_t jA, uint32_t lane)
+{
+    const uint32_t gA = __builtin_amdgcn_readfirstlane(jA >> 6);
+    return ps.base[gA] + 2 * wave_incl_scan((uint32_t)ps.gap[(size_t)gA * 64 + lane]);
+}
+
 __device__ __forceinline__ uint64_t* seg_map(const gbh_batch& b, uint32_t s)
 {
     return b.map + (size_t)s * 8 * b.cw;
@@ -409,19 +445,23 @@ constexpr uint32_t kG = 4;
 constexpr uint32_t kS = 8;
 
 template <int kMode>
-__device__ __forceinline__ void bm_group(const HlCtx& h, double qb0d, const uint32_t* __restrict__ primes,
+__device__ __forceinline__ void bm_group(const HlCtx& h, double qb0d, const gbh_pstream& ps,
                                          uint32_t j0, uint32_t j1, uint32_t k, uint32_t tid, const uint32_t* df,
                                          const uint32_t* dl, const uint32_t* dq, const uint32_t* cmt, uint32_t* cnt,
                                          uint64_t* any_hit, uint32_t* __restrict__ hits, uint32_t cap,
                                          uint32_t* __restrict__ err)
 {
     uint32_t p[kG];
+    const uint32_t jw = j0 + (tid & ~63u);
+#pragma unroll
+    for (uint32_t u = 0; u < kG; u += 2)
+        ps_decode2(ps, jw + (k + u) * 256, jw + (k + u + 1) * 256, tid & 63, p[u], p[u + 1]);
 #pragma unroll
     for (uint32_t u = 0; u < kG; ++u) {
         const uint32_t j = j0 + (k + u) * 256 + tid;
         bool on = j < j1;
         if (kMode == kWrite && on) on = (any_hit[(k + u) * 4 + (tid >> 6)] >> (tid & 63)) & 1;
-        p[u] = on ? primes[j] : 0u;
+        p[u] = on ? p[u] : 0u;
     }
     BmWalk wk[kG];
     uint64_t v[kG];
@@ -476,7 +516,7 @@ __device__ __forceinline__ void hitq_flush(const HlCtx& h, const uint32_t* df, c
 // kGQ primes per lane at a time (primes j0 + (k + u) 256 + tid, u < kGQ);
 // *n: the wave's queued hits (flushed by the caller at the end of the chunk).
 constexpr uint32_t kGQ = 4;
-__device__ __forceinline__ void bm_group_q(const HlCtx& h, double qb0d, const uint32_t* __restrict__ primes,
+__device__ __forceinline__ void bm_group_q(const HlCtx& h, double qb0d, const gbh_pstream& ps,
                                            uint32_t j0, uint32_t j1, uint32_t k, uint32_t tid, const uint32_t* df,
                                            const uint32_t* dl, const uint32_t* dq, const uint32_t* cmt, uint32_t* cnt,
                                            HitQLds& Q, uint32_t& n, uint32_t* __restrict__ err)
@@ -485,10 +525,14 @@ __device__ __forceinline__ void bm_group_q(const HlCtx& h, double qb0d, const ui
     uint32_t* qd = Q.d[wv];
     uint8_t* qc = Q.c[wv];
     uint32_t p[kGQ];
+    const uint32_t jw = j0 + (tid & ~63u);
+#pragma unroll
+    for (uint32_t u = 0; u < kGQ; u += 2)
+        ps_decode2(ps, jw + (k + u) * 256, jw + (k + u + 1) * 256, lane, p[u], p[u + 1]);
 #pragma unroll
     for (uint32_t u = 0; u < kGQ; ++u) {
         const uint32_t j = j0 + (k + u) * 256 + tid;
-        p[u] = j < j1 ? primes[j] : 0u;
+        p[u] = j < j1 ? p[u] : 0u;
     }
     BmWalk wk[kGQ];
     uint64_t v[kGQ];
@@ -552,19 +596,23 @@ __device__ __forceinline__ void bm_group_q(const HlCtx& h, double qb0d, const ui
 // bm_setup + bm_run for them (one quotient, one bit of one word; most
 // primes leave after the span or unit test, before any load).
 template <int kMode>
-__device__ __forceinline__ void bm_single(const HlCtx& h, double qb0d, const uint32_t* __restrict__ primes,
+__device__ __forceinline__ void bm_single(const HlCtx& h, double qb0d, const gbh_pstream& ps,
                                           uint32_t j0, uint32_t j1, uint32_t k, uint32_t tid, const uint32_t* df,
                                           const uint32_t* dl, const uint32_t* dq, const uint32_t* cmt, uint32_t* cnt,
                                           uint64_t* any_hit, uint32_t* __restrict__ hits, uint32_t cap,
                                           uint32_t* __restrict__ err)
 {
     uint32_t p[kS];
+    const uint32_t jw = j0 + (tid & ~63u);
+#pragma unroll
+    for (uint32_t u = 0; u < kS; u += 2)
+        ps_decode2(ps, jw + (k + u) * 256, jw + (k + u + 1) * 256, tid & 63, p[u], p[u + 1]);
 #pragma unroll
     for (uint32_t u = 0; u < kS; ++u) {
         const uint32_t j = j0 + (k + u) * 256 + tid;
         bool on = j < j1;
         if (kMode == kWrite && on) on = (any_hit[(k + u) * 4 + (tid >> 6)] >> (tid & 63)) & 1;
-        p[u] = on ? primes[j] : 0u;
+        p[u] = on ? p[u] : 0u;
     }
     const uint64_t span = 2ull * h.dmax;
     uint64_t w[kS];
@@ -728,7 +776,7 @@ __device__ __forceinline__ void few_scan(const HlCtx& h, const FewRun& f, uint32
 // scans kR primes per lane: at most 64 kR kM' candidates per wave (kM' =
 // 1 unit among 1 consecutive multiplier, 2 among 4), the wave's queue.
 template <uint32_t kM, bool kWide>
-__device__ __forceinline__ void few_chunk(const HlCtx& h, const FewRun& f, const uint32_t* __restrict__ primes,
+__device__ __forceinline__ void few_chunk(const HlCtx& h, const FewRun& f, const gbh_pstream& ps,
                                           uint32_t j0, uint32_t je, uint32_t tid, const uint32_t* df,
                                           const uint32_t* dl, const uint32_t* dq, const uint32_t* cmt, uint32_t* cnt,
                                           FewLds& L, uint32_t* __restrict__ err)
@@ -740,10 +788,13 @@ __device__ __forceinline__ void few_chunk(const HlCtx& h, const FewRun& f, const
     const uint32_t rounds = (je - j0 + 256 * kR - 1) / (256 * kR);
     bool bad = false, outside = false;
     uint32_t p[kR];
+    const uint32_t jw = j0 + (tid & ~63u);
+#pragma unroll
+    for (uint32_t u = 0; u < kR; u += 2) ps_decode2(ps, jw + u * 256, jw + (u + 1) * 256, lane, p[u], p[u + 1]);
 #pragma unroll
     for (uint32_t u = 0; u < kR; ++u) {
         const uint32_t j = j0 + u * 256 + tid;
-        p[u] = j < je ? primes[j] : 0u;
+        p[u] = j < je ? p[u] : 0u;
     }
     uint32_t prev_n = 0;
     // the last pass only resolves the last round's queue
@@ -790,10 +841,14 @@ __device__ __forceinline__ void few_chunk(const HlCtx& h, const FewRun& f, const
             FewCand* q = L.q[wv];
 #pragma unroll
             for (uint32_t u = 0; u < kR; ++u) few_scan<kM, kWide>(h, f, p[u], q, n, bad);
+            const uint32_t jn = jw + (r + 1) * 256 * kR;
+#pragma unroll
+            for (uint32_t u = 0; u < kR; u += 2)
+                ps_decode2(ps, jn + u * 256, jn + (u + 1) * 256, lane, p[u], p[u + 1]);
 #pragma unroll
             for (uint32_t u = 0; u < kR; ++u) {
                 const uint32_t j = j0 + (r + 1) * 256 * kR + u * 256 + tid;
-                p[u] = j < je ? primes[j] : 0u;
+                p[u] = j < je ? p[u] : 0u;
             }
         }
 #pragma unroll
@@ -829,7 +884,7 @@ __device__ __forceinline__ void few_chunk(const HlCtx& h, const FewRun& f, const
 // another's registers (the compacted path's would crowd out the kernels
 // that run beside the harvest).
 template <bool kDirectForm, bool kFew>
-__global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __restrict__ primes,
+__global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const gbh_pstream ps,
                                                  const uint32_t* __restrict__ chunks,
                                                  const gbh_hl_run* __restrict__ runs, uint32_t nruns, uint32_t nbmax,
                                                  uint32_t t_hl, uint32_t t_hi, uint32_t t_hi_end,
@@ -875,12 +930,16 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
         kc = blockIdx.x;
     }
     const uint32_t j0 = chunks[kc], j1 = chunks[kc + 1];
-    if (j1 - j0 > GBH_HL_CHUNK) {            // beyond the pass-2 flags: fail closed
+    // beyond the pass-2 flags, or not at a group of the packed stream: fail closed
+    if (j1 - j0 > GBH_HL_CHUNK || (j0 & (GBH_PS_GROUP - 1)) || j1 > ps.n) {
         if (tid == 0) atomicOr(err, 1u);
         continue;
     }
+    // the chunk's first prime (a group's base) and a bound on its last (the
+    // base of the group after the last prime's)
+    const uint32_t pfirst = ps.base[j0 / GBH_PS_GROUP];
     // a chunk is all wheel primes or all bitmap primes (the host splits at t_hl, t_hi and t_hi_end)
-    const bool wheel = j0 < j1 && (primes[j0] < t_hl || (primes[j0] >= t_hi && primes[j0] < t_hi_end));
+    const bool wheel = j0 < j1 && (pfirst < t_hl || (pfirst >= t_hi && pfirst < t_hi_end));
     for (uint32_t r = 0; r < nruns; ++r) {
         // every condition below is uniform over the work-group
         const uint32_t seg0 = runs ? runs[r].seg0 : 0u, rn = runs ? runs[r].nseg : b.nseg;
@@ -933,47 +992,50 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
         const uint64_t qb1 = qb0 + 2ull * h.dmax;
         const double qb0d = (double)qb0;
         if (!wheel) {                         // the chunk's smallest prime carries the two-factor rule
-            const uint64_t p = primes[j0], p2 = p * p;
+            const uint64_t p = pfirst, p2 = p * p;
             if (__umul64hi(p2, p) == 0 && p2 * p <= qb1) {
                 if (tid == 0) atomicOr(err, 4u);
                 continue;
             }
         }
         // every prime of the chunk above the run's q span: the lean form
-        const bool single = !wheel && (uint64_t)primes[j0] > qb1 - qb0;
+        const bool single = !wheel && (uint64_t)pfirst > qb1 - qb0;
         uint32_t* rh = hits + (size_t)seg0 * 8 * b.nwin * cap;   // the run's lists
         uint32_t* rhn = hit_n + (size_t)seg0 * 8 * b.nwin;
         if constexpr (kDirectForm) {   // one walk into the work-group's regions, then their fill counts
             // bitmap primes with at most 4 multipliers in the span, inside the
             // compacted path's quotient bounds (uniform over the chunk)
-            const uint64_t pmin = primes[j0];
+            const uint64_t pmin = pfirst;
             const bool few = kFew && q_off && !wheel && pmin >= (1u << kFewPminLog2) && (qb1 >> kFewQLog2) < pmin &&
                              qb1 - qb0 < 4 * pmin;
-            const bool wide = primes[je - 1] >= (1u << 31);
+            const bool wide = ps.base[(je - 1) / GBH_PS_GROUP + 1] >= (1u << 31);   // (an upper bound)
             if (wheel) {
-                for (uint32_t j = j0 + tid; j < je; j += 256) walk_wheel<kDirect>(h, qb0d, primes[j], df, dl, dq, wl, cnt, rh, cap, err);
+                for (uint32_t jb = j0; jb < je; jb += 256) {
+                    const uint32_t p = ps_decode1(ps, jb + (tid & ~63u), tid & 63);
+                    if (jb + tid < je) walk_wheel<kDirect>(h, qb0d, p, df, dl, dq, wl, cnt, rh, cap, err);
+                }
             } else if (kFew && few) {
                 // the queues: the dynamic LDS from byte q_off
                 FewLds& few_lds = *reinterpret_cast<FewLds*>(reinterpret_cast<char*>(cnt) + q_off);
                 const uint64_t span = qb1 - qb0;
                 const FewRun f{qb0d, (uint32_t)qb0, (uint32_t)(qb0 % 30), span,
                                (uint32_t)min(span, (uint64_t)0xFFFFFFFFu)};
-                if (single && wide) few_chunk<1, true>(h, f, primes, j0, je, tid, df, dl, dq, cmt, cnt, few_lds, err);
-                else if (single) few_chunk<1, false>(h, f, primes, j0, je, tid, df, dl, dq, cmt, cnt, few_lds, err);
-                else if (wide) few_chunk<4, true>(h, f, primes, j0, je, tid, df, dl, dq, cmt, cnt, few_lds, err);
-                else few_chunk<4, false>(h, f, primes, j0, je, tid, df, dl, dq, cmt, cnt, few_lds, err);
+                if (single && wide) few_chunk<1, true>(h, f, ps, j0, je, tid, df, dl, dq, cmt, cnt, few_lds, err);
+                else if (single) few_chunk<1, false>(h, f, ps, j0, je, tid, df, dl, dq, cmt, cnt, few_lds, err);
+                else if (wide) few_chunk<4, true>(h, f, ps, j0, je, tid, df, dl, dq, cmt, cnt, few_lds, err);
+                else few_chunk<4, false>(h, f, ps, j0, je, tid, df, dl, dq, cmt, cnt, few_lds, err);
             } else if (kFew && !single && q_off && hit_queue) {
                 // (the instance with the compacted path only: its registers
                 // are spent already, the other instance keeps its waves)
                 HitQLds& hq = *reinterpret_cast<HitQLds*>(reinterpret_cast<char*>(cnt) + q_off);
                 uint32_t n = 0;               // this wave's queued hits
                 for (uint32_t k = 0; j0 + k * 256 < je; k += kGQ)
-                    bm_group_q(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, hq, n, err);
+                    bm_group_q(h, qb0d, ps, j0, je, k, tid, df, dl, dq, cmt, cnt, hq, n, err);
                 if (n) hitq_flush(h, df, dl, dq, cnt, err, hq.d[tid >> 6], hq.c[tid >> 6], n, tid & 63);
             } else {
                 for (uint32_t k = 0; j0 + k * 256 < je; k += single ? kS : kG) {
-                    if (single) bm_single<kDirect>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
-                    else bm_group<kDirect>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
+                    if (single) bm_single<kDirect>(h, qb0d, ps, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
+                    else bm_group<kDirect>(h, qb0d, ps, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
                 }
             }
             __syncthreads();
@@ -982,11 +1044,14 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
             continue;
         }
         if (wheel) {
-            for (uint32_t j = j0 + tid; j < je; j += 256) walk_wheel<kCount>(h, qb0d, primes[j], df, dl, dq, wl, cnt, rh, cap, err);
+            for (uint32_t jb = j0; jb < je; jb += 256) {
+                const uint32_t p = ps_decode1(ps, jb + (tid & ~63u), tid & 63);
+                if (jb + tid < je) walk_wheel<kCount>(h, qb0d, p, df, dl, dq, wl, cnt, rh, cap, err);
+            }
         } else {
             for (uint32_t k = 0; j0 + k * 256 < je; k += single ? kS : kG) {
-                if (single) bm_single<kCount>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
-                else bm_group<kCount>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
+                if (single) bm_single<kCount>(h, qb0d, ps, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
+                else bm_group<kCount>(h, qb0d, ps, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
             }
         }
         __syncthreads();
@@ -1006,11 +1071,14 @@ __global__ void __launch_bounds__(256) k_harvest(gbh_batch b, const uint32_t* __
             continue;
         }
         if (wheel) {
-            for (uint32_t j = j0 + tid; j < je; j += 256) walk_wheel<kWrite>(h, qb0d, primes[j], df, dl, dq, wl, cnt, rh, cap, err);
+            for (uint32_t jb = j0; jb < je; jb += 256) {
+                const uint32_t p = ps_decode1(ps, jb + (tid & ~63u), tid & 63);
+                if (jb + tid < je) walk_wheel<kWrite>(h, qb0d, p, df, dl, dq, wl, cnt, rh, cap, err);
+            }
         } else {
             for (uint32_t k = 0; j0 + k * 256 < je; k += single ? kS : kG) {
-                if (single) bm_single<kWrite>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
-                else bm_group<kWrite>(h, qb0d, primes, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
+                if (single) bm_single<kWrite>(h, qb0d, ps, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
+                else bm_group<kWrite>(h, qb0d, ps, j0, je, k, tid, df, dl, dq, cmt, cnt, any_hit, rh, cap, err);
             }
         }
     }

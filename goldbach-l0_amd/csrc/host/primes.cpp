@@ -41,6 +41,37 @@ void parallel_for(uint64_t n, unsigned threads, const std::function<void(uint64_
     for (auto& x : th) x.join();
 }
 
+PackedPrimes pack_primes(const uint32_t* p, uint32_t n, uint32_t group, uint32_t pad)
+{
+    PackedPrimes pk;
+    pk.n = n;
+    if (!group || (group & (group - 1))) throw std::runtime_error("pack_primes: group not a power of two");
+    const uint64_t groups = (uint64_t(n) + pad + group - 1) / group;
+    if (groups * group > UINT32_MAX) throw std::runtime_error("pack_primes: stream beyond 2^32 entries");
+    pk.base.assign(groups, n ? p[n - 1] : 0u);
+    pk.gap.assign(groups * group, 0);
+    for (uint32_t j = 0; j < n; ++j) {
+        if (j % group == 0) {
+            pk.base[j / group] = p[j];
+            continue;
+        }
+        const uint32_t d = p[j] - p[j - 1];
+        if (p[j] <= p[j - 1] || (d & 1) || d / 2 > 255)
+            throw std::runtime_error("pack_primes: a gap of " + std::to_string(d) + " below " + std::to_string(p[j]) +
+                                     " does not fit one byte of half gap");
+        pk.gap[j] = uint8_t(d / 2);
+    }
+    return pk;
+}
+
+uint32_t packed_prime_at(const PackedPrimes& pk, uint32_t j, uint32_t group)
+{
+    const uint32_t g = j / group;
+    uint32_t v = pk.base.at(g);
+    for (uint32_t i = g * group + 1; i <= j; ++i) v += 2u * pk.gap.at(i);
+    return v;
+}
+
 std::vector<uint32_t> base_primes(uint32_t n, unsigned threads)
 {
     if (n < 1u << 20) return odd_primes_up_to(n);

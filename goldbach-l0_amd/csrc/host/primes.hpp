@@ -47,6 +47,21 @@ uint64_t prime_payload_hash(const uint32_t* p, size_t cnt);
 /// that fails its header, hash or content check is reported and ignored.
 std::vector<uint32_t> base_primes_cached(uint32_t n, const char* cache_path, unsigned threads);
 
+/// The large primes as the harvest's packed stream (gbh_pstream: groups of
+/// @p group primes, each group's first prime as its base, one byte of half
+/// gap for every other prime; the reference's pack_large_primes,
+/// src/host_math.cpp:186-209), padded by @p pad entries past the last prime
+/// (gaps 0, bases the last prime). Throws when a half gap exceeds 255.
+struct PackedPrimes {
+    std::vector<uint32_t> base;
+    std::vector<uint8_t> gap;
+    uint32_t n = 0;
+};
+PackedPrimes pack_primes(const uint32_t* p, uint32_t n, uint32_t group, uint32_t pad);
+
+/// Prime j of a packed stream (host mirror of the device decode).
+uint32_t packed_prime_at(const PackedPrimes& pk, uint32_t j, uint32_t group);
+
 /// Deterministic Miller-Rabin (bases 2..37): exact for every uint64_t.
 bool is_prime_u64(uint64_t n);
 

@@ -288,8 +288,9 @@ struct Engine::State {
     std::optional<Ctx> ctx;
     gbh_ctx* c = nullptr;
     int cus = 256;
-    Buf d_win;                           // window primes, then the large primes (one table)
-    uint32_t* d_large = nullptr;         // the large primes inside d_win
+    Buf d_win;                           // window primes, then the large primes below 2^21 (one table)
+    Buf d_pbase, d_pgap;                 // every large prime as the packed stream (gbh_pstream)
+    gbh_pstream large{};                 // the harvest's view of it
     uint64_t win_p_long = 0;             // long runs' window bound (GB_WIN_P_LONG)
     Slots own;                           // the batch buffers of plain runs
     Candidates cand;
@@ -343,10 +344,23 @@ struct Engine::State {
         module_s = since(tm);
         const auto tu = clock_::now();
         const uint32_t nwin = win_hi - win_lo, nlarge = uint32_t(base.size()) - win_hi;
-        d_win.ensure(c, size_t(nwin + nlarge) * 4 + 4, 0);
-        d_large = d_win.as<uint32_t>() + nwin;
-        if (nwin + nlarge)
-            check(gbh_copy_h2d(c, 0, d_win.p, base.data() + win_lo, size_t(nwin + nlarge) * 4), "upload primes");
+        // the window stage reads the window primes and, on long runs, the
+        // large primes below GB_WIN_P_LONG (<= 2^21) from one u32 table; the
+        // harvest reads the large primes only from the packed stream
+        const uint32_t nlow = uint32_t(std::lower_bound(base.begin() + win_hi, base.end(), uint32_t(1) << 21) -
+                                       (base.begin() + win_hi));
+        d_win.ensure(c, size_t(nwin + nlow) * 4 + 4, 0);
+        if (nwin + nlow)
+            check(gbh_copy_h2d(c, 0, d_win.p, base.data() + win_lo, size_t(nwin + nlow) * 4), "upload primes");
+        if (nlarge) {
+            PackedPrimes pk = pack_primes(base.data() + win_hi, nlarge, GBH_PS_GROUP, GBH_PS_PAD);
+            d_pbase.ensure(c, pk.base.size() * 4, 0);
+            d_pgap.ensure(c, pk.gap.size(), 0);
+            check(gbh_copy_h2d(c, 0, d_pbase.p, pk.base.data(), pk.base.size() * 4), "upload packed primes");
+            check(gbh_copy_h2d(c, 0, d_pgap.p, pk.gap.data(), pk.gap.size()), "upload packed primes");
+            check(gbh_stream_sync(c, 0), "upload sync");   // the host vectors are freed next
+            large = gbh_pstream{d_pbase.as<uint32_t>(), static_cast<const uint8_t*>(d_pgap.p), nlarge, 0};
+        }
         own.init(c, kbatch);
         check(gbh_stream_sync(c, 0), "upload sync");
         upload_s = since(tu);
@@ -578,6 +592,11 @@ private:
     // the compacted path pays for its registers only when it serves a
     // quarter of the large primes or more
     bool few_path() const { return nfew_ < nlarge_ && 4ull * (nlarge_ - nfew_) >= nlarge_; }
+    // a prime index rounded up to a group of the packed stream (at most nlarge_)
+    uint32_t ps_up(uint32_t j) const
+    {
+        return std::min<uint32_t>(nlarge_, (j + GBH_PS_GROUP - 1) / GBH_PS_GROUP * GBH_PS_GROUP);
+    }
     // the first chunk boundary above a: the wheel / bitmap / band /
     // random-read boundaries (chunks are all of one kind)
     uint32_t next_cut(uint32_t a, uint32_t end) const
@@ -673,6 +692,9 @@ void Pipeline::plan_bitmap(const PlanBounds& b)
         if (tc <= 8 * t) t = tc;
     }
     nwheel_ = uint32_t(std::lower_bound(lp, lp + nlarge_, uint32_t(std::min<uint64_t>(t, UINT32_MAX))) - lp);
+    // every boundary between prime kinds at a group of the packed stream
+    // (harvest chunks start there): t rises to the next group's first prime
+    nwheel_ = ps_up(nwheel_);
     if (nwheel_ == nlarge_) return;
     // a band of primes walks the wheel instead of reading the bitmap: those
     // whose walk lists fewer than GB_HL_WHEEL_HITS multiples per batch run
@@ -689,8 +711,8 @@ void Pipeline::plan_bitmap(const PlanBounds& b)
         // compacted path scans them: at most four multipliers each)
         const double th2 = std::min(std::sqrt(q * std::log(std::sqrt(std::max(q, 4.0))) / 1920.0), qspan / 4);
         const auto at = [&](double x) {
-            return uint32_t(std::lower_bound(lp + nwheel_, lp + nlarge_,
-                                             uint32_t(std::min(std::max(x, 1.0), double(UINT32_MAX)))) - lp);
+            return ps_up(uint32_t(std::lower_bound(lp + nwheel_, lp + nlarge_,
+                                                   uint32_t(std::min(std::max(x, 1.0), double(UINT32_MAX)))) - lp));
         };
         nrand_ = at(th2);
         if (th < th2 && th < double(UINT32_MAX)) {
@@ -767,7 +789,7 @@ void Pipeline::plan(const PlanBounds& pb)
     if (!multi_ && nseg >= kLongBatch && nwin_ == E.win_hi - E.win_lo && nlarge_) {
         const uint32_t* lp = E.base.data() + E.win_hi;
         const uint32_t n = uint32_t(std::lower_bound(lp, lp + nlarge_, uint32_t(E.win_p_long)) - lp);
-        nx_ = std::min(n, nwheel_);
+        nx_ = std::min(n, nwheel_) / GBH_PS_GROUP * GBH_PS_GROUP;   // the harvest's first chunk at a group
     }
     // the largest segment: the merged last one spans below 2 x span
     const uint64_t qodds_max = (2 * span_ + MAX_GB_PRIME) / 2 + 2;
@@ -961,8 +983,8 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
             const auto first_above = [&](double x) {
                 return uint32_t(std::upper_bound(lp, lp + npmax, uint32_t(std::min(x, 4294967295.0))) - lp);
             };
-            nfew_ = first_above(span_q / 4);
-            nsingle_ = first_above(span_q);
+            nfew_ = std::min(ps_up(first_above(span_q / 4)), npmax);
+            nsingle_ = std::min(ps_up(first_above(span_q)), npmax);
         }
         const auto prime_cost = [&](uint32_t j) {
             return on_wheel(j) ? kWheelCost : random_reads(j) ? kRandCost : j >= nsingle_ ? kPrimeCost
@@ -1020,7 +1042,9 @@ void Pipeline::plan_chunks(const std::vector<RunCost>& runs_in, uint32_t nbb, do
         size_t ai = 0;
         while (j < npmax) {
             while (rs[ai].np <= j) ++ai;   // runs [ai, nr) serve prime j
-            const uint32_t hi = next_cut(j, std::min({npmax, j + GBH_HL_CHUNK, rs[ai].np}));
+            // (a run's last prime ends a chunk at the next group: the
+            // kernel bounds every run's primes itself)
+            const uint32_t hi = next_cut(j, std::min({npmax, j + GBH_HL_CHUNK, ps_up(rs[ai].np)}));
             const double nact = double(nr - ai), bits = suf_bits[ai], bmax = suf_max[ai], q = suf_qlo[ai];
             const double pc = prime_cost(j);
             const double hf = hit_factor(j, q);
@@ -1350,7 +1374,9 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
                 const double ix = E.inv_at(x), im = E.inv_at(m), iy = E.inv_at(y);
                 words += double(m - x) + lq * (im - ix) / 240 + (8.0 / 30) * lq * (iy - im);
             }
-            if (!force_count_) agg_.harvest_bytes += np > nx_ ? 4.0 * (np - nx_) + 8.0 * words + 4.0 * hits : 0.0;
+            // the packed primes: one byte each and a 4-byte base per group
+            if (!force_count_)
+                agg_.harvest_bytes += np > nx_ ? (1.0 + 4.0 / GBH_PS_GROUP) * (np - nx_) + 8.0 * words + 4.0 * hits : 0.0;
             bits_all += bits;
             hits_all += hits;
             win_prime_bytes += 4.0 * run.nseg * 8 * (r.nwin + std::min(nx_, r.nlarge));
@@ -1387,7 +1413,7 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         // direct form needs it only for the partition (zeroed on the side
         // stream, below) and writes every region fill count itself
         if (!direct) check(gbh_zero(c, ms, B.hit_n.p, size_t(nseg) * 8 * nwin * 4), "zero hit counts");
-        gbh_harvest_args h{b, E.d_large, B.chunks.as<uint32_t>(), dr, nruns,
+        gbh_harvest_args h{b, E.large, B.chunks.as<uint32_t>(), dr, nruns,
                            run_max, nchunks_, cap_, t_hl_, t_hi_, t_hi2_, direct ? 0u : stage_,
                            mb_ ? mb_->bits : nullptr,
                            mb_ ? mb_->m_base : 0, mb_ ? mb_->m_span : 0, B.hits.as<uint32_t>(),
@@ -1509,7 +1535,7 @@ void Pipeline::enqueue_verify(uint32_t s, const gbh_batch& b, gbh_segout* out, i
         check(gbh_launch_setup(c, ms, &a), "verify setup");
     }
     check(gbh_zero(c, ms, B.vhit_n.p, size_t(8) * b.nwin * 4), "verify zero");
-    const gbh_harvest_args h{vb, E.d_large, B.vchunks.as<uint32_t>(), nullptr, 0, 0, nvchunks_, cap_v_,
+    const gbh_harvest_args h{vb, E.large, B.vchunks.as<uint32_t>(), nullptr, 0, 0, nvchunks_, cap_v_,
                              UINT32_MAX, UINT32_MAX, UINT32_MAX, 0, nullptr, 0, 0, B.vhits.as<uint32_t>(), B.vhit_n.as<uint32_t>(),
                              nullptr, nullptr, nullptr, &out[s].err, nullptr, 0, 0, 0, 0};
     check(gbh_launch_harvest(c, ms, &h), "verify harvest");

@@ -158,6 +158,24 @@ typedef struct {
  * takes the primes [chunks[g], chunks[g + 1]). The batch's q range must stay
  * below 2^33, and nseg * 8 * nwin <= GBH_HL_BUCKETS. */
 #define GBH_HL_BUCKETS 12288
+/* The large primes as a packed stream, one byte per prime (the reference's
+ * pack_large_primes, src/host_math.cpp:186-209, decoded on the device like
+ * its decode_large, kernels/goldbach.cl:1678-1716; here in groups of one
+ * wave): prime j = base[j / 64] + 2 * (gap[64 (j / 64) + 1] + ... + gap[j]),
+ * gap[j] = half the gap below prime j for j not a multiple of 64 and 0 at
+ * every group's first prime. Both arrays are padded past the last prime by
+ * GBH_PS_PAD entries (gaps 0, bases the last prime), so a wave may decode a
+ * whole group of indices beyond n; base[g] for every group g holding a prime
+ * is at least every prime of group g - 1. The harvest reads only this stream
+ * and takes every chunk of primes from a multiple of 64 (err bit 0 else). */
+#define GBH_PS_GROUP 64
+#define GBH_PS_PAD 4096
+typedef struct {
+    const uint32_t* base;          /* [ceil((n + GBH_PS_PAD) / 64)] */
+    const uint8_t* gap;            /* [64 * ceil((n + GBH_PS_PAD) / 64)] */
+    uint32_t n;                    /* primes in the stream */
+    uint32_t pad0;
+} gbh_pstream;
 #define GBH_HL_CHUNK 65536     /* primes per harvest work-group, at most */
 #define GBH_HL_STAGE 4096      /* staged entries per harvest work-group, at most */
 /* One run of a batch: segments [seg0, seg0 + nseg), consecutive segments of
@@ -173,8 +191,8 @@ typedef struct {
 } gbh_hl_run;
 typedef struct {
     gbh_batch b;
-    const uint32_t* primes;        /* large primes (device), ascending */
-    const uint32_t* chunks;        /* [nchunks + 1] prime index boundaries */
+    gbh_pstream primes;            /* large primes (device), ascending, packed */
+    const uint32_t* chunks;        /* [nchunks + 1] prime index boundaries, each chunk's first a multiple of 64 */
     const gbh_hl_run* runs;        /* device array [nruns]; nullptr: one run of the whole batch, every prime */
     uint32_t nruns;
     uint32_t run_nseg_max;         /* the largest run's nseg (sizes the work-group's counters) */

@@ -117,6 +117,39 @@ void test_primes()
     CHECK(gb::cpu_min_goldbach_prime(4, c, 4, 11) == 2 && gb::cpu_min_goldbach_prime(98, c, 4, 11) == 19);
 }
 
+// ---------------------------------------------------------------- packed large primes
+void test_packed_primes()
+{
+    // every prime of a packed stream decodes back, across group edges,
+    // including the largest prime gap below 2^32 (336 after 3842610773)
+    std::vector<uint32_t> p;
+    for (uint64_t x = 3842600001ull; x < 3842620000ull; x += 2)
+        if (gb::is_prime_u64(x)) p.push_back(uint32_t(x));
+    const auto it = std::find(p.begin(), p.end(), 3842610773u);
+    CHECK(it != p.end() && it + 1 != p.end() && it[1] - it[0] == 336);
+    for (uint32_t group : {64u, 16u}) {
+        const gb::PackedPrimes pk = gb::pack_primes(p.data(), uint32_t(p.size()), group, 4096);
+        CHECK(pk.n == p.size() && pk.gap.size() == pk.base.size() * group);
+        CHECK(pk.base.size() == (p.size() + 4096 + group - 1) / group);
+        bool same = true;
+        for (uint32_t j = 0; j < p.size(); ++j) same = same && gb::packed_prime_at(pk, j, group) == p[j];
+        CHECK(same);
+        for (uint32_t j = uint32_t(p.size()); j < pk.gap.size(); ++j) same = same && pk.gap[j] == 0;
+        CHECK(same);
+        // past the last prime: bases hold the last prime, so the base after
+        // any group bounds every prime of it
+        CHECK(pk.base.back() == p.back() && pk.base[(p.size() - 1) / group + 1] >= p.back());
+        for (uint32_t g = 0; g < p.size() / group; ++g) same = same && pk.gap[g * group] == 0 && pk.base[g] == p[g * group];
+        CHECK(same);
+    }
+    // a half gap of 255 fits, 256 does not; gaps must be even and ascending
+    const uint32_t fit[] = {1000003, 1000003 + 510}, wide[] = {1000003, 1000003 + 512}, odd[] = {5, 8};
+    CHECK(gb::packed_prime_at(gb::pack_primes(fit, 2, 64, 0), 1, 64) == 1000003 + 510);
+    CHECK(throws([&] { gb::pack_primes(wide, 2, 64, 0); }));
+    CHECK(throws([&] { gb::pack_primes(odd, 2, 64, 0); }));
+    CHECK(throws([&] { gb::pack_primes(fit, 2, 48, 0); }));
+}
+
 // ---------------------------------------------------------------- prime cache
 void test_prime_cache()
 {
@@ -384,6 +417,7 @@ int main()
     g_dir = tmpl;
     test_textio();
     test_primes();
+    test_packed_primes();
     test_prime_cache();
     test_ridges();
     test_stats();
