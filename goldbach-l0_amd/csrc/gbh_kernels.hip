@@ -83,20 +83,49 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 // one per lane. jA, jB: the first index of a group each (multiples of 64,
 // uniform over the wave, which must be whole). A group's half-gaps sum
 // below 63 * 256 < 2^16, so both groups scan in one word.
+// Exclusive prefix over the wave (every lane active) of v; the total in *tot.
+__device__ __forceinline__ uint32_t wave_excl_scan_dpp(uint32_t v, uint32_t* tot)
+{
+    const uint32_t x = wave_incl_scan(v);
+    *tot = (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+    return x - v;
+}
+
+// In two steps, so the loads can be in flight while other work is done:
+// ps_load2 issues them (the lane's two gaps in one word, the two bases),
+// ps_finish2 scans.
+struct PsPair {
+    uint32_t gaps, bA, bB;
+};
+// The group bases through the constant address space: the stream is never
+// written while a kernel reads it, so a uniform index is a scalar load.
+typedef const __attribute__((address_space(4))) uint32_t* ConstU32;
+__device__ __forceinline__ uint32_t ps_base(const gbh_pstream& ps, uint32_t g)
+{
+    return ((ConstU32)ps.base)[g];
+}
+__device__ __forceinline__ PsPair ps_load2(const gbh_pstream& ps, uint32_t jA, uint32_t jB, uint32_t lane)
+{
+    const uint32_t gA = __builtin_amdgcn_readfirstlane(jA >> 6), gB = __builtin_amdgcn_readfirstlane(jB >> 6);
+    return PsPair{(uint32_t)ps.gap[(size_t)gA * 64 + lane] | ((uint32_t)ps.gap[(size_t)gB * 64 + lane] << 16),
+                  ps_base(ps, gA), ps_base(ps, gB)};
+}
+__device__ __forceinline__ void ps_finish2(const PsPair& x, uint32_t& pA, uint32_t& pB)
+{
+    const uint32_t v = wave_incl_scan(x.gaps);
+    pA = x.bA + 2 * (v & 0xFFFFu);
+    pB = x.bB + 2 * (v >> 16);
+}
 __device__ __forceinline__ void ps_decode2(const gbh_pstream& ps, uint32_t jA, uint32_t jB, uint32_t lane,
                                            uint32_t& pA, uint32_t& pB)
 {
-    const uint32_t gA = __builtin_amdgcn_readfirstlane(jA >> 6), gB = __builtin_amdgcn_readfirstlane(jB >> 6);
-    const uint32_t v = wave_incl_scan((uint32_t)ps.gap[(size_t)gA * 64 + lane] |
-                                      ((uint32_t)ps.gap[(size_t)gB * 64 + lane] << 16));
-    pA = ps.base[gA] + 2 * (v & 0xFFFFu);
-    pB = ps.base[gB] + 2 * (v >> 16);
+    ps_finish2(ps_load2(ps, jA, jB, lane), pA, pB);
 }
 
 __device__ __forceinline__ uint32_t ps_decode1(const gbh_pstream& ps, uint32_t jA, uint32_t lane)
 {
     const uint32_t gA = __builtin_amdgcn_readfirstlane(jA >> 6);
-    return ps.base[gA] + 2 * wave_incl_scan((uint32_t)ps.gap[(size_t)gA * 64 + lane]);
+    return ps_base(ps, gA) + 2 * wave_incl_scan((uint32_t)ps.gap[(size_t)gA * 64 + lane]);
 }
 
 __device__ __forceinline__ uint64_t* seg_map(const gbh_batch& b, uint32_t s)
@@ -552,7 +581,7 @@ __device__ __forceinline__ void bm_group_q(const HlCtx& h, double qb0d, const gb
                 if (w == kk.w1) x &= kk.hi_mask;
             }
             uint32_t tot;
-            const uint32_t off = wave_excl_scan((uint32_t)__popcll(x), &tot);
+            const uint32_t off = wave_excl_scan_dpp((uint32_t)__popcll(x), &tot);
             if (tot) {
                 if (n + tot > kHitQ) hitq_flush(h, df, dl, dq, cnt, err, qd, qc, n, lane);
                 if (x) {
@@ -719,13 +748,14 @@ struct FewRun {
     uint32_t span32;                   // min(span, 2^32 - 1)
 };
 
-// Scans one prime (pu = 0: an idle lane): queues the multiples in the span
-// whose q is a unit mod 30, at queue[*n ...]; *n is wave-uniform.
+// Scans one prime pc (a prime of the packed stream, at least the chunk's
+// first; on = false: an idle lane, whose prime lies beyond the run's):
+// queues the multiples in the span whose q is a unit mod 30, at
+// queue[*n ...]; *n is wave-uniform.
 template <uint32_t kM, bool kWide>
-__device__ __forceinline__ void few_scan(const HlCtx& h, const FewRun& f, uint32_t pu, FewCand* queue, uint32_t& n,
-                                         bool& bad)
+__device__ __forceinline__ void few_scan(const HlCtx& h, const FewRun& f, uint32_t pc, bool on, FewCand* queue,
+                                         uint32_t& n, bool& bad)
 {
-    const uint32_t pc = pu | (pu == 0);
     const double pd = (double)pc;
     double inv = (double)__builtin_amdgcn_rcpf((float)pc);
     inv = __builtin_fma(inv, __builtin_fma(-pd, inv, 1.0), inv);
@@ -737,13 +767,13 @@ __device__ __forceinline__ void few_scan(const HlCtx& h, const FewRun& f, uint32
         uint64_t r64 = h.qb0 - m_est * pc;
         step = r64 >= pc;
         r64 -= step ? pc : 0u;
-        bad = bad || (pu && r64 >= pc);   // an idle lane's quotient is meaningless
+        bad = bad || (on && r64 >= pc);   // an idle lane's quotient is not used
         r = (uint32_t)r64;
     } else {
         r = f.qb0_lo - (uint32_t)tb * pc;   // exact mod 2^32: the true value is below 2p < 2^32
         step = r >= pc;
         r -= step ? pc : 0u;
-        bad = bad || (pu && r >= pc);
+        bad = bad || (on && r >= pc);
     }
     const uint32_t r0 = r ? pc - r : 0u;   // the first multiple at or above qb0 is qb0 + r0
     const uint32_t adj = (uint32_t)step + (r != 0);
@@ -754,7 +784,7 @@ __device__ __forceinline__ void few_scan(const HlCtx& h, const FewRun& f, uint32
     for (uint32_t k = 0; k < kM; ++k) {
         const uint64_t off = (uint64_t)r0 + (uint64_t)k * pc;
         const bool in_span = kM == 1 ? r0 <= f.span32 : off <= f.span;
-        const bool cand = pu && in_span && ((kUnitMask >> q30) & 1u);
+        const bool cand = on && in_span && ((kUnitMask >> q30) & 1u);
         const uint64_t bal = __ballot(cand);
         if (bal) {                             // wave-uniform
             if (cand) {
@@ -787,15 +817,10 @@ __device__ __forceinline__ void few_chunk(const HlCtx& h, const FewRun& f, const
     const uint32_t lane = tid & 63, wv = tid >> 6;
     const uint32_t rounds = (je - j0 + 256 * kR - 1) / (256 * kR);
     bool bad = false, outside = false;
-    uint32_t p[kR];
+    uint32_t p[kR];                    // this round's primes (lanes past je: idle, their primes unused)
     const uint32_t jw = j0 + (tid & ~63u);
 #pragma unroll
     for (uint32_t u = 0; u < kR; u += 2) ps_decode2(ps, jw + u * 256, jw + (u + 1) * 256, lane, p[u], p[u + 1]);
-#pragma unroll
-    for (uint32_t u = 0; u < kR; ++u) {
-        const uint32_t j = j0 + u * 256 + tid;
-        p[u] = j < je ? p[u] : 0u;
-    }
     uint32_t prev_n = 0;
     // the last pass only resolves the last round's queue
     for (uint32_t r = 0; r <= rounds; ++r) {
@@ -839,17 +864,16 @@ __device__ __forceinline__ void few_chunk(const HlCtx& h, const FewRun& f, const
         uint32_t n = 0;
         if (r < rounds) {
             FewCand* q = L.q[wv];
-#pragma unroll
-            for (uint32_t u = 0; u < kR; ++u) few_scan<kM, kWide>(h, f, p[u], q, n, bad);
+            // the next round's primes: loads in flight during this round's scan
             const uint32_t jn = jw + (r + 1) * 256 * kR;
+            PsPair nx[kR / 2];
 #pragma unroll
-            for (uint32_t u = 0; u < kR; u += 2)
-                ps_decode2(ps, jn + u * 256, jn + (u + 1) * 256, lane, p[u], p[u + 1]);
+            for (uint32_t u = 0; u < kR; u += 2) nx[u / 2] = ps_load2(ps, jn + u * 256, jn + (u + 1) * 256, lane);
+            const uint32_t jr = j0 + r * 256 * kR + tid;
 #pragma unroll
-            for (uint32_t u = 0; u < kR; ++u) {
-                const uint32_t j = j0 + (r + 1) * 256 * kR + u * 256 + tid;
-                p[u] = j < je ? p[u] : 0u;
-            }
+            for (uint32_t u = 0; u < kR; ++u) few_scan<kM, kWide>(h, f, p[u], jr + u * 256 < je, q, n, bad);
+#pragma unroll
+            for (uint32_t u = 0; u < kR; u += 2) ps_finish2(nx[u / 2], p[u], p[u + 1]);
         }
 #pragma unroll
         for (uint32_t x = 0; x < kD; ++x) {
