@@ -404,6 +404,9 @@ def measure_batched(lib, steps=3, warmup=1):
             "identical_to_plain": identical, "parity": identical == len(ranges),
             "kernel_ms": {n: 1e3 * v[0] / k for n, v in kern.items()},
             "kernel_launches": {n: v[1] / k for n, v in kern.items()}, "bitmap_prepare_s": prep_s,
+            # batches launched as one captured HIP graph per pass over the ranges (GB_GRAPH)
+            "graph_launches": (tot[-1]["graph_launches"] - tot[0]["graph_launches"]) / (k - 1) if k > 1 else
+                              tot[-1]["graph_launches"],
             "workload": "256 ranges, seed 1, starts log-uniform over [1e9, 5e18], lengths log-uniform over "
                         "[1e5, 4.675e8] integers (tests/golden/cfg4_ranges.txt)"}
 

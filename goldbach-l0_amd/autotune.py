@@ -13,6 +13,8 @@ passes), the choices that shape the pipeline on a sample workload:
     GB_OVERLAP  sieve phase beside the harvest on a second stream
     GB_HL_WHEEL_HITS  wheel walk instead of random bitmap reads below this
                 many hits per prime
+    GB_GRAPH    each batch's launches as one captured HIP graph, or issued
+                one by one
 
 Each candidate runs the sample range on a fresh engine (one warmup, then
 the best of `--reps` timed runs; evens per second of wall time). Every choice
@@ -50,9 +52,10 @@ SPACE = {
     "GB_HL_MODE": ["auto", "counted"],
     "GB_OVERLAP": ["1", "0"],
     "GB_HL_WHEEL_HITS": ["0", "2", "4", "8"],
+    "GB_GRAPH": ["0", "1"],
 }
 DEFAULT = {"GB_WIN_KB": "0", "GB_WIN_P": "524288", "GB_WIN_P_LONG": "1048576", "GB_KA": "96", "GB_BATCH": "16",
-           "GB_HL_MODE": "auto", "GB_OVERLAP": "1", "GB_HL_WHEEL_HITS": "2"}
+           "GB_HL_MODE": "auto", "GB_OVERLAP": "1", "GB_HL_WHEEL_HITS": "2", "GB_GRAPH": "0"}
 
 
 def band_of(limit: int):

@@ -29,6 +29,10 @@ struct gbh_ctx {
 struct gbh_event {
     hipEvent_t ev = nullptr;
 };
+struct gbh_graph {
+    hipGraphExec_t exec = nullptr;
+    uint64_t instantiated = 0, updated = 0;
+};
 
 namespace {
 
@@ -261,6 +265,74 @@ int gbh_event_elapsed_ms(gbh_ctx*, gbh_event* a, gbh_event* b, float* ms)
 int gbh_stream_wait(gbh_ctx* c, int s, gbh_event* ev)
 {
     GBH_HIP(hipStreamWaitEvent(S(c, s), ev->ev, 0));
+    return GBH_OK;
+}
+
+int gbh_graph_create(gbh_ctx*, gbh_graph** g)
+{
+    *g = new gbh_graph;
+    return GBH_OK;
+}
+
+int gbh_graph_destroy(gbh_ctx*, gbh_graph* g)
+{
+    if (!g) return GBH_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    delete g;
+    return GBH_OK;
+}
+
+int gbh_graph_begin(gbh_ctx* c, int s)
+{
+    GBH_HIP(hipStreamBeginCapture(S(c, s), hipStreamCaptureModeRelaxed));
+    return GBH_OK;
+}
+
+int gbh_graph_end(gbh_ctx* c, int s, gbh_graph* g)
+{
+    hipGraph_t graph = nullptr;
+    GBH_HIP(hipStreamEndCapture(S(c, s), &graph));
+    bool ok = false;
+    if (g->exec) {
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        hipGraphNode_t bad = nullptr;
+        ok = hipGraphExecUpdate(g->exec, graph, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+        if (ok) {
+            ++g->updated;
+        } else {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(g->exec);
+            g->exec = nullptr;
+        }
+    }
+    if (!ok) {
+        const hipError_t e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            g->exec = nullptr;
+            (void)hipGraphDestroy(graph);
+            return hip_fail(e, "hipGraphInstantiate");
+        }
+        ++g->instantiated;
+    }
+    const hipError_t e = hipGraphLaunch(g->exec, S(c, s));
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    return GBH_OK;
+}
+
+int gbh_event_record_timed(gbh_ctx* c, int s, gbh_event* ev)
+{
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    GBH_HIP(hipStreamIsCapturing(S(c, s), &st));
+    if (st == hipStreamCaptureStatusActive) GBH_HIP(hipEventRecordWithFlags(ev->ev, S(c, s), hipEventRecordExternal));
+    else GBH_HIP(hipEventRecord(ev->ev, S(c, s)));
+    return GBH_OK;
+}
+
+int gbh_graph_stats(gbh_ctx*, const gbh_graph* g, uint64_t* instantiated, uint64_t* updated)
+{
+    *instantiated = g->instantiated;
+    *updated = g->updated;
     return GBH_OK;
 }
 

@@ -59,6 +59,7 @@ void fill(const gb::RunStats& s, gba_run_result* out)
             out->window_bytes = s.window_bytes;
             out->host_plan_seconds = s.host_plan_seconds;
             out->harvest_reruns = s.harvest_reruns;
+            out->graph_launches = s.graph_launches;
         }
 }
 }  // namespace

@@ -19,7 +19,7 @@ std::map<std::string, const std::string*>* g_tuned = new std::map<std::string, c
 std::set<std::string>* g_values = new std::set<std::string>();
 
 constexpr const char* kKnobs[] = {"GB_WIN_KB", "GB_WIN_P", "GB_WIN_P_LONG", "GB_KA", "GB_BATCH", "GB_HL_MODE",
-                                  "GB_OVERLAP", "GB_HL_WHEEL_HITS"};
+                                  "GB_OVERLAP", "GB_HL_WHEEL_HITS", "GB_GRAPH"};
 
 bool known(const std::string& k)
 {

@@ -87,6 +87,16 @@ struct Event {
     ~Event() { if (e) gbh_event_destroy(c, e); }
 };
 
+struct Graph {
+    gbh_ctx* c = nullptr;
+    gbh_graph* g = nullptr;
+    Graph() = default;
+    Graph(const Graph&) = delete;
+    Graph& operator=(const Graph&) = delete;
+    void create(gbh_ctx* ctx) { c = ctx; check(gbh_graph_create(c, &g), "gbh_graph_create"); }
+    ~Graph() { if (g) gbh_graph_destroy(c, g); }
+};
+
 double elapsed_s(gbh_ctx* c, const Event& a, const Event& b)
 {
     float ms = 0;
@@ -226,6 +236,11 @@ struct Slots {
     uint32_t out_sel[2] = {0, 0};             // the block the slot's next batch takes
     Buf h_out[2], h_tab[2], h_chunks[2], h_rdesc[2];
     Event ev[2][EV_N];
+    // captured batches (graph mode): the events above order the streams
+    // inside the graph, these twins are its timestamp nodes; one graph per
+    // slot, its executable form updated in place batch after batch
+    Event evt[2][EV_N];
+    Graph graph[2];
     uint64_t slot_plan[2] = {~0ull, ~0ull};   // the chunk plan each slot's device tables hold
     uint64_t budget = 0;                      // device bytes the set may hold (0: unbounded)
     // output block k of a slot (its first clean_n segments) and its harvest
@@ -243,6 +258,8 @@ struct Slots {
             h_out[s].ensure(c, sizeof(gbh_segout) * kbatch, 1);
             h_tab[s].ensure(c, tab_layout(kbatch).bytes, 1);
             for (auto& e : ev[s]) e.create(c);
+            for (auto& e : evt[s]) e.create(c);
+            graph[s].create(c);
         }
         for (auto& B : sl) {
             B.tab.ensure(c, tab_layout(kbatch).bytes, 0);
@@ -529,6 +546,7 @@ public:
     {
         phase_trace_ = env::present("GB_PHASE");
         if (kend_ && kend_ <= E.ka) throw std::runtime_error("GB_PHASEB_SLOTS must lie above GB_KA (0: every slot)");
+        graph_ = env::onoff("GB_GRAPH", false);
     }
     /// The large primes' split and the multiplier bitmap for @p b.
     void plan_bitmap(const PlanBounds& b);
@@ -621,6 +639,25 @@ private:
     // GB_OVERLAP=0: one stream for everything (no stage or batch runs
     // beside another; for profiling and as a fallback)
     bool overlap_ = env::onoff("GB_OVERLAP", true);
+    // GB_GRAPH=1: each batch's launches captured as one HIP graph (the
+    // slot's executable graph updated in place) and launched at once; 0
+    // (default): issued one by one. Measured on MI355X with ROCm 7: graphs
+    // 8% slower on a plain run (its harvest waits for the whole capture
+    // instead of starting while the rest is queued) and 4% slower on the
+    // batched ranges (whose next batch is captured while the GPU runs the
+    // one before)
+    bool graph_ = false;
+    bool captured_[2] = {false, false};   // the slot's batch in flight ran as a graph
+    /// Event `id` of the slot on `stream`: in a captured batch its timing
+    /// twin (a timestamp node) first, then the event the streams wait on.
+    void mark(int stream, int slot, int id)
+    {
+        if (captured_[slot]) check(gbh_event_record_timed(E.c, stream, S.evt[slot][id].e), "event");
+        check(gbh_event_record(E.c, stream, S.ev[slot][id].e), "event");
+    }
+    /// The event that timestamps `id` for the slot's batch in flight.
+    const Event& tev(int slot, int id) const { return captured_[slot] ? S.evt[slot][id] : S.ev[slot][id]; }
+
     int main_stream(int slot) const { return overlap_ ? slot : 0; }
     uint64_t region_entries_ = 0;
     uint64_t planned_bytes_ = 0;       // the batch buffers' device bytes planned by plan()
@@ -1345,7 +1382,9 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     gbh_segout* out = S.d_out[slot][sel].as<gbh_segout>();
     uint32_t* hq = B.hq.as<uint32_t>() + 32 * sel;
     auto& ev = S.ev[slot];
-    check(gbh_event_record(c, ms, ev[EV_BEGIN].e), "event");
+    captured_[slot] = graph_;
+    if (graph_) check(gbh_graph_begin(c, ms), "graph capture");
+    mark(ms, slot, EV_BEGIN);
     harvested_[slot] = nlarge_ != 0;
     std::vector<RunCost> costs;
     {   // algorithmic bytes of this batch's harvest and window launches, per run
@@ -1405,7 +1444,7 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     // the window stage's sieve phase (masks, window primes) on the slot's
     // side stream beside the harvest, once the batch tables are uploaded
     const int ss = overlap_ ? 2 + slot : ms;
-    check(gbh_event_record(c, ms, ev[EV_GEOM].e), "event");
+    mark(ms, slot, EV_GEOM);
     const bool direct = nlarge_ && direct_ && !force_count_;
     direct_used_[slot] = direct;
     if (nlarge_) {
@@ -1421,9 +1460,9 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
                            direct ? B.region.as<uint32_t>() : nullptr, direct ? B.fills.as<uint32_t>() : nullptr,
                            &out[0].err, persist_ ? hq : nullptr, persist_wg_,
                            few_path() ? 1u : 0u, zeroed ? 1u : 0u, hit_queue_ && few_path() ? 1u : 0u};
-        check(gbh_event_record(c, ms, ev[EV_HL0].e), "event");
+        mark(ms, slot, EV_HL0);
         check(gbh_launch_harvest(c, ms, &h), "harvest");
-        check(gbh_event_record(c, ms, ev[EV_HL1].e), "event");
+        mark(ms, slot, EV_HL1);
     }
     if (ss != ms) check(gbh_stream_wait(c, ss, ev[EV_GEOM].e), "stream wait");
     // the partition's list counts, zeroed beside the harvest (the main
@@ -1441,7 +1480,7 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         gbh_setup_args a{b, E.d_win.as<uint32_t>(), nwp, 0, B.i0.as<uint32_t>()};
         check(gbh_launch_setup(c, ss, &a), "setup");
     }
-    check(gbh_event_record(c, ss, ev[EV_S0].e), "event");
+    mark(ss, slot, EV_S0);
     {
         const gbh_window_args w{b, E.d_win.as<uint32_t>(), nwp, ncoop, B.i0.as<uint32_t>(), nullptr, nullptr, 0, 1};
         check(gbh_launch_window(c, ss, &w), "window sieve");
@@ -1456,7 +1495,7 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
         S.clean_gen[slot][nx][0] = S.d_out[slot][nx].gen;
         S.clean_gen[slot][nx][1] = B.hq.gen;
     }
-    check(gbh_event_record(c, ss, ev[EV_S1].e), "event");
+    mark(ss, slot, EV_S1);
     if (direct) {
         if (ss != ms) check(gbh_stream_wait(c, ms, ev[EV_S0].e), "stream wait");
         const gbh_partition_args pa{b, B.rdesc.as<uint32_t>(), B.region.as<uint32_t>(), B.fills.as<uint32_t>(),
@@ -1464,14 +1503,14 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
                                     B.hit_n.as<uint32_t>(), &out[0].err, part_cap_, 0};
         check(gbh_launch_partition(c, ms, &pa), "partition");
     }
-    check(gbh_event_record(c, ms, ev[EV_SETUP].e), "event");
+    mark(ms, slot, EV_SETUP);
     if (ss != ms) check(gbh_stream_wait(c, ms, ev[EV_S1].e), "stream wait");
     if (nlarge_) {   // the hit-list phase: every window's list applied to the sieved map
         const gbh_window_args w{b, E.d_win.as<uint32_t>(), nwp, ncoop, B.i0.as<uint32_t>(),
                                 B.hits.as<uint32_t>(), B.hit_n.as<uint32_t>(), cap_, 2};
         check(gbh_launch_window(c, ms, &w), "window hit lists");
     }
-    check(gbh_event_record(c, ms, ev[EV_WINDOW].e), "event");
+    mark(ms, slot, EV_WINDOW);
     count_side_[slot] = ss != ms && check_and_count && nwords;
     if (count_side_[slot]) check(gbh_stream_wait(c, ss, ev[EV_WINDOW].e), "stream wait");
     if (verify_every_ && nlarge_ && check_and_count)
@@ -1489,7 +1528,7 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
             ca.kinds = 2 | 4;
             if (cgroups == 3) ca.class_groups = 5;   // the slower histogram form: more waves for it
             check(gbh_launch_check_a(c, ss, &ca), "check phase A (histogram)");
-            check(gbh_event_record(c, ss, ev[EV_CA].e), "event");
+            mark(ss, slot, EV_CA);
             check(gbh_stream_wait(c, ms, ev[EV_CA].e), "stream wait");
         } else {
             check(gbh_launch_check_a(c, ms, &ca), "check phase A");
@@ -1498,21 +1537,25 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
                            B.ridge_more.as<uint64_t>()};
         check(gbh_launch_check_b(c, ms, &cb), "check phase B");
     }
-    check(gbh_event_record(c, ms, ev[EV_CHECK].e), "event");
+    mark(ms, slot, EV_CHECK);
     if (check_and_count && nwords) {
         // the counter only reads the map and fills its own output fields:
         // on the side stream it runs beside the checker
         const int cs = count_side_[slot] ? ss : ms;
         gbh_count_args ct{b, dcs, nwords, swords_, B.scratch.as<uint32_t>(), out};
         check(gbh_launch_count(c, cs, &ct), "count");
-        check(gbh_event_record(c, cs, ev[EV_COUNT].e), "event");
+        mark(cs, slot, EV_COUNT);
         if (cs != ms) check(gbh_stream_wait(c, ms, ev[EV_COUNT].e), "stream wait");
     } else {
-        check(gbh_event_record(c, ms, ev[EV_COUNT].e), "event");
+        mark(ms, slot, EV_COUNT);
     }
     // the blocks without their leftover lists (fetched in collect when used)
     check(gbh_copy_d2h_rows(c, ms, S.h_out[slot].p, sizeof(gbh_segout), out, sizeof(gbh_segout), GBH_SEGOUT_HEAD,
                             nseg), "copy outputs");
+    if (graph_) {   // the captured batch: one launch
+        check(gbh_graph_end(c, ms, S.graph[slot].g), "graph launch");
+        ++agg_.graph_launches;
+    }
     check(gbh_event_record(c, ms, ev[EV_END].e), "event");
     if (phase_trace_) std::fprintf(stderr, "host: batch queued %.3f ms after its first segment was formed\n",
                                    since(batch[0].r->t_first) * 1e3);
@@ -1583,17 +1626,18 @@ void Pipeline::collect(std::vector<SegJob>& batch, int slot)
     }
     // the sieve phase ran beside the harvest: its own time plus the hit-list
     // phase count as the window stage
-    const double tl = elapsed_s(c, ev[EV_BEGIN], ev[EV_SETUP]);
-    const double ts = elapsed_s(c, ev[EV_S0], ev[EV_S1]) + elapsed_s(c, ev[EV_SETUP], ev[EV_WINDOW]);
+    const auto t = [&](int id) -> const Event& { return tev(slot, id); };
+    const double tl = elapsed_s(c, t(EV_BEGIN), t(EV_SETUP));
+    const double ts = elapsed_s(c, t(EV_S0), t(EV_S1)) + elapsed_s(c, t(EV_SETUP), t(EV_WINDOW));
     if (harvested_[slot]) {
-        agg_.harvest_kernel_seconds += elapsed_s(c, ev[EV_HL0], ev[EV_HL1]);
+        agg_.harvest_kernel_seconds += elapsed_s(c, t(EV_HL0), t(EV_HL1));
         ++agg_.harvest_launches;
     }
     agg_.window_kernel_seconds += ts;
     ++agg_.window_launches;
-    const double tc = elapsed_s(c, ev[EV_WINDOW], ev[EV_CHECK]);
+    const double tc = elapsed_s(c, t(EV_WINDOW), t(EV_CHECK));
     // the counter ran beside the checker (from the window's end) or after it
-    const double tn = elapsed_s(c, ev[count_side_[slot] ? EV_WINDOW : EV_CHECK], ev[EV_COUNT]);
+    const double tn = elapsed_s(c, t(count_side_[slot] ? EV_WINDOW : EV_CHECK), t(EV_COUNT));
     agg_.sieve_seconds += ts;
     agg_.large_seconds += tl;
     agg_.verify_seconds += tc + tn;
@@ -1735,6 +1779,7 @@ void Pipeline::finish_range(Range& r)
         rs.window_kernel_seconds = agg_.window_kernel_seconds;
         rs.harvest_launches = agg_.harvest_launches;
         rs.harvest_reruns = agg_.harvest_reruns;
+        rs.graph_launches = agg_.graph_launches;
         rs.window_launches = agg_.window_launches;
         rs.harvest_bytes = agg_.harvest_bytes;
         rs.window_bytes = agg_.window_bytes;

@@ -49,6 +49,7 @@ struct RunStats {
     double harvest_bytes = 0, window_bytes = 0;
     double host_plan_seconds = 0;   // range resolution, statistics set-up and buffer planning of this run
     uint64_t harvest_reruns = 0;    // batches re-run with counted lists after a direct-harvest region overflow
+    uint64_t graph_launches = 0;    // batches launched as one captured HIP graph (GB_GRAPH)
     std::vector<SegmentStats> per_segment;
 };
 

@@ -42,6 +42,7 @@ typedef struct {
     double harvest_bytes, window_bytes;
     double host_plan_seconds;
     uint64_t harvest_reruns;       /* batches re-run with counted lists after a harvest region overflow */
+    uint64_t graph_launches;       /* batches launched as one captured HIP graph (GB_GRAPH) */
 } gba_run_result;
 
 /* One segment of a run (the reference's SegmentStats, include/verifier.hpp:

@@ -9,6 +9,11 @@
   that range alone (the plain path is pinned to the oracle by
   test_gpu_parity.py), and the 32 shortest pass the comparison with their
   stats_ref fixtures.
+* Graph replays: one window run again and again with GB_GRAPH=1 (the
+  engine's graphs re-captured and updated in place, run after run),
+  between eager runs of the same and of another window, every run checked
+  against the window's stats_ref fixture; a batcher with and without graphs
+  gives identical results.
 """
 import sys
 
@@ -95,3 +100,47 @@ def test_batcher_refuses_ranges_outside_its_bounds(stats_env):
             bt.result(i)
         assert bt.run_all() == [i]
         assert bt.result(i)["stats_on"]
+
+
+def test_graph_replays_match_stats_ref(stats_env, monkeypatch):
+    a, b, ref = compare.load_fixture(GOLDEN / "ref_w4e18s.txt")
+    a2, b2, ref2 = compare.load_fixture(GOLDEN / "ref_w1e12s.txt")
+    span = 700_000
+
+    def run(lo, hi, graph):
+        monkeypatch.setenv("GB_GRAPH", "1" if graph else "0")
+        for f in ("st.state", "st.log", "st.state.gaps"):
+            (stats_env / f).unlink(missing_ok=True)
+        r = eng.run(hi, span, lo)
+        return r, (stats_env / "st.state").read_text()
+
+    with gbamd.Engine(b, span) as eng:
+        plan = [(a, b, ref, True), (a, b, ref, True), (a, b, ref, False), (a, b, ref, True), (a2, b2, ref2, True),
+                (a, b, ref, True), (a2, b2, ref2, False), (a2, b2, ref2, True)]
+        for k, (lo, hi, rf, graph) in enumerate(plan):
+            _say(f"graph replay {k}: [{lo}, {hi}] graph={graph}")
+            r, state = run(lo, hi, graph)
+            assert (r["graph_launches"] > 0) == graph, k
+            assert r["leftovers"] == 0, k
+            bad = compare.compare_ref(r["stats_line"], state, rf)
+            assert not bad, (k, bad)
+
+
+def test_batcher_with_and_without_graphs_agree(stats_env, monkeypatch):
+    ranges = sorted(_cfg4(), key=lambda x: x[1] - x[0])[:48]
+    lo, hi = min(x for x, _ in ranges), max(y for _, y in ranges)
+    out = {}
+    with gbamd.Engine(hi, SEGMENT) as eng:
+        for graph in ("1", "0", "1"):
+            monkeypatch.setenv("GB_GRAPH", graph)
+            with gbamd.Batcher(eng, hi, lo, SEGMENT) as bt:
+                ids = [bt.submit(x, y) for x, y in ranges]
+                assert sorted(bt.run_all()) == ids
+                res = [(bt.result(i)["stats_line"], bt.result(i)["state"]) for i in ids]
+                launches = bt.totals()["graph_launches"]
+            _say(f"batcher GB_GRAPH={graph}: {launches} graph launches")
+            assert (launches > 0) == (graph == "1")
+            if graph in out:
+                assert out[graph] == res
+            out[graph] = res
+        assert out["1"] == out["0"]
