@@ -320,15 +320,6 @@ int gbh_graph_end(gbh_ctx* c, int s, gbh_graph* g)
     return GBH_OK;
 }
 
-int gbh_event_record_timed(gbh_ctx* c, int s, gbh_event* ev)
-{
-    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
-    GBH_HIP(hipStreamIsCapturing(S(c, s), &st));
-    if (st == hipStreamCaptureStatusActive) GBH_HIP(hipEventRecordWithFlags(ev->ev, S(c, s), hipEventRecordExternal));
-    else GBH_HIP(hipEventRecord(ev->ev, S(c, s)));
-    return GBH_OK;
-}
-
 int gbh_graph_stats(gbh_ctx*, const gbh_graph* g, uint64_t* instantiated, uint64_t* updated)
 {
     *instantiated = g->instantiated;

@@ -162,13 +162,25 @@ struct VectorShelf {
 };
 thread_local VectorShelf t_shelf;
 
-void borrow(std::vector<uint64_t>& v, size_t n)
+void take_shelved(std::vector<uint64_t>& v)
 {
     if (!t_shelf.free.empty()) {
         v.swap(t_shelf.free.back());
         t_shelf.free.pop_back();
     }
+}
+
+void borrow(std::vector<uint64_t>& v, size_t n)
+{
+    take_shelved(v);
     v.assign(n, 0);
+}
+
+// v = src, in a shelved vector when one is free
+void borrow_copy(std::vector<uint64_t>& v, const std::vector<uint64_t>& src)
+{
+    take_shelved(v);
+    v.assign(src.begin(), src.end());
 }
 
 void give_back(std::vector<uint64_t>& v)
@@ -231,12 +243,22 @@ void GapBook::absorb(const GapBook& o)
 
 // ---------------------------------------------------------------- Tally
 
-void Tally::size_for(size_t np)
+void Tally::size_for(size_t np, bool by_class)
 {
-    borrow(hist, 15 * np);
-    borrow(tail_cls, 15 * np);
+    if (by_class) {
+        borrow(hist, 15 * np);
+        borrow(tail_cls, 15 * np);
+    }
     tail_flat.assign(np, 0);
     first.assign(np, 0);
+    zero = true;
+}
+
+void Tally::size_by_class(size_t np)
+{
+    if (hist.size() == 15 * np) return;
+    borrow(hist, 15 * np);
+    borrow(tail_cls, 15 * np);
 }
 
 void Tally::clear()
@@ -244,19 +266,27 @@ void Tally::clear()
     evens = sampled = 0;
     std::fill(std::begin(primes), std::end(primes), 0);
     std::fill(std::begin(twins), std::end(twins), 0);
-    for (auto* v : {&hist, &tail_cls, &tail_flat, &first}) std::fill(v->begin(), v->end(), 0);
+    if (!zero)
+        for (auto* v : {&hist, &tail_cls, &tail_flat, &first}) std::fill(v->begin(), v->end(), 0);
+    zero = true;
     beyond.clear();
     ranges.clear();
 }
 
 void Tally::absorb(const Tally& run)
 {
+    zero = false;
     evens += run.evens;
     sampled += run.sampled;
     for (int i = 0; i < 8; ++i) primes[i] += run.primes[i];
     for (int i = 0; i < 3; ++i) twins[i] += run.twins[i];
-    std::transform(hist.begin(), hist.end(), run.hist.begin(), hist.begin(), std::plus<>());
-    std::transform(tail_cls.begin(), tail_cls.end(), run.tail_cls.begin(), tail_cls.begin(), std::plus<>());
+    if (hist.empty()) {   // no class counts held yet (a fresh campaign): the run's
+        borrow_copy(hist, run.hist);
+        borrow_copy(tail_cls, run.tail_cls);
+    } else {
+        std::transform(hist.begin(), hist.end(), run.hist.begin(), hist.begin(), std::plus<>());
+        std::transform(tail_cls.begin(), tail_cls.end(), run.tail_cls.begin(), tail_cls.begin(), std::plus<>());
+    }
     std::transform(tail_flat.begin(), tail_flat.end(), run.tail_flat.begin(), tail_flat.begin(), std::plus<>());
     for (size_t i = 0; i < first.size(); ++i)
         if (run.first[i] && (!first[i] || run.first[i] < first[i])) first[i] = run.first[i];
@@ -274,8 +304,8 @@ GoldbachStats::GoldbachStats(const std::vector<uint32_t>& cands, uint32_t tail_t
     tail0_ = size_t(std::lower_bound(cands.begin(), cands.end(), tail_t) - cands.begin());
     index_of_.assign(cands.back() / 2 + 1, -1);
     for (size_t i = 0; i < np_; ++i) index_of_[cands[i] / 2] = int32_t(i);
-    run_.size_for(np_);
-    camp_.size_for(np_);
+    run_.size_for(np_, true);
+    camp_.size_for(np_, false);   // its class counts only when a state file holds some (read_state) or a run is absorbed
     want_.assign(np_, 0);
     state_path_ = env::path("GB_STATS_STATE", "goldbach_stats.state");
     log_path_ = env::path("GB_STATS_LOG", "goldbach_stats.log");
@@ -320,18 +350,35 @@ void GoldbachStats::count_hit(bool sampled, uint32_t cr, uint32_t pi, uint64_t c
 uint64_t GoldbachStats::add_hist_slots(bool sampled, const uint32_t* counts, size_t stride,
                                        const std::vector<uint16_t>* slot_pi)
 {
+    // count_hit for every slot, in one pass per class
+    run_.zero = false;
     const uint64_t before = run_.sampled;
     for (uint32_t cr = 0; cr < 15; ++cr) {
         const uint32_t* row = counts + size_t(cr) * stride;
         const std::vector<uint16_t>& pis = slot_pi[cr];
-        for (size_t k = 0; k < pis.size(); ++k)
-            if (row[k]) count_hit(sampled, cr, pis[k], row[k]);
+        const size_t n = pis.size();
+        uint64_t* tc = run_.tail_cls.data() + size_t(cr) * np_;
+        if (sampled) {
+            uint64_t* h = run_.hist.data() + size_t(cr) * np_;
+            uint64_t sum = 0;
+            for (size_t k = 0; k < n; ++k) {
+                const uint32_t c = row[k], pi = pis[k];
+                h[pi] += c;
+                sum += c;
+                if (pi >= tail0_) tc[pi] += c;
+            }
+            run_.sampled += sum;
+        } else {
+            for (size_t k = 0; k < n; ++k)
+                if (pis[k] >= tail0_) run_.tail_flat[pis[k]] += row[k];
+        }
     }
     return run_.sampled - before;
 }
 
 void GoldbachStats::add_first(uint64_t seg_start, const uint32_t* first)
 {
+    run_.zero = false;
     for (size_t i = tail0_; i < np_; ++i)
         if (first[i] != ~0u && want_[i]) note_first(uint32_t(i), seg_start + 2 * uint64_t(first[i]));
 }
@@ -345,7 +392,7 @@ void GoldbachStats::add_counts(const uint32_t cnt[8], const uint32_t tw[3])
 void GoldbachStats::begin_gaps(uint64_t start)
 {
     gaps_from_ = start;
-    last_prime_ = 0;   // prev_prime(start - 3), looked up when the first segment arrives
+    last_prime_ = before_start_ = 0;   // prev_prime(start - 3), looked up when the first segment arrives
 }
 
 void GoldbachStats::add_gap_segment(const SegmentGaps& s)
@@ -365,7 +412,7 @@ void GoldbachStats::add_gap_segment(const SegmentGaps& s)
         return;
     }
     const uint64_t first_q = s.q_first + s.pfirst, last_q = s.q_first + s.plast;
-    if (last_prime_ == 0) last_prime_ = prev_prime(gaps_from_ - 3);
+    if (last_prime_ == 0) last_prime_ = before_start_ = prev_prime(gaps_from_ - 3);
     if (s.walked) {
         // the walked histogram must agree with the segment's own figures:
         // one gap fewer than primes, the gaps summing to the span between
@@ -418,6 +465,7 @@ void GoldbachStats::add_gap_segment(const SegmentGaps& s)
 
 void GoldbachStats::add_cpu(uint64_t n, uint64_t p, bool sampled)
 {
+    run_.zero = false;
     const int32_t pi = (p & 1) && p <= cands_.back() ? index_of_[p / 2] : -1;
     if (pi >= 0) {
         count_hit(sampled, uint32_t(n % 30) / 2, uint32_t(pi), 1);
@@ -438,7 +486,17 @@ void GoldbachStats::finish_run(uint64_t start, uint64_t limit, uint64_t evens)
     limit_ = limit;
     run_.evens = evens;
     run_.ranges = {{start, limit}};
-    const auto prime = [](uint64_t q) { return is_prime_u64(q); };
+    // the edge values are asked about more than once: each Miller-Rabin
+    // test is made once
+    std::pair<uint64_t, bool> memo[8];
+    size_t nmemo = 0;
+    const auto prime = [&](uint64_t q) {
+        for (size_t i = 0; i < nmemo; ++i)
+            if (memo[i].first == q) return memo[i].second;
+        const bool r = is_prime_u64(q);
+        if (nmemo < 8) memo[nmemo++] = {q, r};
+        return r;
+    };
     // the device counted primes q in [start - 3, limit - 3]: the run's are
     // [start + 1, limit - 1]
     for (uint64_t q : {start - 3, start - 1})
@@ -453,7 +511,7 @@ void GoldbachStats::finish_run(uint64_t start, uint64_t limit, uint64_t evens)
         if (int64_t(v) < 0) throw std::runtime_error("stats: negative prime count after the edge corrections");
     if (gaps_from_ != start) return;
     GapBook& b = gaps_;
-    if (last_prime_ == 0) last_prime_ = prev_prime(start - 3);   // no segment held a prime
+    if (last_prime_ == 0) last_prime_ = before_start_ = prev_prime(start - 3);   // no segment held a prime
     b.count = run_primes();
     if (limit < start + 2) return;
     // start - 1 lies outside the run: the walked twin gap start - 3 -> start - 1 goes
@@ -467,7 +525,13 @@ void GoldbachStats::finish_run(uint64_t start, uint64_t limit, uint64_t evens)
     // the sum of the gaps telescopes: last prime - the prime before the first
     uint64_t f = (start + 1) | 1;
     while (f <= limit - 1 && !prime(f)) f += 2;
-    if (f <= limit - 1) b.span = prev_prime(limit) - prev_prime(f);
+    if (f <= limit - 1) {
+        // the prime before f: nothing in (start, f) is prime, so it is
+        // start - 1, start - 3 or prev_prime(start - 3), looked up above
+        const uint64_t before_f = prime(start - 1) ? start - 1 : prime(start - 3) ? start - 3 : before_start_;
+        const uint64_t last = prime(limit - 1) ? limit - 1 : prev_prime(limit - 1);
+        b.span = last - before_f;
+    }
 }
 
 uint64_t GoldbachStats::run_primes() const { return std::accumulate(std::begin(run_.primes), std::end(run_.primes), uint64_t(0)); }
@@ -545,51 +609,120 @@ std::vector<uint64_t> GoldbachStats::per_prime(const std::vector<uint64_t>& by_c
 
 std::string GoldbachStats::gap_fields() const
 {
+    using textio::append_dec;
     const GapBook& b = gaps_;
-    std::string s = " gaps=" + std::to_string(b.count) + " gd1=" + dec(b.span) + " gmax=";
-    s += b.top_exact() && b.top_g ? std::to_string(b.top_g) + ":" + std::to_string(b.top_lower) : "-";
-    std::string tails;
-    for (const auto& [g, t] : b.tail) {
-        if (!tails.empty()) tails += ';';
-        tails += std::to_string(g) + ":" + std::to_string(t.count) + ":" + std::to_string(t.first_lower);
+    std::string s;
+    s.reserve(4096);
+    s += " gaps=";
+    append_dec(s, b.count);
+    s += " gd1=";
+    append_dec(s, b.span);
+    s += " gmax=";
+    if (b.top_exact() && b.top_g) {
+        append_dec(s, b.top_g);
+        s += ':';
+        append_dec(s, b.top_lower);
+    } else {
+        s += '-';
     }
-    s += " gtail=" + (tails.empty() ? "-" : tails);
-    s += " gwalk=" + std::to_string(walked_evens_) + ":" + std::to_string(b.walk_count) + " gsmom=" + dec(b.moment[0]) +
-         "," + dec(b.moment[1]) + "," + dec(b.moment[2]) + "," + dec(b.moment[3]);
-    const std::string h = count_list(b.walk_hist);
-    return s + " gsamp=" + (h.empty() ? "-" : h);
+    s += " gtail=";
+    if (b.tail.empty()) s += '-';
+    bool more = false;
+    for (const auto& [g, t] : b.tail) {
+        if (more) s += ';';
+        more = true;
+        append_dec(s, uint64_t(g));
+        s += ':';
+        append_dec(s, t.count);
+        s += ':';
+        append_dec(s, t.first_lower);
+    }
+    s += " gwalk=";
+    append_dec(s, walked_evens_);
+    s += ':';
+    append_dec(s, b.walk_count);
+    s += " gsmom=";
+    for (int k = 0; k < 4; ++k) {
+        if (k) s += ',';
+        append_dec(s, b.moment[k]);
+    }
+    s += " gsamp=";
+    const size_t at = s.size();
+    textio::append_count_list(s, b.walk_hist.data(), b.walk_hist.size());
+    if (s.size() == at) s += '-';
+    return s;
 }
 
 std::string GoldbachStats::log_line(const char* version) const
 {
-    const auto field = [](const char* k, const std::string& v) { return std::string(" ") + k + "=" + v; };
-    const auto csv = [](const uint64_t* v, size_t n) {
-        std::string s;
-        for (size_t i = 0; i < n; ++i) s += (i ? "," : "") + std::to_string(v[i]);
-        return s;
+    using textio::append_dec;
+    std::string line;
+    line.reserve(8192);
+    const auto num = [&](const char* k, uint64_t v) {
+        line += ' ';
+        line += k;
+        line += '=';
+        append_dec(line, v);
     };
-    std::string line = "GBSTATS v2";
-    line += field("version", version);
-    line += field("start", std::to_string(start_)) + field("limit", std::to_string(limit_));
-    line += field("evens", std::to_string(run_.evens)) + field("evens_sampled", std::to_string(run_.sampled));
-    line += field("tail_t", std::to_string(tail_t_)) + field("tail_evens", std::to_string(run_tail()));
-    line += field("pmax", std::to_string(run_pmax())) + field("primes", std::to_string(run_primes()));
-    line += field("twins", std::to_string(run_twins()));
-    line += field("primes30", csv(run_.primes, 8)) + field("twins30", csv(run_.twins, 3));
-    std::string firsts;
+    const auto csv = [&](const char* k, const uint64_t* v, size_t n) {
+        line += ' ';
+        line += k;
+        line += '=';
+        for (size_t i = 0; i < n; ++i) {
+            if (i) line += ',';
+            append_dec(line, v[i]);
+        }
+    };
+    line += "GBSTATS v2 version=";
+    line += version;
+    num("start", start_);
+    num("limit", limit_);
+    num("evens", run_.evens);
+    num("evens_sampled", run_.sampled);
+    num("tail_t", tail_t_);
+    num("tail_evens", run_tail());
+    num("pmax", run_pmax());
+    num("primes", run_primes());
+    num("twins", run_twins());
+    csv("primes30", run_.primes, 8);
+    csv("twins30", run_.twins, 3);
+    line += " new_first=";
+    size_t at = line.size();
     for (size_t i = 0; i < np_; ++i) {
         const uint64_t f = run_.first[i], c = camp_.first[i];
-        if (f && (!c || f < c)) firsts += (firsts.empty() ? "" : ";") + std::to_string(cands_[i]) + ":" + std::to_string(f);
+        if (f && (!c || f < c)) {
+            if (line.size() != at) line += ';';
+            append_dec(line, uint64_t(cands_[i]));
+            line += ':';
+            append_dec(line, f);
+        }
     }
-    line += field("new_first", firsts.empty() ? "-" : firsts);
-    std::string bey;
-    for (const auto& [n, p] : run_.beyond) bey += (bey.empty() ? "" : ";") + std::to_string(p) + ":" + std::to_string(n);
-    line += field("beyond", bey.empty() ? "-" : bey);
+    if (line.size() == at) line += '-';
+    line += " beyond=";
+    at = line.size();
+    for (const auto& [n, p] : run_.beyond) {
+        if (line.size() != at) line += ';';
+        append_dec(line, p);
+        line += ':';
+        append_dec(line, n);
+    }
+    if (line.size() == at) line += '-';
     if (gaps_from_ == start_) line += gap_fields();
-    const std::vector<uint64_t> t = per_prime(run_.tail_cls, &run_.tail_flat);
-    const std::string tl = tail0_ < np_ ? count_list(t.data() + tail0_, np_ - tail0_) : "";
-    line += field("tail", tl.empty() ? "-" : std::to_string(cands_[tail0_]) + ":" + tl);
-    return line + "\n";
+    line += " tail=";
+    at = line.size();
+    if (tail0_ < np_) {
+        std::vector<uint64_t> t(run_.tail_flat.begin() + tail0_, run_.tail_flat.end());
+        for (size_t cr = 0; cr < 15; ++cr)
+            for (size_t i = tail0_; i < np_; ++i) t[i - tail0_] += run_.tail_cls[cr * np_ + i];
+        append_dec(line, uint64_t(cands_[tail0_]));
+        line += ':';
+        const size_t lst = line.size();
+        textio::append_count_list(line, t.data(), t.size());
+        if (line.size() == lst) line.resize(at);   // an empty list: no "p:" head
+    }
+    if (line.size() == at) line += '-';
+    line += '\n';
+    return line;
 }
 
 void GoldbachStats::log_append(const std::string& line) const
@@ -624,28 +757,69 @@ bool GoldbachStats::log_has_gap_row(uint64_t a, uint64_t b) const
 
 std::string GoldbachStats::state_text(const Tally& t) const
 {
-    std::string s =
-        "# goldbach-l0 statistics state v2 (exact; include/gstats.hpp)\n"
-        "# one record per line; count lists are indexed by odd prime (3, 5, 7, ...), an empty entry is 0\n"
-        "# sampled <n mod 30 / 2> <list>: p(n) of every even of the sampled segments\n"
-        "# tailcls <n mod 30 / 2> <list>, tailagg <list>: p(n) >= tail_t (sampled evens by class, the rest)\n"
-        "# first <p> <n>: the campaign's first even with p(n) = p, for p >= tail_t\n";
-    s += "tail_t " + std::to_string(tail_t_) + "\n";
-    for (const auto& [a, b] : t.ranges) s += "range " + std::to_string(a) + " " + std::to_string(b) + "\n";
-    s += "evens " + std::to_string(t.evens) + " " + std::to_string(t.sampled) + "\n";
-    s += "primes30";
-    for (uint64_t v : t.primes) s += " " + std::to_string(v);
+    using textio::append_count_list;
+    using textio::append_dec;
+    std::string s;
+    s.reserve(size_t(16) << 10);
+    s += "# goldbach-l0 statistics state v2 (exact; include/gstats.hpp)\n"
+         "# one record per line; count lists are indexed by odd prime (3, 5, 7, ...), an empty entry is 0\n"
+         "# sampled <n mod 30 / 2> <list>: p(n) of every even of the sampled segments\n"
+         "# tailcls <n mod 30 / 2> <list>, tailagg <list>: p(n) >= tail_t (sampled evens by class, the rest)\n"
+         "# first <p> <n>: the campaign's first even with p(n) = p, for p >= tail_t\n";
+    s += "tail_t ";
+    append_dec(s, uint64_t(tail_t_));
+    s += '\n';
+    for (const auto& [a, b] : t.ranges) {
+        s += "range ";
+        append_dec(s, a);
+        s += ' ';
+        append_dec(s, b);
+        s += '\n';
+    }
+    s += "evens ";
+    append_dec(s, t.evens);
+    s += ' ';
+    append_dec(s, t.sampled);
+    s += "\nprimes30";
+    for (uint64_t v : t.primes) {
+        s += ' ';
+        append_dec(s, v);
+    }
     s += "\ntwins30";
-    for (uint64_t v : t.twins) s += " " + std::to_string(v);
-    s += "\n";
+    for (uint64_t v : t.twins) {
+        s += ' ';
+        append_dec(s, v);
+    }
+    s += '\n';
     for (size_t i = 0; i < np_; ++i)
-        if (t.first[i]) s += "first " + std::to_string(cands_[i]) + " " + std::to_string(t.first[i]) + "\n";
-    for (const auto& [n, p] : t.beyond) s += "beyond " + std::to_string(p) + " " + std::to_string(n) + "\n";
-    for (size_t cr = 0; cr < 15; ++cr)
-        s += "sampled " + std::to_string(cr) + " " + count_list(t.hist.data() + cr * np_, np_) + "\n";
-    for (size_t cr = 0; cr < 15; ++cr)
-        s += "tailcls " + std::to_string(cr) + " " + count_list(t.tail_cls.data() + cr * np_, np_) + "\n";
-    return s + "tailagg " + count_list(t.tail_flat) + "\n";
+        if (t.first[i]) {
+            s += "first ";
+            append_dec(s, uint64_t(cands_[i]));
+            s += ' ';
+            append_dec(s, t.first[i]);
+            s += '\n';
+        }
+    for (const auto& [n, p] : t.beyond) {
+        s += "beyond ";
+        append_dec(s, p);
+        s += ' ';
+        append_dec(s, n);
+        s += '\n';
+    }
+    for (const char* key : {"sampled ", "tailcls "}) {
+        const std::vector<uint64_t>& v = key[0] == 's' ? t.hist : t.tail_cls;
+        for (size_t cr = 0; cr < 15; ++cr) {
+            s += key;
+            append_dec(s, uint64_t(cr));
+            s += ' ';
+            append_count_list(s, v.data() + cr * np_, np_);
+            s += '\n';
+        }
+    }
+    s += "tailagg ";
+    append_count_list(s, t.tail_flat.data(), t.tail_flat.size());
+    s += '\n';
+    return s;
 }
 
 void GoldbachStats::read_state(Tally& t) const
@@ -653,6 +827,8 @@ void GoldbachStats::read_state(Tally& t) const
     t.clear();
     const std::optional<std::string> text = textio::read_file(state_path_);
     if (!text) return;   // a fresh campaign
+    t.zero = false;      // its counts come from the file
+    t.size_by_class(np_);
     const std::string what = "state " + state_path_;
     if (text->empty()) throw std::runtime_error("stats: " + what + " is empty (truncated write?); refusing to merge into it");
     const auto u64 = [](std::string_view w, uint64_t& v) { return textio::parse(w, v); };
@@ -775,22 +951,53 @@ bool GoldbachStats::commit_gap_file()
     f.ranges.push_back({start_, limit_});
     normalize(f.ranges);
     const GapBook& b = f.book;
-    std::string s =
-        "# goldbach-l0 prime gap state v1 (exact; include/gstats.hpp GapStats)\n"
-        "# gaps_range <lo> <hi>: prime gaps recorded for the upper prime in [lo + 1, hi - 1]\n"
-        "# gaps_evens <evens> <walked evens>; gaps <count> <sum of g> (every gap, exact)\n"
-        "# gaps_walk <count> <sums of g, g^2, g^3, g^4>, gaps_samp <histogram by g / 2>: the walked segments\n"
-        "# gaps_max <g> <lower> <1: the largest, 0: the largest seen>; gaps_tail <g> <count> <first lower>: g >= 700\n";
-    for (const auto& [a, c] : f.ranges) s += "gaps_range " + std::to_string(a) + " " + std::to_string(c) + "\n";
-    s += "gaps_evens " + std::to_string(f.evens) + " " + std::to_string(f.walked) + "\n";
-    s += "gaps " + std::to_string(b.count) + " " + dec(b.span) + "\n";
-    s += "gaps_walk " + std::to_string(b.walk_count);
-    for (const u128& m : b.moment) s += " " + dec(m);
-    s += "\ngaps_samp " + count_list(b.walk_hist) + "\n";
-    s += "gaps_max " + std::to_string(b.top_g) + " " + std::to_string(b.top_lower) + (b.top_exact() ? " 1\n" : " 0\n");
-    for (const auto& [g, t] : b.tail)
-        s += "gaps_tail " + std::to_string(g) + " " + std::to_string(t.count) + " " + std::to_string(t.first_lower) + "\n";
-    textio::replace_file(path, s + "end\n");
+    using textio::append_dec;
+    std::string s;
+    s.reserve(8192);
+    s += "# goldbach-l0 prime gap state v1 (exact; include/gstats.hpp GapStats)\n"
+         "# gaps_range <lo> <hi>: prime gaps recorded for the upper prime in [lo + 1, hi - 1]\n"
+         "# gaps_evens <evens> <walked evens>; gaps <count> <sum of g> (every gap, exact)\n"
+         "# gaps_walk <count> <sums of g, g^2, g^3, g^4>, gaps_samp <histogram by g / 2>: the walked segments\n"
+         "# gaps_max <g> <lower> <1: the largest, 0: the largest seen>; gaps_tail <g> <count> <first lower>: g >= 700\n";
+    for (const auto& [a, c] : f.ranges) {
+        s += "gaps_range ";
+        append_dec(s, a);
+        s += ' ';
+        append_dec(s, c);
+        s += '\n';
+    }
+    s += "gaps_evens ";
+    append_dec(s, f.evens);
+    s += ' ';
+    append_dec(s, f.walked);
+    s += "\ngaps ";
+    append_dec(s, b.count);
+    s += ' ';
+    append_dec(s, b.span);
+    s += "\ngaps_walk ";
+    append_dec(s, b.walk_count);
+    for (const u128& m : b.moment) {
+        s += ' ';
+        append_dec(s, m);
+    }
+    s += "\ngaps_samp ";
+    textio::append_count_list(s, b.walk_hist.data(), b.walk_hist.size());
+    s += "\ngaps_max ";
+    append_dec(s, b.top_g);
+    s += ' ';
+    append_dec(s, b.top_lower);
+    s += b.top_exact() ? " 1\n" : " 0\n";
+    for (const auto& [g, t] : b.tail) {
+        s += "gaps_tail ";
+        append_dec(s, uint64_t(g));
+        s += ' ';
+        append_dec(s, t.count);
+        s += ' ';
+        append_dec(s, t.first_lower);
+        s += '\n';
+    }
+    s += "end\n";
+    textio::replace_file(path, s);
     return true;
 }
 
@@ -871,11 +1078,17 @@ void GoldbachStats::write_live(uint64_t start, uint64_t limit, uint64_t done, ui
             pm = p;
             pm_n = n;
         }
+    // per prime, its evens with p(n) >= tail_t (zero below tail0_, where
+    // nothing is counted as tail)
     const uint64_t bounds[3] = {8000, 9000, RECORD_P + 1};
-    const std::vector<uint64_t> t = per_prime(run_.tail_cls, &run_.tail_flat);
-    for (size_t i = 0; i < np_; ++i)
+    uint64_t tail_evens = run_.beyond.size();
+    for (size_t i = tail0_; i < np_; ++i) {
+        uint64_t t = run_.tail_flat[i];
+        for (size_t cr = 0; cr < 15; ++cr) t += run_.tail_cls[cr * np_ + i];
+        tail_evens += t;
         for (int k = 0; k < 3; ++k)
-            if (cands_[i] >= bounds[k]) at_least[k] += t[i];
+            if (cands_[i] >= bounds[k]) at_least[k] += t;
+    }
     at_least[2] += run_.beyond.size();
     char head[512];
     std::snprintf(head, sizeof head,
@@ -884,7 +1097,7 @@ void GoldbachStats::write_live(uint64_t start, uint64_t limit, uint64_t done, ui
                   "\"ge8000\": %llu, \"ge9000\": %llu, \"gt9781\": %llu, \"primes\": %llu, \"twins\": %llu, "
                   "\"events\": [",
                   now, (unsigned long long)start, (unsigned long long)limit, (unsigned long long)done,
-                  (unsigned long long)total, elapsed, tail_t_, (unsigned long long)run_tail(), (unsigned long long)pm,
+                  (unsigned long long)total, elapsed, tail_t_, (unsigned long long)tail_evens, (unsigned long long)pm,
                   (unsigned long long)pm_n, (unsigned long long)at_least[0], (unsigned long long)at_least[1],
                   (unsigned long long)at_least[2], (unsigned long long)run_primes(), (unsigned long long)run_twins());
     std::string json = head;

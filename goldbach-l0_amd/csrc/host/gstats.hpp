@@ -89,8 +89,12 @@ struct Tally {
     std::vector<uint64_t> first;       // [i]: first even with p(n) = p_i (0: none)
     std::vector<std::pair<uint64_t, uint64_t>> beyond;   // (n, p): p(n) not among the candidates
     std::vector<std::pair<uint64_t, uint64_t>> ranges;   // covered [start, limit], merged
+    bool zero = false;                 // the four count vectors are known to be all zero (clear() skips them)
 
-    void size_for(size_t np);
+    /// Zeroed vectors for np candidates; by_class false leaves hist and
+    /// tail_cls empty (size_by_class or absorb sizes them when needed).
+    void size_for(size_t np, bool by_class);
+    void size_by_class(size_t np);
     void clear();
     /// Sums, first-occurrence minima, the union of ranges.
     void absorb(const Tally& run);
@@ -178,6 +182,7 @@ private:
     GapBook gaps_;
     uint64_t gaps_from_ = ~0ull;       // the run start the gaps were begun for
     uint64_t last_prime_ = 0;          // the last prime before the next segment's first
+    uint64_t before_start_ = 0;        // prev_prime(start - 3) of the gap run (set with last_prime_'s first value)
     bool last_walked_ = false;
     uint64_t walked_evens_ = 0;
     uint64_t start_ = 0, limit_ = 0;

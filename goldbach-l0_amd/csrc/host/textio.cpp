@@ -6,6 +6,7 @@
 #include <unistd.h>
 
 #include <cerrno>
+#include <charconv>
 #include <cstring>
 #include <stdexcept>
 
@@ -157,14 +158,39 @@ std::string dec(u128 v)
     return std::string(digits + n, sizeof digits - n);
 }
 
-std::string count_list(const uint64_t* v, size_t n)
+void append_dec(std::string& s, uint64_t v)
+{
+    char buf[20];
+    const auto r = std::to_chars(buf, buf + sizeof buf, v);
+    s.append(buf, size_t(r.ptr - buf));
+}
+
+void append_dec(std::string& s, u128 v)
+{
+    if (v <= UINT64_MAX) return append_dec(s, uint64_t(v));
+    s += dec(v);
+}
+
+void append_count_list(std::string& s, const uint64_t* v, size_t n)
 {
     while (n && !v[n - 1]) --n;
-    std::string s;
+    if (!n) return;
+    // at most 20 digits and a comma per entry: written in place, then cut
+    const size_t at = s.size();
+    s.resize(at + 21 * n);
+    char* const out = s.data() + at;
+    char* p = out;
     for (size_t i = 0; i < n; ++i) {
-        if (i) s.push_back(',');
-        if (v[i]) s += std::to_string(v[i]);
+        if (i) *p++ = ',';
+        if (v[i]) p = std::to_chars(p, p + 20, v[i]).ptr;
     }
+    s.resize(at + size_t(p - out));
+}
+
+std::string count_list(const uint64_t* v, size_t n)
+{
+    std::string s;
+    append_count_list(s, v, n);
     return s;
 }
 

@@ -56,6 +56,12 @@ std::string dec(u128 v);
 std::string count_list(const uint64_t* v, size_t n);
 inline std::string count_list(const std::vector<uint64_t>& v) { return count_list(v.data(), v.size()); }
 
+/// In-place forms for text built piece by piece (no temporaries): the
+/// decimal of @p v, and count_list of v[0, n), appended to @p s.
+void append_dec(std::string& s, uint64_t v);
+void append_dec(std::string& s, u128 v);
+void append_count_list(std::string& s, const uint64_t* v, size_t n);
+
 /// The inverse of count_list into @p n entries; false on a malformed entry
 /// or more than n of them.
 bool parse_count_list(std::string_view s, uint64_t* out, size_t n);

@@ -236,10 +236,11 @@ struct Slots {
     uint32_t out_sel[2] = {0, 0};             // the block the slot's next batch takes
     Buf h_out[2], h_tab[2], h_chunks[2], h_rdesc[2];
     Event ev[2][EV_N];
-    // captured batches (graph mode): the events above order the streams
-    // inside the graph, these twins are its timestamp nodes; one graph per
-    // slot, its executable form updated in place batch after batch
-    Event evt[2][EV_N];
+    // captured batches (graph mode): the events above only order the streams
+    // inside the graph; tbeg times a captured batch from before its launch
+    // (to ev[EV_END]); one graph per slot, its executable form updated in
+    // place batch after batch
+    Event tbeg[2];
     Graph graph[2];
     uint64_t slot_plan[2] = {~0ull, ~0ull};   // the chunk plan each slot's device tables hold
     uint64_t budget = 0;                      // device bytes the set may hold (0: unbounded)
@@ -258,7 +259,7 @@ struct Slots {
             h_out[s].ensure(c, sizeof(gbh_segout) * kbatch, 1);
             h_tab[s].ensure(c, tab_layout(kbatch).bytes, 1);
             for (auto& e : ev[s]) e.create(c);
-            for (auto& e : evt[s]) e.create(c);
+            tbeg[s].create(c);
             graph[s].create(c);
         }
         for (auto& B : sl) {
@@ -648,15 +649,9 @@ private:
     // one before)
     bool graph_ = false;
     bool captured_[2] = {false, false};   // the slot's batch in flight ran as a graph
-    /// Event `id` of the slot on `stream`: in a captured batch its timing
-    /// twin (a timestamp node) first, then the event the streams wait on.
-    void mark(int stream, int slot, int id)
-    {
-        if (captured_[slot]) check(gbh_event_record_timed(E.c, stream, S.evt[slot][id].e), "event");
-        check(gbh_event_record(E.c, stream, S.ev[slot][id].e), "event");
-    }
-    /// The event that timestamps `id` for the slot's batch in flight.
-    const Event& tev(int slot, int id) const { return captured_[slot] ? S.evt[slot][id] : S.ev[slot][id]; }
+    /// Event `id` of the slot on `stream` (in a captured batch an ordering
+    /// node of the graph, no timestamp).
+    void mark(int stream, int slot, int id) { check(gbh_event_record(E.c, stream, S.ev[slot][id].e), "event"); }
 
     int main_stream(int slot) const { return overlap_ ? slot : 0; }
     uint64_t region_entries_ = 0;
@@ -1383,7 +1378,10 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     uint32_t* hq = B.hq.as<uint32_t>() + 32 * sel;
     auto& ev = S.ev[slot];
     captured_[slot] = graph_;
-    if (graph_) check(gbh_graph_begin(c, ms), "graph capture");
+    if (graph_) {
+        check(gbh_event_record(c, ms, S.tbeg[slot].e), "event");
+        check(gbh_graph_begin(c, ms), "graph capture");
+    }
     mark(ms, slot, EV_BEGIN);
     harvested_[slot] = nlarge_ != 0;
     std::vector<RunCost> costs;
@@ -1626,18 +1624,23 @@ void Pipeline::collect(std::vector<SegJob>& batch, int slot)
     }
     // the sieve phase ran beside the harvest: its own time plus the hit-list
     // phase count as the window stage
-    const auto t = [&](int id) -> const Event& { return tev(slot, id); };
-    const double tl = elapsed_s(c, t(EV_BEGIN), t(EV_SETUP));
-    const double ts = elapsed_s(c, t(EV_S0), t(EV_S1)) + elapsed_s(c, t(EV_SETUP), t(EV_WINDOW));
-    if (harvested_[slot]) {
+    // stage times from the events between the stages; a captured batch has
+    // only its whole time (launch to end), counted as its harvest stage
+    const auto t = [&](int id) -> const Event& { return ev[id]; };
+    const bool cap = captured_[slot];
+    const double tl = cap ? elapsed_s(c, S.tbeg[slot], ev[EV_END]) : elapsed_s(c, t(EV_BEGIN), t(EV_SETUP));
+    const double ts = cap ? 0.0 : elapsed_s(c, t(EV_S0), t(EV_S1)) + elapsed_s(c, t(EV_SETUP), t(EV_WINDOW));
+    if (harvested_[slot] && !cap) {
         agg_.harvest_kernel_seconds += elapsed_s(c, t(EV_HL0), t(EV_HL1));
         ++agg_.harvest_launches;
     }
-    agg_.window_kernel_seconds += ts;
-    ++agg_.window_launches;
-    const double tc = elapsed_s(c, t(EV_WINDOW), t(EV_CHECK));
+    if (!cap) {
+        agg_.window_kernel_seconds += ts;
+        ++agg_.window_launches;
+    }
+    const double tc = cap ? 0.0 : elapsed_s(c, t(EV_WINDOW), t(EV_CHECK));
     // the counter ran beside the checker (from the window's end) or after it
-    const double tn = elapsed_s(c, t(count_side_[slot] ? EV_WINDOW : EV_CHECK), t(EV_COUNT));
+    const double tn = cap ? 0.0 : elapsed_s(c, t(count_side_[slot] ? EV_WINDOW : EV_CHECK), t(EV_COUNT));
     agg_.sieve_seconds += ts;
     agg_.large_seconds += tl;
     agg_.verify_seconds += tc + tn;

@@ -414,11 +414,10 @@ int gbh_stream_wait(gbh_ctx* ctx, int stream, gbh_event* ev);
  * stream must have been joined back by a wait on `stream`), makes the
  * graph's executable form (in place, when the graph `g` ran before with
  * the same nodes and edges; else anew) and launches it on `stream`. While
- * capturing, gbh_event_record records a dependency only; gbh_event_record_timed
- * records a timestamp node (for gbh_event_elapsed_ms once the graph ran);
- * outside a capture both record an ordinary event. The library's
- * launchers never queue a memset (clears are fill kernels), so a captured
- * graph holds kernels, copies and event nodes only. A graph's executable
+ * capturing, gbh_event_record records a dependency only (time a captured
+ * batch by events recorded before gbh_graph_begin and after gbh_graph_end).
+ * The library's launchers never queue a memset (clears are fill kernels),
+ * so a captured graph holds kernels and copies only. A graph's executable
  * form is relaunched only after its previous launch completed (the caller
  * synchronises on an event recorded after gbh_graph_end). */
 typedef struct gbh_graph gbh_graph;
@@ -426,7 +425,6 @@ int gbh_graph_create(gbh_ctx* ctx, gbh_graph** g);
 int gbh_graph_destroy(gbh_ctx* ctx, gbh_graph* g);
 int gbh_graph_begin(gbh_ctx* ctx, int stream);
 int gbh_graph_end(gbh_ctx* ctx, int stream, gbh_graph* g);
-int gbh_event_record_timed(gbh_ctx* ctx, int stream, gbh_event* ev);
 /* Instantiations and in-place updates made by gbh_graph_end on `g`. */
 int gbh_graph_stats(gbh_ctx* ctx, const gbh_graph* g, uint64_t* instantiated, uint64_t* updated);
 
