@@ -640,6 +640,7 @@ private:
     // GB_OVERLAP=0: one stream for everything (no stage or batch runs
     // beside another; for profiling and as a fallback)
     bool overlap_ = env::onoff("GB_OVERLAP", true);
+    bool checkb_split_ = env::onoff("GB_CHECKB_SPLIT", true);
     // GB_GRAPH=1: each batch's launches captured as one HIP graph (the
     // slot's executable graph updated in place) and launched at once; 0
     // (default): issued one by one. Measured on MI355X with ROCm 7: graphs
@@ -1514,32 +1515,46 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     if (verify_every_ && nlarge_ && check_and_count)
         for (uint32_t s = 0; s < nseg; ++s)
             if (batch[s].index % verify_every_ == 0) enqueue_verify(s, b, out, slot);
+    // both kinds with the counter beside the checker (split): phase A of
+    // the sampled segments and then phase B on the side stream, phase A of
+    // the others and then the counter here (the counter's VALU work beside
+    // phase B's latency-bound probes); else phase B after both phase A's
+    // here and the counter on the side stream from the window's end
+    bool split = false;
     if (check_and_count) {
         uint32_t kinds = 0;
         for (const SegJob& j : batch) kinds |= j.sampled ? 2u : 1u;
         ca.kinds = kinds | 4;                // the candidate tables are built
+        gbh_checkb_args cb{b, E.ka, CHECK_TAIL, B.coff.as<uint32_t>(), B.pending.as<uint32_t>(), pcap_, kend_, out,
+                           B.ridge_more.as<uint64_t>()};
         if (kinds == 3 && count_side_[slot]) {
-            // both kinds: the histogram segments on the side stream (ahead
-            // of the counter), the others here, phase B after both
+            // both kinds: the histogram segments on the side stream, the
+            // others here, phase B after both
+            split = checkb_split_ && nwords;
             ca.kinds = 1 | 4;
             check(gbh_launch_check_a(c, ms, &ca), "check phase A");
+            if (split) mark(ms, slot, EV_CA);
             ca.kinds = 2 | 4;
             if (cgroups == 3) ca.class_groups = 5;   // the slower histogram form: more waves for it
             check(gbh_launch_check_a(c, ss, &ca), "check phase A (histogram)");
-            mark(ss, slot, EV_CA);
-            check(gbh_stream_wait(c, ms, ev[EV_CA].e), "stream wait");
+            if (split) {
+                check(gbh_stream_wait(c, ss, ev[EV_CA].e), "stream wait");
+                check(gbh_launch_check_b(c, ss, &cb), "check phase B");
+                mark(ss, slot, EV_CHECK);
+            } else {
+                mark(ss, slot, EV_CA);
+                check(gbh_stream_wait(c, ms, ev[EV_CA].e), "stream wait");
+            }
         } else {
             check(gbh_launch_check_a(c, ms, &ca), "check phase A");
         }
-        gbh_checkb_args cb{b, E.ka, CHECK_TAIL, B.coff.as<uint32_t>(), B.pending.as<uint32_t>(), pcap_, kend_, out,
-                           B.ridge_more.as<uint64_t>()};
-        check(gbh_launch_check_b(c, ms, &cb), "check phase B");
+        if (!split) check(gbh_launch_check_b(c, ms, &cb), "check phase B");
     }
-    mark(ms, slot, EV_CHECK);
+    if (!split) mark(ms, slot, EV_CHECK);
     if (check_and_count && nwords) {
         // the counter only reads the map and fills its own output fields:
         // on the side stream it runs beside the checker
-        const int cs = count_side_[slot] ? ss : ms;
+        const int cs = count_side_[slot] && !split ? ss : ms;
         gbh_count_args ct{b, dcs, nwords, swords_, B.scratch.as<uint32_t>(), out};
         check(gbh_launch_count(c, cs, &ct), "count");
         mark(cs, slot, EV_COUNT);
@@ -1547,6 +1562,7 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
     } else {
         mark(ms, slot, EV_COUNT);
     }
+    if (split) check(gbh_stream_wait(c, ms, ev[EV_CHECK].e), "stream wait");
     // the blocks without their leftover lists (fetched in collect when used)
     check(gbh_copy_d2h_rows(c, ms, S.h_out[slot].p, sizeof(gbh_segout), out, sizeof(gbh_segout), GBH_SEGOUT_HEAD,
                             nseg), "copy outputs");

@@ -27,6 +27,6 @@ def stats_env(tmp_path, monkeypatch):
     monkeypatch.setenv("GB_RECORD_HOOK", "")
     for k in ("GB_STATS", "GB_GAPS", "GB_STATS_SAMPLE", "GB_WIN_KB", "GB_WIN_P", "GB_WIN_P_LONG", "GB_KA", "GB_BATCH",
               "GB_HL_MODE", "GB_OVERLAP", "GB_TUNE_FILE", "GB_HL_WHEEL_HITS", "GB_HL_PERSIST", "GB_HL_PERSIST_WG", "GB_HL_CHUNKS_CU", "GB_HL_HITQ",
-              "GB_PHASEB_SLOTS", "GB_HL_REGION", "GB_GRAPH"):
+              "GB_PHASEB_SLOTS", "GB_HL_REGION", "GB_GRAPH", "GB_CHECKB_SPLIT"):
         monkeypatch.delenv(k, raising=False)
     return tmp_path
