@@ -215,21 +215,27 @@ def tmpfs():
 
 
 def kernel_totals(runs):
+    """per kernel: (seconds, launches, algorithmic bytes, bitmap words of those bytes)"""
     return {"k_harvest": (sum(r["harvest_kernel_seconds"] for r in runs), sum(r["harvest_launches"] for r in runs),
-                          sum(r["harvest_bytes"] for r in runs)),
+                          sum(r["harvest_bytes"] for r in runs), sum(r["harvest_words"] for r in runs)),
             "k_window": (sum(r["window_kernel_seconds"] for r in runs), sum(r["window_launches"] for r in runs),
-                         sum(r["window_bytes"] for r in runs))}
+                         sum(r["window_bytes"] for r in runs), 0.0)}
 
 
 def hbm_roofline(kern):
     """The dominant kernel's HBM roofline over kernel totals."""
     dom = max(kern, key=lambda n: kern[n][0])
-    sec, launches, nbytes = kern[dom]
+    sec, launches, nbytes, words = kern[dom]
     launch_s = sec / max(launches, 1)
     bytes_per_launch = nbytes / max(launches, 1)
     achieved = bytes_per_launch / launch_s / 1e9 if launch_s > 0 else 0.0
+    # the same bytes with every bitmap word read counted as the 64-byte
+    # sector the memory delivers for it (the words a prime reads are a
+    # random access each: one sector per word, not 8 bytes)
+    sector = bytes_per_launch + 56.0 * words / max(launches, 1)
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "kernel": dom, "bytes_per_launch": bytes_per_launch,
+            "bitmap_words_per_launch": words / max(launches, 1), "bytes_per_launch_sector": sector,
             "launch_ms": 1e3 * launch_s}
 
 
@@ -272,6 +278,7 @@ def measure_headline(args, lib):
     k = len(runs)
     kern = kernel_totals(runs)
     roof = hbm_roofline(kern)
+    tr = traffic() if roof["kernel"] == "k_harvest" else None
     out = {
         "metric": "G evens/s at 4e18 w/ exact stats",
         "value": EVENS * k / wall / 1e9,
@@ -295,8 +302,14 @@ def measure_headline(args, lib):
                    "host_plan_ms": 1e3 * per_step(runs, "host_plan_seconds"),
                    "engine_init_s": init_s, "bitmap_prepare_s": prep_s},
         "roofline": {**{n: roof[n] for n in ("bound", "achieved", "peak", "unit", "frac")},
-                     "traffic": traffic() if roof["kernel"] == "k_harvest" else None,
-                     **{n: roof[n] for n in ("kernel", "bytes_per_launch", "launch_ms")},
+                     "traffic": tr,
+                     **{n: roof[n] for n in ("kernel", "bytes_per_launch", "bitmap_words_per_launch",
+                                             "bytes_per_launch_sector", "launch_ms")},
+                     # counter traffic over the algorithmic bytes (8 bytes per bitmap word) and over the
+                     # same bytes at the 64-byte sector of a random word read
+                     "traffic_ratio": tr / roof["bytes_per_launch"] if tr and roof["bytes_per_launch"] else None,
+                     "traffic_ratio_sector": (tr / roof["bytes_per_launch_sector"]
+                                              if tr and roof["bytes_per_launch_sector"] else None),
                      "binding": binding(COUNTERS_SEGMENT, kern, k)},
     }
     return out, runs[-1]["stats_line"], state

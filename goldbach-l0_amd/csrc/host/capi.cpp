@@ -60,6 +60,7 @@ void fill(const gb::RunStats& s, gba_run_result* out)
             out->host_plan_seconds = s.host_plan_seconds;
             out->harvest_reruns = s.harvest_reruns;
             out->graph_launches = s.graph_launches;
+            out->harvest_words = s.harvest_words;
         }
 }
 }  // namespace

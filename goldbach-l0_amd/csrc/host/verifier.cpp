@@ -1400,21 +1400,29 @@ void Pipeline::enqueue(std::vector<SegJob>& batch, int slot, bool check_and_coun
             // Words a prime must read: below the run's q span one per prime
             // plus its multiplier range (q span / p / 240); above it, one
             // only when its single multiple lies in the span and its
-            // multiplier is a unit mod 30 ((8/30) q span / p).
+            // multiplier is a unit mod 30 ((8/30) q span / p); so too from a
+            // quarter of the span on when the compacted path takes those
+            // primes (one word per multiple whose multiplier is a unit)
             const uint32_t np = r.nlarge, a0 = std::min(np, nwheel_), a1 = std::min(np, nwheel_hi_),
                            b0 = std::min(np, nwheel_hi2_);
             const uint32_t* lp = E.base.data() + E.win_hi;
-            const uint32_t js = uint32_t(std::upper_bound(lp, lp + np, uint32_t(std::min(lq, 4294967295.0))) - lp);
+            const auto first_above = [&](double v) {
+                return uint32_t(std::upper_bound(lp, lp + np, uint32_t(std::min(v, 4294967295.0))) - lp);
+            };
+            const uint32_t js = first_above(lq);
+            const uint32_t jf = few_path() && direct_ && !force_count_ ? first_above(lq / 4) : js;
             double words = 0;
             for (const auto& [x, y] : {std::pair<uint32_t, uint32_t>{a0, a1}, {b0, np}}) {
                 if (x >= y) continue;
-                const uint32_t m = std::clamp(js, x, y);
-                const double ix = E.inv_at(x), im = E.inv_at(m), iy = E.inv_at(y);
-                words += double(m - x) + lq * (im - ix) / 240 + (8.0 / 30) * lq * (iy - im);
+                const uint32_t f = std::clamp(std::min(jf, js), x, y);
+                const double ix = E.inv_at(x), i_f = E.inv_at(f), iy = E.inv_at(y);
+                words += double(f - x) + lq * (i_f - ix) / 240 + (8.0 / 30) * lq * (iy - i_f);
             }
             // the packed primes: one byte each and a 4-byte base per group
-            if (!force_count_)
-                agg_.harvest_bytes += np > nx_ ? (1.0 + 4.0 / GBH_PS_GROUP) * (np - nx_) + 8.0 * words + 4.0 * hits : 0.0;
+            if (!force_count_ && np > nx_) {
+                agg_.harvest_bytes += (1.0 + 4.0 / GBH_PS_GROUP) * (np - nx_) + 8.0 * words + 4.0 * hits;
+                agg_.harvest_words += words;
+            }
             bits_all += bits;
             hits_all += hits;
             win_prime_bytes += 4.0 * run.nseg * 8 * (r.nwin + std::min(nx_, r.nlarge));
@@ -1799,6 +1807,7 @@ void Pipeline::finish_range(Range& r)
         rs.harvest_launches = agg_.harvest_launches;
         rs.harvest_reruns = agg_.harvest_reruns;
         rs.graph_launches = agg_.graph_launches;
+        rs.harvest_words = agg_.harvest_words;
         rs.window_launches = agg_.window_launches;
         rs.harvest_bytes = agg_.harvest_bytes;
         rs.window_bytes = agg_.window_bytes;

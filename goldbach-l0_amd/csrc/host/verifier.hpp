@@ -50,6 +50,7 @@ struct RunStats {
     double host_plan_seconds = 0;   // range resolution, statistics set-up and buffer planning of this run
     uint64_t harvest_reruns = 0;    // batches re-run with counted lists after a direct-harvest region overflow
     uint64_t graph_launches = 0;    // batches launched as one captured HIP graph (GB_GRAPH)
+    double harvest_words = 0;       // of harvest_bytes: the 8-byte multiplier-bitmap words read (modelled)
     std::vector<SegmentStats> per_segment;
 };
 

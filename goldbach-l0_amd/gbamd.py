@@ -35,7 +35,7 @@ class GbaRunResult(ctypes.Structure):
         ("harvest_launches", ctypes.c_uint64), ("window_launches", ctypes.c_uint64),
         ("harvest_bytes", ctypes.c_double), ("window_bytes", ctypes.c_double),
         ("host_plan_seconds", ctypes.c_double), ("harvest_reruns", ctypes.c_uint64),
-        ("graph_launches", ctypes.c_uint64),
+        ("graph_launches", ctypes.c_uint64), ("harvest_words", ctypes.c_double),
     ]
 
 

@@ -43,6 +43,7 @@ typedef struct {
     double host_plan_seconds;
     uint64_t harvest_reruns;       /* batches re-run with counted lists after a harvest region overflow */
     uint64_t graph_launches;       /* batches launched as one captured HIP graph (GB_GRAPH) */
+    double harvest_words;          /* of harvest_bytes: the 8-byte multiplier-bitmap words read (a model, as the bytes) */
 } gba_run_result;
 
 /* One segment of a run (the reference's SegmentStats, include/verifier.hpp:
