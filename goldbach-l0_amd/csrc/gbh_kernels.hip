@@ -1551,21 +1551,30 @@ __global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_batch b, const uin
                 // the sixteen wave totals by a halving exchange: each step
                 // a lane keeps half of its counts, adds its partner's copy
                 // of that half and hands over the other half. Partners:
-                // lane ^ 32 and ^ 16 (bpermute), then inside a row of 16
-                // lane ^ 15 (DPP row_mirror) and ^ 7 (row_half_mirror),
-                // each differing in the bit that picks the half. After
-                // four steps lane l holds count (l >> 2) & 15 summed over
-                // eight lanes of its row; quad_perm ^ 1 and ^ 2 add the
-                // rest. Fields stay below 2^10 before the unpacking (64
+                // lane ^ 32 and ^ 16, then inside a row of 16 lane ^ 15
+                // (DPP row_mirror) and ^ 7 (row_half_mirror), each
+                // differing in the bit that picks the half. The first two
+                // steps are register swaps across rows: permlane32_swap(x,
+                // y) trades x's upper half-wave for y's lower one, so the
+                // sum of its two results is x summed over lanes l and
+                // l + 32 in the lower half and y so summed in the upper;
+                // permlane16_swap does the same for odd and even rows.
+                // After four steps lane l holds count (l >> 2) & 15 summed
+                // over eight lanes of its row; quad_perm ^ 1 and ^ 2 add
+                // the rest. Fields stay below 2^10 before the unpacking (64
                 // per lane, doubling each step).
-                const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4;
+                const bool b3 = lane & 8, b2 = lane & 4;
                 uint32_t h4[4], h2[2];
 #pragma unroll
-                for (uint32_t k = 0; k < 4; ++k)
-                    h4[k] = (b5 ? hc[k + 4] : hc[k]) + (uint32_t)__shfl_xor((int)(b5 ? hc[k] : hc[k + 4]), 32, 64);
+                for (uint32_t k = 0; k < 4; ++k) {
+                    const auto sw = __builtin_amdgcn_permlane32_swap(hc[k], hc[k + 4], false, false);
+                    h4[k] = sw[0] + sw[1];
+                }
 #pragma unroll
-                for (uint32_t k = 0; k < 2; ++k)
-                    h2[k] = (b4 ? h4[k + 2] : h4[k]) + (uint32_t)__shfl_xor((int)(b4 ? h4[k] : h4[k + 2]), 16, 64);
+                for (uint32_t k = 0; k < 2; ++k) {
+                    const auto sw = __builtin_amdgcn_permlane16_swap(h4[k], h4[k + 2], false, false);
+                    h2[k] = sw[0] + sw[1];
+                }
                 const uint32_t h1 = (b3 ? h2[1] : h2[0]) +
                                     (uint32_t)__builtin_amdgcn_mov_dpp((int)(b3 ? h2[0] : h2[1]), 0x140, 0xF, 0xF, false);
                 uint32_t tv = (b2 ? h1 >> 16 : h1 & 0xFFFFu) +
@@ -1757,6 +1766,48 @@ __device__ __forceinline__ void flush_extremes(uint32_t first, uint32_t last, bo
     }
 }
 
+// Eight class words of 32 blocks each into four 64-bit masks in position
+// order: bit 8 j + k of m[g] is bit 8 g + j of r[k]. First an 8 x 8 bit
+// transpose inside every byte column across the eight registers (three
+// rounds of block swaps: 4 x 4, 2 x 2, 1 x 1), after which byte g of r[j]
+// holds bit 8 g + j of every class; then a 4 x 4 byte transpose of each
+// half (byte permutes, {s0, s1} with s1 the low dword) gathers byte g of
+// the eight registers into m[g].
+__device__ __forceinline__ void class_words_to_blocks(uint32_t (&r)[8], uint64_t (&m)[4])
+{
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t t = ((r[k] >> 4) ^ r[k + 4]) & 0x0F0F0F0Fu;
+        r[k + 4] ^= t;
+        r[k] ^= t << 4;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k += (k & 1) ? 3 : 1) {   // 0, 1, 4, 5
+        const uint32_t t = ((r[k] >> 2) ^ r[k + 2]) & 0x33333333u;
+        r[k + 2] ^= t;
+        r[k] ^= t << 2;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k += 2) {
+        const uint32_t t = ((r[k] >> 1) ^ r[k + 1]) & 0x55555555u;
+        r[k + 1] ^= t;
+        r[k] ^= t << 1;
+    }
+    uint32_t w[2][4];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t* v = r + 4 * h;
+        const uint32_t t0 = __builtin_amdgcn_perm(v[1], v[0], 0x06020400u), t1 = __builtin_amdgcn_perm(v[1], v[0], 0x07030501u);
+        const uint32_t t2 = __builtin_amdgcn_perm(v[3], v[2], 0x06020400u), t3 = __builtin_amdgcn_perm(v[3], v[2], 0x07030501u);
+        w[h][0] = __builtin_amdgcn_perm(t2, t0, 0x05040100u);
+        w[h][1] = __builtin_amdgcn_perm(t3, t1, 0x05040100u);
+        w[h][2] = __builtin_amdgcn_perm(t2, t0, 0x07060302u);
+        w[h][3] = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
+    }
+#pragma unroll
+    for (int g = 0; g < 4; ++g) m[g] = ((uint64_t)w[1][g] << 32) | w[0][g];
+}
+
 __device__ __forceinline__ void note_gap(uint32_t g, uint32_t up, uint32_t* hl, gbh_segout* out, uint64_t& best)
 {
     const uint32_t bin = g >> 1;
@@ -1809,13 +1860,12 @@ __global__ void __launch_bounds__(256) k_count_a(gbh_batch b, const gbh_countseg
     for (uint32_t y = blockIdx.x * 256 + tid; y < npieces; y += gridDim.x * 256) {
         const uint32_t x = y / kSub, sh = kBits * (y % kSub);
         const uint64_t pm = ((1ull << kBits) - 1) << sh;   // the piece's bits of the word
-        uint32_t h[8], any = 0;
+        uint32_t h[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const uint64_t w = map[(size_t)a.cls[k] * b.cw + x] & range_mask(x, a.lo[k], a.hi[k]) & pm;
             pcs[k] += (uint32_t)__popcll(w);
             h[k] = (uint32_t)(w >> sh);
-            any |= h[k];
         }
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
@@ -1825,25 +1875,27 @@ __global__ void __launch_bounds__(256) k_count_a(gbh_batch b, const gbh_countseg
             pcs[8 + j] += (uint32_t)__popcll(lw & pw);
         }
         // the walk: primes in position order (block t, then ascending
-        // phase); per block the eight class bits gathered into m8, then one
-        // step per prime
+        // phase): the piece as four masks of eight blocks each, one step
+        // per prime; a lane moves to its next non-empty mask by itself, so
+        // the wave steps as often as its lane with the most primes
+        uint64_t mk[4];
+        class_words_to_blocks(h, mk);
         uint32_t first = 0xFFFFFFFFu, prev = 0;
         bool have = false;
-        while (any) {
-            const uint32_t bt = (uint32_t)__ffs(any) - 1;
-            any &= any - 1;
-            uint32_t m8 = 0;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) m8 |= ((h[k] >> bt) & 1u) << k;
-            const uint32_t base = 30u * (x * 64 + sh + bt);
-            do {
-                const uint32_t k = (uint32_t)__ffs(m8) - 1;
-                m8 &= m8 - 1;
-                const uint32_t pos = base + 2 * ((ph4 >> (4 * k)) & 15u);
-                if (have) note_gap(pos - prev, pos, hl, out, best);
-                else { first = pos; have = true; }
-                prev = pos;
-            } while (m8);
+        uint64_t m = mk[0];
+        uint32_t g = 0;
+        for (;;) {
+            while (!m && g < 3) {
+                ++g;
+                m = g == 1 ? mk[1] : g == 2 ? mk[2] : mk[3];
+            }
+            if (!m) break;
+            const uint32_t i = (uint32_t)__ffsll((long long)m) - 1;
+            m &= m - 1;
+            const uint32_t pos = 30u * (x * 64 + sh + 8 * g + (i >> 3)) + 2 * ((ph4 >> (4 * (i & 7))) & 15u);
+            if (have) note_gap(pos - prev, pos, hl, out, best);
+            else { first = pos; have = true; }
+            prev = pos;
         }
         fl[2 * y] = first;
         fl[2 * y + 1] = have ? prev : 0xFFFFFFFFu;
