@@ -1292,24 +1292,31 @@ __global__ void __launch_bounds__(256) k_window(gbh_batch b, const uint32_t* __r
     const uint32_t nwords = b.wbits >> 6;
     const uint64_t gw0 = (uint64_t)blockIdx.x * nwords;     // first word of the window in class c
     const uint32_t wb0 = (uint32_t)(gw0 * 64);              // first bit (class arrays < 2^31 bits)
-    const uint32_t wb1 = wb0 + b.wbits;
     const uint32_t tid = threadIdx.x;
     const uint32_t* i0 = i0tab + ((size_t)s * 8 + c) * np;
     const bool sieve = phases & 1, apply = (phases & 2) && hits;
     uint64_t* dst = seg_map(b, s) + (size_t)c * b.cw + gw0;
+    // the words of the window that hold bits of the class array (a short
+    // segment's window, or a segment's last one, ends early): only they are
+    // sieved and stored; the rest of the window is stored as zeros (no
+    // primes), which every reader masks off anyway
+    const uint32_t ncc = g.nc[c];
+    const uint32_t vwords = ncc > wb0 ? min(nwords, (ncc - wb0 + 63) >> 6) : 0u;
+    const uint32_t vlim = vwords * 64;                      // marks below it
+    const uint32_t wb1 = wb0 + vlim;
 
     // In LDS a window holds the composites as set bits (marks are ORs; the
     // store inverts it back to the map's primes-as-ones).
     if (!sieve) {   // the hit lists only: the window as the sieve phase stored it
         uint32_t x = tid;
-        for (; x + 3 * 256 < nwords; x += 4 * 256) {   // four loads in flight per thread
+        for (; x + 3 * 256 < vwords; x += 4 * 256) {   // four loads in flight per thread
             uint64_t v[4];
 #pragma unroll
             for (uint32_t u = 0; u < 4; ++u) v[u] = dst[x + u * 256];
 #pragma unroll
             for (uint32_t u = 0; u < 4; ++u) win[x + u * 256] = ~v[u];
         }
-        for (; x < nwords; x += 256) win[x] = ~dst[x];
+        for (; x < vwords; x += 256) win[x] = ~dst[x];
     } else {        // 1) mask stage: every multiple of 7..61 set by whole-word masks
         uint32_t ph[GBH_NSMALL];
 #pragma unroll
@@ -1318,7 +1325,7 @@ __global__ void __launch_bounds__(256) k_window(gbh_batch b, const uint32_t* __r
             const uint32_t a = (uint32_t)(((gw0 + tid) * 64) % p);
             ph[k] = (g.s0[c][k] + p - a) % p;
         }
-        for (uint32_t x = tid; x < nwords; x += 256) {
+        for (uint32_t x = tid; x < vwords; x += 256) {
             uint64_t m = 0;
 #pragma unroll
             for (int k = 0; k < GBH_NSMALL; ++k) {
@@ -1335,7 +1342,7 @@ __global__ void __launch_bounds__(256) k_window(gbh_batch b, const uint32_t* __r
 
     // 2) window primes below wbits / 128 (at least 128 marks each): one
     //    wave per prime, lane l takes marks l, l + 64, ...
-    const uint32_t lane = tid & 63, wv = tid >> 6, wbits = b.wbits;
+    const uint32_t lane = tid & 63, wv = tid >> 6;
     np = sieve ? min(np, g.np_win) : 0u;   // the segment's primes (larger ones have no multiple in it)
     ncoop = min(ncoop, np);
     {   // the wave's primes are uniform: scalar loads, the next one's issued
@@ -1349,7 +1356,7 @@ __global__ void __launch_bounds__(256) k_window(gbh_batch b, const uint32_t* __r
                 pn = primes[j + 4];
             }
             if (t0 == 0xFFFFFFFFu) continue;
-            mark_run(win32, first_off(t0, wb0, p) + lane * p, 64 * p, wbits);
+            mark_run(win32, first_off(t0, wb0, p) + lane * p, 64 * p, vlim);
         }
     }
     // 3) the other window primes: one prime per lane, kWL of them loaded at
@@ -1367,7 +1374,7 @@ __global__ void __launch_bounds__(256) k_window(gbh_batch b, const uint32_t* __r
         for (uint32_t u = 0; u < kWL; ++u) {
             if (t0[u] == 0xFFFFFFFFu || t0[u] >= wb1) continue;
             const uint32_t p = pr[u];
-            mark_run(win32, first_off_big(t0[u], wb0, p, __builtin_amdgcn_rcpf((float)p)), p, wbits);
+            mark_run(win32, first_off_big(t0[u], wb0, p, __builtin_amdgcn_rcpf((float)p)), p, vlim);
         }
     }
     // 4) the large primes' hits of this window
@@ -1391,7 +1398,7 @@ __global__ void __launch_bounds__(256) k_window(gbh_batch b, const uint32_t* __r
     __syncthreads();
 
     // 5) coalesced store of the window (primes as set bits again)
-    for (uint32_t x = tid; x < nwords; x += 256) dst[x] = ~win[x];
+    for (uint32_t x = tid; x < nwords; x += 256) dst[x] = x < vwords ? ~win[x] : 0ull;
 }
 
 // ---------------------------------------------------------------- checker
