@@ -75,6 +75,12 @@ WINDOWS = [
     ("w4e18l", 467_500_000, {}),
     ("w4e18l", 20_000_000, {"GB_STATS_SAMPLE": "3"}),
     ("cfg1", 467_500_000, {}),
+    # windows far longer than the segments' class arrays (2^19 bits for
+    # arrays of ~10^4): only a window's words that hold class bits are
+    # sieved and stored
+    ("w1e6", 500_000, {"GB_WIN_KB": "64"}),
+    ("w4e18s", 600_000, {"GB_WIN_KB": "64", "GB_STATS_SAMPLE": "1"}),
+    ("wtop", 1_000_000, {"GB_WIN_KB": "64"}),
     # the large-prime split: every large prime on the wheel walk (no
     # multiplier bitmap), and a small bitmap budget (wheel below ~1.7e7)
     ("w4e18", 2_000_000, {"GB_MT_GB": "0"}),
