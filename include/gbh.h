@@ -406,8 +406,9 @@ int gbh_stream_wait(gbh_ctx* ctx, int stream, gbh_event* ev);
 
 /* ---- a batch's launches as one HIP graph ----
  * No counterpart in the reference (its host appends a segment's kernels to
- * Level Zero command lists one by one, src/verifier.cpp:2487-2517); the
- * north star's hipGraph-captured per-batch pipeline. gbh_graph_begin makes
+ * Level Zero command lists one by one, src/verifier.cpp:2487-2517): a
+ * batch's whole pipeline captured once and launched as one graph, for
+ * callers that replay batches of one shape. gbh_graph_begin makes
  * `stream` capture instead of run: every copy, launch, event record and
  * wait queued on it, and on every stream that waits on an event recorded in
  * the capture, becomes a node. gbh_graph_end ends the capture (every such
