@@ -29,3 +29,19 @@ def test_tune_file_shapes_runs_without_changing_results(stats_env, monkeypatch, 
     env = _run(stats_env)
     assert env["split_used"] == base["split_used"]
     assert compare.strip_version(env["stats_line"]) == compare.strip_version(base["stats_line"])
+
+
+@pytest.mark.parametrize("name,segment", [("w1e12", 1_000_000), ("w4e18", 2_000_000), ("w4e18l", 20_000_000)])
+def test_shipped_tune_bands_match_stats_ref(stats_env, monkeypatch, name, segment):
+    """The autotuned bands shipped with the package (tune_mi355x.txt) keep
+    every statistic equal to the reference's fixture on windows of their
+    decades."""
+    from conftest import GOLDEN, PKG
+
+    a, b, ref = compare.load_fixture(GOLDEN / f"ref_{name}.txt")
+    monkeypatch.setenv("GB_TUNE_FILE", str(PKG / "tune_mi355x.txt"))
+    print(f"tuned case {name} [{a}, {b}] segment {segment}", flush=True)
+    r = gbamd.run_gpu(b, segment, start_even=a)
+    assert r["leftovers"] == 0
+    bad = compare.compare_ref(r["stats_line"], (stats_env / "st.state").read_text(), ref)
+    assert not bad, bad
