@@ -1537,12 +1537,14 @@ __global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_batch b, const uin
         if (lane < kWaveWords && k0 < e.cnt) valid = (e.cnt - k0 >= 64) ? ~0ull : ((1ull << (e.cnt - k0)) - 1);
         const uint32_t v0 = (uint32_t)valid, v1 = (uint32_t)(valid >> 32);
         uint32_t u0 = v0, u1 = v1;                        // unresolved evens of the word
+        uint32_t nu = kHist ? (uint32_t)__popc(u0) + (uint32_t)__popc(u1) : 0u;   // how many
         // ka is a multiple of 16 and at most every class's useful candidates
         const uint32_t* ct = coffa + ((size_t)s * 15 + cr) * GBH_NCAND_PAD;
         for (uint32_t j0 = 0; j0 < ka; j0 += 16) {
             if (kHist) {
-                // evens first resolved by candidate j0 + u, two counts per
-                // register (16 bits each: at most 64 x 64 per wave)
+                // evens first resolved by candidate j0 + u (the drop in the
+                // unresolved count), two counts per register (16 bits
+                // each: at most 64 x 64 per wave)
                 uint32_t hc[8];
 #pragma unroll
                 for (uint32_t u = 0; u < 16; ++u) {
@@ -1550,10 +1552,12 @@ __global__ void __launch_bounds__(GBH_CHECK_WG) k_check_a(gbh_batch b, const uin
                     const uint32_t a = sb[ent & 0x1FFF], bw = sb[(ent >> 13) & 0x1FFF], cw = from_next_lane(a);
                     const uint32_t x0 = __builtin_amdgcn_alignbit(bw, a, sh);
                     const uint32_t x1 = __builtin_amdgcn_alignbit(cw, bw, sh);
-                    const uint32_t h = (uint32_t)__popc(x0 & u0) + (uint32_t)__popc(x1 & u1);
-                    hc[u / 2] = (u & 1) ? (hc[u / 2] | (h << 16)) : h;
                     u0 &= ~x0;
                     u1 &= ~x1;
+                    const uint32_t left = (uint32_t)__popc(u0) + (uint32_t)__popc(u1);
+                    const uint32_t h = nu - left;
+                    nu = left;
+                    hc[u / 2] = (u & 1) ? (hc[u / 2] | (h << 16)) : h;
                 }
                 // the sixteen wave totals by a halving exchange: each step
                 // a lane keeps half of its counts, adds its partner's copy
